@@ -1,0 +1,138 @@
+/*
+ * rt_abi.h — C ABI of the MI355X path-tracing library (librt_hip.so).
+ *
+ * The reference renderer has no device boundary; its hot path is the
+ * virtual call Integrator::integrate(const Ray&, const Intersectable&,
+ * std::vector<const Light*>, gsl_rng*) (src/integrator.h:32-37) made per
+ * camera sample by PerPixelRenderer::render (src/perpixelrenderer.cpp:43-101),
+ * which reaches BVHNode::intersect (src/bvh.cpp:135-149) for every ray.
+ * This ABI replaces that path at batch granularity:
+ *   - rt_scene_upload  replaces scene_builder::getScene + BVHNode::create
+ *                      (src/sceneast.cpp:673-695, src/bvh.cpp:67-94)
+ *   - rt_trace_closest replaces Intersectable::intersect on the scene root
+ *                      (src/intersectable.h:40, src/bvh.cpp:135-149)
+ *   - rt_trace_any     replaces the shadow queries inside Light::getIntensity
+ *                      (src/arealight.cpp:35-51, src/pointlight.cpp:60-65)
+ *   - rt_render        replaces PerPixelRenderer::render's loop over pixels and
+ *                      samples calling UniDiPathTracingIntegrator::integrate
+ *                      (src/perpixelrenderer.cpp:62-98,
+ *                       src/unidipathtracingintegrator.cpp:34-107)
+ *
+ * Conventions: every function returns 0 on success and a negative rt_status
+ * on failure; no C++ exception crosses the ABI.  All calls are synchronous
+ * (they return after the GPU work finished).  A context is not re-entrant:
+ * one thread uses it at a time.  The caller owns every array it passes.
+ * "_device" variants take device pointers (HBM-resident inputs / outputs);
+ * the others take host pointers and copy.
+ */
+#ifndef RT_ABI_H
+#define RT_ABI_H
+
+#include <stdint.h>
+#include "rt_scene.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct rt_ctx rt_ctx;
+
+enum rt_status {
+    RT_OK = 0,
+    RT_ERR_INVALID = -1,     /* bad argument */
+    RT_ERR_NO_SCENE = -2,    /* no scene uploaded */
+    RT_ERR_HIP = -3,         /* a HIP runtime call failed */
+    RT_ERR_BUILD = -4,       /* BVH construction failed (e.g. depth limit) */
+    RT_ERR_CANCELLED = -5,   /* rt_request_stop was called during a render */
+    RT_ERR_FAULT = -6,       /* injected by rt_set_fault (test hook) */
+    RT_ERR_NO_GUIDE = -7     /* guided mode without a guiding network */
+};
+
+/* Counters of the last call (and totals since rt_create). */
+typedef struct rt_stats {
+    int64_t camera_rays;      /* primary rays traced */
+    int64_t extension_rays;   /* closest-hit rays after the first bounce */
+    int64_t shadow_rays;      /* any-hit rays */
+    double render_ms;         /* wall time of the last rt_render, GPU events */
+    double extend_ms;         /* summed duration of closest-hit kernel launches */
+    double shadow_ms;         /* summed duration of any-hit kernel launches */
+    int64_t extend_launches;  /* number of closest-hit kernel launches timed */
+    int64_t bvh_nodes;        /* interior nodes of the uploaded BVH */
+    int64_t bvh_prims;        /* primitive records (triangles, spheres, parallelograms) */
+    int32_t bvh_depth;        /* depth of the uploaded BVH */
+    int32_t paths_per_pass;   /* path slots used by the last render */
+    int32_t passes;           /* passes of the last render */
+    int32_t reserved;
+    int64_t node_visits;      /* interior nodes fetched by closest-hit traversal */
+    int64_t prim_tests;       /* primitive records tested by closest-hit traversal */
+} rt_stats;
+
+/* Library-wide. */
+const char* rt_error_string(int status);
+int rt_version(void);
+
+/* Context. */
+int rt_create(int device, rt_ctx** out);
+int rt_destroy(rt_ctx* ctx);
+const char* rt_last_error(rt_ctx* ctx);
+
+/* Scene: builds the flattened BVH on the host (median split on the longest
+ * centroid axis, leaves of at most 4 primitives of one kind) and uploads
+ * nodes, primitive records, shading data, materials, lights and the
+ * environment map with its sampling tables. */
+int rt_scene_upload(rt_ctx* ctx, const rt_scene_desc* desc);
+
+/* Ray batches.  Ray i: origin o_tmin[4i..4i+2], range start o_tmin[4i+3],
+ * direction d_tmax[4i..4i+2], range end d_tmax[4i+3].  Results: prim[i] =
+ * primitive ID of the closest hit in [tmin, tmax] or -1, t[i] = its ray
+ * parameter or NaN.  The environment is not a primitive here. */
+int rt_trace_closest(rt_ctx* ctx, const float* o_tmin, const float* d_tmax, int64_t n,
+                     int32_t* prim, float* t);
+int rt_trace_closest_device(rt_ctx* ctx, const float* o_tmin, const float* d_tmax, int64_t n,
+                            int32_t* prim, float* t);
+int rt_trace_any(rt_ctx* ctx, const float* o_tmin, const float* d_tmax, int64_t n, uint8_t* occluded);
+int rt_trace_any_device(rt_ctx* ctx, const float* o_tmin, const float* d_tmax, int64_t n,
+                        uint8_t* occluded);
+
+/* Traversal statistics of a ray batch (device pointers): interior nodes
+ * visited and primitives tested per ray by the closest-hit traversal. */
+int rt_trace_count_device(rt_ctx* ctx, const float* o_tmin, const float* d_tmax, int64_t n,
+                          int32_t* nodes, int32_t* prims);
+
+/* Render.  films[v] receives view v's image, height*width*3 RGB float,
+ * accumulated as film += sample / spp in sample order (the caller zeroes it
+ * or passes an image to add to).  Several views of different resolution and
+ * sample count are traced together in one pipeline of path slots. */
+int rt_render(rt_ctx* ctx, const rt_view* views, int32_t n_views, const rt_params* params,
+              float* const* films);
+int rt_render_device(rt_ctx* ctx, const rt_view* views, int32_t n_views, const rt_params* params,
+                     float* const* films);
+
+/* Launch configuration used by rt_render; 0 leaves a field unchanged.
+ * path_slots: paths in flight per pass; block: threads per block of the
+ * traversal kernels; waves_per_cu: persistent blocks are sized from it. */
+typedef struct rt_launch_config {
+    int32_t path_slots;
+    int32_t block;
+    int32_t blocks_per_cu;
+    int32_t use_graph;     /* 1 = replay the per-pass kernel sequence as a hipGraph */
+    int32_t count_traversal; /* 1 = closest-hit kernels also count nodes and primitives */
+} rt_launch_config;
+int rt_set_launch_config(rt_ctx* ctx, const rt_launch_config* cfg);
+int rt_get_launch_config(rt_ctx* ctx, rt_launch_config* cfg);
+
+int rt_get_stats(rt_ctx* ctx, rt_stats* out);
+
+/* Cooperative cancel, polled between passes (the reference's doStop,
+ * src/renderer.cpp:125-129). */
+int rt_request_stop(rt_ctx* ctx, int32_t stop);
+
+/* Test hook: when set, the next ABI call that would launch GPU work
+ * returns RT_ERR_FAULT instead. */
+int rt_set_fault(rt_ctx* ctx, int32_t fault);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RT_ABI_H */

@@ -1,0 +1,132 @@
+/*
+ * rt_scene.h — flat, pointer-free-at-rest scene description shared by the
+ * MI355X path tracer (librt_hip.so) and its CPU checker.
+ *
+ * The reference renderer builds its scene as a graph of heap objects
+ * (src/sceneast.cpp:673-695: root list = [lights..., "intersectable"],
+ * then BVHNode::create, src/bvh.cpp:67-94).  Crossing a C ABI needs plain
+ * arrays instead, so a scene is handed over as this struct of typed arrays.
+ * The caller owns every array; rt_scene_upload copies what it needs.
+ *
+ * Primitive IDs: the reference has no primitive ID.  Here it is the index
+ * of the primitive in the flattened root list of the reference
+ * (src/sceneast.cpp:676-678 then src/intersectablelist.cpp:67-78): lights
+ * first, in file order, then the components of "intersectable".  A quad is
+ * one primitive (two triangles that carry the same ID, src/quad.h:32).
+ *
+ * Colours are RGB.  The reference keeps OpenCV BGR order in cv::Vec3f
+ * (src/sceneast.cpp:410-414); channel-wise arithmetic is identical, only
+ * the luminance weights (src/mathhelper.cpp:102-105) are applied to the
+ * matching channels.
+ */
+#ifndef RT_SCENE_H
+#define RT_SCENE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Material kinds: src/diffusematerial.cpp, src/mirrormaterial.cpp,
+ * src/phongmaterial.cpp, src/darkmatter.cpp. */
+enum rt_material_type {
+    RT_MAT_DIFFUSE = 0,
+    RT_MAT_MIRROR = 1,
+    RT_MAT_PHONG = 2,
+    RT_MAT_DARK = 3
+};
+
+/* Light kinds: src/arealight.cpp, src/pointlight.cpp, src/environmentmap.cpp */
+enum rt_light_type {
+    RT_LIGHT_AREA = 0,
+    RT_LIGHT_POINT = 1,
+    RT_LIGHT_ENV = 2
+};
+
+typedef struct rt_material {
+    int32_t type;        /* rt_material_type */
+    float kd[3];         /* diffuse colour (RGB); DiffuseMaterial colour */
+    float ks[3];         /* Phong specular colour (RGB) */
+    float exponent;      /* Phong specular exponent */
+} rt_material;
+
+typedef struct rt_light {
+    int32_t type;        /* rt_light_type */
+    float p[3];          /* area: parallelogram origin; point: position */
+    float u[3];          /* area: first edge  (normal = u x v, |u x v| = area) */
+    float v[3];          /* area: second edge */
+    float intensity[3];  /* RGB; env: coefficient multiplying the map */
+} rt_light;
+
+typedef struct rt_scene_desc {
+    /* indexed triangle meshes (quads are given as two triangles (p1,p2,p4),
+     * (p4,p2,p3) in that order, src/quad.h:32) */
+    int32_t n_vertices;
+    const float* positions;      /* 3 * n_vertices */
+    const float* normals;        /* 3 * n_vertices, or NULL = flat normals */
+    int32_t n_triangles;
+    const int32_t* tri_vertices; /* 3 * n_triangles */
+    const int32_t* tri_material; /* n_triangles */
+    const int32_t* tri_prim_id;  /* n_triangles */
+
+    /* spheres (cx, cy, cz, r) */
+    int32_t n_spheres;
+    const float* spheres;         /* 4 * n_spheres */
+    const int32_t* sphere_material;
+    const int32_t* sphere_prim_id;
+
+    int32_t n_materials;
+    const rt_material* materials;
+
+    /* lights, in reference file order.  The primitive ID of light i is i. */
+    int32_t n_lights;
+    const rt_light* lights;
+
+    /* latitude-longitude environment map, row-major, RGB float, row 0 = +y
+     * pole.  Used by an RT_LIGHT_ENV light; NULL when there is none. */
+    int32_t env_width, env_height;
+    const float* env_rgb;
+} rt_scene_desc;
+
+/* Pinhole camera, src/camera.h:34-101: eye, look-at, up, vertical field of
+ * view in radians, resolution. */
+typedef struct rt_camera {
+    float eye[3];
+    float look_at[3];
+    float up[3];
+    float fov;
+    int32_t width, height;
+} rt_camera;
+
+/* One view of a (possibly batched) render: camera, stratified samples per
+ * pixel spp_x * spp_y (src/perpixelrenderer.cpp:78, pt-x/y-samples). */
+typedef struct rt_view {
+    rt_camera camera;
+    int32_t spp_x, spp_y;
+} rt_view;
+
+/* Estimator modes.
+ *  COMPAT:   the reference estimator, quirks included
+ *            (src/renderer.cpp:47-106, src/unidipathtracingintegrator.cpp:34-107)
+ *            with max_depth as a runtime parameter.
+ *  UNBIASED: next-event estimation with light-selection weight, emission only
+ *            on camera / specular chains, compensated Russian roulette.
+ *  GUIDED:   UNBIASED with BSDF sampling replaced by a one-sample MIS mixture
+ *            of the BSDF and a learned von Mises-Fisher mixture. */
+enum rt_mode { RT_MODE_COMPAT = 0, RT_MODE_UNBIASED = 1, RT_MODE_GUIDED = 2 };
+
+typedef struct rt_params {
+    int32_t mode;          /* rt_mode */
+    int32_t max_depth;     /* reference MAX_DEPTH (src/renderer.h:38) */
+    int32_t rr_start;      /* Russian roulette start (src/renderer.cpp:52) */
+    float rr_prob;         /* termination probability (--pt-termination-prob) */
+    uint32_t seed;         /* --seed */
+    int32_t reserved[3];
+} rt_params;
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RT_SCENE_H */

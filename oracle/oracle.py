@@ -1,0 +1,146 @@
+"""ctypes wrapper of the CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by the smoke check and by the
+CPU-baseline leg of bench.py, as the checker.  The product never imports it.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+
+def build(verbose=False):
+    res = subprocess.run(["make", "-C", HERE, "-B", "liboracle.so"], capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError("oracle build failed:\n" + res.stdout + res.stderr)
+    if verbose:
+        print(res.stdout)
+    return LIB_PATH
+
+
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        build()
+    from renderer_amd import _abi  # scene / view / params structs (input format)
+
+    lib = ctypes.CDLL(LIB_PATH)
+    P, I64 = ctypes.c_void_p, ctypes.c_int64
+    lib.oc_philox4x32_10.argtypes = [P, P, P]
+    lib.oc_scene_create.restype = P
+    lib.oc_scene_create.argtypes = [ctypes.POINTER(_abi.RtSceneDesc), ctypes.c_int]
+    lib.oc_scene_destroy.argtypes = [P]
+    lib.oc_trace_closest.argtypes = [P, I64, P, P, P, P, ctypes.c_int]
+    lib.oc_trace_any.argtypes = [P, I64, P, P, P, ctypes.c_int]
+    lib.oc_intersect_prim.restype = ctypes.c_float
+    lib.oc_intersect_prim.argtypes = [P, ctypes.c_int32, P, P]
+    lib.oc_render.restype = I64
+    lib.oc_render.argtypes = [P, ctypes.POINTER(_abi.RtView), ctypes.POINTER(_abi.RtParams), ctypes.c_uint32, P,
+                              ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    lib.oc_render_sample.argtypes = [P, ctypes.POINTER(_abi.RtView), ctypes.POINTER(_abi.RtParams),
+                                     ctypes.c_uint32, ctypes.c_int, ctypes.c_int, ctypes.c_int, P]
+    lib.oc_env_sample.argtypes = [P, ctypes.c_float, ctypes.c_float, P, P]
+    lib.oc_env_pdf.restype = ctypes.c_float
+    lib.oc_env_pdf.argtypes = [P, P]
+    _lib = lib
+    return lib
+
+
+def philox(ctr, key):
+    lib = load()
+    c = np.ascontiguousarray(ctr, np.uint32)
+    k = np.ascontiguousarray(key, np.uint32)
+    out = np.zeros(4, np.uint32)
+    lib.oc_philox4x32_10(c.ctypes.data, k.ctypes.data, out.ctypes.data)
+    return out
+
+
+class Oracle:
+    """The reference algorithms on the CPU over a flattened scene.
+    accel=1: the reference BVH (leaf = 1, both children); 0: linear list."""
+
+    def __init__(self, arrays, accel=1):
+        self.lib = load()
+        self.arrays = arrays
+        self.h = self.lib.oc_scene_create(ctypes.byref(arrays.desc()), int(accel))
+
+    def close(self):
+        if self.h:
+            self.lib.oc_scene_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def trace_closest(self, o_tmin, d_tmax, threads=8):
+        o = np.ascontiguousarray(o_tmin, np.float32)
+        d = np.ascontiguousarray(d_tmax, np.float32)
+        n = len(o)
+        prim = np.empty(n, np.int32)
+        t = np.empty(n, np.float32)
+        self.lib.oc_trace_closest(self.h, n, o.ctypes.data, d.ctypes.data, prim.ctypes.data, t.ctypes.data, threads)
+        return prim, t
+
+    def trace_any(self, o_tmin, d_tmax, threads=8):
+        o = np.ascontiguousarray(o_tmin, np.float32)
+        d = np.ascontiguousarray(d_tmax, np.float32)
+        occ = np.empty(len(o), np.uint8)
+        self.lib.oc_trace_any(self.h, len(o), o.ctypes.data, d.ctypes.data, occ.ctypes.data, threads)
+        return occ
+
+    def intersect_prim(self, prim, o_tmin, d_tmax):
+        o = np.ascontiguousarray(o_tmin, np.float32).reshape(4)
+        d = np.ascontiguousarray(d_tmax, np.float32).reshape(4)
+        return float(self.lib.oc_intersect_prim(self.h, int(prim), o.ctypes.data, d.ctypes.data))
+
+    @staticmethod
+    def _view_params(view, mode, max_depth, rr_start, rr_prob, seed):
+        from renderer_amd import _abi
+
+        cam, sx, sy = view
+        v = _abi.RtView()
+        v.camera = cam.to_c()
+        v.spp_x, v.spp_y = int(sx), int(sy)
+        p = _abi.RtParams()
+        p.mode, p.max_depth, p.rr_start, p.rr_prob, p.seed = int(mode), int(max_depth), int(rr_start), float(rr_prob), int(seed)
+        return v, p
+
+    def render(self, view, mode=1, max_depth=8, rr_start=3, rr_prob=0.5, seed=0, view_id=0, rows=None, threads=8,
+               film=None):
+        """view = (Camera, spp_x, spp_y).  Returns (film (H, W, 3), rays traced)."""
+        v, p = self._view_params(view, mode, max_depth, rr_start, rr_prob, seed)
+        cam = view[0]
+        if film is None:
+            film = np.zeros((cam.height, cam.width, 3), np.float32)
+        r0, r1 = (0, cam.height) if rows is None else rows
+        rays = self.lib.oc_render(self.h, ctypes.byref(v), ctypes.byref(p), int(view_id), film.ctypes.data, r0, r1,
+                                  threads)
+        return film, int(rays)
+
+    def render_sample(self, view, px, py, s, mode=1, max_depth=8, rr_start=3, rr_prob=0.5, seed=0, view_id=0):
+        v, p = self._view_params(view, mode, max_depth, rr_start, rr_prob, seed)
+        out = np.zeros(3, np.float32)
+        self.lib.oc_render_sample(self.h, ctypes.byref(v), ctypes.byref(p), int(view_id), px, py, s, out.ctypes.data)
+        return out
+
+    def env_sample(self, u0, u1):
+        d = np.zeros(3, np.float32)
+        pdf = np.zeros(1, np.float32)
+        self.lib.oc_env_sample(self.h, float(u0), float(u1), d.ctypes.data, pdf.ctypes.data)
+        return d, float(pdf[0])
+
+    def env_pdf(self, d):
+        d = np.ascontiguousarray(d, np.float32)
+        return float(self.lib.oc_env_pdf(self.h, d.ctypes.data))

@@ -1,0 +1,1214 @@
+/*
+ * pt_oracle.cpp — CPU oracle for the path-tracing hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP path and the CPU
+ * baseline of the benchmark.  Nothing in the product links or calls it.
+ *
+ * This is a restatement of the reference renderer's own algorithms in its
+ * own object model (virtual Intersectable / Material / Light classes, a
+ * leaf-1 pointer BVH whose traversal visits both children, std::list paths,
+ * an OpenMP loop over rows with a critical section per sample), written
+ * without Qt, OpenCV, GSL or Boost.  Each function cites the reference
+ * file:line it follows.  Vector arithmetic follows the Qt 5 semantics the
+ * reference relies on: float component arithmetic, length() and
+ * normalized() evaluated in double and rounded to float.
+ *
+ * Random numbers: the reference draws from one GSL taus2 stream per image
+ * row (src/perpixelrenderer.cpp:67-68).  Here every draw comes from the
+ * counter-based Philox-4x32-10 stream that the GPU path uses; the layout of
+ * the stream is fixed by philox_uniforms() below and is the contract that
+ * lets the GPU image be compared pixel by pixel with this one.
+ */
+#include "pt_oracle.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <list>
+#include <memory>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Philox-4x32-10, Salmon et al., "Parallel random numbers: as easy as 1, 2, 3"
+// (SC'11).  Constants from the paper: multipliers 0xD2511F53 / 0xCD9E8D57,
+// Weyl key increments 0x9E3779B9 / 0xBB67AE85.
+// ---------------------------------------------------------------------------
+void philox(const uint32_t in[4], const uint32_t k[2], uint32_t out[4])
+{
+    uint32_t c0 = in[0], c1 = in[1], c2 = in[2], c3 = in[3];
+    uint32_t k0 = k[0], k1 = k[1];
+    for (int r = 0; r < 10; ++r) {
+        if (r > 0) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+        uint64_t a = (uint64_t)0xD2511F53u * c0;
+        uint64_t b = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t n0 = (uint32_t)(b >> 32) ^ c1 ^ k0;
+        uint32_t n2 = (uint32_t)(a >> 32) ^ c3 ^ k1;
+        c0 = n0; c1 = (uint32_t)b; c2 = n2; c3 = (uint32_t)a;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+// The random stream of one camera sample: counter = (pixel, sample index,
+// dimension group, 0), key = (seed, view).  Group 0: pixel jitter (u0,u1)
+// and the path-length draw (u2).  Group 1+2i: BSDF sample of bounce i (u0,u1),
+// lobe pick (u2), roulette (u3).  Group 2+2k: light sample of stored vertex /
+// bounce k (u0,u1), light index (u2).  Uniforms are the top 24 bits / 2^24.
+struct U4 { float u[4]; };
+U4 philox_uniforms(uint32_t seed, uint32_t view, uint32_t pixel, uint32_t sample, uint32_t group)
+{
+    uint32_t c[4] = {pixel, sample, group, 0u}, k[2] = {seed, view}, o[4];
+    philox(c, k, o);
+    U4 r;
+    for (int i = 0; i < 4; ++i) r.u[i] = (float)(o[i] >> 8) * (1.0f / 16777216.0f);
+    return r;
+}
+
+// ---------------------------------------------------------------------------
+// Vector type with the Qt 5 QVector3D semantics used by the reference.
+// ---------------------------------------------------------------------------
+struct V3 {
+    float x = 0, y = 0, z = 0;
+    V3() = default;
+    V3(float a, float b, float c) : x(a), y(b), z(c) {}
+    float operator[](int i) const { return i == 0 ? x : (i == 1 ? y : z); }
+    float& at(int i) { return i == 0 ? x : (i == 1 ? y : z); }
+};
+inline V3 operator+(V3 a, V3 b) { return V3(a.x + b.x, a.y + b.y, a.z + b.z); }
+inline V3 operator-(V3 a, V3 b) { return V3(a.x - b.x, a.y - b.y, a.z - b.z); }
+inline V3 operator-(V3 a) { return V3(-a.x, -a.y, -a.z); }
+inline V3 operator*(float s, V3 a) { return V3(a.x * s, a.y * s, a.z * s); }
+inline V3 operator*(V3 a, float s) { return V3(a.x * s, a.y * s, a.z * s); }
+inline bool is_zero(V3 a) { return a.x == 0 && a.y == 0 && a.z == 0; }
+inline float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline V3 cross(V3 a, V3 b)
+{
+    return V3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+inline float length_sq(V3 a) { return a.x * a.x + a.y * a.y + a.z * a.z; }
+inline float length(V3 a)
+{
+    double l = (double)a.x * a.x + (double)a.y * a.y + (double)a.z * a.z;
+    return (float)std::sqrt(l);
+}
+inline V3 normalized(V3 a)
+{
+    double l = (double)a.x * a.x + (double)a.y * a.y + (double)a.z * a.z;
+    if (l == 0.0) return V3();
+    double s = std::sqrt(l);
+    return V3((float)(a.x / s), (float)(a.y / s), (float)(a.z / s));
+}
+// src/vechelper.cpp:3-6
+inline V3 reflect(V3 v, V3 n) { return v - (2 * dot(v, n)) * n; }
+
+// RGB colour (cv::Vec3f in the reference, stored there in BGR order).
+struct Col {
+    float r = 0, g = 0, b = 0;
+    Col() = default;
+    Col(float a, float b_, float c) : r(a), g(b_), b(c) {}
+    bool zero() const { return r == 0 && g == 0 && b == 0; }
+};
+inline Col operator+(Col a, Col b) { return Col(a.r + b.r, a.g + b.g, a.b + b.b); }
+inline Col operator*(Col a, float s) { return Col(a.r * s, a.g * s, a.b * s); }
+inline Col mul(Col a, Col b) { return Col(a.r * b.r, a.g * b.g, a.b * b.b); }
+// src/mathhelper.cpp:102-105 (.299 R + .587 G + .114 B, evaluated in double)
+inline float lum(Col c) { return (float)(.299 * c.r + .587 * c.g + .114 * c.b); }
+
+const float kEps = 0.001f;   // src/ray.h:34 (EPSILON 0.001, passed as float)
+const float kInf = std::numeric_limits<float>::infinity();
+const float kNaN = std::numeric_limits<float>::quiet_NaN();
+
+// src/ray.h:36-111
+struct Ray {
+    V3 o, d;
+    float from = kEps, to = kInf;
+    Ray() = default;
+    Ray(V3 o_, V3 d_, float f = kEps, float t = kInf) : o(o_), d(d_), from(f), to(t) {}
+    bool in_range(float u) const { return from <= u && u <= to; }
+};
+
+struct Material;
+
+// src/hitrecord.h:35-121: t = NaN is a miss; the point has w = 0 at t = inf.
+struct Hit {
+    float t = kNaN;
+    V3 p;
+    float pw = 1;
+    V3 n;
+    Ray ray;
+    const Material* mat = nullptr;
+    int prim = -1;
+    bool hits() const { return !std::isnan(t); }
+};
+Hit make_hit(float t, const Ray& ray, const Material* m, V3 normal, int prim)
+{
+    Hit h;
+    h.t = t;
+    h.ray = ray;
+    h.mat = m;
+    h.prim = prim;
+    h.n = normalized(normal);
+    if (std::isinf(t)) {  // src/ray.h:91-101
+        h.p = (t > 0 ? 1.f : -1.f) * normalized(ray.d);
+        h.pw = 0;
+    } else {
+        h.p = ray.o + t * ray.d;
+        h.pw = 1;
+    }
+    return h;
+}
+
+// ---------------------------------------------------------------------------
+// Sampling warps, src/sampler.cpp:33-86.  The sample is a QPointF (double).
+// ---------------------------------------------------------------------------
+struct Sample { double x, y; };
+
+V3 cosine_power_dir(Sample s, V3 w, float& pdf, float n)
+{
+    const double rf = 1.0;  // sin(openingAngle = pi/2)
+    w = normalized(w);
+    float e2 = 2 / (n + 1), e1 = 1 / (n + 1);
+    double rad = std::sqrt(1 - std::pow(s.x, (double)e2) * rf * rf);
+    V3 nd((float)(std::cos(2 * M_PI * s.y) * rad), (float)(std::sin(2 * M_PI * s.y) * rad),
+          (float)(std::pow(s.x, (double)e1) * rf));
+    pdf = (float)(std::pow((double)nd.z, (double)n) * (n + 1) / (2 * M_PI));
+    V3 v;
+    if (std::fabs(w.x) < std::fabs(w.y) && std::fabs(w.x) < std::fabs(w.z))
+        v = V3(0, w.z, -w.y);
+    else if (std::fabs(w.y) < std::fabs(w.z))
+        v = V3(w.z, 0, -w.x);
+    else
+        v = V3(w.y, -w.x, 0);
+    v = normalized(v);
+    V3 u = cross(v, w);
+    V3 r(nd.x * u.x + nd.y * v.x + nd.z * w.x, nd.x * u.y + nd.y * v.y + nd.z * w.y,
+         nd.x * u.z + nd.y * v.z + nd.z * w.z);
+    return normalized(r);
+}
+inline V3 cosine_dir(Sample s, V3 w, float& pdf) { return cosine_power_dir(s, w, pdf, 1); }
+
+// ---------------------------------------------------------------------------
+// Intersectables
+// ---------------------------------------------------------------------------
+struct Box { V3 mn{kInf, kInf, kInf}, mx{-kInf, -kInf, -kInf}; };
+void include_point(Box& b, V3 p)  // src/axisalignedbox.cpp:90-99
+{
+    for (int i = 0; i < 3; ++i) {
+        if (p[i] < b.mn[i]) b.mn.at(i) = p[i];
+        if (p[i] > b.mx[i]) b.mx.at(i) = p[i];
+    }
+}
+void include_box(Box& b, const Box& o) { include_point(b, o.mn); include_point(b, o.mx); }
+
+// Slab test, src/axisalignedbox.cpp:30-73.  Returns false on tmin > tmax.
+bool slab(const Box& b, const Ray& ray, float& tmin, float& tmax)
+{
+    tmin = -kInf;
+    tmax = kInf;
+    for (int i = 0; i < 3; ++i) {
+        float a = 1 / ray.d[i];
+        float t0, t1;
+        if (a >= 0) { t0 = b.mn[i] - ray.o[i]; t1 = b.mx[i] - ray.o[i]; }
+        else { t0 = b.mx[i] - ray.o[i]; t1 = b.mn[i] - ray.o[i]; }
+        t0 *= a;
+        t1 *= a;
+        if (t0 > tmin) tmin = t0;
+        if (t1 < tmax) tmax = t1;
+        if (tmin > tmax) return false;
+    }
+    return true;
+}
+
+struct Intersectable {
+    Box box;
+    virtual ~Intersectable() {}
+    virtual Hit intersect(const Ray& ray) const = 0;
+};
+
+// src/triangle.cpp:30-56, ctor src/triangle.h:35-38
+struct Triangle : Intersectable {
+    V3 p1, p2, p3, n1, n2, n3;
+    const Material* m;
+    int prim;
+    Triangle(V3 a, V3 b, V3 c, const Material* mat, int id) : p1(a), p2(b), p3(c), m(mat), prim(id)
+    {
+        n1 = n2 = n3 = normalized(cross(p1 - p2, p1 - p3));
+        include_point(box, p1); include_point(box, p2); include_point(box, p3);
+    }
+    void set_normals(V3 a, V3 b, V3 c) { n1 = normalized(a); n2 = normalized(b); n3 = normalized(c); }
+    Hit intersect(const Ray& ray) const override
+    {
+        V3 ad = ray.o - p1;
+        V3 abxac = cross(p2 - p1, p3 - p1);
+        float w = -dot(abxac, ray.d);
+        if (w == 0) return Hit();
+        float t = dot(abxac, ad) / w;
+        if (!ray.in_range(t)) return Hit();
+        V3 adxde = cross(ad, ray.d);
+        float u = dot(adxde, p3 - p1) / w;
+        if (u < 0 || u > 1) return Hit();
+        float v = -dot(adxde, p2 - p1) / w;
+        if (v < 0 || u + v > 1) return Hit();
+        V3 normal = (1 - u - v) * n1 + u * n2 + v * n3;
+        if (is_zero(normal)) return Hit();
+        return make_hit(t, ray, m, normal, prim);
+    }
+};
+
+// src/quad.h:32, src/quad.cpp:23-41: t1 = (p1,p2,p4) first, then t2 = (p4,p2,p3)
+struct Quad : Intersectable {
+    Triangle t1, t2;
+    Quad(Triangle a, Triangle b) : t1(a), t2(b) { box = t1.box; include_box(box, t2.box); }
+    Hit intersect(const Ray& ray) const override
+    {
+        Hit h = t1.intersect(ray);
+        return h.hits() ? h : t2.intersect(ray);
+    }
+};
+
+// src/sphere.cpp:29-56 + src/csgobject.cpp:25-44
+struct Sphere : Intersectable {
+    V3 c;
+    float r;
+    const Material* m;
+    int prim;
+    Sphere(V3 c_, float r_, const Material* mat, int id) : c(c_), r(r_), m(mat), prim(id)
+    {
+        V3 rv(r, r, r);
+        box.mn = c - rv;
+        box.mx = c + rv;
+    }
+    Hit intersect(const Ray& ray) const override
+    {
+        V3 diff = ray.o - c;
+        float r2 = r * r;
+        float a = length_sq(ray.d);
+        float b = 2 * dot(ray.d, diff);
+        float cc = length_sq(diff) - r2;
+        float disc = b * b - 4 * a * cc;
+        if (!(disc >= 0)) return Hit();
+        float root = std::sqrt(disc);
+        float ts[2] = {(-b - root) / (2 * a), (-b + root) / (2 * a)};
+        if (ray.from > ts[1]) return Hit();
+        for (float t : ts) {
+            if (t >= ray.from) {
+                if (t > ray.to) return Hit();
+                V3 n = (ray.o + t * ray.d) - c;
+                return make_hit(t, ray, m, n, prim);
+            }
+        }
+        return Hit();
+    }
+};
+
+// src/intersectablelist.cpp:30-48: linear scan with ray shortening; ties go
+// to the later component.
+struct List : Intersectable {
+    std::vector<Intersectable*> items;
+    Hit intersect(const Ray& r0) const override
+    {
+        Ray ray = r0;
+        Hit hit;
+        for (auto* it : items) {
+            Hit h = it->intersect(ray);
+            if (h.hits()) {
+                ray = Ray(ray.o, ray.d, ray.from, h.t);
+                hit = h;
+            }
+        }
+        return hit;
+    }
+};
+
+// src/bvh.cpp:135-149: slab test of the node's own box with the unshortened
+// ray, then both children; the smaller t wins, ties go to the right child.
+struct BVHNode : Intersectable {
+    Intersectable *l, *r;
+    Hit intersect(const Ray& ray) const override
+    {
+        float t0, t1;
+        if (!slab(box, ray, t0, t1) || t0 > ray.to || t1 < ray.from) return Hit();
+        Hit lh = l->intersect(ray);
+        Hit rh = r->intersect(ray);
+        if (!rh.hits() || lh.t < rh.t) return lh;
+        return rh;
+    }
+};
+
+// src/bvh.cpp:67-94: longest axis (strict >, ties to the later axis), sort by
+// box centre, split at n/2, one primitive per leaf.
+Intersectable* build_bvh(std::vector<Intersectable*> v, std::vector<std::unique_ptr<Intersectable>>& pool)
+{
+    if (v.size() == 1) return v[0];
+    Box bb;
+    for (auto* it : v) include_box(bb, it->box);
+    V3 diff = bb.mx - bb.mn;
+    int axis = diff.x > diff.y && diff.x > diff.z ? 0 : (diff.y > diff.z ? 1 : 2);
+    std::sort(v.begin(), v.end(), [axis](Intersectable* a, Intersectable* b) {
+        return (a->box.mx + a->box.mn)[axis] < (b->box.mx + b->box.mn)[axis];
+    });
+    size_t half = v.size() / 2;
+    std::vector<Intersectable*> left(v.begin(), v.begin() + half), right(v.begin() + half, v.end());
+    auto* node = new BVHNode();
+    pool.emplace_back(node);
+    node->box = bb;
+    node->l = build_bvh(left, pool);
+    node->r = build_bvh(right, pool);
+    return node;
+}
+
+// ---------------------------------------------------------------------------
+// Materials, src/material.h:30-50
+// ---------------------------------------------------------------------------
+struct Material {
+    virtual ~Material() {}
+    virtual Col brdf(const Hit&, V3) const { return Col(); }
+    virtual bool specular() const { return false; }
+    virtual Col emission(const Hit&) const { return Col(); }
+    virtual bool emits() const { return false; }
+    virtual V3 out_direction(const Hit& hit, Sample s, float& pdf, Col& brdf) const = 0;
+    virtual int kind() const = 0;
+};
+
+// src/darkmatter.cpp:32-39
+struct DarkMatter : Material {
+    V3 out_direction(const Hit&, Sample, float& pdf, Col& brdf) const override
+    {
+        brdf = Col();
+        pdf = 0;
+        return V3();
+    }
+    int kind() const override { return RT_MAT_DARK; }
+};
+
+// src/abstractdiffusematerial.cpp:8-15, src/diffusematerial.cpp:25-28
+struct Diffuse : Material {
+    Col kd;
+    explicit Diffuse(Col c) : kd(c) {}
+    Col brdf(const Hit&, V3) const override { return kd * (float)(1 / M_PI); }
+    V3 out_direction(const Hit& hit, Sample s, float& pdf, Col& b) const override
+    {
+        V3 n = hit.n;
+        if (dot(normalized(n), normalized(hit.ray.d)) > 0) n = n * -1.f;
+        V3 out = cosine_dir(s, n, pdf);
+        b = brdf(hit, out);
+        return out;
+    }
+    int kind() const override { return RT_MAT_DIFFUSE; }
+};
+
+// src/mirrormaterial.cpp:31-41
+struct Mirror : Material {
+    V3 out_direction(const Hit& hit, Sample, float& pdf, Col& b) const override
+    {
+        pdf = 1;
+        b = Col();
+        return reflect(hit.ray.d, normalized(hit.n));
+    }
+    bool specular() const override { return true; }
+    int kind() const override { return RT_MAT_MIRROR; }
+};
+
+// src/phongmaterial.cpp:31-81
+struct Phong : Material {
+    Col kd, ks;
+    float n;
+    Phong(Col d, Col s, float e) : kd(d), ks(s), n(e) {}
+    Col diffuse_brdf() const { return kd * (float)(1 / M_PI); }
+    Col specular_brdf(const Hit& hit, V3 dir) const
+    {
+        V3 normal = normalized(hit.n);
+        V3 refl = reflect(dir, normal);
+        float dp = std::max(dot(normalized(refl), -normalized(hit.ray.d)), 0.f);
+        float spec = (float)((n + 2) / (2 * M_PI) * std::pow((double)dp, (double)n));
+        return ks * spec;
+    }
+    Col brdf(const Hit& hit, V3 dir) const override
+    {
+        return diffuse_brdf() + specular_brdf(hit, dir);
+    }
+    V3 out_direction(const Hit& hit, Sample s, float& pdf, Col& b) const override
+    {
+        V3 nrm = hit.n;
+        if (dot(hit.ray.d, nrm) > 0) nrm = nrm * -1.f;
+        nrm = normalized(nrm);
+        V3 spec_dir = reflect(hit.ray.d, nrm);
+        float rd = lum(kd), rs = lum(ks);
+        if (s.x < rd) {
+            s.x /= rd;
+            V3 out = cosine_dir(s, nrm, pdf);
+            b = diffuse_brdf();
+            return out;
+        } else if (s.x < rd + rs) {
+            s.x = (s.x - rd) / rs;
+            V3 out = cosine_power_dir(s, spec_dir, pdf, n);
+            if (dot(out, nrm) < 1) pdf = 0;
+            b = specular_brdf(hit, out);
+            return out;
+        }
+        pdf = 0;
+        b = Col();
+        return V3();
+    }
+    int kind() const override { return RT_MAT_PHONG; }
+};
+
+// ---------------------------------------------------------------------------
+// Lights, src/light.h:31-37
+// ---------------------------------------------------------------------------
+struct Scene;
+
+struct Light {
+    virtual ~Light() {}
+    // NEE: returns the light's contribution and sets `direction` to the
+    // vector from the light towards the shaded point.
+    virtual Col intensity(const Hit& hit, V3& direction, const Scene& sc, Sample s, int64_t& rays) const = 0;
+};
+
+struct EnvMap;
+
+struct Scene {
+    std::vector<std::unique_ptr<Intersectable>> owned;
+    std::vector<std::unique_ptr<Material>> mats;
+    std::vector<Light*> lights;
+    std::vector<Intersectable*> prims;   // by primitive ID (null for non-intersectable lights)
+    Intersectable* root = nullptr;
+    const EnvMap* env = nullptr;
+    Hit intersect(const Ray& ray) const;
+};
+
+// src/arealight.cpp:31-119
+struct AreaLight : Intersectable, Material, Light {
+    V3 o, u, v, nrm;
+    Col I;
+    int prim;
+    AreaLight(V3 o_, V3 u_, V3 v_, Col i, int id) : o(o_), u(u_), v(v_), I(i), prim(id)
+    {
+        nrm = cross(u, v);
+        include_point(box, o); include_point(box, o + u); include_point(box, o + v);
+        include_point(box, o + u + v);
+    }
+    V3 location(Sample s) const { return o + (float)s.x * u + (float)s.y * v; }
+    Col intensity_dir(V3 direction) const
+    {
+        float c = dot(normalized(direction), normalized(nrm)) / length_sq(direction) * length(nrm);
+        if (c < 0) c = 0;
+        return I * c;
+    }
+    Col intensity(const Hit& hit, V3& direction, const Scene& sc, Sample s, int64_t& rays) const override
+    {
+        V3 at = hit.p;
+        V3 y = location(s);
+        direction = at - y;
+        Ray sr(at, -normalized(direction), kEps, (float)((double)length(direction) - 0.001));
+        ++rays;
+        Hit sh = sc.intersect(sr);
+        if (sh.hits() && sh.mat != static_cast<const Material*>(this)) return Col();
+        return intensity_dir(direction);
+    }
+    Hit intersect(const Ray& ray) const override
+    {
+        V3 ad = ray.o - o;
+        V3 abxac = cross(u, v);
+        float w = -dot(abxac, ray.d);
+        if (w == 0) return Hit();
+        float t = dot(abxac, ad) / w;
+        if (!ray.in_range(t)) return Hit();
+        V3 adxde = cross(ad, ray.d);
+        float uu = dot(adxde, v) / w;
+        if (uu < 0 || uu > 1) return Hit();
+        float vv = -dot(adxde, u) / w;
+        if (vv < 0 || vv > 1) return Hit();
+        return make_hit(t, ray, this, nrm, prim);
+    }
+    Col emission(const Hit& hit) const override { return intensity_dir(-hit.ray.d); }
+    bool emits() const override { return true; }
+    V3 out_direction(const Hit&, Sample, float& pdf, Col& b) const override
+    {
+        b = Col();
+        pdf = 0;
+        return V3();
+    }
+    int kind() const override { return -1; }
+};
+
+// src/pointlight.cpp:29-70
+struct PointLight : Intersectable, Light {
+    V3 p;
+    Col I;
+    PointLight(V3 p_, Col i) : p(p_), I(i) { box.mn = V3(); box.mx = V3(); }
+    Col intensity(const Hit& hit, V3& direction, const Scene& sc, Sample, int64_t& rays) const override
+    {
+        V3 at = hit.p;
+        direction = at - p;
+        Ray sr(at, -normalized(direction), kEps, (float)((double)length(direction) - 0.001));
+        ++rays;
+        if (sc.intersect(sr).hits()) return Col();
+        return I * (1 / length_sq(direction));
+    }
+    Hit intersect(const Ray&) const override { return Hit(); }
+};
+
+// Latitude-longitude environment light.  Lookup direction -> texel:
+// theta = acos(y), phi = atan2(x, -z), column = (phi/2pi + 1/2) W,
+// row = theta/pi H (nearest texel).  NEE follows src/environmentmap.cpp:40-55
+// (cosine sample about the unflipped normal, backwards shadow ray, value over
+// pdf) in the reference-compatible mode.
+struct EnvMap : Material, Light {
+    int W = 0, H = 0;
+    std::vector<float> rgb;
+    Col coef;
+    int prim;
+    std::vector<double> row_cdf;              // H + 1
+    std::vector<std::vector<double>> col_cdf; // H x (W + 1)
+    std::vector<double> texel_p;              // W * H, probability of each texel
+    Col texel(int x, int y) const
+    {
+        const float* c = &rgb[3 * ((size_t)y * W + x)];
+        return Col(c[0], c[1], c[2]);
+    }
+    void texel_of(V3 d, int& x, int& y) const
+    {
+        double dy = std::max(-1.0, std::min(1.0, (double)d.y));
+        double theta = std::acos(dy);
+        double phi = std::atan2((double)d.x, -(double)d.z);
+        double u = phi / (2 * M_PI) + 0.5, v = theta / M_PI;
+        x = std::min(std::max((int)(u * W), 0), W - 1);
+        y = std::min(std::max((int)(v * H), 0), H - 1);
+    }
+    Col get(V3 d) const
+    {
+        int x, y;
+        texel_of(normalized(d), x, y);
+        return mul(texel(x, y), coef);
+    }
+    void build_sampling()
+    {
+        row_cdf.assign(H + 1, 0.0);
+        col_cdf.assign(H, std::vector<double>(W + 1, 0.0));
+        texel_p.assign((size_t)W * H, 0.0);
+        std::vector<double> roww(H, 0.0);
+        double total = 0;
+        for (int y = 0; y < H; ++y) {
+            double st = std::sin((y + 0.5) / H * M_PI);
+            double acc = 0;
+            for (int x = 0; x < W; ++x) {
+                double w = (double)lum(mul(texel(x, y), coef)) * st;
+                texel_p[(size_t)y * W + x] = w;
+                acc += w;
+                col_cdf[y][x + 1] = acc;
+            }
+            for (int x = 0; x <= W; ++x) col_cdf[y][x] = acc > 0 ? col_cdf[y][x] / acc : (double)x / W;
+            roww[y] = acc;
+            total += acc;
+        }
+        double acc = 0;
+        for (int y = 0; y < H; ++y) { acc += roww[y]; row_cdf[y + 1] = total > 0 ? acc / total : (double)(y + 1) / H; }
+        for (auto& p : texel_p) p = total > 0 ? p / total : 1.0 / ((double)W * H);
+    }
+    static int pick(const std::vector<double>& cdf, double u, double& frac)
+    {
+        int n = (int)cdf.size() - 1;
+        int i = (int)(std::upper_bound(cdf.begin(), cdf.end(), u) - cdf.begin()) - 1;
+        i = std::min(std::max(i, 0), n - 1);
+        double w = cdf[i + 1] - cdf[i];
+        frac = w > 0 ? (u - cdf[i]) / w : 0.5;
+        frac = std::min(std::max(frac, 0.0), 1.0 - 1e-7);
+        return i;
+    }
+    void sample(float u0, float u1, V3& d, float& pdf) const
+    {
+        double fy, fx;
+        int y = pick(row_cdf, u0, fy);
+        int x = pick(col_cdf[y], u1, fx);
+        double theta = (y + fy) / H * M_PI, phi = ((x + fx) / W - 0.5) * 2 * M_PI;
+        double st = std::sin(theta);
+        d = V3((float)(st * std::sin(phi)), (float)std::cos(theta), (float)(-st * std::cos(phi)));
+        pdf = st > 0 ? (float)(texel_p[(size_t)y * W + x] * W * H / (2 * M_PI * M_PI * st)) : 0.f;
+    }
+    float pdf(V3 d) const
+    {
+        d = normalized(d);
+        int x, y;
+        texel_of(d, x, y);
+        double st = std::sqrt(std::max(0.0, 1.0 - (double)d.y * d.y));
+        return st > 0 ? (float)(texel_p[(size_t)y * W + x] * W * H / (2 * M_PI * M_PI * st)) : 0.f;
+    }
+    Col intensity(const Hit& hit, V3& direction, const Scene& sc, Sample s, int64_t& rays) const override
+    {
+        float pdfv;
+        direction = -cosine_dir(s, hit.n, pdfv);
+        Ray sr(hit.p, direction, -kInf, -kEps);
+        ++rays;
+        if (sc.intersect(sr).hits()) return Col();
+        return get(direction) * (1 / pdfv);
+    }
+    Col emission(const Hit& hit) const override { return get(normalized(hit.ray.d)); }
+    bool emits() const override { return true; }
+    V3 out_direction(const Hit&, Sample, float& pdf, Col& b) const override
+    {
+        b = Col();
+        pdf = 0;
+        return V3();
+    }
+    int kind() const override { return -2; }
+};
+
+// Closest hit against the scene; the environment answers rays that leave the
+// scene with an unbounded range (src/environmentmap.cpp:73-83).
+Hit Scene::intersect(const Ray& ray) const
+{
+    Hit h = root ? root->intersect(ray) : Hit();
+    if (!h.hits() && env && std::isinf(ray.to) && ray.to > 0)
+        h = make_hit(ray.to, ray, env, -normalized(ray.d), env->prim);
+    return h;
+}
+
+// Shadow query of the unbiased estimator: geometry only, no environment.
+Hit root_any(const Scene& sc, const Ray& r) { return sc.root ? sc.root->intersect(r) : Hit(); }
+
+// ---------------------------------------------------------------------------
+// Camera, src/camera.cpp:34-58
+// ---------------------------------------------------------------------------
+struct Camera {
+    V3 eye, u, v, w;
+    float t, b, l, r;
+    int W, H;
+    explicit Camera(const rt_camera& c)
+    {
+        eye = V3(c.eye[0], c.eye[1], c.eye[2]);
+        V3 la(c.look_at[0], c.look_at[1], c.look_at[2]), up(c.up[0], c.up[1], c.up[2]);
+        W = c.width;
+        H = c.height;
+        w = normalized(eye - la);
+        u = normalized(cross(up, w));
+        v = cross(w, u);
+        float aspect = (float)W / H;
+        t = (float)std::tan((double)(c.fov / 2));
+        b = -t;
+        r = aspect * t;
+        l = -r;
+    }
+    Ray ray(double px, double py) const
+    {
+        float x = (float)(l + (r - l) * (px + 0.5) / W);
+        float y = (float)(t + (b - t) * (py + 0.5) / H);
+        V3 d(x * u.x + y * v.x + -1.f * w.x, x * u.y + y * v.y + -1.f * w.y, x * u.z + y * v.z + -1.f * w.z);
+        return Ray(eye, d);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Reference path sampler and integrator ("compat" mode)
+// ---------------------------------------------------------------------------
+struct Path {  // src/path.h:31-36
+    std::list<Col> alpha;
+    std::list<Hit> hits;
+};
+
+struct SampleKey { uint32_t seed, view, pixel, sample; };
+
+// Renderer::createPath, src/renderer.cpp:47-106, with MAX_DEPTH = max_depth
+// and the negative-binomial length draw made from one uniform by inversion.
+Path create_path(const Scene& sc, const Ray& primary, const SampleKey& k, int max_depth, int rr_start,
+                 float p, int64_t& rays)
+{
+    U4 g0 = philox_uniforms(k.seed, k.view, k.pixel, k.sample, 0);
+    int L;
+    if (p > 0) {
+        int nb = 0;
+        if (p < 1) {
+            double u = 1.0 - (double)g0.u[2];
+            double q = std::floor(std::log(u) / std::log(1.0 - (double)p));
+            nb = q > 1e6 ? 1000000 : (int)q;
+        }
+        L = std::min(rr_start + nb, max_depth);
+    } else {
+        p = 1;
+        L = max_depth;
+    }
+    Path res;
+    Col alpha(1, 1, 1);
+    ++rays;
+    Hit hit = sc.intersect(primary);
+    for (int i = 0; i < L; ++i) {
+        if (!hit.hits()) return res;
+        U4 gs = philox_uniforms(k.seed, k.view, k.pixel, k.sample, 1 + 2 * i);
+        Sample s{gs.u[0], gs.u[1]};
+        float pdf = 0;
+        Col brdf;
+        V3 out = hit.mat->out_direction(hit, s, pdf, brdf);
+        if (hit.mat->emits()) {
+            res.alpha.push_back(alpha);
+            res.hits.push_back(hit);
+            return res;
+        } else if (hit.mat->specular()) {
+            if (i == L - 1 && L < max_depth) ++L;
+        } else {
+            res.alpha.push_back(alpha);
+            res.hits.push_back(hit);
+            if (pdf == 0 || alpha.zero()) return res;
+            float c = std::fabs(dot(normalized(out), normalized(hit.n)));
+            alpha = mul(alpha, brdf) * (c / pdf);
+            if (i > rr_start) alpha = alpha * (1 / p);
+        }
+        ++rays;
+        hit = sc.intersect(Ray(hit.p, out));
+    }
+    return res;
+}
+
+inline int signum(float x) { return (0.f < x) - (x < 0.f); }
+
+// UniDiPathTracingIntegrator::integrate(Ray) + integrate(Path),
+// src/unidipathtracingintegrator.cpp:34-107
+Col integrate_compat(const Scene& sc, const Ray& ray, const SampleKey& k, int max_depth, int rr_start, float p,
+                     int64_t& rays)
+{
+    Path path = create_path(sc, ray, k, max_depth, rr_start, p, rays);
+    Col color;
+    auto ait = path.alpha.begin();
+    auto hit = path.hits.begin();
+    if (hit != path.hits.end() && !hit->mat->emission(*hit).zero()) return hit->mat->emission(*hit);
+    int i = 0;
+    const size_t nl = sc.lights.size();
+    while (ait != path.alpha.end() && hit != path.hits.end()) {
+        V3 direction;
+        Col li;
+        auto next = hit;
+        ++next;
+        if (next != path.hits.end() && next->mat->emits()) {
+            li = next->mat->emission(*next);
+            if (next->pw == 0)
+                direction = -next->p;
+            else
+                direction = hit->p - next->p;
+        } else if (nl > 0) {
+            U4 gl = philox_uniforms(k.seed, k.view, k.pixel, k.sample, 2 + 2 * i);
+            size_t idx = std::min((size_t)(gl.u[2] * (float)nl), nl - 1);
+            li = sc.lights[idx]->intensity(*hit, direction, sc, Sample{gl.u[0], gl.u[1]}, rays);
+        }
+        Col brdf = hit->mat->brdf(*hit, direction);
+        float in_cos = dot(-normalized(direction), normalized(hit->n));
+        float out_cos = dot(-normalized(hit->ray.d), normalized(hit->n));
+        if (signum(in_cos) == signum(out_cos)) color = color + mul(mul(*ait, brdf), li) * std::fabs(in_cos);
+        ++ait;
+        ++hit;
+        ++i;
+    }
+    return color;
+}
+
+// ---------------------------------------------------------------------------
+// "Unbiased" estimator: next-event estimation with the light-selection
+// weight, emission counted on camera and specular chains only, Russian
+// roulette compensated by 1/(1-q), Phong sampled as a proper lobe mixture.
+// This is not in the reference; it is the textbook estimator (Pharr, Jakob,
+// Humphreys, PBRT 3rd ed., ch. 14) restated with the reference materials.
+// ---------------------------------------------------------------------------
+struct Frame { V3 t, b, n; };
+Frame frame_about(V3 w)  // the same basis rule as src/sampler.cpp:55-68
+{
+    V3 v;
+    if (std::fabs(w.x) < std::fabs(w.y) && std::fabs(w.x) < std::fabs(w.z))
+        v = V3(0, w.z, -w.y);
+    else if (std::fabs(w.y) < std::fabs(w.z))
+        v = V3(w.z, 0, -w.x);
+    else
+        v = V3(w.y, -w.x, 0);
+    v = normalized(v);
+    return Frame{cross(v, w), v, w};
+}
+V3 lobe_dir(float u0, float u1, V3 axis, float expo)  // cos^expo lobe about axis (unit)
+{
+    float z = std::pow(u0, 1.f / (expo + 1.f));
+    float r = std::sqrt(std::max(0.f, 1.f - z * z));
+    float ph = 6.28318530718f * u1;
+    Frame f = frame_about(axis);
+    V3 d = (r * std::cos(ph)) * f.t + (r * std::sin(ph)) * f.b + z * f.n;
+    return normalized(d);
+}
+
+struct UnbiasedCtx { const Scene* sc; int max_depth, rr_start; float rr_prob; };
+
+// BSDF value for wo -> wi at a non-specular vertex with viewer-side normal nf
+Col bsdf_eval(const Material* m, V3 wo, V3 wi, V3 nf)
+{
+    const float inv_pi = 0.318309886f;
+    if (m->kind() == RT_MAT_DIFFUSE) return static_cast<const Diffuse*>(m)->kd * inv_pi;
+    if (m->kind() == RT_MAT_PHONG) {
+        auto* ph = static_cast<const Phong*>(m);
+        V3 r = reflect(-wo, nf);
+        float c = std::max(0.f, dot(r, wi));
+        float s = (ph->n + 2) * (0.5f * inv_pi) * std::pow(c, ph->n);
+        return ph->kd * inv_pi + ph->ks * s;
+    }
+    return Col();
+}
+float bsdf_pdf(const Material* m, V3 wo, V3 wi, V3 nf)
+{
+    const float inv_pi = 0.318309886f;
+    float cs = dot(wi, nf);
+    if (cs <= 0) return 0;
+    if (m->kind() == RT_MAT_DIFFUSE) return cs * inv_pi;
+    if (m->kind() == RT_MAT_PHONG) {
+        auto* ph = static_cast<const Phong*>(m);
+        float rd = lum(ph->kd), rs = lum(ph->ks);
+        float pd = rd / (rd + rs);
+        V3 r = reflect(-wo, nf);
+        float c = std::max(0.f, dot(r, wi));
+        return pd * cs * inv_pi + (1 - pd) * (ph->n + 1) * (0.5f * inv_pi) * std::pow(c, ph->n);
+    }
+    return 0;
+}
+// Samples wi; returns false when the path must stop.
+bool bsdf_sample(const Material* m, V3 wo, V3 nf, float u0, float u1, float u2, V3& wi)
+{
+    if (m->kind() == RT_MAT_DIFFUSE) {
+        wi = lobe_dir(u0, u1, nf, 1.f);
+        return true;
+    }
+    if (m->kind() == RT_MAT_PHONG) {
+        auto* ph = static_cast<const Phong*>(m);
+        float rd = lum(ph->kd), rs = lum(ph->ks);
+        if (!(rd + rs > 0)) return false;
+        float pd = rd / (rd + rs);
+        if (u2 < pd) wi = lobe_dir(u0, u1, nf, 1.f);
+        else wi = lobe_dir(u0, u1, reflect(-wo, nf), ph->n);
+        return dot(wi, nf) > 0;
+    }
+    return false;
+}
+
+Col integrate_unbiased(const UnbiasedCtx& c, Ray ray, const SampleKey& k, int64_t& rays)
+{
+    const Scene& sc = *c.sc;
+    Col L, alpha(1, 1, 1);
+    bool spec = true;
+    const int nl = (int)sc.lights.size();
+    for (int depth = 0; depth < c.max_depth; ++depth) {
+        ++rays;
+        Hit hit = sc.intersect(ray);
+        if (!hit.hits()) break;
+        if (hit.mat->emits()) {
+            if (spec) {
+                if (hit.pw == 0) {
+                    L = L + mul(alpha, static_cast<const EnvMap*>(hit.mat)->get(ray.d));
+                } else {
+                    auto* al = static_cast<const AreaLight*>(hit.mat);
+                    if (dot(ray.d, al->nrm) < 0) L = L + mul(alpha, al->I);
+                }
+            }
+            break;
+        }
+        const Material* m = hit.mat;
+        V3 n = hit.n;
+        V3 d = normalized(ray.d);
+        V3 nf = dot(n, d) > 0 ? -n : n;
+        V3 x = hit.p;
+        if (m->kind() == RT_MAT_MIRROR) {
+            ray = Ray(x, reflect(d, n));
+            spec = true;
+            continue;
+        }
+        if (m->kind() == RT_MAT_DARK) break;
+        V3 wo = -d;
+        U4 gb = philox_uniforms(k.seed, k.view, k.pixel, k.sample, 1 + 2 * depth);
+        U4 gl = philox_uniforms(k.seed, k.view, k.pixel, k.sample, 2 + 2 * depth);
+        if (nl > 0) {
+            int j = std::min((int)(gl.u[2] * (float)nl), nl - 1);
+            const Light* lt = sc.lights[j];
+            V3 wi;
+            float dist = kInf;
+            Col le;
+            bool ok = false;
+            if (auto* al = dynamic_cast<const AreaLight*>(lt)) {
+                V3 y = al->o + gl.u[0] * al->u + gl.u[1] * al->v;
+                V3 dv = y - x;
+                float d2 = dot(dv, dv);
+                dist = std::sqrt(d2);
+                wi = dv * (1 / dist);
+                float an = std::sqrt(dot(al->nrm, al->nrm));
+                float cl = -dot(wi, al->nrm) / an;
+                if (cl > 0) { le = al->I * (cl * an / d2); ok = true; }
+            } else if (auto* pl = dynamic_cast<const PointLight*>(lt)) {
+                V3 dv = pl->p - x;
+                float d2 = dot(dv, dv);
+                dist = std::sqrt(d2);
+                wi = dv * (1 / dist);
+                le = pl->I * (1 / d2);
+                ok = true;
+            } else if (auto* em = dynamic_cast<const EnvMap*>(lt)) {
+                float pdf;
+                em->sample(gl.u[0], gl.u[1], wi, pdf);
+                if (pdf > 0) { le = em->get(wi) * (1 / pdf); ok = true; }
+            }
+            float cx = dot(wi, nf);
+            if (ok && cx > 0) {
+                Col f = bsdf_eval(m, wo, wi, nf);
+                Col contrib = mul(mul(alpha, f), le) * (cx * (float)nl);
+                if (!contrib.zero()) {
+                    float tmax = std::isinf(dist) ? kInf : dist - kEps;
+                    ++rays;
+                    Hit sh = root_any(sc, Ray(x, wi, kEps, tmax));
+                    if (!sh.hits()) L = L + contrib;
+                }
+            }
+        }
+        V3 wi;
+        if (!bsdf_sample(m, wo, nf, gb.u[0], gb.u[1], gb.u[2], wi)) break;
+        float pdf = bsdf_pdf(m, wo, wi, nf);
+        if (!(pdf > 0)) break;
+        Col f = bsdf_eval(m, wo, wi, nf);
+        alpha = mul(alpha, f) * (dot(wi, nf) / pdf);
+        spec = false;
+        if (depth + 1 >= c.rr_start) {
+            if (gb.u[3] < c.rr_prob) break;
+            alpha = alpha * (1 / (1 - c.rr_prob));
+        }
+        ray = Ray(x, wi);
+    }
+    return L;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C API
+// ---------------------------------------------------------------------------
+extern "C" {
+
+void oc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4])
+{
+    philox(ctr, key, out);
+}
+
+void* oc_scene_create(const rt_scene_desc* d, int accel)
+{
+    auto* sc = new Scene();
+    std::vector<Material*> mats;
+    for (int i = 0; i < d->n_materials; ++i) {
+        const rt_material& m = d->materials[i];
+        Material* mm;
+        switch (m.type) {
+        case RT_MAT_DIFFUSE: mm = new Diffuse(Col(m.kd[0], m.kd[1], m.kd[2])); break;
+        case RT_MAT_MIRROR: mm = new Mirror(); break;
+        case RT_MAT_PHONG:
+            mm = new Phong(Col(m.kd[0], m.kd[1], m.kd[2]), Col(m.ks[0], m.ks[1], m.ks[2]), m.exponent);
+            break;
+        default: mm = new DarkMatter(); break;
+        }
+        sc->mats.emplace_back(mm);
+        mats.push_back(mm);
+    }
+    int max_prim = d->n_lights - 1;
+    for (int i = 0; i < d->n_triangles; ++i) max_prim = std::max(max_prim, d->tri_prim_id[i]);
+    for (int i = 0; i < d->n_spheres; ++i) max_prim = std::max(max_prim, d->sphere_prim_id[i]);
+    sc->prims.assign(max_prim + 1, nullptr);
+
+    std::vector<Intersectable*> root_items;
+    for (int i = 0; i < d->n_lights; ++i) {
+        const rt_light& l = d->lights[i];
+        V3 p(l.p[0], l.p[1], l.p[2]), u(l.u[0], l.u[1], l.u[2]), v(l.v[0], l.v[1], l.v[2]);
+        Col I(l.intensity[0], l.intensity[1], l.intensity[2]);
+        if (l.type == RT_LIGHT_AREA) {
+            auto* al = new AreaLight(p, u, v, I, i);
+            sc->owned.emplace_back(al);
+            sc->lights.push_back(al);
+            sc->prims[i] = al;
+            root_items.push_back(al);
+        } else if (l.type == RT_LIGHT_POINT) {
+            auto* pl = new PointLight(p, I);
+            sc->owned.emplace_back(pl);
+            sc->lights.push_back(pl);
+            root_items.push_back(pl);
+        } else {
+            auto* em = new EnvMap();
+            em->W = d->env_width;
+            em->H = d->env_height;
+            em->rgb.assign(d->env_rgb, d->env_rgb + (size_t)3 * em->W * em->H);
+            em->coef = I;
+            em->prim = i;
+            em->build_sampling();
+            sc->mats.emplace_back(em);
+            sc->lights.push_back(em);
+            sc->env = em;
+        }
+    }
+    auto vert = [d](int idx) {
+        return V3(d->positions[3 * idx], d->positions[3 * idx + 1], d->positions[3 * idx + 2]);
+    };
+    auto nrm = [d](int idx) {
+        return V3(d->normals[3 * idx], d->normals[3 * idx + 1], d->normals[3 * idx + 2]);
+    };
+    auto make_tri = [&](int i) {
+        const int32_t* tv = d->tri_vertices + 3 * i;
+        Triangle t(vert(tv[0]), vert(tv[1]), vert(tv[2]), mats[d->tri_material[i]], d->tri_prim_id[i]);
+        if (d->normals) t.set_normals(nrm(tv[0]), nrm(tv[1]), nrm(tv[2]));
+        return t;
+    };
+    // Components in primitive-ID order; two consecutive triangles with one ID form a quad.
+    struct Item { int id; Intersectable* obj; };
+    std::vector<Item> items;
+    for (int i = 0; i < d->n_triangles;) {
+        int id = d->tri_prim_id[i];
+        if (i + 1 < d->n_triangles && d->tri_prim_id[i + 1] == id) {
+            auto* q = new Quad(make_tri(i), make_tri(i + 1));
+            sc->owned.emplace_back(q);
+            items.push_back({id, q});
+            i += 2;
+        } else {
+            auto* t = new Triangle(make_tri(i));
+            sc->owned.emplace_back(t);
+            items.push_back({id, t});
+            i += 1;
+        }
+    }
+    for (int i = 0; i < d->n_spheres; ++i) {
+        const float* s = d->spheres + 4 * i;
+        auto* sp = new Sphere(V3(s[0], s[1], s[2]), s[3], mats[d->sphere_material[i]], d->sphere_prim_id[i]);
+        sc->owned.emplace_back(sp);
+        items.push_back({d->sphere_prim_id[i], sp});
+    }
+    std::stable_sort(items.begin(), items.end(), [](const Item& a, const Item& b) { return a.id < b.id; });
+    for (auto& it : items) {
+        sc->prims[it.id] = it.obj;
+        root_items.push_back(it.obj);
+    }
+    if (root_items.empty()) {
+        sc->root = nullptr;
+    } else if (accel == 0 || root_items.size() == 1) {
+        auto* l = new List();
+        l->items = root_items;
+        sc->owned.emplace_back(l);
+        sc->root = l;
+    } else {
+        sc->root = build_bvh(root_items, sc->owned);
+    }
+    return sc;
+}
+
+void oc_scene_destroy(void* s) { delete static_cast<Scene*>(s); }
+
+static inline Ray ray_from(const float* o, const float* d, int64_t i)
+{
+    return Ray(V3(o[4 * i], o[4 * i + 1], o[4 * i + 2]), V3(d[4 * i], d[4 * i + 1], d[4 * i + 2]), o[4 * i + 3],
+               d[4 * i + 3]);
+}
+
+void oc_trace_closest(void* s, int64_t n, const float* o, const float* d, int32_t* prim, float* t, int threads)
+{
+    const Scene& sc = *static_cast<Scene*>(s);
+#pragma omp parallel for schedule(dynamic, 64) num_threads(threads > 0 ? threads : 1)
+    for (int64_t i = 0; i < n; ++i) {
+        Hit h = sc.root ? sc.root->intersect(ray_from(o, d, i)) : Hit();
+        prim[i] = h.hits() ? h.prim : -1;
+        t[i] = h.t;
+    }
+}
+
+void oc_trace_any(void* s, int64_t n, const float* o, const float* d, uint8_t* occ, int threads)
+{
+    const Scene& sc = *static_cast<Scene*>(s);
+#pragma omp parallel for schedule(dynamic, 64) num_threads(threads > 0 ? threads : 1)
+    for (int64_t i = 0; i < n; ++i) {
+        Hit h = sc.root ? sc.root->intersect(ray_from(o, d, i)) : Hit();
+        occ[i] = h.hits() ? 1 : 0;
+    }
+}
+
+float oc_intersect_prim(void* s, int32_t prim, const float* o, const float* d)
+{
+    const Scene& sc = *static_cast<Scene*>(s);
+    if (prim < 0 || prim >= (int)sc.prims.size() || !sc.prims[prim]) return kNaN;
+    return sc.prims[prim]->intersect(ray_from(o, d, 0)).t;
+}
+
+static Col render_one(const Scene& sc, const Camera& cam, const rt_view* view, const rt_params* p,
+                      uint32_t view_id, int px, int py, int s, int64_t& rays)
+{
+    const int ny = view->spp_y;
+    SampleKey k{p->seed, view_id, (uint32_t)(py * cam.W + px), (uint32_t)s};
+    U4 g0 = philox_uniforms(k.seed, k.view, k.pixel, k.sample, 0);
+    // src/jitteredsampler.cpp:23-35: stratum (a, b) with a = s / ny, b = s % ny
+    int a = s / ny, b = s % ny;
+    double sx = ((double)g0.u[0] + a) / view->spp_x, sy = ((double)g0.u[1] + b) / ny;
+    // src/perpixelrenderer.cpp:79-84: point = pixel + jitter, camera adds 0.5
+    Ray ray = cam.ray(px + sx, py + sy);
+    if (p->mode == RT_MODE_COMPAT)
+        return integrate_compat(sc, ray, k, p->max_depth, p->rr_start, p->rr_prob, rays);
+    ray.d = normalized(ray.d);
+    UnbiasedCtx c{&sc, p->max_depth, p->rr_start, p->rr_prob};
+    return integrate_unbiased(c, ray, k, rays);
+}
+
+int64_t oc_render(void* s, const rt_view* view, const rt_params* p, uint32_t view_id, float* film, int row0,
+                  int row1, int threads)
+{
+    const Scene& sc = *static_cast<Scene*>(s);
+    Camera cam(view->camera);
+    const int spp = view->spp_x * view->spp_y;
+    int64_t total_rays = 0;
+    Col sum;
+    // src/perpixelrenderer.cpp:62-98: rows in parallel, film[i][j] += s / spp,
+    // a critical section per sample for the luminance sum.
+#pragma omp parallel for schedule(dynamic) num_threads(threads > 0 ? threads : 1) reduction(+ : total_rays)
+    for (int i = row0; i < row1; ++i) {
+        for (int j = 0; j < cam.W; ++j) {
+            float* px = film + 3 * ((size_t)i * cam.W + j);
+            for (int si = 0; si < spp; ++si) {
+                int64_t rays = 0;
+                Col c = render_one(sc, cam, view, p, view_id, j, i, si, rays);
+                total_rays += rays;
+                Col w = c * (1.f / spp);
+                px[0] += w.r;
+                px[1] += w.g;
+                px[2] += w.b;
+#pragma omp critical
+                sum = sum + w;
+            }
+        }
+    }
+    return total_rays;
+}
+
+void oc_render_sample(void* s, const rt_view* view, const rt_params* p, uint32_t view_id, int px, int py, int si,
+                      float out[3])
+{
+    const Scene& sc = *static_cast<Scene*>(s);
+    Camera cam(view->camera);
+    int64_t rays = 0;
+    Col c = render_one(sc, cam, view, p, view_id, px, py, si, rays);
+    out[0] = c.r;
+    out[1] = c.g;
+    out[2] = c.b;
+}
+
+void oc_env_sample(void* s, float u0, float u1, float dir[3], float* pdf)
+{
+    const Scene& sc = *static_cast<Scene*>(s);
+    V3 d;
+    *pdf = 0;
+    dir[0] = dir[1] = dir[2] = 0;
+    if (!sc.env) return;
+    sc.env->sample(u0, u1, d, *pdf);
+    dir[0] = d.x;
+    dir[1] = d.y;
+    dir[2] = d.z;
+}
+
+float oc_env_pdf(void* s, const float dir[3])
+{
+    const Scene& sc = *static_cast<Scene*>(s);
+    if (!sc.env) return 0;
+    return sc.env->pdf(V3(dir[0], dir[1], dir[2]));
+}
+
+}  // extern "C"

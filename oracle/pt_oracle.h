@@ -1,0 +1,81 @@
+/*
+ * pt_oracle.h — CPU oracle for the path-tracing hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: tests, the smoke
+ * check and the CPU-baseline leg of the benchmark call it; the product
+ * (librt_hip.so and renderer_amd/) never links, loads or calls it.
+ *
+ * It restates the reference renderer's algorithms on the CPU, following
+ * (all paths relative to the reference tree):
+ *   src/ray.h:34-111, src/hitrecord.h:62-84         rays and hit records
+ *   src/triangle.cpp:30-56, src/quad.cpp:23-34      triangle / quad tests
+ *   src/sphere.cpp:29-56, src/csgobject.cpp:25-44   sphere test
+ *   src/axisalignedbox.cpp:30-104, src/bvh.cpp:34-149, src/intersectablelist.cpp:30-78
+ *   src/abstractdiffusematerial.cpp:8-15, src/diffusematerial.cpp:25-28,
+ *   src/mirrormaterial.cpp:31-41, src/phongmaterial.cpp:31-81
+ *   src/arealight.cpp:31-119, src/pointlight.cpp:29-60, src/environmentmap.cpp:35-108
+ *   src/sampler.cpp:33-91, src/jitteredsampler.cpp:23-35, src/camera.cpp:34-58
+ *   src/renderer.cpp:47-106, src/unidipathtracingintegrator.cpp:34-107,
+ *   src/perpixelrenderer.cpp:43-101
+ * with two deliberate changes: the GSL taus2 stream is replaced by the
+ * Philox-4x32-10 stream keyed by (seed, view, pixel, sample, dimension) that
+ * the GPU path uses, and MAX_DEPTH / RR start are runtime parameters.
+ *
+ * The environment light is a latitude-longitude map (the reference reads an
+ * angular light probe through OpenCV); its lookup and importance sampling
+ * are restated from the textbook method and are parity-unpinned against the
+ * reference.
+ */
+#ifndef PT_ORACLE_H
+#define PT_ORACLE_H
+
+#include <stdint.h>
+#include "../include/rt_scene.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Philox-4x32-10 block function (Salmon, Moraes, Dror, Shaw, SC'11). */
+void oc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
+/* accel: 0 = linear IntersectableList (reference --no-bvh),
+ *        1 = reference BVH (leaf = 1 primitive, both children visited). */
+void* oc_scene_create(const rt_scene_desc* desc, int accel);
+void oc_scene_destroy(void* scene);
+
+/* Closest hit for n rays; ray i = (o[4i..4i+2], d[4i..4i+2]) over
+ * [o[4i+3], d[4i+3]].  Miss: prim = -1, t = NaN.  Area lights are hit like
+ * any primitive (their ID is their light index). */
+void oc_trace_closest(void* scene, int64_t n, const float* o_tmin, const float* d_tmax,
+                      int32_t* prim, float* t, int threads);
+
+/* Any hit in range: occluded[i] = 1 if some primitive is hit. */
+void oc_trace_any(void* scene, int64_t n, const float* o_tmin, const float* d_tmax,
+                  uint8_t* occluded, int threads);
+
+/* Single-primitive test with the reference arithmetic, used to judge hit
+ * mismatches at shared edges: returns t or NaN. */
+float oc_intersect_prim(void* scene, int32_t prim, const float* o_tmin, const float* d_tmax);
+
+/* Render rows [row0, row1) of one view into film (width*height*3 RGB, the
+ * caller zeroes it).  Returns the number of rays traced (camera, extension
+ * and shadow rays).  view_id keys the random stream like the GPU path. */
+int64_t oc_render(void* scene, const rt_view* view, const rt_params* params, uint32_t view_id,
+                  float* film, int row0, int row1, int threads);
+
+/* Radiance of one camera sample (pixel (px,py), sample index s) — the
+ * per-sample Integrator::integrate entry of the reference. */
+void oc_render_sample(void* scene, const rt_view* view, const rt_params* params, uint32_t view_id,
+                      int px, int py, int s, float out_rgb[3]);
+
+/* Environment importance sampling: direction and solid-angle pdf for
+ * uniforms (u0, u1); and pdf of a direction. */
+void oc_env_sample(void* scene, float u0, float u1, float dir[3], float* pdf);
+float oc_env_pdf(void* scene, const float dir[3]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,16 @@
+"""MI355X path-tracing hot path: HIP kernels for gfx950 behind a C ABI
+(librt_hip.so, include/rt_abi.h), with a host-side scene model and
+integrator interface named after the reference renderer's classes."""
+from . import _abi
+from ._abi import (LIGHT_AREA, LIGHT_ENV, LIGHT_POINT, MAT_DARK, MAT_DIFFUSE, MAT_MIRROR, MAT_PHONG,
+                   MODE_COMPAT, MODE_GUIDED, MODE_UNBIASED, RtError)
+from .integrator import GpuContext, GpuUniDiPathTracingIntegrator, PerPixelRenderer
+from .scene import (AreaLight, Camera, DarkMatter, DiffuseMaterial, EnvironmentMap, IntersectableList, Mesh,
+                    MirrorMaterial, PhongMaterial, PointLight, Quad, Scene, Sphere, Triangle)
+
+__all__ = [
+    "GpuContext", "GpuUniDiPathTracingIntegrator", "PerPixelRenderer", "RtError",
+    "AreaLight", "Camera", "DarkMatter", "DiffuseMaterial", "EnvironmentMap", "IntersectableList", "Mesh",
+    "MirrorMaterial", "PhongMaterial", "PointLight", "Quad", "Scene", "Sphere", "Triangle",
+    "MODE_COMPAT", "MODE_UNBIASED", "MODE_GUIDED",
+]

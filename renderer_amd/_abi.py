@@ -1,0 +1,140 @@
+"""ctypes mirror of include/rt_scene.h and include/rt_abi.h, and the loader of
+librt_hip.so.  Loading fails loudly when the library has not been built."""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "librt_hip.so")
+
+c_float3 = ctypes.c_float * 3
+
+
+class RtMaterial(ctypes.Structure):
+    _fields_ = [("type", ctypes.c_int32), ("kd", c_float3), ("ks", c_float3), ("exponent", ctypes.c_float)]
+
+
+class RtLight(ctypes.Structure):
+    _fields_ = [("type", ctypes.c_int32), ("p", c_float3), ("u", c_float3), ("v", c_float3),
+                ("intensity", c_float3)]
+
+
+class RtSceneDesc(ctypes.Structure):
+    _fields_ = [
+        ("n_vertices", ctypes.c_int32),
+        ("positions", ctypes.c_void_p),
+        ("normals", ctypes.c_void_p),
+        ("n_triangles", ctypes.c_int32),
+        ("tri_vertices", ctypes.c_void_p),
+        ("tri_material", ctypes.c_void_p),
+        ("tri_prim_id", ctypes.c_void_p),
+        ("n_spheres", ctypes.c_int32),
+        ("spheres", ctypes.c_void_p),
+        ("sphere_material", ctypes.c_void_p),
+        ("sphere_prim_id", ctypes.c_void_p),
+        ("n_materials", ctypes.c_int32),
+        ("materials", ctypes.c_void_p),
+        ("n_lights", ctypes.c_int32),
+        ("lights", ctypes.c_void_p),
+        ("env_width", ctypes.c_int32),
+        ("env_height", ctypes.c_int32),
+        ("env_rgb", ctypes.c_void_p),
+    ]
+
+
+class RtCamera(ctypes.Structure):
+    _fields_ = [("eye", c_float3), ("look_at", c_float3), ("up", c_float3), ("fov", ctypes.c_float),
+                ("width", ctypes.c_int32), ("height", ctypes.c_int32)]
+
+
+class RtView(ctypes.Structure):
+    _fields_ = [("camera", RtCamera), ("spp_x", ctypes.c_int32), ("spp_y", ctypes.c_int32)]
+
+
+class RtParams(ctypes.Structure):
+    _fields_ = [("mode", ctypes.c_int32), ("max_depth", ctypes.c_int32), ("rr_start", ctypes.c_int32),
+                ("rr_prob", ctypes.c_float), ("seed", ctypes.c_uint32), ("reserved", ctypes.c_int32 * 3)]
+
+
+class RtStats(ctypes.Structure):
+    _fields_ = [
+        ("camera_rays", ctypes.c_int64),
+        ("extension_rays", ctypes.c_int64),
+        ("shadow_rays", ctypes.c_int64),
+        ("render_ms", ctypes.c_double),
+        ("extend_ms", ctypes.c_double),
+        ("shadow_ms", ctypes.c_double),
+        ("extend_launches", ctypes.c_int64),
+        ("bvh_nodes", ctypes.c_int64),
+        ("bvh_prims", ctypes.c_int64),
+        ("bvh_depth", ctypes.c_int32),
+        ("paths_per_pass", ctypes.c_int32),
+        ("passes", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("node_visits", ctypes.c_int64),
+        ("prim_tests", ctypes.c_int64),
+    ]
+
+
+class RtLaunchConfig(ctypes.Structure):
+    _fields_ = [("path_slots", ctypes.c_int32), ("block", ctypes.c_int32), ("blocks_per_cu", ctypes.c_int32),
+                ("use_graph", ctypes.c_int32),
+                ("count_traversal", ctypes.c_int32)]
+
+
+MAT_DIFFUSE, MAT_MIRROR, MAT_PHONG, MAT_DARK = 0, 1, 2, 3
+LIGHT_AREA, LIGHT_POINT, LIGHT_ENV = 0, 1, 2
+MODE_COMPAT, MODE_UNBIASED, MODE_GUIDED = 0, 1, 2
+
+STATUS = {0: "ok", -1: "invalid argument", -2: "no scene uploaded", -3: "HIP runtime error",
+          -4: "BVH build failed", -5: "cancelled", -6: "injected fault", -7: "no guiding network"}
+
+_P = ctypes.c_void_p
+_I64 = ctypes.c_int64
+_I32 = ctypes.c_int32
+
+# name -> (restype, argtypes); every symbol declared in include/rt_abi.h
+PROTOTYPES = {
+    "rt_error_string": (ctypes.c_char_p, [ctypes.c_int]),
+    "rt_version": (ctypes.c_int, []),
+    "rt_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
+    "rt_destroy": (ctypes.c_int, [_P]),
+    "rt_last_error": (ctypes.c_char_p, [_P]),
+    "rt_scene_upload": (ctypes.c_int, [_P, ctypes.POINTER(RtSceneDesc)]),
+    "rt_trace_closest": (ctypes.c_int, [_P, _P, _P, _I64, _P, _P]),
+    "rt_trace_closest_device": (ctypes.c_int, [_P, _P, _P, _I64, _P, _P]),
+    "rt_trace_any": (ctypes.c_int, [_P, _P, _P, _I64, _P]),
+    "rt_trace_any_device": (ctypes.c_int, [_P, _P, _P, _I64, _P]),
+    "rt_trace_count_device": (ctypes.c_int, [_P, _P, _P, _I64, _P, _P]),
+    "rt_render": (ctypes.c_int, [_P, ctypes.POINTER(RtView), _I32, ctypes.POINTER(RtParams), _P]),
+    "rt_render_device": (ctypes.c_int, [_P, ctypes.POINTER(RtView), _I32, ctypes.POINTER(RtParams), _P]),
+    "rt_set_launch_config": (ctypes.c_int, [_P, ctypes.POINTER(RtLaunchConfig)]),
+    "rt_get_launch_config": (ctypes.c_int, [_P, ctypes.POINTER(RtLaunchConfig)]),
+    "rt_get_stats": (ctypes.c_int, [_P, ctypes.POINTER(RtStats)]),
+    "rt_request_stop": (ctypes.c_int, [_P, _I32]),
+    "rt_set_fault": (ctypes.c_int, [_P, _I32]),
+}
+
+_lib = None
+
+
+class RtError(RuntimeError):
+    def __init__(self, status, msg=""):
+        super().__init__(f"librt_hip: {STATUS.get(status, status)} ({status}){': ' + msg if msg else ''}")
+        self.status = status
+
+
+def load():
+    """Load librt_hip.so (built by renderer_amd.build); raise if it is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is not built; run `python -m renderer_amd.build` "
+                           "(the GPU path has no fallback)")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in PROTOTYPES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib

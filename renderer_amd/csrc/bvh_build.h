@@ -1,0 +1,35 @@
+/*
+ * bvh_build.h — host BVH builder interface (see bvh_build.cpp).
+ */
+#ifndef RT_BVH_BUILD_H
+#define RT_BVH_BUILD_H
+
+#include <cstdint>
+#include <vector>
+
+#include "rt_layout.h"
+
+namespace rtb {
+
+// One primitive to place: its box, kind (RT_KIND_*) and shading index g.
+// build() reorders the array into leaf order; the primitive records are
+// then written in that order by the caller.
+struct PrimRef {
+    float lo[3], hi[3];
+    float c[3];
+    int32_t kind;
+    int32_t g;
+};
+
+struct BuildResult {
+    std::vector<float> nodes;  // 16 floats (64 B) per interior node
+    int depth = 0;             // deepest interior level
+};
+
+// Returns 0 on success, -1 if there are too many primitives for the leaf
+// encoding, -2 if the tree is deeper than the traversal stack.
+int build(std::vector<PrimRef>& refs, BuildResult& out);
+
+}  // namespace rtb
+
+#endif

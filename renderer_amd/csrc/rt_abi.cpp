@@ -1,0 +1,804 @@
+/*
+ * rt_abi.cpp — the C ABI of librt_hip.so (declared in include/rt_abi.h):
+ * context and device memory, scene upload (host BVH build + flattening),
+ * ray-batch entry points, and the wavefront render driver.
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/rt_abi.h"
+#include "bvh_build.h"
+#include "rt_kernels.h"
+
+namespace {
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+}  // namespace
+
+struct rt_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    int fault = 0;
+    volatile int stop = 0;
+    int n_cu = 256;
+
+    // scene
+    bool has_scene = false;
+    HostDevScene scene{};
+    DevBuf nodes, prims, prim_id, mat, vnormals, geo, mats, lights, env_rgb, env_row, env_col, env_p;
+    int bvh_depth = 0;
+
+    // ray batches (host entry points)
+    DevBuf rb_o, rb_d, rb_out0, rb_out1;
+
+    // render
+    PathState ps{};
+    size_t ps_cap = 0;
+    std::vector<DevBuf> ps_bufs;
+    DevBuf counters, views, tiles, passes, films;
+    rt_launch_config cfg{0, 256, 4, 0, 0};
+    int timing = 1;
+    hipEvent_t ev_a = nullptr, ev_b = nullptr;
+    std::vector<hipEvent_t> ev_pool;
+    rt_stats stats{};
+};
+
+namespace {
+
+int set_err(rt_ctx* c, int code, const std::string& msg)
+{
+    if (c) c->err = msg;
+    return code;
+}
+
+#define HIP_TRY(ctx, call)                                                                         \
+    do {                                                                                           \
+        hipError_t e_ = (call);                                                                    \
+        if (e_ != hipSuccess)                                                                      \
+            return set_err(ctx, RT_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_));    \
+    } while (0)
+
+int ensure(rt_ctx* c, DevBuf& b, size_t bytes)
+{
+    if (b.bytes >= bytes && b.p) return 0;
+    if (b.p) {
+        hipFree(b.p);
+        b.p = nullptr;
+        b.bytes = 0;
+    }
+    if (bytes == 0) return 0;
+    hipError_t e = hipMalloc(&b.p, bytes);
+    if (e != hipSuccess) return set_err(c, RT_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+    b.bytes = bytes;
+    return 0;
+}
+
+void release(DevBuf& b)
+{
+    if (b.p) hipFree(b.p);
+    b.p = nullptr;
+    b.bytes = 0;
+}
+
+int upload(rt_ctx* c, DevBuf& b, const void* src, size_t bytes)
+{
+    if (bytes == 0) {
+        release(b);
+        return 0;
+    }
+    int r = ensure(c, b, bytes);
+    if (r) return r;
+    HIP_TRY(c, hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice));
+    return 0;
+}
+
+int check_fault(rt_ctx* c)
+{
+    if (c->fault) {
+        c->fault = 0;
+        return set_err(c, RT_ERR_FAULT, "injected fault");
+    }
+    return 0;
+}
+
+// Qt 5 QVector3D::normalized(): length in double, components rounded to float.
+void qt_normalize(const float in[3], float out[3])
+{
+    double l = (double)in[0] * in[0] + (double)in[1] * in[1] + (double)in[2] * in[2];
+    if (l == 0.0) {
+        out[0] = out[1] = out[2] = 0.f;
+        return;
+    }
+    double s = std::sqrt(l);
+    for (int k = 0; k < 3; ++k) out[k] = (float)(in[k] / s);
+}
+void cross3(const float a[3], const float b[3], float o[3])
+{
+    o[0] = a[1] * b[2] - a[2] * b[1];
+    o[1] = a[2] * b[0] - a[0] * b[2];
+    o[2] = a[0] * b[1] - a[1] * b[0];
+}
+float lum_rgb(const float c[3]) { return (float)(.299 * c[0] + .587 * c[1] + .114 * c[2]); }
+
+int grid_for(rt_ctx* c) { return c->n_cu * std::max(1, c->cfg.blocks_per_cu); }
+
+}  // namespace
+
+extern "C" {
+
+const char* rt_error_string(int s)
+{
+    switch (s) {
+    case RT_OK: return "ok";
+    case RT_ERR_INVALID: return "invalid argument";
+    case RT_ERR_NO_SCENE: return "no scene uploaded";
+    case RT_ERR_HIP: return "HIP runtime error";
+    case RT_ERR_BUILD: return "BVH build failed";
+    case RT_ERR_CANCELLED: return "cancelled";
+    case RT_ERR_FAULT: return "injected fault";
+    case RT_ERR_NO_GUIDE: return "no guiding network";
+    default: return "unknown error";
+    }
+}
+
+int rt_version(void) { return 1; }
+
+int rt_create(int device, rt_ctx** out)
+{
+    if (!out) return RT_ERR_INVALID;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return RT_ERR_HIP;
+    rt_ctx* c = new (std::nothrow) rt_ctx();
+    if (!c) return RT_ERR_INVALID;
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return RT_ERR_HIP;
+    }
+    int cu = 0;
+    if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cu > 0) c->n_cu = cu;
+    hipEventCreate(&c->ev_a);
+    hipEventCreate(&c->ev_b);
+    *out = c;
+    return RT_OK;
+}
+
+int rt_destroy(rt_ctx* c)
+{
+    if (!c) return RT_ERR_INVALID;
+    hipSetDevice(c->device);
+    hipStreamSynchronize(c->stream);
+    DevBuf* all[] = {&c->nodes, &c->prims, &c->prim_id, &c->mat, &c->vnormals, &c->geo, &c->mats, &c->lights,
+                     &c->env_rgb, &c->env_row, &c->env_col, &c->env_p, &c->rb_o, &c->rb_d, &c->rb_out0,
+                     &c->rb_out1, &c->counters, &c->views, &c->tiles, &c->passes, &c->films};
+    for (DevBuf* b : all) release(*b);
+    for (auto& b : c->ps_bufs) release(b);
+    for (auto e : c->ev_pool) hipEventDestroy(e);
+    if (c->ev_a) hipEventDestroy(c->ev_a);
+    if (c->ev_b) hipEventDestroy(c->ev_b);
+    hipStreamDestroy(c->stream);
+    delete c;
+    return RT_OK;
+}
+
+const char* rt_last_error(rt_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int rt_set_fault(rt_ctx* c, int32_t f)
+{
+    if (!c) return RT_ERR_INVALID;
+    c->fault = f;
+    return RT_OK;
+}
+
+int rt_request_stop(rt_ctx* c, int32_t s)
+{
+    if (!c) return RT_ERR_INVALID;
+    c->stop = s;
+    return RT_OK;
+}
+
+int rt_set_launch_config(rt_ctx* c, const rt_launch_config* cfg)
+{
+    if (!c || !cfg) return RT_ERR_INVALID;
+    if (cfg->path_slots < 0 || cfg->block < 0 || cfg->blocks_per_cu < 0) return set_err(c, RT_ERR_INVALID, "negative launch config");
+    if (cfg->path_slots) c->cfg.path_slots = cfg->path_slots;
+    if (cfg->block) c->cfg.block = cfg->block;
+    if (cfg->blocks_per_cu) c->cfg.blocks_per_cu = std::min(cfg->blocks_per_cu, 64);
+    c->cfg.use_graph = cfg->use_graph;
+    c->cfg.count_traversal = cfg->count_traversal;
+    return RT_OK;
+}
+
+int rt_get_launch_config(rt_ctx* c, rt_launch_config* cfg)
+{
+    if (!c || !cfg) return RT_ERR_INVALID;
+    *cfg = c->cfg;
+    return RT_OK;
+}
+
+int rt_get_stats(rt_ctx* c, rt_stats* out)
+{
+    if (!c || !out) return RT_ERR_INVALID;
+    *out = c->stats;
+    return RT_OK;
+}
+
+int rt_scene_upload(rt_ctx* c, const rt_scene_desc* d)
+{
+    if (!c || !d) return RT_ERR_INVALID;
+    if (int r = check_fault(c)) return r;
+    if (d->n_vertices < 0 || d->n_triangles < 0 || d->n_spheres < 0 || d->n_materials < 0 || d->n_lights < 0)
+        return set_err(c, RT_ERR_INVALID, "negative count");
+    if ((d->n_triangles && (!d->tri_vertices || !d->tri_material || !d->tri_prim_id || !d->positions)) ||
+        (d->n_spheres && (!d->spheres || !d->sphere_material || !d->sphere_prim_id)) ||
+        (d->n_materials && !d->materials) || (d->n_lights && !d->lights))
+        return set_err(c, RT_ERR_INVALID, "missing array");
+    hipSetDevice(c->device);
+    const int nT = d->n_triangles, nS = d->n_spheres;
+    int nP = 0, env = -1;
+    for (int i = 0; i < d->n_lights; ++i) {
+        if (d->lights[i].type == RT_LIGHT_AREA) ++nP;
+        else if (d->lights[i].type == RT_LIGHT_ENV) {
+            if (env >= 0 || !d->env_rgb || d->env_width <= 0 || d->env_height <= 0)
+                return set_err(c, RT_ERR_INVALID, "environment light needs exactly one map");
+            env = i;
+        } else if (d->lights[i].type != RT_LIGHT_POINT)
+            return set_err(c, RT_ERR_INVALID, "unknown light type");
+    }
+    for (int i = 0; i < nT; ++i) {
+        for (int k = 0; k < 3; ++k)
+            if (d->tri_vertices[3 * i + k] < 0 || d->tri_vertices[3 * i + k] >= d->n_vertices)
+                return set_err(c, RT_ERR_INVALID, "triangle vertex index out of range");
+        if (d->tri_material[i] < 0 || d->tri_material[i] >= d->n_materials)
+            return set_err(c, RT_ERR_INVALID, "triangle material out of range");
+    }
+    for (int i = 0; i < nS; ++i)
+        if (d->sphere_material[i] < 0 || d->sphere_material[i] >= d->n_materials)
+            return set_err(c, RT_ERR_INVALID, "sphere material out of range");
+    const int G = nT + nS + nP;
+
+    std::vector<rtb::PrimRef> refs;
+    refs.reserve(G);
+    std::vector<float> geo((size_t)4 * G, 0.f);
+    std::vector<int32_t> pid(G), mat(G);
+    auto P = [d](int v, int k) { return d->positions[3 * v + k]; };
+    for (int i = 0; i < nT; ++i) {
+        const int32_t* tv = d->tri_vertices + 3 * i;
+        rtb::PrimRef r;
+        for (int k = 0; k < 3; ++k) {
+            r.lo[k] = std::min(P(tv[0], k), std::min(P(tv[1], k), P(tv[2], k)));
+            r.hi[k] = std::max(P(tv[0], k), std::max(P(tv[1], k), P(tv[2], k)));
+        }
+        r.kind = RT_KIND_TRI;
+        r.g = i;
+        refs.push_back(r);
+        float a[3], b[3], n[3], nn[3];
+        for (int k = 0; k < 3; ++k) {
+            a[k] = P(tv[0], k) - P(tv[1], k);
+            b[k] = P(tv[0], k) - P(tv[2], k);
+        }
+        cross3(a, b, n);
+        qt_normalize(n, nn);
+        geo[4 * i] = nn[0];
+        geo[4 * i + 1] = nn[1];
+        geo[4 * i + 2] = nn[2];
+        pid[i] = d->tri_prim_id[i];
+        mat[i] = d->tri_material[i];
+    }
+    for (int i = 0; i < nS; ++i) {
+        const float* s = d->spheres + 4 * i;
+        rtb::PrimRef r;
+        for (int k = 0; k < 3; ++k) {
+            r.lo[k] = s[k] - s[3];
+            r.hi[k] = s[k] + s[3];
+        }
+        r.kind = RT_KIND_SPHERE;
+        r.g = nT + i;
+        refs.push_back(r);
+        for (int k = 0; k < 4; ++k) geo[4 * (nT + i) + k] = s[k];
+        pid[nT + i] = d->sphere_prim_id[i];
+        mat[nT + i] = d->sphere_material[i];
+    }
+    std::vector<RtLight> lights(d->n_lights);
+    {
+        int j = 0;
+        for (int i = 0; i < d->n_lights; ++i) {
+            const rt_light& l = d->lights[i];
+            RtLight& L = lights[i];
+            std::memset(&L, 0, sizeof(L));
+            L.type = l.type;
+            for (int k = 0; k < 3; ++k) {
+                L.p[k] = l.p[k];
+                L.u[k] = l.u[k];
+                L.v[k] = l.v[k];
+                L.I[k] = l.intensity[k];
+            }
+            cross3(L.u, L.v, L.n);
+            L.area = std::sqrt(L.n[0] * L.n[0] + L.n[1] * L.n[1] + L.n[2] * L.n[2]);
+            L.g = -1;
+            if (l.type != RT_LIGHT_AREA) continue;
+            int g = nT + nS + j++;
+            L.g = g;
+            rtb::PrimRef r;
+            for (int k = 0; k < 3; ++k) {
+                float c0 = l.p[k], c1 = l.p[k] + l.u[k], c2 = l.p[k] + l.v[k], c3 = l.p[k] + l.u[k] + l.v[k];
+                r.lo[k] = std::min(std::min(c0, c1), std::min(c2, c3));
+                r.hi[k] = std::max(std::max(c0, c1), std::max(c2, c3));
+            }
+            r.kind = RT_KIND_PARA;
+            r.g = g;
+            refs.push_back(r);
+            float nn[3];
+            qt_normalize(L.n, nn);
+            for (int k = 0; k < 3; ++k) geo[4 * g + k] = nn[k];
+            pid[g] = i;
+            mat[g] = -(i + 1);
+        }
+    }
+    rtb::BuildResult br;
+    int br_status = rtb::build(refs, br);
+    if (br_status != 0)
+        return set_err(c, RT_ERR_BUILD, br_status == -2 ? "BVH deeper than the traversal stack" : "too many primitives");
+
+    // primitive records in leaf order
+    std::vector<float> rec((size_t)12 * G, 0.f);
+    for (size_t i = 0; i < refs.size(); ++i) {
+        int g = refs[i].g;
+        float* o = &rec[12 * i];
+        std::memcpy(&o[3], &g, 4);
+        if (g < nT) {
+            const int32_t* tv = d->tri_vertices + 3 * g;
+            for (int k = 0; k < 3; ++k) {
+                o[k] = P(tv[0], k);
+                o[4 + k] = P(tv[1], k) - P(tv[0], k);
+                o[8 + k] = P(tv[2], k) - P(tv[0], k);
+            }
+        } else if (g < nT + nS) {
+            const float* s = d->spheres + 4 * (g - nT);
+            for (int k = 0; k < 3; ++k) o[k] = s[k];
+            o[4] = s[3];
+        } else {
+            const RtLight& L = lights[pid[g]];
+            for (int k = 0; k < 3; ++k) {
+                o[k] = L.p[k];
+                o[4 + k] = L.u[k];
+                o[8 + k] = L.v[k];
+            }
+        }
+    }
+    std::vector<float> vn;
+    if (d->normals && nT) {
+        vn.resize((size_t)9 * nT);
+        for (int i = 0; i < nT; ++i)
+            for (int j = 0; j < 3; ++j) qt_normalize(d->normals + 3 * d->tri_vertices[3 * i + j], &vn[9 * i + 3 * j]);
+    }
+    std::vector<RtMaterial> mats(d->n_materials);
+    for (int i = 0; i < d->n_materials; ++i) {
+        const rt_material& m = d->materials[i];
+        RtMaterial& M = mats[i];
+        std::memset(&M, 0, sizeof(M));
+        M.type = m.type;
+        for (int k = 0; k < 3; ++k) {
+            M.kd[k] = m.kd[k];
+            M.ks[k] = m.ks[k];
+        }
+        M.exponent = m.exponent;
+        M.rho_d = lum_rgb(m.kd);
+        M.rho_s = lum_rgb(m.ks);
+    }
+    // environment: texels x coefficient, sampling tables in double precision
+    std::vector<float> erow, ecol, ep, ergb;
+    int W = 0, H = 0;
+    if (env >= 0) {
+        W = d->env_width;
+        H = d->env_height;
+        const float* coef = d->lights[env].intensity;
+        ergb.resize((size_t)4 * W * H);
+        std::vector<double> tp((size_t)W * H), roww(H, 0.0);
+        std::vector<double> colc((size_t)H * (W + 1), 0.0);
+        double total = 0;
+        for (int y = 0; y < H; ++y) {
+            double st = std::sin((y + 0.5) / H * M_PI);
+            double acc = 0;
+            for (int x = 0; x < W; ++x) {
+                float rgb[3];
+                for (int k = 0; k < 3; ++k) {
+                    rgb[k] = d->env_rgb[3 * ((size_t)y * W + x) + k] * coef[k];
+                    ergb[4 * ((size_t)y * W + x) + k] = rgb[k];
+                }
+                double w = (double)lum_rgb(rgb) * st;
+                tp[(size_t)y * W + x] = w;
+                acc += w;
+                colc[(size_t)y * (W + 1) + x + 1] = acc;
+            }
+            for (int x = 0; x <= W; ++x) {
+                double& v = colc[(size_t)y * (W + 1) + x];
+                v = acc > 0 ? v / acc : (double)x / W;
+            }
+            roww[y] = acc;
+            total += acc;
+        }
+        erow.resize(H + 1);
+        erow[0] = 0.f;
+        double acc = 0;
+        for (int y = 0; y < H; ++y) {
+            acc += roww[y];
+            erow[y + 1] = (float)(total > 0 ? acc / total : (double)(y + 1) / H);
+        }
+        ecol.resize(colc.size());
+        for (size_t i = 0; i < colc.size(); ++i) ecol[i] = (float)colc[i];
+        ep.resize(tp.size());
+        for (size_t i = 0; i < tp.size(); ++i) ep[i] = (float)(total > 0 ? tp[i] / total : 1.0 / ((double)W * H));
+    }
+
+    int r = 0;
+    if ((r = upload(c, c->nodes, br.nodes.data(), br.nodes.size() * 4)) ||
+        (r = upload(c, c->prims, rec.data(), rec.size() * 4)) ||
+        (r = upload(c, c->prim_id, pid.data(), pid.size() * 4)) || (r = upload(c, c->mat, mat.data(), mat.size() * 4)) ||
+        (r = upload(c, c->geo, geo.data(), geo.size() * 4)) ||
+        (r = upload(c, c->vnormals, vn.data(), vn.size() * 4)) ||
+        (r = upload(c, c->mats, mats.data(), mats.size() * sizeof(RtMaterial))) ||
+        (r = upload(c, c->lights, lights.data(), lights.size() * sizeof(RtLight))) ||
+        (r = upload(c, c->env_rgb, ergb.data(), ergb.size() * 4)) ||
+        (r = upload(c, c->env_row, erow.data(), erow.size() * 4)) ||
+        (r = upload(c, c->env_col, ecol.data(), ecol.size() * 4)) || (r = upload(c, c->env_p, ep.data(), ep.size() * 4)))
+        return r;
+    HostDevScene& s = c->scene;
+    s.nodes = c->nodes.p;
+    s.prims = c->prims.p;
+    s.prim_id = (const int32_t*)c->prim_id.p;
+    s.mat = (const int32_t*)c->mat.p;
+    s.vnormals = vn.empty() ? nullptr : (const float*)c->vnormals.p;
+    s.geo = c->geo.p;
+    s.n_nodes = (int32_t)(br.nodes.size() / 16);
+    s.n_tri = nT;
+    s.n_sphere = nS;
+    s.n_para = nP;
+    s.mats = (const RtMaterial*)c->mats.p;
+    s.lights = (const RtLight*)c->lights.p;
+    s.n_lights = d->n_lights;
+    s.env_light = env;
+    s.env_w = W;
+    s.env_h = H;
+    s.env_rgb = c->env_rgb.p;
+    s.env_row_cdf = (const float*)c->env_row.p;
+    s.env_col_cdf = (const float*)c->env_col.p;
+    s.env_texel_p = (const float*)c->env_p.p;
+    c->has_scene = true;
+    c->bvh_depth = br.depth;
+    c->stats.bvh_nodes = s.n_nodes;
+    c->stats.bvh_prims = G;
+    c->stats.bvh_depth = br.depth;
+    return RT_OK;
+}
+
+static int trace_device(rt_ctx* c, int kind, const float* o, const float* d, int64_t n, void* out0, void* out1)
+{
+    if (!c->has_scene) return set_err(c, RT_ERR_NO_SCENE, "no scene uploaded");
+    if (n == 0) return RT_OK;
+    hipSetDevice(c->device);
+    int grid = (int)std::min<int64_t>((n + 255) / 256, (int64_t)grid_for(c) * 4);
+    hipError_t e;
+    HIP_TRY(c, hipEventRecord(c->ev_a, c->stream));
+    if (kind == 0)
+        e = launch_trace_closest(c->scene, (const float4*)o, (const float4*)d, n, (int32_t*)out0, (float*)out1, 256,
+                                 grid, c->stream);
+    else if (kind == 1)
+        e = launch_trace_any(c->scene, (const float4*)o, (const float4*)d, n, (uint8_t*)out0, 256, grid, c->stream);
+    else
+        e = launch_trace_count(c->scene, (const float4*)o, (const float4*)d, n, (int32_t*)out0, (int32_t*)out1, 256,
+                               grid, c->stream);
+    if (e != hipSuccess) return set_err(c, RT_ERR_HIP, std::string("trace launch: ") + hipGetErrorString(e));
+    HIP_TRY(c, hipEventRecord(c->ev_b, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    float ms = 0;
+    hipEventElapsedTime(&ms, c->ev_a, c->ev_b);
+    c->stats.extend_ms = ms;
+    c->stats.extend_launches = 1;
+    return RT_OK;
+}
+
+int rt_trace_closest_device(rt_ctx* c, const float* o, const float* d, int64_t n, int32_t* prim, float* t)
+{
+    if (!c || n < 0 || (n && (!o || !d || !prim || !t))) return RT_ERR_INVALID;
+    if (int r = check_fault(c)) return r;
+    return trace_device(c, 0, o, d, n, prim, t);
+}
+
+int rt_trace_any_device(rt_ctx* c, const float* o, const float* d, int64_t n, uint8_t* occ)
+{
+    if (!c || n < 0 || (n && (!o || !d || !occ))) return RT_ERR_INVALID;
+    if (int r = check_fault(c)) return r;
+    return trace_device(c, 1, o, d, n, occ, nullptr);
+}
+
+int rt_trace_count_device(rt_ctx* c, const float* o, const float* d, int64_t n, int32_t* nodes, int32_t* prims)
+{
+    if (!c || n < 0 || (n && (!o || !d || !nodes || !prims))) return RT_ERR_INVALID;
+    if (int r = check_fault(c)) return r;
+    return trace_device(c, 2, o, d, n, nodes, prims);
+}
+
+static int trace_host(rt_ctx* c, int kind, const float* o, const float* d, int64_t n, void* out0, void* out1)
+{
+    if (!c->has_scene) return set_err(c, RT_ERR_NO_SCENE, "no scene uploaded");
+    if (n == 0) return RT_OK;
+    hipSetDevice(c->device);
+    size_t rb = (size_t)n * 16;
+    int r;
+    if ((r = ensure(c, c->rb_o, rb)) || (r = ensure(c, c->rb_d, rb)) || (r = ensure(c, c->rb_out0, (size_t)n * 4)) ||
+        (r = ensure(c, c->rb_out1, (size_t)n * 4)))
+        return r;
+    HIP_TRY(c, hipMemcpyAsync(c->rb_o.p, o, rb, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->rb_d.p, d, rb, hipMemcpyHostToDevice, c->stream));
+    if ((r = trace_device(c, kind, (const float*)c->rb_o.p, (const float*)c->rb_d.p, n, c->rb_out0.p, c->rb_out1.p)))
+        return r;
+    size_t b0 = kind == 1 ? (size_t)n : (size_t)n * 4;
+    HIP_TRY(c, hipMemcpy(out0, c->rb_out0.p, b0, hipMemcpyDeviceToHost));
+    if (out1) HIP_TRY(c, hipMemcpy(out1, c->rb_out1.p, (size_t)n * 4, hipMemcpyDeviceToHost));
+    return RT_OK;
+}
+
+int rt_trace_closest(rt_ctx* c, const float* o, const float* d, int64_t n, int32_t* prim, float* t)
+{
+    if (!c || n < 0 || (n && (!o || !d || !prim || !t))) return RT_ERR_INVALID;
+    if (int r = check_fault(c)) return r;
+    return trace_host(c, 0, o, d, n, prim, t);
+}
+
+int rt_trace_any(rt_ctx* c, const float* o, const float* d, int64_t n, uint8_t* occ)
+{
+    if (!c || n < 0 || (n && (!o || !d || !occ))) return RT_ERR_INVALID;
+    if (int r = check_fault(c)) return r;
+    return trace_host(c, 1, o, d, n, occ, nullptr);
+}
+
+/* ---------------------------------------------------------------------------
+ * Render driver
+ * ------------------------------------------------------------------------- */
+namespace {
+
+// src/camera.cpp:34-46 with the Qt arithmetic of the reference.
+DevView make_view(const rt_view& v, float* film)
+{
+    DevView V;
+    std::memset(&V, 0, sizeof(V));
+    const rt_camera& c = v.camera;
+    float wv[3] = {c.eye[0] - c.look_at[0], c.eye[1] - c.look_at[1], c.eye[2] - c.look_at[2]};
+    qt_normalize(wv, V.w);
+    float uc[3];
+    cross3(c.up, V.w, uc);
+    qt_normalize(uc, V.u);
+    cross3(V.w, V.u, V.v);
+    for (int k = 0; k < 3; ++k) V.eye[k] = c.eye[k];
+    float aspect = (float)c.width / c.height;
+    V.t = (float)std::tan((double)(c.fov / 2));
+    V.b = -V.t;
+    V.r = aspect * V.t;
+    V.l = -V.r;
+    V.W = c.width;
+    V.H = c.height;
+    V.spp_x = v.spp_x;
+    V.spp_y = v.spp_y;
+    V.inv_spp = 1.f / (float)(v.spp_x * v.spp_y);
+    V.film = film;
+    return V;
+}
+
+int alloc_path_state(rt_ctx* c, size_t cap)
+{
+    if (c->ps_cap >= cap) return 0;
+    for (auto& b : c->ps_bufs) release(b);
+    c->ps_bufs.assign(16, DevBuf());
+    float4** f4[] = {&c->ps.ray_o, &c->ps.ray_d, &c->ps.hit, &c->ps.alpha, &c->ps.radiance, &c->ps.pend_x,
+                     &c->ps.pend_n, &c->ps.pend_d, &c->ps.pend_a, &c->ps.pend_c, &c->ps.sh_o, &c->ps.sh_d,
+                     &c->ps.sh_c};
+    int i = 0;
+    for (float4** p : f4) {
+        if (int r = ensure(c, c->ps_bufs[i], cap * sizeof(float4))) return r;
+        *p = (float4*)c->ps_bufs[i++].p;
+    }
+    if (int r = ensure(c, c->ps_bufs[i], cap * sizeof(int4))) return r;
+    c->ps.info = (int4*)c->ps_bufs[i++].p;
+    for (int q = 0; q < 2; ++q) {
+        if (int r = ensure(c, c->ps_bufs[i], cap * sizeof(int32_t))) return r;
+        c->ps.queue[q] = (int32_t*)c->ps_bufs[i++].p;
+    }
+    if (int r = ensure(c, c->counters, sizeof(DevCounters))) return r;
+    c->ps.cnt = (DevCounters*)c->counters.p;
+    c->ps_cap = cap;
+    return 0;
+}
+
+}  // namespace
+
+static int render_impl(rt_ctx* c, const rt_view* views, int32_t nv, const rt_params* prm, float* const* films,
+                       bool device_films)
+{
+    if (!c || !views || nv <= 0 || !prm || !films) return RT_ERR_INVALID;
+    if (int r = check_fault(c)) return r;
+    if (!c->has_scene) return set_err(c, RT_ERR_NO_SCENE, "no scene uploaded");
+    if (prm->mode != RT_MODE_COMPAT && prm->mode != RT_MODE_UNBIASED)
+        return set_err(c, prm->mode == RT_MODE_GUIDED ? RT_ERR_NO_GUIDE : RT_ERR_INVALID, "unsupported mode");
+    if (prm->max_depth < 1 || prm->max_depth > 64 || prm->rr_prob < 0.f || prm->rr_prob >= 1.f || prm->rr_start < 0)
+        return set_err(c, RT_ERR_INVALID, "bad render parameters");
+    int64_t total_samples = 0;
+    int64_t max_pix = 0;
+    for (int i = 0; i < nv; ++i) {
+        const rt_view& v = views[i];
+        if (v.camera.width <= 0 || v.camera.height <= 0 || v.spp_x <= 0 || v.spp_y <= 0 || !films[i])
+            return set_err(c, RT_ERR_INVALID, "bad view");
+        int64_t npix = (int64_t)v.camera.width * v.camera.height;
+        if (npix > (1 << 26)) return set_err(c, RT_ERR_INVALID, "view too large");
+        total_samples += npix * v.spp_x * v.spp_y;
+        max_pix = std::max(max_pix, npix);
+    }
+    hipSetDevice(c->device);
+    int64_t P = c->cfg.path_slots > 0 ? c->cfg.path_slots : (int64_t)1 << 21;
+    P = std::min<int64_t>(P, std::max<int64_t>(total_samples, 1));
+    P = std::max<int64_t>(P, 256);
+
+    // film storage on the device for the host entry point
+    std::vector<float*> dfilm(nv);
+    if (!device_films) {
+        size_t tot = 0;
+        for (int i = 0; i < nv; ++i) tot += (size_t)views[i].camera.width * views[i].camera.height * 3;
+        if (int r = ensure(c, c->films, tot * 4)) return r;
+        size_t off = 0;
+        for (int i = 0; i < nv; ++i) {
+            size_t n = (size_t)views[i].camera.width * views[i].camera.height * 3;
+            dfilm[i] = (float*)c->films.p + off;
+            HIP_TRY(c, hipMemcpyAsync(dfilm[i], films[i], n * 4, hipMemcpyHostToDevice, c->stream));
+            off += n;
+        }
+    } else {
+        for (int i = 0; i < nv; ++i) dfilm[i] = films[i];
+    }
+
+    // tiles and passes: each pass holds at most P slots and at most one tile
+    // per (view, pixel); tiles of a view run in increasing sample order.
+    std::vector<DevView> dv(nv);
+    for (int i = 0; i < nv; ++i) dv[i] = make_view(views[i], dfilm[i]);
+    std::vector<DevTile> tiles;
+    std::vector<DevPass> passes;
+    DevPass cur{0, 0, 0, 0};
+    std::vector<int> in_pass;  // views present in the current pass
+    auto close_pass = [&]() {
+        if (cur.tile1 > cur.tile0) passes.push_back(cur);
+        cur = DevPass{(int32_t)tiles.size(), (int32_t)tiles.size(), 0, 0};
+        in_pass.clear();
+    };
+    for (int v = 0; v < nv; ++v) {
+        int64_t npix = (int64_t)views[v].camera.width * views[v].camera.height;
+        int spp = views[v].spp_x * views[v].spp_y;
+        int64_t chunk = std::min<int64_t>(npix, P);
+        for (int s0 = 0; s0 < spp;) {
+            int ns = (int)std::max<int64_t>(1, std::min<int64_t>(spp - s0, P / chunk));
+            for (int64_t p0 = 0; p0 < npix; p0 += chunk) {
+                int64_t np_ = std::min<int64_t>(chunk, npix - p0);
+                int64_t slots = np_ * ns;
+                bool dup = std::find(in_pass.begin(), in_pass.end(), v) != in_pass.end() && chunk == npix;
+                if (cur.slots + slots > P || dup) close_pass();
+                DevTile t{v, (int32_t)p0, (int32_t)np_, s0, ns, cur.slots, cur.pixels, 0};
+                tiles.push_back(t);
+                cur.tile1 = (int32_t)tiles.size();
+                cur.slots += (int32_t)slots;
+                cur.pixels += (int32_t)np_;
+                in_pass.push_back(v);
+            }
+            s0 += ns;
+            if (chunk < npix) close_pass();  // pixel-chunked views: one sample block per pass
+        }
+    }
+    close_pass();
+
+    int r;
+    if ((r = alloc_path_state(c, (size_t)P))) return r;
+    if ((r = upload(c, c->views, dv.data(), dv.size() * sizeof(DevView))) ||
+        (r = upload(c, c->tiles, tiles.data(), tiles.size() * sizeof(DevTile))) ||
+        (r = upload(c, c->passes, passes.data(), passes.size() * sizeof(DevPass))))
+        return r;
+    const DevView* d_views = (const DevView*)c->views.p;
+    const DevTile* d_tiles = (const DevTile*)c->tiles.p;
+    const DevPass* d_passes = (const DevPass*)c->passes.p;
+
+    PtParams pp{prm->mode, prm->max_depth, prm->rr_start, prm->rr_prob, prm->seed};
+    const int grid = grid_for(c);
+    const int depth = prm->max_depth;
+    const bool timing = c->timing != 0;
+    size_t need_ev = timing ? (size_t)passes.size() * depth * 4 : 0;
+    while (c->ev_pool.size() < need_ev) {
+        hipEvent_t e;
+        HIP_TRY(c, hipEventCreate(&e));
+        c->ev_pool.push_back(e);
+    }
+    HIP_TRY(c, hipGetLastError());
+    if ((r = (launch_reset_counters(c->ps, c->stream) == hipSuccess ? 0 : RT_ERR_HIP))) return set_err(c, r, "reset");
+    HIP_TRY(c, hipEventRecord(c->ev_a, c->stream));
+    size_t ev = 0;
+    bool cancelled = false;
+#define LCHK(x)                                                                                  \
+    do {                                                                                         \
+        hipError_t e_ = (x);                                                                     \
+        if (e_ != hipSuccess) return set_err(c, RT_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+    for (size_t pi = 0; pi < passes.size(); ++pi) {
+        if (c->stop) {
+            cancelled = true;
+            break;
+        }
+        LCHK(launch_generate(c->ps, d_views, d_tiles, d_passes, pp, grid, c->stream));
+        for (int dd = 0; dd < depth; ++dd) {
+            int q = dd & 1;
+            if (timing) LCHK(hipEventRecord(c->ev_pool[ev++], c->stream));
+            LCHK(launch_extend(c->scene, c->ps, q, c->cfg.count_traversal, grid, c->stream));
+            if (timing) LCHK(hipEventRecord(c->ev_pool[ev++], c->stream));
+            LCHK(launch_shade(c->scene, c->ps, pp, dd, q, grid, c->stream));
+            if (timing) LCHK(hipEventRecord(c->ev_pool[ev++], c->stream));
+            LCHK(launch_shadow(c->scene, c->ps, pp, 256, grid, c->stream));
+            if (timing) LCHK(hipEventRecord(c->ev_pool[ev++], c->stream));
+            LCHK(launch_advance(c->ps, q, c->stream));
+        }
+        LCHK(launch_accumulate(c->ps, d_views, d_tiles, d_passes, grid, c->stream));
+        LCHK(launch_pass_end(c->ps, c->stream));
+    }
+#undef LCHK
+    HIP_TRY(c, hipEventRecord(c->ev_b, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    float ms = 0;
+    hipEventElapsedTime(&ms, c->ev_a, c->ev_b);
+    c->stats.render_ms = ms;
+    double ext = 0, sh = 0;
+    for (size_t i = 0; i + 3 < ev; i += 4) {
+        float a = 0, b = 0;
+        hipEventElapsedTime(&a, c->ev_pool[i], c->ev_pool[i + 1]);
+        hipEventElapsedTime(&b, c->ev_pool[i + 2], c->ev_pool[i + 3]);
+        ext += a;
+        sh += b;
+    }
+    c->stats.extend_ms = ext;
+    c->stats.shadow_ms = sh;
+    c->stats.extend_launches = (int64_t)(ev / 4);
+    DevCounters hc;
+    HIP_TRY(c, hipMemcpy(&hc, c->ps.cnt, sizeof(hc), hipMemcpyDeviceToHost));
+    c->stats.camera_rays = (int64_t)hc.camera_rays;
+    c->stats.extension_rays = (int64_t)hc.extension_rays;
+    c->stats.shadow_rays = (int64_t)hc.shadow_rays;
+    c->stats.node_visits = (int64_t)hc.node_visits;
+    c->stats.prim_tests = (int64_t)hc.prim_tests;
+    c->stats.paths_per_pass = (int32_t)P;
+    c->stats.passes = (int32_t)passes.size();
+    if (!device_films) {
+        for (int i = 0; i < nv; ++i) {
+            size_t n = (size_t)views[i].camera.width * views[i].camera.height * 3;
+            HIP_TRY(c, hipMemcpy(films[i], dfilm[i], n * 4, hipMemcpyDeviceToHost));
+        }
+    }
+    if (cancelled) return set_err(c, RT_ERR_CANCELLED, "render cancelled");
+    return RT_OK;
+}
+
+int rt_render(rt_ctx* c, const rt_view* views, int32_t nv, const rt_params* p, float* const* films)
+{
+    return render_impl(c, views, nv, p, films, false);
+}
+
+int rt_render_device(rt_ctx* c, const rt_view* views, int32_t nv, const rt_params* p, float* const* films)
+{
+    return render_impl(c, views, nv, p, films, true);
+}
+
+}  // extern "C"

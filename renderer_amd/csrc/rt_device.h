@@ -1,0 +1,292 @@
+/*
+ * rt_device.h — device-side math, random stream, scene view and BVH
+ * traversal for gfx950.
+ */
+#ifndef RT_DEVICE_H
+#define RT_DEVICE_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/rt_scene.h"
+#include "rt_layout.h"
+
+#define RT_INF __builtin_inff()
+#define RT_EPS 0.001f  /* src/ray.h:34 */
+
+struct vec3 {
+    float x, y, z;
+};
+__device__ __forceinline__ vec3 mk(float x, float y, float z) { return vec3{x, y, z}; }
+__device__ __forceinline__ vec3 operator+(vec3 a, vec3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ vec3 operator-(vec3 a, vec3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ vec3 operator-(vec3 a) { return mk(-a.x, -a.y, -a.z); }
+__device__ __forceinline__ vec3 operator*(float s, vec3 a) { return mk(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ vec3 operator*(vec3 a, float s) { return mk(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ vec3 mul3(vec3 a, vec3 b) { return mk(a.x * b.x, a.y * b.y, a.z * b.z); }
+__device__ __forceinline__ float dot(vec3 a, vec3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ vec3 cross(vec3 a, vec3 b)
+{
+    return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ float len(vec3 a) { return sqrtf(dot(a, a)); }
+__device__ __forceinline__ vec3 norm(vec3 a)
+{
+    float l2 = dot(a, a);
+    return l2 > 0.f ? a * (1.0f / sqrtf(l2)) : mk(0.f, 0.f, 0.f);
+}
+/* src/vechelper.cpp:3-6 */
+__device__ __forceinline__ vec3 reflect(vec3 v, vec3 n) { return v - (2.f * dot(v, n)) * n; }
+__device__ __forceinline__ bool is_zero(vec3 a) { return a.x == 0.f && a.y == 0.f && a.z == 0.f; }
+__device__ __forceinline__ vec3 ld3(const float* p) { return mk(p[0], p[1], p[2]); }
+__device__ __forceinline__ vec3 xyz(float4 a) { return mk(a.x, a.y, a.z); }
+/* src/mathhelper.cpp:102-105 with RGB channel order */
+__device__ __forceinline__ float lum(vec3 c) { return 0.299f * c.x + 0.587f * c.y + 0.114f * c.z; }
+
+/* ---------------------------------------------------------------------------
+ * Philox-4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11), counter-based:
+ * counter = (pixel, sample, dimension group, stream), key = (seed, view).
+ * The dimension groups are fixed per use (see the path kernels), so every
+ * draw of a path is addressable without carried RNG state.
+ * ------------------------------------------------------------------------- */
+struct U4 {
+    float u[4];
+};
+__device__ __forceinline__ uint4 philox4x32_10(uint4 c, uint2 k)
+{
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        if (r > 0) {
+            k.x += 0x9E3779B9u;
+            k.y += 0xBB67AE85u;
+        }
+        uint32_t lo0 = 0xD2511F53u * c.x, hi0 = __umulhi(0xD2511F53u, c.x);
+        uint32_t lo1 = 0xCD9E8D57u * c.z, hi1 = __umulhi(0xCD9E8D57u, c.z);
+        c = make_uint4(hi1 ^ c.y ^ k.x, lo1, hi0 ^ c.w ^ k.y, lo0);
+    }
+    return c;
+}
+__device__ __forceinline__ U4 rng4(uint32_t seed, uint32_t view, uint32_t pixel, uint32_t sample, uint32_t group,
+                                   uint32_t stream = 0)
+{
+    uint4 o = philox4x32_10(make_uint4(pixel, sample, group, stream), make_uint2(seed, view));
+    const float s = 1.0f / 16777216.0f;
+    U4 r;
+    r.u[0] = (float)(o.x >> 8) * s;
+    r.u[1] = (float)(o.y >> 8) * s;
+    r.u[2] = (float)(o.z >> 8) * s;
+    r.u[3] = (float)(o.w >> 8) * s;
+    return r;
+}
+
+/* ---------------------------------------------------------------------------
+ * Scene as seen by the kernels (all pointers are device pointers).
+ * ------------------------------------------------------------------------- */
+struct DevScene {
+    const float4* nodes;    /* 4 float4 per interior node */
+    const float4* prims;    /* 3 float4 per primitive record */
+    const int32_t* prim_id; /* shading index g -> reference primitive ID */
+    const int32_t* mat;     /* g -> material index, or -(light+1) for an area light */
+    const float* vnormals;  /* 9 floats per triangle g, or null (flat) */
+    const float4* geo;      /* g -> flat normal (tri, para) or centre+radius (sphere) */
+    int32_t n_nodes;
+    int32_t n_tri, n_sphere, n_para; /* g ranges: tri [0,n_tri), sphere, parallelogram */
+    const RtMaterial* mats;
+    const RtLight* lights;
+    int32_t n_lights;
+    int32_t env_light;      /* index of the env light, -1 if none */
+    int32_t env_w, env_h;
+    const float4* env_rgb;  /* texels (rgb * coefficient) */
+    const float* env_row_cdf;  /* env_h + 1 */
+    const float* env_col_cdf;  /* env_h * (env_w + 1) */
+    const float* env_texel_p;  /* env_w * env_h */
+};
+
+/* ---------------------------------------------------------------------------
+ * Primitive tests.  The arithmetic follows the reference formulation
+ * (Cramer's rule on the un-normalised plane normal, src/triangle.cpp:30-56;
+ * quadratic with both roots, src/sphere.cpp:29-56 + src/csgobject.cpp:25-44),
+ * not Moller-Trumbore, so hit/miss decisions and t agree with it to rounding.
+ * Accept: tmin <= t <= tmax and t < best.
+ * ------------------------------------------------------------------------- */
+__device__ __forceinline__ bool hit_tri(vec3 o, vec3 d, float4 r0, float4 r1, float4 r2, float tmin, float tmax,
+                                        float best, float& t, float& u, float& v, bool para)
+{
+    vec3 p1 = xyz(r0), e1 = xyz(r1), e2 = xyz(r2);
+    vec3 ad = o - p1;
+    vec3 n = cross(e1, e2);
+    float w = -dot(n, d);
+    t = dot(n, ad) / w;
+    vec3 adxde = cross(ad, d);
+    u = dot(adxde, e2) / w;
+    v = -dot(adxde, e1) / w;
+    bool ok = (w != 0.f) && (t >= tmin) && (t <= tmax) && (t < best) && (u >= 0.f) && (u <= 1.f) && (v >= 0.f);
+    ok = ok && (para ? (v <= 1.f) : (u + v <= 1.f));
+    return ok;
+}
+
+__device__ __forceinline__ bool hit_sphere(vec3 o, vec3 d, float4 r0, float4 r1, float tmin, float tmax,
+                                           float best, float& t)
+{
+    vec3 c = xyz(r0);
+    float rad = r1.x;
+    vec3 diff = o - c;
+    float a = dot(d, d);
+    float b = 2.f * dot(d, diff);
+    float cc = dot(diff, diff) - rad * rad;
+    float disc = b * b - 4.f * a * cc;
+    if (!(disc >= 0.f)) return false;
+    float root = sqrtf(disc);
+    float t0 = (-b - root) / (2.f * a), t1 = (-b + root) / (2.f * a);
+    if (tmin > t1) return false;
+    t = t0 >= tmin ? t0 : t1;
+    return t <= tmax && t < best;
+}
+
+/* Slab test against one child box, with the reference's choice of near and
+ * far plane by the sign of 1/d (src/axisalignedbox.cpp:30-73): a NaN from
+ * 0 * inf on an axis constrains nothing (fmaxf/fminf drop NaN operands),
+ * and the comparison is inclusive so flat boxes of axis-aligned quads pass. */
+__device__ __forceinline__ float slab_enter(float lo, float hi, float o, float inv, float& far)
+{
+    float a = (lo - o) * inv, b = (hi - o) * inv;
+    bool pos = inv >= 0.f;
+    far = pos ? b : a;
+    return pos ? a : b;
+}
+
+/* Leaf word decoding (rt_layout.h). */
+__device__ __forceinline__ void leaf_decode(int32_t w, uint32_t& first, uint32_t& count, uint32_t& kind)
+{
+    uint32_t x = ~(uint32_t)w;
+    count = x & 7u;
+    kind = (x >> 3) & 3u;
+    first = x >> 5;
+}
+
+struct TraceOut {
+    float t, u, v;
+    int32_t g;
+};
+
+/*
+ * Stack-based traversal of the flattened BVH, nearest child first, pruned
+ * by the closest hit so far.  The traversal stack lives in LDS, laid out
+ * [depth][thread] so the 64 lanes of a wave hit 64 different banks.
+ * ANY: return at the first hit in range (shadow rays).
+ * COUNT: count interior nodes fetched and primitives tested.
+ */
+template <bool ANY, bool COUNT>
+__device__ __forceinline__ TraceOut traverse(const DevScene& S, vec3 o, vec3 d, float tmin, float tmax,
+                                             int32_t* stack, int stride, int* n_nodes = nullptr,
+                                             int* n_prims = nullptr)
+{
+    TraceOut out;
+    out.t = RT_INF;
+    out.u = out.v = 0.f;
+    out.g = -1;
+    if (S.n_nodes == 0) return out;
+    vec3 inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    float best = RT_INF;
+    int sp = 0;
+    int32_t cur = 0;
+    int cn = 0, cp = 0;
+    while (true) {
+        const float4* nd = S.nodes + 4 * (size_t)cur;
+        float4 f0 = nd[0], f1 = nd[1], f2 = nd[2], f3 = nd[3];
+        if (COUNT) ++cn;
+        int32_t c0 = __float_as_int(f3.x), c1 = __float_as_int(f3.y);
+        float bound = fminf(tmax, best);
+        bound = bound + fabsf(bound) * 9.5367431640625e-07f; /* 2^-20 slack for rounding */
+        float fx, fy, fz;
+        float nx = slab_enter(f0.x, f0.y, o.x, inv.x, fx);
+        float ny = slab_enter(f0.z, f0.w, o.y, inv.y, fy);
+        float nz = slab_enter(f1.x, f1.y, o.z, inv.z, fz);
+        float tl0 = fmaxf(fmaxf(nx, ny), fmaxf(nz, tmin));
+        float th0 = fminf(fminf(fx, fy), fminf(fz, bound));
+        nx = slab_enter(f1.z, f1.w, o.x, inv.x, fx);
+        ny = slab_enter(f2.x, f2.y, o.y, inv.y, fy);
+        nz = slab_enter(f2.z, f2.w, o.z, inv.z, fz);
+        float tl1 = fmaxf(fmaxf(nx, ny), fmaxf(nz, tmin));
+        float th1 = fminf(fminf(fx, fy), fminf(fz, bound));
+        bool h0 = tl0 <= th0, h1 = tl1 <= th1;
+        /* order: nearer child first */
+        bool swap = h0 && h1 && tl1 < tl0;
+        int32_t a = swap ? c1 : c0, b = swap ? c0 : c1;
+        bool ha = swap ? h1 : h0, hb = swap ? h0 : h1;
+        int32_t next = -1;
+        bool have_next = false;
+#pragma unroll
+        for (int side = 0; side < 2; ++side) {
+            int32_t c = side == 0 ? a : b;
+            bool hc = side == 0 ? ha : hb;
+            if (!hc) continue;
+            if (c >= 0) {
+                if (!have_next) {
+                    next = c;
+                    have_next = true;
+                } else {
+                    stack[sp * stride] = c;
+                    ++sp;
+                }
+                continue;
+            }
+            uint32_t first, count, kind;
+            leaf_decode(c, first, count, kind);
+            for (uint32_t j = 0; j < count; ++j) {
+                const float4* pr = S.prims + 3 * (size_t)(first + j);
+                float4 r0 = pr[0], r1 = pr[1];
+                if (COUNT) ++cp;
+                float t, u = 0.f, v = 0.f;
+                bool hit;
+                if (kind == RT_KIND_SPHERE) {
+                    hit = hit_sphere(o, d, r0, r1, tmin, tmax, best, t);
+                } else {
+                    float4 r2 = pr[2];
+                    hit = hit_tri(o, d, r0, r1, r2, tmin, tmax, best, t, u, v, kind == RT_KIND_PARA);
+                }
+                if (hit) {
+                    best = t;
+                    out.t = t;
+                    out.u = u;
+                    out.v = v;
+                    out.g = __float_as_int(r0.w);
+                    if (ANY) {
+                        if (COUNT) { *n_nodes = cn; *n_prims = cp; }
+                        return out;
+                    }
+                }
+            }
+        }
+        if (have_next) {
+            cur = next;
+        } else {
+            if (sp == 0) break;
+            --sp;
+            cur = stack[sp * stride];
+        }
+    }
+    if (COUNT) { *n_nodes = cn; *n_prims = cp; }
+    return out;
+}
+
+/* Shading normal of a hit (normalised), src/hitrecord.h:66-70: triangles
+ * interpolate vertex normals or use the flat normal (p1-p2)x(p1-p3)
+ * (src/triangle.h:35-38, precomputed in geo[g]); spheres use p - c
+ * (src/sphere.cpp:46); parallelograms u x v (src/arealight.cpp:31-33). */
+__device__ __forceinline__ vec3 hit_normal(const DevScene& S, int32_t g, vec3 p, float u, float v)
+{
+    float4 gg = S.geo[g];
+    if (g < S.n_tri) {
+        if (S.vnormals) {
+            const float* n = S.vnormals + 9 * (size_t)g;
+            vec3 nn = (1.f - u - v) * ld3(n) + u * ld3(n + 3) + v * ld3(n + 6);
+            return norm(nn);
+        }
+        return xyz(gg);
+    }
+    if (g < S.n_tri + S.n_sphere) return norm(p - xyz(gg));
+    return xyz(gg);
+}
+
+#endif

@@ -1,0 +1,116 @@
+/*
+ * rt_kernels.h — host-side launch interface of the gfx950 kernels
+ * (rt_kernels.hip) used by the C ABI layer (rt_abi.cpp).
+ */
+#ifndef RT_KERNELS_H
+#define RT_KERNELS_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "rt_layout.h"
+
+/* Device scene view; mirrors DevScene in rt_device.h field by field. */
+struct HostDevScene {
+    const void* nodes;
+    const void* prims;
+    const int32_t* prim_id;
+    const int32_t* mat;
+    const float* vnormals;
+    const void* geo;
+    int32_t n_nodes;
+    int32_t n_tri, n_sphere, n_para;
+    const RtMaterial* mats;
+    const RtLight* lights;
+    int32_t n_lights;
+    int32_t env_light;
+    int32_t env_w, env_h;
+    const void* env_rgb;
+    const float* env_row_cdf;
+    const float* env_col_cdf;
+    const float* env_texel_p;
+};
+
+/* One view of a render: camera basis computed on the host with the
+ * reference's arithmetic (src/camera.cpp:34-46). */
+struct DevView {
+    float eye[3], u[3], v[3], w[3];
+    float l, r, t, b;
+    int32_t W, H, spp_x, spp_y;
+    float inv_spp;
+    int32_t pad;
+    float* film;
+};
+
+/* A tile: samples [s0, s0+ns) of pixels [pix0, pix0+npix) of one view,
+ * occupying path slots [slot0, slot0 + npix*ns) of its pass.  Slot
+ * slot0 + j maps to pixel pix0 + j % npix, sample s0 + j / npix. */
+struct DevTile {
+    int32_t view, pix0, npix, s0, ns, slot0;
+    int32_t acc0;  /* first pixel of this tile among the pass's pixels */
+    int32_t pad;
+};
+
+/* A pass: tiles [tile0, tile1), `slots` path slots in total. */
+struct DevPass {
+    int32_t tile0, tile1, slots, pixels;  /* pixels = sum of npix over its tiles */
+};
+
+/* Device counters of the wavefront pipeline. */
+struct DevCounters {
+    int32_t q_count[2];      /* path queue lengths (ping-pong) */
+    int32_t shadow_count;
+    int32_t pass;            /* current pass index */
+    unsigned long long camera_rays, extension_rays, shadow_rays;
+    unsigned long long node_visits, prim_tests;
+};
+
+/* Path state in HBM (structure of arrays, capacity = path slots). */
+struct PathState {
+    float4* ray_o;   /* o.xyz, tmin */
+    float4* ray_d;   /* d.xyz, tmax */
+    float4* hit;     /* t, g (int bits), u, v */
+    float4* alpha;   /* throughput rgb */
+    float4* radiance;/* accumulated radiance rgb */
+    int4* info;      /* pixel, sample, view, packed depth/stored/length/flags */
+    float4* pend_x;  /* compat: previous stored vertex: position, material */
+    float4* pend_n;  /* compat: its normal */
+    float4* pend_d;  /* compat: its incoming direction (unit) */
+    float4* pend_a;  /* compat: its throughput */
+    float4* pend_c;  /* compat: its pending next-event contribution */
+    float4* sh_o;    /* shadow rays: o, tmin */
+    float4* sh_d;    /* d, tmax */
+    float4* sh_c;    /* contribution rgb, path slot (int bits) */
+    int32_t* queue[2];
+    DevCounters* cnt;
+};
+
+struct PtParams {
+    int32_t mode, max_depth, rr_start;
+    float rr_prob;
+    uint32_t seed;
+};
+
+/* Batch traversal kernels (parity surface and benchmarks). */
+hipError_t launch_trace_closest(const HostDevScene& s, const float4* o, const float4* d, int64_t n, int32_t* prim,
+                                float* t, int block, int grid, hipStream_t st);
+hipError_t launch_trace_any(const HostDevScene& s, const float4* o, const float4* d, int64_t n, uint8_t* occ,
+                            int block, int grid, hipStream_t st);
+hipError_t launch_trace_count(const HostDevScene& s, const float4* o, const float4* d, int64_t n, int32_t* nodes,
+                              int32_t* prims, int block, int grid, hipStream_t st);
+
+/* Wavefront pipeline kernels. */
+hipError_t launch_generate(const PathState& ps, const DevView* views, const DevTile* tiles, const DevPass* passes,
+                           PtParams p, int grid, hipStream_t st);
+hipError_t launch_extend(const HostDevScene& s, const PathState& ps, int qsel, int count, int grid, hipStream_t st);
+hipError_t launch_shade(const HostDevScene& s, const PathState& ps, PtParams p, int depth, int qsel, int grid,
+                        hipStream_t st);
+hipError_t launch_shadow(const HostDevScene& s, const PathState& ps, PtParams p, int block, int grid,
+                         hipStream_t st);
+hipError_t launch_advance(const PathState& ps, int qsel, hipStream_t st);
+hipError_t launch_accumulate(const PathState& ps, const DevView* views, const DevTile* tiles,
+                             const DevPass* passes, int grid, hipStream_t st);
+hipError_t launch_pass_end(const PathState& ps, hipStream_t st);
+hipError_t launch_reset_counters(const PathState& ps, hipStream_t st);
+
+#endif

@@ -1,0 +1,833 @@
+/*
+ * rt_kernels.hip — gfx950 kernels of the path-tracing hot path.
+ *
+ * Wavefront organisation (one pass = a set of path slots):
+ *   generate  -> camera rays for every slot            (src/camera.cpp:50-58)
+ *   repeat max_depth times:
+ *     extend  -> closest hit of every live path        (src/bvh.cpp:135-149)
+ *     shade   -> emission, BSDF sample, next-event ray (src/renderer.cpp:69-106,
+ *                src/unidipathtracingintegrator.cpp:48-107)
+ *     shadow  -> any-hit test of next-event rays       (src/arealight.cpp:35-51)
+ *     advance -> swap the live-path queues
+ *   accumulate -> film += radiance / spp in sample order (src/perpixelrenderer.cpp:92)
+ * Every kernel is persistent (a fixed grid strides over a queue whose length
+ * it reads from device memory), so the host never waits inside a pass and
+ * the whole sequence can be replayed as one hipGraph.  Queues are compacted
+ * with a wave-wide ballot + mbcnt prefix and one atomic per wave.
+ */
+#include "rt_device.h"
+#include "rt_kernels.h"
+
+#include <cstring>
+
+#define PT_BLOCK 256
+#define RT_PI 3.14159265358979f
+#define RT_INV_PI 0.318309886183791f
+#define RT_INV_2PI 0.159154943091895f
+
+static_assert(sizeof(HostDevScene) == sizeof(DevScene), "scene view layouts differ");
+
+static inline DevScene as_dev(const HostDevScene& s)
+{
+    DevScene d;
+    std::memcpy(&d, &s, sizeof(d));
+    return d;
+}
+
+/* Append to a queue from a wave in uniform control flow: one atomic per
+ * wave, lane offsets from the ballot mask. */
+__device__ __forceinline__ int wave_append(bool pred, int32_t* counter)
+{
+    unsigned long long m = __ballot(pred);
+    uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+    int total = __popcll(m);
+    int base = 0;
+    if (__lane_id() == 0 && total > 0) base = atomicAdd(counter, total);
+    base = __shfl(base, 0);
+    return base + (int)below;
+}
+
+__device__ __forceinline__ void wave_count(bool pred, unsigned long long* counter)
+{
+    unsigned long long m = __ballot(pred);
+    if (__lane_id() == 0 && m) atomicAdd(counter, (unsigned long long)__popcll(m));
+}
+
+/* ===========================================================================
+ * Batch traversal kernels
+ * ========================================================================= */
+template <bool ANY, bool COUNT>
+__global__ __launch_bounds__(PT_BLOCK) void k_trace(DevScene S, const float4* __restrict__ o,
+                                                   const float4* __restrict__ d, int64_t n, int32_t* prim, float* tout,
+                                                   uint8_t* occ, int32_t* nn, int32_t* np)
+{
+    __shared__ int32_t stack[RT_STACK_DEPTH * PT_BLOCK];
+    int32_t* my = stack + threadIdx.x;
+    for (int64_t i = (int64_t)blockIdx.x * PT_BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * PT_BLOCK) {
+        float4 a = o[i], b = d[i];
+        int cn = 0, cp = 0;
+        TraceOut h = traverse<ANY, COUNT>(S, xyz(a), xyz(b), a.w, b.w, my, PT_BLOCK, &cn, &cp);
+        if (ANY) {
+            occ[i] = h.g >= 0 ? 1 : 0;
+        } else {
+            prim[i] = h.g >= 0 ? S.prim_id[h.g] : -1;
+            tout[i] = h.g >= 0 ? h.t : __builtin_nanf("");
+        }
+        if (COUNT) {
+            nn[i] = cn;
+            np[i] = cp;
+        }
+    }
+}
+
+hipError_t launch_trace_closest(const HostDevScene& s, const float4* o, const float4* d, int64_t n, int32_t* prim,
+                                float* t, int block, int grid, hipStream_t st)
+{
+    (void)block;
+    hipLaunchKernelGGL((k_trace<false, false>), dim3(grid), dim3(PT_BLOCK), 0, st, as_dev(s), o, d, n, prim, t,
+                       (uint8_t*)nullptr, (int32_t*)nullptr, (int32_t*)nullptr);
+    return hipGetLastError();
+}
+hipError_t launch_trace_any(const HostDevScene& s, const float4* o, const float4* d, int64_t n, uint8_t* occ,
+                            int block, int grid, hipStream_t st)
+{
+    (void)block;
+    hipLaunchKernelGGL((k_trace<true, false>), dim3(grid), dim3(PT_BLOCK), 0, st, as_dev(s), o, d, n,
+                       (int32_t*)nullptr, (float*)nullptr, occ, (int32_t*)nullptr, (int32_t*)nullptr);
+    return hipGetLastError();
+}
+hipError_t launch_trace_count(const HostDevScene& s, const float4* o, const float4* d, int64_t n, int32_t* nodes,
+                              int32_t* prims, int block, int grid, hipStream_t st)
+{
+    (void)block;
+    hipLaunchKernelGGL((k_trace<false, true>), dim3(grid), dim3(PT_BLOCK), 0, st, as_dev(s), o, d, n,
+                       (int32_t*)nullptr, (float*)nullptr, (uint8_t*)nullptr, nodes, prims);
+    return hipGetLastError();
+}
+
+/* ===========================================================================
+ * Sampling and material helpers
+ * ========================================================================= */
+
+/* Cosine-power lobe about w, src/sampler.cpp:48-76: z = u0^(1/(n+1)),
+ * r = sqrt(1 - u0^(2/(n+1))), phi = 2 pi u1, pdf = z^n (n+1) / 2pi, frame
+ * vector chosen by the smallest component of w. */
+__device__ __forceinline__ vec3 basis_v(vec3 w)
+{
+    vec3 v;
+    if (fabsf(w.x) < fabsf(w.y) && fabsf(w.x) < fabsf(w.z))
+        v = mk(0.f, w.z, -w.y);
+    else if (fabsf(w.y) < fabsf(w.z))
+        v = mk(w.z, 0.f, -w.x);
+    else
+        v = mk(w.y, -w.x, 0.f);
+    return norm(v);
+}
+__device__ __forceinline__ vec3 cos_pow_dir(float u0, float u1, vec3 w, float n, float& pdf)
+{
+    w = norm(w);
+    float z, r;
+    if (n == 1.f) {
+        z = sqrtf(u0);
+        r = sqrtf(1.f - u0);
+        pdf = z * RT_INV_PI;
+    } else {
+        z = powf(u0, 1.f / (n + 1.f));
+        r = sqrtf(fmaxf(0.f, 1.f - powf(u0, 2.f / (n + 1.f))));
+        pdf = powf(z, n) * (n + 1.f) * RT_INV_2PI;
+    }
+    float sp, cp;
+    sincosf(2.f * RT_PI * u1, &sp, &cp);
+    vec3 v = basis_v(w);
+    vec3 u = cross(v, w);
+    return norm(u * (cp * r) + v * (sp * r) + w * z);
+}
+
+/* Environment: lat-long lookup (nearest texel) and importance sampling by a
+ * row-marginal / column-conditional table weighted by luminance x sin(theta). */
+__device__ __forceinline__ int env_texel(const DevScene& S, vec3 d)
+{
+    float dy = fminf(1.f, fmaxf(-1.f, d.y));
+    float theta = acosf(dy);
+    float phi = atan2f(d.x, -d.z);
+    float u = phi * RT_INV_2PI + 0.5f, v = theta * RT_INV_PI;
+    int x = min(max((int)(u * S.env_w), 0), S.env_w - 1);
+    int y = min(max((int)(v * S.env_h), 0), S.env_h - 1);
+    return y * S.env_w + x;
+}
+__device__ __forceinline__ vec3 env_value(const DevScene& S, vec3 d)
+{
+    return xyz(S.env_rgb[env_texel(S, norm(d))]);
+}
+__device__ __forceinline__ int cdf_pick(const float* cdf, int n, float u, float& frac)
+{
+    int lo = 0, hi = n; /* find largest i in [0,n) with cdf[i] <= u */
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (cdf[mid] <= u) lo = mid;
+        else hi = mid;
+    }
+    float w = cdf[lo + 1] - cdf[lo];
+    frac = w > 0.f ? (u - cdf[lo]) / w : 0.5f;
+    frac = fminf(fmaxf(frac, 0.f), 0.99999988f);
+    return lo;
+}
+__device__ __forceinline__ vec3 env_sample(const DevScene& S, float u0, float u1, float& pdf)
+{
+    float fy, fx;
+    int y = cdf_pick(S.env_row_cdf, S.env_h, u0, fy);
+    int x = cdf_pick(S.env_col_cdf + (size_t)y * (S.env_w + 1), S.env_w, u1, fx);
+    float theta = ((float)y + fy) / (float)S.env_h * RT_PI;
+    float phi = (((float)x + fx) / (float)S.env_w - 0.5f) * 2.f * RT_PI;
+    float st, ct, sp, cp;
+    sincosf(theta, &st, &ct);
+    sincosf(phi, &sp, &cp);
+    float p = S.env_texel_p[(size_t)y * S.env_w + x];
+    pdf = st > 0.f ? p * (float)S.env_w * (float)S.env_h / (2.f * RT_PI * RT_PI * st) : 0.f;
+    return mk(st * sp, ct, -st * cp);
+}
+
+/* Area-light intensity towards `direction` (from the light to the point),
+ * src/arealight.cpp:53-59: I * max(0, cos) * |n| / |direction|^2. */
+__device__ __forceinline__ vec3 area_intensity(const RtLight& L, vec3 direction)
+{
+    vec3 n = ld3(L.n);
+    float c = dot(norm(direction), norm(n)) / dot(direction, direction) * len(n);
+    c = fmaxf(c, 0.f);
+    return ld3(L.I) * c;
+}
+
+/* Reference BRDF value, src/diffusematerial.cpp:25-28, src/phongmaterial.cpp:31-49.
+ * `n` is the unflipped unit normal, `din` the unit incoming ray direction,
+ * `dir` the direction from the light towards the point. */
+__device__ __forceinline__ vec3 brdf_ref(const RtMaterial& m, vec3 n, vec3 din, vec3 dir)
+{
+    if (m.type == RT_MAT_DIFFUSE) return ld3(m.kd) * RT_INV_PI;
+    if (m.type == RT_MAT_PHONG) {
+        vec3 refl = reflect(dir, n);
+        float dp = fmaxf(dot(norm(refl), -din), 0.f);
+        float spec = (m.exponent + 2.f) * RT_INV_2PI * powf(dp, m.exponent);
+        return ld3(m.kd) * RT_INV_PI + ld3(m.ks) * spec;
+    }
+    return mk(0.f, 0.f, 0.f);
+}
+/* Phong specular term alone, src/phongmaterial.cpp:42-49. */
+__device__ __forceinline__ vec3 phong_spec_ref(const RtMaterial& m, vec3 n, vec3 din, vec3 dir)
+{
+    vec3 refl = reflect(dir, n);
+    float dp = fmaxf(dot(norm(refl), -din), 0.f);
+    return ld3(m.ks) * ((m.exponent + 2.f) * RT_INV_2PI * powf(dp, m.exponent));
+}
+
+/* Unbiased-mode BSDF helpers: Phong as a luminance-weighted mixture of the
+ * diffuse lobe and the cos^n lobe about the mirror direction. */
+__device__ __forceinline__ vec3 lobe_dir(float u0, float u1, vec3 axis, float expo)
+{
+    float z = expo == 1.f ? sqrtf(u0) : powf(u0, 1.f / (expo + 1.f));
+    float r = sqrtf(fmaxf(0.f, 1.f - z * z));
+    float sp, cp;
+    sincosf(2.f * RT_PI * u1, &sp, &cp);
+    vec3 v = basis_v(axis);
+    vec3 t = cross(v, axis);
+    return norm(t * (r * cp) + v * (r * sp) + axis * z);
+}
+__device__ __forceinline__ vec3 bsdf_eval(const RtMaterial& m, vec3 wo, vec3 wi, vec3 nf)
+{
+    if (m.type == RT_MAT_DIFFUSE) return ld3(m.kd) * RT_INV_PI;
+    if (m.type == RT_MAT_PHONG) {
+        vec3 r = reflect(-wo, nf);
+        float c = fmaxf(0.f, dot(r, wi));
+        float s = (m.exponent + 2.f) * RT_INV_2PI * powf(c, m.exponent);
+        return ld3(m.kd) * RT_INV_PI + ld3(m.ks) * s;
+    }
+    return mk(0.f, 0.f, 0.f);
+}
+__device__ __forceinline__ float bsdf_pdf(const RtMaterial& m, vec3 wo, vec3 wi, vec3 nf)
+{
+    float cs = dot(wi, nf);
+    if (cs <= 0.f) return 0.f;
+    if (m.type == RT_MAT_DIFFUSE) return cs * RT_INV_PI;
+    if (m.type == RT_MAT_PHONG) {
+        float pd = m.rho_d / (m.rho_d + m.rho_s);
+        vec3 r = reflect(-wo, nf);
+        float c = fmaxf(0.f, dot(r, wi));
+        return pd * cs * RT_INV_PI + (1.f - pd) * (m.exponent + 1.f) * RT_INV_2PI * powf(c, m.exponent);
+    }
+    return 0.f;
+}
+__device__ __forceinline__ bool bsdf_sample(const RtMaterial& m, vec3 wo, vec3 nf, float u0, float u1, float u2,
+                                            vec3& wi)
+{
+    if (m.type == RT_MAT_DIFFUSE) {
+        wi = lobe_dir(u0, u1, nf, 1.f);
+        return true;
+    }
+    if (m.type == RT_MAT_PHONG) {
+        if (!(m.rho_d + m.rho_s > 0.f)) return false;
+        float pd = m.rho_d / (m.rho_d + m.rho_s);
+        wi = u2 < pd ? lobe_dir(u0, u1, nf, 1.f) : lobe_dir(u0, u1, reflect(-wo, nf), m.exponent);
+        return dot(wi, nf) > 0.f;
+    }
+    return false;
+}
+
+__device__ __forceinline__ int sgn(float x) { return (0.f < x) - (x < 0.f); }
+
+/* Packed path bookkeeping in info.w: stored vertices (8 bits), path length
+ * (8 bits), specular-chain flag (bit 16). */
+__device__ __forceinline__ int pk(int k, int L, int spec) { return k | (L << 8) | (spec << 16); }
+
+/* ===========================================================================
+ * generate
+ * ========================================================================= */
+__global__ __launch_bounds__(PT_BLOCK) void k_generate(PathState ps, const DevView* __restrict__ views,
+                                                      const DevTile* __restrict__ tiles,
+                                                      const DevPass* __restrict__ passes, PtParams P)
+{
+    const DevPass pass = passes[ps.cnt->pass];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ps.cnt->q_count[0] = pass.slots;
+        ps.cnt->q_count[1] = 0;
+        ps.cnt->shadow_count = 0;
+        ps.cnt->camera_rays += (unsigned long long)pass.slots;
+    }
+    for (int slot = blockIdx.x * PT_BLOCK + threadIdx.x; slot < pass.slots; slot += gridDim.x * PT_BLOCK) {
+        int lo = pass.tile0, hi = pass.tile1;
+        while (hi - lo > 1) {
+            int mid = (lo + hi) >> 1;
+            if (tiles[mid].slot0 <= slot) lo = mid;
+            else hi = mid;
+        }
+        const DevTile T = tiles[lo];
+        const DevView& V = views[T.view];
+        int j = slot - T.slot0;
+        int pixel = T.pix0 + j % T.npix;
+        int s = T.s0 + j / T.npix;
+        U4 g0 = rng4(P.seed, (uint32_t)T.view, (uint32_t)pixel, (uint32_t)s, 0u);
+        int a = s / V.spp_y, b = s % V.spp_y;
+        float sx = (g0.u[0] + (float)a) / (float)V.spp_x, sy = (g0.u[1] + (float)b) / (float)V.spp_y;
+        float px = (float)(pixel % V.W) + sx, py = (float)(pixel / V.W) + sy;
+        float x = V.l + (V.r - V.l) * (px + 0.5f) / (float)V.W;
+        float y = V.t + (V.b - V.t) * (py + 0.5f) / (float)V.H;
+        vec3 d = ld3(V.u) * x + ld3(V.v) * y - ld3(V.w);
+        if (P.mode != RT_MODE_COMPAT) d = norm(d);
+        int L = P.max_depth;
+        if (P.mode == RT_MODE_COMPAT && P.rr_prob > 0.f) {
+            int nb = 0;
+            if (P.rr_prob < 1.f) {
+                float q = floorf(logf(1.f - g0.u[2]) / logf(1.f - P.rr_prob));
+                nb = q > 1000.f ? 1000 : (int)q;
+            }
+            L = min(P.rr_start + nb, P.max_depth);
+        }
+        ps.ray_o[slot] = make_float4(V.eye[0], V.eye[1], V.eye[2], RT_EPS);
+        ps.ray_d[slot] = make_float4(d.x, d.y, d.z, RT_INF);
+        ps.alpha[slot] = make_float4(1.f, 1.f, 1.f, 0.f);
+        ps.radiance[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
+        ps.pend_c[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
+        ps.info[slot] = make_int4(pixel, s, T.view, pk(0, L, 1));
+        ps.queue[0][slot] = slot;
+    }
+}
+
+hipError_t launch_generate(const PathState& ps, const DevView* views, const DevTile* tiles, const DevPass* passes,
+                           PtParams p, int grid, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_generate, dim3(grid), dim3(PT_BLOCK), 0, st, ps, views, tiles, passes, p);
+    return hipGetLastError();
+}
+
+/* ===========================================================================
+ * extend: closest hit for every live path
+ * ========================================================================= */
+template <bool COUNT>
+__global__ __launch_bounds__(PT_BLOCK) void k_extend(DevScene S, PathState ps, int qsel)
+{
+    __shared__ int32_t stack[RT_STACK_DEPTH * PT_BLOCK];
+    int32_t* my = stack + threadIdx.x;
+    const int n = ps.cnt->q_count[qsel];
+    const int32_t* q = ps.queue[qsel];
+    unsigned long long tn = 0, tp = 0;
+    for (int i = blockIdx.x * PT_BLOCK + threadIdx.x; i < n; i += gridDim.x * PT_BLOCK) {
+        int p = q[i];
+        float4 a = ps.ray_o[p], b = ps.ray_d[p];
+        int cn = 0, cp = 0;
+        TraceOut h = traverse<false, COUNT>(S, xyz(a), xyz(b), a.w, b.w, my, PT_BLOCK, &cn, &cp);
+        ps.hit[p] = make_float4(h.t, __int_as_float(h.g), h.u, h.v);
+        if (COUNT) {
+            tn += cn;
+            tp += cp;
+        }
+    }
+    if (COUNT && (tn || tp)) {
+        atomicAdd(&ps.cnt->node_visits, tn);
+        atomicAdd(&ps.cnt->prim_tests, tp);
+    }
+}
+
+hipError_t launch_extend(const HostDevScene& s, const PathState& ps, int qsel, int count, int grid, hipStream_t st)
+{
+    if (count)
+        hipLaunchKernelGGL(k_extend<true>, dim3(grid), dim3(PT_BLOCK), 0, st, as_dev(s), ps, qsel);
+    else
+        hipLaunchKernelGGL(k_extend<false>, dim3(grid), dim3(PT_BLOCK), 0, st, as_dev(s), ps, qsel);
+    return hipGetLastError();
+}
+
+/* ===========================================================================
+ * shade, reference ("compat") estimator
+ *
+ * Restates Renderer::createPath (src/renderer.cpp:69-106) and
+ * UniDiPathTracingIntegrator::integrate(Path) (src/unidipathtracingintegrator.cpp:48-107)
+ * incrementally.  The reference first builds the whole path and then adds,
+ * for every stored vertex, either its next-event estimate or - when the next
+ * stored vertex is an emitter - that emitter's emission.  Here the
+ * next-event estimate of the latest stored vertex is held "pending" in the
+ * path state (with that vertex's position, normal, direction, material and
+ * throughput) until the next stored vertex is known: it is committed when
+ * that vertex is an ordinary surface or the path ends, and replaced by the
+ * emission term when it is an emitter.
+ * ========================================================================= */
+__device__ void shade_compat(const DevScene& S, const PathState& ps, const PtParams& P, int depth, int p,
+                             bool& cont, bool& shadow, float4& so, float4& sd, float4& sc)
+{
+    cont = false;
+    shadow = false;
+    int4 info = ps.info[p];
+    int k = info.w & 0xff, L = (info.w >> 8) & 0xff;
+    float4 h = ps.hit[p];
+    int g = __float_as_int(h.y);
+    float4 ro = ps.ray_o[p], rd = ps.ray_d[p];
+    vec3 o = xyz(ro), d = xyz(rd);
+    float4 pc = ps.pend_c[p];
+
+    bool emitter = false;
+    vec3 E, edir;  /* emission, and direction light -> previous stored vertex */
+    if (g < 0) {
+        if (S.env_light >= 0 && rd.w == RT_INF) {
+            /* src/environmentmap.cpp:73-83: background hit at t = inf,
+             * point = (unit direction, w = 0) */
+            emitter = true;
+            vec3 dn = norm(d);
+            E = env_value(S, dn);
+            edir = -dn;
+        } else {
+            return; /* miss: pending estimate stays and is committed at accumulate */
+        }
+    } else {
+        int m = S.mat[g];
+        if (m < 0) {
+            const RtLight& Lt = S.lights[-m - 1];
+            emitter = true;
+            vec3 x = o + h.x * d;
+            E = area_intensity(Lt, -d);
+            edir = xyz(ps.pend_x[p]) - x;
+        }
+    }
+    float4 al = ps.alpha[p];
+    vec3 alpha = xyz(al);
+    if (emitter) {
+        if (k == 0) {
+            /* first stored vertex emits: return its emission (:55-58) */
+            if (!is_zero(E)) ps.radiance[p] = make_float4(E.x, E.y, E.z, 0.f);
+            return;
+        }
+        /* replace the pending estimate of the previous stored vertex (:70-82) */
+        float4 px = ps.pend_x[p];
+        vec3 pn = xyz(ps.pend_n[p]), pd = xyz(ps.pend_d[p]), pa = xyz(ps.pend_a[p]);
+        const RtMaterial& pm = S.mats[__float_as_int(px.w)];
+        vec3 f = brdf_ref(pm, pn, pd, edir);
+        float in_cos = dot(-norm(edir), pn), out_cos = dot(-pd, pn);
+        float4 rad = ps.radiance[p];
+        if (sgn(in_cos) == sgn(out_cos)) {
+            vec3 c = mul3(mul3(pa, f), E) * fabsf(in_cos);
+            rad.x += c.x; rad.y += c.y; rad.z += c.z;
+        }
+        ps.radiance[p] = rad;
+        ps.pend_c[p] = make_float4(0.f, 0.f, 0.f, 0.f);
+        return;
+    }
+
+    const RtMaterial& M = S.mats[S.mat[g]];
+    vec3 x = o + h.x * d;
+    vec3 n = hit_normal(S, g, x, h.z, h.w);
+    U4 gs = rng4(P.seed, (uint32_t)info.z, (uint32_t)info.x, (uint32_t)info.y, 1u + 2u * depth);
+
+    if (M.type == RT_MAT_MIRROR) {
+        /* specular vertex: not stored; extend the path length once (:87-91) */
+        if (depth == L - 1 && L < P.max_depth) ++L;
+        vec3 out = reflect(d, n);
+        if (depth + 1 < L) {
+            ps.ray_o[p] = make_float4(x.x, x.y, x.z, RT_EPS);
+            ps.ray_d[p] = make_float4(out.x, out.y, out.z, RT_INF);
+            ps.info[p] = make_int4(info.x, info.y, info.z, pk(k, L, 0));
+            cont = true;
+        }
+        return;
+    }
+
+    /* ordinary stored vertex: the previous pending estimate is committed */
+    float4 rad = ps.radiance[p];
+    rad.x += pc.x; rad.y += pc.y; rad.z += pc.z;
+    ps.radiance[p] = rad;
+    ps.pend_c[p] = make_float4(0.f, 0.f, 0.f, 0.f);
+
+    vec3 dn = norm(d);
+    /* BSDF sample, Material::outDirection */
+    float pdf = 0.f;
+    vec3 brdf = mk(0.f, 0.f, 0.f), out = mk(0.f, 0.f, 0.f);
+    if (M.type == RT_MAT_DIFFUSE) {
+        vec3 nf = dot(n, dn) > 0.f ? -n : n;
+        out = cos_pow_dir(gs.u[0], gs.u[1], nf, 1.f, pdf);
+        brdf = ld3(M.kd) * RT_INV_PI;
+    } else if (M.type == RT_MAT_PHONG) {
+        vec3 nrm = dot(d, n) > 0.f ? -n : n;
+        vec3 spec_dir = reflect(d, nrm);
+        float u0 = gs.u[0];
+        if (u0 < M.rho_d) {
+            out = cos_pow_dir(u0 / M.rho_d, gs.u[1], nrm, 1.f, pdf);
+            brdf = ld3(M.kd) * RT_INV_PI;
+        } else if (u0 < M.rho_d + M.rho_s) {
+            out = cos_pow_dir((u0 - M.rho_d) / M.rho_s, gs.u[1], spec_dir, M.exponent, pdf);
+            if (dot(out, nrm) < 1.f) pdf = 0.f;
+            brdf = phong_spec_ref(M, n, dn, out);
+        }
+    }
+
+    /* next-event estimate for this stored vertex k (Light::getIntensity) */
+    if (S.n_lights > 0) {
+        U4 gl = rng4(P.seed, (uint32_t)info.z, (uint32_t)info.x, (uint32_t)info.y, 2u + 2u * k);
+        int li = min((int)(gl.u[2] * (float)S.n_lights), S.n_lights - 1);
+        const RtLight& Lt = S.lights[li];
+        vec3 dir, Li = mk(0.f, 0.f, 0.f);
+        bool has_ray = false;
+        if (Lt.type == RT_LIGHT_AREA) {
+            vec3 y = ld3(Lt.p) + ld3(Lt.u) * gl.u[0] + ld3(Lt.v) * gl.u[1];
+            dir = x - y;
+            float dl = len(dir);
+            so = make_float4(x.x, x.y, x.z, RT_EPS);
+            vec3 sdir = -norm(dir);
+            sd = make_float4(sdir.x, sdir.y, sdir.z, dl - RT_EPS);
+            Li = area_intensity(Lt, dir);
+            has_ray = true;
+        } else if (Lt.type == RT_LIGHT_POINT) {
+            dir = x - ld3(Lt.p);
+            float dl = len(dir);
+            so = make_float4(x.x, x.y, x.z, RT_EPS);
+            vec3 sdir = -norm(dir);
+            sd = make_float4(sdir.x, sdir.y, sdir.z, dl - RT_EPS);
+            Li = ld3(Lt.I) * (1.f / dot(dir, dir));
+            has_ray = true;
+        } else {
+            /* src/environmentmap.cpp:40-55: cosine sample about the unflipped
+             * normal, shadow ray over negative t, value / pdf looked up at
+             * the negated sample.  The backwards ray is traced forwards. */
+            float epdf;
+            vec3 s = cos_pow_dir(gl.u[0], gl.u[1], n, 1.f, epdf);
+            dir = -s;
+            so = make_float4(x.x, x.y, x.z, RT_EPS);
+            sd = make_float4(s.x, s.y, s.z, RT_INF);
+            Li = env_value(S, dir) * (1.f / epdf);
+            has_ray = true;
+        }
+        vec3 f = brdf_ref(M, n, dn, dir);
+        float in_cos = dot(-norm(dir), n), out_cos = dot(-dn, n);
+        if (has_ray && sgn(in_cos) == sgn(out_cos)) {
+            vec3 c = mul3(mul3(alpha, f), Li) * fabsf(in_cos);
+            if (!is_zero(c)) {
+                shadow = true;
+                sc = make_float4(c.x, c.y, c.z, __int_as_float(p));
+            }
+        }
+    }
+    ps.pend_x[p] = make_float4(x.x, x.y, x.z, __int_as_float(S.mat[g]));
+    ps.pend_n[p] = make_float4(n.x, n.y, n.z, 0.f);
+    ps.pend_d[p] = make_float4(dn.x, dn.y, dn.z, 0.f);
+    ps.pend_a[p] = al;
+    ++k;
+    if (pdf == 0.f || is_zero(alpha)) {
+        ps.info[p] = make_int4(info.x, info.y, info.z, pk(k, L, 0));
+        return;
+    }
+    float c = fabsf(dot(norm(out), n));
+    alpha = mul3(alpha, brdf) * (c / pdf);
+    /* src/renderer.cpp:53-61,101: a termination probability of 0 means
+     * "no roulette" and a compensation factor of 1 */
+    if (depth > P.rr_start) alpha = alpha * (1.f / (P.rr_prob > 0.f ? P.rr_prob : 1.f));
+    ps.alpha[p] = make_float4(alpha.x, alpha.y, alpha.z, 0.f);
+    ps.info[p] = make_int4(info.x, info.y, info.z, pk(k, L, 0));
+    if (depth + 1 < L) {
+        ps.ray_o[p] = make_float4(x.x, x.y, x.z, RT_EPS);
+        ps.ray_d[p] = make_float4(out.x, out.y, out.z, RT_INF);
+        cont = true;
+    }
+}
+
+/* ===========================================================================
+ * shade, unbiased estimator (next-event estimation with the light-selection
+ * weight, emission on camera / specular chains, compensated roulette).
+ * ========================================================================= */
+__device__ void shade_unbiased(const DevScene& S, const PathState& ps, const PtParams& P, int depth, int p,
+                               bool& cont, bool& shadow, float4& so, float4& sd, float4& sc)
+{
+    cont = false;
+    shadow = false;
+    int4 info = ps.info[p];
+    int spec = (info.w >> 16) & 1;
+    float4 h = ps.hit[p];
+    int g = __float_as_int(h.y);
+    float4 ro = ps.ray_o[p], rd = ps.ray_d[p];
+    vec3 o = xyz(ro), d = xyz(rd);
+    float4 al = ps.alpha[p];
+    vec3 alpha = xyz(al);
+    if (g < 0) {
+        if (S.env_light >= 0 && spec) {
+            vec3 e = mul3(alpha, env_value(S, d));
+            float4 r = ps.radiance[p];
+            ps.radiance[p] = make_float4(r.x + e.x, r.y + e.y, r.z + e.z, 0.f);
+        }
+        return;
+    }
+    int m = S.mat[g];
+    if (m < 0) {
+        const RtLight& Lt = S.lights[-m - 1];
+        if (spec && dot(d, ld3(Lt.n)) < 0.f) {
+            vec3 e = mul3(alpha, ld3(Lt.I));
+            float4 r = ps.radiance[p];
+            ps.radiance[p] = make_float4(r.x + e.x, r.y + e.y, r.z + e.z, 0.f);
+        }
+        return;
+    }
+    const RtMaterial& M = S.mats[m];
+    vec3 x = o + h.x * d;
+    vec3 n = hit_normal(S, g, x, h.z, h.w);
+    vec3 nf = dot(n, d) > 0.f ? -n : n;
+    if (M.type == RT_MAT_MIRROR) {
+        if (depth + 1 < P.max_depth) {
+            vec3 out = reflect(d, n);
+            ps.ray_o[p] = make_float4(x.x, x.y, x.z, RT_EPS);
+            ps.ray_d[p] = make_float4(out.x, out.y, out.z, RT_INF);
+            ps.info[p] = make_int4(info.x, info.y, info.z, pk(0, 0, 1));
+            cont = true;
+        }
+        return;
+    }
+    if (M.type != RT_MAT_DIFFUSE && M.type != RT_MAT_PHONG) return;
+    vec3 wo = -d;
+    U4 gb = rng4(P.seed, (uint32_t)info.z, (uint32_t)info.x, (uint32_t)info.y, 1u + 2u * depth);
+    U4 gl = rng4(P.seed, (uint32_t)info.z, (uint32_t)info.x, (uint32_t)info.y, 2u + 2u * depth);
+    if (S.n_lights > 0) {
+        int li = min((int)(gl.u[2] * (float)S.n_lights), S.n_lights - 1);
+        const RtLight& Lt = S.lights[li];
+        vec3 wi = mk(0.f, 0.f, 1.f), le = mk(0.f, 0.f, 0.f);
+        float dist = RT_INF;
+        bool ok = false;
+        if (Lt.type == RT_LIGHT_AREA) {
+            vec3 y = ld3(Lt.p) + ld3(Lt.u) * gl.u[0] + ld3(Lt.v) * gl.u[1];
+            vec3 dv = y - x;
+            float d2 = dot(dv, dv);
+            dist = sqrtf(d2);
+            wi = dv * (1.f / dist);
+            vec3 ln = ld3(Lt.n);
+            float an = sqrtf(dot(ln, ln));
+            float cl = -dot(wi, ln) / an;
+            if (cl > 0.f) {
+                le = ld3(Lt.I) * (cl * an / d2);
+                ok = true;
+            }
+        } else if (Lt.type == RT_LIGHT_POINT) {
+            vec3 dv = ld3(Lt.p) - x;
+            float d2 = dot(dv, dv);
+            dist = sqrtf(d2);
+            wi = dv * (1.f / dist);
+            le = ld3(Lt.I) * (1.f / d2);
+            ok = true;
+        } else {
+            float pdf;
+            wi = env_sample(S, gl.u[0], gl.u[1], pdf);
+            if (pdf > 0.f) {
+                le = env_value(S, wi) * (1.f / pdf);
+                ok = true;
+            }
+        }
+        float cx = dot(wi, nf);
+        if (ok && cx > 0.f) {
+            vec3 f = bsdf_eval(M, wo, wi, nf);
+            vec3 c = mul3(mul3(alpha, f), le) * (cx * (float)S.n_lights);
+            if (!is_zero(c)) {
+                shadow = true;
+                so = make_float4(x.x, x.y, x.z, RT_EPS);
+                sd = make_float4(wi.x, wi.y, wi.z, dist == RT_INF ? RT_INF : dist - RT_EPS);
+                sc = make_float4(c.x, c.y, c.z, __int_as_float(p));
+            }
+        }
+    }
+    if (depth + 1 >= P.max_depth) return;
+    vec3 wi;
+    if (!bsdf_sample(M, wo, nf, gb.u[0], gb.u[1], gb.u[2], wi)) return;
+    float pdf = bsdf_pdf(M, wo, wi, nf);
+    if (!(pdf > 0.f)) return;
+    vec3 f = bsdf_eval(M, wo, wi, nf);
+    alpha = mul3(alpha, f) * (dot(wi, nf) / pdf);
+    if (depth + 1 >= P.rr_start) {
+        if (gb.u[3] < P.rr_prob) return;
+        alpha = alpha * (1.f / (1.f - P.rr_prob));
+    }
+    ps.alpha[p] = make_float4(alpha.x, alpha.y, alpha.z, 0.f);
+    ps.ray_o[p] = make_float4(x.x, x.y, x.z, RT_EPS);
+    ps.ray_d[p] = make_float4(wi.x, wi.y, wi.z, RT_INF);
+    ps.info[p] = make_int4(info.x, info.y, info.z, pk(0, 0, 0));
+    cont = true;
+}
+
+template <int MODE>
+__global__ __launch_bounds__(PT_BLOCK) void k_shade(DevScene S, PathState ps, PtParams P, int depth, int qsel)
+{
+    const int n = ps.cnt->q_count[qsel];
+    const int32_t* q = ps.queue[qsel];
+    int32_t* qn = ps.queue[qsel ^ 1];
+    for (int base = blockIdx.x * PT_BLOCK; base < n; base += gridDim.x * PT_BLOCK) {
+        int i = base + threadIdx.x;
+        bool cont = false, shadow = false;
+        float4 so, sd, sc;
+        int p = -1;
+        if (i < n) {
+            p = q[i];
+            if (MODE == RT_MODE_COMPAT)
+                shade_compat(S, ps, P, depth, p, cont, shadow, so, sd, sc);
+            else
+                shade_unbiased(S, ps, P, depth, p, cont, shadow, so, sd, sc);
+        }
+        int at = wave_append(cont, &ps.cnt->q_count[qsel ^ 1]);
+        if (cont) qn[at] = p;
+        int as = wave_append(shadow, &ps.cnt->shadow_count);
+        if (shadow) {
+            ps.sh_o[as] = so;
+            ps.sh_d[as] = sd;
+            ps.sh_c[as] = sc;
+        }
+        if (depth > 0) wave_count(i < n, &ps.cnt->extension_rays);
+    }
+}
+
+hipError_t launch_shade(const HostDevScene& s, const PathState& ps, PtParams p, int depth, int qsel, int grid,
+                        hipStream_t st)
+{
+    if (p.mode == RT_MODE_COMPAT)
+        hipLaunchKernelGGL(k_shade<RT_MODE_COMPAT>, dim3(grid), dim3(PT_BLOCK), 0, st, as_dev(s), ps, p, depth, qsel);
+    else
+        hipLaunchKernelGGL(k_shade<RT_MODE_UNBIASED>, dim3(grid), dim3(PT_BLOCK), 0, st, as_dev(s), ps, p, depth,
+                           qsel);
+    return hipGetLastError();
+}
+
+/* ===========================================================================
+ * shadow: any-hit test of next-event rays
+ * ========================================================================= */
+__global__ __launch_bounds__(PT_BLOCK) void k_shadow(DevScene S, PathState ps, int compat)
+{
+    __shared__ int32_t stack[RT_STACK_DEPTH * PT_BLOCK];
+    int32_t* my = stack + threadIdx.x;
+    const int n = ps.cnt->shadow_count;
+    for (int base = blockIdx.x * PT_BLOCK; base < n; base += gridDim.x * PT_BLOCK) {
+        int i = base + threadIdx.x;
+        if (i < n) {
+            float4 a = ps.sh_o[i], b = ps.sh_d[i], c = ps.sh_c[i];
+            TraceOut h = traverse<true, false>(S, xyz(a), xyz(b), a.w, b.w, my, PT_BLOCK);
+            if (h.g < 0) {
+                int p = __float_as_int(c.w);
+                if (compat) {
+                    ps.pend_c[p] = make_float4(c.x, c.y, c.z, 0.f);
+                } else {
+                    float4 r = ps.radiance[p];
+                    ps.radiance[p] = make_float4(r.x + c.x, r.y + c.y, r.z + c.z, 0.f);
+                }
+            }
+        }
+        wave_count(i < n, &ps.cnt->shadow_rays);
+    }
+}
+
+hipError_t launch_shadow(const HostDevScene& s, const PathState& ps, PtParams p, int block, int grid,
+                         hipStream_t st)
+{
+    (void)block;
+    hipLaunchKernelGGL(k_shadow, dim3(grid), dim3(PT_BLOCK), 0, st, as_dev(s), ps,
+                       p.mode == RT_MODE_COMPAT ? 1 : 0);
+    return hipGetLastError();
+}
+
+__global__ void k_advance(PathState ps, int qsel)
+{
+    ps.cnt->q_count[qsel] = 0;
+    ps.cnt->shadow_count = 0;
+}
+hipError_t launch_advance(const PathState& ps, int qsel, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, st, ps, qsel);
+    return hipGetLastError();
+}
+
+/* ===========================================================================
+ * accumulate: film += (radiance + pending) / spp, samples of a pixel added in
+ * increasing sample order (src/perpixelrenderer.cpp:92) - deterministic and
+ * independent of how samples were split into passes.
+ * ========================================================================= */
+__global__ __launch_bounds__(PT_BLOCK) void k_accumulate(PathState ps, const DevView* __restrict__ views,
+                                                        const DevTile* __restrict__ tiles,
+                                                        const DevPass* __restrict__ passes)
+{
+    const DevPass pass = passes[ps.cnt->pass];
+    for (int j = blockIdx.x * PT_BLOCK + threadIdx.x; j < pass.pixels; j += gridDim.x * PT_BLOCK) {
+        int lo = pass.tile0, hi = pass.tile1;
+        while (hi - lo > 1) {
+            int mid = (lo + hi) >> 1;
+            if (tiles[mid].acc0 <= j) lo = mid;
+            else hi = mid;
+        }
+        const DevTile T = tiles[lo];
+        const DevView& V = views[T.view];
+        int local = j - T.acc0;
+        int pixel = T.pix0 + local;
+        float* f = V.film + 3 * (size_t)pixel;
+        float r = f[0], g = f[1], b = f[2];
+        for (int s = 0; s < T.ns; ++s) {
+            int slot = T.slot0 + s * T.npix + local;
+            float4 L = ps.radiance[slot], c = ps.pend_c[slot];
+            r += (L.x + c.x) * V.inv_spp;
+            g += (L.y + c.y) * V.inv_spp;
+            b += (L.z + c.z) * V.inv_spp;
+        }
+        f[0] = r;
+        f[1] = g;
+        f[2] = b;
+    }
+}
+hipError_t launch_accumulate(const PathState& ps, const DevView* views, const DevTile* tiles,
+                             const DevPass* passes, int grid, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_accumulate, dim3(grid), dim3(PT_BLOCK), 0, st, ps, views, tiles, passes);
+    return hipGetLastError();
+}
+
+__global__ void k_pass_end(PathState ps) { ps.cnt->pass += 1; }
+hipError_t launch_pass_end(const PathState& ps, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_pass_end, dim3(1), dim3(1), 0, st, ps);
+    return hipGetLastError();
+}
+
+__global__ void k_reset(PathState ps)
+{
+    DevCounters* c = ps.cnt;
+    c->q_count[0] = c->q_count[1] = 0;
+    c->shadow_count = 0;
+    c->pass = 0;
+    c->camera_rays = c->extension_rays = c->shadow_rays = 0ull;
+    c->node_visits = c->prim_tests = 0ull;
+}
+hipError_t launch_reset_counters(const PathState& ps, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_reset, dim3(1), dim3(1), 0, st, ps);
+    return hipGetLastError();
+}

@@ -1,0 +1,181 @@
+"""Host interface over librt_hip.so, named after the reference's classes.
+
+GpuContext                      one rt_ctx (device, stream, uploaded scene)
+GpuUniDiPathTracingIntegrator   the drop-in Integrator: same construction
+                                argument as UniDiPathTracingIntegrator
+                                (terminationProb, src/unidipathtracingintegrator.h:30)
+PerPixelRenderer                render(scene, spp_x, spp_y, seed) -> film, the
+                                batch form of src/perpixelrenderer.cpp:43-101
+
+All work runs in the HIP library; there is no CPU fallback.  Errors raise
+RtError carrying the library's status code.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _abi
+from .scene import Camera, Scene
+
+
+def _check(lib, ctx, status):
+    if status != 0:
+        msg = lib.rt_last_error(ctx).decode() if ctx else ""
+        raise _abi.RtError(status, msg)
+
+
+def _f32(a, shape_last=4):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if a.ndim != 2 or a.shape[1] != shape_last:
+        raise ValueError(f"expected (n, {shape_last}) float32 array")
+    return a
+
+
+class GpuContext:
+    def __init__(self, device=0):
+        self.lib = _abi.load()
+        h = ctypes.c_void_p()
+        st = self.lib.rt_create(int(device), ctypes.byref(h))
+        if st != 0:
+            raise _abi.RtError(st, "rt_create")
+        self.h = h
+        self.scene_arrays = None
+
+    def close(self):
+        if self.h:
+            self.lib.rt_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- scene ---------------------------------------------------------------
+    def upload(self, scene):
+        arrays = scene.flatten() if isinstance(scene, Scene) else scene
+        _check(self.lib, self.h, self.lib.rt_scene_upload(self.h, ctypes.byref(arrays.desc())))
+        self.scene_arrays = arrays
+        return arrays
+
+    # -- ray batches (host arrays) ---------------------------------------------
+    def trace_closest(self, o_tmin, d_tmax):
+        o, d = _f32(o_tmin), _f32(d_tmax)
+        n = len(o)
+        prim = np.empty(n, np.int32)
+        t = np.empty(n, np.float32)
+        _check(self.lib, self.h, self.lib.rt_trace_closest(self.h, o.ctypes.data, d.ctypes.data, n,
+                                                           prim.ctypes.data, t.ctypes.data))
+        return prim, t
+
+    def trace_any(self, o_tmin, d_tmax):
+        o, d = _f32(o_tmin), _f32(d_tmax)
+        n = len(o)
+        occ = np.empty(n, np.uint8)
+        _check(self.lib, self.h, self.lib.rt_trace_any(self.h, o.ctypes.data, d.ctypes.data, n, occ.ctypes.data))
+        return occ
+
+    # -- ray batches (device pointers, e.g. torch tensors) ---------------------
+    def trace_closest_device(self, o_ptr, d_ptr, n, prim_ptr, t_ptr):
+        _check(self.lib, self.h, self.lib.rt_trace_closest_device(self.h, o_ptr, d_ptr, int(n), prim_ptr, t_ptr))
+
+    def trace_any_device(self, o_ptr, d_ptr, n, occ_ptr):
+        _check(self.lib, self.h, self.lib.rt_trace_any_device(self.h, o_ptr, d_ptr, int(n), occ_ptr))
+
+    def trace_count_device(self, o_ptr, d_ptr, n, nodes_ptr, prims_ptr):
+        _check(self.lib, self.h, self.lib.rt_trace_count_device(self.h, o_ptr, d_ptr, int(n), nodes_ptr, prims_ptr))
+
+    # -- render ------------------------------------------------------------------
+    @staticmethod
+    def make_views(views):
+        """views: list of (Camera, spp_x, spp_y)."""
+        arr = (_abi.RtView * len(views))()
+        for i, (cam, sx, sy) in enumerate(views):
+            arr[i].camera = cam.to_c()
+            arr[i].spp_x, arr[i].spp_y = int(sx), int(sy)
+        return arr
+
+    @staticmethod
+    def make_params(mode=_abi.MODE_UNBIASED, max_depth=8, rr_start=3, rr_prob=0.5, seed=0):
+        p = _abi.RtParams()
+        p.mode, p.max_depth, p.rr_start, p.rr_prob, p.seed = int(mode), int(max_depth), int(rr_start), float(rr_prob), int(seed)
+        return p
+
+    def render(self, views, films=None, **params):
+        """Render views (list of (Camera, spp_x, spp_y)) into host films
+        (height, width, 3) float32, returned as a list."""
+        va = self.make_views(views)
+        if films is None:
+            films = [np.zeros((cam.height, cam.width, 3), np.float32) for cam, _, _ in views]
+        ptrs = (ctypes.c_void_p * len(films))(*[f.ctypes.data for f in films])
+        p = self.make_params(**params)
+        _check(self.lib, self.h, self.lib.rt_render(self.h, va, len(views), ctypes.byref(p), ptrs))
+        return films
+
+    def render_device(self, views, film_ptrs, **params):
+        va = self.make_views(views)
+        ptrs = (ctypes.c_void_p * len(film_ptrs))(*film_ptrs)
+        p = self.make_params(**params)
+        _check(self.lib, self.h, self.lib.rt_render_device(self.h, va, len(views), ctypes.byref(p), ptrs))
+
+    def set_launch_config(self, path_slots=0, block=0, blocks_per_cu=0, use_graph=0, count_traversal=0):
+        c = _abi.RtLaunchConfig(int(path_slots), int(block), int(blocks_per_cu), int(use_graph), int(count_traversal))
+        _check(self.lib, self.h, self.lib.rt_set_launch_config(self.h, ctypes.byref(c)))
+
+    def launch_config(self):
+        c = _abi.RtLaunchConfig()
+        _check(self.lib, self.h, self.lib.rt_get_launch_config(self.h, ctypes.byref(c)))
+        return {f: getattr(c, f) for f, _ in c._fields_}
+
+    def stats(self):
+        s = _abi.RtStats()
+        _check(self.lib, self.h, self.lib.rt_get_stats(self.h, ctypes.byref(s)))
+        return {f: getattr(s, f) for f, _ in s._fields_ if f != "reserved"}
+
+    def request_stop(self, flag=True):
+        _check(self.lib, self.h, self.lib.rt_request_stop(self.h, 1 if flag else 0))
+
+    def set_fault(self, flag=True):
+        _check(self.lib, self.h, self.lib.rt_set_fault(self.h, 1 if flag else 0))
+
+
+class GpuUniDiPathTracingIntegrator:
+    """The GPU drop-in for UniDiPathTracingIntegrator(terminationProb)."""
+
+    def __init__(self, termination_prob=0.5, mode=_abi.MODE_COMPAT, max_depth=5, rr_start=3, context=None):
+        self.termination_prob = float(termination_prob)
+        self.mode, self.max_depth, self.rr_start = int(mode), int(max_depth), int(rr_start)
+        self.ctx = context or GpuContext()
+        self._scene = None
+
+    def prepare(self, scene):
+        if scene is not self._scene:
+            self.ctx.upload(scene)
+            self._scene = scene
+
+    def render_views(self, scene, views, seed=0):
+        self.prepare(scene)
+        return self.ctx.render(views, mode=self.mode, max_depth=self.max_depth, rr_start=self.rr_start,
+                               rr_prob=self.termination_prob, seed=seed)
+
+
+class PerPixelRenderer:
+    """Batch form of PerPixelRenderer::render: one call per film, pt-x/y-samples
+    strata per pixel, --seed."""
+
+    name = "pathtracing"
+
+    def __init__(self, integrator, x_samples=4, y_samples=4, seed=0):
+        self.integrator = integrator
+        self.x_samples, self.y_samples, self.seed = int(x_samples), int(y_samples), int(seed)
+
+    def render(self, scene):
+        cam = scene.camera
+        return self.integrator.render_views(scene, [(cam, self.x_samples, self.y_samples)], seed=self.seed)[0]

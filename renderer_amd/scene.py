@@ -1,0 +1,322 @@
+"""Host-side scene model.
+
+Mirrors the reference's scene classes (Camera, DiffuseMaterial,
+MirrorMaterial, PhongMaterial, DarkMatter, AreaLight, PointLight,
+EnvironmentMap, Triangle, Quad, Sphere, IntersectableList, Scene) with the
+same constructor arguments, and flattens a scene into the pointer-free
+arrays of ``rt_scene_desc`` (include/rt_scene.h).
+
+Primitive IDs follow the reference's flattened root list
+(src/sceneast.cpp:676-678, src/intersectablelist.cpp:67-78): lights first in
+file order, then the components of the intersectable tree depth-first.  A
+Quad is one primitive made of the triangles (p1,p2,p4), (p4,p2,p3)
+(src/quad.h:32); every triangle of a Mesh is its own primitive, as the
+triangles of an OBJ file are.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _abi
+
+
+def _v3(x):
+    a = np.asarray(x, dtype=np.float32).reshape(3)
+    return a
+
+
+# ---------------------------------------------------------------------------
+# materials, src/material.h:30-50
+# ---------------------------------------------------------------------------
+class Material:
+    kind = None
+
+
+class DiffuseMaterial(Material):
+    """src/diffusematerial.h: colour, brdf = colour / pi."""
+    kind = _abi.MAT_DIFFUSE
+
+    def __init__(self, color):
+        self.color = _v3(color)
+
+
+class MirrorMaterial(Material):
+    """src/mirrormaterial.h: perfect specular reflector (the argument is unused there too)."""
+    kind = _abi.MAT_MIRROR
+
+    def __init__(self, k=1.0):
+        self.k = float(k)
+
+
+class PhongMaterial(Material):
+    """src/phongmaterial.h: diffuse colour, specular colour, exponent."""
+    kind = _abi.MAT_PHONG
+
+    def __init__(self, color, specular_color, specular_coefficient):
+        self.color = _v3(color)
+        self.specular_color = _v3(specular_color)
+        self.specular_coefficient = float(specular_coefficient)
+
+
+class DarkMatter(Material):
+    """src/darkmatter.h: absorbs everything."""
+    kind = _abi.MAT_DARK
+
+
+# ---------------------------------------------------------------------------
+# lights, src/light.h:31-37
+# ---------------------------------------------------------------------------
+class Light:
+    pass
+
+
+class AreaLight(Light):
+    """src/arealight.h: parallelogram origin + s*u + t*v, normal u x v, intensity."""
+
+    def __init__(self, origin, u_direction, v_direction, intensity):
+        self.origin, self.u, self.v, self.intensity = _v3(origin), _v3(u_direction), _v3(v_direction), _v3(intensity)
+
+
+class PointLight(Light):
+    """src/pointlight.h: location, intensity."""
+
+    def __init__(self, location, intensity):
+        self.location, self.intensity = _v3(location), _v3(intensity)
+
+
+class EnvironmentMap(Light):
+    """Environment light: a latitude-longitude RGB float image (H, W, 3), row 0
+    at +y, times a coefficient (src/environmentmap.h holds an image and a
+    coefficient; its image there is an angular light probe)."""
+
+    def __init__(self, image, coefficient=(1.0, 1.0, 1.0)):
+        img = np.ascontiguousarray(image, dtype=np.float32)
+        if img.ndim != 3 or img.shape[2] != 3:
+            raise ValueError("environment image must be (H, W, 3)")
+        self.image, self.coefficient = img, _v3(coefficient)
+
+
+# ---------------------------------------------------------------------------
+# intersectables
+# ---------------------------------------------------------------------------
+class Intersectable:
+    pass
+
+
+class Triangle(Intersectable):
+    """src/triangle.h: three points, optional vertex normals, material."""
+
+    def __init__(self, p1, p2, p3, material, normals=None):
+        self.p = np.stack([_v3(p1), _v3(p2), _v3(p3)])
+        self.normals = None if normals is None else np.asarray(normals, np.float32).reshape(3, 3)
+        self.material = material
+
+
+class Quad(Intersectable):
+    """src/quad.h: p1..p4, split into (p1,p2,p4) and (p4,p2,p3)."""
+
+    def __init__(self, p1, p2, p3, p4, material):
+        self.p = np.stack([_v3(p1), _v3(p2), _v3(p3), _v3(p4)])
+        self.material = material
+
+
+class Sphere(Intersectable):
+    """src/sphere.h: centre, radius, material."""
+
+    def __init__(self, center, radius, material):
+        self.center, self.radius, self.material = _v3(center), float(radius), material
+
+
+class Mesh(Intersectable):
+    """Indexed triangle mesh (what the reference's obj(...) produces): one
+    primitive per triangle, one material for all or one per face."""
+
+    def __init__(self, positions, faces, material, normals=None, face_materials=None):
+        self.positions = np.ascontiguousarray(positions, dtype=np.float32)
+        self.faces = np.ascontiguousarray(faces, dtype=np.int32)
+        self.normals = None if normals is None else np.ascontiguousarray(normals, dtype=np.float32)
+        self.material = material
+        self.face_materials = face_materials  # (list of materials, per-face index array) or None
+
+
+class IntersectableList(Intersectable):
+    """src/intersectablelist.h: ordered list of intersectables."""
+
+    def __init__(self, items):
+        self.items = list(items)
+
+
+class Camera:
+    """src/camera.h: eye, look-at, up, vertical fov in radians, resolution."""
+
+    def __init__(self, cop, look_at, up, fov, width, height):
+        self.cop, self.look_at, self.up = _v3(cop), _v3(look_at), _v3(up)
+        self.fov, self.width, self.height = float(fov), int(width), int(height)
+
+    def to_c(self):
+        c = _abi.RtCamera()
+        c.eye[:] = self.cop.tolist()
+        c.look_at[:] = self.look_at.tolist()
+        c.up[:] = self.up.tolist()
+        c.fov, c.width, c.height = self.fov, self.width, self.height
+        return c
+
+
+class SceneArrays:
+    """Flattened scene: numpy arrays + ctypes descriptor (keeps arrays alive)."""
+
+    def __init__(self, positions, normals, tri_vertices, tri_material, tri_prim, spheres, sphere_material,
+                 sphere_prim, materials, lights, env):
+        self.positions, self.normals = positions, normals
+        self.tri_vertices, self.tri_material, self.tri_prim = tri_vertices, tri_material, tri_prim
+        self.spheres, self.sphere_material, self.sphere_prim = spheres, sphere_material, sphere_prim
+        self.materials, self.lights, self.env = materials, lights, env
+        self._desc = None
+
+    @property
+    def n_prims(self):
+        ids = [len(self.lights) - 1]
+        if len(self.tri_prim):
+            ids.append(int(self.tri_prim.max()))
+        if len(self.sphere_prim):
+            ids.append(int(self.sphere_prim.max()))
+        return max(ids) + 1
+
+    def desc(self):
+        if self._desc is not None:
+            return self._desc
+        d = _abi.RtSceneDesc()
+
+        def ptr(a):
+            return a.ctypes.data if a is not None and a.size else None
+
+        d.n_vertices = len(self.positions)
+        d.positions = ptr(self.positions)
+        d.normals = ptr(self.normals)
+        d.n_triangles = len(self.tri_vertices)
+        d.tri_vertices, d.tri_material, d.tri_prim_id = ptr(self.tri_vertices), ptr(self.tri_material), ptr(self.tri_prim)
+        d.n_spheres = len(self.spheres)
+        d.spheres, d.sphere_material, d.sphere_prim_id = ptr(self.spheres), ptr(self.sphere_material), ptr(self.sphere_prim)
+        self._mats_c = (_abi.RtMaterial * max(1, len(self.materials)))(*self.materials)
+        self._lights_c = (_abi.RtLight * max(1, len(self.lights)))(*self.lights)
+        d.n_materials = len(self.materials)
+        d.materials = ctypes.addressof(self._mats_c) if self.materials else None
+        d.n_lights = len(self.lights)
+        d.lights = ctypes.addressof(self._lights_c) if self.lights else None
+        if self.env is not None:
+            d.env_height, d.env_width = self.env.shape[0], self.env.shape[1]
+            d.env_rgb = self.env.ctypes.data
+        self._desc = d
+        return d
+
+
+class Scene:
+    """src/scene.h: camera, light list, and the root intersectable."""
+
+    def __init__(self, camera, lights, intersectable):
+        self.camera = camera
+        self.light = list(lights)
+        self.object = intersectable
+
+    def flatten(self):
+        mats, mat_index = [], {}
+
+        def mid(m):
+            if id(m) not in mat_index:
+                c = _abi.RtMaterial()
+                c.type = m.kind
+                if isinstance(m, DiffuseMaterial):
+                    c.kd[:] = m.color.tolist()
+                elif isinstance(m, PhongMaterial):
+                    c.kd[:] = m.color.tolist()
+                    c.ks[:] = m.specular_color.tolist()
+                    c.exponent = m.specular_coefficient
+                mat_index[id(m)] = len(mats)
+                mats.append(c)
+            return mat_index[id(m)]
+
+        lights, env = [], None
+        for L in self.light:
+            c = _abi.RtLight()
+            if isinstance(L, AreaLight):
+                c.type = _abi.LIGHT_AREA
+                c.p[:], c.u[:], c.v[:] = L.origin.tolist(), L.u.tolist(), L.v.tolist()
+            elif isinstance(L, PointLight):
+                c.type = _abi.LIGHT_POINT
+                c.p[:] = L.location.tolist()
+            elif isinstance(L, EnvironmentMap):
+                if env is not None:
+                    raise ValueError("at most one environment map")
+                c.type = _abi.LIGHT_ENV
+                env = L.image
+            else:
+                raise TypeError(type(L))
+            c.intensity[:] = (L.coefficient if isinstance(L, EnvironmentMap) else L.intensity).tolist()
+            lights.append(c)
+
+        pos, nrm, tv, tm, tp = [], [], [], [], []
+        sph, sm, sp = [], [], []
+        nv = [0]
+        has_normals = [False]
+        next_id = [len(lights)]
+
+        def add_tris(P, F, N, mats_per_face, ids):
+            pos.append(P)
+            nrm.append(N)
+            if N is not None:
+                has_normals[0] = True
+            tv.append(F + nv[0])
+            tm.append(mats_per_face)
+            tp.append(ids)
+            nv[0] += len(P)
+
+        def walk(o):
+            if isinstance(o, IntersectableList):
+                for it in o.items:
+                    walk(it)
+            elif isinstance(o, Triangle):
+                add_tris(o.p, np.array([[0, 1, 2]], np.int32), o.normals, np.array([mid(o.material)], np.int32),
+                         np.array([next_id[0]], np.int32))
+                next_id[0] += 1
+            elif isinstance(o, Quad):
+                add_tris(o.p, np.array([[0, 1, 3], [3, 1, 2]], np.int32), None,
+                         np.full(2, mid(o.material), np.int32), np.full(2, next_id[0], np.int32))
+                next_id[0] += 1
+            elif isinstance(o, Sphere):
+                sph.append(np.array([*o.center.tolist(), o.radius], np.float32))
+                sm.append(mid(o.material))
+                sp.append(next_id[0])
+                next_id[0] += 1
+            elif isinstance(o, Mesh):
+                n = len(o.faces)
+                if o.face_materials is not None:
+                    mlist, fidx = o.face_materials
+                    lut = np.array([mid(m) for m in mlist], np.int32)
+                    fm = lut[np.asarray(fidx)]
+                else:
+                    fm = np.full(n, mid(o.material), np.int32)
+                add_tris(o.positions, o.faces, o.normals, fm.astype(np.int32),
+                         np.arange(next_id[0], next_id[0] + n, dtype=np.int32))
+                next_id[0] += n
+            else:
+                raise TypeError(type(o))
+
+        if self.object is not None:
+            walk(self.object)
+        if has_normals[0]:
+            nrm = [n if n is not None else _flat_vertex_normals(P, F - off)
+                   for n, P, F, off in zip(nrm, pos, tv, np.cumsum([0] + [len(p) for p in pos])[:-1])]
+        positions = np.ascontiguousarray(np.concatenate(pos) if pos else np.zeros((0, 3), np.float32), np.float32)
+        normals = np.ascontiguousarray(np.concatenate(nrm), np.float32) if has_normals[0] else None
+        tri_vertices = np.ascontiguousarray(np.concatenate(tv) if tv else np.zeros((0, 3), np.int32), np.int32)
+        tri_material = np.ascontiguousarray(np.concatenate(tm) if tm else np.zeros(0, np.int32), np.int32)
+        tri_prim = np.ascontiguousarray(np.concatenate(tp) if tp else np.zeros(0, np.int32), np.int32)
+        spheres = np.ascontiguousarray(np.stack(sph) if sph else np.zeros((0, 4), np.float32), np.float32)
+        return SceneArrays(positions, normals, tri_vertices, tri_material, tri_prim, spheres,
+                           np.asarray(sm, np.int32), np.asarray(sp, np.int32), mats, lights, env)
+
+
+def _flat_vertex_normals(P, F):
+    """Per-vertex copies of flat normals for a triangle set without normals,
+    used only when another part of the scene has vertex normals."""
+    raise ValueError("mixing meshes with and without vertex normals is not supported")
