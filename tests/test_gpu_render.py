@@ -39,6 +39,8 @@ def test_box_same_seed_pixelwise(gpu_ctx, mode):
 
 @pytest.mark.parametrize("mode", [MODE_COMPAT, MODE_UNBIASED])
 def test_box_relmse_ratio(gpu_ctx, mode):
+    """relMSE(GPU) / relMSE(oracle) <= 1.05 at equal spp and equal seeds
+    against a 1024-spp oracle reference, averaged over 4 seeds."""
     sc = scenes.box_scene(48, 48)
     arr = sc.flatten()
     orc = Oracle(arr, accel=1)
@@ -47,12 +49,39 @@ def test_box_relmse_ratio(gpu_ctx, mode):
     rg, ro = [], []
     for seed in range(4):
         img = gpu_ctx.render([(sc.camera, 4, 4)], mode=mode, max_depth=8, seed=seed)[0]
-        cpu, _ = orc.render((sc.camera, 4, 4), mode=mode, max_depth=8, seed=100 + seed)
+        cpu, _ = orc.render((sc.camera, 4, 4), mode=mode, max_depth=8, seed=seed)
         rg.append(rel_mse(img, R))
         ro.append(rel_mse(cpu, R))
     ratio = np.mean(rg) / np.mean(ro)
     print(f"mode {mode}: relMSE gpu {np.mean(rg):.5f} oracle {np.mean(ro):.5f} ratio {ratio:.4f}", flush=True)
     assert ratio <= 1.05
+
+
+@pytest.mark.parametrize("mode", [MODE_COMPAT, MODE_UNBIASED])
+def test_box_independent_streams_converge_alike(gpu_ctx, mode):
+    """With disjoint seeds the two estimators must have the same error
+    distribution: the mean relMSE over 16 seeds agrees within 4 standard
+    errors of the difference, and the mean images agree within 4 standard
+    errors per channel."""
+    sc = scenes.box_scene(32, 32)
+    arr = sc.flatten()
+    orc = Oracle(arr, accel=1)
+    gpu_ctx.upload(arr)
+    R, _ = orc.render((sc.camera, 32, 32), mode=mode, max_depth=8, seed=5000)
+    rg, ro, mg, mo = [], [], [], []
+    for seed in range(16):
+        img = gpu_ctx.render([(sc.camera, 4, 4)], mode=mode, max_depth=8, seed=seed)[0]
+        cpu, _ = orc.render((sc.camera, 4, 4), mode=mode, max_depth=8, seed=200 + seed)
+        rg.append(rel_mse(img, R))
+        ro.append(rel_mse(cpu, R))
+        mg.append(img.mean(axis=(0, 1)))
+        mo.append(cpu.mean(axis=(0, 1)))
+    rg, ro, mg, mo = map(np.asarray, (rg, ro, mg, mo))
+    se = np.sqrt(rg.var(ddof=1) / len(rg) + ro.var(ddof=1) / len(ro))
+    print(f"mode {mode}: relMSE {rg.mean():.5f} vs {ro.mean():.5f} (se {se:.5f})", flush=True)
+    assert abs(rg.mean() - ro.mean()) <= 4 * se + 1e-6
+    se_m = np.sqrt(mg.var(axis=0, ddof=1) / len(mg) + mo.var(axis=0, ddof=1) / len(mo))
+    assert np.all(np.abs(mg.mean(0) - mo.mean(0)) <= 4 * se_m + 1e-6)
 
 
 def test_mesh_scene_pixelwise(gpu_ctx):
