@@ -131,6 +131,34 @@ int rt_request_stop(rt_ctx* ctx, int32_t stop);
  * returns RT_ERR_FAULT instead. */
 int rt_set_fault(rt_ctx* ctx, int32_t fault);
 
+/* ---------------------------------------------------------------------------
+ * Learned path guiding (no counterpart in the reference, which samples
+ * directions from the BSDF only, src/renderer.cpp:80): a 3 x 64 MLP with
+ * FP16 weights emitting 8 von Mises-Fisher lobes and the probability of
+ * sampling them instead of the BSDF.  RT_MODE_GUIDED renders sample the
+ * one-sample mixture and, with rt_params.guide_train, train the network
+ * online (Adam) on the radiance their own paths carry.
+ *
+ * Parameters: a flat FP32 vector of rt_guide_param_count() values,
+ * W1[64][80] b1[64] W2[64][64] b2[64] W3[64][64] b3[64] W4[48][64] b4[48].
+ * Training records: 16 floats each - position (3), viewer-side normal (3),
+ * outgoing direction (3), sampled direction (3), BSDF pdf of it, weight,
+ * luminance of the throughput, unused.  Positions are encoded relative to
+ * the scene box of the last rt_scene_upload unless rt_guide_set_bounds set
+ * another box.
+ * ------------------------------------------------------------------------- */
+int rt_guide_param_count(void);
+int rt_guide_init(rt_ctx* ctx, uint32_t seed);
+int rt_guide_set_params(rt_ctx* ctx, const float* theta, int32_t n);
+int rt_guide_get_params(rt_ctx* ctx, float* theta, int32_t n);
+int rt_guide_set_bounds(rt_ctx* ctx, const float* lo, const float* hi);
+/* raw network outputs y[n][48] for n records (host arrays) */
+int rt_guide_forward(rt_ctx* ctx, const float* records, int32_t n, float* y);
+/* gradient of L = -sum_i w_i log p_mix_i / sum_i w_i over n records, and L */
+int rt_guide_gradient(rt_ctx* ctx, const float* records, int32_t n, float* grad, float* loss);
+/* `steps` Adam steps on n records with learning rate lr */
+int rt_guide_train(rt_ctx* ctx, const float* records, int32_t n, int32_t steps, float lr);
+
 #ifdef __cplusplus
 }
 #endif

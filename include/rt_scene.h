@@ -122,7 +122,9 @@ typedef struct rt_params {
     int32_t rr_start;      /* Russian roulette start (src/renderer.cpp:52) */
     float rr_prob;         /* termination probability (--pt-termination-prob) */
     uint32_t seed;         /* --seed */
-    int32_t reserved[3];
+    int32_t guide_train;   /* GUIDED: 1 = train the network online after every pass */
+    float guide_lr;        /* GUIDED: Adam learning rate (0 = 1e-3) */
+    int32_t guide_batch;   /* GUIDED: records per training step (0 = 262144) */
 } rt_params;
 
 #ifdef __cplusplus

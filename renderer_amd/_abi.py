@@ -52,7 +52,8 @@ class RtView(ctypes.Structure):
 
 class RtParams(ctypes.Structure):
     _fields_ = [("mode", ctypes.c_int32), ("max_depth", ctypes.c_int32), ("rr_start", ctypes.c_int32),
-                ("rr_prob", ctypes.c_float), ("seed", ctypes.c_uint32), ("reserved", ctypes.c_int32 * 3)]
+                ("rr_prob", ctypes.c_float), ("seed", ctypes.c_uint32), ("guide_train", ctypes.c_int32),
+                ("guide_lr", ctypes.c_float), ("guide_batch", ctypes.c_int32)]
 
 
 class RtStats(ctypes.Structure):
@@ -112,6 +113,14 @@ PROTOTYPES = {
     "rt_get_stats": (ctypes.c_int, [_P, ctypes.POINTER(RtStats)]),
     "rt_request_stop": (ctypes.c_int, [_P, _I32]),
     "rt_set_fault": (ctypes.c_int, [_P, _I32]),
+    "rt_guide_param_count": (ctypes.c_int, []),
+    "rt_guide_init": (ctypes.c_int, [_P, ctypes.c_uint32]),
+    "rt_guide_set_params": (ctypes.c_int, [_P, _P, _I32]),
+    "rt_guide_get_params": (ctypes.c_int, [_P, _P, _I32]),
+    "rt_guide_set_bounds": (ctypes.c_int, [_P, _P, _P]),
+    "rt_guide_forward": (ctypes.c_int, [_P, _P, _I32, _P]),
+    "rt_guide_gradient": (ctypes.c_int, [_P, _P, _I32, _P, _P]),
+    "rt_guide_train": (ctypes.c_int, [_P, _P, _I32, _I32, ctypes.c_float]),
 }
 
 _lib = None

@@ -15,7 +15,10 @@
 
 #include "../../include/rt_abi.h"
 #include "bvh_build.h"
+#include "rt_guide.h"
 #include "rt_kernels.h"
+
+#include <random>
 
 namespace {
 
@@ -49,6 +52,15 @@ struct rt_ctx {
     std::vector<DevBuf> ps_bufs;
     DevBuf counters, views, tiles, passes, films;
     rt_launch_config cfg{0, 256, 4, 0, 0};
+
+    // guiding network
+    GuideState guide{};
+    bool guide_ready = false;
+    bool guide_bounds_set = false;
+    int guide_step = 0;
+    size_t guide_slots = 0;
+    DevBuf g_theta, g_w16, g_m, g_v, g_grad, g_acc, g_count, g_rec, g_aux, g_open, g_out, g_tmp_rec, g_tmp_y;
+    float scene_lo[3] = {0, 0, 0}, scene_hi[3] = {1, 1, 1};
     int timing = 1;
     hipEvent_t ev_a = nullptr, ev_b = nullptr;
     std::vector<hipEvent_t> ev_pool;
@@ -482,6 +494,26 @@ int rt_scene_upload(rt_ctx* c, const rt_scene_desc* d)
     c->stats.bvh_nodes = s.n_nodes;
     c->stats.bvh_prims = G;
     c->stats.bvh_depth = br.depth;
+    // scene box (finite primitives) for the guiding network's position encoding
+    {
+        float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (const auto& r : refs)
+            for (int k = 0; k < 3; ++k) {
+                lo[k] = std::min(lo[k], r.lo[k]);
+                hi[k] = std::max(hi[k], r.hi[k]);
+            }
+        for (int k = 0; k < 3; ++k) {
+            if (!(lo[k] <= hi[k])) { lo[k] = 0.f; hi[k] = 1.f; }
+            float pad = 1e-3f * std::max(1e-3f, hi[k] - lo[k]);
+            c->scene_lo[k] = lo[k] - pad;
+            c->scene_hi[k] = hi[k] + pad;
+        }
+        if (!c->guide_bounds_set)
+            for (int k = 0; k < 3; ++k) {
+                c->guide.lo[k] = c->scene_lo[k];
+                c->guide.inv_ext[k] = 1.f / (c->scene_hi[k] - c->scene_lo[k]);
+            }
+    }
     return RT_OK;
 }
 
@@ -625,14 +657,216 @@ int alloc_path_state(rt_ctx* c, size_t cap)
 
 }  // namespace
 
+/* ---------------------------------------------------------------------------
+ * Guiding network state
+ * ------------------------------------------------------------------------- */
+static int guide_alloc_base(rt_ctx* c)
+{
+    int r;
+    if ((r = ensure(c, c->g_theta, GD_NPARAM * 4)) || (r = ensure(c, c->g_w16, GD_NW16 * 2)) ||
+        (r = ensure(c, c->g_m, GD_NPARAM * 4)) || (r = ensure(c, c->g_v, GD_NPARAM * 4)) ||
+        (r = ensure(c, c->g_grad, GD_NPARAM * 4)) || (r = ensure(c, c->g_acc, 16)) || (r = ensure(c, c->g_count, 16)))
+        return r;
+    GuideState& g = c->guide;
+    g.theta = (float*)c->g_theta.p;
+    g.w16 = (_Float16*)c->g_w16.p;
+    g.adam_m = (float*)c->g_m.p;
+    g.adam_v = (float*)c->g_v.p;
+    g.grad = (float*)c->g_grad.p;
+    g.acc = (float*)c->g_acc.p;
+    g.rec_count = (int32_t*)c->g_count.p;
+    return 0;
+}
+
+static int guide_alloc_slots(rt_ctx* c, size_t slots, int batch)
+{
+    int r;
+    size_t cap = (size_t)std::max(batch * 2, 1 << 16);
+    if ((r = ensure(c, c->g_rec, cap * GD_REC * 4)) || (r = ensure(c, c->g_aux, cap * 16)) ||
+        (r = ensure(c, c->g_open, slots * 4)) || (r = ensure(c, c->g_out, slots * GD_PARAMS * 4)))
+        return r;
+    GuideState& g = c->guide;
+    g.rec = (float*)c->g_rec.p;
+    g.rec_aux = (float4*)c->g_aux.p;
+    g.open_rec = (int32_t*)c->g_open.p;
+    g.out = (float*)c->g_out.p;
+    g.rec_cap = (int32_t)std::min<size_t>(cap, c->g_rec.bytes / (GD_REC * 4));
+    if (c->guide_slots < slots) {
+        HIP_TRY(c, launch_guide_reset_records(g, (int)slots, c->stream));
+        c->guide_slots = slots;
+    } else {
+        HIP_TRY(c, launch_guide_reset_records(g, (int)c->guide_slots, c->stream));
+    }
+    return 0;
+}
+
+static int guide_upload_params(rt_ctx* c, const float* theta)
+{
+    std::vector<_Float16> w16(GD_NW16);
+    auto copy = [&](int off, int w16off, int n) {
+        for (int i = 0; i < n; ++i) w16[w16off + i] = (_Float16)theta[off + i];
+    };
+    copy(GD_OFF_W1, GD_W16_W1, 64 * 80);
+    copy(GD_OFF_W2, GD_W16_W2, 64 * 64);
+    copy(GD_OFF_W3, GD_W16_W3, 64 * 64);
+    copy(GD_OFF_W4, GD_W16_W4, 48 * 64);
+    HIP_TRY(c, hipMemcpy(c->guide.theta, theta, GD_NPARAM * 4, hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(c->guide.w16, w16.data(), GD_NW16 * 2, hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemset(c->guide.adam_m, 0, GD_NPARAM * 4));
+    HIP_TRY(c, hipMemset(c->guide.adam_v, 0, GD_NPARAM * 4));
+    HIP_TRY(c, hipMemset(c->guide.grad, 0, GD_NPARAM * 4));
+    HIP_TRY(c, hipMemset(c->guide.acc, 0, 16));
+    HIP_TRY(c, hipMemset(c->guide.rec_count, 0, 16));
+    c->guide_step = 0;
+    c->guide_ready = true;
+    return 0;
+}
+
+int rt_guide_param_count(void) { return GD_NPARAM; }
+
+int rt_guide_init(rt_ctx* c, uint32_t seed)
+{
+    if (!c) return RT_ERR_INVALID;
+    if (int r = check_fault(c)) return r;
+    hipSetDevice(c->device);
+    if (int r = guide_alloc_base(c)) return r;
+    // Xavier-uniform weights, zero biases
+    std::mt19937 gen(seed);
+    std::vector<float> th(GD_NPARAM, 0.f);
+    const int dims[4][3] = {{GD_OFF_W1, 64, 80}, {GD_OFF_W2, 64, 64}, {GD_OFF_W3, 64, 64}, {GD_OFF_W4, 48, 64}};
+    for (auto& L : dims) {
+        float lim = std::sqrt(6.f / (float)(L[1] + L[2]));
+        std::uniform_real_distribution<float> U(-lim, lim);
+        for (int i = 0; i < L[1] * L[2]; ++i) th[L[0] + i] = U(gen);
+    }
+    if (!c->guide_bounds_set)
+        for (int k = 0; k < 3; ++k) {
+            c->guide.lo[k] = c->scene_lo[k];
+            c->guide.inv_ext[k] = 1.f / (c->scene_hi[k] - c->scene_lo[k]);
+        }
+    return guide_upload_params(c, th.data());
+}
+
+int rt_guide_set_params(rt_ctx* c, const float* theta, int32_t n)
+{
+    if (!c || !theta || n != GD_NPARAM) return RT_ERR_INVALID;
+    hipSetDevice(c->device);
+    if (int r = guide_alloc_base(c)) return r;
+    if (!c->guide_ready && !c->guide_bounds_set)
+        for (int k = 0; k < 3; ++k) {
+            c->guide.lo[k] = c->scene_lo[k];
+            c->guide.inv_ext[k] = 1.f / (c->scene_hi[k] - c->scene_lo[k]);
+        }
+    return guide_upload_params(c, theta);
+}
+
+int rt_guide_get_params(rt_ctx* c, float* theta, int32_t n)
+{
+    if (!c || !theta || n != GD_NPARAM) return RT_ERR_INVALID;
+    if (!c->guide_ready) return set_err(c, RT_ERR_NO_GUIDE, "no guiding network");
+    hipSetDevice(c->device);
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    HIP_TRY(c, hipMemcpy(theta, c->guide.theta, GD_NPARAM * 4, hipMemcpyDeviceToHost));
+    return RT_OK;
+}
+
+int rt_guide_set_bounds(rt_ctx* c, const float* lo, const float* hi)
+{
+    if (!c || !lo || !hi) return RT_ERR_INVALID;
+    for (int k = 0; k < 3; ++k) {
+        if (!(hi[k] > lo[k])) return set_err(c, RT_ERR_INVALID, "empty bounds");
+        c->guide.lo[k] = lo[k];
+        c->guide.inv_ext[k] = 1.f / (hi[k] - lo[k]);
+    }
+    c->guide_bounds_set = true;
+    return RT_OK;
+}
+
+static int guide_records_to_device(rt_ctx* c, const float* rec, int32_t n)
+{
+    if (int r = ensure(c, c->g_tmp_rec, (size_t)std::max(n, 1) * GD_REC * 4)) return r;
+    HIP_TRY(c, hipMemcpy(c->g_tmp_rec.p, rec, (size_t)n * GD_REC * 4, hipMemcpyHostToDevice));
+    return 0;
+}
+
+int rt_guide_forward(rt_ctx* c, const float* rec, int32_t n, float* y)
+{
+    if (!c || n < 0 || (n && (!rec || !y))) return RT_ERR_INVALID;
+    if (int r = check_fault(c)) return r;
+    if (!c->guide_ready) return set_err(c, RT_ERR_NO_GUIDE, "no guiding network");
+    if (n == 0) return RT_OK;
+    hipSetDevice(c->device);
+    int r;
+    if ((r = guide_records_to_device(c, rec, n)) || (r = ensure(c, c->g_tmp_y, (size_t)n * GD_OUT * 4))) return r;
+    int grid = std::max(1, std::min(grid_for(c), (n + 63) / 64));
+    HIP_TRY(c, launch_guide_forward(c->guide, (const float*)c->g_tmp_rec.p, n, (float*)c->g_tmp_y.p, grid, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    HIP_TRY(c, hipMemcpy(y, c->g_tmp_y.p, (size_t)n * GD_OUT * 4, hipMemcpyDeviceToHost));
+    return RT_OK;
+}
+
+/* Records given by the caller: weights are taken as given (record[13]). */
+static int guide_grad_on(rt_ctx* c, int32_t n)
+{
+    GuideState g = c->guide;
+    g.rec = (float*)c->g_tmp_rec.p;
+    g.rec_cap = n;
+    HIP_TRY(c, hipMemcpyAsync(g.rec_count, &n, 4, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemsetAsync(g.acc, 0, 16, c->stream));
+    PathState none{};
+    int grid = std::max(1, std::min(grid_for(c), (n + 255) / 256));
+    HIP_TRY(c, launch_guide_weights(g, none, n, grid, c->stream));
+    int tgrid = std::max(1, std::min(grid_for(c), (n + 63) / 64));
+    HIP_TRY(c, launch_guide_train(g, g.rec, g.rec_count, n, tgrid, c->stream));
+    return 0;
+}
+
+int rt_guide_gradient(rt_ctx* c, const float* rec, int32_t n, float* grad, float* loss)
+{
+    if (!c || n <= 0 || !rec || !grad) return RT_ERR_INVALID;
+    if (int r = check_fault(c)) return r;
+    if (!c->guide_ready) return set_err(c, RT_ERR_NO_GUIDE, "no guiding network");
+    hipSetDevice(c->device);
+    if (int r = guide_records_to_device(c, rec, n)) return r;
+    HIP_TRY(c, hipMemsetAsync(c->guide.grad, 0, GD_NPARAM * 4, c->stream));
+    if (int r = guide_grad_on(c, n)) return r;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    HIP_TRY(c, hipMemcpy(grad, c->guide.grad, GD_NPARAM * 4, hipMemcpyDeviceToHost));
+    for (int i = 0; i < GD_NPARAM; ++i) grad[i] /= (float)n;
+    float acc[4];
+    HIP_TRY(c, hipMemcpy(acc, c->guide.acc, 16, hipMemcpyDeviceToHost));
+    if (loss) *loss = acc[1] / (float)n;
+    HIP_TRY(c, hipMemset(c->guide.grad, 0, GD_NPARAM * 4));
+    return RT_OK;
+}
+
+int rt_guide_train(rt_ctx* c, const float* rec, int32_t n, int32_t steps, float lr)
+{
+    if (!c || n <= 0 || !rec || steps < 0) return RT_ERR_INVALID;
+    if (int r = check_fault(c)) return r;
+    if (!c->guide_ready) return set_err(c, RT_ERR_NO_GUIDE, "no guiding network");
+    hipSetDevice(c->device);
+    if (int r = guide_records_to_device(c, rec, n)) return r;
+    for (int s = 0; s < steps; ++s) {
+        HIP_TRY(c, hipMemsetAsync(c->guide.grad, 0, GD_NPARAM * 4, c->stream));
+        if (int r = guide_grad_on(c, n)) return r;
+        ++c->guide_step;
+        HIP_TRY(c, launch_guide_adam(c->guide, lr > 0.f ? lr : 1e-3f, c->guide_step, c->guide.rec_count, n, c->stream));
+    }
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return RT_OK;
+}
+
 static int render_impl(rt_ctx* c, const rt_view* views, int32_t nv, const rt_params* prm, float* const* films,
                        bool device_films)
 {
     if (!c || !views || nv <= 0 || !prm || !films) return RT_ERR_INVALID;
     if (int r = check_fault(c)) return r;
     if (!c->has_scene) return set_err(c, RT_ERR_NO_SCENE, "no scene uploaded");
-    if (prm->mode != RT_MODE_COMPAT && prm->mode != RT_MODE_UNBIASED)
-        return set_err(c, prm->mode == RT_MODE_GUIDED ? RT_ERR_NO_GUIDE : RT_ERR_INVALID, "unsupported mode");
+    if (prm->mode != RT_MODE_COMPAT && prm->mode != RT_MODE_UNBIASED && prm->mode != RT_MODE_GUIDED)
+        return set_err(c, RT_ERR_INVALID, "unsupported mode");
+    if (prm->mode == RT_MODE_GUIDED && !c->guide_ready)
+        return set_err(c, RT_ERR_NO_GUIDE, "guided mode needs rt_guide_init first");
     if (prm->max_depth < 1 || prm->max_depth > 64 || prm->rr_prob < 0.f || prm->rr_prob >= 1.f || prm->rr_start < 0)
         return set_err(c, RT_ERR_INVALID, "bad render parameters");
     int64_t total_samples = 0;
@@ -707,6 +941,13 @@ static int render_impl(rt_ctx* c, const rt_view* views, int32_t nv, const rt_par
 
     int r;
     if ((r = alloc_path_state(c, (size_t)P))) return r;
+    const bool guided = prm->mode == RT_MODE_GUIDED;
+    const bool train = guided && prm->guide_train != 0;
+    const int batch = prm->guide_batch > 0 ? prm->guide_batch : (1 << 18);
+    const float lr = prm->guide_lr > 0.f ? prm->guide_lr : 1e-3f;
+    if (guided) {
+        if ((r = guide_alloc_slots(c, (size_t)P, batch))) return r;
+    }
     if ((r = upload(c, c->views, dv.data(), dv.size() * sizeof(DevView))) ||
         (r = upload(c, c->tiles, tiles.data(), tiles.size() * sizeof(DevTile))) ||
         (r = upload(c, c->passes, passes.data(), passes.size() * sizeof(DevPass))))
@@ -715,7 +956,8 @@ static int render_impl(rt_ctx* c, const rt_view* views, int32_t nv, const rt_par
     const DevTile* d_tiles = (const DevTile*)c->tiles.p;
     const DevPass* d_passes = (const DevPass*)c->passes.p;
 
-    PtParams pp{prm->mode, prm->max_depth, prm->rr_start, prm->rr_prob, prm->seed};
+    PtParams pp{prm->mode, prm->max_depth, prm->rr_start, prm->rr_prob, prm->seed, train ? 1 : 0,
+                (float)std::min(1.0, 2.0 * batch / std::max(1.0, (double)P * 2.0))};
     const int grid = grid_for(c);
     const int depth = prm->max_depth;
     const bool timing = c->timing != 0;
@@ -746,13 +988,25 @@ static int render_impl(rt_ctx* c, const rt_view* views, int32_t nv, const rt_par
             if (timing) LCHK(hipEventRecord(c->ev_pool[ev++], c->stream));
             LCHK(launch_extend(c->scene, c->ps, q, c->cfg.count_traversal, grid, c->stream));
             if (timing) LCHK(hipEventRecord(c->ev_pool[ev++], c->stream));
-            LCHK(launch_shade(c->scene, c->ps, pp, dd, q, grid, c->stream));
+            if (guided) {
+                LCHK(launch_guide_infer(c->scene, c->ps, c->guide, q, grid, c->stream));
+                LCHK(launch_shade_guided(c->scene, c->ps, c->guide, pp, dd, q, grid, c->stream));
+            } else {
+                LCHK(launch_shade(c->scene, c->ps, pp, dd, q, grid, c->stream));
+            }
             if (timing) LCHK(hipEventRecord(c->ev_pool[ev++], c->stream));
             LCHK(launch_shadow(c->scene, c->ps, pp, 256, grid, c->stream));
             if (timing) LCHK(hipEventRecord(c->ev_pool[ev++], c->stream));
             LCHK(launch_advance(c->ps, q, c->stream));
         }
         LCHK(launch_accumulate(c->ps, d_views, d_tiles, d_passes, grid, c->stream));
+        if (train) {
+            ++c->guide_step;
+            LCHK(launch_guide_weights(c->guide, c->ps, batch, grid, c->stream));
+            LCHK(launch_guide_train(c->guide, c->guide.rec, c->guide.rec_count, batch, grid, c->stream));
+            LCHK(launch_guide_adam(c->guide, lr, c->guide_step, c->guide.rec_count, batch, c->stream));
+            LCHK(launch_guide_reset_records(c->guide, (int)P, c->stream));
+        }
         LCHK(launch_pass_end(c->ps, c->stream));
     }
 #undef LCHK

@@ -1,0 +1,79 @@
+/*
+ * rt_guide.h — learned path-guiding network: device state and launchers.
+ *
+ * Network: 80 input features (12-frequency encoding of the position in the
+ * scene box, shading normal, outgoing direction) -> 64 -> 64 -> 64 (ReLU)
+ * -> 48 outputs = 8 von Mises-Fisher lobes (mean direction, log
+ * concentration, mixture logit) + the logit of the probability of sampling
+ * the guide instead of the BSDF.  FP16 weights and activations, FP32
+ * accumulation (v_mfma_f32_16x16x16_f16), FP32 master weights and Adam.
+ *
+ * Flat parameter layout (FP32 master, 16,624 values):
+ *   W1[64][80] b1[64] W2[64][64] b2[64] W3[64][64] b3[64] W4[48][64] b4[48]
+ * FP16 weight copy (16,384 values): W1, W2, W3, W4 in the same order.
+ */
+#ifndef RT_GUIDE_H
+#define RT_GUIDE_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define GD_IN 80
+#define GD_H 64
+#define GD_OUT 48
+#define GD_LOBES 8
+#define GD_NPARAM 16624
+#define GD_NW16 16384
+#define GD_OFF_W1 0
+#define GD_OFF_B1 5120
+#define GD_OFF_W2 5184
+#define GD_OFF_B2 9280
+#define GD_OFF_W3 9344
+#define GD_OFF_B3 13440
+#define GD_OFF_W4 13504
+#define GD_OFF_B4 16576
+#define GD_W16_W1 0
+#define GD_W16_W2 5120
+#define GD_W16_W3 9216
+#define GD_W16_W4 13312
+#define GD_KLO -3.0f
+#define GD_KHI 10.0f
+#define GD_REC 16        /* floats per training record */
+#define GD_PARAMS 48     /* floats of head output per path */
+#define GD_WCLAMP 100.0f /* cap of a normalised sample weight */
+
+/* Training record (16 floats):
+ *   0..2 position, 3..5 viewer-side normal, 6..8 outgoing direction,
+ *   9..11 sampled direction, 12 BSDF pdf of it, 13 weight, 14 luminance of
+ *   the throughput before the bounce, 15 unused.
+ * Auxiliary record (float4): radiance of the path when the record was
+ * closed (rgb) and the path slot (int bits). */
+struct GuideState {
+    float* theta;       /* FP32 master parameters */
+    _Float16* w16;      /* FP16 weights */
+    float* adam_m;
+    float* adam_v;
+    float* grad;
+    float* acc;         /* [0] weight sum, [1] loss sum, [2] records trained */
+    int32_t* rec_count;
+    float* rec;         /* capacity x GD_REC */
+    float4* rec_aux;    /* capacity */
+    int32_t* open_rec;  /* per path slot: index of its open record or -1 */
+    float* out;         /* per path slot: GD_PARAMS head outputs */
+    int32_t rec_cap;
+    float lo[3], inv_ext[3];
+};
+
+struct GuideHost;  /* forward declaration for the ABI layer */
+
+hipError_t launch_guide_infer(const struct HostDevScene& s, const struct PathState& ps, const GuideState& g,
+                              int qsel, int grid, hipStream_t st);
+hipError_t launch_guide_forward(const GuideState& g, const float* rec, int n, float* y, int grid, hipStream_t st);
+hipError_t launch_guide_weights(const GuideState& g, const struct PathState& ps, int max_n, int grid, hipStream_t st);
+hipError_t launch_guide_train(const GuideState& g, const float* rec, const int32_t* count, int max_n,
+                              int grid, hipStream_t st);
+hipError_t launch_guide_adam(const GuideState& g, float lr, int step, const int32_t* count, int max_n,
+                             hipStream_t st);
+hipError_t launch_guide_reset_records(const GuideState& g, int slots, hipStream_t st);
+
+#endif

@@ -1,0 +1,574 @@
+/*
+ * rt_guide.hip — gfx950 kernels of the path-guiding network.
+ *
+ * Matrix products use v_mfma_f32_16x16x16_f16 in the transposed form
+ * Z[out x samples] = W[out x in] * X[in x samples]: a wave carries 16
+ * samples; lane l holds sample l % 16 and the feature quad 4 (l / 16)..+3.
+ * With that orientation the D tile of one layer (4 consecutive outputs of
+ * one sample per lane) is exactly the B operand of the next layer's K block,
+ * so activations never leave registers between layers.  Weight gradients
+ * (a reduction over samples) read activations and deltas back from LDS in
+ * [neuron][sample] order, which gives both MFMA operands as contiguous
+ * 8-byte reads.
+ */
+#include "rt_device.h"
+#include "rt_guide.h"
+#include "rt_guide_dev.h"
+#include "rt_kernels.h"
+
+#include <cstring>
+
+typedef _Float16 h4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+#define GB 256  /* threads per block: 4 waves */
+
+static inline DevScene as_dev_g(const HostDevScene& s)
+{
+    DevScene d;
+    std::memcpy(&d, &s, sizeof(d));
+    return d;
+}
+
+__device__ __forceinline__ f4 mfma16(h4 a, h4 b, f4 c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0);
+}
+
+struct WLds {
+    _Float16 w[GD_NW16];
+    float b[3 * GD_H + GD_OUT];
+};
+
+__device__ __forceinline__ void load_weights(WLds& L, const GuideState& g)
+{
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(g.w16);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(L.w);
+    for (int i = threadIdx.x; i < GD_NW16 / 2; i += blockDim.x) dst[i] = src[i];
+    for (int i = threadIdx.x; i < 3 * GD_H + GD_OUT; i += blockDim.x) {
+        int off = i < 64 ? GD_OFF_B1 + i : i < 128 ? GD_OFF_B2 + i - 64 : i < 192 ? GD_OFF_B3 + i - 128 : GD_OFF_B4 + i - 192;
+        L.b[i] = g.theta[off];
+    }
+    __syncthreads();
+}
+
+/* Input feature block kb (features 16 kb + 4 q .. +3) of one sample. */
+__device__ __forceinline__ h4 feature_block(int kb, int q, const float* p01, vec3 n, vec3 wo)
+{
+    h4 r;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        int f = 16 * kb + 4 * q + i;
+        float v;
+        if (f < 72) {
+            int d = f / 24, rr = f % 24, k = rr >> 1;
+            float a = ldexpf(p01[d], k);
+            v = (rr & 1) ? cospif(a) : sinpif(a);
+        } else if (f < 75) {
+            v = f == 72 ? n.x : (f == 73 ? n.y : n.z);
+        } else if (f < 78) {
+            v = f == 75 ? wo.x : (f == 76 ? wo.y : wo.z);
+        } else {
+            v = 0.f;
+        }
+        r[i] = (_Float16)v;
+    }
+    return r;
+}
+
+/* out[ot] = W[16 ot .. +15][:] * in (+ bias), W row-major [OT*16][KB*16]. */
+template <int KB, int OT>
+__device__ __forceinline__ void dense(const _Float16* W, const float* b, const h4* in, f4* out)
+{
+    const int lane = __lane_id(), m = lane & 15, q = lane >> 4;
+#pragma unroll
+    for (int ot = 0; ot < OT; ++ot) {
+        f4 acc = {b[16 * ot + 4 * q], b[16 * ot + 4 * q + 1], b[16 * ot + 4 * q + 2], b[16 * ot + 4 * q + 3]};
+#pragma unroll
+        for (int kb = 0; kb < KB; ++kb) {
+            h4 a = *reinterpret_cast<const h4*>(W + (16 * ot + m) * (KB * 16) + 16 * kb + 4 * q);
+            acc = mfma16(a, in[kb], acc);
+        }
+        out[ot] = acc;
+    }
+}
+
+/* out[it] = W^T[16 it .. +15][:] * in, W row-major [KB*16][IT*16]
+ * (back-propagation through a layer: input-neuron rows). */
+template <int KB, int IT>
+__device__ __forceinline__ void dense_t(const _Float16* W, const h4* in, f4* out)
+{
+    const int lane = __lane_id(), m = lane & 15, q = lane >> 4;
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+        f4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kb = 0; kb < KB; ++kb) {
+            h4 a;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) a[i] = W[(16 * kb + 4 * q + i) * (IT * 16) + 16 * it + m];
+            acc = mfma16(a, in[kb], acc);
+        }
+        out[it] = acc;
+    }
+}
+
+template <int T>
+__device__ __forceinline__ void relu_h(const f4* z, h4* h)
+{
+#pragma unroll
+    for (int t = 0; t < T; ++t)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) h[t][i] = (_Float16)fmaxf(z[t][i], 0.f);
+}
+
+/* Forward pass of 16 samples held by one wave.  Returns y tiles (3 x f4). */
+__device__ __forceinline__ void forward16(const WLds& L, const h4* x, h4* h1, h4* h2, h4* h3, f4* y)
+{
+    f4 z[4];
+    dense<5, 4>(L.w + GD_W16_W1, L.b, x, z);
+    relu_h<4>(z, h1);
+    dense<4, 4>(L.w + GD_W16_W2, L.b + 64, h1, z);
+    relu_h<4>(z, h2);
+    dense<4, 4>(L.w + GD_W16_W3, L.b + 128, h2, z);
+    relu_h<4>(z, h3);
+    dense<4, 3>(L.w + GD_W16_W4, L.b + 192, h3, y);
+}
+
+/* Head of one lobe from its five raw outputs. */
+__device__ __forceinline__ void lobe_head(const float* r, vec3& mu, float& mlen, float& kappa, bool& kin)
+{
+    vec3 m = mk(r[0], r[1], r[2]);
+    mlen = fmaxf(sqrtf(dot(m, m)), 1e-6f);
+    mu = m * (1.f / mlen);
+    kin = r[3] > GD_KLO && r[3] < GD_KHI;
+    kappa = expf(fminf(fmaxf(r[3], GD_KLO), GD_KHI));
+}
+
+__device__ __forceinline__ float quad_max(float v)
+{
+    v = fmaxf(v, __shfl_xor(v, 16));
+    return fmaxf(v, __shfl_xor(v, 32));
+}
+__device__ __forceinline__ float quad_sum(float v)
+{
+    v += __shfl_xor(v, 16);
+    return v + __shfl_xor(v, 32);
+}
+
+/* ===========================================================================
+ * inference: head outputs for every live path at a surface hit
+ * ========================================================================= */
+__global__ __launch_bounds__(GB) void k_guide_infer(DevScene S, PathState ps, GuideState g, int qsel)
+{
+    __shared__ WLds L;
+    __shared__ float Y[4][GD_OUT][16];
+    load_weights(L, g);
+    const int lane = __lane_id(), n = lane & 15, q = lane >> 4, w = threadIdx.x >> 6;
+    const int cnt = ps.cnt->q_count[qsel];
+    for (int base = blockIdx.x * 64; base < cnt; base += gridDim.x * 64) {
+        int i = base + 16 * w + n;
+        bool valid = i < cnt;
+        int p = valid ? ps.queue[qsel][i] : 0;
+        float p01[3] = {0.f, 0.f, 0.f};
+        vec3 nf = mk(0.f, 0.f, 0.f), wo = mk(0.f, 0.f, 0.f);
+        if (valid) {
+            float4 h = ps.hit[p];
+            int gg = __float_as_int(h.y);
+            if (gg >= 0) {
+                vec3 o = xyz(ps.ray_o[p]), d = xyz(ps.ray_d[p]);
+                vec3 x = o + h.x * d;
+                vec3 nn = hit_normal(S, gg, x, h.z, h.w);
+                nf = dot(nn, d) > 0.f ? -nn : nn;
+                wo = -d;
+                p01[0] = (x.x - g.lo[0]) * g.inv_ext[0];
+                p01[1] = (x.y - g.lo[1]) * g.inv_ext[1];
+                p01[2] = (x.z - g.lo[2]) * g.inv_ext[2];
+            } else {
+                valid = false;
+            }
+        }
+        h4 x[5], h1[4], h2[4], h3[4];
+        f4 y[3];
+#pragma unroll
+        for (int kb = 0; kb < 5; ++kb) x[kb] = feature_block(kb, q, p01, nf, wo);
+        forward16(L, x, h1, h2, h3, y);
+#pragma unroll
+        for (int t = 0; t < 3; ++t)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) Y[w][16 * t + 4 * q + r][n] = y[t][r];
+        __syncthreads();
+        float a[2];
+        vec3 mu[2];
+        float kap[2];
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            int j = 2 * q + s;
+            float r[5];
+#pragma unroll
+            for (int c = 0; c < 5; ++c) r[c] = Y[w][5 * j + c][n];
+            float mlen;
+            bool kin;
+            lobe_head(r, mu[s], mlen, kap[s], kin);
+            a[s] = r[4];
+        }
+        float mx = quad_max(fmaxf(a[0], a[1]));
+        float e0 = expf(a[0] - mx), e1 = expf(a[1] - mx);
+        float inv = 1.f / quad_sum(e0 + e1);
+        if (valid) {
+            float* o = g.out + (size_t)p * GD_PARAMS;
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                int j = 2 * q + s;
+                o[5 * j] = mu[s].x;
+                o[5 * j + 1] = mu[s].y;
+                o[5 * j + 2] = mu[s].z;
+                o[5 * j + 3] = kap[s];
+                o[5 * j + 4] = (s ? e1 : e0) * inv;
+            }
+            if (q == 0) o[40] = 0.05f + 0.9f / (1.f + expf(-Y[w][40][n]));
+        }
+        __syncthreads();
+    }
+}
+
+hipError_t launch_guide_infer(const HostDevScene& s, const PathState& ps, const GuideState& g, int qsel, int grid,
+                              hipStream_t st)
+{
+    hipLaunchKernelGGL(k_guide_infer, dim3(grid), dim3(GB), 0, st, as_dev_g(s), ps, g, qsel);
+    return hipGetLastError();
+}
+
+/* ===========================================================================
+ * forward only on records (parity surface): raw outputs y [n x 48]
+ * ========================================================================= */
+__device__ __forceinline__ void record_inputs(const GuideState& g, const float* r, float* p01, vec3& nf, vec3& wo)
+{
+    p01[0] = (r[0] - g.lo[0]) * g.inv_ext[0];
+    p01[1] = (r[1] - g.lo[1]) * g.inv_ext[1];
+    p01[2] = (r[2] - g.lo[2]) * g.inv_ext[2];
+    nf = mk(r[3], r[4], r[5]);
+    wo = mk(r[6], r[7], r[8]);
+}
+
+__global__ __launch_bounds__(GB) void k_guide_forward(GuideState g, const float* rec, int N, float* yout)
+{
+    __shared__ WLds L;
+    load_weights(L, g);
+    const int lane = __lane_id(), n = lane & 15, q = lane >> 4, w = threadIdx.x >> 6;
+    for (int base = blockIdx.x * 64; base < N; base += gridDim.x * 64) {
+        int i = base + 16 * w + n;
+        bool valid = i < N;
+        float p01[3] = {0.f, 0.f, 0.f};
+        vec3 nf = mk(0.f, 0.f, 0.f), wo = nf;
+        if (valid) record_inputs(g, rec + (size_t)i * GD_REC, p01, nf, wo);
+        h4 x[5], h1[4], h2[4], h3[4];
+        f4 y[3];
+#pragma unroll
+        for (int kb = 0; kb < 5; ++kb) x[kb] = feature_block(kb, q, p01, nf, wo);
+        forward16(L, x, h1, h2, h3, y);
+        if (valid)
+#pragma unroll
+            for (int t = 0; t < 3; ++t)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) yout[(size_t)i * GD_OUT + 16 * t + 4 * q + r] = y[t][r];
+    }
+}
+
+hipError_t launch_guide_forward(const GuideState& g, const float* rec, int n, float* y, int grid, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_guide_forward, dim3(grid), dim3(GB), 0, st, g, rec, n, y);
+    return hipGetLastError();
+}
+
+/* ===========================================================================
+ * record weights: w = lum(radiance_final - radiance_at_close) / lum(alpha),
+ * and the weight sum of the records a training step will use
+ * ========================================================================= */
+__global__ __launch_bounds__(GB) void k_guide_weights(GuideState g, PathState ps, int from_paths, int max_n)
+{
+    int N = min(*g.rec_count, g.rec_cap);
+    N = min(N, max_n);
+    float local = 0.f;
+    for (int i = blockIdx.x * GB + threadIdx.x; i < N; i += gridDim.x * GB) {
+        float* r = g.rec + (size_t)i * GD_REC;
+        float w;
+        if (from_paths) {
+            float4 aux = g.rec_aux[i];
+            int p = __float_as_int(aux.w);
+            float4 L = ps.radiance[p];
+            vec3 inc = mk(L.x - aux.x, L.y - aux.y, L.z - aux.z);
+            w = fmaxf(lum(inc), 0.f) / fmaxf(r[14], 1e-8f);
+            if (!(w < 3.0e38f)) w = 0.f;
+            r[13] = w;
+        } else {
+            w = r[13];
+        }
+        local += w;
+    }
+    for (int o = 32; o > 0; o >>= 1) local += __shfl_xor(local, o);
+    if (__lane_id() == 0 && local != 0.f) atomicAdd(&g.acc[0], local);
+}
+
+hipError_t launch_guide_weights(const GuideState& g, const PathState& ps, int max_n, int grid, hipStream_t st)
+{
+    /* from_paths = 1 when the records come from a render (radiance known) */
+    hipLaunchKernelGGL(k_guide_weights, dim3(grid), dim3(GB), 0, st, g, ps, ps.radiance != nullptr ? 1 : 0,
+                       max_n);
+    return hipGetLastError();
+}
+
+/* ===========================================================================
+ * training: loss -sum s_i log p_mix_i with s_i = w_i N / sum w (capped),
+ * gradients summed over the records into g.grad (Adam divides by N)
+ * ========================================================================= */
+struct TrainLds {
+    WLds wl;
+    float Y[4][GD_OUT][16];
+    _Float16 act[GD_IN + 3 * GD_H][64];   /* x, h1, h2, h3 of the block's 64 samples */
+    _Float16 dz[3 * GD_H + GD_OUT][64];   /* dz1, dz2, dz3, dy */
+};
+
+/* tile t of the block's 64 weight-gradient tiles -> (layer, out tile, in tile) */
+__device__ __forceinline__ void grad_tile(int t, int& layer, int& ot, int& it)
+{
+    if (t < 20) { layer = 0; ot = t / 5; it = t % 5; }
+    else if (t < 36) { layer = 1; ot = (t - 20) / 4; it = (t - 20) % 4; }
+    else if (t < 52) { layer = 2; ot = (t - 36) / 4; it = (t - 36) % 4; }
+    else { layer = 3; ot = (t - 52) / 4; it = (t - 52) % 4; }
+}
+
+__global__ __launch_bounds__(GB) void k_guide_train(GuideState g, const float* rec, const int32_t* count, int max_n)
+{
+    __shared__ TrainLds S;
+    load_weights(S.wl, g);
+    const int lane = __lane_id(), n = lane & 15, q = lane >> 4, w = threadIdx.x >> 6;
+    const int N = min(min(*count, max_n), g.rec_cap);
+    const float wsum = g.acc[0];
+    const float wscale = wsum > 0.f ? (float)N / wsum : 0.f;
+    f4 gacc[16];
+#pragma unroll
+    for (int t = 0; t < 16; ++t) gacc[t] = f4{0.f, 0.f, 0.f, 0.f};
+    float bacc = 0.f, lacc = 0.f;
+    const int col = 16 * w + n;
+    for (int base = blockIdx.x * 64; base < N; base += gridDim.x * 64) {
+        int i = base + col;
+        bool valid = i < N;
+        const float* r = rec + (size_t)(valid ? i : 0) * GD_REC;
+        float p01[3] = {0.f, 0.f, 0.f};
+        vec3 nf = mk(0.f, 0.f, 0.f), wo = nf, om = mk(0.f, 0.f, 1.f);
+        float pb = 1.f, sw = 0.f;
+        if (valid) {
+            record_inputs(g, r, p01, nf, wo);
+            om = mk(r[9], r[10], r[11]);
+            pb = r[12];
+            sw = fminf(r[13] * wscale, GD_WCLAMP);
+        }
+        h4 x[5], h1[4], h2[4], h3[4];
+        f4 y[3];
+#pragma unroll
+        for (int kb = 0; kb < 5; ++kb) x[kb] = feature_block(kb, q, p01, nf, wo);
+        forward16(S.wl, x, h1, h2, h3, y);
+        /* activations to LDS, [neuron][sample] */
+#pragma unroll
+        for (int kb = 0; kb < 5; ++kb)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) S.act[16 * kb + 4 * q + e][col] = x[kb][e];
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                S.act[GD_IN + 16 * t + 4 * q + e][col] = h1[t][e];
+                S.act[GD_IN + 64 + 16 * t + 4 * q + e][col] = h2[t][e];
+                S.act[GD_IN + 128 + 16 * t + 4 * q + e][col] = h3[t][e];
+            }
+#pragma unroll
+        for (int t = 0; t < 3; ++t)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) S.Y[w][16 * t + 4 * q + e][n] = y[t][e];
+        __syncthreads();
+
+        /* head: loss and dL/dy of lobes 2q, 2q+1 (and alpha on q == 0) */
+        float rr[2][5];
+        vec3 mu[2];
+        float mlen[2], kap[2], v[2];
+        bool kin[2];
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            int j = 2 * q + s;
+#pragma unroll
+            for (int c = 0; c < 5; ++c) rr[s][c] = S.Y[w][5 * j + c][n];
+            lobe_head(rr[s], mu[s], mlen[s], kap[s], kin[s]);
+            v[s] = vmf_pdf(mu[s], kap[s], om);
+        }
+        float mx = quad_max(fmaxf(rr[0][4], rr[1][4]));
+        float e0 = expf(rr[0][4] - mx), e1 = expf(rr[1][4] - mx);
+        float inv = 1.f / quad_sum(e0 + e1);
+        float pi0 = e0 * inv, pi1 = e1 * inv;
+        float qd = quad_sum(pi0 * v[0] + pi1 * v[1]);
+        float sg = 1.f / (1.f + expf(-S.Y[w][40][n]));
+        float alpha = 0.05f + 0.9f * sg;
+        float pm = alpha * qd + (1.f - alpha) * pb;
+        float c = valid ? -sw / pm : 0.f;
+        float dq = c * alpha;
+        if (valid && q == 0) lacc += -sw * logf(pm);
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            int j = 2 * q + s;
+            float pij = s ? pi1 : pi0;
+            float dv = dq * pij;
+            float cw = dot(mu[s], om);
+            float em = expf(-2.f * kap[s]);
+            float dlogc = 1.f - 2.f * kap[s] * em / (-expm1f(-2.f * kap[s]));
+            float dk = kin[s] ? dv * v[s] * (dlogc + kap[s] * (cw - 1.f)) : 0.f;
+            float gm = dv * v[s] * kap[s] / mlen[s];
+            vec3 dm = (om - mu[s] * cw) * gm;
+            float da = dq * pij * (v[s] - qd);
+            S.dz[3 * GD_H + 5 * j][col] = (_Float16)dm.x;
+            S.dz[3 * GD_H + 5 * j + 1][col] = (_Float16)dm.y;
+            S.dz[3 * GD_H + 5 * j + 2][col] = (_Float16)dm.z;
+            S.dz[3 * GD_H + 5 * j + 3][col] = (_Float16)dk;
+            S.dz[3 * GD_H + 5 * j + 4][col] = (_Float16)da;
+        }
+        if (q == 0) S.dz[3 * GD_H + 40][col] = (_Float16)(c * (qd - pb) * 0.9f * sg * (1.f - sg));
+        if (q == 1)
+            for (int o = 41; o < 48; ++o) S.dz[3 * GD_H + o][col] = (_Float16)0.f;
+        __syncthreads();
+
+        /* back-propagate: dy -> dz3 -> dz2 -> dz1 (B-operand layout in registers) */
+        h4 dy[3], d3[4], d2[4], d1[4];
+        f4 z[4];
+#pragma unroll
+        for (int t = 0; t < 3; ++t)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) dy[t][e] = S.dz[3 * GD_H + 16 * t + 4 * q + e][col];
+        dense_t<3, 4>(S.wl.w + GD_W16_W4, dy, z);
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                d3[t][e] = h3[t][e] > (_Float16)0.f ? (_Float16)z[t][e] : (_Float16)0.f;
+                S.dz[2 * GD_H + 16 * t + 4 * q + e][col] = d3[t][e];
+            }
+        dense_t<4, 4>(S.wl.w + GD_W16_W3, d3, z);
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                d2[t][e] = h2[t][e] > (_Float16)0.f ? (_Float16)z[t][e] : (_Float16)0.f;
+                S.dz[GD_H + 16 * t + 4 * q + e][col] = d2[t][e];
+            }
+        dense_t<4, 4>(S.wl.w + GD_W16_W2, d2, z);
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                d1[t][e] = h1[t][e] > (_Float16)0.f ? (_Float16)z[t][e] : (_Float16)0.f;
+                S.dz[16 * t + 4 * q + e][col] = d1[t][e];
+            }
+        __syncthreads();
+
+        /* weight gradients: wave w owns tiles 16 w .. 16 w + 15, K = 64 samples */
+#pragma unroll
+        for (int tt = 0; tt < 16; ++tt) {
+            int layer, ot, it;
+            grad_tile(16 * w + tt, layer, ot, it);
+            int zrow = layer == 3 ? 3 * GD_H : layer * GD_H;   /* delta rows */
+            int arow = layer == 0 ? 0 : GD_IN + (layer - 1) * GD_H; /* input rows */
+#pragma unroll
+            for (int ks = 0; ks < 4; ++ks) {
+                h4 a = *reinterpret_cast<const h4*>(&S.dz[zrow + 16 * ot + n][16 * ks + 4 * q]);
+                h4 b = *reinterpret_cast<const h4*>(&S.act[arow + 16 * it + n][16 * ks + 4 * q]);
+                gacc[tt] = mfma16(a, b, gacc[tt]);
+            }
+        }
+        /* bias gradients: thread t < 240 sums one delta row */
+        if (threadIdx.x < 3 * GD_H + GD_OUT) {
+            int row = threadIdx.x;  /* dz1 0..63, dz2 64..127, dz3 128..191, dy 192..239 */
+            float s = 0.f;
+            for (int k = 0; k < 64; ++k) s += (float)S.dz[row][k];
+            bacc += s;
+        }
+        __syncthreads();
+    }
+    /* flush */
+#pragma unroll
+    for (int tt = 0; tt < 16; ++tt) {
+        int layer, ot, it;
+        grad_tile(16 * w + tt, layer, ot, it);
+        int in_dim = layer == 0 ? GD_IN : GD_H;
+        int off = layer == 0 ? GD_OFF_W1 : layer == 1 ? GD_OFF_W2 : layer == 2 ? GD_OFF_W3 : GD_OFF_W4;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            int o = 16 * ot + 4 * q + e, ii = 16 * it + n;
+            float val = gacc[tt][e];
+            if (val != 0.f) atomicAdd(&g.grad[off + o * in_dim + ii], val);
+        }
+    }
+    if (threadIdx.x < 3 * GD_H + GD_OUT && bacc != 0.f) {
+        int row = threadIdx.x;
+        int off = row < 64 ? GD_OFF_B1 + row : row < 128 ? GD_OFF_B2 + row - 64 : row < 192 ? GD_OFF_B3 + row - 128 : GD_OFF_B4 + row - 192;
+        atomicAdd(&g.grad[off], bacc);
+    }
+    for (int o = 32; o > 0; o >>= 1) lacc += __shfl_xor(lacc, o);
+    if (lane == 0 && lacc != 0.f) atomicAdd(&g.acc[1], lacc);
+}
+
+hipError_t launch_guide_train(const GuideState& g, const float* rec, const int32_t* count, int max_n, int grid,
+                              hipStream_t st)
+{
+    hipLaunchKernelGGL(k_guide_train, dim3(grid), dim3(GB), 0, st, g, rec, count, max_n);
+    return hipGetLastError();
+}
+
+/* ===========================================================================
+ * Adam (Kingma & Ba, 2015) on the FP32 master copy, refresh FP16 weights,
+ * clear gradients and accumulators
+ * ========================================================================= */
+__global__ __launch_bounds__(GB) void k_guide_adam(GuideState g, float lr, int step, const int32_t* count, int max_n)
+{
+    const int N = min(min(*count, max_n), g.rec_cap);
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    const float c1 = 1.f - powf(b1, (float)step), c2 = 1.f - powf(b2, (float)step);
+    for (int i = blockIdx.x * GB + threadIdx.x; i < GD_NPARAM; i += gridDim.x * GB) {
+        float gr = g.grad[i];
+        g.grad[i] = 0.f;
+        if (N <= 0) continue;
+        gr /= (float)N;
+        float m = b1 * g.adam_m[i] + (1.f - b1) * gr;
+        float v = b2 * g.adam_v[i] + (1.f - b2) * gr * gr;
+        g.adam_m[i] = m;
+        g.adam_v[i] = v;
+        float th = g.theta[i] - lr * (m / c1) / (sqrtf(v / c2) + eps);
+        g.theta[i] = th;
+        int w16 = -1;
+        if (i < GD_OFF_B1) w16 = GD_W16_W1 + i;
+        else if (i >= GD_OFF_W2 && i < GD_OFF_B2) w16 = GD_W16_W2 + i - GD_OFF_W2;
+        else if (i >= GD_OFF_W3 && i < GD_OFF_B3) w16 = GD_W16_W3 + i - GD_OFF_W3;
+        else if (i >= GD_OFF_W4 && i < GD_OFF_B4) w16 = GD_W16_W4 + i - GD_OFF_W4;
+        if (w16 >= 0) g.w16[w16] = (_Float16)th;
+    }
+}
+
+hipError_t launch_guide_adam(const GuideState& g, float lr, int step, const int32_t* count, int max_n, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_guide_adam, dim3(32), dim3(GB), 0, st, g, lr, step, count, max_n);
+    return hipGetLastError();
+}
+
+__global__ void k_guide_reset(GuideState g, int slots)
+{
+    int i = blockIdx.x * GB + threadIdx.x;
+    if (i == 0) {
+        *g.rec_count = 0;
+        g.acc[0] = g.acc[1] = g.acc[2] = 0.f;
+    }
+    for (; i < slots; i += gridDim.x * GB) g.open_rec[i] = -1;
+}
+
+hipError_t launch_guide_reset_records(const GuideState& g, int slots, hipStream_t st)
+{
+    int grid = std::max(1, std::min(1024, (slots + GB - 1) / GB));
+    hipLaunchKernelGGL(k_guide_reset, dim3(grid), dim3(GB), 0, st, g, slots);
+    return hipGetLastError();
+}

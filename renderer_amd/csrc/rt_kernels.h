@@ -89,7 +89,13 @@ struct PtParams {
     int32_t mode, max_depth, rr_start;
     float rr_prob;
     uint32_t seed;
+    int32_t guide_train;  /* record training samples */
+    float rec_prob;       /* probability of keeping a training record */
 };
+
+struct GuideState;
+hipError_t launch_shade_guided(const HostDevScene& s, const PathState& ps, const GuideState& g, PtParams p,
+                               int depth, int qsel, int grid, hipStream_t st);
 
 /* Batch traversal kernels (parity surface and benchmarks). */
 hipError_t launch_trace_closest(const HostDevScene& s, const float4* o, const float4* d, int64_t n, int32_t* prim,

@@ -16,6 +16,8 @@
  * with a wave-wide ballot + mbcnt prefix and one atomic per wave.
  */
 #include "rt_device.h"
+#include "rt_guide.h"
+#include "rt_guide_dev.h"
 #include "rt_kernels.h"
 
 #include <cstring>
@@ -678,6 +680,215 @@ __device__ void shade_unbiased(const DevScene& S, const PathState& ps, const PtP
     ps.ray_d[p] = make_float4(wi.x, wi.y, wi.z, RT_INF);
     ps.info[p] = make_int4(info.x, info.y, info.z, pk(0, 0, 0));
     cont = true;
+}
+
+/* ===========================================================================
+ * shade, guided estimator: the unbiased estimator with the BSDF sample
+ * replaced by a one-sample mixture of the learned vMF proposal (probability
+ * alpha) and the BSDF (1 - alpha), weighted by the mixture pdf
+ * alpha q(w) + (1 - alpha) p_bsdf(w), so the estimator stays unbiased for
+ * any network output.  Optionally records (position, normal, wo, w,
+ * p_bsdf(w), throughput) for online training; the record is "closed" at the
+ * next shade of the path, when the radiance gathered so far (including this
+ * vertex's next-event estimate) is known.
+ * ========================================================================= */
+struct GuideRecord {
+    vec3 x, nf, wo, wi;
+    float pb, la;
+};
+
+__device__ void shade_guided(const DevScene& S, const PathState& ps, const GuideState& G, const PtParams& P,
+                             int depth, int p, bool& cont, bool& shadow, float4& so, float4& sd, float4& sc,
+                             bool& want_rec, GuideRecord& rec)
+{
+    cont = false;
+    shadow = false;
+    want_rec = false;
+    if (P.guide_train) {
+        int open = G.open_rec[p];
+        if (open >= 0) {
+            float4 r = ps.radiance[p];
+            float4 a = G.rec_aux[open];
+            G.rec_aux[open] = make_float4(r.x, r.y, r.z, a.w);
+            G.open_rec[p] = -1;
+        }
+    }
+    int4 info = ps.info[p];
+    int spec = (info.w >> 16) & 1;
+    float4 h = ps.hit[p];
+    int g = __float_as_int(h.y);
+    float4 ro = ps.ray_o[p], rd = ps.ray_d[p];
+    vec3 o = xyz(ro), d = xyz(rd);
+    float4 al = ps.alpha[p];
+    vec3 alpha = xyz(al);
+    if (g < 0) {
+        if (S.env_light >= 0 && spec) {
+            vec3 e = mul3(alpha, env_value(S, d));
+            float4 r = ps.radiance[p];
+            ps.radiance[p] = make_float4(r.x + e.x, r.y + e.y, r.z + e.z, 0.f);
+        }
+        return;
+    }
+    int m = S.mat[g];
+    if (m < 0) {
+        const RtLight& Lt = S.lights[-m - 1];
+        if (spec && dot(d, ld3(Lt.n)) < 0.f) {
+            vec3 e = mul3(alpha, ld3(Lt.I));
+            float4 r = ps.radiance[p];
+            ps.radiance[p] = make_float4(r.x + e.x, r.y + e.y, r.z + e.z, 0.f);
+        }
+        return;
+    }
+    const RtMaterial& M = S.mats[m];
+    vec3 x = o + h.x * d;
+    vec3 n = hit_normal(S, g, x, h.z, h.w);
+    vec3 nf = dot(n, d) > 0.f ? -n : n;
+    if (M.type == RT_MAT_MIRROR) {
+        if (depth + 1 < P.max_depth) {
+            vec3 out = reflect(d, n);
+            ps.ray_o[p] = make_float4(x.x, x.y, x.z, RT_EPS);
+            ps.ray_d[p] = make_float4(out.x, out.y, out.z, RT_INF);
+            ps.info[p] = make_int4(info.x, info.y, info.z, pk(0, 0, 1));
+            cont = true;
+        }
+        return;
+    }
+    if (M.type != RT_MAT_DIFFUSE && M.type != RT_MAT_PHONG) return;
+    vec3 wo = -d;
+    U4 gb = rng4(P.seed, (uint32_t)info.z, (uint32_t)info.x, (uint32_t)info.y, 1u + 2u * depth);
+    U4 gl = rng4(P.seed, (uint32_t)info.z, (uint32_t)info.x, (uint32_t)info.y, 2u + 2u * depth);
+    U4 gx = rng4(P.seed, (uint32_t)info.z, (uint32_t)info.x, (uint32_t)info.y, 1u + 2u * depth, 1u);
+    if (S.n_lights > 0) {
+        int li = min((int)(gl.u[2] * (float)S.n_lights), S.n_lights - 1);
+        const RtLight& Lt = S.lights[li];
+        vec3 wi = mk(0.f, 0.f, 1.f), le = mk(0.f, 0.f, 0.f);
+        float dist = RT_INF;
+        bool ok = false;
+        if (Lt.type == RT_LIGHT_AREA) {
+            vec3 y = ld3(Lt.p) + ld3(Lt.u) * gl.u[0] + ld3(Lt.v) * gl.u[1];
+            vec3 dv = y - x;
+            float d2 = dot(dv, dv);
+            dist = sqrtf(d2);
+            wi = dv * (1.f / dist);
+            vec3 ln = ld3(Lt.n);
+            float an = sqrtf(dot(ln, ln));
+            float cl = -dot(wi, ln) / an;
+            if (cl > 0.f) {
+                le = ld3(Lt.I) * (cl * an / d2);
+                ok = true;
+            }
+        } else if (Lt.type == RT_LIGHT_POINT) {
+            vec3 dv = ld3(Lt.p) - x;
+            float d2 = dot(dv, dv);
+            dist = sqrtf(d2);
+            wi = dv * (1.f / dist);
+            le = ld3(Lt.I) * (1.f / d2);
+            ok = true;
+        } else {
+            float pdf;
+            wi = env_sample(S, gl.u[0], gl.u[1], pdf);
+            if (pdf > 0.f) {
+                le = env_value(S, wi) * (1.f / pdf);
+                ok = true;
+            }
+        }
+        float cx = dot(wi, nf);
+        if (ok && cx > 0.f) {
+            vec3 f = bsdf_eval(M, wo, wi, nf);
+            vec3 c = mul3(mul3(alpha, f), le) * (cx * (float)S.n_lights);
+            if (!is_zero(c)) {
+                shadow = true;
+                so = make_float4(x.x, x.y, x.z, RT_EPS);
+                sd = make_float4(wi.x, wi.y, wi.z, dist == RT_INF ? RT_INF : dist - RT_EPS);
+                sc = make_float4(c.x, c.y, c.z, __int_as_float(p));
+            }
+        }
+    }
+    if (depth + 1 >= P.max_depth) return;
+    const float* go = G.out + (size_t)p * GD_PARAMS;
+    float ag = go[40];
+    vec3 wi;
+    if (gx.u[0] < ag) {
+        wi = guide_sample(go, gx.u[1], gb.u[0], gb.u[1]);
+    } else if (!bsdf_sample(M, wo, nf, gb.u[0], gb.u[1], gb.u[2], wi)) {
+        return;
+    }
+    float cs = dot(wi, nf);
+    if (!(cs > 0.f)) return;
+    float pb = bsdf_pdf(M, wo, wi, nf);
+    float pdf = ag * guide_pdf(go, wi) + (1.f - ag) * pb;
+    if (!(pdf > 0.f)) return;
+    vec3 f = bsdf_eval(M, wo, wi, nf);
+    float la = lum(alpha);
+    alpha = mul3(alpha, f) * (cs / pdf);
+    if (depth + 1 >= P.rr_start) {
+        if (gb.u[3] < P.rr_prob) return;
+        alpha = alpha * (1.f / (1.f - P.rr_prob));
+    }
+    if (!(lum(alpha) < 3.0e38f)) return;
+    ps.alpha[p] = make_float4(alpha.x, alpha.y, alpha.z, 0.f);
+    ps.ray_o[p] = make_float4(x.x, x.y, x.z, RT_EPS);
+    ps.ray_d[p] = make_float4(wi.x, wi.y, wi.z, RT_INF);
+    ps.info[p] = make_int4(info.x, info.y, info.z, pk(0, 0, 0));
+    cont = true;
+    if (P.guide_train && gx.u[2] < P.rec_prob && la > 0.f) {
+        want_rec = true;
+        rec.x = x;
+        rec.nf = nf;
+        rec.wo = wo;
+        rec.wi = wi;
+        rec.pb = pb;
+        rec.la = la;
+    }
+}
+
+__global__ __launch_bounds__(PT_BLOCK) void k_shade_guided(DevScene S, PathState ps, GuideState G, PtParams P,
+                                                          int depth, int qsel)
+{
+    const int n = ps.cnt->q_count[qsel];
+    const int32_t* q = ps.queue[qsel];
+    int32_t* qn = ps.queue[qsel ^ 1];
+    for (int base = blockIdx.x * PT_BLOCK; base < n; base += gridDim.x * PT_BLOCK) {
+        int i = base + threadIdx.x;
+        bool cont = false, shadow = false, want = false;
+        float4 so, sd, sc;
+        GuideRecord rec;
+        int p = -1;
+        if (i < n) {
+            p = q[i];
+            shade_guided(S, ps, G, P, depth, p, cont, shadow, so, sd, sc, want, rec);
+        }
+        int at = wave_append(cont, &ps.cnt->q_count[qsel ^ 1]);
+        if (cont) qn[at] = p;
+        int as = wave_append(shadow, &ps.cnt->shadow_count);
+        if (shadow) {
+            ps.sh_o[as] = so;
+            ps.sh_d[as] = sd;
+            ps.sh_c[as] = sc;
+        }
+        if (P.guide_train) {
+            int ar = wave_append(want, G.rec_count);
+            if (want && ar < G.rec_cap) {
+                float* r = G.rec + (size_t)ar * GD_REC;
+                r[0] = rec.x.x; r[1] = rec.x.y; r[2] = rec.x.z;
+                r[3] = rec.nf.x; r[4] = rec.nf.y; r[5] = rec.nf.z;
+                r[6] = rec.wo.x; r[7] = rec.wo.y; r[8] = rec.wo.z;
+                r[9] = rec.wi.x; r[10] = rec.wi.y; r[11] = rec.wi.z;
+                r[12] = rec.pb; r[13] = 0.f; r[14] = rec.la; r[15] = 0.f;
+                float4 rad = ps.radiance[p];
+                G.rec_aux[ar] = make_float4(rad.x, rad.y, rad.z, __int_as_float(p));
+                G.open_rec[p] = ar;
+            }
+        }
+        if (depth > 0) wave_count(i < n, &ps.cnt->extension_rays);
+    }
+}
+
+hipError_t launch_shade_guided(const HostDevScene& s, const PathState& ps, const GuideState& g, PtParams p,
+                               int depth, int qsel, int grid, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_shade_guided, dim3(grid), dim3(PT_BLOCK), 0, st, as_dev(s), ps, g, p, depth, qsel);
+    return hipGetLastError();
 }
 
 template <int MODE>

@@ -103,9 +103,11 @@ class GpuContext:
         return arr
 
     @staticmethod
-    def make_params(mode=_abi.MODE_UNBIASED, max_depth=8, rr_start=3, rr_prob=0.5, seed=0):
+    def make_params(mode=_abi.MODE_UNBIASED, max_depth=8, rr_start=3, rr_prob=0.5, seed=0, guide_train=0,
+                    guide_lr=1e-3, guide_batch=1 << 18):
         p = _abi.RtParams()
         p.mode, p.max_depth, p.rr_start, p.rr_prob, p.seed = int(mode), int(max_depth), int(rr_start), float(rr_prob), int(seed)
+        p.guide_train, p.guide_lr, p.guide_batch = int(guide_train), float(guide_lr), int(guide_batch)
         return p
 
     def render(self, views, films=None, **params):
@@ -144,6 +146,45 @@ class GpuContext:
 
     def set_fault(self, flag=True):
         _check(self.lib, self.h, self.lib.rt_set_fault(self.h, 1 if flag else 0))
+
+    # -- guiding network -----------------------------------------------------------
+    def guide_init(self, seed=0):
+        _check(self.lib, self.h, self.lib.rt_guide_init(self.h, int(seed)))
+
+    def guide_param_count(self):
+        return int(self.lib.rt_guide_param_count())
+
+    def guide_params(self):
+        th = np.empty(self.guide_param_count(), np.float32)
+        _check(self.lib, self.h, self.lib.rt_guide_get_params(self.h, th.ctypes.data, len(th)))
+        return th
+
+    def guide_set_params(self, theta):
+        th = np.ascontiguousarray(theta, np.float32)
+        _check(self.lib, self.h, self.lib.rt_guide_set_params(self.h, th.ctypes.data, len(th)))
+
+    def guide_set_bounds(self, lo, hi):
+        lo = np.ascontiguousarray(lo, np.float32)
+        hi = np.ascontiguousarray(hi, np.float32)
+        _check(self.lib, self.h, self.lib.rt_guide_set_bounds(self.h, lo.ctypes.data, hi.ctypes.data))
+
+    def guide_forward(self, records):
+        rec = _f32(records, 16)
+        y = np.empty((len(rec), 48), np.float32)
+        _check(self.lib, self.h, self.lib.rt_guide_forward(self.h, rec.ctypes.data, len(rec), y.ctypes.data))
+        return y
+
+    def guide_gradient(self, records):
+        rec = _f32(records, 16)
+        g = np.empty(self.guide_param_count(), np.float32)
+        loss = np.zeros(1, np.float32)
+        _check(self.lib, self.h, self.lib.rt_guide_gradient(self.h, rec.ctypes.data, len(rec), g.ctypes.data,
+                                                            loss.ctypes.data))
+        return float(loss[0]), g
+
+    def guide_train(self, records, steps=1, lr=1e-3):
+        rec = _f32(records, 16)
+        _check(self.lib, self.h, self.lib.rt_guide_train(self.h, rec.ctypes.data, len(rec), int(steps), float(lr)))
 
 
 class GpuUniDiPathTracingIntegrator:
