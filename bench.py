@@ -37,13 +37,16 @@ def parse():
     ap.add_argument("--subdiv", type=int, default=512, help="tetrahedron segments per edge (4 n^2 triangles)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a 1 GiB device-to-device copy, the reference point for HBM counters")
     return ap.parse_args()
 
 
 def cpu_baseline(arrays, scene, budget_s):
-    """The oracle (reference algorithms: leaf-1 BVH visiting both children,
-    per-sample critical section) on rows of the same film at 1 spp until the
-    time budget is spent.  Returns Mrays/s over the rows rendered."""
+    """The oracle (the reference's algorithms: leaf-1 BVH visiting both
+    children, per-sample critical section) rendering whole frames of the same
+    workload at 1 spp each, new seed per frame, until the time budget is
+    spent.  Returns Mrays/s over the frames rendered."""
     from oracle.oracle import Oracle
 
     threads = os.cpu_count() or 1
@@ -52,23 +55,18 @@ def cpu_baseline(arrays, scene, budget_s):
     build_s = time.time() - t0
     cam = scene.camera
     rays = 0
-    rows = 0
+    frames = 0
     t_render = 0.0
-    step = max(1, threads)
-    y = 0
-    while t_render < budget_s and y < cam.height:
+    while t_render < budget_s:
         t1 = time.time()
-        _, r = orc.render((cam, 1, 1), mode=1, max_depth=8, rr_start=3, rr_prob=0.5, seed=0,
-                          rows=(y, min(cam.height, y + step)), threads=threads)
+        _, r = orc.render((cam, 1, 1), mode=1, max_depth=8, rr_start=3, rr_prob=0.5, seed=frames, threads=threads)
         t_render += time.time() - t1
         rays += r
-        rows += min(cam.height, y + step) - y
-        y += step * 8  # spread the sample over the image
+        frames += 1
     orc.close()
-    return {"value": rays / t_render / 1e6 if t_render > 0 else None, "unit": "Mrays/s", "cores": threads,
-            "kind": "port",
-            "sample": f"{rows} rows of {cam.width} px at 1 spp (rows spread over the image), {rays} rays in "
-                      f"{t_render:.1f} s; reference BVH build {build_s:.1f} s not timed"}
+    return {"value": round(rays / t_render / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": f"{frames} frames of {cam.width}x{cam.height} at 1 spp (seeds 0..{frames - 1}), {rays} rays "
+                      f"in {t_render:.1f} s on {threads} threads; reference BVH build {build_s:.1f} s not timed"}
 
 
 def main():
@@ -131,6 +129,15 @@ def main():
     if os.path.exists(tfile):
         with open(tfile) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
+
+    if args.copy_calibration:
+        a = torch.ones(1 << 28, dtype=torch.float32, device="cuda:0")  # 1 GiB
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        log("copy calibration: 1 GiB read + 1 GiB written")
+        del a, b
 
     cpu = None
     if not args.no_cpu:

@@ -109,17 +109,28 @@ struct DevScene {
  * not Moller-Trumbore, so hit/miss decisions and t agree with it to rounding.
  * Accept: tmin <= t <= tmax and t < best.
  * ------------------------------------------------------------------------- */
+/* Primitive arithmetic is compiled without FMA contraction so that every
+ * product and sum is rounded as in the reference's written expression
+ * order; t then agrees bit for bit with a CPU evaluation of the same
+ * formulas.  (The pragma marks the instructions of these functions, and the
+ * marks survive inlining.) */
+#pragma clang fp contract(off)
+__device__ __forceinline__ float dot_x(vec3 a, vec3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ vec3 cross_x(vec3 a, vec3 b)
+{
+    return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
 __device__ __forceinline__ bool hit_tri(vec3 o, vec3 d, float4 r0, float4 r1, float4 r2, float tmin, float tmax,
                                         float best, float& t, float& u, float& v, bool para)
 {
     vec3 p1 = xyz(r0), e1 = xyz(r1), e2 = xyz(r2);
     vec3 ad = o - p1;
-    vec3 n = cross(e1, e2);
-    float w = -dot(n, d);
-    t = dot(n, ad) / w;
-    vec3 adxde = cross(ad, d);
-    u = dot(adxde, e2) / w;
-    v = -dot(adxde, e1) / w;
+    vec3 n = cross_x(e1, e2);
+    float w = -dot_x(n, d);
+    t = dot_x(n, ad) / w;
+    vec3 adxde = cross_x(ad, d);
+    u = dot_x(adxde, e2) / w;
+    v = -dot_x(adxde, e1) / w;
     bool ok = (w != 0.f) && (t >= tmin) && (t <= tmax) && (t < best) && (u >= 0.f) && (u <= 1.f) && (v >= 0.f);
     ok = ok && (para ? (v <= 1.f) : (u + v <= 1.f));
     return ok;
@@ -131,9 +142,9 @@ __device__ __forceinline__ bool hit_sphere(vec3 o, vec3 d, float4 r0, float4 r1,
     vec3 c = xyz(r0);
     float rad = r1.x;
     vec3 diff = o - c;
-    float a = dot(d, d);
-    float b = 2.f * dot(d, diff);
-    float cc = dot(diff, diff) - rad * rad;
+    float a = dot_x(d, d);
+    float b = 2.f * dot_x(d, diff);
+    float cc = dot_x(diff, diff) - rad * rad;
     float disc = b * b - 4.f * a * cc;
     if (!(disc >= 0.f)) return false;
     float root = sqrtf(disc);
@@ -142,6 +153,7 @@ __device__ __forceinline__ bool hit_sphere(vec3 o, vec3 d, float4 r0, float4 r1,
     t = t0 >= tmin ? t0 : t1;
     return t <= tmax && t < best;
 }
+#pragma clang fp contract(fast)
 
 /* Slab test against one child box, with the reference's choice of near and
  * far plane by the sign of 1/d (src/axisalignedbox.cpp:30-73): a NaN from

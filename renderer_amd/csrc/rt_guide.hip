@@ -527,7 +527,7 @@ hipError_t launch_guide_train(const GuideState& g, const float* rec, const int32
  * ========================================================================= */
 __global__ __launch_bounds__(GB) void k_guide_adam(GuideState g, float lr, int step, const int32_t* count, int max_n)
 {
-    const int N = min(min(*count, max_n), g.rec_cap);
+    const int N = min(*count, max_n);
     const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
     const float c1 = 1.f - powf(b1, (float)step), c2 = 1.f - powf(b2, (float)step);
     for (int i = blockIdx.x * GB + threadIdx.x; i < GD_NPARAM; i += gridDim.x * GB) {

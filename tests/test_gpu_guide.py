@@ -120,6 +120,7 @@ def test_learns_a_known_lobe(guide_ctx):
     rng = np.random.default_rng(8)
     B = 16384
     rec = _records(B, 9)
+    rec[:, 0:9] = rec[0, 0:9]  # one shading point: the network cannot memorise per-record positions
     om = _unit(rng.normal(size=(B, 3)))
     mu = np.array([0.6, 0.0, 0.8])
     kappa = 20.0
@@ -135,10 +136,14 @@ def test_learns_a_known_lobe(guide_ctx):
     x, omo, pbo, wo_ = _oracle_inputs(rec[:256])
     y, _ = G.forward(_fp16_weights(th), x, fp16_acts=True)
     pm, q, alpha = G.mixture_pdf(y, np.tile(mu, (256, 1)), np.full(256, 1 / (4 * math.pi)))
-    print(f"loss {l0:.4f} -> {l1:.4f}; guide density at the lobe mean {np.median(q):.2f} "
-          f"(target {target.max():.2f}); alpha {np.median(alpha):.3f}", flush=True)
-    assert l1 < l0 - 0.5
-    assert np.median(q) > 0.3 * kappa / (2 * math.pi)
+    # cross-entropy of the target with itself (its entropy) bounds the loss
+    # from below; E[mu.w] = coth(k) - 1/k for a vMF
+    h = -(math.log(kappa / (2 * math.pi * (1 - math.exp(-2 * kappa)))) + kappa * (1 / math.tanh(kappa) - 1 / kappa - 1))
+    print(f"loss {l0:.4f} -> {l1:.4f} (target entropy {h:.4f}); guide density at the lobe mean {np.median(q):.2f} "
+          f"(target {kappa / (2 * math.pi):.2f}); alpha {np.median(alpha):.3f}", flush=True)
+    assert l1 < l0 - 1.0
+    assert l1 > h - 0.2  # a proper density: no better than the target itself (up to sampling noise)
+    assert np.median(q) > 0.4 * kappa / (2 * math.pi)
     assert np.median(alpha) > 0.7
 
 
