@@ -51,7 +51,10 @@ struct rt_ctx {
     size_t ps_cap = 0;
     std::vector<DevBuf> ps_bufs;
     DevBuf counters, views, tiles, passes, films;
-    rt_launch_config cfg{0, 256, 4, 0, 0};
+    /* defaults from a sweep on MI355X (1M-triangle mesh, 1024^2 x 64 spp):
+     * 16M path slots per pass, 12 persistent blocks per CU (two rounds of
+     * the 6 blocks whose traversal stacks fit one CU's LDS) */
+    rt_launch_config cfg{0, 256, 12, 0, 0};
 
     // guiding network
     GuideState guide{};
@@ -881,7 +884,7 @@ static int render_impl(rt_ctx* c, const rt_view* views, int32_t nv, const rt_par
         max_pix = std::max(max_pix, npix);
     }
     hipSetDevice(c->device);
-    int64_t P = c->cfg.path_slots > 0 ? c->cfg.path_slots : (int64_t)1 << 21;
+    int64_t P = c->cfg.path_slots > 0 ? c->cfg.path_slots : (int64_t)1 << 24;
     P = std::min<int64_t>(P, std::max<int64_t>(total_samples, 1));
     P = std::max<int64_t>(P, 256);
 
@@ -960,6 +963,7 @@ static int render_impl(rt_ctx* c, const rt_view* views, int32_t nv, const rt_par
                 (float)std::min(1.0, 2.0 * batch / std::max(1.0, (double)P * 2.0))};
     const int grid = grid_for(c);
     const int depth = prm->max_depth;
+    const int stack = c->bvh_depth < 24 ? 24 : RT_STACK_DEPTH;
     const bool timing = c->timing != 0;
     size_t need_ev = timing ? (size_t)passes.size() * depth * 4 : 0;
     while (c->ev_pool.size() < need_ev) {
@@ -986,7 +990,7 @@ static int render_impl(rt_ctx* c, const rt_view* views, int32_t nv, const rt_par
         for (int dd = 0; dd < depth; ++dd) {
             int q = dd & 1;
             if (timing) LCHK(hipEventRecord(c->ev_pool[ev++], c->stream));
-            LCHK(launch_extend(c->scene, c->ps, q, c->cfg.count_traversal, grid, c->stream));
+            LCHK(launch_extend(c->scene, c->ps, q, c->cfg.count_traversal, stack, grid, c->stream));
             if (timing) LCHK(hipEventRecord(c->ev_pool[ev++], c->stream));
             if (guided) {
                 LCHK(launch_guide_infer(c->scene, c->ps, c->guide, q, grid, c->stream));
@@ -995,7 +999,7 @@ static int render_impl(rt_ctx* c, const rt_view* views, int32_t nv, const rt_par
                 LCHK(launch_shade(c->scene, c->ps, pp, dd, q, grid, c->stream));
             }
             if (timing) LCHK(hipEventRecord(c->ev_pool[ev++], c->stream));
-            LCHK(launch_shadow(c->scene, c->ps, pp, 256, grid, c->stream));
+            LCHK(launch_shadow(c->scene, c->ps, pp, stack, grid, c->stream));
             if (timing) LCHK(hipEventRecord(c->ev_pool[ev++], c->stream));
             LCHK(launch_advance(c->ps, q, c->stream));
         }

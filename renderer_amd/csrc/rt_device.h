@@ -127,13 +127,27 @@ __device__ __forceinline__ bool hit_tri(vec3 o, vec3 d, float4 r0, float4 r1, fl
     vec3 ad = o - p1;
     vec3 n = cross_x(e1, e2);
     float w = -dot_x(n, d);
-    t = dot_x(n, ad) / w;
+    float tn = dot_x(n, ad);
     vec3 adxde = cross_x(ad, d);
-    u = dot_x(adxde, e2) / w;
-    v = -dot_x(adxde, e1) / w;
-    bool ok = (w != 0.f) && (t >= tmin) && (t <= tmax) && (t < best) && (u >= 0.f) && (u <= 1.f) && (v >= 0.f);
-    ok = ok && (para ? (v <= 1.f) : (u + v <= 1.f));
-    return ok;
+    float un = dot_x(adxde, e2);
+    float vn = -dot_x(adxde, e1);
+    if (w == 0.f) return false;
+    /* Prefilter with the hardware reciprocal (within 1 ulp): x * rcp(w)
+     * differs from x / w by at most 2 ulps and has the same sign, so these
+     * widened tests accept every ray the exact tests accept; only the
+     * candidates pay for the three correctly rounded divisions. */
+    const float m = 1e-6f;
+    float rw = __builtin_amdgcn_rcpf(w);
+    float ta = tn * rw, ua = un * rw, va = vn * rw;
+    float bound = fminf(tmax, best);
+    bool cand = (ta >= tmin - m * fabsf(tmin)) && (ta <= bound + m * fabsf(bound)) && (ua >= 0.f) &&
+                (ua <= 1.f + m) && (va >= 0.f) && (para ? (va <= 1.f + m) : (ua + va <= 1.f + 2.f * m));
+    if (!cand) return false;
+    t = tn / w;
+    u = un / w;
+    v = vn / w;
+    bool ok = (t >= tmin) && (t <= tmax) && (t < best) && (u >= 0.f) && (u <= 1.f) && (v >= 0.f);
+    return ok && (para ? (v <= 1.f) : (u + v <= 1.f));
 }
 
 __device__ __forceinline__ bool hit_sphere(vec3 o, vec3 d, float4 r0, float4 r1, float tmin, float tmax,
@@ -155,18 +169,6 @@ __device__ __forceinline__ bool hit_sphere(vec3 o, vec3 d, float4 r0, float4 r1,
 }
 #pragma clang fp contract(fast)
 
-/* Slab test against one child box, with the reference's choice of near and
- * far plane by the sign of 1/d (src/axisalignedbox.cpp:30-73): a NaN from
- * 0 * inf on an axis constrains nothing (fmaxf/fminf drop NaN operands),
- * and the comparison is inclusive so flat boxes of axis-aligned quads pass. */
-__device__ __forceinline__ float slab_enter(float lo, float hi, float o, float inv, float& far)
-{
-    float a = (lo - o) * inv, b = (hi - o) * inv;
-    bool pos = inv >= 0.f;
-    far = pos ? b : a;
-    return pos ? a : b;
-}
-
 /* Leaf word decoding (rt_layout.h). */
 __device__ __forceinline__ void leaf_decode(int32_t w, uint32_t& first, uint32_t& count, uint32_t& kind)
 {
@@ -182,103 +184,151 @@ struct TraceOut {
 };
 
 /*
- * Stack-based traversal of the flattened BVH, nearest child first, pruned
- * by the closest hit so far.  The traversal stack lives in LDS, laid out
- * [depth][thread] so the 64 lanes of a wave hit 64 different banks.
- * ANY: return at the first hit in range (shadow rays).
- * COUNT: count interior nodes fetched and primitives tested.
+ * Resumable traversal state of one ray: nearest child first, pruned by the
+ * closest hit so far, the stack of deferred subtrees in LDS laid out
+ * [depth][thread] (the 64 lanes of a wave hit 64 different banks).
+ *
+ * Box tests use t = lo * inv - o * inv (one FMA per plane) with a zero
+ * direction component replaced by a tiny signed value, and accept a box
+ * when  max(near planes, tmin) <= min(far planes, closest) * (1 + 2^-20):
+ * the relative slack absorbs the rounding of the plane distances, so a box
+ * is never rejected where the reference's exact-order slab test
+ * (src/axisalignedbox.cpp:30-73) would accept it.  Box tests only prune;
+ * the reported hit comes from the primitive tests above.
  */
+struct Trav {
+    vec3 o, d, inv, oi;
+    float tmin, tmax, best, u, v;
+    int32_t g, cur, sp;
+};
+
+__device__ __forceinline__ float safe_inv(float x)
+{
+    const float tiny = 1e-30f;
+    return 1.0f / (fabsf(x) < tiny ? copysignf(tiny, x) : x);
+}
+
+__device__ __forceinline__ void trav_init(const DevScene& S, Trav& T, vec3 o, vec3 d, float tmin, float tmax)
+{
+    T.o = o;
+    T.d = d;
+    T.inv = mk(safe_inv(d.x), safe_inv(d.y), safe_inv(d.z));
+    T.oi = mk(o.x * T.inv.x, o.y * T.inv.y, o.z * T.inv.z);
+    T.tmin = tmin;
+    T.tmax = tmax;
+    T.best = RT_INF;
+    T.u = T.v = 0.f;
+    T.g = -1;
+    T.sp = 0;
+    T.cur = S.n_nodes > 0 && tmin <= tmax ? 0 : -1;
+}
+
+__device__ __forceinline__ void box2(const Trav& T, float lo, float hi, float inv, float oi, float& n, float& f)
+{
+    float a = fmaf(lo, inv, -oi), b = fmaf(hi, inv, -oi);
+    bool pos = inv >= 0.f;
+    n = pos ? a : b;
+    f = pos ? b : a;
+}
+
+/* Visit node T.cur (T.cur >= 0): test both child boxes, test the
+ * primitives of hit leaves, continue with the nearer hit interior child
+ * and defer the other; pop when neither is hit.  Returns true when the
+ * traversal of this ray is complete. */
+template <bool ANY, bool COUNT>
+__device__ __forceinline__ bool trav_step(const DevScene& S, Trav& T, int32_t* stack, int stride, int& cn, int& cp)
+{
+    const float4* nd = S.nodes + 4 * (size_t)T.cur;
+    float4 f0 = nd[0], f1 = nd[1], f2 = nd[2], f3 = nd[3];
+    if (COUNT) ++cn;
+    int32_t c0 = __float_as_int(f3.x), c1 = __float_as_int(f3.y);
+    float bound = fminf(T.tmax, T.best);
+    bound = bound + fabsf(bound) * 9.5367431640625e-07f;
+    float nx, ny, nz, fx, fy, fz;
+    box2(T, f0.x, f0.y, T.inv.x, T.oi.x, nx, fx);
+    box2(T, f0.z, f0.w, T.inv.y, T.oi.y, ny, fy);
+    box2(T, f1.x, f1.y, T.inv.z, T.oi.z, nz, fz);
+    float tl0 = fmaxf(fmaxf(nx, ny), fmaxf(nz, T.tmin));
+    float th0 = fminf(fminf(fx, fy), fminf(fz, bound));
+    box2(T, f1.z, f1.w, T.inv.x, T.oi.x, nx, fx);
+    box2(T, f2.x, f2.y, T.inv.y, T.oi.y, ny, fy);
+    box2(T, f2.z, f2.w, T.inv.z, T.oi.z, nz, fz);
+    float tl1 = fmaxf(fmaxf(nx, ny), fmaxf(nz, T.tmin));
+    float th1 = fminf(fminf(fx, fy), fminf(fz, bound));
+    bool h0 = tl0 <= th0 + fabsf(th0) * 9.5367431640625e-07f;
+    bool h1 = tl1 <= th1 + fabsf(th1) * 9.5367431640625e-07f;
+    bool swap = !ANY && h0 && h1 && tl1 < tl0;
+    int32_t a = swap ? c1 : c0, b = swap ? c0 : c1;
+    bool ha = swap ? h1 : h0, hb = swap ? h0 : h1;
+    int32_t next = -1;
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+        int32_t c = side == 0 ? a : b;
+        bool hc = side == 0 ? ha : hb;
+        if (!hc) continue;
+        if (c >= 0) {
+            if (next < 0) {
+                next = c;
+            } else {
+                stack[T.sp * stride] = c;
+                ++T.sp;
+            }
+            continue;
+        }
+        uint32_t first, count, kind;
+        leaf_decode(c, first, count, kind);
+        for (uint32_t j = 0; j < count; ++j) {
+            const float4* pr = S.prims + 3 * (size_t)(first + j);
+            float4 r0 = pr[0], r1 = pr[1];
+            if (COUNT) ++cp;
+            float t, u = 0.f, v = 0.f;
+            bool hit;
+            if (kind == RT_KIND_SPHERE) {
+                hit = hit_sphere(T.o, T.d, r0, r1, T.tmin, T.tmax, T.best, t);
+            } else {
+                float4 r2 = pr[2];
+                hit = hit_tri(T.o, T.d, r0, r1, r2, T.tmin, T.tmax, T.best, t, u, v, kind == RT_KIND_PARA);
+            }
+            if (hit) {
+                T.best = t;
+                T.u = u;
+                T.v = v;
+                T.g = __float_as_int(r0.w);
+                if (ANY) return true;
+            }
+        }
+    }
+    if (next >= 0) {
+        T.cur = next;
+        return false;
+    }
+    if (T.sp == 0) return true;
+    --T.sp;
+    T.cur = stack[T.sp * stride];
+    return false;
+}
+
+/* Whole traversal of one ray (batch kernels and counting). */
 template <bool ANY, bool COUNT>
 __device__ __forceinline__ TraceOut traverse(const DevScene& S, vec3 o, vec3 d, float tmin, float tmax,
                                              int32_t* stack, int stride, int* n_nodes = nullptr,
                                              int* n_prims = nullptr)
 {
-    TraceOut out;
-    out.t = RT_INF;
-    out.u = out.v = 0.f;
-    out.g = -1;
-    if (S.n_nodes == 0) return out;
-    vec3 inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
-    float best = RT_INF;
-    int sp = 0;
-    int32_t cur = 0;
+    Trav T;
+    trav_init(S, T, o, d, tmin, tmax);
     int cn = 0, cp = 0;
-    while (true) {
-        const float4* nd = S.nodes + 4 * (size_t)cur;
-        float4 f0 = nd[0], f1 = nd[1], f2 = nd[2], f3 = nd[3];
-        if (COUNT) ++cn;
-        int32_t c0 = __float_as_int(f3.x), c1 = __float_as_int(f3.y);
-        float bound = fminf(tmax, best);
-        bound = bound + fabsf(bound) * 9.5367431640625e-07f; /* 2^-20 slack for rounding */
-        float fx, fy, fz;
-        float nx = slab_enter(f0.x, f0.y, o.x, inv.x, fx);
-        float ny = slab_enter(f0.z, f0.w, o.y, inv.y, fy);
-        float nz = slab_enter(f1.x, f1.y, o.z, inv.z, fz);
-        float tl0 = fmaxf(fmaxf(nx, ny), fmaxf(nz, tmin));
-        float th0 = fminf(fminf(fx, fy), fminf(fz, bound));
-        nx = slab_enter(f1.z, f1.w, o.x, inv.x, fx);
-        ny = slab_enter(f2.x, f2.y, o.y, inv.y, fy);
-        nz = slab_enter(f2.z, f2.w, o.z, inv.z, fz);
-        float tl1 = fmaxf(fmaxf(nx, ny), fmaxf(nz, tmin));
-        float th1 = fminf(fminf(fx, fy), fminf(fz, bound));
-        bool h0 = tl0 <= th0, h1 = tl1 <= th1;
-        /* order: nearer child first */
-        bool swap = h0 && h1 && tl1 < tl0;
-        int32_t a = swap ? c1 : c0, b = swap ? c0 : c1;
-        bool ha = swap ? h1 : h0, hb = swap ? h0 : h1;
-        int32_t next = -1;
-        bool have_next = false;
-#pragma unroll
-        for (int side = 0; side < 2; ++side) {
-            int32_t c = side == 0 ? a : b;
-            bool hc = side == 0 ? ha : hb;
-            if (!hc) continue;
-            if (c >= 0) {
-                if (!have_next) {
-                    next = c;
-                    have_next = true;
-                } else {
-                    stack[sp * stride] = c;
-                    ++sp;
-                }
-                continue;
-            }
-            uint32_t first, count, kind;
-            leaf_decode(c, first, count, kind);
-            for (uint32_t j = 0; j < count; ++j) {
-                const float4* pr = S.prims + 3 * (size_t)(first + j);
-                float4 r0 = pr[0], r1 = pr[1];
-                if (COUNT) ++cp;
-                float t, u = 0.f, v = 0.f;
-                bool hit;
-                if (kind == RT_KIND_SPHERE) {
-                    hit = hit_sphere(o, d, r0, r1, tmin, tmax, best, t);
-                } else {
-                    float4 r2 = pr[2];
-                    hit = hit_tri(o, d, r0, r1, r2, tmin, tmax, best, t, u, v, kind == RT_KIND_PARA);
-                }
-                if (hit) {
-                    best = t;
-                    out.t = t;
-                    out.u = u;
-                    out.v = v;
-                    out.g = __float_as_int(r0.w);
-                    if (ANY) {
-                        if (COUNT) { *n_nodes = cn; *n_prims = cp; }
-                        return out;
-                    }
-                }
-            }
+    if (T.cur >= 0)
+        while (!trav_step<ANY, COUNT>(S, T, stack, stride, cn, cp)) {
         }
-        if (have_next) {
-            cur = next;
-        } else {
-            if (sp == 0) break;
-            --sp;
-            cur = stack[sp * stride];
-        }
+    if (COUNT) {
+        *n_nodes = cn;
+        *n_prims = cp;
     }
-    if (COUNT) { *n_nodes = cn; *n_prims = cp; }
+    TraceOut out;
+    out.t = T.g >= 0 ? T.best : RT_INF;
+    out.u = T.u;
+    out.v = T.v;
+    out.g = T.g;
     return out;
 }
 

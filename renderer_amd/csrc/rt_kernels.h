@@ -108,10 +108,11 @@ hipError_t launch_trace_count(const HostDevScene& s, const float4* o, const floa
 /* Wavefront pipeline kernels. */
 hipError_t launch_generate(const PathState& ps, const DevView* views, const DevTile* tiles, const DevPass* passes,
                            PtParams p, int grid, hipStream_t st);
-hipError_t launch_extend(const HostDevScene& s, const PathState& ps, int qsel, int count, int grid, hipStream_t st);
+hipError_t launch_extend(const HostDevScene& s, const PathState& ps, int qsel, int count, int stack, int grid,
+                         hipStream_t st);
 hipError_t launch_shade(const HostDevScene& s, const PathState& ps, PtParams p, int depth, int qsel, int grid,
                         hipStream_t st);
-hipError_t launch_shadow(const HostDevScene& s, const PathState& ps, PtParams p, int block, int grid,
+hipError_t launch_shadow(const HostDevScene& s, const PathState& ps, PtParams p, int stack, int grid,
                          hipStream_t st);
 hipError_t launch_advance(const PathState& ps, int qsel, hipStream_t st);
 hipError_t launch_accumulate(const PathState& ps, const DevView* views, const DevTile* tiles,

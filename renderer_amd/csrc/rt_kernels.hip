@@ -342,37 +342,59 @@ hipError_t launch_generate(const PathState& ps, const DevView* views, const DevT
 /* ===========================================================================
  * extend: closest hit for every live path
  * ========================================================================= */
-template <bool COUNT>
+/* Persistent traversal with per-lane ray replacement: a lane whose ray is
+ * finished stores its result and starts its next ray (stride = grid size)
+ * at once, instead of idling until the slowest ray of its wave finishes.
+ * STACK is the LDS stack depth (>= the BVH depth + 1); the smaller it is,
+ * the more waves fit in a CU's LDS. */
+template <bool COUNT, int STACK>
 __global__ __launch_bounds__(PT_BLOCK) void k_extend(DevScene S, PathState ps, int qsel)
 {
-    __shared__ int32_t stack[RT_STACK_DEPTH * PT_BLOCK];
+    __shared__ int32_t stack[STACK * PT_BLOCK];
     int32_t* my = stack + threadIdx.x;
     const int n = ps.cnt->q_count[qsel];
     const int32_t* q = ps.queue[qsel];
-    unsigned long long tn = 0, tp = 0;
-    for (int i = blockIdx.x * PT_BLOCK + threadIdx.x; i < n; i += gridDim.x * PT_BLOCK) {
-        int p = q[i];
-        float4 a = ps.ray_o[p], b = ps.ray_d[p];
-        int cn = 0, cp = 0;
-        TraceOut h = traverse<false, COUNT>(S, xyz(a), xyz(b), a.w, b.w, my, PT_BLOCK, &cn, &cp);
-        ps.hit[p] = make_float4(h.t, __int_as_float(h.g), h.u, h.v);
-        if (COUNT) {
-            tn += cn;
-            tp += cp;
+    const int stride = gridDim.x * PT_BLOCK;
+    int i = blockIdx.x * PT_BLOCK + threadIdx.x;
+    int cn = 0, cp = 0, p = 0;
+    bool live = false;
+    Trav T;
+    while (true) {
+        if (!live) {
+            if (i >= n) break;
+            p = q[i];
+            float4 a = ps.ray_o[p], b = ps.ray_d[p];
+            trav_init(S, T, xyz(a), xyz(b), a.w, b.w);
+            live = true;
+        }
+        bool done = T.cur < 0 || trav_step<false, COUNT>(S, T, my, PT_BLOCK, cn, cp);
+        if (done) {
+            ps.hit[p] = make_float4(T.g >= 0 ? T.best : RT_INF, __int_as_float(T.g), T.u, T.v);
+            i += stride;
+            live = false;
         }
     }
-    if (COUNT && (tn || tp)) {
-        atomicAdd(&ps.cnt->node_visits, tn);
-        atomicAdd(&ps.cnt->prim_tests, tp);
+    if (COUNT && (cn || cp)) {
+        atomicAdd(&ps.cnt->node_visits, (unsigned long long)cn);
+        atomicAdd(&ps.cnt->prim_tests, (unsigned long long)cp);
     }
 }
 
-hipError_t launch_extend(const HostDevScene& s, const PathState& ps, int qsel, int count, int grid, hipStream_t st)
+hipError_t launch_extend(const HostDevScene& s, const PathState& ps, int qsel, int count, int stack, int grid,
+                         hipStream_t st)
 {
-    if (count)
-        hipLaunchKernelGGL(k_extend<true>, dim3(grid), dim3(PT_BLOCK), 0, st, as_dev(s), ps, qsel);
-    else
-        hipLaunchKernelGGL(k_extend<false>, dim3(grid), dim3(PT_BLOCK), 0, st, as_dev(s), ps, qsel);
+    if (stack <= 24) {
+        if (count)
+            hipLaunchKernelGGL((k_extend<true, 24>), dim3(grid), dim3(PT_BLOCK), 0, st, as_dev(s), ps, qsel);
+        else
+            hipLaunchKernelGGL((k_extend<false, 24>), dim3(grid), dim3(PT_BLOCK), 0, st, as_dev(s), ps, qsel);
+    } else {
+        if (count)
+            hipLaunchKernelGGL((k_extend<true, RT_STACK_DEPTH>), dim3(grid), dim3(PT_BLOCK), 0, st, as_dev(s), ps, qsel);
+        else
+            hipLaunchKernelGGL((k_extend<false, RT_STACK_DEPTH>), dim3(grid), dim3(PT_BLOCK), 0, st, as_dev(s), ps,
+                               qsel);
+    }
     return hipGetLastError();
 }
 
@@ -935,17 +957,29 @@ hipError_t launch_shade(const HostDevScene& s, const PathState& ps, PtParams p, 
 /* ===========================================================================
  * shadow: any-hit test of next-event rays
  * ========================================================================= */
+template <int STACK>
 __global__ __launch_bounds__(PT_BLOCK) void k_shadow(DevScene S, PathState ps, int compat)
 {
-    __shared__ int32_t stack[RT_STACK_DEPTH * PT_BLOCK];
+    __shared__ int32_t stack[STACK * PT_BLOCK];
     int32_t* my = stack + threadIdx.x;
     const int n = ps.cnt->shadow_count;
-    for (int base = blockIdx.x * PT_BLOCK; base < n; base += gridDim.x * PT_BLOCK) {
-        int i = base + threadIdx.x;
-        if (i < n) {
-            float4 a = ps.sh_o[i], b = ps.sh_d[i], c = ps.sh_c[i];
-            TraceOut h = traverse<true, false>(S, xyz(a), xyz(b), a.w, b.w, my, PT_BLOCK);
-            if (h.g < 0) {
+    const int stride = gridDim.x * PT_BLOCK;
+    int i = blockIdx.x * PT_BLOCK + threadIdx.x;
+    int cn = 0, cp = 0, traced = 0;
+    bool live = false;
+    float4 c;
+    Trav T;
+    while (true) {
+        if (!live) {
+            if (i >= n) break;
+            float4 a = ps.sh_o[i], b = ps.sh_d[i];
+            c = ps.sh_c[i];
+            trav_init(S, T, xyz(a), xyz(b), a.w, b.w);
+            live = true;
+        }
+        bool done = T.cur < 0 || trav_step<true, false>(S, T, my, PT_BLOCK, cn, cp);
+        if (done) {
+            if (T.g < 0) {
                 int p = __float_as_int(c.w);
                 if (compat) {
                     ps.pend_c[p] = make_float4(c.x, c.y, c.z, 0.f);
@@ -954,17 +988,23 @@ __global__ __launch_bounds__(PT_BLOCK) void k_shadow(DevScene S, PathState ps, i
                     ps.radiance[p] = make_float4(r.x + c.x, r.y + c.y, r.z + c.z, 0.f);
                 }
             }
+            ++traced;
+            i += stride;
+            live = false;
         }
-        wave_count(i < n, &ps.cnt->shadow_rays);
     }
+    for (int o = 32; o > 0; o >>= 1) traced += __shfl_xor(traced, o);
+    if (__lane_id() == 0 && traced) atomicAdd(&ps.cnt->shadow_rays, (unsigned long long)traced);
 }
 
-hipError_t launch_shadow(const HostDevScene& s, const PathState& ps, PtParams p, int block, int grid,
+hipError_t launch_shadow(const HostDevScene& s, const PathState& ps, PtParams p, int stack, int grid,
                          hipStream_t st)
 {
-    (void)block;
-    hipLaunchKernelGGL(k_shadow, dim3(grid), dim3(PT_BLOCK), 0, st, as_dev(s), ps,
-                       p.mode == RT_MODE_COMPAT ? 1 : 0);
+    int compat = p.mode == RT_MODE_COMPAT ? 1 : 0;
+    if (stack <= 24)
+        hipLaunchKernelGGL(k_shadow<24>, dim3(grid), dim3(PT_BLOCK), 0, st, as_dev(s), ps, compat);
+    else
+        hipLaunchKernelGGL(k_shadow<RT_STACK_DEPTH>, dim3(grid), dim3(PT_BLOCK), 0, st, as_dev(s), ps, compat);
     return hipGetLastError();
 }
 

@@ -114,7 +114,7 @@ def test_batched_views_match_single_views(gpu_ctx):
     try:
         small = gpu_ctx.render(views, mode=MODE_UNBIASED, max_depth=6, seed=3)
     finally:
-        gpu_ctx.set_launch_config(path_slots=1 << 21)
+        gpu_ctx.set_launch_config(path_slots=1 << 24)
     for a, b in zip(batch, small):
         assert np.array_equal(a, b)
 
