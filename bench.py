@@ -39,7 +39,102 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 1 GiB device-to-device copy, the reference point for HBM counters")
+    ap.add_argument("--no-guided", action="store_true", help="skip the equal-time guided vs plain comparison")
+    ap.add_argument("--room-res", type=str, default="480x270")
+    ap.add_argument("--room-subdiv", type=int, default=64, help="octahedron segments per edge of the 8 objects")
+    ap.add_argument("--room-seconds", type=float, default=6.0, help="wall-time budget of each estimator")
+    ap.add_argument("--room-ref-spp", type=int, default=4096)
+    ap.add_argument("--room-pass-spp", type=int, default=16)
+    ap.add_argument("--room-train-passes", type=int, default=24)
     return ap.parse_args()
+
+
+def rel_mse(img, ref, eps=1e-2):
+    """mean over pixels and channels of (I - R)^2 / (R^2 + eps)."""
+    return float(((img - ref) ** 2 / (ref ** 2 + eps)).mean())
+
+
+def equal_time_guided(args):
+    """Occluded-light room (the only emitter outside, behind a door gap of
+    5% of the wall width), max depth 16: plain unbiased PT and guided PT each
+    get the same wall-time budget; the guided budget includes the online
+    training of its network.  Both are scored by relMSE against a
+    high-sample plain render.  Returns the comparison as a dict."""
+    import numpy as np
+    import torch
+
+    from renderer_amd import MODE_GUIDED, MODE_UNBIASED, GpuContext, scenes
+
+    W, H = (int(x) for x in args.room_res.split("x"))
+    t0 = time.time()
+    scene = scenes.room_scene(W, H, n=args.room_subdiv, seed=3)
+    arrays = scene.flatten()
+    log(f"room: {len(arrays.tri_vertices)} triangles, {W}x{H}, built in {time.time() - t0:.1f} s")
+    ctx = GpuContext(0)
+    ctx.upload(arrays)
+    common = dict(max_depth=16, rr_start=3, rr_prob=0.5)
+    dev = "cuda:0"
+
+    def render(spp, seed, mode, train=0):
+        f = torch.zeros((H, W, 3), dtype=torch.float32, device=dev)
+        torch.cuda.synchronize()
+        sx = int(np.sqrt(spp))
+        ctx.render_device([(scene.camera, sx, spp // sx)], [f.data_ptr()], seed=seed, mode=mode, guide_train=train,
+                          **common)
+        return f
+
+    # reference: plain PT at ref spp, rendered in passes
+    t0 = time.time()
+    ref = torch.zeros((H, W, 3), dtype=torch.float64, device=dev)
+    per = 256
+    for k in range(args.room_ref_spp // per):
+        ref += render(per, 100000 + k, MODE_UNBIASED).double()
+    ref = (ref / (args.room_ref_spp // per)).cpu().numpy()
+    log(f"reference {args.room_ref_spp} spp in {time.time() - t0:.1f} s")
+
+    def progressive(mode, budget, train_passes):
+        acc = torch.zeros((H, W, 3), dtype=torch.float64, device=dev)
+        n = 0
+        curve = []
+        torch.cuda.synchronize()
+        t_start = time.perf_counter()
+        k = 0
+        while True:
+            training = mode == MODE_GUIDED and k < train_passes
+            f = render(args.room_pass_spp, k, mode, train=1 if mode == MODE_GUIDED else 0)
+            torch.cuda.synchronize()
+            el = time.perf_counter() - t_start
+            if not training:
+                acc += f.double()
+                n += 1
+                img = (acc / n).cpu().numpy()
+                curve.append((el, n * args.room_pass_spp, rel_mse(img, ref)))
+            k += 1
+            if el >= budget:
+                break
+        return curve
+
+    plain = progressive(MODE_UNBIASED, args.room_seconds, 0)
+    ctx.guide_init(7)
+    guided = progressive(MODE_GUIDED, args.room_seconds, args.room_train_passes)
+    ctx.close()
+    rp = plain[-1][2]
+    # time at which guided first reaches the plain estimator's final relMSE
+    t_reach = next((t for t, _, e in guided if e <= rp), None)
+    out = {
+        "scene": f"occluded-light room, 12 quads + 8 displaced meshes ({len(arrays.tri_vertices)} triangles), "
+                 f"{W}x{H}, max depth 16, light outside behind a 5% door gap",
+        "budget_s": args.room_seconds,
+        "reference_spp": args.room_ref_spp,
+        "plain": {"relmse": round(rp, 6), "spp": plain[-1][1], "seconds": round(plain[-1][0], 3)},
+        "guided": {"relmse": round(guided[-1][2], 6) if guided else None, "spp": guided[-1][1] if guided else 0,
+                   "seconds": round(guided[-1][0], 3) if guided else None,
+                   "training_passes": args.room_train_passes},
+        "relmse_ratio_guided_over_plain": round(guided[-1][2] / rp, 4) if guided else None,
+        "time_to_plain_relmse_s": round(t_reach, 3) if t_reach else None,
+        "speedup_to_equal_relmse": round(plain[-1][0] / t_reach, 3) if t_reach else None,
+    }
+    return out
 
 
 def cpu_baseline(arrays, scene, budget_s):
@@ -143,6 +238,12 @@ def main():
     if not args.no_cpu:
         log("cpu baseline")
         cpu = cpu_baseline(arrays, scene, args.cpu_seconds)
+    ctx.close()
+
+    guided = None
+    if not args.no_guided:
+        log("equal-time guided vs plain")
+        guided = equal_time_guided(args)
 
     value = rays / dt / 1e6
     out = {
@@ -186,9 +287,9 @@ def main():
             "prims_per_ray": round(cs["prim_tests"] / max(1, ext_rays), 2),
         },
         "cpu_baseline": cpu,
+        "equal_time_guided_vs_plain": guided,
     }
     print(json.dumps(out), flush=True)
-    ctx.close()
 
 
 if __name__ == "__main__":
