@@ -198,7 +198,9 @@ int rt_destroy(rt_ctx* c)
     hipStreamSynchronize(c->stream);
     DevBuf* all[] = {&c->nodes, &c->prims, &c->prim_id, &c->mat, &c->vnormals, &c->geo, &c->mats, &c->lights,
                      &c->env_rgb, &c->env_row, &c->env_col, &c->env_p, &c->rb_o, &c->rb_d, &c->rb_out0,
-                     &c->rb_out1, &c->counters, &c->views, &c->tiles, &c->passes, &c->films};
+                     &c->rb_out1, &c->counters, &c->views, &c->tiles, &c->passes, &c->films, &c->g_theta,
+                     &c->g_w16, &c->g_m, &c->g_v, &c->g_grad, &c->g_acc, &c->g_count, &c->g_rec, &c->g_aux,
+                     &c->g_open, &c->g_out, &c->g_tmp_rec, &c->g_tmp_y};
     for (DevBuf* b : all) release(*b);
     for (auto& b : c->ps_bufs) release(b);
     for (auto e : c->ev_pool) hipEventDestroy(e);
@@ -684,7 +686,7 @@ static int guide_alloc_base(rt_ctx* c)
 static int guide_alloc_slots(rt_ctx* c, size_t slots, int batch)
 {
     int r;
-    size_t cap = (size_t)std::max(batch * 2, 1 << 16);
+    size_t cap = (size_t)GD_TRAIN_CHUNKS * std::max(batch, 1 << 14);
     if ((r = ensure(c, c->g_rec, cap * GD_REC * 4)) || (r = ensure(c, c->g_aux, cap * 16)) ||
         (r = ensure(c, c->g_open, slots * 4)) || (r = ensure(c, c->g_out, slots * GD_PARAMS * 4)))
         return r;
@@ -742,6 +744,8 @@ int rt_guide_init(rt_ctx* c, uint32_t seed)
         std::uniform_real_distribution<float> U(-lim, lim);
         for (int i = 0; i < L[1] * L[2]; ++i) th[L[0] + i] = U(gen);
     }
+    /* guide fraction starts at 0.05 + 0.9 sigmoid(-1.609) = 0.2 */
+    th[GD_OFF_B4 + 40] = -1.609f;
     if (!c->guide_bounds_set)
         for (int k = 0; k < 3; ++k) {
             c->guide.lo[k] = c->scene_lo[k];
@@ -809,18 +813,20 @@ int rt_guide_forward(rt_ctx* c, const float* rec, int32_t n, float* y)
 }
 
 /* Records given by the caller: weights are taken as given (record[13]). */
-static int guide_grad_on(rt_ctx* c, int32_t n)
+static int guide_grad_on(rt_ctx* c, int32_t n, GuideState* used)
 {
     GuideState g = c->guide;
     g.rec = (float*)c->g_tmp_rec.p;
     g.rec_cap = n;
-    HIP_TRY(c, hipMemcpyAsync(g.rec_count, &n, 4, hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(c, hipMemsetAsync(g.acc, 0, 16, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    HIP_TRY(c, hipMemcpy(g.rec_count, &n, 4, hipMemcpyHostToDevice));
+    HIP_TRY(c, launch_guide_zero_acc(g, c->stream));
     PathState none{};
     int grid = std::max(1, std::min(grid_for(c), (n + 255) / 256));
-    HIP_TRY(c, launch_guide_weights(g, none, n, grid, c->stream));
+    HIP_TRY(c, launch_guide_weights(g, none, 0, n, grid, c->stream));
     int tgrid = std::max(1, std::min(grid_for(c), (n + 63) / 64));
-    HIP_TRY(c, launch_guide_train(g, g.rec, g.rec_count, n, tgrid, c->stream));
+    HIP_TRY(c, launch_guide_train(g, g.rec, g.rec_count, 0, n, tgrid, c->stream));
+    if (used) *used = g;
     return 0;
 }
 
@@ -832,7 +838,7 @@ int rt_guide_gradient(rt_ctx* c, const float* rec, int32_t n, float* grad, float
     hipSetDevice(c->device);
     if (int r = guide_records_to_device(c, rec, n)) return r;
     HIP_TRY(c, hipMemsetAsync(c->guide.grad, 0, GD_NPARAM * 4, c->stream));
-    if (int r = guide_grad_on(c, n)) return r;
+    if (int r = guide_grad_on(c, n, nullptr)) return r;
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     HIP_TRY(c, hipMemcpy(grad, c->guide.grad, GD_NPARAM * 4, hipMemcpyDeviceToHost));
     for (int i = 0; i < GD_NPARAM; ++i) grad[i] /= (float)n;
@@ -852,9 +858,10 @@ int rt_guide_train(rt_ctx* c, const float* rec, int32_t n, int32_t steps, float 
     if (int r = guide_records_to_device(c, rec, n)) return r;
     for (int s = 0; s < steps; ++s) {
         HIP_TRY(c, hipMemsetAsync(c->guide.grad, 0, GD_NPARAM * 4, c->stream));
-        if (int r = guide_grad_on(c, n)) return r;
+        GuideState g;
+        if (int r = guide_grad_on(c, n, &g)) return r;
         ++c->guide_step;
-        HIP_TRY(c, launch_guide_adam(c->guide, lr > 0.f ? lr : 1e-3f, c->guide_step, c->guide.rec_count, n, c->stream));
+        HIP_TRY(c, launch_guide_adam(g, lr > 0.f ? lr : 1e-3f, c->guide_step, g.rec_count, 0, n, c->stream));
     }
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     return RT_OK;
@@ -960,7 +967,9 @@ static int render_impl(rt_ctx* c, const rt_view* views, int32_t nv, const rt_par
     const DevPass* d_passes = (const DevPass*)c->passes.p;
 
     PtParams pp{prm->mode, prm->max_depth, prm->rr_start, prm->rr_prob, prm->seed, train ? 1 : 0,
-                (float)std::min(1.0, 2.0 * batch / std::max(1.0, (double)P * 2.0))};
+                /* keep about cap / 6 records per path slot: deep vertices are
+                 * represented as well as first hits (paths average < 6 vertices) */
+                (float)std::min(1.0, (double)GD_TRAIN_CHUNKS * batch / std::max(1.0, (double)P * 6.0))};
     const int grid = grid_for(c);
     const int depth = prm->max_depth;
     const int stack = c->bvh_depth < 24 ? 24 : RT_STACK_DEPTH;
@@ -1005,10 +1014,14 @@ static int render_impl(rt_ctx* c, const rt_view* views, int32_t nv, const rt_par
         }
         LCHK(launch_accumulate(c->ps, d_views, d_tiles, d_passes, grid, c->stream));
         if (train) {
-            ++c->guide_step;
-            LCHK(launch_guide_weights(c->guide, c->ps, batch, grid, c->stream));
-            LCHK(launch_guide_train(c->guide, c->guide.rec, c->guide.rec_count, batch, grid, c->stream));
-            LCHK(launch_guide_adam(c->guide, lr, c->guide_step, c->guide.rec_count, batch, c->stream));
+            /* the pass's records in GD_TRAIN_CHUNKS chunks of `batch`: one Adam step each */
+            for (int k = 0; k < GD_TRAIN_CHUNKS; ++k) {
+                ++c->guide_step;
+                LCHK(launch_guide_zero_acc(c->guide, c->stream));
+                LCHK(launch_guide_weights(c->guide, c->ps, k * batch, batch, grid, c->stream));
+                LCHK(launch_guide_train(c->guide, c->guide.rec, c->guide.rec_count, k * batch, batch, grid, c->stream));
+                LCHK(launch_guide_adam(c->guide, lr, c->guide_step, c->guide.rec_count, k * batch, batch, c->stream));
+            }
             LCHK(launch_guide_reset_records(c->guide, (int)P, c->stream));
         }
         LCHK(launch_pass_end(c->ps, c->stream));

@@ -41,6 +41,7 @@
 #define GD_REC 16        /* floats per training record */
 #define GD_PARAMS 48     /* floats of head output per path */
 #define GD_WCLAMP 100.0f /* cap of a normalised sample weight */
+#define GD_TRAIN_CHUNKS 4 /* training steps per render pass (record capacity = 4 batches) */
 
 /* Training record (16 floats):
  *   0..2 position, 3..5 viewer-side normal, 6..8 outgoing direction,
@@ -69,10 +70,13 @@ struct GuideHost;  /* forward declaration for the ABI layer */
 hipError_t launch_guide_infer(const struct HostDevScene& s, const struct PathState& ps, const GuideState& g,
                               int qsel, int grid, hipStream_t st);
 hipError_t launch_guide_forward(const GuideState& g, const float* rec, int n, float* y, int grid, hipStream_t st);
-hipError_t launch_guide_weights(const GuideState& g, const struct PathState& ps, int max_n, int grid, hipStream_t st);
-hipError_t launch_guide_train(const GuideState& g, const float* rec, const int32_t* count, int max_n,
+/* Training works on chunks of records [off, off + max_n) of the stored ones. */
+hipError_t launch_guide_weights(const GuideState& g, const struct PathState& ps, int off, int max_n, int grid,
+                                hipStream_t st);
+hipError_t launch_guide_zero_acc(const GuideState& g, hipStream_t st);
+hipError_t launch_guide_train(const GuideState& g, const float* rec, const int32_t* count, int off, int max_n,
                               int grid, hipStream_t st);
-hipError_t launch_guide_adam(const GuideState& g, float lr, int step, const int32_t* count, int max_n,
+hipError_t launch_guide_adam(const GuideState& g, float lr, int step, const int32_t* count, int off, int max_n,
                              hipStream_t st);
 hipError_t launch_guide_reset_records(const GuideState& g, int slots, hipStream_t st);
 

@@ -285,12 +285,17 @@ hipError_t launch_guide_forward(const GuideState& g, const float* rec, int n, fl
  * record weights: w = lum(radiance_final - radiance_at_close) / lum(alpha),
  * and the weight sum of the records a training step will use
  * ========================================================================= */
-__global__ __launch_bounds__(GB) void k_guide_weights(GuideState g, PathState ps, int from_paths, int max_n)
+/* records of one training chunk: [off, off + max_n) clipped to the stored ones */
+__device__ __forceinline__ int chunk_len(const GuideState& g, const int32_t* count, int off, int max_n)
 {
-    int N = min(*g.rec_count, g.rec_cap);
-    N = min(N, max_n);
+    return max(0, min(min(*count, g.rec_cap) - off, max_n));
+}
+
+__global__ __launch_bounds__(GB) void k_guide_weights(GuideState g, PathState ps, int from_paths, int off, int max_n)
+{
+    const int N = off + chunk_len(g, g.rec_count, off, max_n);
     float local = 0.f;
-    for (int i = blockIdx.x * GB + threadIdx.x; i < N; i += gridDim.x * GB) {
+    for (int i = off + blockIdx.x * GB + threadIdx.x; i < N; i += gridDim.x * GB) {
         float* r = g.rec + (size_t)i * GD_REC;
         float w;
         if (from_paths) {
@@ -310,11 +315,23 @@ __global__ __launch_bounds__(GB) void k_guide_weights(GuideState g, PathState ps
     if (__lane_id() == 0 && local != 0.f) atomicAdd(&g.acc[0], local);
 }
 
-hipError_t launch_guide_weights(const GuideState& g, const PathState& ps, int max_n, int grid, hipStream_t st)
+hipError_t launch_guide_weights(const GuideState& g, const PathState& ps, int off, int max_n, int grid,
+                                hipStream_t st)
 {
     /* from_paths = 1 when the records come from a render (radiance known) */
-    hipLaunchKernelGGL(k_guide_weights, dim3(grid), dim3(GB), 0, st, g, ps, ps.radiance != nullptr ? 1 : 0,
+    hipLaunchKernelGGL(k_guide_weights, dim3(grid), dim3(GB), 0, st, g, ps, ps.radiance != nullptr ? 1 : 0, off,
                        max_n);
+    return hipGetLastError();
+}
+
+__global__ void k_guide_zero_acc(GuideState g)
+{
+    if (threadIdx.x < 3) g.acc[threadIdx.x] = 0.f;
+}
+
+hipError_t launch_guide_zero_acc(const GuideState& g, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_guide_zero_acc, dim3(1), dim3(64), 0, st, g);
     return hipGetLastError();
 }
 
@@ -338,12 +355,14 @@ __device__ __forceinline__ void grad_tile(int t, int& layer, int& ot, int& it)
     else { layer = 3; ot = (t - 52) / 4; it = (t - 52) % 4; }
 }
 
-__global__ __launch_bounds__(GB) void k_guide_train(GuideState g, const float* rec, const int32_t* count, int max_n)
+__global__ __launch_bounds__(GB) void k_guide_train(GuideState g, const float* rec0, const int32_t* count, int off,
+                                                   int max_n)
 {
     __shared__ TrainLds S;
     load_weights(S.wl, g);
     const int lane = __lane_id(), n = lane & 15, q = lane >> 4, w = threadIdx.x >> 6;
-    const int N = min(min(*count, max_n), g.rec_cap);
+    const int N = chunk_len(g, count, off, max_n);
+    const float* rec = rec0 + (size_t)off * GD_REC;
     const float wsum = g.acc[0];
     const float wscale = wsum > 0.f ? (float)N / wsum : 0.f;
     f4 gacc[16];
@@ -514,10 +533,10 @@ __global__ __launch_bounds__(GB) void k_guide_train(GuideState g, const float* r
     if (lane == 0 && lacc != 0.f) atomicAdd(&g.acc[1], lacc);
 }
 
-hipError_t launch_guide_train(const GuideState& g, const float* rec, const int32_t* count, int max_n, int grid,
-                              hipStream_t st)
+hipError_t launch_guide_train(const GuideState& g, const float* rec, const int32_t* count, int off, int max_n,
+                              int grid, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_guide_train, dim3(grid), dim3(GB), 0, st, g, rec, count, max_n);
+    hipLaunchKernelGGL(k_guide_train, dim3(grid), dim3(GB), 0, st, g, rec, count, off, max_n);
     return hipGetLastError();
 }
 
@@ -525,9 +544,10 @@ hipError_t launch_guide_train(const GuideState& g, const float* rec, const int32
  * Adam (Kingma & Ba, 2015) on the FP32 master copy, refresh FP16 weights,
  * clear gradients and accumulators
  * ========================================================================= */
-__global__ __launch_bounds__(GB) void k_guide_adam(GuideState g, float lr, int step, const int32_t* count, int max_n)
+__global__ __launch_bounds__(GB) void k_guide_adam(GuideState g, float lr, int step, const int32_t* count, int off,
+                                                  int max_n)
 {
-    const int N = min(*count, max_n);
+    const int N = chunk_len(g, count, off, max_n);
     const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
     const float c1 = 1.f - powf(b1, (float)step), c2 = 1.f - powf(b2, (float)step);
     for (int i = blockIdx.x * GB + threadIdx.x; i < GD_NPARAM; i += gridDim.x * GB) {
@@ -550,9 +570,10 @@ __global__ __launch_bounds__(GB) void k_guide_adam(GuideState g, float lr, int s
     }
 }
 
-hipError_t launch_guide_adam(const GuideState& g, float lr, int step, const int32_t* count, int max_n, hipStream_t st)
+hipError_t launch_guide_adam(const GuideState& g, float lr, int step, const int32_t* count, int off, int max_n,
+                             hipStream_t st)
 {
-    hipLaunchKernelGGL(k_guide_adam, dim3(32), dim3(GB), 0, st, g, lr, step, count, max_n);
+    hipLaunchKernelGGL(k_guide_adam, dim3(32), dim3(GB), 0, st, g, lr, step, count, off, max_n);
     return hipGetLastError();
 }
 
