@@ -46,6 +46,8 @@ def parse():
     ap.add_argument("--room-ref-spp", type=int, default=4096)
     ap.add_argument("--room-pass-spp", type=int, default=16)
     ap.add_argument("--room-train-passes", type=int, default=24)
+    ap.add_argument("--room-epochs", type=int, default=1, help="training epochs over each pass's records")
+    ap.add_argument("--room-lr", type=float, default=1e-3)
     return ap.parse_args()
 
 
@@ -72,6 +74,7 @@ def equal_time_guided(args):
     log(f"room: {len(arrays.tri_vertices)} triangles, {W}x{H}, built in {time.time() - t0:.1f} s")
     ctx = GpuContext(0)
     ctx.upload(arrays)
+    ctx.set_launch_config(guide_epochs=args.room_epochs)
     common = dict(max_depth=16, rr_start=3, rr_prob=0.5)
     dev = "cuda:0"
 
@@ -80,7 +83,7 @@ def equal_time_guided(args):
         torch.cuda.synchronize()
         sx = int(np.sqrt(spp))
         ctx.render_device([(scene.camera, sx, spp // sx)], [f.data_ptr()], seed=seed, mode=mode, guide_train=train,
-                          **common)
+                          guide_lr=args.room_lr, **common)
         return f
 
     # reference: plain PT at ref spp, rendered in passes

@@ -117,6 +117,7 @@ typedef struct rt_launch_config {
     int32_t blocks_per_cu;
     int32_t use_graph;     /* 1 = replay the per-pass kernel sequence as a hipGraph */
     int32_t count_traversal; /* 1 = closest-hit kernels also count nodes and primitives */
+    int32_t guide_epochs;  /* GUIDED + training: passes over each render pass's records (0 = 1) */
 } rt_launch_config;
 int rt_set_launch_config(rt_ctx* ctx, const rt_launch_config* cfg);
 int rt_get_launch_config(rt_ctx* ctx, rt_launch_config* cfg);

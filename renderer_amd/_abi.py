@@ -79,7 +79,7 @@ class RtStats(ctypes.Structure):
 class RtLaunchConfig(ctypes.Structure):
     _fields_ = [("path_slots", ctypes.c_int32), ("block", ctypes.c_int32), ("blocks_per_cu", ctypes.c_int32),
                 ("use_graph", ctypes.c_int32),
-                ("count_traversal", ctypes.c_int32)]
+                ("count_traversal", ctypes.c_int32), ("guide_epochs", ctypes.c_int32)]
 
 
 MAT_DIFFUSE, MAT_MIRROR, MAT_PHONG, MAT_DARK = 0, 1, 2, 3

@@ -54,7 +54,7 @@ struct rt_ctx {
     /* defaults from a sweep on MI355X (1M-triangle mesh, 1024^2 x 64 spp):
      * 16M path slots per pass, 12 persistent blocks per CU (two rounds of
      * the 6 blocks whose traversal stacks fit one CU's LDS) */
-    rt_launch_config cfg{0, 256, 12, 0, 0};
+    rt_launch_config cfg{0, 256, 12, 0, 0, 1};
 
     // guiding network
     GuideState guide{};
@@ -236,6 +236,7 @@ int rt_set_launch_config(rt_ctx* c, const rt_launch_config* cfg)
     if (cfg->blocks_per_cu) c->cfg.blocks_per_cu = std::min(cfg->blocks_per_cu, 64);
     c->cfg.use_graph = cfg->use_graph;
     c->cfg.count_traversal = cfg->count_traversal;
+    if (cfg->guide_epochs > 0) c->cfg.guide_epochs = std::min(cfg->guide_epochs, 64);
     return RT_OK;
 }
 
@@ -1015,12 +1016,12 @@ static int render_impl(rt_ctx* c, const rt_view* views, int32_t nv, const rt_par
         LCHK(launch_accumulate(c->ps, d_views, d_tiles, d_passes, grid, c->stream));
         if (train) {
             /* the pass's records in GD_TRAIN_CHUNKS chunks of `batch`: one Adam step each */
-            for (int k = 0; k < GD_TRAIN_CHUNKS; ++k) {
+            for (int k = 0; k < GD_TRAIN_CHUNKS * std::max(1, c->cfg.guide_epochs); ++k) {
                 ++c->guide_step;
                 LCHK(launch_guide_zero_acc(c->guide, c->stream));
-                LCHK(launch_guide_weights(c->guide, c->ps, k * batch, batch, grid, c->stream));
-                LCHK(launch_guide_train(c->guide, c->guide.rec, c->guide.rec_count, k * batch, batch, grid, c->stream));
-                LCHK(launch_guide_adam(c->guide, lr, c->guide_step, c->guide.rec_count, k * batch, batch, c->stream));
+                LCHK(launch_guide_weights(c->guide, c->ps, (k % GD_TRAIN_CHUNKS) * batch, batch, grid, c->stream));
+                LCHK(launch_guide_train(c->guide, c->guide.rec, c->guide.rec_count, (k % GD_TRAIN_CHUNKS) * batch, batch, grid, c->stream));
+                LCHK(launch_guide_adam(c->guide, lr, c->guide_step, c->guide.rec_count, (k % GD_TRAIN_CHUNKS) * batch, batch, c->stream));
             }
             LCHK(launch_guide_reset_records(c->guide, (int)P, c->stream));
         }

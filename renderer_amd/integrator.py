@@ -127,8 +127,10 @@ class GpuContext:
         p = self.make_params(**params)
         _check(self.lib, self.h, self.lib.rt_render_device(self.h, va, len(views), ctypes.byref(p), ptrs))
 
-    def set_launch_config(self, path_slots=0, block=0, blocks_per_cu=0, use_graph=0, count_traversal=0):
-        c = _abi.RtLaunchConfig(int(path_slots), int(block), int(blocks_per_cu), int(use_graph), int(count_traversal))
+    def set_launch_config(self, path_slots=0, block=0, blocks_per_cu=0, use_graph=0, count_traversal=0,
+                          guide_epochs=0):
+        c = _abi.RtLaunchConfig(int(path_slots), int(block), int(blocks_per_cu), int(use_graph), int(count_traversal),
+                                int(guide_epochs))
         _check(self.lib, self.h, self.lib.rt_set_launch_config(self.h, ctypes.byref(c)))
 
     def launch_config(self):
