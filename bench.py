@@ -40,6 +40,7 @@ def parse():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 1 GiB device-to-device copy, the reference point for HBM counters")
     ap.add_argument("--no-guided", action="store_true", help="skip the equal-time guided vs plain comparison")
+    ap.add_argument("--no-batched", action="store_true", help="skip the 32-view batched render")
     ap.add_argument("--room-res", type=str, default="480x270")
     ap.add_argument("--room-subdiv", type=int, default=64, help="octahedron segments per edge of the 8 objects")
     ap.add_argument("--room-seconds", type=float, default=6.0, help="wall-time budget of each estimator")
@@ -54,6 +55,47 @@ def parse():
 def rel_mse(img, ref, eps=1e-2):
     """mean over pixels and channels of (I - R)^2 / (R^2 + eps)."""
     return float(((img - ref) ** 2 / (ref ** 2 + eps)).mean())
+
+
+def batched_views(args, scene, arrays):
+    """32 seeded views of the mesh scene (resolutions from {256^2, 512x384,
+    1024x768, 1920x1080}, spp uniform in [8, 128], eyes on a sphere of radius
+    3) traced together in one rt_render pipeline, with the launch
+    configuration picked by the autotuner.  Returns Mrays/s and the choice."""
+    import torch
+
+    from renderer_amd import MODE_UNBIASED, GpuContext, scenes
+    from renderer_amd.autotune import autotune
+
+    views = scenes.batched_views(32, seed=7)
+    ctx = GpuContext(0)
+    ctx.upload(arrays)
+    params = dict(mode=MODE_UNBIASED, max_depth=8, rr_start=3, rr_prob=0.5)
+    t0 = time.time()
+    tune = autotune(ctx, views, params, spp_divisor=16, repeats=1)
+    tune_s = time.time() - t0
+    films = [torch.zeros((c.height, c.width, 3), dtype=torch.float32, device="cuda:0") for c, _, _ in views]
+    torch.cuda.synchronize()
+    ctx.render_device(views, [f.data_ptr() for f in films], seed=1, **params)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ctx.render_device(views, [f.data_ptr() for f in films], seed=2, **params)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    st = ctx.stats()
+    ctx.close()
+    rays = st["camera_rays"] + st["extension_rays"] + st["shadow_rays"]
+    return {
+        "views": len(views),
+        "pixels": int(sum(c.width * c.height for c, _, _ in views)),
+        "paths": int(sum(c.width * c.height * sx * sy for c, sx, sy in views)),
+        "seconds": round(dt, 3),
+        "mrays_per_s": round(rays / dt / 1e6, 2),
+        "rays": int(rays),
+        "passes": st["passes"],
+        "autotuned": {"path_slots": tune["path_slots"], "blocks_per_cu": tune["blocks_per_cu"],
+                      "trials": len(tune["trials"]), "tuning_seconds": round(tune_s, 1)},
+    }
 
 
 def equal_time_guided(args):
@@ -99,17 +141,20 @@ def equal_time_guided(args):
         acc = torch.zeros((H, W, 3), dtype=torch.float64, device=dev)
         n = 0
         curve = []
-        torch.cuda.synchronize()
-        t_start = time.perf_counter()
         k = 0
+        # every pass is an unbiased estimate (the guided one for any network
+        # state), so passes rendered while the network trains count too;
+        # train_passes only delays the first score
+        el = 0.0  # render (and training) time only; scoring is not timed
         while True:
-            training = mode == MODE_GUIDED and k < train_passes
-            f = render(args.room_pass_spp, k, mode, train=1 if mode == MODE_GUIDED else 0)
             torch.cuda.synchronize()
-            el = time.perf_counter() - t_start
-            if not training:
-                acc += f.double()
-                n += 1
+            t_start = time.perf_counter()
+            f = render(args.room_pass_spp, k, mode, train=1 if mode == MODE_GUIDED else 0)
+            acc += f.double()
+            torch.cuda.synchronize()
+            el += time.perf_counter() - t_start
+            n += 1
+            if k >= train_passes:
                 img = (acc / n).cpu().numpy()
                 curve.append((el, n * args.room_pass_spp, rel_mse(img, ref)))
             k += 1
@@ -243,6 +288,11 @@ def main():
         cpu = cpu_baseline(arrays, scene, args.cpu_seconds)
     ctx.close()
 
+    batched = None
+    if not args.no_batched:
+        log("batched views")
+        batched = batched_views(args, scene, arrays)
+
     guided = None
     if not args.no_guided:
         log("equal-time guided vs plain")
@@ -290,6 +340,7 @@ def main():
             "prims_per_ray": round(cs["prim_tests"] / max(1, ext_rays), 2),
         },
         "cpu_baseline": cpu,
+        "batched_views": batched,
         "equal_time_guided_vs_plain": guided,
     }
     print(json.dumps(out), flush=True)
