@@ -43,8 +43,8 @@ def parse():
     ap.add_argument("--no-batched", action="store_true", help="skip the 32-view batched render")
     ap.add_argument("--room-res", type=str, default="480x270")
     ap.add_argument("--room-subdiv", type=int, default=64, help="octahedron segments per edge of the 8 objects")
-    ap.add_argument("--room-seconds", type=float, default=6.0, help="wall-time budget of each estimator")
-    ap.add_argument("--room-ref-spp", type=int, default=4096)
+    ap.add_argument("--room-seconds", type=float, default=2.0, help="wall-time budget of each estimator")
+    ap.add_argument("--room-ref-spp", type=int, default=16384)
     ap.add_argument("--room-pass-spp", type=int, default=16)
     ap.add_argument("--room-train-passes", type=int, default=24)
     ap.add_argument("--room-epochs", type=int, default=1, help="training epochs over each pass's records")

@@ -40,5 +40,24 @@ def build_library(verbose=False):
     return LIB
 
 
+EXAMPLE = os.path.join(HERE, "host", "example_box")
+
+
+def build_host_example(verbose=False):
+    """The C++ host example (renderer_amd/host/example_box.cpp), linked
+    against librt_hip.so with an rpath to the package directory."""
+    cmd = ["g++", "-O2", "-std=c++17", "-I", os.path.join(HERE, "..", "include"),
+           os.path.join(HERE, "host", "example_box.cpp"), "-L", HERE, "-lrt_hip", "-Wl,-rpath,$ORIGIN/..",
+           "-o", EXAMPLE]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError("g++ failed building the host example")
+    return EXAMPLE
+
+
 if __name__ == "__main__":
     print(build_library(verbose=True))
+    print(build_host_example(verbose=True))

@@ -1,0 +1,246 @@
+/*
+ * gpu_integrator.h — C++ host interface over librt_hip.so, shaped like the
+ * reference renderer's own classes:
+ *
+ *   SceneBuilder                  the objects of a scene file (materials,
+ *                                 quads, triangles, meshes, spheres, lights)
+ *                                 in the reference's root-list order
+ *                                 (src/sceneast.cpp:673-695)
+ *   Camera                        src/camera.h:34-101 (eye, look-at, up,
+ *                                 vertical fov in radians, resolution)
+ *   Integrator                    the integrator interface, batch form
+ *                                 (src/integrator.h:32-37)
+ *   GpuUniDiPathTracingIntegrator UniDiPathTracingIntegrator(terminationProb)
+ *                                 (src/unidipathtracingintegrator.h:30) on the GPU
+ *   PerPixelRenderer              PerPixelRenderer::render (src/perpixelrenderer.cpp:43-101)
+ *                                 with pt-x-samples / pt-y-samples / seed
+ *
+ * Header-only; link with -lrt_hip.  Errors throw RtError (C++ callers) while
+ * the library itself only returns status codes.
+ */
+#ifndef RENDERER_AMD_GPU_INTEGRATOR_H
+#define RENDERER_AMD_GPU_INTEGRATOR_H
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/rt_abi.h"
+
+namespace rtgpu {
+
+struct RtError : std::runtime_error {
+    int status;
+    RtError(int s, const std::string& what) : std::runtime_error(what), status(s) {}
+};
+
+inline void check(rt_ctx* ctx, int status, const char* call)
+{
+    if (status != RT_OK)
+        throw RtError(status, std::string(call) + ": " + rt_error_string(status) + (ctx ? std::string(" (") + rt_last_error(ctx) + ")" : ""));
+}
+
+struct Vec3 {
+    float x, y, z;
+};
+
+/* Scene objects in file order; primitive IDs follow the reference's root
+ * list: lights first, then intersectables in insertion order. */
+class SceneBuilder {
+public:
+    int diffuse(Vec3 rgb) { return material(RT_MAT_DIFFUSE, rgb, Vec3{0, 0, 0}, 0.f); }
+    int mirror() { return material(RT_MAT_MIRROR, Vec3{0, 0, 0}, Vec3{0, 0, 0}, 0.f); }
+    int phong(Vec3 kd, Vec3 ks, float n) { return material(RT_MAT_PHONG, kd, ks, n); }
+
+    void areaLight(Vec3 o, Vec3 u, Vec3 v, Vec3 intensity) { light(RT_LIGHT_AREA, o, u, v, intensity); }
+    void pointLight(Vec3 p, Vec3 intensity) { light(RT_LIGHT_POINT, p, Vec3{0, 0, 0}, Vec3{0, 0, 0}, intensity); }
+    /* latitude-longitude RGB image, row 0 at +y */
+    void environmentMap(int w, int h, const std::vector<float>& rgb, Vec3 coefficient)
+    {
+        env_w_ = w;
+        env_h_ = h;
+        env_ = rgb;
+        light(RT_LIGHT_ENV, Vec3{0, 0, 0}, Vec3{0, 0, 0}, Vec3{0, 0, 0}, coefficient);
+    }
+
+    /* src/quad.h:32: triangles (p1,p2,p4), (p4,p2,p3), one primitive ID */
+    void quad(Vec3 p1, Vec3 p2, Vec3 p3, Vec3 p4, int mat)
+    {
+        int id = nextId();
+        int b = vertex(p1), c = vertex(p2), d = vertex(p3), e = vertex(p4);
+        tri(b, c, e, mat, id);
+        tri(e, c, d, mat, id);
+    }
+    void triangle(Vec3 p1, Vec3 p2, Vec3 p3, int mat) { tri(vertex(p1), vertex(p2), vertex(p3), mat, nextId()); }
+    void sphere(Vec3 c, float r, int mat)
+    {
+        spheres_.insert(spheres_.end(), {c.x, c.y, c.z, r});
+        sphere_mat_.push_back(mat);
+        sphere_id_.push_back(nextId());
+    }
+    /* indexed mesh: every triangle is its own primitive (as an OBJ import) */
+    void mesh(const std::vector<float>& positions, const std::vector<int32_t>& faces, int mat)
+    {
+        int base = (int)(pos_.size() / 3);
+        pos_.insert(pos_.end(), positions.begin(), positions.end());
+        for (size_t i = 0; i + 2 < faces.size(); i += 3) tri(base + faces[i], base + faces[i + 1], base + faces[i + 2], mat, nextId());
+    }
+
+    /* Pointer-free view of the arrays (valid while the builder lives and is
+     * not modified). */
+    rt_scene_desc desc() const
+    {
+        rt_scene_desc d{};
+        d.n_vertices = (int32_t)(pos_.size() / 3);
+        d.positions = pos_.data();
+        d.normals = nullptr;
+        d.n_triangles = (int32_t)tri_mat_.size();
+        d.tri_vertices = tri_v_.data();
+        d.tri_material = tri_mat_.data();
+        d.tri_prim_id = tri_id_.data();
+        d.n_spheres = (int32_t)sphere_mat_.size();
+        d.spheres = spheres_.data();
+        d.sphere_material = sphere_mat_.data();
+        d.sphere_prim_id = sphere_id_.data();
+        d.n_materials = (int32_t)mats_.size();
+        d.materials = mats_.data();
+        d.n_lights = (int32_t)lights_.size();
+        d.lights = lights_.data();
+        d.env_width = env_w_;
+        d.env_height = env_h_;
+        d.env_rgb = env_.empty() ? nullptr : env_.data();
+        return d;
+    }
+
+private:
+    int material(int type, Vec3 kd, Vec3 ks, float n)
+    {
+        rt_material m{};
+        m.type = type;
+        m.kd[0] = kd.x; m.kd[1] = kd.y; m.kd[2] = kd.z;
+        m.ks[0] = ks.x; m.ks[1] = ks.y; m.ks[2] = ks.z;
+        m.exponent = n;
+        mats_.push_back(m);
+        return (int)mats_.size() - 1;
+    }
+    void light(int type, Vec3 p, Vec3 u, Vec3 v, Vec3 i)
+    {
+        if (n_objects_ > 0) throw std::logic_error("lights precede intersectables in the root list");
+        rt_light l{};
+        l.type = type;
+        l.p[0] = p.x; l.p[1] = p.y; l.p[2] = p.z;
+        l.u[0] = u.x; l.u[1] = u.y; l.u[2] = u.z;
+        l.v[0] = v.x; l.v[1] = v.y; l.v[2] = v.z;
+        l.intensity[0] = i.x; l.intensity[1] = i.y; l.intensity[2] = i.z;
+        lights_.push_back(l);
+    }
+    int vertex(Vec3 p)
+    {
+        pos_.insert(pos_.end(), {p.x, p.y, p.z});
+        return (int)(pos_.size() / 3) - 1;
+    }
+    void tri(int a, int b, int c, int mat, int id)
+    {
+        tri_v_.insert(tri_v_.end(), {a, b, c});
+        tri_mat_.push_back(mat);
+        tri_id_.push_back(id);
+    }
+    int nextId() { return (int)lights_.size() + n_objects_++; }
+
+    std::vector<float> pos_, spheres_, env_;
+    std::vector<int32_t> tri_v_, tri_mat_, tri_id_, sphere_mat_, sphere_id_;
+    std::vector<rt_material> mats_;
+    std::vector<rt_light> lights_;
+    int n_objects_ = 0, env_w_ = 0, env_h_ = 0;
+};
+
+struct Camera {
+    Vec3 cop, lookAt, up;
+    float fov;
+    int width, height;
+    rt_camera c() const
+    {
+        rt_camera r{};
+        r.eye[0] = cop.x; r.eye[1] = cop.y; r.eye[2] = cop.z;
+        r.look_at[0] = lookAt.x; r.look_at[1] = lookAt.y; r.look_at[2] = lookAt.z;
+        r.up[0] = up.x; r.up[1] = up.y; r.up[2] = up.z;
+        r.fov = fov;
+        r.width = width;
+        r.height = height;
+        return r;
+    }
+};
+
+/* Batch form of the reference's Integrator: one call renders a film. */
+class Integrator {
+public:
+    virtual ~Integrator() {}
+    virtual void render(const Camera& cam, int sppX, int sppY, unsigned long seed, float* film) = 0;
+};
+
+class GpuUniDiPathTracingIntegrator : public Integrator {
+public:
+    explicit GpuUniDiPathTracingIntegrator(float terminationProb = 0.5f, int mode = RT_MODE_COMPAT, int maxDepth = 5,
+                                           int device = 0)
+        : p_(terminationProb), mode_(mode), depth_(maxDepth)
+    {
+        check(nullptr, rt_create(device, &ctx_), "rt_create");
+    }
+    ~GpuUniDiPathTracingIntegrator() override { rt_destroy(ctx_); }
+    GpuUniDiPathTracingIntegrator(const GpuUniDiPathTracingIntegrator&) = delete;
+    GpuUniDiPathTracingIntegrator& operator=(const GpuUniDiPathTracingIntegrator&) = delete;
+
+    void setScene(const SceneBuilder& scene)
+    {
+        rt_scene_desc d = scene.desc();
+        check(ctx_, rt_scene_upload(ctx_, &d), "rt_scene_upload");
+    }
+    /* film: height * width * 3 RGB floats, accumulated as film += sample / spp */
+    void render(const Camera& cam, int sppX, int sppY, unsigned long seed, float* film) override
+    {
+        rt_view v{};
+        v.camera = cam.c();
+        v.spp_x = sppX;
+        v.spp_y = sppY;
+        rt_params p{};
+        p.mode = mode_;
+        p.max_depth = depth_;
+        p.rr_start = 3;
+        p.rr_prob = p_;
+        p.seed = (uint32_t)seed;
+        float* films[1] = {film};
+        check(ctx_, rt_render(ctx_, &v, 1, &p, films), "rt_render");
+    }
+    void stop() { rt_request_stop(ctx_, 1); }
+    rt_ctx* context() { return ctx_; }
+
+private:
+    rt_ctx* ctx_ = nullptr;
+    float p_;
+    int mode_, depth_;
+};
+
+/* PerPixelRenderer with the reference's options: pt-x-samples, pt-y-samples, seed. */
+class PerPixelRenderer {
+public:
+    PerPixelRenderer(Integrator& integrator, int xSamples = 4, int ySamples = 4, unsigned long seed = 0)
+        : integ_(integrator), sx_(xSamples), sy_(ySamples), seed_(seed)
+    {
+    }
+    std::vector<float> render(const Camera& cam)
+    {
+        std::vector<float> film((size_t)cam.width * cam.height * 3, 0.f);
+        integ_.render(cam, sx_, sy_, seed_, film.data());
+        return film;
+    }
+
+private:
+    Integrator& integ_;
+    int sx_, sy_;
+    unsigned long seed_;
+};
+
+}  // namespace rtgpu
+
+#endif
