@@ -887,25 +887,48 @@ bool bsdf_sample(const Material* m, V3 wo, V3 nf, float u0, float u1, float u2, 
     return false;
 }
 
+// Power heuristic with exponent 2 (Veach, "Robust Monte Carlo Methods for
+// Light Transport Simulation", 1997, sec. 9.2.1): weight of a sample drawn
+// with density a when a second strategy has density b there.  Requires a > 0.
+float mis_power(float a, float b)
+{
+    float r = b / a;
+    return 1.f / (1.f + r * r);
+}
+
+// Next-event estimation and BSDF sampling are combined with the power
+// heuristic.  The next-event weight is 1 at the last vertex (its BSDF ray is
+// not traced) and for point lights; emission reached by a camera ray or
+// through mirrors only has weight 1.
 Col integrate_unbiased(const UnbiasedCtx& c, Ray ray, const SampleKey& k, int64_t& rays)
 {
     const Scene& sc = *c.sc;
     Col L, alpha(1, 1, 1);
     bool spec = true;
+    float pdf_prev = 0;  // solid-angle density of the current ray's direction (non-specular vertex)
     const int nl = (int)sc.lights.size();
     for (int depth = 0; depth < c.max_depth; ++depth) {
         ++rays;
         Hit hit = sc.intersect(ray);
         if (!hit.hits()) break;
         if (hit.mat->emits()) {
-            if (spec) {
-                if (hit.pw == 0) {
-                    L = L + mul(alpha, static_cast<const EnvMap*>(hit.mat)->get(ray.d));
-                } else {
-                    auto* al = static_cast<const AreaLight*>(hit.mat);
-                    if (dot(ray.d, al->nrm) < 0) L = L + mul(alpha, al->I);
+            Col e;
+            float pl = 0;
+            if (hit.pw == 0) {
+                auto* em = static_cast<const EnvMap*>(hit.mat);
+                e = em->get(ray.d);
+                if (!spec) pl = em->pdf(ray.d) / (float)nl;
+            } else {
+                auto* al = static_cast<const AreaLight*>(hit.mat);
+                float dn = dot(ray.d, al->nrm);
+                if (dn < 0) {
+                    e = al->I;
+                    // area density t^2 / (A cos) with A cos = -dot(d, u x v)
+                    if (!spec) pl = hit.t * hit.t / (-dn) / (float)nl;
                 }
             }
+            float w = spec ? 1.f : mis_power(pdf_prev, pl);
+            L = L + mul(alpha, e) * w;
             break;
         }
         const Material* m = hit.mat;
@@ -922,11 +945,13 @@ Col integrate_unbiased(const UnbiasedCtx& c, Ray ray, const SampleKey& k, int64_
         V3 wo = -d;
         U4 gb = philox_uniforms(k.seed, k.view, k.pixel, k.sample, 1 + 2 * depth);
         U4 gl = philox_uniforms(k.seed, k.view, k.pixel, k.sample, 2 + 2 * depth);
+        const bool last = depth + 1 >= c.max_depth;
         if (nl > 0) {
             int j = std::min((int)(gl.u[2] * (float)nl), nl - 1);
             const Light* lt = sc.lights[j];
             V3 wi;
             float dist = kInf;
+            float pl = 0;  // solid-angle density incl. the 1/nl light choice; 0 for a point light
             Col le;
             bool ok = false;
             if (auto* al = dynamic_cast<const AreaLight*>(lt)) {
@@ -937,23 +962,32 @@ Col integrate_unbiased(const UnbiasedCtx& c, Ray ray, const SampleKey& k, int64_
                 wi = dv * (1 / dist);
                 float an = std::sqrt(dot(al->nrm, al->nrm));
                 float cl = -dot(wi, al->nrm) / an;
-                if (cl > 0) { le = al->I * (cl * an / d2); ok = true; }
-            } else if (auto* pl = dynamic_cast<const PointLight*>(lt)) {
-                V3 dv = pl->p - x;
+                if (cl > 0) {
+                    le = al->I * (cl * an / d2);
+                    pl = d2 / (cl * an) / (float)nl;
+                    ok = true;
+                }
+            } else if (auto* pt = dynamic_cast<const PointLight*>(lt)) {
+                V3 dv = pt->p - x;
                 float d2 = dot(dv, dv);
                 dist = std::sqrt(d2);
                 wi = dv * (1 / dist);
-                le = pl->I * (1 / d2);
+                le = pt->I * (1 / d2);
                 ok = true;
             } else if (auto* em = dynamic_cast<const EnvMap*>(lt)) {
                 float pdf;
                 em->sample(gl.u[0], gl.u[1], wi, pdf);
-                if (pdf > 0) { le = em->get(wi) * (1 / pdf); ok = true; }
+                if (pdf > 0) {
+                    le = em->get(wi) * (1 / pdf);
+                    pl = pdf / (float)nl;
+                    ok = true;
+                }
             }
             float cx = dot(wi, nf);
             if (ok && cx > 0) {
+                float w = (pl > 0 && !last) ? mis_power(pl, bsdf_pdf(m, wo, wi, nf)) : 1.f;
                 Col f = bsdf_eval(m, wo, wi, nf);
-                Col contrib = mul(mul(alpha, f), le) * (cx * (float)nl);
+                Col contrib = mul(mul(alpha, f), le) * (cx * (float)nl * w);
                 if (!contrib.zero()) {
                     float tmax = std::isinf(dist) ? kInf : dist - kEps;
                     ++rays;
@@ -969,6 +1003,7 @@ Col integrate_unbiased(const UnbiasedCtx& c, Ray ray, const SampleKey& k, int64_
         Col f = bsdf_eval(m, wo, wi, nf);
         alpha = mul(alpha, f) * (dot(wi, nf) / pdf);
         spec = false;
+        pdf_prev = pdf;
         if (depth + 1 >= c.rr_start) {
             if (gb.u[3] < c.rr_prob) break;
             alpha = alpha * (1 / (1 - c.rr_prob));

@@ -188,6 +188,20 @@ __device__ __forceinline__ vec3 env_sample(const DevScene& S, float u0, float u1
     pdf = st > 0.f ? p * (float)S.env_w * (float)S.env_h / (2.f * RT_PI * RT_PI * st) : 0.f;
     return mk(st * sp, ct, -st * cp);
 }
+/* Solid-angle density with which env_sample returns the unit direction d. */
+__device__ __forceinline__ float env_pdf(const DevScene& S, vec3 d)
+{
+    float st = sqrtf(fmaxf(0.f, 1.f - d.y * d.y));
+    float p = S.env_texel_p[env_texel(S, d)];
+    return st > 0.f ? p * (float)S.env_w * (float)S.env_h / (2.f * RT_PI * RT_PI * st) : 0.f;
+}
+/* Power heuristic (exponent 2): weight of a sample drawn with density a when
+ * a second strategy could have drawn it with density b.  Requires a > 0. */
+__device__ __forceinline__ float mis_power(float a, float b)
+{
+    float r = b / a;
+    return 1.f / (1.f + r * r);
+}
 
 /* Area-light intensity towards `direction` (from the light to the point),
  * src/arealight.cpp:53-59: I * max(0, cos) * |n| / |direction|^2. */
@@ -278,6 +292,82 @@ __device__ __forceinline__ int sgn(float x) { return (0.f < x) - (x < 0.f); }
 /* Packed path bookkeeping in info.w: stored vertices (8 bits), path length
  * (8 bits), specular-chain flag (bit 16). */
 __device__ __forceinline__ int pk(int k, int L, int spec) { return k | (L << 8) | (spec << 16); }
+
+/* Next-event sample of the unbiased estimators: picks light floor(u2 N) and a
+ * direction towards it.  Returns false when nothing can arrive.  `le` is the
+ * light's contribution over the direction density (without the 1/N of the
+ * light choice), `pl` the solid-angle density of the direction including the
+ * 1/N (0 for a point light, which no other strategy can hit). */
+__device__ __forceinline__ bool light_sample(const DevScene& S, vec3 x, const U4& gl, vec3& wi, vec3& le,
+                                             float& dist, float& pl)
+{
+    int li = min((int)(gl.u[2] * (float)S.n_lights), S.n_lights - 1);
+    const RtLight& Lt = S.lights[li];
+    const float nl = (float)S.n_lights;
+    dist = RT_INF;
+    pl = 0.f;
+    if (Lt.type == RT_LIGHT_AREA) {
+        vec3 y = ld3(Lt.p) + ld3(Lt.u) * gl.u[0] + ld3(Lt.v) * gl.u[1];
+        vec3 dv = y - x;
+        float d2 = dot(dv, dv);
+        dist = sqrtf(d2);
+        wi = dv * (1.f / dist);
+        vec3 ln = ld3(Lt.n);
+        float an = sqrtf(dot(ln, ln));
+        float cl = -dot(wi, ln) / an;
+        if (!(cl > 0.f)) return false;
+        le = ld3(Lt.I) * (cl * an / d2);
+        pl = d2 / (cl * an) / nl;
+        return true;
+    }
+    if (Lt.type == RT_LIGHT_POINT) {
+        vec3 dv = ld3(Lt.p) - x;
+        float d2 = dot(dv, dv);
+        dist = sqrtf(d2);
+        wi = dv * (1.f / dist);
+        le = ld3(Lt.I) * (1.f / d2);
+        return true;
+    }
+    float pdf;
+    wi = env_sample(S, gl.u[0], gl.u[1], pdf);
+    if (!(pdf > 0.f)) return false;
+    le = env_value(S, wi) * (1.f / pdf);
+    pl = pdf / nl;
+    return true;
+}
+
+/* Emission reached by an extension ray of the unbiased estimators: the
+ * environment at a miss, or a front-facing area light.  Camera rays and rays
+ * leaving a mirror (`spec`) count it in full.  A ray sampled at a
+ * non-specular vertex carries its sampling density in alpha.w and counts the
+ * emission with the power-heuristic weight against next-event estimation,
+ * which could have picked the same direction.  `d` is the unit ray direction
+ * and `t` the hit distance.  Returns true when the path ends here. */
+__device__ __forceinline__ bool add_emission(const DevScene& S, const PathState& ps, int p, int g, bool spec,
+                                             float t, vec3 d, float4 al)
+{
+    vec3 e;
+    float pl = 0.f;
+    if (g < 0) {
+        if (S.env_light < 0) return true;
+        e = env_value(S, d);
+        if (!spec) pl = env_pdf(S, d) / (float)S.n_lights;
+    } else {
+        int m = S.mat[g];
+        if (m >= 0) return false;
+        const RtLight& Lt = S.lights[-m - 1];
+        float dn = dot(d, ld3(Lt.n));
+        if (!(dn < 0.f)) return true;
+        e = ld3(Lt.I);
+        /* area density t^2 / (A cos) with A cos = -dot(d, u x v) */
+        if (!spec) pl = t * t / (-dn) / (float)S.n_lights;
+    }
+    float w = spec ? 1.f : mis_power(al.w, pl);
+    e = mul3(xyz(al), e) * w;
+    float4 r = ps.radiance[p];
+    ps.radiance[p] = make_float4(r.x + e.x, r.y + e.y, r.z + e.z, 0.f);
+    return true;
+}
 
 /* ===========================================================================
  * generate
@@ -588,8 +678,10 @@ __device__ void shade_compat(const DevScene& S, const PathState& ps, const PtPar
 }
 
 /* ===========================================================================
- * shade, unbiased estimator (next-event estimation with the light-selection
- * weight, emission on camera / specular chains, compensated roulette).
+ * shade, unbiased estimator: next-event estimation and BSDF sampling combined
+ * by multiple importance sampling (power heuristic), compensated roulette.
+ * The next-event weight is 1 at the last vertex, whose BSDF ray is not
+ * traced, and for point lights.
  * ========================================================================= */
 __device__ void shade_unbiased(const DevScene& S, const PathState& ps, const PtParams& P, int depth, int p,
                                bool& cont, bool& shadow, float4& so, float4& sd, float4& sc)
@@ -604,25 +696,8 @@ __device__ void shade_unbiased(const DevScene& S, const PathState& ps, const PtP
     vec3 o = xyz(ro), d = xyz(rd);
     float4 al = ps.alpha[p];
     vec3 alpha = xyz(al);
-    if (g < 0) {
-        if (S.env_light >= 0 && spec) {
-            vec3 e = mul3(alpha, env_value(S, d));
-            float4 r = ps.radiance[p];
-            ps.radiance[p] = make_float4(r.x + e.x, r.y + e.y, r.z + e.z, 0.f);
-        }
-        return;
-    }
-    int m = S.mat[g];
-    if (m < 0) {
-        const RtLight& Lt = S.lights[-m - 1];
-        if (spec && dot(d, ld3(Lt.n)) < 0.f) {
-            vec3 e = mul3(alpha, ld3(Lt.I));
-            float4 r = ps.radiance[p];
-            ps.radiance[p] = make_float4(r.x + e.x, r.y + e.y, r.z + e.z, 0.f);
-        }
-        return;
-    }
-    const RtMaterial& M = S.mats[m];
+    if (add_emission(S, ps, p, g, spec, h.x, d, al)) return;
+    const RtMaterial& M = S.mats[S.mat[g]];
     vec3 x = o + h.x * d;
     vec3 n = hit_normal(S, g, x, h.z, h.w);
     vec3 nf = dot(n, d) > 0.f ? -n : n;
@@ -640,53 +715,26 @@ __device__ void shade_unbiased(const DevScene& S, const PathState& ps, const PtP
     vec3 wo = -d;
     U4 gb = rng4(P.seed, (uint32_t)info.z, (uint32_t)info.x, (uint32_t)info.y, 1u + 2u * depth);
     U4 gl = rng4(P.seed, (uint32_t)info.z, (uint32_t)info.x, (uint32_t)info.y, 2u + 2u * depth);
+    const bool last = depth + 1 >= P.max_depth;
     if (S.n_lights > 0) {
-        int li = min((int)(gl.u[2] * (float)S.n_lights), S.n_lights - 1);
-        const RtLight& Lt = S.lights[li];
-        vec3 wi = mk(0.f, 0.f, 1.f), le = mk(0.f, 0.f, 0.f);
-        float dist = RT_INF;
-        bool ok = false;
-        if (Lt.type == RT_LIGHT_AREA) {
-            vec3 y = ld3(Lt.p) + ld3(Lt.u) * gl.u[0] + ld3(Lt.v) * gl.u[1];
-            vec3 dv = y - x;
-            float d2 = dot(dv, dv);
-            dist = sqrtf(d2);
-            wi = dv * (1.f / dist);
-            vec3 ln = ld3(Lt.n);
-            float an = sqrtf(dot(ln, ln));
-            float cl = -dot(wi, ln) / an;
-            if (cl > 0.f) {
-                le = ld3(Lt.I) * (cl * an / d2);
-                ok = true;
-            }
-        } else if (Lt.type == RT_LIGHT_POINT) {
-            vec3 dv = ld3(Lt.p) - x;
-            float d2 = dot(dv, dv);
-            dist = sqrtf(d2);
-            wi = dv * (1.f / dist);
-            le = ld3(Lt.I) * (1.f / d2);
-            ok = true;
-        } else {
-            float pdf;
-            wi = env_sample(S, gl.u[0], gl.u[1], pdf);
-            if (pdf > 0.f) {
-                le = env_value(S, wi) * (1.f / pdf);
-                ok = true;
-            }
-        }
-        float cx = dot(wi, nf);
-        if (ok && cx > 0.f) {
-            vec3 f = bsdf_eval(M, wo, wi, nf);
-            vec3 c = mul3(mul3(alpha, f), le) * (cx * (float)S.n_lights);
-            if (!is_zero(c)) {
-                shadow = true;
-                so = make_float4(x.x, x.y, x.z, RT_EPS);
-                sd = make_float4(wi.x, wi.y, wi.z, dist == RT_INF ? RT_INF : dist - RT_EPS);
-                sc = make_float4(c.x, c.y, c.z, __int_as_float(p));
+        vec3 wi, le;
+        float dist, pl;
+        if (light_sample(S, x, gl, wi, le, dist, pl)) {
+            float cx = dot(wi, nf);
+            if (cx > 0.f) {
+                float w = (pl > 0.f && !last) ? mis_power(pl, bsdf_pdf(M, wo, wi, nf)) : 1.f;
+                vec3 f = bsdf_eval(M, wo, wi, nf);
+                vec3 c = mul3(mul3(alpha, f), le) * (cx * (float)S.n_lights * w);
+                if (!is_zero(c)) {
+                    shadow = true;
+                    so = make_float4(x.x, x.y, x.z, RT_EPS);
+                    sd = make_float4(wi.x, wi.y, wi.z, dist == RT_INF ? RT_INF : dist - RT_EPS);
+                    sc = make_float4(c.x, c.y, c.z, __int_as_float(p));
+                }
             }
         }
     }
-    if (depth + 1 >= P.max_depth) return;
+    if (last) return;
     vec3 wi;
     if (!bsdf_sample(M, wo, nf, gb.u[0], gb.u[1], gb.u[2], wi)) return;
     float pdf = bsdf_pdf(M, wo, wi, nf);
@@ -697,7 +745,7 @@ __device__ void shade_unbiased(const DevScene& S, const PathState& ps, const PtP
         if (gb.u[3] < P.rr_prob) return;
         alpha = alpha * (1.f / (1.f - P.rr_prob));
     }
-    ps.alpha[p] = make_float4(alpha.x, alpha.y, alpha.z, 0.f);
+    ps.alpha[p] = make_float4(alpha.x, alpha.y, alpha.z, pdf);
     ps.ray_o[p] = make_float4(x.x, x.y, x.z, RT_EPS);
     ps.ray_d[p] = make_float4(wi.x, wi.y, wi.z, RT_INF);
     ps.info[p] = make_int4(info.x, info.y, info.z, pk(0, 0, 0));
@@ -709,7 +757,8 @@ __device__ void shade_unbiased(const DevScene& S, const PathState& ps, const PtP
  * replaced by a one-sample mixture of the learned vMF proposal (probability
  * alpha) and the BSDF (1 - alpha), weighted by the mixture pdf
  * alpha q(w) + (1 - alpha) p_bsdf(w), so the estimator stays unbiased for
- * any network output.  Optionally records (position, normal, wo, w,
+ * any network output.  That mixture density is also the one next-event
+ * estimation is weighted against.  Optionally records (position, normal, wo, w,
  * p_bsdf(w), throughput) for online training; the record is "closed" at the
  * next shade of the path, when the radiance gathered so far (including this
  * vertex's next-event estimate) is known.
@@ -743,25 +792,8 @@ __device__ void shade_guided(const DevScene& S, const PathState& ps, const Guide
     vec3 o = xyz(ro), d = xyz(rd);
     float4 al = ps.alpha[p];
     vec3 alpha = xyz(al);
-    if (g < 0) {
-        if (S.env_light >= 0 && spec) {
-            vec3 e = mul3(alpha, env_value(S, d));
-            float4 r = ps.radiance[p];
-            ps.radiance[p] = make_float4(r.x + e.x, r.y + e.y, r.z + e.z, 0.f);
-        }
-        return;
-    }
-    int m = S.mat[g];
-    if (m < 0) {
-        const RtLight& Lt = S.lights[-m - 1];
-        if (spec && dot(d, ld3(Lt.n)) < 0.f) {
-            vec3 e = mul3(alpha, ld3(Lt.I));
-            float4 r = ps.radiance[p];
-            ps.radiance[p] = make_float4(r.x + e.x, r.y + e.y, r.z + e.z, 0.f);
-        }
-        return;
-    }
-    const RtMaterial& M = S.mats[m];
+    if (add_emission(S, ps, p, g, spec, h.x, d, al)) return;
+    const RtMaterial& M = S.mats[S.mat[g]];
     vec3 x = o + h.x * d;
     vec3 n = hit_normal(S, g, x, h.z, h.w);
     vec3 nf = dot(n, d) > 0.f ? -n : n;
@@ -780,55 +812,31 @@ __device__ void shade_guided(const DevScene& S, const PathState& ps, const Guide
     U4 gb = rng4(P.seed, (uint32_t)info.z, (uint32_t)info.x, (uint32_t)info.y, 1u + 2u * depth);
     U4 gl = rng4(P.seed, (uint32_t)info.z, (uint32_t)info.x, (uint32_t)info.y, 2u + 2u * depth);
     U4 gx = rng4(P.seed, (uint32_t)info.z, (uint32_t)info.x, (uint32_t)info.y, 1u + 2u * depth, 1u);
+    const bool last = depth + 1 >= P.max_depth;
+    const float* go = G.out + (size_t)p * GD_PARAMS;
+    float ag = go[40];
     if (S.n_lights > 0) {
-        int li = min((int)(gl.u[2] * (float)S.n_lights), S.n_lights - 1);
-        const RtLight& Lt = S.lights[li];
-        vec3 wi = mk(0.f, 0.f, 1.f), le = mk(0.f, 0.f, 0.f);
-        float dist = RT_INF;
-        bool ok = false;
-        if (Lt.type == RT_LIGHT_AREA) {
-            vec3 y = ld3(Lt.p) + ld3(Lt.u) * gl.u[0] + ld3(Lt.v) * gl.u[1];
-            vec3 dv = y - x;
-            float d2 = dot(dv, dv);
-            dist = sqrtf(d2);
-            wi = dv * (1.f / dist);
-            vec3 ln = ld3(Lt.n);
-            float an = sqrtf(dot(ln, ln));
-            float cl = -dot(wi, ln) / an;
-            if (cl > 0.f) {
-                le = ld3(Lt.I) * (cl * an / d2);
-                ok = true;
-            }
-        } else if (Lt.type == RT_LIGHT_POINT) {
-            vec3 dv = ld3(Lt.p) - x;
-            float d2 = dot(dv, dv);
-            dist = sqrtf(d2);
-            wi = dv * (1.f / dist);
-            le = ld3(Lt.I) * (1.f / d2);
-            ok = true;
-        } else {
-            float pdf;
-            wi = env_sample(S, gl.u[0], gl.u[1], pdf);
-            if (pdf > 0.f) {
-                le = env_value(S, wi) * (1.f / pdf);
-                ok = true;
-            }
-        }
-        float cx = dot(wi, nf);
-        if (ok && cx > 0.f) {
-            vec3 f = bsdf_eval(M, wo, wi, nf);
-            vec3 c = mul3(mul3(alpha, f), le) * (cx * (float)S.n_lights);
-            if (!is_zero(c)) {
-                shadow = true;
-                so = make_float4(x.x, x.y, x.z, RT_EPS);
-                sd = make_float4(wi.x, wi.y, wi.z, dist == RT_INF ? RT_INF : dist - RT_EPS);
-                sc = make_float4(c.x, c.y, c.z, __int_as_float(p));
+        vec3 wi, le;
+        float dist, pl;
+        if (light_sample(S, x, gl, wi, le, dist, pl)) {
+            float cx = dot(wi, nf);
+            if (cx > 0.f) {
+                /* the competing strategy is the one-sample mixture */
+                float w = 1.f;
+                if (pl > 0.f && !last)
+                    w = mis_power(pl, ag * guide_pdf(go, wi) + (1.f - ag) * bsdf_pdf(M, wo, wi, nf));
+                vec3 f = bsdf_eval(M, wo, wi, nf);
+                vec3 c = mul3(mul3(alpha, f), le) * (cx * (float)S.n_lights * w);
+                if (!is_zero(c)) {
+                    shadow = true;
+                    so = make_float4(x.x, x.y, x.z, RT_EPS);
+                    sd = make_float4(wi.x, wi.y, wi.z, dist == RT_INF ? RT_INF : dist - RT_EPS);
+                    sc = make_float4(c.x, c.y, c.z, __int_as_float(p));
+                }
             }
         }
     }
-    if (depth + 1 >= P.max_depth) return;
-    const float* go = G.out + (size_t)p * GD_PARAMS;
-    float ag = go[40];
+    if (last) return;
     vec3 wi;
     if (gx.u[0] < ag) {
         wi = guide_sample(go, gx.u[1], gb.u[0], gb.u[1]);
@@ -848,7 +856,7 @@ __device__ void shade_guided(const DevScene& S, const PathState& ps, const Guide
         alpha = alpha * (1.f / (1.f - P.rr_prob));
     }
     if (!(lum(alpha) < 3.0e38f)) return;
-    ps.alpha[p] = make_float4(alpha.x, alpha.y, alpha.z, 0.f);
+    ps.alpha[p] = make_float4(alpha.x, alpha.y, alpha.z, pdf);
     ps.ray_o[p] = make_float4(x.x, x.y, x.z, RT_EPS);
     ps.ray_d[p] = make_float4(wi.x, wi.y, wi.z, RT_INF);
     ps.info[p] = make_int4(info.x, info.y, info.z, pk(0, 0, 0));

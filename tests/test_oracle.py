@@ -197,6 +197,20 @@ def test_area_light_irradiance_closed_form():
     assert img.mean() == pytest.approx(0.5 * F, rel=0.01)
 
 
+def test_split_area_light_closed_form():
+    """The same unit light as two halves: next-event estimation then picks a
+    light with probability 1/2, and the MIS weights on both strategies must
+    carry that choice or the sum is off the form factor."""
+    halves = [AreaLight((-0.5, 1, -0.5), (0.5, 0, 0), (0, 0, 1), (1, 1, 1)),
+              AreaLight((0.0, 1, -0.5), (0.5, 0, 0), (0, 0, 1), (1, 1, 1))]
+    floor = Quad((-5, 0, -5), (-5, 0, 5), (5, 0, 5), (5, 0, -5), DiffuseMaterial((0.5, 0.5, 0.5)))
+    cam = Camera((0, 0.5, 0), (0, 0, 0), (0, 0, -1), 0.02, 8, 8)
+    orc = Oracle(Scene(cam, halves, IntersectableList([floor])).flatten())
+    img, _ = orc.render((cam, 32, 32), mode=1, max_depth=3, seed=6)
+    F = 4 * _rect_form_factor(0.5, 0.5, 1.0)
+    assert img.mean() == pytest.approx(0.5 * F, rel=0.01)
+
+
 def test_point_light_closed_form_both_estimators():
     sc = _floor_scene(PointLight((0, 1, 0), (2, 2, 2)))
     orc = Oracle(sc.flatten())
