@@ -7,6 +7,11 @@ One step = one full render of that film (67.1M camera paths) with the scene
 JSON line with Mrays/s (camera + extension + shadow rays per second), the
 roofline of the closest-hit traversal kernel and a CPU baseline (the oracle,
 a restatement of the reference's PerPixelRenderer, on a bounded sample).
+
+Beside the timed steps (not part of `value`): the 32-view batched render
+with autotuned launch configuration, and two equal-time comparisons of
+guided vs plain PT (occluded-light room; mesh under a sky with a sun), each
+with the guiding network's FP16 MFMA throughput.
 """
 import argparse
 import json
@@ -39,16 +44,24 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 1 GiB device-to-device copy, the reference point for HBM counters")
-    ap.add_argument("--no-guided", action="store_true", help="skip the equal-time guided vs plain comparison")
+    ap.add_argument("--no-guided", action="store_true", help="skip the equal-time guided vs plain comparisons")
+    ap.add_argument("--no-env", action="store_true", help="skip the env-lit mesh comparison")
     ap.add_argument("--no-batched", action="store_true", help="skip the 32-view batched render")
     ap.add_argument("--room-res", type=str, default="480x270")
     ap.add_argument("--room-subdiv", type=int, default=64, help="octahedron segments per edge of the 8 objects")
     ap.add_argument("--room-seconds", type=float, default=2.0, help="wall-time budget of each estimator")
     ap.add_argument("--room-ref-spp", type=int, default=16384)
     ap.add_argument("--room-pass-spp", type=int, default=16)
-    ap.add_argument("--room-train-passes", type=int, default=24)
-    ap.add_argument("--room-epochs", type=int, default=1, help="training epochs over each pass's records")
-    ap.add_argument("--room-lr", type=float, default=1e-3)
+    ap.add_argument("--room-train-passes", type=int, default=24, help="passes before the first score")
+    ap.add_argument("--room-epochs", type=int, default=4, help="training epochs over each pass's records")
+    ap.add_argument("--room-lr", type=float, default=5e-3)
+    ap.add_argument("--env-res", type=int, default=1024)
+    ap.add_argument("--env-seconds", type=float, default=1.5)
+    ap.add_argument("--env-ref-spp", type=int, default=4096)
+    ap.add_argument("--env-pass-spp", type=int, default=16)
+    ap.add_argument("--env-train-passes", type=int, default=4)
+    ap.add_argument("--env-epochs", type=int, default=4)
+    ap.add_argument("--env-lr", type=float, default=5e-3)
     return ap.parse_args()
 
 
@@ -98,91 +111,141 @@ def batched_views(args, scene, arrays):
     }
 
 
-def equal_time_guided(args):
-    """Occluded-light room (the only emitter outside, behind a door gap of
-    5% of the wall width), max depth 16: plain unbiased PT and guided PT each
-    get the same wall-time budget; the guided budget includes the online
-    training of its network.  Both are scored by relMSE against a
-    high-sample plain render.  Returns the comparison as a dict."""
+MLP_FWD_FLOP = 2 * (80 * 64 + 64 * 64 + 64 * 64 + 64 * 48)        # per sample: 32,768
+MLP_TRAIN_FLOP = MLP_FWD_FLOP + 2 * (48 * 64 + 64 * 64 + 64 * 64) + 2 * (80 * 64 + 64 * 64 + 64 * 64 + 48 * 64)
+MFMA_FP16_PEAK_TFLOPS = 2500.0
+
+
+def equal_time(scene, desc, max_depth, budget_s, ref_spp, pass_spp, train_passes, epochs, lr):
+    """Plain unbiased PT and guided PT of one scene each get the same
+    wall-time budget; the guided budget includes the online training of its
+    network (every guided pass trains).  Both are scored by relMSE against a
+    high-sample plain render; scoring is not timed.  Also reports the
+    network kernels' FP16 MFMA throughput from the library's own timers.
+    Returns the comparison as a dict."""
     import numpy as np
     import torch
 
-    from renderer_amd import MODE_GUIDED, MODE_UNBIASED, GpuContext, scenes
+    from renderer_amd import MODE_GUIDED, MODE_UNBIASED, GpuContext
 
-    W, H = (int(x) for x in args.room_res.split("x"))
-    t0 = time.time()
-    scene = scenes.room_scene(W, H, n=args.room_subdiv, seed=3)
+    W, H = scene.camera.width, scene.camera.height
     arrays = scene.flatten()
-    log(f"room: {len(arrays.tri_vertices)} triangles, {W}x{H}, built in {time.time() - t0:.1f} s")
     ctx = GpuContext(0)
     ctx.upload(arrays)
-    ctx.set_launch_config(guide_epochs=args.room_epochs)
-    common = dict(max_depth=16, rr_start=3, rr_prob=0.5)
+    ctx.set_launch_config(guide_epochs=epochs)
+    common = dict(max_depth=max_depth, rr_start=3, rr_prob=0.5)
     dev = "cuda:0"
+    net = dict(infer_ms=0.0, infer_samples=0, train_ms=0.0, train_samples=0, train_steps=0)
 
     def render(spp, seed, mode, train=0):
         f = torch.zeros((H, W, 3), dtype=torch.float32, device=dev)
         torch.cuda.synchronize()
         sx = int(np.sqrt(spp))
         ctx.render_device([(scene.camera, sx, spp // sx)], [f.data_ptr()], seed=seed, mode=mode, guide_train=train,
-                          guide_lr=args.room_lr, **common)
+                          guide_lr=lr, **common)
+        if mode == MODE_GUIDED:
+            s = ctx.stats()
+            for k in net:
+                net[k] += s["guide_" + k]
         return f
 
     # reference: plain PT at ref spp, rendered in passes
     t0 = time.time()
     ref = torch.zeros((H, W, 3), dtype=torch.float64, device=dev)
-    per = 256
-    for k in range(args.room_ref_spp // per):
+    per = min(256, ref_spp)
+    for k in range(ref_spp // per):
         ref += render(per, 100000 + k, MODE_UNBIASED).double()
-    ref = (ref / (args.room_ref_spp // per)).cpu().numpy()
-    log(f"reference {args.room_ref_spp} spp in {time.time() - t0:.1f} s")
+    ref = (ref / (ref_spp // per)).cpu().numpy()
+    log(f"reference {ref_spp} spp in {time.time() - t0:.1f} s")
 
-    def progressive(mode, budget, train_passes):
+    def progressive(mode, budget, first_score):
         acc = torch.zeros((H, W, 3), dtype=torch.float64, device=dev)
         n = 0
         curve = []
-        k = 0
         # every pass is an unbiased estimate (the guided one for any network
         # state), so passes rendered while the network trains count too;
-        # train_passes only delays the first score
-        el = 0.0  # render (and training) time only; scoring is not timed
+        # first_score only delays the first score
+        el = 0.0  # render (and training) time only
         while True:
             torch.cuda.synchronize()
             t_start = time.perf_counter()
-            f = render(args.room_pass_spp, k, mode, train=1 if mode == MODE_GUIDED else 0)
+            f = render(pass_spp, n, mode, train=1 if mode == MODE_GUIDED else 0)
             acc += f.double()
             torch.cuda.synchronize()
             el += time.perf_counter() - t_start
             n += 1
-            if k >= train_passes:
-                img = (acc / n).cpu().numpy()
-                curve.append((el, n * args.room_pass_spp, rel_mse(img, ref)))
-            k += 1
+            if n > first_score:
+                curve.append((el, n * pass_spp, rel_mse((acc / n).cpu().numpy(), ref)))
             if el >= budget:
                 break
         return curve
 
-    plain = progressive(MODE_UNBIASED, args.room_seconds, 0)
+    plain = progressive(MODE_UNBIASED, budget_s, 0)
     ctx.guide_init(7)
-    guided = progressive(MODE_GUIDED, args.room_seconds, args.room_train_passes)
+    guided = progressive(MODE_GUIDED, budget_s, train_passes)
     ctx.close()
     rp = plain[-1][2]
     # time at which guided first reaches the plain estimator's final relMSE
     t_reach = next((t for t, _, e in guided if e <= rp), None)
-    out = {
-        "scene": f"occluded-light room, 12 quads + 8 displaced meshes ({len(arrays.tri_vertices)} triangles), "
-                 f"{W}x{H}, max depth 16, light outside behind a 5% door gap",
-        "budget_s": args.room_seconds,
-        "reference_spp": args.room_ref_spp,
+
+    def tflops(flop, ms):
+        return round(flop / (ms * 1e-3) / 1e12, 2) if ms > 0 else None
+
+    inf_tf = tflops(net["infer_samples"] * MLP_FWD_FLOP, net["infer_ms"])
+    trn_tf = tflops(net["train_samples"] * MLP_TRAIN_FLOP, net["train_ms"])
+    return {
+        "scene": desc,
+        "budget_s": budget_s,
+        "reference_spp": ref_spp,
+        "pass_spp": pass_spp,
         "plain": {"relmse": round(rp, 6), "spp": plain[-1][1], "seconds": round(plain[-1][0], 3)},
         "guided": {"relmse": round(guided[-1][2], 6) if guided else None, "spp": guided[-1][1] if guided else 0,
                    "seconds": round(guided[-1][0], 3) if guided else None,
-                   "training_passes": args.room_train_passes},
+                   "scored_after_passes": train_passes, "epochs": epochs, "lr": lr},
         "relmse_ratio_guided_over_plain": round(guided[-1][2] / rp, 4) if guided else None,
         "time_to_plain_relmse_s": round(t_reach, 3) if t_reach else None,
         "speedup_to_equal_relmse": round(plain[-1][0] / t_reach, 3) if t_reach else None,
+        "network": {
+            "infer_ms": round(net["infer_ms"], 2), "infer_samples": int(net["infer_samples"]),
+            "infer_tflops": inf_tf,
+            "train_ms": round(net["train_ms"], 2), "train_samples": int(net["train_samples"]),
+            "train_steps": int(net["train_steps"]), "train_tflops": trn_tf,
+            "flop_per_sample": {"forward": MLP_FWD_FLOP, "train": MLP_TRAIN_FLOP},
+            "mfma_peak_tflops": MFMA_FP16_PEAK_TFLOPS,
+            "infer_mfma_frac": round(inf_tf / MFMA_FP16_PEAK_TFLOPS, 4) if inf_tf else None,
+            "train_mfma_frac": round(trn_tf / MFMA_FP16_PEAK_TFLOPS, 4) if trn_tf else None,
+        },
     }
-    return out
+
+
+def equal_time_room(args):
+    """Occluded-light room: the only emitter outside, behind a door gap of 5%
+    of the wall width; max depth 16."""
+    from renderer_amd import scenes
+
+    W, H = (int(x) for x in args.room_res.split("x"))
+    t0 = time.time()
+    scene = scenes.room_scene(W, H, n=args.room_subdiv, seed=3)
+    ntri = 8 * 8 * args.room_subdiv ** 2 + 24
+    log(f"room: {ntri} triangles, {W}x{H}, built in {time.time() - t0:.1f} s")
+    desc = (f"occluded-light room, 12 quads + 8 displaced meshes ({ntri} triangles), {W}x{H}, max depth 16, "
+            f"light outside behind a 5% door gap")
+    return equal_time(scene, desc, 16, args.room_seconds, args.room_ref_spp, args.room_pass_spp,
+                      args.room_train_passes, args.room_epochs, args.room_lr)
+
+
+def equal_time_env(args):
+    """The 1,048,576-triangle mesh lit only by the procedural sky (512 x 256
+    lat-long, sun disk of 0.5 degrees at radiance 5e4); max depth 8."""
+    from renderer_amd import scenes
+
+    t0 = time.time()
+    scene = scenes.mesh_scene(args.env_res, args.env_res, n=args.subdiv, seed=1, env=True)
+    log(f"env mesh scene built in {time.time() - t0:.1f} s")
+    desc = (f"displaced sphere {4 * args.subdiv ** 2} tris + ground, lat-long sky 512x256 with a 0.5 deg sun at 5e4, "
+            f"{args.env_res}x{args.env_res}, max depth 8")
+    return equal_time(scene, desc, 8, args.env_seconds, args.env_ref_spp, args.env_pass_spp,
+                      args.env_train_passes, args.env_epochs, args.env_lr)
 
 
 def cpu_baseline(arrays, scene, budget_s):
@@ -293,10 +356,13 @@ def main():
         log("batched views")
         batched = batched_views(args, scene, arrays)
 
-    guided = None
+    guided = env = None
     if not args.no_guided:
-        log("equal-time guided vs plain")
-        guided = equal_time_guided(args)
+        log("equal-time guided vs plain: occluded-light room")
+        guided = equal_time_room(args)
+        if not args.no_env:
+            log("equal-time guided vs plain: env-lit mesh")
+            env = equal_time_env(args)
 
     value = rays / dt / 1e6
     out = {
@@ -342,6 +408,7 @@ def main():
         "cpu_baseline": cpu,
         "batched_views": batched,
         "equal_time_guided_vs_plain": guided,
+        "equal_time_guided_vs_plain_env": env,
     }
     print(json.dumps(out), flush=True)
 

@@ -65,6 +65,12 @@ typedef struct rt_stats {
     int32_t reserved;
     int64_t node_visits;      /* interior nodes fetched by closest-hit traversal */
     int64_t prim_tests;       /* primitive records tested by closest-hit traversal */
+    /* RT_MODE_GUIDED only (0 otherwise): network time and work of the last render */
+    double guide_infer_ms;        /* summed duration of the per-bounce inference launches */
+    double guide_train_ms;        /* summed duration of the training steps (weights, gradients, Adam) */
+    int64_t guide_infer_samples;  /* path vertices the network was evaluated for */
+    int64_t guide_train_samples;  /* records summed over all training steps */
+    int64_t guide_train_steps;    /* Adam steps */
 } rt_stats;
 
 /* Library-wide. */

@@ -73,6 +73,11 @@ class RtStats(ctypes.Structure):
         ("reserved", ctypes.c_int32),
         ("node_visits", ctypes.c_int64),
         ("prim_tests", ctypes.c_int64),
+        ("guide_infer_ms", ctypes.c_double),
+        ("guide_train_ms", ctypes.c_double),
+        ("guide_infer_samples", ctypes.c_int64),
+        ("guide_train_samples", ctypes.c_int64),
+        ("guide_train_steps", ctypes.c_int64),
     ]
 
 

@@ -681,6 +681,7 @@ static int guide_alloc_base(rt_ctx* c)
     g.grad = (float*)c->g_grad.p;
     g.acc = (float*)c->g_acc.p;
     g.rec_count = (int32_t*)c->g_count.p;
+    g.trained = (unsigned long long*)((char*)c->g_count.p + 8);
     return 0;
 }
 
@@ -709,13 +710,12 @@ static int guide_alloc_slots(rt_ctx* c, size_t slots, int batch)
 static int guide_upload_params(rt_ctx* c, const float* theta)
 {
     std::vector<_Float16> w16(GD_NW16);
-    auto copy = [&](int off, int w16off, int n) {
-        for (int i = 0; i < n; ++i) w16[w16off + i] = (_Float16)theta[off + i];
-    };
-    copy(GD_OFF_W1, GD_W16_W1, 64 * 80);
-    copy(GD_OFF_W2, GD_W16_W2, 64 * 64);
-    copy(GD_OFF_W3, GD_W16_W3, 64 * 64);
-    copy(GD_OFF_W4, GD_W16_W4, 48 * 64);
+    for (int i = 0; i < GD_NPARAM; ++i) {
+        int fw, bw;
+        gd_w16_slots(i, &fw, &bw);
+        if (fw >= 0) w16[fw] = (_Float16)theta[i];
+        if (bw >= 0) w16[bw] = (_Float16)theta[i];
+    }
     HIP_TRY(c, hipMemcpy(c->guide.theta, theta, GD_NPARAM * 4, hipMemcpyHostToDevice));
     HIP_TRY(c, hipMemcpy(c->guide.w16, w16.data(), GD_NW16 * 2, hipMemcpyHostToDevice));
     HIP_TRY(c, hipMemset(c->guide.adam_m, 0, GD_NPARAM * 4));
@@ -975,7 +975,9 @@ static int render_impl(rt_ctx* c, const rt_view* views, int32_t nv, const rt_par
     const int depth = prm->max_depth;
     const int stack = c->bvh_depth < 24 ? 24 : RT_STACK_DEPTH;
     const bool timing = c->timing != 0;
-    size_t need_ev = timing ? (size_t)passes.size() * depth * 4 : 0;
+    /* per bounce: before extend, after extend, after inference, after shade,
+     * after shadow; per pass with training: before and after the steps */
+    size_t need_ev = timing ? (size_t)passes.size() * (depth * 5 + (train ? 2 : 0)) : 0;
     while (c->ev_pool.size() < need_ev) {
         hipEvent_t e;
         HIP_TRY(c, hipEventCreate(&e));
@@ -983,8 +985,12 @@ static int render_impl(rt_ctx* c, const rt_view* views, int32_t nv, const rt_par
     }
     HIP_TRY(c, hipGetLastError());
     if ((r = (launch_reset_counters(c->ps, c->stream) == hipSuccess ? 0 : RT_ERR_HIP))) return set_err(c, r, "reset");
+    unsigned long long trained0 = 0;
+    if (train) HIP_TRY(c, hipMemcpy(&trained0, c->guide.trained, sizeof(trained0), hipMemcpyDeviceToHost));
+    const int steps0 = c->guide_step;
     HIP_TRY(c, hipEventRecord(c->ev_a, c->stream));
     size_t ev = 0;
+    std::vector<size_t> bounce_ev, train_ev;  /* index of the first event of each group */
     bool cancelled = false;
 #define LCHK(x)                                                                                  \
     do {                                                                                         \
@@ -999,22 +1005,30 @@ static int render_impl(rt_ctx* c, const rt_view* views, int32_t nv, const rt_par
         LCHK(launch_generate(c->ps, d_views, d_tiles, d_passes, pp, grid, c->stream));
         for (int dd = 0; dd < depth; ++dd) {
             int q = dd & 1;
-            if (timing) LCHK(hipEventRecord(c->ev_pool[ev++], c->stream));
+            auto mark = [&]() { return timing ? hipEventRecord(c->ev_pool[ev++], c->stream) : hipSuccess; };
+            if (timing) bounce_ev.push_back(ev);
+            LCHK(mark());
             LCHK(launch_extend(c->scene, c->ps, q, c->cfg.count_traversal, stack, grid, c->stream));
-            if (timing) LCHK(hipEventRecord(c->ev_pool[ev++], c->stream));
+            LCHK(mark());
             if (guided) {
                 LCHK(launch_guide_infer(c->scene, c->ps, c->guide, q, grid, c->stream));
+                LCHK(mark());
                 LCHK(launch_shade_guided(c->scene, c->ps, c->guide, pp, dd, q, grid, c->stream));
             } else {
+                LCHK(mark());
                 LCHK(launch_shade(c->scene, c->ps, pp, dd, q, grid, c->stream));
             }
-            if (timing) LCHK(hipEventRecord(c->ev_pool[ev++], c->stream));
+            LCHK(mark());
             LCHK(launch_shadow(c->scene, c->ps, pp, stack, grid, c->stream));
-            if (timing) LCHK(hipEventRecord(c->ev_pool[ev++], c->stream));
+            LCHK(mark());
             LCHK(launch_advance(c->ps, q, c->stream));
         }
         LCHK(launch_accumulate(c->ps, d_views, d_tiles, d_passes, grid, c->stream));
         if (train) {
+            if (timing) {
+                train_ev.push_back(ev);
+                LCHK(hipEventRecord(c->ev_pool[ev++], c->stream));
+            }
             /* the pass's records in GD_TRAIN_CHUNKS chunks of `batch`: one Adam step each */
             for (int k = 0; k < GD_TRAIN_CHUNKS * std::max(1, c->cfg.guide_epochs); ++k) {
                 ++c->guide_step;
@@ -1023,6 +1037,7 @@ static int render_impl(rt_ctx* c, const rt_view* views, int32_t nv, const rt_par
                 LCHK(launch_guide_train(c->guide, c->guide.rec, c->guide.rec_count, (k % GD_TRAIN_CHUNKS) * batch, batch, grid, c->stream));
                 LCHK(launch_guide_adam(c->guide, lr, c->guide_step, c->guide.rec_count, (k % GD_TRAIN_CHUNKS) * batch, batch, c->stream));
             }
+            if (timing) LCHK(hipEventRecord(c->ev_pool[ev++], c->stream));
             LCHK(launch_guide_reset_records(c->guide, (int)P, c->stream));
         }
         LCHK(launch_pass_end(c->ps, c->stream));
@@ -1033,17 +1048,30 @@ static int render_impl(rt_ctx* c, const rt_view* views, int32_t nv, const rt_par
     float ms = 0;
     hipEventElapsedTime(&ms, c->ev_a, c->ev_b);
     c->stats.render_ms = ms;
-    double ext = 0, sh = 0;
-    for (size_t i = 0; i + 3 < ev; i += 4) {
-        float a = 0, b = 0;
-        hipEventElapsedTime(&a, c->ev_pool[i], c->ev_pool[i + 1]);
-        hipEventElapsedTime(&b, c->ev_pool[i + 2], c->ev_pool[i + 3]);
-        ext += a;
-        sh += b;
+    auto span = [&](size_t i, size_t j) {
+        float t = 0;
+        hipEventElapsedTime(&t, c->ev_pool[i], c->ev_pool[j]);
+        return (double)t;
+    };
+    double ext = 0, sh = 0, inf = 0, trn = 0;
+    for (size_t i : bounce_ev) {
+        ext += span(i, i + 1);
+        inf += span(i + 1, i + 2);
+        sh += span(i + 3, i + 4);
     }
+    for (size_t i : train_ev) trn += span(i, i + 1);
     c->stats.extend_ms = ext;
     c->stats.shadow_ms = sh;
-    c->stats.extend_launches = (int64_t)(ev / 4);
+    c->stats.extend_launches = (int64_t)bounce_ev.size();
+    c->stats.guide_infer_ms = guided ? inf : 0.0;
+    c->stats.guide_train_ms = trn;
+    c->stats.guide_train_steps = c->guide_step - steps0;
+    c->stats.guide_train_samples = 0;
+    if (train) {
+        unsigned long long trained1 = 0;
+        HIP_TRY(c, hipMemcpy(&trained1, c->guide.trained, sizeof(trained1), hipMemcpyDeviceToHost));
+        c->stats.guide_train_samples = (int64_t)(trained1 - trained0);
+    }
     DevCounters hc;
     HIP_TRY(c, hipMemcpy(&hc, c->ps.cnt, sizeof(hc), hipMemcpyDeviceToHost));
     c->stats.camera_rays = (int64_t)hc.camera_rays;
@@ -1051,6 +1079,8 @@ static int render_impl(rt_ctx* c, const rt_view* views, int32_t nv, const rt_par
     c->stats.shadow_rays = (int64_t)hc.shadow_rays;
     c->stats.node_visits = (int64_t)hc.node_visits;
     c->stats.prim_tests = (int64_t)hc.prim_tests;
+    /* the network runs once per shaded path vertex: every camera and extension ray */
+    c->stats.guide_infer_samples = guided ? (int64_t)(hc.camera_rays + hc.extension_rays) : 0;
     c->stats.paths_per_pass = (int32_t)P;
     c->stats.passes = (int32_t)passes.size();
     if (!device_films) {

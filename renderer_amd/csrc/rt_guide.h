@@ -10,7 +10,11 @@
  *
  * Flat parameter layout (FP32 master, 16,624 values):
  *   W1[64][80] b1[64] W2[64][64] b2[64] W3[64][64] b3[64] W4[48][64] b4[48]
- * FP16 weight copy (16,384 values): W1, W2, W3, W4 in the same order.
+ * FP16 weight copy (27,648 values), pre-tiled as MFMA A operands so that a
+ * wave reads each 16x16 operand as 64 contiguous 8-byte words (no LDS bank
+ * conflicts): W1..W4 for the forward pass (tile (ot, kb), lane m + 16 q
+ * holds W[16 ot + m][16 kb + 4 q .. +3]), then W2..W4 transposed for the
+ * backward pass (tile (it, kb), lane m + 16 q holds W[16 kb + 4 q .. +3][16 it + m]).
  */
 #ifndef RT_GUIDE_H
 #define RT_GUIDE_H
@@ -23,7 +27,8 @@
 #define GD_OUT 48
 #define GD_LOBES 8
 #define GD_NPARAM 16624
-#define GD_NW16 16384
+#define GD_NW16F 16384  /* forward tiles */
+#define GD_NW16 27648   /* forward + transposed tiles */
 #define GD_OFF_W1 0
 #define GD_OFF_B1 5120
 #define GD_OFF_W2 5184
@@ -36,6 +41,9 @@
 #define GD_W16_W2 5120
 #define GD_W16_W3 9216
 #define GD_W16_W4 13312
+#define GD_W16T_W2 16384
+#define GD_W16T_W3 20480
+#define GD_W16T_W4 24576
 #define GD_KLO -3.0f
 #define GD_KHI 10.0f
 #define GD_REC 16        /* floats per training record */
@@ -57,6 +65,7 @@ struct GuideState {
     float* grad;
     float* acc;         /* [0] weight sum, [1] loss sum, [2] records trained */
     int32_t* rec_count;
+    unsigned long long* trained;  /* running total of records over all Adam steps */
     float* rec;         /* capacity x GD_REC */
     float4* rec_aux;    /* capacity */
     int32_t* open_rec;  /* per path slot: index of its open record or -1 */
@@ -64,6 +73,24 @@ struct GuideState {
     int32_t rec_cap;
     float lo[3], inv_ext[3];
 };
+
+/* Positions of master parameter i in the FP16 copy: forward tile slot and
+ * transposed tile slot (-1 where none: biases, and W1 has no transposed copy). */
+__host__ __device__ inline void gd_w16_slots(int i, int* fwd, int* bwd)
+{
+    *fwd = -1;
+    *bwd = -1;
+    int off, O, I, bf, bb;
+    if (i < GD_OFF_B1) { off = GD_OFF_W1; O = 64; I = 80; bf = GD_W16_W1; bb = -1; }
+    else if (i >= GD_OFF_W2 && i < GD_OFF_B2) { off = GD_OFF_W2; O = 64; I = 64; bf = GD_W16_W2; bb = GD_W16T_W2; }
+    else if (i >= GD_OFF_W3 && i < GD_OFF_B3) { off = GD_OFF_W3; O = 64; I = 64; bf = GD_W16_W3; bb = GD_W16T_W3; }
+    else if (i >= GD_OFF_W4 && i < GD_OFF_B4) { off = GD_OFF_W4; O = 48; I = 64; bf = GD_W16_W4; bb = GD_W16T_W4; }
+    else return;
+    int r = i - off, o = r / I, c = r % I;
+    *fwd = bf + ((((o >> 4) * (I >> 4) + (c >> 4)) * 64 + (o & 15) + 16 * ((c & 15) >> 2)) << 2) + (c & 3);
+    if (bb >= 0)
+        *bwd = bb + ((((c >> 4) * (O >> 4) + (o >> 4)) * 64 + (c & 15) + 16 * ((o & 15) >> 2)) << 2) + (o & 3);
+}
 
 struct GuideHost;  /* forward declaration for the ABI layer */
 

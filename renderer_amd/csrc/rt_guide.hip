@@ -35,16 +35,19 @@ __device__ __forceinline__ f4 mfma16(h4 a, h4 b, f4 c)
     return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0);
 }
 
-struct WLds {
-    _Float16 w[GD_NW16];
+/* FP16 weights (the first NW values of the tiled copy) and FP32 biases in LDS */
+template <int NW>
+struct alignas(16) WLds {
+    _Float16 w[NW];
     float b[3 * GD_H + GD_OUT];
 };
 
-__device__ __forceinline__ void load_weights(WLds& L, const GuideState& g)
+template <int NW>
+__device__ __forceinline__ void load_weights(WLds<NW>& L, const GuideState& g)
 {
-    const uint32_t* src = reinterpret_cast<const uint32_t*>(g.w16);
-    uint32_t* dst = reinterpret_cast<uint32_t*>(L.w);
-    for (int i = threadIdx.x; i < GD_NW16 / 2; i += blockDim.x) dst[i] = src[i];
+    const uint4* src = reinterpret_cast<const uint4*>(g.w16);
+    uint4* dst = reinterpret_cast<uint4*>(L.w);
+    for (int i = threadIdx.x; i < NW / 8; i += blockDim.x) dst[i] = src[i];
     for (int i = threadIdx.x; i < 3 * GD_H + GD_OUT; i += blockDim.x) {
         int off = i < 64 ? GD_OFF_B1 + i : i < 128 ? GD_OFF_B2 + i - 64 : i < 192 ? GD_OFF_B3 + i - 128 : GD_OFF_B4 + i - 192;
         L.b[i] = g.theta[off];
@@ -52,61 +55,84 @@ __device__ __forceinline__ void load_weights(WLds& L, const GuideState& g)
     __syncthreads();
 }
 
-/* Input feature block kb (features 16 kb + 4 q .. +3) of one sample. */
-__device__ __forceinline__ h4 feature_block(int kb, int q, const float* p01, vec3 n, vec3 wo)
+/* conversion of a gradient term to FP16 that saturates instead of overflowing */
+__device__ __forceinline__ _Float16 to_h_sat(float v)
 {
-    h4 r;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        int f = 16 * kb + 4 * q + i;
-        float v;
-        if (f < 72) {
-            int d = f / 24, rr = f % 24, k = rr >> 1;
-            float a = ldexpf(p01[d], k);
-            v = (rr & 1) ? cospif(a) : sinpif(a);
-        } else if (f < 75) {
-            v = f == 72 ? n.x : (f == 73 ? n.y : n.z);
-        } else if (f < 78) {
-            v = f == 75 ? wo.x : (f == 76 ? wo.y : wo.z);
-        } else {
-            v = 0.f;
-        }
-        r[i] = (_Float16)v;
-    }
-    return r;
+    return (_Float16)fminf(fmaxf(v, -65000.f), 65000.f);
 }
 
-/* out[ot] = W[16 ot .. +15][:] * in (+ bias), W row-major [OT*16][KB*16]. */
+/* Input features of a wave's 16 samples, staged through LDS rows of
+ * GD_FROW halves (the padding keeps the 8-byte reads of 16 rows on distinct
+ * banks).  Lane (n, q) evaluates sin/cos(2^k pi p_d) for k = 3q .. 3q+2 and
+ * the three coordinates of sample n: one sincospi at k = 3q, then two angle
+ * doublings (sin 2a = 2 sin a cos a, cos 2a = (cos a - sin a)(cos a + sin a));
+ * lane q = 3 adds the normal, the outgoing direction and the zero padding.
+ * After a barrier, x[kb] (features 16 kb + 4 q .. +3 of sample n, the MFMA
+ * B-operand quad) is one 8-byte read. */
+#define GD_FROW (GD_IN + 4)
+__device__ __forceinline__ void features_to_lds(_Float16 (*F)[GD_FROW], int n, int q, const float* p01, vec3 nf,
+                                                vec3 wo)
+{
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        float s, c;
+        sincospif(ldexpf(p01[d], 3 * q), &s, &c);
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            int f = 24 * d + 2 * (3 * q + j);
+            F[n][f] = (_Float16)s;
+            F[n][f + 1] = (_Float16)c;
+            float s2 = 2.f * s * c, c2 = (c - s) * (c + s);
+            s = s2;
+            c = c2;
+        }
+    }
+    if (q == 3) {
+        F[n][72] = (_Float16)nf.x;
+        F[n][73] = (_Float16)nf.y;
+        F[n][74] = (_Float16)nf.z;
+        F[n][75] = (_Float16)wo.x;
+        F[n][76] = (_Float16)wo.y;
+        F[n][77] = (_Float16)wo.z;
+        F[n][78] = (_Float16)0.f;
+        F[n][79] = (_Float16)0.f;
+    }
+}
+__device__ __forceinline__ void features_from_lds(const _Float16 (*F)[GD_FROW], int n, int q, h4* x)
+{
+#pragma unroll
+    for (int kb = 0; kb < 5; ++kb) x[kb] = *reinterpret_cast<const h4*>(&F[n][16 * kb + 4 * q]);
+}
+
+/* out[ot] = W[16 ot .. +15][:] * in (+ bias), W as forward tiles (ot, kb). */
 template <int KB, int OT>
 __device__ __forceinline__ void dense(const _Float16* W, const float* b, const h4* in, f4* out)
 {
-    const int lane = __lane_id(), m = lane & 15, q = lane >> 4;
+    const int lane = __lane_id(), q = lane >> 4;
 #pragma unroll
     for (int ot = 0; ot < OT; ++ot) {
         f4 acc = {b[16 * ot + 4 * q], b[16 * ot + 4 * q + 1], b[16 * ot + 4 * q + 2], b[16 * ot + 4 * q + 3]};
 #pragma unroll
         for (int kb = 0; kb < KB; ++kb) {
-            h4 a = *reinterpret_cast<const h4*>(W + (16 * ot + m) * (KB * 16) + 16 * kb + 4 * q);
+            h4 a = *reinterpret_cast<const h4*>(W + (((ot * KB + kb) << 6) + lane) * 4);
             acc = mfma16(a, in[kb], acc);
         }
         out[ot] = acc;
     }
 }
 
-/* out[it] = W^T[16 it .. +15][:] * in, W row-major [KB*16][IT*16]
- * (back-propagation through a layer: input-neuron rows). */
+/* out[it] = W^T[16 it .. +15][:] * in, W^T as transposed tiles (it, kb)
+ * (back-propagation through a layer with KB * 16 outputs). */
 template <int KB, int IT>
-__device__ __forceinline__ void dense_t(const _Float16* W, const h4* in, f4* out)
+__device__ __forceinline__ void dense_t(const _Float16* WT, const h4* in, f4* out)
 {
-    const int lane = __lane_id(), m = lane & 15, q = lane >> 4;
+    const int lane = __lane_id();
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
         f4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int kb = 0; kb < KB; ++kb) {
-            h4 a;
-#pragma unroll
-            for (int i = 0; i < 4; ++i) a[i] = W[(16 * kb + 4 * q + i) * (IT * 16) + 16 * it + m];
+            h4 a = *reinterpret_cast<const h4*>(WT + (((it * KB + kb) << 6) + lane) * 4);
             acc = mfma16(a, in[kb], acc);
         }
         out[it] = acc;
@@ -123,16 +149,17 @@ __device__ __forceinline__ void relu_h(const f4* z, h4* h)
 }
 
 /* Forward pass of 16 samples held by one wave.  Returns y tiles (3 x f4). */
-__device__ __forceinline__ void forward16(const WLds& L, const h4* x, h4* h1, h4* h2, h4* h3, f4* y)
+__device__ __forceinline__ void forward16(const _Float16* w, const float* b, const h4* x, h4* h1, h4* h2, h4* h3,
+                                          f4* y)
 {
     f4 z[4];
-    dense<5, 4>(L.w + GD_W16_W1, L.b, x, z);
+    dense<5, 4>(w + GD_W16_W1, b, x, z);
     relu_h<4>(z, h1);
-    dense<4, 4>(L.w + GD_W16_W2, L.b + 64, h1, z);
+    dense<4, 4>(w + GD_W16_W2, b + 64, h1, z);
     relu_h<4>(z, h2);
-    dense<4, 4>(L.w + GD_W16_W3, L.b + 128, h2, z);
+    dense<4, 4>(w + GD_W16_W3, b + 128, h2, z);
     relu_h<4>(z, h3);
-    dense<4, 3>(L.w + GD_W16_W4, L.b + 192, h3, y);
+    dense<4, 3>(w + GD_W16_W4, b + 192, h3, y);
 }
 
 /* Head of one lobe from its five raw outputs. */
@@ -161,7 +188,7 @@ __device__ __forceinline__ float quad_sum(float v)
  * ========================================================================= */
 __global__ __launch_bounds__(GB) void k_guide_infer(DevScene S, PathState ps, GuideState g, int qsel)
 {
-    __shared__ WLds L;
+    __shared__ WLds<GD_NW16F> L;
     __shared__ float Y[4][GD_OUT][16];
     load_weights(L, g);
     const int lane = __lane_id(), n = lane & 15, q = lane >> 4, w = threadIdx.x >> 6;
@@ -190,9 +217,12 @@ __global__ __launch_bounds__(GB) void k_guide_infer(DevScene S, PathState ps, Gu
         }
         h4 x[5], h1[4], h2[4], h3[4];
         f4 y[3];
-#pragma unroll
-        for (int kb = 0; kb < 5; ++kb) x[kb] = feature_block(kb, q, p01, nf, wo);
-        forward16(L, x, h1, h2, h3, y);
+        /* the wave's feature rows share its output staging area */
+        _Float16 (*F)[GD_FROW] = reinterpret_cast<_Float16 (*)[GD_FROW]>(&Y[w][0][0]);
+        features_to_lds(F, n, q, p01, nf, wo);
+        __syncthreads();
+        features_from_lds(F, n, q, x);
+        forward16(L.w, L.b, x, h1, h2, h3, y);
 #pragma unroll
         for (int t = 0; t < 3; ++t)
 #pragma unroll
@@ -253,7 +283,8 @@ __device__ __forceinline__ void record_inputs(const GuideState& g, const float* 
 
 __global__ __launch_bounds__(GB) void k_guide_forward(GuideState g, const float* rec, int N, float* yout)
 {
-    __shared__ WLds L;
+    __shared__ WLds<GD_NW16F> L;
+    __shared__ _Float16 F[4][16][GD_FROW];
     load_weights(L, g);
     const int lane = __lane_id(), n = lane & 15, q = lane >> 4, w = threadIdx.x >> 6;
     for (int base = blockIdx.x * 64; base < N; base += gridDim.x * 64) {
@@ -264,9 +295,10 @@ __global__ __launch_bounds__(GB) void k_guide_forward(GuideState g, const float*
         if (valid) record_inputs(g, rec + (size_t)i * GD_REC, p01, nf, wo);
         h4 x[5], h1[4], h2[4], h3[4];
         f4 y[3];
-#pragma unroll
-        for (int kb = 0; kb < 5; ++kb) x[kb] = feature_block(kb, q, p01, nf, wo);
-        forward16(L, x, h1, h2, h3, y);
+        features_to_lds(F[w], n, q, p01, nf, wo);
+        __syncthreads();
+        features_from_lds(F[w], n, q, x);
+        forward16(L.w, L.b, x, h1, h2, h3, y);
         if (valid)
 #pragma unroll
             for (int t = 0; t < 3; ++t)
@@ -339,11 +371,14 @@ hipError_t launch_guide_zero_acc(const GuideState& g, hipStream_t st)
  * training: loss -sum s_i log p_mix_i with s_i = w_i N / sum w (capped),
  * gradients summed over the records into g.grad (Adam divides by N)
  * ========================================================================= */
+/* Rows of 64 samples padded by 4 halves: a 16-lane group reading 8 bytes
+ * from 16 consecutive rows then touches 32 distinct 4-byte banks. */
+#define GD_ROW (64 + 4)
 struct TrainLds {
-    WLds wl;
+    WLds<GD_NW16> wl;
     float Y[4][GD_OUT][16];
-    _Float16 act[GD_IN + 3 * GD_H][64];   /* x, h1, h2, h3 of the block's 64 samples */
-    _Float16 dz[3 * GD_H + GD_OUT][64];   /* dz1, dz2, dz3, dy */
+    _Float16 act[GD_IN + 3 * GD_H][GD_ROW];   /* x, h1, h2, h3 of the block's 64 samples */
+    _Float16 dz[3 * GD_H + GD_OUT][GD_ROW];   /* dz1, dz2, dz3, dy */
 };
 
 /* tile t of the block's 64 weight-gradient tiles -> (layer, out tile, in tile) */
@@ -385,9 +420,11 @@ __global__ __launch_bounds__(GB) void k_guide_train(GuideState g, const float* r
         }
         h4 x[5], h1[4], h2[4], h3[4];
         f4 y[3];
-#pragma unroll
-        for (int kb = 0; kb < 5; ++kb) x[kb] = feature_block(kb, q, p01, nf, wo);
-        forward16(S.wl, x, h1, h2, h3, y);
+        _Float16 (*F)[GD_FROW] = reinterpret_cast<_Float16 (*)[GD_FROW]>(&S.Y[w][0][0]);
+        features_to_lds(F, n, q, p01, nf, wo);
+        __syncthreads();
+        features_from_lds(F, n, q, x);
+        forward16(S.wl.w, S.wl.b, x, h1, h2, h3, y);
         /* activations to LDS, [neuron][sample] */
 #pragma unroll
         for (int kb = 0; kb < 5; ++kb)
@@ -443,13 +480,13 @@ __global__ __launch_bounds__(GB) void k_guide_train(GuideState g, const float* r
             float gm = dv * v[s] * kap[s] / mlen[s];
             vec3 dm = (om - mu[s] * cw) * gm;
             float da = dq * pij * (v[s] - qd);
-            S.dz[3 * GD_H + 5 * j][col] = (_Float16)dm.x;
-            S.dz[3 * GD_H + 5 * j + 1][col] = (_Float16)dm.y;
-            S.dz[3 * GD_H + 5 * j + 2][col] = (_Float16)dm.z;
-            S.dz[3 * GD_H + 5 * j + 3][col] = (_Float16)dk;
-            S.dz[3 * GD_H + 5 * j + 4][col] = (_Float16)da;
+            S.dz[3 * GD_H + 5 * j][col] = to_h_sat(dm.x);
+            S.dz[3 * GD_H + 5 * j + 1][col] = to_h_sat(dm.y);
+            S.dz[3 * GD_H + 5 * j + 2][col] = to_h_sat(dm.z);
+            S.dz[3 * GD_H + 5 * j + 3][col] = to_h_sat(dk);
+            S.dz[3 * GD_H + 5 * j + 4][col] = to_h_sat(da);
         }
-        if (q == 0) S.dz[3 * GD_H + 40][col] = (_Float16)(c * (qd - pb) * 0.9f * sg * (1.f - sg));
+        if (q == 0) S.dz[3 * GD_H + 40][col] = to_h_sat(c * (qd - pb) * 0.9f * sg * (1.f - sg));
         if (q == 1)
             for (int o = 41; o < 48; ++o) S.dz[3 * GD_H + o][col] = (_Float16)0.f;
         __syncthreads();
@@ -461,28 +498,28 @@ __global__ __launch_bounds__(GB) void k_guide_train(GuideState g, const float* r
         for (int t = 0; t < 3; ++t)
 #pragma unroll
             for (int e = 0; e < 4; ++e) dy[t][e] = S.dz[3 * GD_H + 16 * t + 4 * q + e][col];
-        dense_t<3, 4>(S.wl.w + GD_W16_W4, dy, z);
+        dense_t<3, 4>(S.wl.w + GD_W16T_W4, dy, z);
 #pragma unroll
         for (int t = 0; t < 4; ++t)
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
-                d3[t][e] = h3[t][e] > (_Float16)0.f ? (_Float16)z[t][e] : (_Float16)0.f;
+                d3[t][e] = h3[t][e] > (_Float16)0.f ? to_h_sat(z[t][e]) : (_Float16)0.f;
                 S.dz[2 * GD_H + 16 * t + 4 * q + e][col] = d3[t][e];
             }
-        dense_t<4, 4>(S.wl.w + GD_W16_W3, d3, z);
+        dense_t<4, 4>(S.wl.w + GD_W16T_W3, d3, z);
 #pragma unroll
         for (int t = 0; t < 4; ++t)
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
-                d2[t][e] = h2[t][e] > (_Float16)0.f ? (_Float16)z[t][e] : (_Float16)0.f;
+                d2[t][e] = h2[t][e] > (_Float16)0.f ? to_h_sat(z[t][e]) : (_Float16)0.f;
                 S.dz[GD_H + 16 * t + 4 * q + e][col] = d2[t][e];
             }
-        dense_t<4, 4>(S.wl.w + GD_W16_W2, d2, z);
+        dense_t<4, 4>(S.wl.w + GD_W16T_W2, d2, z);
 #pragma unroll
         for (int t = 0; t < 4; ++t)
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
-                d1[t][e] = h1[t][e] > (_Float16)0.f ? (_Float16)z[t][e] : (_Float16)0.f;
+                d1[t][e] = h1[t][e] > (_Float16)0.f ? to_h_sat(z[t][e]) : (_Float16)0.f;
                 S.dz[16 * t + 4 * q + e][col] = d1[t][e];
             }
         __syncthreads();
@@ -505,7 +542,11 @@ __global__ __launch_bounds__(GB) void k_guide_train(GuideState g, const float* r
         if (threadIdx.x < 3 * GD_H + GD_OUT) {
             int row = threadIdx.x;  /* dz1 0..63, dz2 64..127, dz3 128..191, dy 192..239 */
             float s = 0.f;
-            for (int k = 0; k < 64; ++k) s += (float)S.dz[row][k];
+#pragma unroll
+            for (int k = 0; k < 64; k += 4) {
+                h4 v = *reinterpret_cast<const h4*>(&S.dz[row][k]);
+                s += ((float)v[0] + (float)v[1]) + ((float)v[2] + (float)v[3]);
+            }
             bacc += s;
         }
         __syncthreads();
@@ -550,6 +591,7 @@ __global__ __launch_bounds__(GB) void k_guide_adam(GuideState g, float lr, int s
     const int N = chunk_len(g, count, off, max_n);
     const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
     const float c1 = 1.f - powf(b1, (float)step), c2 = 1.f - powf(b2, (float)step);
+    if (blockIdx.x == 0 && threadIdx.x == 0 && N > 0) atomicAdd(g.trained, (unsigned long long)N);
     for (int i = blockIdx.x * GB + threadIdx.x; i < GD_NPARAM; i += gridDim.x * GB) {
         float gr = g.grad[i];
         g.grad[i] = 0.f;
@@ -561,12 +603,10 @@ __global__ __launch_bounds__(GB) void k_guide_adam(GuideState g, float lr, int s
         g.adam_v[i] = v;
         float th = g.theta[i] - lr * (m / c1) / (sqrtf(v / c2) + eps);
         g.theta[i] = th;
-        int w16 = -1;
-        if (i < GD_OFF_B1) w16 = GD_W16_W1 + i;
-        else if (i >= GD_OFF_W2 && i < GD_OFF_B2) w16 = GD_W16_W2 + i - GD_OFF_W2;
-        else if (i >= GD_OFF_W3 && i < GD_OFF_B3) w16 = GD_W16_W3 + i - GD_OFF_W3;
-        else if (i >= GD_OFF_W4 && i < GD_OFF_B4) w16 = GD_W16_W4 + i - GD_OFF_W4;
-        if (w16 >= 0) g.w16[w16] = (_Float16)th;
+        int fw, bw;
+        gd_w16_slots(i, &fw, &bw);
+        if (fw >= 0) g.w16[fw] = (_Float16)th;
+        if (bw >= 0) g.w16[bw] = (_Float16)th;
     }
 }
 

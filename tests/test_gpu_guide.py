@@ -174,3 +174,22 @@ def test_guided_render_is_unbiased(gpu_ctx):
           flush=True)
     assert np.all(np.abs(mg.mean(0) - mp.mean(0)) <= 4 * se + 1e-5)
     assert np.mean(eg) < 3 * np.mean(ep)
+
+
+def test_guided_render_reports_network_work(gpu_ctx):
+    """The library's own counters for the network: one inference per shaded
+    vertex, GD_TRAIN_CHUNKS x epochs Adam steps per pass, timers running."""
+    sc = scenes.box_scene(32, 32)
+    gpu_ctx.upload(sc.flatten())
+    gpu_ctx.guide_init(3)
+    gpu_ctx.set_launch_config(guide_epochs=2)
+    gpu_ctx.render([(sc.camera, 4, 4)], mode=MODE_GUIDED, max_depth=8, seed=1, guide_train=1, guide_batch=4096)
+    st = gpu_ctx.stats()
+    gpu_ctx.set_launch_config(guide_epochs=1)
+    assert st["guide_infer_samples"] == st["camera_rays"] + st["extension_rays"]
+    assert st["guide_train_steps"] == st["passes"] * 4 * 2
+    assert 0 < st["guide_train_samples"] <= st["guide_train_steps"] * 4096
+    assert st["guide_infer_ms"] > 0 and st["guide_train_ms"] > 0
+    gpu_ctx.render([(sc.camera, 4, 4)], mode=MODE_UNBIASED, max_depth=8, seed=1)
+    st = gpu_ctx.stats()
+    assert st["guide_infer_samples"] == 0 and st["guide_train_steps"] == 0 and st["guide_infer_ms"] == 0
