@@ -52,14 +52,14 @@ def parse():
     ap.add_argument("--room-seconds", type=float, default=2.0, help="wall-time budget of each estimator")
     ap.add_argument("--room-ref-spp", type=int, default=16384)
     ap.add_argument("--room-pass-spp", type=int, default=16)
-    ap.add_argument("--room-train-passes", type=int, default=24, help="passes before the first score")
+    ap.add_argument("--room-train-passes", type=int, default=24, help="guided passes that train the network")
     ap.add_argument("--room-epochs", type=int, default=4, help="training epochs over each pass's records")
     ap.add_argument("--room-lr", type=float, default=5e-3)
     ap.add_argument("--env-res", type=int, default=1024)
     ap.add_argument("--env-seconds", type=float, default=1.5)
     ap.add_argument("--env-ref-spp", type=int, default=4096)
     ap.add_argument("--env-pass-spp", type=int, default=16)
-    ap.add_argument("--env-train-passes", type=int, default=4)
+    ap.add_argument("--env-train-passes", type=int, default=8)
     ap.add_argument("--env-epochs", type=int, default=4)
     ap.add_argument("--env-lr", type=float, default=5e-3)
     return ap.parse_args()
@@ -119,10 +119,11 @@ MFMA_FP16_PEAK_TFLOPS = 2500.0
 def equal_time(scene, desc, max_depth, budget_s, ref_spp, pass_spp, train_passes, epochs, lr):
     """Plain unbiased PT and guided PT of one scene each get the same
     wall-time budget; the guided budget includes the online training of its
-    network (every guided pass trains).  Both are scored by relMSE against a
-    high-sample plain render; scoring is not timed.  Also reports the
-    network kernels' FP16 MFMA throughput from the library's own timers.
-    Returns the comparison as a dict."""
+    network, which trains during its first `train_passes` passes and then
+    only samples.  Every pass counts towards the image.  Both are scored by
+    relMSE against a high-sample plain render after every pass; scoring is
+    not timed.  Also reports the network kernels' FP16 MFMA throughput from
+    the library's own timers.  Returns the comparison as a dict."""
     import numpy as np
     import torch
 
@@ -158,24 +159,22 @@ def equal_time(scene, desc, max_depth, budget_s, ref_spp, pass_spp, train_passes
     ref = (ref / (ref_spp // per)).cpu().numpy()
     log(f"reference {ref_spp} spp in {time.time() - t0:.1f} s")
 
-    def progressive(mode, budget, first_score):
+    def progressive(mode, budget, training):
         acc = torch.zeros((H, W, 3), dtype=torch.float64, device=dev)
         n = 0
         curve = []
         # every pass is an unbiased estimate (the guided one for any network
-        # state), so passes rendered while the network trains count too;
-        # first_score only delays the first score
+        # state), so passes rendered while the network trains count too
         el = 0.0  # render (and training) time only
         while True:
             torch.cuda.synchronize()
             t_start = time.perf_counter()
-            f = render(pass_spp, n, mode, train=1 if mode == MODE_GUIDED else 0)
+            f = render(pass_spp, n, mode, train=1 if n < training else 0)
             acc += f.double()
             torch.cuda.synchronize()
             el += time.perf_counter() - t_start
             n += 1
-            if n > first_score:
-                curve.append((el, n * pass_spp, rel_mse((acc / n).cpu().numpy(), ref)))
+            curve.append((el, n * pass_spp, rel_mse((acc / n).cpu().numpy(), ref)))
             if el >= budget:
                 break
         return curve
@@ -201,7 +200,7 @@ def equal_time(scene, desc, max_depth, budget_s, ref_spp, pass_spp, train_passes
         "plain": {"relmse": round(rp, 6), "spp": plain[-1][1], "seconds": round(plain[-1][0], 3)},
         "guided": {"relmse": round(guided[-1][2], 6) if guided else None, "spp": guided[-1][1] if guided else 0,
                    "seconds": round(guided[-1][0], 3) if guided else None,
-                   "scored_after_passes": train_passes, "epochs": epochs, "lr": lr},
+                   "training_passes": train_passes, "epochs": epochs, "lr": lr},
         "relmse_ratio_guided_over_plain": round(guided[-1][2] / rp, 4) if guided else None,
         "time_to_plain_relmse_s": round(t_reach, 3) if t_reach else None,
         "speedup_to_equal_relmse": round(plain[-1][0] / t_reach, 3) if t_reach else None,

@@ -671,7 +671,7 @@ static int guide_alloc_base(rt_ctx* c)
     int r;
     if ((r = ensure(c, c->g_theta, GD_NPARAM * 4)) || (r = ensure(c, c->g_w16, GD_NW16 * 2)) ||
         (r = ensure(c, c->g_m, GD_NPARAM * 4)) || (r = ensure(c, c->g_v, GD_NPARAM * 4)) ||
-        (r = ensure(c, c->g_grad, GD_NPARAM * 4)) || (r = ensure(c, c->g_acc, 16)) || (r = ensure(c, c->g_count, 16)))
+        (r = ensure(c, c->g_grad, GD_NPARAM * 4)) || (r = ensure(c, c->g_acc, 16)) || (r = ensure(c, c->g_count, 32)))
         return r;
     GuideState& g = c->guide;
     g.theta = (float*)c->g_theta.p;
@@ -682,6 +682,7 @@ static int guide_alloc_base(rt_ctx* c)
     g.acc = (float*)c->g_acc.p;
     g.rec_count = (int32_t*)c->g_count.p;
     g.trained = (unsigned long long*)((char*)c->g_count.p + 8);
+    g.inferred = (unsigned long long*)((char*)c->g_count.p + 16);
     return 0;
 }
 
@@ -722,7 +723,7 @@ static int guide_upload_params(rt_ctx* c, const float* theta)
     HIP_TRY(c, hipMemset(c->guide.adam_v, 0, GD_NPARAM * 4));
     HIP_TRY(c, hipMemset(c->guide.grad, 0, GD_NPARAM * 4));
     HIP_TRY(c, hipMemset(c->guide.acc, 0, 16));
-    HIP_TRY(c, hipMemset(c->guide.rec_count, 0, 16));
+    HIP_TRY(c, hipMemset(c->guide.rec_count, 0, 32));
     c->guide_step = 0;
     c->guide_ready = true;
     return 0;
@@ -985,8 +986,8 @@ static int render_impl(rt_ctx* c, const rt_view* views, int32_t nv, const rt_par
     }
     HIP_TRY(c, hipGetLastError());
     if ((r = (launch_reset_counters(c->ps, c->stream) == hipSuccess ? 0 : RT_ERR_HIP))) return set_err(c, r, "reset");
-    unsigned long long trained0 = 0;
-    if (train) HIP_TRY(c, hipMemcpy(&trained0, c->guide.trained, sizeof(trained0), hipMemcpyDeviceToHost));
+    unsigned long long gcount0[2] = {0, 0};  /* records trained, vertices inferred */
+    if (guided) HIP_TRY(c, hipMemcpy(gcount0, c->guide.trained, sizeof(gcount0), hipMemcpyDeviceToHost));
     const int steps0 = c->guide_step;
     HIP_TRY(c, hipEventRecord(c->ev_a, c->stream));
     size_t ev = 0;
@@ -1011,7 +1012,8 @@ static int render_impl(rt_ctx* c, const rt_view* views, int32_t nv, const rt_par
             LCHK(launch_extend(c->scene, c->ps, q, c->cfg.count_traversal, stack, grid, c->stream));
             LCHK(mark());
             if (guided) {
-                LCHK(launch_guide_infer(c->scene, c->ps, c->guide, q, grid, c->stream));
+                /* the last bounce neither samples the guide nor weights against it */
+                if (dd + 1 < depth) LCHK(launch_guide_infer(c->scene, c->ps, c->guide, q, grid, c->stream));
                 LCHK(mark());
                 LCHK(launch_shade_guided(c->scene, c->ps, c->guide, pp, dd, q, grid, c->stream));
             } else {
@@ -1067,10 +1069,12 @@ static int render_impl(rt_ctx* c, const rt_view* views, int32_t nv, const rt_par
     c->stats.guide_train_ms = trn;
     c->stats.guide_train_steps = c->guide_step - steps0;
     c->stats.guide_train_samples = 0;
-    if (train) {
-        unsigned long long trained1 = 0;
-        HIP_TRY(c, hipMemcpy(&trained1, c->guide.trained, sizeof(trained1), hipMemcpyDeviceToHost));
-        c->stats.guide_train_samples = (int64_t)(trained1 - trained0);
+    c->stats.guide_infer_samples = 0;
+    if (guided) {
+        unsigned long long gcount1[2] = {0, 0};
+        HIP_TRY(c, hipMemcpy(gcount1, c->guide.trained, sizeof(gcount1), hipMemcpyDeviceToHost));
+        c->stats.guide_train_samples = (int64_t)(gcount1[0] - gcount0[0]);
+        c->stats.guide_infer_samples = (int64_t)(gcount1[1] - gcount0[1]);
     }
     DevCounters hc;
     HIP_TRY(c, hipMemcpy(&hc, c->ps.cnt, sizeof(hc), hipMemcpyDeviceToHost));
@@ -1079,8 +1083,6 @@ static int render_impl(rt_ctx* c, const rt_view* views, int32_t nv, const rt_par
     c->stats.shadow_rays = (int64_t)hc.shadow_rays;
     c->stats.node_visits = (int64_t)hc.node_visits;
     c->stats.prim_tests = (int64_t)hc.prim_tests;
-    /* the network runs once per shaded path vertex: every camera and extension ray */
-    c->stats.guide_infer_samples = guided ? (int64_t)(hc.camera_rays + hc.extension_rays) : 0;
     c->stats.paths_per_pass = (int32_t)P;
     c->stats.passes = (int32_t)passes.size();
     if (!device_films) {

@@ -66,6 +66,7 @@ struct GuideState {
     float* acc;         /* [0] weight sum, [1] loss sum, [2] records trained */
     int32_t* rec_count;
     unsigned long long* trained;  /* running total of records over all Adam steps */
+    unsigned long long* inferred; /* running total of path vertices evaluated by k_guide_infer */
     float* rec;         /* capacity x GD_REC */
     float4* rec_aux;    /* capacity */
     int32_t* open_rec;  /* per path slot: index of its open record or -1 */

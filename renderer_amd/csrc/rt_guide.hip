@@ -193,6 +193,7 @@ __global__ __launch_bounds__(GB) void k_guide_infer(DevScene S, PathState ps, Gu
     load_weights(L, g);
     const int lane = __lane_id(), n = lane & 15, q = lane >> 4, w = threadIdx.x >> 6;
     const int cnt = ps.cnt->q_count[qsel];
+    if (blockIdx.x == 0 && threadIdx.x == 0 && cnt > 0) atomicAdd(g.inferred, (unsigned long long)cnt);
     for (int base = blockIdx.x * 64; base < cnt; base += gridDim.x * 64) {
         int i = base + 16 * w + n;
         bool valid = i < cnt;

@@ -813,8 +813,9 @@ __device__ void shade_guided(const DevScene& S, const PathState& ps, const Guide
     U4 gl = rng4(P.seed, (uint32_t)info.z, (uint32_t)info.x, (uint32_t)info.y, 2u + 2u * depth);
     U4 gx = rng4(P.seed, (uint32_t)info.z, (uint32_t)info.x, (uint32_t)info.y, 1u + 2u * depth, 1u);
     const bool last = depth + 1 >= P.max_depth;
+    /* the network is not evaluated for the last bounce */
     const float* go = G.out + (size_t)p * GD_PARAMS;
-    float ag = go[40];
+    float ag = last ? 0.f : go[40];
     if (S.n_lights > 0) {
         vec3 wi, le;
         float dist, pl;
