@@ -114,14 +114,18 @@ int rt_render(rt_ctx* ctx, const rt_view* views, int32_t n_views, const rt_param
 int rt_render_device(rt_ctx* ctx, const rt_view* views, int32_t n_views, const rt_params* params,
                      float* const* films);
 
-/* Launch configuration used by rt_render; 0 leaves a field unchanged.
- * path_slots: paths in flight per pass; block: threads per block of the
- * traversal kernels; waves_per_cu: persistent blocks are sized from it. */
+/* Launch configuration used by rt_render; 0 leaves path_slots, block,
+ * blocks_per_cu and guide_epochs unchanged, while the two flags are set as
+ * given.  path_slots: paths in flight per pass; block: threads per block;
+ * blocks_per_cu: persistent blocks per CU of the pipeline kernels (the
+ * grid is CUs x blocks_per_cu). */
 typedef struct rt_launch_config {
     int32_t path_slots;
     int32_t block;
     int32_t blocks_per_cu;
-    int32_t use_graph;     /* 1 = replay the per-pass kernel sequence as a hipGraph */
+    int32_t use_graph;     /* 1 = capture one pass's kernel sequence once and replay it as a
+                              hipGraph (ignored while the guide trains); the per-kernel
+                              times in rt_stats are then 0 */
     int32_t count_traversal; /* 1 = closest-hit kernels also count nodes and primitives */
     int32_t guide_epochs;  /* GUIDED + training: passes over each render pass's records (0 = 1) */
 } rt_launch_config;
@@ -152,7 +156,7 @@ int rt_set_fault(rt_ctx* ctx, int32_t fault);
  * outgoing direction (3), sampled direction (3), BSDF pdf of it, weight,
  * luminance of the throughput, unused.  Positions are encoded relative to
  * the scene box of the last rt_scene_upload unless rt_guide_set_bounds set
- * another box.
+ * another box after it (an upload resets the box to the new scene's).
  * ------------------------------------------------------------------------- */
 int rt_guide_param_count(void);
 int rt_guide_init(rt_ctx* ctx, uint32_t seed);

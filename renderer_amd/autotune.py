@@ -6,8 +6,8 @@ fixed seed gives bit-identical films for every choice):
 
   path_slots     paths in flight per pass (more slots: fewer passes and
                  fuller late-bounce launches, more HBM for path state)
-  blocks_per_cu  persistent blocks per CU (occupancy and load balance of the
-                 traversal kernels, whose LDS stacks limit resident blocks)
+  blocks_per_cu  persistent blocks per CU (the traversal kernels' LDS stacks
+                 allow 8 resident blocks per CU; shading kernels fit fewer)
 
 `autotune` times a reduced version of the caller's workload (the same views,
 each sample count divided by `spp_divisor`) for every candidate pair and
@@ -19,7 +19,7 @@ import math
 import os
 
 DEFAULT_SLOTS = (1 << 22, 1 << 23, 1 << 24, 1 << 25)
-DEFAULT_BPC = (6, 12, 18)
+DEFAULT_BPC = (4, 8, 16)
 
 
 def workload_signature(ctx, views, params):

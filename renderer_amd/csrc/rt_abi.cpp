@@ -52,9 +52,10 @@ struct rt_ctx {
     std::vector<DevBuf> ps_bufs;
     DevBuf counters, views, tiles, passes, films;
     /* defaults from a sweep on MI355X (1M-triangle mesh, 1024^2 x 64 spp):
-     * 16M path slots per pass, 12 persistent blocks per CU (two rounds of
-     * the 6 blocks whose traversal stacks fit one CU's LDS) */
-    rt_launch_config cfg{0, 256, 12, 0, 0, 1};
+     * 32M path slots per pass (see render_impl), 8 persistent blocks per CU
+     * (all resident: the traversal kernels claim work dynamically, so extra
+     * blocks only add launch overhead) */
+    rt_launch_config cfg{0, 256, 8, 0, 0, 1};
 
     // guiding network
     GuideState guide{};
@@ -514,11 +515,12 @@ int rt_scene_upload(rt_ctx* c, const rt_scene_desc* d)
             c->scene_lo[k] = lo[k] - pad;
             c->scene_hi[k] = hi[k] + pad;
         }
-        if (!c->guide_bounds_set)
-            for (int k = 0; k < 3; ++k) {
-                c->guide.lo[k] = c->scene_lo[k];
-                c->guide.inv_ext[k] = 1.f / (c->scene_hi[k] - c->scene_lo[k]);
-            }
+        /* a new scene replaces any box given by rt_guide_set_bounds */
+        c->guide_bounds_set = false;
+        for (int k = 0; k < 3; ++k) {
+            c->guide.lo[k] = c->scene_lo[k];
+            c->guide.inv_ext[k] = 1.f / (c->scene_hi[k] - c->scene_lo[k]);
+        }
     }
     return RT_OK;
 }
@@ -893,7 +895,9 @@ static int render_impl(rt_ctx* c, const rt_view* views, int32_t nv, const rt_par
         max_pix = std::max(max_pix, npix);
     }
     hipSetDevice(c->device);
-    int64_t P = c->cfg.path_slots > 0 ? c->cfg.path_slots : (int64_t)1 << 24;
+    /* default 32M path slots (~7.4 GB of path state): fewer passes and fuller
+     * late-bounce launches; the 1024^2 x 64 spp film takes two passes */
+    int64_t P = c->cfg.path_slots > 0 ? c->cfg.path_slots : (int64_t)1 << 25;
     P = std::min<int64_t>(P, std::max<int64_t>(total_samples, 1));
     P = std::max<int64_t>(P, 256);
 
@@ -975,7 +979,12 @@ static int render_impl(rt_ctx* c, const rt_view* views, int32_t nv, const rt_par
     const int grid = grid_for(c);
     const int depth = prm->max_depth;
     const int stack = c->bvh_depth < 24 ? 24 : RT_STACK_DEPTH;
-    const bool timing = c->timing != 0;
+    /* Every pass enqueues the same kernels with the same arguments (the pass
+     * index lives in the device counters), so one captured pass can be
+     * replayed as a hipGraph - except with training, whose Adam step number
+     * is a per-step argument.  A graph carries no per-kernel timing events. */
+    const bool graph = c->cfg.use_graph != 0 && !train;
+    const bool timing = c->timing != 0 && !graph;
     /* per bounce: before extend, after extend, after inference, after shade,
      * after shadow; per pass with training: before and after the steps */
     size_t need_ev = timing ? (size_t)passes.size() * (depth * 5 + (train ? 2 : 0)) : 0;
@@ -993,60 +1002,83 @@ static int render_impl(rt_ctx* c, const rt_view* views, int32_t nv, const rt_par
     size_t ev = 0;
     std::vector<size_t> bounce_ev, train_ev;  /* index of the first event of each group */
     bool cancelled = false;
-#define LCHK(x)                                                                                  \
-    do {                                                                                         \
-        hipError_t e_ = (x);                                                                     \
-        if (e_ != hipSuccess) return set_err(c, RT_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    /* one pass: generate, max_depth x (extend, [inference], shade, shadow,
+     * advance), accumulate, [training steps], pass end */
+    auto enqueue_pass = [&]() -> hipError_t {
+        hipError_t e;
+#define QCHK(x)                              \
+    do {                                     \
+        if ((e = (x)) != hipSuccess) return e; \
     } while (0)
+        auto mark = [&]() { return timing ? hipEventRecord(c->ev_pool[ev++], c->stream) : hipSuccess; };
+        QCHK(launch_generate(c->ps, d_views, d_tiles, d_passes, pp, grid, c->stream));
+        for (int dd = 0; dd < depth; ++dd) {
+            int q = dd & 1;
+            if (timing) bounce_ev.push_back(ev);
+            QCHK(mark());
+            QCHK(launch_extend(c->scene, c->ps, q, c->cfg.count_traversal, stack, grid, c->stream));
+            QCHK(mark());
+            if (guided) {
+                /* the last bounce neither samples the guide nor weights against it */
+                if (dd + 1 < depth) QCHK(launch_guide_infer(c->scene, c->ps, c->guide, q, grid, c->stream));
+                QCHK(mark());
+                QCHK(launch_shade_guided(c->scene, c->ps, c->guide, pp, dd, q, grid, c->stream));
+            } else {
+                QCHK(mark());
+                QCHK(launch_shade(c->scene, c->ps, pp, dd, q, grid, c->stream));
+            }
+            QCHK(mark());
+            QCHK(launch_shadow(c->scene, c->ps, pp, stack, grid, c->stream));
+            QCHK(mark());
+            QCHK(launch_advance(c->ps, q, c->stream));
+        }
+        QCHK(launch_accumulate(c->ps, d_views, d_tiles, d_passes, grid, c->stream));
+        if (train) {
+            if (timing) train_ev.push_back(ev);
+            QCHK(mark());
+            /* the pass's records in GD_TRAIN_CHUNKS chunks of `batch`: one Adam step each */
+            for (int k = 0; k < GD_TRAIN_CHUNKS * std::max(1, c->cfg.guide_epochs); ++k) {
+                const int off = (k % GD_TRAIN_CHUNKS) * batch;
+                ++c->guide_step;
+                QCHK(launch_guide_zero_acc(c->guide, c->stream));
+                QCHK(launch_guide_weights(c->guide, c->ps, off, batch, grid, c->stream));
+                QCHK(launch_guide_train(c->guide, c->guide.rec, c->guide.rec_count, off, batch, grid, c->stream));
+                QCHK(launch_guide_adam(c->guide, lr, c->guide_step, c->guide.rec_count, off, batch, c->stream));
+            }
+            QCHK(mark());
+            QCHK(launch_guide_reset_records(c->guide, (int)P, c->stream));
+        }
+        QCHK(launch_pass_end(c->ps, c->stream));
+#undef QCHK
+        return hipSuccess;
+    };
+    hipGraphExec_t gexec = nullptr;
+    if (graph) {
+        hipGraph_t gr = nullptr;
+        hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
+        if (e == hipSuccess) {
+            hipError_t e1 = enqueue_pass();
+            hipError_t e2 = hipStreamEndCapture(c->stream, &gr);
+            e = e1 != hipSuccess ? e1 : e2;
+        }
+        if (e == hipSuccess) e = hipGraphInstantiate(&gexec, gr, nullptr, nullptr, 0);
+        if (gr) hipGraphDestroy(gr);
+        if (e != hipSuccess) return set_err(c, RT_ERR_HIP, std::string("pass graph: ") + hipGetErrorString(e));
+    }
     for (size_t pi = 0; pi < passes.size(); ++pi) {
         if (c->stop) {
             cancelled = true;
             break;
         }
-        LCHK(launch_generate(c->ps, d_views, d_tiles, d_passes, pp, grid, c->stream));
-        for (int dd = 0; dd < depth; ++dd) {
-            int q = dd & 1;
-            auto mark = [&]() { return timing ? hipEventRecord(c->ev_pool[ev++], c->stream) : hipSuccess; };
-            if (timing) bounce_ev.push_back(ev);
-            LCHK(mark());
-            LCHK(launch_extend(c->scene, c->ps, q, c->cfg.count_traversal, stack, grid, c->stream));
-            LCHK(mark());
-            if (guided) {
-                /* the last bounce neither samples the guide nor weights against it */
-                if (dd + 1 < depth) LCHK(launch_guide_infer(c->scene, c->ps, c->guide, q, grid, c->stream));
-                LCHK(mark());
-                LCHK(launch_shade_guided(c->scene, c->ps, c->guide, pp, dd, q, grid, c->stream));
-            } else {
-                LCHK(mark());
-                LCHK(launch_shade(c->scene, c->ps, pp, dd, q, grid, c->stream));
-            }
-            LCHK(mark());
-            LCHK(launch_shadow(c->scene, c->ps, pp, stack, grid, c->stream));
-            LCHK(mark());
-            LCHK(launch_advance(c->ps, q, c->stream));
+        hipError_t e = graph ? hipGraphLaunch(gexec, c->stream) : enqueue_pass();
+        if (e != hipSuccess) {
+            if (gexec) hipGraphExecDestroy(gexec);
+            return set_err(c, RT_ERR_HIP, std::string("render pass: ") + hipGetErrorString(e));
         }
-        LCHK(launch_accumulate(c->ps, d_views, d_tiles, d_passes, grid, c->stream));
-        if (train) {
-            if (timing) {
-                train_ev.push_back(ev);
-                LCHK(hipEventRecord(c->ev_pool[ev++], c->stream));
-            }
-            /* the pass's records in GD_TRAIN_CHUNKS chunks of `batch`: one Adam step each */
-            for (int k = 0; k < GD_TRAIN_CHUNKS * std::max(1, c->cfg.guide_epochs); ++k) {
-                ++c->guide_step;
-                LCHK(launch_guide_zero_acc(c->guide, c->stream));
-                LCHK(launch_guide_weights(c->guide, c->ps, (k % GD_TRAIN_CHUNKS) * batch, batch, grid, c->stream));
-                LCHK(launch_guide_train(c->guide, c->guide.rec, c->guide.rec_count, (k % GD_TRAIN_CHUNKS) * batch, batch, grid, c->stream));
-                LCHK(launch_guide_adam(c->guide, lr, c->guide_step, c->guide.rec_count, (k % GD_TRAIN_CHUNKS) * batch, batch, c->stream));
-            }
-            if (timing) LCHK(hipEventRecord(c->ev_pool[ev++], c->stream));
-            LCHK(launch_guide_reset_records(c->guide, (int)P, c->stream));
-        }
-        LCHK(launch_pass_end(c->ps, c->stream));
     }
-#undef LCHK
     HIP_TRY(c, hipEventRecord(c->ev_b, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (gexec) HIP_TRY(c, hipGraphExecDestroy(gexec));
     float ms = 0;
     hipEventElapsedTime(&ms, c->ev_a, c->ev_b);
     c->stats.render_ms = ms;

@@ -231,12 +231,46 @@ __device__ __forceinline__ void box2(const Trav& T, float lo, float hi, float in
     f = pos ? b : a;
 }
 
+/* Traversal stack of one lane holding LDSN + RT_STACK_REG entries: the
+ * first LDSN in LDS (one column per lane, `stride` words apart), the last
+ * RT_STACK_REG in registers.  The LDS part bounds how many waves fit on a
+ * CU, so it is kept short; the register part gives the rare deep rays room
+ * at the cost of a few VGPRs. */
+#define RT_STACK_REG 4
+template <int LDSN>
+struct LaneStack {
+    int32_t* lds;
+    int stride;
+    int32_t r0 = 0, r1 = 0, r2 = 0, r3 = 0;  /* entries LDSN .. LDSN + 3 (scalars, so they stay in VGPRs) */
+    static constexpr int capacity = LDSN + RT_STACK_REG;
+    __device__ __forceinline__ LaneStack(int32_t* column, int s) : lds(column), stride(s) {}
+    __device__ __forceinline__ void put(int sp, int32_t v)
+    {
+        if (sp < LDSN) {
+            lds[sp * stride] = v;
+            return;
+        }
+        int k = sp - LDSN;
+        r0 = k == 0 ? v : r0;
+        r1 = k == 1 ? v : r1;
+        r2 = k == 2 ? v : r2;
+        r3 = k == 3 ? v : r3;
+    }
+    __device__ __forceinline__ int32_t get(int sp) const
+    {
+        if (sp < LDSN) return lds[sp * stride];
+        int k = sp - LDSN;
+        return k == 0 ? r0 : k == 1 ? r1 : k == 2 ? r2 : r3;
+    }
+};
+
 /* Visit node T.cur (T.cur >= 0): test both child boxes, test the
  * primitives of hit leaves, continue with the nearer hit interior child
  * and defer the other; pop when neither is hit.  Returns true when the
- * traversal of this ray is complete. */
-template <bool ANY, bool COUNT>
-__device__ __forceinline__ bool trav_step(const DevScene& S, Trav& T, int32_t* stack, int stride, int& cn, int& cp)
+ * traversal of this ray is complete.  The stack needs room for the BVH
+ * depth. */
+template <bool ANY, bool COUNT, class Stack>
+__device__ __forceinline__ bool trav_step(const DevScene& S, Trav& T, Stack& stack, int& cn, int& cp)
 {
     const float4* nd = S.nodes + 4 * (size_t)T.cur;
     float4 f0 = nd[0], f1 = nd[1], f2 = nd[2], f3 = nd[3];
@@ -270,7 +304,7 @@ __device__ __forceinline__ bool trav_step(const DevScene& S, Trav& T, int32_t* s
             if (next < 0) {
                 next = c;
             } else {
-                stack[T.sp * stride] = c;
+                stack.put(T.sp, c);
                 ++T.sp;
             }
             continue;
@@ -304,21 +338,20 @@ __device__ __forceinline__ bool trav_step(const DevScene& S, Trav& T, int32_t* s
     }
     if (T.sp == 0) return true;
     --T.sp;
-    T.cur = stack[T.sp * stride];
+    T.cur = stack.get(T.sp);
     return false;
 }
 
 /* Whole traversal of one ray (batch kernels and counting). */
-template <bool ANY, bool COUNT>
+template <bool ANY, bool COUNT, class Stack>
 __device__ __forceinline__ TraceOut traverse(const DevScene& S, vec3 o, vec3 d, float tmin, float tmax,
-                                             int32_t* stack, int stride, int* n_nodes = nullptr,
-                                             int* n_prims = nullptr)
+                                             Stack& stack, int* n_nodes = nullptr, int* n_prims = nullptr)
 {
     Trav T;
     trav_init(S, T, o, d, tmin, tmax);
     int cn = 0, cp = 0;
     if (T.cur >= 0)
-        while (!trav_step<ANY, COUNT>(S, T, stack, stride, cn, cp)) {
+        while (!trav_step<ANY, COUNT>(S, T, stack, cn, cp)) {
         }
     if (COUNT) {
         *n_nodes = cn;

@@ -61,6 +61,8 @@ struct DevCounters {
     int32_t q_count[2];      /* path queue lengths (ping-pong) */
     int32_t shadow_count;
     int32_t pass;            /* current pass index */
+    int32_t ext_next;        /* work counters of the closest-hit and any-hit launches: */
+    int32_t sh_next;         /* the next unclaimed queue position (zero before each launch) */
     unsigned long long camera_rays, extension_rays, shadow_rays;
     unsigned long long node_visits, prim_tests;
 };

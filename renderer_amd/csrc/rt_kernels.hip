@@ -55,6 +55,38 @@ __device__ __forceinline__ void wave_count(bool pred, unsigned long long* counte
     if (__lane_id() == 0 && m) atomicAdd(counter, (unsigned long long)__popcll(m));
 }
 
+/* Dynamic work distribution of the persistent traversal kernels: a wave
+ * claims queue positions from a global counter in chunks of 64 (one atomic
+ * per chunk) and hands them out, in lane order, to the lanes that have
+ * finished their previous ray.  Call in wave-uniform control flow.  `base`
+ * and `used` are the wave's chunk (uniform across the wave); returns the
+ * position for lanes with `want` (>= the queue length once it is drained). */
+struct WaveFeed {
+    int base = 0, used = 64;
+};
+__device__ __forceinline__ int feed_take(WaveFeed& f, bool want, int32_t* counter)
+{
+    const int lane = __lane_id();
+    const unsigned long long below = (1ull << lane) - 1ull;
+    int idx = 0x7fffffff;
+    unsigned long long m = __ballot(want);
+    while (m) {
+        if (f.used == 64) {
+            int b = 0;
+            if (lane == 0) b = atomicAdd(counter, 64);
+            f.base = __builtin_amdgcn_readfirstlane(b);
+            f.used = 0;
+        }
+        int avail = 64 - f.used;
+        int rank = __popcll(m & below);
+        bool mine = ((m >> lane) & 1ull) && rank < avail;
+        if (mine) idx = f.base + f.used + rank;
+        f.used += min(__popcll(m), avail);
+        m &= ~__ballot(mine);
+    }
+    return idx;
+}
+
 /* ===========================================================================
  * Batch traversal kernels
  * ========================================================================= */
@@ -63,12 +95,13 @@ __global__ __launch_bounds__(PT_BLOCK) void k_trace(DevScene S, const float4* __
                                                    const float4* __restrict__ d, int64_t n, int32_t* prim, float* tout,
                                                    uint8_t* occ, int32_t* nn, int32_t* np)
 {
-    __shared__ int32_t stack[RT_STACK_DEPTH * PT_BLOCK];
-    int32_t* my = stack + threadIdx.x;
+    constexpr int LDSN = RT_STACK_DEPTH - RT_STACK_REG;
+    __shared__ int32_t stack[LDSN * PT_BLOCK];
+    LaneStack<LDSN> my(stack + threadIdx.x, PT_BLOCK);
     for (int64_t i = (int64_t)blockIdx.x * PT_BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * PT_BLOCK) {
         float4 a = o[i], b = d[i];
         int cn = 0, cp = 0;
-        TraceOut h = traverse<ANY, COUNT>(S, xyz(a), xyz(b), a.w, b.w, my, PT_BLOCK, &cn, &cp);
+        TraceOut h = traverse<ANY, COUNT>(S, xyz(a), xyz(b), a.w, b.w, my, &cn, &cp);
         if (ANY) {
             occ[i] = h.g >= 0 ? 1 : 0;
         } else {
@@ -381,6 +414,8 @@ __global__ __launch_bounds__(PT_BLOCK) void k_generate(PathState ps, const DevVi
         ps.cnt->q_count[0] = pass.slots;
         ps.cnt->q_count[1] = 0;
         ps.cnt->shadow_count = 0;
+        ps.cnt->ext_next = 0;
+        ps.cnt->sh_next = 0;
         ps.cnt->camera_rays += (unsigned long long)pass.slots;
     }
     for (int slot = blockIdx.x * PT_BLOCK + threadIdx.x; slot < pass.slots; slot += gridDim.x * PT_BLOCK) {
@@ -433,34 +468,42 @@ hipError_t launch_generate(const PathState& ps, const DevView* views, const DevT
  * extend: closest hit for every live path
  * ========================================================================= */
 /* Persistent traversal with per-lane ray replacement: a lane whose ray is
- * finished stores its result and starts its next ray (stride = grid size)
- * at once, instead of idling until the slowest ray of its wave finishes.
- * STACK is the LDS stack depth (>= the BVH depth + 1); the smaller it is,
- * the more waves fit in a CU's LDS. */
+ * finished stores its result and takes the next unclaimed ray at once
+ * (WaveFeed), instead of idling until the slowest ray of its wave finishes;
+ * waves claim rays dynamically, so the launch ends evenly.
+ * STACK is the stack capacity (>= the BVH depth + 1), of which all but
+ * RT_STACK_REG entries live in LDS; the smaller it is, the more waves fit in
+ * a CU's LDS (capacity 24: 20 KB per block, 8 blocks per CU). */
 template <bool COUNT, int STACK>
 __global__ __launch_bounds__(PT_BLOCK) void k_extend(DevScene S, PathState ps, int qsel)
 {
-    __shared__ int32_t stack[STACK * PT_BLOCK];
-    int32_t* my = stack + threadIdx.x;
+    constexpr int LDSN = STACK - RT_STACK_REG;
+    __shared__ int32_t stack[LDSN * PT_BLOCK];
+    LaneStack<LDSN> my(stack + threadIdx.x, PT_BLOCK);
     const int n = ps.cnt->q_count[qsel];
     const int32_t* q = ps.queue[qsel];
-    const int stride = gridDim.x * PT_BLOCK;
-    int i = blockIdx.x * PT_BLOCK + threadIdx.x;
     int cn = 0, cp = 0, p = 0;
-    bool live = false;
+    bool live = false, more = true;
+    WaveFeed feed;
     Trav T;
     while (true) {
-        if (!live) {
-            if (i >= n) break;
-            p = q[i];
-            float4 a = ps.ray_o[p], b = ps.ray_d[p];
-            trav_init(S, T, xyz(a), xyz(b), a.w, b.w);
-            live = true;
+        bool want = !live && more;
+        if (__ballot(want)) {
+            int i = feed_take(feed, want, &ps.cnt->ext_next);
+            if (want) {
+                if (i < n) {
+                    p = q[i];
+                    float4 a = ps.ray_o[p], b = ps.ray_d[p];
+                    trav_init(S, T, xyz(a), xyz(b), a.w, b.w);
+                    live = true;
+                } else {
+                    more = false;
+                }
+            }
         }
-        bool done = T.cur < 0 || trav_step<false, COUNT>(S, T, my, PT_BLOCK, cn, cp);
-        if (done) {
+        if (!__ballot(live)) break;  /* queue drained and every lane idle */
+        if (live && (T.cur < 0 || trav_step<false, COUNT>(S, T, my, cn, cp))) {
             ps.hit[p] = make_float4(T.g >= 0 ? T.best : RT_INF, __int_as_float(T.g), T.u, T.v);
-            i += stride;
             live = false;
         }
     }
@@ -969,25 +1012,32 @@ hipError_t launch_shade(const HostDevScene& s, const PathState& ps, PtParams p, 
 template <int STACK>
 __global__ __launch_bounds__(PT_BLOCK) void k_shadow(DevScene S, PathState ps, int compat)
 {
-    __shared__ int32_t stack[STACK * PT_BLOCK];
-    int32_t* my = stack + threadIdx.x;
+    constexpr int LDSN = STACK - RT_STACK_REG;
+    __shared__ int32_t stack[LDSN * PT_BLOCK];
+    LaneStack<LDSN> my(stack + threadIdx.x, PT_BLOCK);
     const int n = ps.cnt->shadow_count;
-    const int stride = gridDim.x * PT_BLOCK;
-    int i = blockIdx.x * PT_BLOCK + threadIdx.x;
     int cn = 0, cp = 0, traced = 0;
-    bool live = false;
+    bool live = false, more = true;
+    WaveFeed feed;
     float4 c;
     Trav T;
     while (true) {
-        if (!live) {
-            if (i >= n) break;
-            float4 a = ps.sh_o[i], b = ps.sh_d[i];
-            c = ps.sh_c[i];
-            trav_init(S, T, xyz(a), xyz(b), a.w, b.w);
-            live = true;
+        bool want = !live && more;
+        if (__ballot(want)) {
+            int i = feed_take(feed, want, &ps.cnt->sh_next);
+            if (want) {
+                if (i < n) {
+                    float4 a = ps.sh_o[i], b = ps.sh_d[i];
+                    c = ps.sh_c[i];
+                    trav_init(S, T, xyz(a), xyz(b), a.w, b.w);
+                    live = true;
+                } else {
+                    more = false;
+                }
+            }
         }
-        bool done = T.cur < 0 || trav_step<true, false>(S, T, my, PT_BLOCK, cn, cp);
-        if (done) {
+        if (!__ballot(live)) break;  /* queue drained and every lane idle */
+        if (live && (T.cur < 0 || trav_step<true, false>(S, T, my, cn, cp))) {
             if (T.g < 0) {
                 int p = __float_as_int(c.w);
                 if (compat) {
@@ -998,7 +1048,6 @@ __global__ __launch_bounds__(PT_BLOCK) void k_shadow(DevScene S, PathState ps, i
                 }
             }
             ++traced;
-            i += stride;
             live = false;
         }
     }
@@ -1021,6 +1070,8 @@ __global__ void k_advance(PathState ps, int qsel)
 {
     ps.cnt->q_count[qsel] = 0;
     ps.cnt->shadow_count = 0;
+    ps.cnt->ext_next = 0;
+    ps.cnt->sh_next = 0;
 }
 hipError_t launch_advance(const PathState& ps, int qsel, hipStream_t st)
 {
@@ -1083,6 +1134,7 @@ __global__ void k_reset(PathState ps)
     c->q_count[0] = c->q_count[1] = 0;
     c->shadow_count = 0;
     c->pass = 0;
+    c->ext_next = c->sh_next = 0;
     c->camera_rays = c->extension_rays = c->shadow_rays = 0ull;
     c->node_visits = c->prim_tests = 0ull;
 }

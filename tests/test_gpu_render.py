@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 from oracle.oracle import Oracle
-from renderer_amd import MODE_COMPAT, MODE_UNBIASED, scenes
+from renderer_amd import MODE_COMPAT, MODE_GUIDED, MODE_UNBIASED, scenes
 from tests.helpers import rel_mse
 
 pytestmark = pytest.mark.gpu
@@ -117,6 +117,34 @@ def test_batched_views_match_single_views(gpu_ctx):
         gpu_ctx.set_launch_config(path_slots=1 << 24)
     for a, b in zip(batch, small):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("mode", [MODE_COMPAT, MODE_UNBIASED, MODE_GUIDED])
+def test_graph_replay_matches_eager(gpu_ctx, mode):
+    """One pass captured as a hipGraph and replayed for every pass (small
+    path-slot count: many passes, so many replays) gives the eager film,
+    also with eager renders in between and on a second graph render."""
+    sc = scenes.box_scene(40, 24)
+    gpu_ctx.upload(sc.flatten())
+    if mode == MODE_GUIDED:
+        gpu_ctx.guide_init(4)
+    views = [(sc.camera, 3, 2)]
+    kw = dict(mode=mode, max_depth=6, seed=12)
+    gpu_ctx.set_launch_config(path_slots=1000)
+    try:
+        eager = gpu_ctx.render(views, **kw)[0]
+        assert gpu_ctx.stats()["passes"] > 4
+        films = []
+        for _ in range(2):
+            gpu_ctx.set_launch_config(use_graph=1)
+            films.append(gpu_ctx.render(views, **kw)[0])
+            gpu_ctx.set_launch_config(use_graph=0)
+            films.append(gpu_ctx.render(views, **kw)[0])
+    finally:
+        gpu_ctx.set_launch_config(path_slots=1 << 24, use_graph=0)
+    assert eager.mean() > 0
+    for f in films:
+        assert np.array_equal(f, eager)
 
 
 def test_render_is_deterministic(gpu_ctx):
