@@ -756,10 +756,9 @@ __device__ void shade_unbiased(const DevScene& S, const PathState& ps, const PtP
     }
     if (M.type != RT_MAT_DIFFUSE && M.type != RT_MAT_PHONG) return;
     vec3 wo = -d;
-    U4 gb = rng4(P.seed, (uint32_t)info.z, (uint32_t)info.x, (uint32_t)info.y, 1u + 2u * depth);
-    U4 gl = rng4(P.seed, (uint32_t)info.z, (uint32_t)info.x, (uint32_t)info.y, 2u + 2u * depth);
     const bool last = depth + 1 >= P.max_depth;
     if (S.n_lights > 0) {
+        U4 gl = rng4(P.seed, (uint32_t)info.z, (uint32_t)info.x, (uint32_t)info.y, 2u + 2u * depth);
         vec3 wi, le;
         float dist, pl;
         if (light_sample(S, x, gl, wi, le, dist, pl)) {
@@ -778,6 +777,8 @@ __device__ void shade_unbiased(const DevScene& S, const PathState& ps, const PtP
         }
     }
     if (last) return;
+    /* drawn after next-event estimation: fewer values live across it */
+    U4 gb = rng4(P.seed, (uint32_t)info.z, (uint32_t)info.x, (uint32_t)info.y, 1u + 2u * depth);
     vec3 wi;
     if (!bsdf_sample(M, wo, nf, gb.u[0], gb.u[1], gb.u[2], wi)) return;
     float pdf = bsdf_pdf(M, wo, wi, nf);
@@ -852,14 +853,12 @@ __device__ void shade_guided(const DevScene& S, const PathState& ps, const Guide
     }
     if (M.type != RT_MAT_DIFFUSE && M.type != RT_MAT_PHONG) return;
     vec3 wo = -d;
-    U4 gb = rng4(P.seed, (uint32_t)info.z, (uint32_t)info.x, (uint32_t)info.y, 1u + 2u * depth);
-    U4 gl = rng4(P.seed, (uint32_t)info.z, (uint32_t)info.x, (uint32_t)info.y, 2u + 2u * depth);
-    U4 gx = rng4(P.seed, (uint32_t)info.z, (uint32_t)info.x, (uint32_t)info.y, 1u + 2u * depth, 1u);
     const bool last = depth + 1 >= P.max_depth;
     /* the network is not evaluated for the last bounce */
     const float* go = G.out + (size_t)p * GD_PARAMS;
     float ag = last ? 0.f : go[40];
     if (S.n_lights > 0) {
+        U4 gl = rng4(P.seed, (uint32_t)info.z, (uint32_t)info.x, (uint32_t)info.y, 2u + 2u * depth);
         vec3 wi, le;
         float dist, pl;
         if (light_sample(S, x, gl, wi, le, dist, pl)) {
@@ -881,6 +880,9 @@ __device__ void shade_guided(const DevScene& S, const PathState& ps, const Guide
         }
     }
     if (last) return;
+    /* drawn after next-event estimation: fewer values live across it */
+    U4 gb = rng4(P.seed, (uint32_t)info.z, (uint32_t)info.x, (uint32_t)info.y, 1u + 2u * depth);
+    U4 gx = rng4(P.seed, (uint32_t)info.z, (uint32_t)info.x, (uint32_t)info.y, 1u + 2u * depth, 1u);
     vec3 wi;
     if (gx.u[0] < ag) {
         wi = guide_sample(go, gx.u[1], gb.u[0], gb.u[1]);
