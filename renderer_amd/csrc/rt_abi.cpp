@@ -1030,7 +1030,7 @@ static int render_impl(rt_ctx* c, const rt_view* views, int32_t nv, const rt_par
             QCHK(mark());
             QCHK(launch_shadow(c->scene, c->ps, pp, stack, grid, c->stream));
             QCHK(mark());
-            QCHK(launch_advance(c->ps, q, c->stream));
+            QCHK(launch_advance(c->ps, q, dd, c->stream));
         }
         QCHK(launch_accumulate(c->ps, d_views, d_tiles, d_passes, grid, c->stream));
         if (train) {

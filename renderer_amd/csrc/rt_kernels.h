@@ -116,7 +116,7 @@ hipError_t launch_shade(const HostDevScene& s, const PathState& ps, PtParams p, 
                         hipStream_t st);
 hipError_t launch_shadow(const HostDevScene& s, const PathState& ps, PtParams p, int stack, int grid,
                          hipStream_t st);
-hipError_t launch_advance(const PathState& ps, int qsel, hipStream_t st);
+hipError_t launch_advance(const PathState& ps, int qsel, int depth, hipStream_t st);
 hipError_t launch_accumulate(const PathState& ps, const DevView* views, const DevTile* tiles,
                              const DevPass* passes, int grid, hipStream_t st);
 hipError_t launch_pass_end(const PathState& ps, hipStream_t st);

@@ -36,23 +36,41 @@ static inline DevScene as_dev(const HostDevScene& s)
     return d;
 }
 
-/* Append to a queue from a wave in uniform control flow: one atomic per
- * wave, lane offsets from the ballot mask. */
-__device__ __forceinline__ int wave_append(bool pred, int32_t* counter)
+/* Append to NQ queues from a whole block (PT_BLOCK threads, uniform control
+ * flow across the block): per queue one atomic per block on its counter,
+ * wave offsets from an LDS prefix over the block's 4 waves, lane offsets
+ * from the ballot mask.  Same-address atomics serialise in L2, so one per
+ * block instead of one per wave matters: the shading kernels issue one per
+ * 256 paths and queue.  pos[k] is this thread's slot where pred[k]. */
+template <int NQ>
+__device__ __forceinline__ void block_append(const bool* pred, int32_t* const* counter, int* pos)
 {
-    unsigned long long m = __ballot(pred);
-    uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-    int total = __popcll(m);
-    int base = 0;
-    if (__lane_id() == 0 && total > 0) base = atomicAdd(counter, total);
-    base = __shfl(base, 0);
-    return base + (int)below;
-}
-
-__device__ __forceinline__ void wave_count(bool pred, unsigned long long* counter)
-{
-    unsigned long long m = __ballot(pred);
-    if (__lane_id() == 0 && m) atomicAdd(counter, (unsigned long long)__popcll(m));
+    __shared__ int cnt[NQ][PT_BLOCK / 64];
+    __shared__ int base[NQ];
+    const int w = threadIdx.x >> 6, lane = __lane_id();
+    unsigned long long m[NQ];
+#pragma unroll
+    for (int k = 0; k < NQ; ++k) {
+        m[k] = __ballot(pred[k]);
+        if (lane == 0) cnt[k][w] = __popcll(m[k]);
+    }
+    __syncthreads();
+    if (threadIdx.x < NQ) {
+        int k = threadIdx.x, t = 0;
+#pragma unroll
+        for (int j = 0; j < PT_BLOCK / 64; ++j) t += cnt[k][j];
+        base[k] = t > 0 ? atomicAdd(counter[k], t) : 0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NQ; ++k) {
+        int off = base[k];
+        for (int j = 0; j < w; ++j) off += cnt[k][j];
+        uint32_t below =
+            __builtin_amdgcn_mbcnt_hi((uint32_t)(m[k] >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m[k], 0u));
+        pos[k] = off + (int)below;
+    }
+    __syncthreads();  /* cnt and base are reused by the next call */
 }
 
 /* Dynamic work distribution of the persistent traversal kernels: a wave
@@ -934,17 +952,19 @@ __global__ __launch_bounds__(PT_BLOCK) void k_shade_guided(DevScene S, PathState
             p = q[i];
             shade_guided(S, ps, G, P, depth, p, cont, shadow, so, sd, sc, want, rec);
         }
-        int at = wave_append(cont, &ps.cnt->q_count[qsel ^ 1]);
-        if (cont) qn[at] = p;
-        int as = wave_append(shadow, &ps.cnt->shadow_count);
+        const bool pred[3] = {cont, shadow, want && P.guide_train};
+        int32_t* const ctr[3] = {&ps.cnt->q_count[qsel ^ 1], &ps.cnt->shadow_count, G.rec_count};
+        int pos[3];
+        block_append<3>(pred, ctr, pos);
+        if (cont) qn[pos[0]] = p;
         if (shadow) {
-            ps.sh_o[as] = so;
-            ps.sh_d[as] = sd;
-            ps.sh_c[as] = sc;
+            ps.sh_o[pos[1]] = so;
+            ps.sh_d[pos[1]] = sd;
+            ps.sh_c[pos[1]] = sc;
         }
-        if (P.guide_train) {
-            int ar = wave_append(want, G.rec_count);
-            if (want && ar < G.rec_cap) {
+        if (pred[2]) {
+            int ar = pos[2];
+            if (ar < G.rec_cap) {
                 float* r = G.rec + (size_t)ar * GD_REC;
                 r[0] = rec.x.x; r[1] = rec.x.y; r[2] = rec.x.z;
                 r[3] = rec.nf.x; r[4] = rec.nf.y; r[5] = rec.nf.z;
@@ -956,7 +976,6 @@ __global__ __launch_bounds__(PT_BLOCK) void k_shade_guided(DevScene S, PathState
                 G.open_rec[p] = ar;
             }
         }
-        if (depth > 0) wave_count(i < n, &ps.cnt->extension_rays);
     }
 }
 
@@ -985,15 +1004,16 @@ __global__ __launch_bounds__(PT_BLOCK) void k_shade(DevScene S, PathState ps, Pt
             else
                 shade_unbiased(S, ps, P, depth, p, cont, shadow, so, sd, sc);
         }
-        int at = wave_append(cont, &ps.cnt->q_count[qsel ^ 1]);
-        if (cont) qn[at] = p;
-        int as = wave_append(shadow, &ps.cnt->shadow_count);
+        const bool pred[2] = {cont, shadow};
+        int32_t* const ctr[2] = {&ps.cnt->q_count[qsel ^ 1], &ps.cnt->shadow_count};
+        int pos[2];
+        block_append<2>(pred, ctr, pos);
+        if (cont) qn[pos[0]] = p;
         if (shadow) {
-            ps.sh_o[as] = so;
-            ps.sh_d[as] = sd;
-            ps.sh_c[as] = sc;
+            ps.sh_o[pos[1]] = so;
+            ps.sh_d[pos[1]] = sd;
+            ps.sh_c[pos[1]] = sc;
         }
-        if (depth > 0) wave_count(i < n, &ps.cnt->extension_rays);
     }
 }
 
@@ -1018,10 +1038,9 @@ __global__ __launch_bounds__(PT_BLOCK) void k_shadow(DevScene S, PathState ps, i
     __shared__ int32_t stack[LDSN * PT_BLOCK];
     LaneStack<LDSN> my(stack + threadIdx.x, PT_BLOCK);
     const int n = ps.cnt->shadow_count;
-    int cn = 0, cp = 0, traced = 0;
+    int cn = 0, cp = 0, item = 0;
     bool live = false, more = true;
     WaveFeed feed;
-    float4 c;
     Trav T;
     while (true) {
         bool want = !live && more;
@@ -1030,7 +1049,7 @@ __global__ __launch_bounds__(PT_BLOCK) void k_shadow(DevScene S, PathState ps, i
             if (want) {
                 if (i < n) {
                     float4 a = ps.sh_o[i], b = ps.sh_d[i];
-                    c = ps.sh_c[i];
+                    item = i;
                     trav_init(S, T, xyz(a), xyz(b), a.w, b.w);
                     live = true;
                 } else {
@@ -1041,6 +1060,8 @@ __global__ __launch_bounds__(PT_BLOCK) void k_shadow(DevScene S, PathState ps, i
         if (!__ballot(live)) break;  /* queue drained and every lane idle */
         if (live && (T.cur < 0 || trav_step<true, false>(S, T, my, cn, cp))) {
             if (T.g < 0) {
+                /* unoccluded: add the contribution to its path */
+                float4 c = ps.sh_c[item];
                 int p = __float_as_int(c.w);
                 if (compat) {
                     ps.pend_c[p] = make_float4(c.x, c.y, c.z, 0.f);
@@ -1049,12 +1070,9 @@ __global__ __launch_bounds__(PT_BLOCK) void k_shadow(DevScene S, PathState ps, i
                     ps.radiance[p] = make_float4(r.x + c.x, r.y + c.y, r.z + c.z, 0.f);
                 }
             }
-            ++traced;
             live = false;
         }
     }
-    for (int o = 32; o > 0; o >>= 1) traced += __shfl_xor(traced, o);
-    if (__lane_id() == 0 && traced) atomicAdd(&ps.cnt->shadow_rays, (unsigned long long)traced);
 }
 
 hipError_t launch_shadow(const HostDevScene& s, const PathState& ps, PtParams p, int stack, int grid,
@@ -1068,16 +1086,22 @@ hipError_t launch_shadow(const HostDevScene& s, const PathState& ps, PtParams p,
     return hipGetLastError();
 }
 
-__global__ void k_advance(PathState ps, int qsel)
+/* End of a bounce: count the rays the bounce traced (its queue length and
+ * its shadow-ray count; the queue of bounce 0 holds the camera rays, counted
+ * by k_generate), then clear the consumed queue and the work counters. */
+__global__ void k_advance(PathState ps, int qsel, int depth)
 {
-    ps.cnt->q_count[qsel] = 0;
-    ps.cnt->shadow_count = 0;
-    ps.cnt->ext_next = 0;
-    ps.cnt->sh_next = 0;
+    DevCounters* c = ps.cnt;
+    if (depth > 0) c->extension_rays += (unsigned long long)c->q_count[qsel];
+    c->shadow_rays += (unsigned long long)c->shadow_count;
+    c->q_count[qsel] = 0;
+    c->shadow_count = 0;
+    c->ext_next = 0;
+    c->sh_next = 0;
 }
-hipError_t launch_advance(const PathState& ps, int qsel, hipStream_t st)
+hipError_t launch_advance(const PathState& ps, int qsel, int depth, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, st, ps, qsel);
+    hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, st, ps, qsel, depth);
     return hipGetLastError();
 }
 
