@@ -13,7 +13,8 @@
  * Every kernel is persistent (a fixed grid strides over a queue whose length
  * it reads from device memory), so the host never waits inside a pass and
  * the whole sequence can be replayed as one hipGraph.  Queues are compacted
- * with a wave-wide ballot + mbcnt prefix and one atomic per wave.
+ * with a ballot + mbcnt prefix per wave, an LDS prefix over the block's
+ * waves and one atomic per block.
  */
 #include "rt_device.h"
 #include "rt_guide.h"
