@@ -828,7 +828,7 @@ static int guide_grad_on(rt_ctx* c, int32_t n, GuideState* used)
     PathState none{};
     int grid = std::max(1, std::min(grid_for(c), (n + 255) / 256));
     HIP_TRY(c, launch_guide_weights(g, none, 0, n, grid, c->stream));
-    int tgrid = std::max(1, std::min(grid_for(c), (n + 63) / 64));
+    int tgrid = std::max(1, std::min(c->n_cu * GD_TRAIN_BLOCKS_PER_CU, (n + 63) / 64));
     HIP_TRY(c, launch_guide_train(g, g.rec, g.rec_count, 0, n, tgrid, c->stream));
     if (used) *used = g;
     return 0;
@@ -1020,7 +1020,9 @@ static int render_impl(rt_ctx* c, const rt_view* views, int32_t nv, const rt_par
             QCHK(mark());
             if (guided) {
                 /* the last bounce neither samples the guide nor weights against it */
-                if (dd + 1 < depth) QCHK(launch_guide_infer(c->scene, c->ps, c->guide, q, grid, c->stream));
+                if (dd + 1 < depth)
+                    QCHK(launch_guide_infer(c->scene, c->ps, c->guide, q, c->n_cu * GD_INFER_BLOCKS_PER_CU,
+                                            c->stream));
                 QCHK(mark());
                 QCHK(launch_shade_guided(c->scene, c->ps, c->guide, pp, dd, q, grid, c->stream));
             } else {
@@ -1041,8 +1043,9 @@ static int render_impl(rt_ctx* c, const rt_view* views, int32_t nv, const rt_par
                 const int off = (k % GD_TRAIN_CHUNKS) * batch;
                 ++c->guide_step;
                 QCHK(launch_guide_zero_acc(c->guide, c->stream));
-                QCHK(launch_guide_weights(c->guide, c->ps, off, batch, grid, c->stream));
-                QCHK(launch_guide_train(c->guide, c->guide.rec, c->guide.rec_count, off, batch, grid, c->stream));
+                QCHK(launch_guide_weights(c->guide, c->ps, off, batch, c->n_cu * 4, c->stream));
+                QCHK(launch_guide_train(c->guide, c->guide.rec, c->guide.rec_count, off, batch,
+                                        c->n_cu * GD_TRAIN_BLOCKS_PER_CU, c->stream));
                 QCHK(launch_guide_adam(c->guide, lr, c->guide_step, c->guide.rec_count, off, batch, c->stream));
             }
             QCHK(mark());

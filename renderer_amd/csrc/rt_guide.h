@@ -50,6 +50,12 @@
 #define GD_PARAMS 48     /* floats of head output per path */
 #define GD_WCLAMP 100.0f /* cap of a normalised sample weight */
 #define GD_TRAIN_CHUNKS 4 /* training steps per render pass (record capacity = 4 batches) */
+/* Persistent grids of the network kernels: the blocks one CU holds at once
+ * (LDS: 46 KB per inference block, 138 KB per training block).  More blocks
+ * would only reload the weights into LDS and, for training, add one round
+ * of gradient atomics per block. */
+#define GD_INFER_BLOCKS_PER_CU 3
+#define GD_TRAIN_BLOCKS_PER_CU 1
 
 /* Training record (16 floats):
  *   0..2 position, 3..5 viewer-side normal, 6..8 outgoing direction,

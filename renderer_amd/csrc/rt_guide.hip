@@ -344,8 +344,16 @@ __global__ __launch_bounds__(GB) void k_guide_weights(GuideState g, PathState ps
         }
         local += w;
     }
+    /* one atomic per block: same-address atomics serialise */
+    __shared__ float part[GB / 64];
     for (int o = 32; o > 0; o >>= 1) local += __shfl_xor(local, o);
-    if (__lane_id() == 0 && local != 0.f) atomicAdd(&g.acc[0], local);
+    if (__lane_id() == 0) part[threadIdx.x >> 6] = local;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float s = 0.f;
+        for (int k = 0; k < GB / 64; ++k) s += part[k];
+        if (s != 0.f) atomicAdd(&g.acc[0], s);
+    }
 }
 
 hipError_t launch_guide_weights(const GuideState& g, const PathState& ps, int off, int max_n, int grid,
