@@ -150,14 +150,25 @@ def equal_time(scene, desc, max_depth, budget_s, ref_spp, pass_spp, train_passes
                 net[k] += s["guide_" + k]
         return f
 
-    # reference: plain PT at ref spp, rendered in passes
+    # reference: plain PT at ref spp, rendered in K passes.  Its own noise
+    # adds to every measured relMSE, E[(I - R)^2] = Var I + Var R, which
+    # pulls the guided/plain ratio towards 1 when the compared images are not
+    # far below the reference's spp; the spread of the K passes estimates
+    # Var R, and the corrected relMSE subtracts its relMSE share.
     t0 = time.time()
     ref = torch.zeros((H, W, 3), dtype=torch.float64, device=dev)
+    ref2 = torch.zeros_like(ref)
     per = min(256, ref_spp)
-    for k in range(ref_spp // per):
-        ref += render(per, 100000 + k, MODE_UNBIASED).double()
-    ref = (ref / (ref_spp // per)).cpu().numpy()
-    log(f"reference {ref_spp} spp in {time.time() - t0:.1f} s")
+    K = max(2, ref_spp // per)
+    for k in range(K):
+        f = render(per, 100000 + k, MODE_UNBIASED).double()
+        ref += f
+        ref2 += f * f
+    ref /= K
+    var_ref = torch.clamp(ref2 - K * ref * ref, min=0.0) / (K * (K - 1))
+    ref = ref.cpu().numpy()
+    ref_noise = float((var_ref.cpu().numpy() / (ref ** 2 + 1e-2)).mean())
+    log(f"reference {K * per} spp in {time.time() - t0:.1f} s (its relMSE share {ref_noise:.3g})")
 
     def progressive(mode, budget, training):
         acc = torch.zeros((H, W, 3), dtype=torch.float64, device=dev)
@@ -174,7 +185,7 @@ def equal_time(scene, desc, max_depth, budget_s, ref_spp, pass_spp, train_passes
             torch.cuda.synchronize()
             el += time.perf_counter() - t_start
             n += 1
-            curve.append((el, n * pass_spp, rel_mse((acc / n).cpu().numpy(), ref)))
+            curve.append((el, n * pass_spp, rel_mse((acc / n).cpu().numpy(), ref) - ref_noise))
             if el >= budget:
                 break
         return curve
@@ -195,7 +206,9 @@ def equal_time(scene, desc, max_depth, budget_s, ref_spp, pass_spp, train_passes
     return {
         "scene": desc,
         "budget_s": budget_s,
-        "reference_spp": ref_spp,
+        "reference_spp": K * per,
+        "reference_relmse_share": round(ref_noise, 6),
+        "relmse": "against the reference, minus the reference's own share (estimated from its passes)",
         "pass_spp": pass_spp,
         "plain": {"relmse": round(rp, 6), "spp": plain[-1][1], "seconds": round(plain[-1][0], 3)},
         "guided": {"relmse": round(guided[-1][2], 6) if guided else None, "spp": guided[-1][1] if guided else 0,
