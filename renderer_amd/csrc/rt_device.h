@@ -188,7 +188,8 @@ struct TraceOut {
  * closest hit so far, the stack of deferred subtrees in LDS laid out
  * [depth][thread] (the 64 lanes of a wave hit 64 different banks).
  *
- * Box tests use t = lo * inv - o * inv (one FMA per plane) with a zero
+ * Box tests use t = (lo - o) * inv (no per-ray o * inv kept: three fewer
+ * VGPRs let 8 waves per SIMD fit) with a zero
  * direction component replaced by a tiny signed value, and accept a box
  * when  max(near planes, tmin) <= min(far planes, closest) * (1 + 2^-20):
  * the relative slack absorbs the rounding of the plane distances, so a box
@@ -196,9 +197,12 @@ struct TraceOut {
  * (src/axisalignedbox.cpp:30-73) would accept it.  Box tests only prune;
  * the reported hit comes from the primitive tests above.
  */
+/* `best` starts one ulp above tmax, so that "t < best" alone enforces the
+ * inclusive t <= tmax of the reference's range test (no float lies between
+ * tmax and its successor) and the range end needs no register of its own. */
 struct Trav {
-    vec3 o, d, inv, oi;
-    float tmin, tmax, best, u, v;
+    vec3 o, d, inv;
+    float tmin, best, u, v;
     int32_t g, cur, sp;
 };
 
@@ -213,19 +217,17 @@ __device__ __forceinline__ void trav_init(const DevScene& S, Trav& T, vec3 o, ve
     T.o = o;
     T.d = d;
     T.inv = mk(safe_inv(d.x), safe_inv(d.y), safe_inv(d.z));
-    T.oi = mk(o.x * T.inv.x, o.y * T.inv.y, o.z * T.inv.z);
     T.tmin = tmin;
-    T.tmax = tmax;
-    T.best = RT_INF;
+    T.best = nextafterf(tmax, RT_INF);
     T.u = T.v = 0.f;
     T.g = -1;
     T.sp = 0;
     T.cur = S.n_nodes > 0 && tmin <= tmax ? 0 : -1;
 }
 
-__device__ __forceinline__ void box2(const Trav& T, float lo, float hi, float inv, float oi, float& n, float& f)
+__device__ __forceinline__ void box2(const Trav& T, float lo, float hi, float inv, float o, float& n, float& f)
 {
-    float a = fmaf(lo, inv, -oi), b = fmaf(hi, inv, -oi);
+    float a = (lo - o) * inv, b = (hi - o) * inv;
     bool pos = inv >= 0.f;
     n = pos ? a : b;
     f = pos ? b : a;
@@ -276,17 +278,17 @@ __device__ __forceinline__ bool trav_step(const DevScene& S, Trav& T, Stack& sta
     float4 f0 = nd[0], f1 = nd[1], f2 = nd[2], f3 = nd[3];
     if (COUNT) ++cn;
     int32_t c0 = __float_as_int(f3.x), c1 = __float_as_int(f3.y);
-    float bound = fminf(T.tmax, T.best);
+    float bound = T.best;
     bound = bound + fabsf(bound) * 9.5367431640625e-07f;
     float nx, ny, nz, fx, fy, fz;
-    box2(T, f0.x, f0.y, T.inv.x, T.oi.x, nx, fx);
-    box2(T, f0.z, f0.w, T.inv.y, T.oi.y, ny, fy);
-    box2(T, f1.x, f1.y, T.inv.z, T.oi.z, nz, fz);
+    box2(T, f0.x, f0.y, T.inv.x, T.o.x, nx, fx);
+    box2(T, f0.z, f0.w, T.inv.y, T.o.y, ny, fy);
+    box2(T, f1.x, f1.y, T.inv.z, T.o.z, nz, fz);
     float tl0 = fmaxf(fmaxf(nx, ny), fmaxf(nz, T.tmin));
     float th0 = fminf(fminf(fx, fy), fminf(fz, bound));
-    box2(T, f1.z, f1.w, T.inv.x, T.oi.x, nx, fx);
-    box2(T, f2.x, f2.y, T.inv.y, T.oi.y, ny, fy);
-    box2(T, f2.z, f2.w, T.inv.z, T.oi.z, nz, fz);
+    box2(T, f1.z, f1.w, T.inv.x, T.o.x, nx, fx);
+    box2(T, f2.x, f2.y, T.inv.y, T.o.y, ny, fy);
+    box2(T, f2.z, f2.w, T.inv.z, T.o.z, nz, fz);
     float tl1 = fmaxf(fmaxf(nx, ny), fmaxf(nz, T.tmin));
     float th1 = fminf(fminf(fx, fy), fminf(fz, bound));
     bool h0 = tl0 <= th0 + fabsf(th0) * 9.5367431640625e-07f;
@@ -318,10 +320,10 @@ __device__ __forceinline__ bool trav_step(const DevScene& S, Trav& T, Stack& sta
             float t, u = 0.f, v = 0.f;
             bool hit;
             if (kind == RT_KIND_SPHERE) {
-                hit = hit_sphere(T.o, T.d, r0, r1, T.tmin, T.tmax, T.best, t);
+                hit = hit_sphere(T.o, T.d, r0, r1, T.tmin, RT_INF, T.best, t);
             } else {
                 float4 r2 = pr[2];
-                hit = hit_tri(T.o, T.d, r0, r1, r2, T.tmin, T.tmax, T.best, t, u, v, kind == RT_KIND_PARA);
+                hit = hit_tri(T.o, T.d, r0, r1, r2, T.tmin, RT_INF, T.best, t, u, v, kind == RT_KIND_PARA);
             }
             if (hit) {
                 T.best = t;
