@@ -313,25 +313,34 @@ __device__ __forceinline__ bool trav_step(const DevScene& S, Trav& T, Stack& sta
         }
         uint32_t first, count, kind;
         leaf_decode(c, first, count, kind);
-        for (uint32_t j = 0; j < count; ++j) {
-            const float4* pr = S.prims + 3 * (size_t)(first + j);
-            float4 r0 = pr[0], r1 = pr[1];
+        /* records two at a time: both loads are in flight before the first
+         * test (a single last record is loaded twice, the second from cache);
+         * tests stay in record order, so the closest hit is unchanged (and
+         * measured faster than one at a time for any-hit rays too) */
+        auto test = [&](float4 r0, float4 r1, float4 r2) {
             if (COUNT) ++cp;
             float t, u = 0.f, v = 0.f;
             bool hit;
-            if (kind == RT_KIND_SPHERE) {
+            if (kind == RT_KIND_SPHERE)
                 hit = hit_sphere(T.o, T.d, r0, r1, T.tmin, RT_INF, T.best, t);
-            } else {
-                float4 r2 = pr[2];
+            else
                 hit = hit_tri(T.o, T.d, r0, r1, r2, T.tmin, RT_INF, T.best, t, u, v, kind == RT_KIND_PARA);
-            }
             if (hit) {
                 T.best = t;
                 T.u = u;
                 T.v = v;
                 T.g = __float_as_int(r0.w);
-                if (ANY) return true;
             }
+            return hit;
+        };
+        for (uint32_t j = 0; j < count; j += 2) {
+            const float4* pa = S.prims + 3 * (size_t)(first + j);
+            const bool two = j + 1 < count;
+            const float4* pb = two ? pa + 3 : pa;
+            float4 a0 = pa[0], a1 = pa[1], a2 = pa[2];
+            float4 b0 = pb[0], b1 = pb[1], b2 = pb[2];
+            if (test(a0, a1, a2) && ANY) return true;
+            if (two && test(b0, b1, b2) && ANY) return true;
         }
     }
     if (next >= 0) {
