@@ -421,6 +421,37 @@ __device__ __forceinline__ bool add_emission(const DevScene& S, const PathState&
     return true;
 }
 
+/* Pixel of the q-th slot of a tile's sample block.  A tile of whole image
+ * rows is walked in 8x8 pixel blocks (bands of 8 rows, blocks left to right;
+ * a narrower last block and a shorter last band keep row order), so the 64
+ * camera rays of a wave start from one compact block rather than a 64-pixel
+ * row and share more BVH nodes.  Any other tile is walked in row order.
+ * This is a bijection on [0, npix); the random stream is keyed by the pixel,
+ * so the film does not depend on the order. */
+__device__ __forceinline__ int tile_pixel(const DevTile& T, const DevView& V, int q)
+{
+    const int W = V.W;
+    if (W < 8 || T.pix0 % W != 0 || T.npix % W != 0) return T.pix0 + q;
+    const int band = 8 * W, g = q / band, r = q - g * band;
+    const int rows = min(8, T.npix / W - 8 * g);
+    int x, y;
+    if (rows < 8) {
+        x = r % W;
+        y = r / W;
+    } else {
+        const int full = W - W % 8;  /* columns covered by whole 8x8 blocks */
+        if (r < 8 * full) {
+            x = (r >> 6) * 8 + (r & 7);
+            y = (r >> 3) & 7;
+        } else {
+            const int rr = r - 8 * full, w = W - full;
+            x = full + rr % w;
+            y = rr / w;
+        }
+    }
+    return T.pix0 + (8 * g + y) * W + x;
+}
+
 /* ===========================================================================
  * generate
  * ========================================================================= */
@@ -447,7 +478,7 @@ __global__ __launch_bounds__(PT_BLOCK) void k_generate(PathState ps, const DevVi
         const DevTile T = tiles[lo];
         const DevView& V = views[T.view];
         int j = slot - T.slot0;
-        int pixel = T.pix0 + j % T.npix;
+        int pixel = tile_pixel(T, V, j % T.npix);
         int s = T.s0 + j / T.npix;
         U4 g0 = rng4(P.seed, (uint32_t)T.view, (uint32_t)pixel, (uint32_t)s, 0u);
         int a = s / V.spp_y, b = s % V.spp_y;
@@ -1125,8 +1156,8 @@ __global__ __launch_bounds__(PT_BLOCK) void k_accumulate(PathState ps, const Dev
         }
         const DevTile T = tiles[lo];
         const DevView& V = views[T.view];
-        int local = j - T.acc0;
-        int pixel = T.pix0 + local;
+        int local = j - T.acc0;  /* slot offset within the tile's sample block */
+        int pixel = tile_pixel(T, V, local);
         float* f = V.film + 3 * (size_t)pixel;
         float r = f[0], g = f[1], b = f[2];
         for (int s = 0; s < T.ns; ++s) {
