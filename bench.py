@@ -24,7 +24,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 RAY_BYTES = 32 + 16      # ray (origin, tmin, direction, tmax) + hit (t, primitive, u, v)
-NODE_BYTES = 64          # one interior node: both child boxes + child words
+NODE_BYTES = 128         # one 4-wide node: four child boxes + child words
 PRIM_BYTES = 48          # one primitive record
 
 

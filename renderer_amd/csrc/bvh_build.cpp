@@ -75,42 +75,89 @@ void partition(std::vector<PrimRef>& refs, size_t b, size_t e, int level)
     }
 }
 
-struct Layout {
+/* Binary tree of the median splits (intermediate form). */
+struct BinNode {
+    Box3 box[2];
+    int32_t child[2];  // >= 0: binary interior node, < 0: leaf word
+};
+
+struct Binary {
     const std::vector<PrimRef>& refs;
-    std::vector<float>& nodes;
-    int max_depth = 0;
+    std::vector<BinNode> nodes;
 
     // Returns the child word for range [b, e) and its box.
-    int32_t emit(size_t b, size_t e, int depth, Box3& box)
+    int32_t emit(size_t b, size_t e, Box3& box)
     {
-        max_depth = std::max(max_depth, depth);
         if (is_leaf(refs, b, e)) {
             for (size_t i = b; i < e; ++i) box.grow(refs[i].lo, refs[i].hi);
             uint32_t x = ((uint32_t)b << 5) | ((uint32_t)refs[b].kind << 3) | (uint32_t)(e - b);
             return (int32_t)~x;
         }
-        size_t idx = nodes.size() / 16;
-        nodes.resize(nodes.size() + 16);
+        size_t idx = nodes.size();
+        nodes.emplace_back();
         size_t mid = b + (e - b) / 2;
-        Box3 l, r;
-        int32_t cl = emit(b, mid, depth + 1, l);
-        int32_t cr = emit(mid, e, depth + 1, r);
-        write(idx, l, r, cl, cr);
-        box.grow(l.lo, l.hi);
-        box.grow(r.lo, r.hi);
+        BinNode n;
+        n.child[0] = emit(b, mid, n.box[0]);
+        n.child[1] = emit(mid, e, n.box[1]);
+        nodes[idx] = n;
+        box.grow(n.box[0].lo, n.box[0].hi);
+        box.grow(n.box[1].lo, n.box[1].hi);
         return (int32_t)idx;
     }
-    void write(size_t idx, const Box3& l, const Box3& r, int32_t cl, int32_t cr)
+};
+
+/* 4-wide nodes: each binary node absorbs its interior children, so a node
+ * holds its grandchildren (2 to 4 children) and the tree is half as deep. */
+struct Wide {
+    const std::vector<BinNode>& bin;
+    std::vector<float>& nodes;
+    int max_depth = 0;
+
+    // Returns the 4-wide node index for binary interior node b, and the
+    // stack entries a traversal below it can need at once (it continues
+    // with one hit interior child and defers the others).
+    int32_t collapse(int32_t b, int depth, int& need)
     {
-        float* n = &nodes[idx * 16];
-        n[0] = l.lo[0]; n[1] = l.hi[0]; n[2] = l.lo[1]; n[3] = l.hi[1];
-        n[4] = l.lo[2]; n[5] = l.hi[2]; n[6] = r.lo[0]; n[7] = r.hi[0];
-        n[8] = r.lo[1]; n[9] = r.hi[1]; n[10] = r.lo[2]; n[11] = r.hi[2];
-        std::memcpy(&n[12], &cl, 4);
-        std::memcpy(&n[13], &cr, 4);
-        int32_t z = 0;
-        std::memcpy(&n[14], &z, 4);
-        std::memcpy(&n[15], &z, 4);
+        max_depth = std::max(max_depth, depth);
+        int32_t word[4];
+        const Box3* box[4];
+        int m = 0;
+        for (int s = 0; s < 2; ++s) {
+            int32_t c = bin[b].child[s];
+            if (c >= 0) {
+                for (int t = 0; t < 2; ++t) {
+                    word[m] = bin[c].child[t];
+                    box[m] = &bin[c].box[t];
+                    ++m;
+                }
+            } else {
+                word[m] = c;
+                box[m] = &bin[b].box[s];
+                ++m;
+            }
+        }
+        size_t idx = nodes.size() / RT_NODE_FLOATS;
+        nodes.resize(nodes.size() + RT_NODE_FLOATS, 0.f);
+        int interior = 0, deepest = 0;
+        for (int k = 0; k < m; ++k)
+            if (word[k] >= 0) {
+                int sub = 0;
+                word[k] = collapse(word[k], depth + 1, sub);
+                deepest = std::max(deepest, sub);
+                ++interior;
+            }
+        need = std::max(0, interior - 1) + deepest;
+        float* n = &nodes[idx * RT_NODE_FLOATS];
+        for (int k = 0; k < 4; ++k) {
+            bool used = k < m;
+            for (int a = 0; a < 3; ++a) {
+                n[8 * a + k] = used ? box[k]->lo[a] : INFINITY;       // lo of axis a
+                n[8 * a + 4 + k] = used ? box[k]->hi[a] : -INFINITY;  // hi of axis a
+            }
+            int32_t w = used ? word[k] : (int32_t)~(uint32_t)0;  // empty leaf, never hit
+            std::memcpy(&n[24 + k], &w, 4);
+        }
+        return (int32_t)idx;
     }
 };
 
@@ -120,27 +167,31 @@ int build(std::vector<PrimRef>& refs, BuildResult& out)
 {
     out.nodes.clear();
     out.depth = 0;
+    out.stack_need = 0;
     if (refs.size() >= (size_t(1) << 26)) return -1;
     for (auto& r : refs)
         for (int k = 0; k < 3; ++k) r.c[k] = 0.5f * (r.lo[k] + r.hi[k]);
     if (refs.empty()) return 0;
     partition(refs, 0, refs.size(), 0);
-    Layout lay{refs, out.nodes};
-    if (is_leaf(refs, 0, refs.size())) {
-        // A root that is a leaf still gets an interior node: child 1 is an
-        // empty leaf whose box is empty (never hit).
-        out.nodes.resize(16);
-        Box3 l, r;
-        int32_t cl = lay.emit(0, refs.size(), 1, l);
-        int32_t cr = (int32_t)~(uint32_t)0;
-        lay.write(0, l, r, cl, cr);
-        out.depth = 1;
-        return 0;
-    }
+    Binary bt{refs, {}};
+    bt.nodes.reserve(refs.size() / 2 + 1);
     Box3 root;
-    lay.emit(0, refs.size(), 0, root);
-    out.depth = lay.max_depth;
-    if (out.depth >= RT_STACK_DEPTH) return -2;
+    int32_t r = bt.emit(0, refs.size(), root);
+    if (r < 0) {
+        // A root that is a leaf still gets an interior node.
+        BinNode n;
+        n.box[0] = root;
+        n.child[0] = r;
+        n.child[1] = (int32_t)~(uint32_t)0;  // empty leaf with an empty box
+        bt.nodes.push_back(n);
+        r = 0;
+    }
+    Wide wt{bt.nodes, out.nodes};
+    int need = 0;
+    wt.collapse(r, 0, need);
+    out.depth = wt.max_depth + 1;
+    out.stack_need = need;
+    if (need + 1 > RT_STACK_DEPTH) return -2;
     return 0;
 }
 

@@ -22,12 +22,13 @@ struct PrimRef {
 };
 
 struct BuildResult {
-    std::vector<float> nodes;  // 16 floats (64 B) per interior node
-    int depth = 0;             // deepest interior level
+    std::vector<float> nodes;  // RT_NODE_FLOATS floats (128 B) per 4-wide node
+    int depth = 0;             // levels of 4-wide nodes
+    int stack_need = 0;        // most traversal-stack entries a ray can hold at once
 };
 
 // Returns 0 on success, -1 if there are too many primitives for the leaf
-// encoding, -2 if the tree is deeper than the traversal stack.
+// encoding, -2 if a traversal could need more stack than RT_STACK_DEPTH.
 int build(std::vector<PrimRef>& refs, BuildResult& out);
 
 }  // namespace rtb

@@ -42,6 +42,7 @@ struct rt_ctx {
     HostDevScene scene{};
     DevBuf nodes, prims, prim_id, mat, vnormals, geo, mats, lights, env_rgb, env_row, env_col, env_p;
     int bvh_depth = 0;
+    int bvh_stack = 0;  /* most stack entries a traversal can need */
 
     // ray batches (host entry points)
     DevBuf rb_o, rb_d, rb_out0, rb_out1;
@@ -482,7 +483,7 @@ int rt_scene_upload(rt_ctx* c, const rt_scene_desc* d)
     s.mat = (const int32_t*)c->mat.p;
     s.vnormals = vn.empty() ? nullptr : (const float*)c->vnormals.p;
     s.geo = c->geo.p;
-    s.n_nodes = (int32_t)(br.nodes.size() / 16);
+    s.n_nodes = (int32_t)(br.nodes.size() / RT_NODE_FLOATS);
     s.n_tri = nT;
     s.n_sphere = nS;
     s.n_para = nP;
@@ -498,6 +499,7 @@ int rt_scene_upload(rt_ctx* c, const rt_scene_desc* d)
     s.env_texel_p = (const float*)c->env_p.p;
     c->has_scene = true;
     c->bvh_depth = br.depth;
+    c->bvh_stack = br.stack_need;
     c->stats.bvh_nodes = s.n_nodes;
     c->stats.bvh_prims = G;
     c->stats.bvh_depth = br.depth;
@@ -978,7 +980,8 @@ static int render_impl(rt_ctx* c, const rt_view* views, int32_t nv, const rt_par
                 (float)std::min(1.0, (double)GD_TRAIN_CHUNKS * batch / std::max(1.0, (double)P * 6.0))};
     const int grid = grid_for(c);
     const int depth = prm->max_depth;
-    const int stack = c->bvh_depth < 24 ? 24 : RT_STACK_DEPTH;
+    /* traversal stack capacity: 24 entries when the tree needs no more */
+    const int stack = c->bvh_stack + 1 <= 24 ? 24 : RT_STACK_DEPTH;
     /* Every pass enqueues the same kernels with the same arguments (the pass
      * index lives in the device counters), so one captured pass can be
      * replayed as a hipGraph - except with training, whose Adam step number

@@ -266,82 +266,103 @@ struct LaneStack {
     }
 };
 
-/* Visit node T.cur (T.cur >= 0): test both child boxes, test the
- * primitives of hit leaves, continue with the nearer hit interior child
- * and defer the other; pop when neither is hit.  Returns true when the
- * traversal of this ray is complete.  The stack needs room for the BVH
- * depth. */
+/* Visit 4-wide node T.cur (T.cur >= 0): test the 2-4 child boxes, test
+ * the primitives of hit leaves (nearest box first), continue with the
+ * nearest hit interior child and defer the others, nearest on top; pop when
+ * none is hit.  Children whose box entry lies beyond the closest hit found
+ * meanwhile are dropped.  Any-hit rays skip the ordering.  Returns true
+ * when the traversal of this ray is complete.  The stack needs room for the
+ * builder's stack_need. */
 template <bool ANY, bool COUNT, class Stack>
 __device__ __forceinline__ bool trav_step(const DevScene& S, Trav& T, Stack& stack, int& cn, int& cp)
 {
-    const float4* nd = S.nodes + 4 * (size_t)T.cur;
-    float4 f0 = nd[0], f1 = nd[1], f2 = nd[2], f3 = nd[3];
+    const float4* nd = S.nodes + (RT_NODE_FLOATS / 4) * (size_t)T.cur;
+    const float4 lx = nd[0], hx = nd[1], ly = nd[2], hy = nd[3], lz = nd[4], hz = nd[5], cw = nd[6];
     if (COUNT) ++cn;
-    int32_t c0 = __float_as_int(f3.x), c1 = __float_as_int(f3.y);
+    const float slack = 9.5367431640625e-07f; /* 2^-20 */
     float bound = T.best;
-    bound = bound + fabsf(bound) * 9.5367431640625e-07f;
-    float nx, ny, nz, fx, fy, fz;
-    box2(T, f0.x, f0.y, T.inv.x, T.o.x, nx, fx);
-    box2(T, f0.z, f0.w, T.inv.y, T.o.y, ny, fy);
-    box2(T, f1.x, f1.y, T.inv.z, T.o.z, nz, fz);
-    float tl0 = fmaxf(fmaxf(nx, ny), fmaxf(nz, T.tmin));
-    float th0 = fminf(fminf(fx, fy), fminf(fz, bound));
-    box2(T, f1.z, f1.w, T.inv.x, T.o.x, nx, fx);
-    box2(T, f2.x, f2.y, T.inv.y, T.o.y, ny, fy);
-    box2(T, f2.z, f2.w, T.inv.z, T.o.z, nz, fz);
-    float tl1 = fmaxf(fmaxf(nx, ny), fmaxf(nz, T.tmin));
-    float th1 = fminf(fminf(fx, fy), fminf(fz, bound));
-    bool h0 = tl0 <= th0 + fabsf(th0) * 9.5367431640625e-07f;
-    bool h1 = tl1 <= th1 + fabsf(th1) * 9.5367431640625e-07f;
-    bool swap = !ANY && h0 && h1 && tl1 < tl0;
-    int32_t a = swap ? c1 : c0, b = swap ? c0 : c1;
-    bool ha = swap ? h1 : h0, hb = swap ? h0 : h1;
-    int32_t next = -1;
+    bound = bound + fabsf(bound) * slack;
+    const float lo_x[4] = {lx.x, lx.y, lx.z, lx.w}, hi_x[4] = {hx.x, hx.y, hx.z, hx.w};
+    const float lo_y[4] = {ly.x, ly.y, ly.z, ly.w}, hi_y[4] = {hy.x, hy.y, hy.z, hy.w};
+    const float lo_z[4] = {lz.x, lz.y, lz.z, lz.w}, hi_z[4] = {hz.x, hz.y, hz.z, hz.w};
+    float key[4];
+    int32_t word[4] = {__float_as_int(cw.x), __float_as_int(cw.y), __float_as_int(cw.z), __float_as_int(cw.w)};
 #pragma unroll
-    for (int side = 0; side < 2; ++side) {
-        int32_t c = side == 0 ? a : b;
-        bool hc = side == 0 ? ha : hb;
-        if (!hc) continue;
-        if (c >= 0) {
-            if (next < 0) {
-                next = c;
-            } else {
-                stack.put(T.sp, c);
-                ++T.sp;
-            }
-            continue;
+    for (int k = 0; k < 4; ++k) {
+        float nx, ny, nz, fx, fy, fz;
+        box2(T, lo_x[k], hi_x[k], T.inv.x, T.o.x, nx, fx);
+        box2(T, lo_y[k], hi_y[k], T.inv.y, T.o.y, ny, fy);
+        box2(T, lo_z[k], hi_z[k], T.inv.z, T.o.z, nz, fz);
+        float tl = fmaxf(fmaxf(nx, ny), fmaxf(nz, T.tmin));
+        float th = fminf(fminf(fx, fy), fminf(fz, bound));
+        key[k] = tl <= th + fabsf(th) * slack ? tl : RT_INF;
+    }
+    if (!ANY) {
+        /* ascending box entry; misses (key = inf) go last */
+#define RT_CSWAP(i, j)                                  \
+    if (key[j] < key[i]) {                              \
+        float tk = key[i]; key[i] = key[j]; key[j] = tk; \
+        int32_t tw = word[i]; word[i] = word[j]; word[j] = tw; \
+    }
+        RT_CSWAP(0, 1) RT_CSWAP(2, 3) RT_CSWAP(0, 2) RT_CSWAP(1, 3) RT_CSWAP(1, 2)
+#undef RT_CSWAP
+    }
+    /* leaves, nearest first: records two at a time, both loads in flight
+     * before the first test (a single last record is loaded twice, the
+     * second from cache); tests stay in record order within a leaf */
+    auto test = [&](float4 r0, float4 r1, float4 r2, uint32_t kind) {
+        if (COUNT) ++cp;
+        float t, u = 0.f, v = 0.f;
+        bool hit;
+        if (kind == RT_KIND_SPHERE)
+            hit = hit_sphere(T.o, T.d, r0, r1, T.tmin, RT_INF, T.best, t);
+        else
+            hit = hit_tri(T.o, T.d, r0, r1, r2, T.tmin, RT_INF, T.best, t, u, v, kind == RT_KIND_PARA);
+        if (hit) {
+            T.best = t;
+            T.u = u;
+            T.v = v;
+            T.g = __float_as_int(r0.w);
         }
+        return hit;
+    };
+    /* one loop body over the hit leaves (a bit mask in slot order), so the
+     * record registers are allocated once rather than per slot;
+     * key = inf marks a missed box (a hit box is entered at a finite t) */
+    uint32_t leaves = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (word[k] < 0 && key[k] < RT_INF) leaves |= 1u << k;
+    while (leaves) {
+        const int k = __builtin_ctz(leaves);
+        leaves &= leaves - 1u;
+        const float kk = k == 0 ? key[0] : k == 1 ? key[1] : k == 2 ? key[2] : key[3];
+        const int32_t wk = k == 0 ? word[0] : k == 1 ? word[1] : k == 2 ? word[2] : word[3];
+        if (!(kk <= T.best + fabsf(T.best) * slack)) continue;
         uint32_t first, count, kind;
-        leaf_decode(c, first, count, kind);
-        /* records two at a time: both loads are in flight before the first
-         * test (a single last record is loaded twice, the second from cache);
-         * tests stay in record order, so the closest hit is unchanged (and
-         * measured faster than one at a time for any-hit rays too) */
-        auto test = [&](float4 r0, float4 r1, float4 r2) {
-            if (COUNT) ++cp;
-            float t, u = 0.f, v = 0.f;
-            bool hit;
-            if (kind == RT_KIND_SPHERE)
-                hit = hit_sphere(T.o, T.d, r0, r1, T.tmin, RT_INF, T.best, t);
-            else
-                hit = hit_tri(T.o, T.d, r0, r1, r2, T.tmin, RT_INF, T.best, t, u, v, kind == RT_KIND_PARA);
-            if (hit) {
-                T.best = t;
-                T.u = u;
-                T.v = v;
-                T.g = __float_as_int(r0.w);
-            }
-            return hit;
-        };
+        leaf_decode(wk, first, count, kind);
         for (uint32_t j = 0; j < count; j += 2) {
             const float4* pa = S.prims + 3 * (size_t)(first + j);
             const bool two = j + 1 < count;
             const float4* pb = two ? pa + 3 : pa;
             float4 a0 = pa[0], a1 = pa[1], a2 = pa[2];
             float4 b0 = pb[0], b1 = pb[1], b2 = pb[2];
-            if (test(a0, a1, a2) && ANY) return true;
-            if (two && test(b0, b1, b2) && ANY) return true;
+            if (test(a0, a1, a2, kind) && ANY) return true;
+            if (two && test(b0, b1, b2, kind) && ANY) return true;
         }
+    }
+    /* interior children, farthest first: each earlier choice is deferred,
+     * so the nearest continues and the next nearest is on top */
+    int32_t next = -1;
+    const float cut = T.best + fabsf(T.best) * slack;
+#pragma unroll
+    for (int k = 3; k >= 0; --k) {
+        if (word[k] < 0 || !(key[k] < RT_INF) || !(key[k] <= cut)) continue;
+        if (next >= 0) {
+            stack.put(T.sp, next);
+            ++T.sp;
+        }
+        next = word[k];
     }
     if (next >= 0) {
         T.cur = next;

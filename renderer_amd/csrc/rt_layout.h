@@ -2,17 +2,21 @@
  * rt_layout.h — HBM data layout shared by the host builder and the gfx950
  * kernels.  Plain structs only; included by C++ host code and HIP code.
  *
- * BVH: binary tree, flattened depth-first, one 64-byte node per interior
- * node holding the boxes of BOTH children (one 64-B fetch answers both
- * child tests), then 4 child words.  Leaves are not nodes: a child word < 0
- * names a run of at most 4 primitive records of one kind.
+ * BVH: 4-wide tree (the median-split binary tree with every node absorbing
+ * its interior children), flattened depth-first, one 128-byte node (one
+ * cache line) per interior node holding the boxes of its 2-4 children as
+ * structure of arrays, so one fetch answers all child tests, then the 4
+ * child words.  Leaves are not nodes: a child word < 0 names a run of at
+ * most 4 primitive records of one kind.
  *
- *   node[i].f0 = (c0.lo.x, c0.hi.x, c0.lo.y, c0.hi.y)
- *   node[i].f1 = (c0.lo.z, c0.hi.z, c1.lo.x, c1.hi.x)
- *   node[i].f2 = (c1.lo.y, c1.hi.y, c1.lo.z, c1.hi.z)
- *   node[i].f3 = (child0, child1, meta0, meta1)  (ints)
- *     child >= 0 : interior node index
- *     child <  0 : leaf, first record = -child - 1, meta = count | kind << 8
+ *   node[i].f0 = lo.x of children 0..3     node[i].f1 = hi.x of children 0..3
+ *   node[i].f2 = lo.y                      node[i].f3 = hi.y
+ *   node[i].f4 = lo.z                      node[i].f5 = hi.z
+ *   node[i].f6 = child words 0..3 (ints)   node[i].f7 = 0
+ *     word >= 0 : interior node index
+ *     word <  0 : leaf, ~word = first record << 5 | kind << 3 | count
+ *   An unused child slot has the empty box (lo = +inf, hi = -inf) and a
+ *   leaf word of count 0.
  *
  * Primitive records: 48 bytes = 3 x float4, in leaf order.
  *   triangle      (p1.xyz, g) (p2-p1, 0) (p3-p1, 0)   src/triangle.cpp:32-44
@@ -32,6 +36,7 @@
 
 #define RT_LEAF_MAX 4
 #define RT_STACK_DEPTH 32
+#define RT_NODE_FLOATS 32  /* 128 B per 4-wide node */
 
 /* Material record (64 B).  type: rt_material_type. */
 struct RtMaterial {
