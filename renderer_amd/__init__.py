@@ -7,10 +7,11 @@ from ._abi import (LIGHT_AREA, LIGHT_ENV, LIGHT_POINT, MAT_DARK, MAT_DIFFUSE, MA
 from .integrator import GpuContext, GpuUniDiPathTracingIntegrator, PerPixelRenderer
 from .scene import (AreaLight, Camera, DarkMatter, DiffuseMaterial, EnvironmentMap, IntersectableList, Mesh,
                     MirrorMaterial, PhongMaterial, PointLight, Quad, Scene, Sphere, Triangle)
+from .scene_file import SceneFile
 
 __all__ = [
     "GpuContext", "GpuUniDiPathTracingIntegrator", "PerPixelRenderer", "RtError",
     "AreaLight", "Camera", "DarkMatter", "DiffuseMaterial", "EnvironmentMap", "IntersectableList", "Mesh",
-    "MirrorMaterial", "PhongMaterial", "PointLight", "Quad", "Scene", "Sphere", "Triangle",
+    "MirrorMaterial", "PhongMaterial", "PointLight", "Quad", "Scene", "Sphere", "Triangle", "SceneFile",
     "MODE_COMPAT", "MODE_UNBIASED", "MODE_GUIDED",
 ]

@@ -4,20 +4,56 @@
  * SceneBuilder -> GpuUniDiPathTracingIntegrator -> PerPixelRenderer.
  *
  *   example_box <out.raw> [width height spp_x spp_y mode max_depth seed]
+ *   example_box <out.raw> --scene <file.scene> [spp_x spp_y mode max_depth seed]
  *
- * Writes height * width * 3 float32 RGB and prints the mean radiance.
+ * The second form reads the scene and its camera from a scene file
+ * (SceneFile, the reference's scene language).  Writes height * width * 3
+ * float32 RGB and prints the mean radiance.
  */
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <string>
+#include <vector>
 
 #include "gpu_integrator.h"
+
+static int write_film(const char* path, const std::vector<float>& film)
+{
+    double sum = 0;
+    for (float v : film) sum += v;
+    FILE* f = std::fopen(path, "wb");
+    if (!f || std::fwrite(film.data(), sizeof(float), film.size(), f) != film.size()) {
+        std::fprintf(stderr, "cannot write %s\n", path);
+        if (f) std::fclose(f);
+        return 1;
+    }
+    std::fclose(f);
+    std::printf("mean %.6f\n", sum / film.size());
+    return 0;
+}
 
 int main(int argc, char** argv)
 {
     if (argc < 2) {
         std::fprintf(stderr, "usage: %s out.raw [w h sx sy mode depth seed]\n", argv[0]);
         return 2;
+    }
+    if (argc > 3 && std::string(argv[2]) == "--scene") {
+        int sx = argc > 4 ? std::atoi(argv[4]) : 2, sy = argc > 5 ? std::atoi(argv[5]) : 2;
+        int mode = argc > 6 ? std::atoi(argv[6]) : RT_MODE_UNBIASED, depth = argc > 7 ? std::atoi(argv[7]) : 8;
+        unsigned long seed = argc > 8 ? std::strtoul(argv[8], nullptr, 10) : 3;
+        try {
+            rtgpu::SceneFile file(argv[3]);
+            const rt_scene_desc& d = file.scene();
+            rtgpu::GpuUniDiPathTracingIntegrator integrator(0.5f, mode, depth);
+            integrator.setScene(d);
+            rtgpu::PerPixelRenderer renderer(integrator, sx, sy, seed);
+            return write_film(argv[1], renderer.render(file.camera()));
+        } catch (const rtgpu::RtError& e) {
+            std::fprintf(stderr, "error %d: %s\n", e.status, e.what());
+            return 1;
+        }
     }
     int w = argc > 2 ? std::atoi(argv[2]) : 64, h = argc > 3 ? std::atoi(argv[3]) : 64;
     int sx = argc > 4 ? std::atoi(argv[4]) : 2, sy = argc > 5 ? std::atoi(argv[5]) : 2;
@@ -43,16 +79,7 @@ int main(int argc, char** argv)
         rtgpu::GpuUniDiPathTracingIntegrator integrator(0.5f, mode, depth);
         integrator.setScene(s);
         rtgpu::PerPixelRenderer renderer(integrator, sx, sy, seed);
-        std::vector<float> film = renderer.render(cam);
-        double sum = 0;
-        for (float v : film) sum += v;
-        FILE* f = std::fopen(argv[1], "wb");
-        if (!f || std::fwrite(film.data(), sizeof(float), film.size(), f) != film.size()) {
-            std::fprintf(stderr, "cannot write %s\n", argv[1]);
-            return 1;
-        }
-        std::fclose(f);
-        std::printf("mean %.6f\n", sum / film.size());
+        return write_film(argv[1], renderer.render(cam));
     } catch (const rtgpu::RtError& e) {
         std::fprintf(stderr, "error %d: %s\n", e.status, e.what());
         return 1;

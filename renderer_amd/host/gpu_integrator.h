@@ -14,6 +14,9 @@
  *                                 (src/unidipathtracingintegrator.h:30) on the GPU
  *   PerPixelRenderer              PerPixelRenderer::render (src/perpixelrenderer.cpp:43-101)
  *                                 with pt-x-samples / pt-y-samples / seed
+ *   SceneFile                     a scene file read by the library's front end
+ *                                 (SceneGrammar::parse + resolveVars + buildScene,
+ *                                 src/main.cpp:150-182; --dump-bvh)
  *
  * Header-only; link with -lrt_hip.  Errors throw RtError (C++ callers) while
  * the library itself only returns status codes.
@@ -27,6 +30,7 @@
 #include <vector>
 
 #include "../../include/rt_abi.h"
+#include "../../include/rt_scene_file.h"
 
 namespace rtgpu {
 
@@ -172,6 +176,46 @@ struct Camera {
     }
 };
 
+/* A scene file in the reference's scene language.  scene() evaluates it; the
+ * description stays valid while this object lives. */
+class SceneFile {
+public:
+    explicit SceneFile(const std::string& path)
+    {
+        int st = rt_scene_file_load(path.c_str(), &f_);
+        if (st != RT_OK) throw RtError(st, std::string("rt_scene_file_load: ") + rt_scene_file_last_error());
+    }
+    ~SceneFile() { rt_scene_file_free(f_); }
+    SceneFile(const SceneFile&) = delete;
+    SceneFile& operator=(const SceneFile&) = delete;
+
+    const rt_scene_desc& scene()
+    {
+        int st = rt_scene_file_scene(f_, &desc_, &cam_);
+        if (st != RT_OK) throw RtError(st, std::string("rt_scene_file_scene: ") + rt_scene_file_last_error());
+        return desc_;
+    }
+    /* the "camera" variable (call scene() first) */
+    Camera camera() const
+    {
+        return Camera{{cam_.eye[0], cam_.eye[1], cam_.eye[2]},
+                      {cam_.look_at[0], cam_.look_at[1], cam_.look_at[2]},
+                      {cam_.up[0], cam_.up[1], cam_.up[2]},
+                      cam_.fov, cam_.width, cam_.height};
+    }
+    void dumpBvh(const std::string& path) const
+    {
+        int st = rt_scene_file_dump_bvh(f_, path.c_str());
+        if (st != RT_OK) throw RtError(st, std::string("rt_scene_file_dump_bvh: ") + rt_scene_file_last_error());
+    }
+    std::string messages() const { return rt_scene_file_messages(f_); }
+
+private:
+    rt_scene_file* f_ = nullptr;
+    rt_scene_desc desc_{};
+    rt_camera cam_{};
+};
+
 /* Batch form of the reference's Integrator: one call renders a film. */
 class Integrator {
 public:
@@ -196,6 +240,7 @@ public:
         rt_scene_desc d = scene.desc();
         check(ctx_, rt_scene_upload(ctx_, &d), "rt_scene_upload");
     }
+    void setScene(const rt_scene_desc& d) { check(ctx_, rt_scene_upload(ctx_, &d), "rt_scene_upload"); }
     /* film: height * width * 3 RGB floats, accumulated as film += sample / spp */
     void render(const Camera& cam, int sppX, int sppY, unsigned long seed, float* film) override
     {

@@ -28,3 +28,20 @@ def test_cpp_host_matches_python_host(gpu_ctx, tmp_path, mode):
     py = gpu_ctx.render([(sc.camera, 2, 3)], mode=mode, max_depth=8, rr_start=3, rr_prob=0.5, seed=11)[0]
     assert np.array_equal(cpp, py)
     assert cpp.mean() > 0
+
+
+def test_cpp_scene_file_matches_python_host(gpu_ctx, tmp_path):
+    """example_box --scene: the box read from its scene file by the C++ host
+    (SceneFile) renders the same film as the Python host's box."""
+    if not os.path.exists(EXAMPLE):
+        pytest.fail("renderer_amd/host/example_box is not built (run __graft_entry__.build())")
+    scene = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "box.scene")
+    out = tmp_path / "film.raw"
+    r = subprocess.run([EXAMPLE, str(out), "--scene", scene, "1", "2", str(MODE_UNBIASED), "8", "7"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    cpp = np.fromfile(out, np.float32).reshape(256, 256, 3)
+    sc = scenes.box_scene(256, 256)
+    gpu_ctx.upload(sc.flatten())
+    py = gpu_ctx.render([(sc.camera, 1, 2)], mode=MODE_UNBIASED, max_depth=8, rr_start=3, rr_prob=0.5, seed=7)[0]
+    assert np.array_equal(cpp, py)
