@@ -47,6 +47,8 @@ def parse():
     ap.add_argument("--no-guided", action="store_true", help="skip the equal-time guided vs plain comparisons")
     ap.add_argument("--no-env", action="store_true", help="skip the env-lit mesh comparison")
     ap.add_argument("--no-batched", action="store_true", help="skip the 32-view batched render")
+    ap.add_argument("--no-scene-file", action="store_true", help="skip the scene-file front-end timing")
+    ap.add_argument("--no-film", action="store_true", help="skip the tonemapper and film-writer timing")
     ap.add_argument("--room-res", type=str, default="480x270")
     ap.add_argument("--room-subdiv", type=int, default=64, help="octahedron segments per edge of the 8 objects")
     ap.add_argument("--room-seconds", type=float, default=2.0, help="wall-time budget of each estimator")
@@ -260,6 +262,106 @@ def equal_time_env(args):
                       args.env_train_passes, args.env_epochs, args.env_lr)
 
 
+def scene_file_load(args):
+    """The scene-language front end on the benchmark mesh: the displaced
+    sphere written as an OBJ file, read through obj(...) of a scene file
+    (SceneGrammar::parse + resolveVars), evaluated into the scene description
+    (buildScene), and dumped as --dump-bvh text.  Host work only."""
+    import shutil
+    import tempfile
+
+    import numpy as np
+
+    from renderer_amd import scenes
+    from renderer_amd.scene_file import SceneFile
+
+    P, F = scenes.displaced_sphere("tetra", args.subdiv, 0.1, 1)
+    tmp = tempfile.mkdtemp(prefix="scene_file_")
+    try:
+        obj = os.path.join(tmp, "sphere.obj")
+        with open(obj, "w") as f:
+            f.write("".join(f"v {x:.9g} {y:.9g} {z:.9g}\n" for x, y, z in P.tolist()))
+            f.write("".join(f"f {a + 1} {b + 1} {c + 1}\n" for a, b, c in F.tolist()))
+        with open(os.path.join(tmp, "mesh.scene"), "w") as f:
+            f.write("camera = camera([0 0.9 3.6], [0 -0.05 0], [0 1 0], 0.6981317008, 1024, 1024);\n"
+                    "lights = lights{ arealight([-0.75 3 -0.75], [1.5 0 0], [0 0 1.5], [6 6 6]) "
+                    "pointlight([2.5 2.5 2], [8 8 8]) };\n"
+                    "intersectable = intersectables{ obj(\"sphere.obj\", phong([0.6 0.6 0.6], [0.3 0.3 0.3], 50)) "
+                    "quad([-6 -1.15 -6], [-6 -1.15 6], [6 -1.15 6], [6 -1.15 -6], diffuse([0.5 0.5 0.5])) };\n")
+        mb = os.path.getsize(obj) / 1e6
+        t0 = time.perf_counter()
+        sf = SceneFile.load(os.path.join(tmp, "mesh.scene"))
+        t1 = time.perf_counter()
+        a = sf.arrays()
+        t2 = time.perf_counter()
+        sf.dump_bvh(os.path.join(tmp, "mesh.bvh"))
+        t3 = time.perf_counter()
+        dump_mb = os.path.getsize(os.path.join(tmp, "mesh.bvh")) / 1e6
+        same = bool(np.array_equal(a.positions.reshape(-1, 3, 3)[:len(F)], P[F]))
+        sf.close()
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    return {
+        "workload": f"displaced sphere as OBJ ({len(F)} triangles, {mb:.1f} MB) + ground quad, 2 lights",
+        "triangles": int(len(F)),
+        "load_s": round(t1 - t0, 3),
+        "evaluate_s": round(t2 - t1, 3),
+        "obj_mb_per_s": round(mb / (t1 - t0), 1),
+        "mtris_per_s": round(len(F) / (t2 - t0) / 1e6, 2),
+        "dump_bvh_s": round(t3 - t2, 3),
+        "dump_bvh_mb": round(dump_mb, 1),
+        "geometry_equals_generator": same,
+        "cores": 1,
+    }
+
+
+def film_outputs(ctx, film):
+    """Film outputs: the tonemapper kernel on a large resident film (its HBM
+    roofline: 12 B read + 3 B written per pixel), then the three writers on
+    the benchmark film (host work)."""
+    import shutil
+    import tempfile
+
+    import torch
+
+    from renderer_amd.film import write_exr, write_image, write_mat
+
+    side = 8192
+    big = torch.rand((side, side, 3), dtype=torch.float32, device="cuda:0") * 1.5
+    out = torch.empty((side, side, 3), dtype=torch.uint8, device="cuda:0")
+    torch.cuda.synchronize()
+    ms = []
+    for _ in range(6):
+        ctx.tonemap_device(big.data_ptr(), side, side, out.data_ptr(), 2.2)
+        ms.append(ctx.stats()["tonemap_ms"])
+    t = sorted(ms[1:])[len(ms[1:]) // 2]
+    nbytes = side * side * 3 * 5
+    gbs = nbytes / (t * 1e-3) / 1e9
+    del big, out
+    h, w = film.shape[0], film.shape[1]
+    f = film.cpu().numpy()
+    tmp = tempfile.mkdtemp(prefix="film_")
+    try:
+        t0 = time.perf_counter()
+        rgb = ctx.tonemap(f, 2.2)
+        t1 = time.perf_counter()
+        write_image(os.path.join(tmp, "film.png"), rgb)
+        t2 = time.perf_counter()
+        write_exr(os.path.join(tmp, "film.exr"), f)
+        t3 = time.perf_counter()
+        write_mat(os.path.join(tmp, "film.m"), f)
+        t4 = time.perf_counter()
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    return {
+        "tonemap": {"film": f"{side}x{side} float RGB resident in HBM, gamma 2.2", "kernel_ms": round(t, 4),
+                    "bytes": nbytes, "achieved_gbs": round(gbs, 1), "peak_gbs": HBM_PEAK_GBS,
+                    "frac": round(gbs / HBM_PEAK_GBS, 4)},
+        "writers": {"film": f"{w}x{h}", "tonemap_host_s": round(t1 - t0, 4), "png_s": round(t2 - t1, 4),
+                    "exr_s": round(t3 - t2, 4), "mat_s": round(t4 - t3, 3)},
+    }
+
+
 def cpu_baseline(arrays, scene, budget_s):
     """The oracle (the reference's algorithms: leaf-1 BVH visiting both
     children, per-sample critical section) rendering whole frames of the same
@@ -357,6 +459,11 @@ def main():
         log("copy calibration: 1 GiB read + 1 GiB written")
         del a, b
 
+    fout = None
+    if not args.no_film:
+        log("film outputs")
+        fout = film_outputs(ctx, film)
+
     cpu = None
     if not args.no_cpu:
         log("cpu baseline")
@@ -367,6 +474,11 @@ def main():
     if not args.no_batched:
         log("batched views")
         batched = batched_views(args, scene, arrays)
+
+    sfile = None
+    if not args.no_scene_file:
+        log("scene-file front end")
+        sfile = scene_file_load(args)
 
     guided = env = None
     if not args.no_guided:
@@ -419,6 +531,8 @@ def main():
         },
         "cpu_baseline": cpu,
         "batched_views": batched,
+        "scene_file": sfile,
+        "film_outputs": fout,
         "equal_time_guided_vs_plain": guided,
         "equal_time_guided_vs_plain_env": env,
     }

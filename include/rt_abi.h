@@ -45,7 +45,10 @@ enum rt_status {
     RT_ERR_BUILD = -4,       /* BVH construction failed (e.g. depth limit) */
     RT_ERR_CANCELLED = -5,   /* rt_request_stop was called during a render */
     RT_ERR_FAULT = -6,       /* injected by rt_set_fault (test hook) */
-    RT_ERR_NO_GUIDE = -7     /* guided mode without a guiding network */
+    RT_ERR_NO_GUIDE = -7,    /* guided mode without a guiding network */
+    RT_ERR_PARSE = -8,       /* scene text or OBJ file rejected (rt_scene_file.h) */
+    RT_ERR_UNSUPPORTED = -9, /* scene construct the GPU path does not render */
+    RT_ERR_IO = -10          /* file could not be read or written */
 };
 
 /* Counters of the last call (and totals since rt_create). */
@@ -71,6 +74,7 @@ typedef struct rt_stats {
     int64_t guide_infer_samples;  /* path vertices the network was evaluated for */
     int64_t guide_train_samples;  /* records summed over all training steps */
     int64_t guide_train_steps;    /* Adam steps */
+    double tonemap_ms;            /* duration of the last rt_film_tonemap kernel (GPU events) */
 } rt_stats;
 
 /* Library-wide. */

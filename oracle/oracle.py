@@ -51,6 +51,7 @@ def load():
     lib.oc_env_sample.argtypes = [P, ctypes.c_float, ctypes.c_float, P, P]
     lib.oc_env_pdf.restype = ctypes.c_float
     lib.oc_env_pdf.argtypes = [P, P]
+    lib.oc_tonemap.argtypes = [P, I64, ctypes.c_float, P]
     _lib = lib
     return lib
 
@@ -61,6 +62,15 @@ def philox(ctr, key):
     k = np.ascontiguousarray(key, np.uint32)
     out = np.zeros(4, np.uint32)
     lib.oc_philox4x32_10(c.ctypes.data, k.ctypes.data, out.ctypes.data)
+    return out
+
+
+def tonemap(film, gamma=2.2):
+    """Tonemapper::map of every channel value: uint8 array of film's shape."""
+    lib = load()
+    f = np.ascontiguousarray(film, np.float32)
+    out = np.zeros(f.shape, np.uint8)
+    lib.oc_tonemap(f.ctypes.data, f.size, float(gamma), out.ctypes.data)
     return out
 
 

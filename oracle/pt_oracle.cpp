@@ -1246,4 +1246,22 @@ float oc_env_pdf(void* s, const float dir[3])
     return sc.env->pdf(V3(dir[0], dir[1], dir[2]));
 }
 
+// Tonemapper::clamp (src/tonemapper.cpp:31-36): the float argument is
+// converted to the int result by truncation.  x86 converts NaN and values
+// outside int to INT_MIN; the explicit test keeps that defined here.
+static int tonemap_clamp(float x)
+{
+    if (x <= 0) return 0;
+    if (x >= 255) return 255;
+    if (x != x) return INT32_MIN;
+    return (int)x;
+}
+
+// Tonemapper::map (src/tonemapper.cpp:50-56): std::pow(float, double) is the
+// double pow; qRgb keeps the low 8 bits of each channel.
+void oc_tonemap(const float* film, int64_t n, float gamma, uint8_t* out)
+{
+    for (int64_t i = 0; i < n; ++i) out[i] = (uint8_t)(tonemap_clamp(255 * std::pow(film[i], 1. / gamma)) & 0xff);
+}
+
 }  // extern "C"

@@ -74,6 +74,10 @@ void oc_render_sample(void* scene, const rt_view* view, const rt_params* params,
 void oc_env_sample(void* scene, float u0, float u1, float dir[3], float* pdf);
 float oc_env_pdf(void* scene, const float dir[3]);
 
+/* Tonemapper::map per channel value (src/tonemapper.cpp:31-56): n floats to
+ * n bytes. */
+void oc_tonemap(const float* film, int64_t n, float gamma, uint8_t* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -78,6 +78,7 @@ class RtStats(ctypes.Structure):
         ("guide_infer_samples", ctypes.c_int64),
         ("guide_train_samples", ctypes.c_int64),
         ("guide_train_steps", ctypes.c_int64),
+        ("tonemap_ms", ctypes.c_double),
     ]
 
 
@@ -92,7 +93,9 @@ LIGHT_AREA, LIGHT_POINT, LIGHT_ENV = 0, 1, 2
 MODE_COMPAT, MODE_UNBIASED, MODE_GUIDED = 0, 1, 2
 
 STATUS = {0: "ok", -1: "invalid argument", -2: "no scene uploaded", -3: "HIP runtime error",
-          -4: "BVH build failed", -5: "cancelled", -6: "injected fault", -7: "no guiding network"}
+          -4: "BVH build failed", -5: "cancelled", -6: "injected fault", -7: "no guiding network",
+          -8: "scene parse error", -9: "unsupported scene construct", -10: "file error"}
+ERR_INVALID, ERR_PARSE, ERR_UNSUPPORTED, ERR_IO = -1, -8, -9, -10
 
 _P = ctypes.c_void_p
 _I64 = ctypes.c_int64
@@ -126,6 +129,23 @@ PROTOTYPES = {
     "rt_guide_forward": (ctypes.c_int, [_P, _P, _I32, _P]),
     "rt_guide_gradient": (ctypes.c_int, [_P, _P, _I32, _P, _P]),
     "rt_guide_train": (ctypes.c_int, [_P, _P, _I32, _I32, ctypes.c_float]),
+    # include/rt_scene_file.h
+    "rt_scene_file_load": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_void_p)]),
+    "rt_scene_file_parse": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(ctypes.c_void_p)]),
+    "rt_scene_file_last_error": (ctypes.c_char_p, []),
+    "rt_scene_file_messages": (ctypes.c_char_p, [_P]),
+    "rt_scene_file_scene": (ctypes.c_int, [_P, ctypes.POINTER(RtSceneDesc), ctypes.POINTER(RtCamera)]),
+    "rt_scene_file_dump_bvh": (ctypes.c_int, [_P, ctypes.c_char_p]),
+    "rt_scene_file_dump_bvh_text": (_I64, [_P, _P, _I64]),
+    "rt_scene_file_free": (None, [_P]),
+    # include/rt_film.h
+    "rt_film_tonemap": (ctypes.c_int, [_P, _P, _I32, _I32, ctypes.c_float, _P]),
+    "rt_film_tonemap_device": (ctypes.c_int, [_P, _P, _I32, _I32, ctypes.c_float, _P]),
+    "rt_film_write_exr": (ctypes.c_int, [ctypes.c_char_p, _P, _I32, _I32]),
+    "rt_film_write_image": (ctypes.c_int, [ctypes.c_char_p, _P, _I32, _I32]),
+    "rt_film_write_mat": (ctypes.c_int, [ctypes.c_char_p, _P, _I32, _I32]),
+    "rt_unique_filename": (_I64, [ctypes.c_char_p, _P, _I64]),
+    "rt_film_last_error": (ctypes.c_char_p, []),
 }
 
 _lib = None

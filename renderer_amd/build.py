@@ -11,7 +11,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "librt_hip.so")
 
-SOURCES = ["rt_kernels.hip", "rt_guide.hip", "rt_abi.cpp", "bvh_build.cpp"]
+SOURCES = ["rt_kernels.hip", "rt_guide.hip", "rt_film.hip", "rt_abi.cpp", "bvh_build.cpp", "scene_file.cpp",
+           "film_io.cpp"]
 
 # -packed-fp32-ops: keep the compiler from forming packed FP32 instructions.
 HIPCC_FLAGS = [
@@ -29,7 +30,7 @@ def hipcc():
 def build_library(verbose=False):
     srcs = [os.path.join(CSRC, s) for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
     tmp = LIB + ".tmp"
-    cmd = [hipcc()] + HIPCC_FLAGS + ["-I", os.path.join(HERE, "..", "include"), "-o", tmp] + srcs + ["-lpthread"]
+    cmd = [hipcc()] + HIPCC_FLAGS + ["-I", os.path.join(HERE, "..", "include"), "-o", tmp] + srcs + ["-lpthread", "-lz"]
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)

@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "../../include/rt_abi.h"
+#include "../../include/rt_film.h"
 #include "bvh_build.h"
 #include "rt_guide.h"
 #include "rt_kernels.h"
@@ -166,6 +167,9 @@ const char* rt_error_string(int s)
     case RT_ERR_CANCELLED: return "cancelled";
     case RT_ERR_FAULT: return "injected fault";
     case RT_ERR_NO_GUIDE: return "no guiding network";
+    case RT_ERR_PARSE: return "scene parse error";
+    case RT_ERR_UNSUPPORTED: return "unsupported scene construct";
+    case RT_ERR_IO: return "file error";
     default: return "unknown error";
     }
 }
@@ -606,6 +610,47 @@ int rt_trace_any(rt_ctx* c, const float* o, const float* d, int64_t n, uint8_t* 
     if (!c || n < 0 || (n && (!o || !d || !occ))) return RT_ERR_INVALID;
     if (int r = check_fault(c)) return r;
     return trace_host(c, 1, o, d, n, occ, nullptr);
+}
+
+/* ---------------------------------------------------------------------------
+ * Tonemapper (include/rt_film.h)
+ * ------------------------------------------------------------------------- */
+static int tonemap_impl(rt_ctx* c, const float* film, int64_t n, float gamma, uint8_t* out)
+{
+    HIP_TRY(c, hipEventRecord(c->ev_a, c->stream));
+    hipError_t e = launch_tonemap(film, out, n, gamma, c->n_cu * 8, c->stream);
+    if (e != hipSuccess) return set_err(c, RT_ERR_HIP, std::string("tonemap launch: ") + hipGetErrorString(e));
+    HIP_TRY(c, hipEventRecord(c->ev_b, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    float ms = 0;
+    hipEventElapsedTime(&ms, c->ev_a, c->ev_b);
+    c->stats.tonemap_ms = ms;
+    return RT_OK;
+}
+
+int rt_film_tonemap_device(rt_ctx* c, const float* film, int32_t w, int32_t h, float gamma, uint8_t* rgb8)
+{
+    if (!c || w < 0 || h < 0 || ((int64_t)w * h && (!film || !rgb8))) return RT_ERR_INVALID;
+    if (int r = check_fault(c)) return r;
+    const int64_t n = (int64_t)w * h * 3;
+    if (n == 0) return RT_OK;
+    hipSetDevice(c->device);
+    return tonemap_impl(c, film, n, gamma, rgb8);
+}
+
+int rt_film_tonemap(rt_ctx* c, const float* film, int32_t w, int32_t h, float gamma, uint8_t* rgb8)
+{
+    if (!c || w < 0 || h < 0 || ((int64_t)w * h && (!film || !rgb8))) return RT_ERR_INVALID;
+    if (int r = check_fault(c)) return r;
+    const int64_t n = (int64_t)w * h * 3;
+    if (n == 0) return RT_OK;
+    hipSetDevice(c->device);
+    int r;
+    if ((r = ensure(c, c->rb_o, (size_t)n * 4)) || (r = ensure(c, c->rb_out0, (size_t)n))) return r;
+    HIP_TRY(c, hipMemcpyAsync(c->rb_o.p, film, (size_t)n * 4, hipMemcpyHostToDevice, c->stream));
+    if ((r = tonemap_impl(c, (const float*)c->rb_o.p, n, gamma, (uint8_t*)c->rb_out0.p))) return r;
+    HIP_TRY(c, hipMemcpy(rgb8, c->rb_out0.p, (size_t)n, hipMemcpyDeviceToHost));
+    return RT_OK;
 }
 
 /* ---------------------------------------------------------------------------

@@ -138,6 +138,23 @@ class GpuContext:
         _check(self.lib, self.h, self.lib.rt_get_launch_config(self.h, ctypes.byref(c)))
         return {f: getattr(c, f) for f, _ in c._fields_}
 
+    # -- film outputs (include/rt_film.h) ------------------------------------------
+    def tonemap(self, film, gamma=2.2):
+        """Tonemapper::tonemap of a (height, width, 3) float film on the GPU:
+        uint8 RGB of the same shape."""
+        f = np.ascontiguousarray(film, np.float32)
+        if f.ndim != 3 or f.shape[2] != 3:
+            raise ValueError("film must be (height, width, 3)")
+        out = np.empty(f.shape, np.uint8)
+        _check(self.lib, self.h, self.lib.rt_film_tonemap(self.h, f.ctypes.data, f.shape[1], f.shape[0],
+                                                          float(gamma), out.ctypes.data))
+        return out
+
+    def tonemap_device(self, film_ptr, width, height, out_ptr, gamma=2.2):
+        """Device pointers: height*width*3 floats in, as many bytes out."""
+        _check(self.lib, self.h, self.lib.rt_film_tonemap_device(self.h, film_ptr, int(width), int(height),
+                                                                 float(gamma), out_ptr))
+
     def stats(self):
         s = _abi.RtStats()
         _check(self.lib, self.h, self.lib.rt_get_stats(self.h, ctypes.byref(s)))
