@@ -165,6 +165,14 @@ def load():
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(f"{LIB_PATH} is not built; run `python -m renderer_amd.build` "
                            "(the GPU path has no fallback)")
+    try:
+        # torch ships its own HIP runtime under an unversioned file name; loaded
+        # after this library's (/opt/rocm) it would be a second runtime that sees
+        # no GPU.  Loading torch first makes this library bind to torch's copy,
+        # so both share one runtime.
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = ctypes.CDLL(LIB_PATH)
     for name, (res, args) in PROTOTYPES.items():
         fn = getattr(lib, name)

@@ -47,6 +47,7 @@ struct rt_ctx {
 
     // ray batches (host entry points)
     DevBuf rb_o, rb_d, rb_out0, rb_out1;
+    DevBuf stack_ovf;  /* traversal stack overflow of the largest launch so far */
 
     // render
     PathState ps{};
@@ -152,6 +153,16 @@ float lum_rgb(const float c[3]) { return (float)(.299 * c[0] + .587 * c[1] + .11
 
 int grid_for(rt_ctx* c) { return c->n_cu * std::max(1, c->cfg.blocks_per_cu); }
 
+/* The overflow part of the traversal stacks for a launch of `grid` blocks:
+ * RT_STACK_OVF int32 entries per lane.  Grows, never shrinks. */
+int ensure_stack_overflow(rt_ctx* c, int grid)
+{
+    size_t bytes = (size_t)trav_lanes(grid) * RT_STACK_OVF * sizeof(int32_t);
+    if (int r = ensure(c, c->stack_ovf, std::max<size_t>(bytes, 4))) return r;
+    c->scene.stack_ovf = (int32_t*)c->stack_ovf.p;
+    return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -204,7 +215,7 @@ int rt_destroy(rt_ctx* c)
     hipStreamSynchronize(c->stream);
     DevBuf* all[] = {&c->nodes, &c->prims, &c->prim_id, &c->mat, &c->vnormals, &c->geo, &c->mats, &c->lights,
                      &c->env_rgb, &c->env_row, &c->env_col, &c->env_p, &c->rb_o, &c->rb_d, &c->rb_out0,
-                     &c->rb_out1, &c->counters, &c->views, &c->tiles, &c->passes, &c->films, &c->g_theta,
+                     &c->rb_out1, &c->stack_ovf, &c->counters, &c->views, &c->tiles, &c->passes, &c->films, &c->g_theta,
                      &c->g_w16, &c->g_m, &c->g_v, &c->g_grad, &c->g_acc, &c->g_count, &c->g_rec, &c->g_aux,
                      &c->g_open, &c->g_out, &c->g_tmp_rec, &c->g_tmp_y};
     for (DevBuf* b : all) release(*b);
@@ -537,6 +548,7 @@ static int trace_device(rt_ctx* c, int kind, const float* o, const float* d, int
     if (n == 0) return RT_OK;
     hipSetDevice(c->device);
     int grid = (int)std::min<int64_t>((n + 255) / 256, (int64_t)grid_for(c) * 4);
+    if (int r = ensure_stack_overflow(c, grid)) return r;
     hipError_t e;
     HIP_TRY(c, hipEventRecord(c->ev_a, c->stream));
     if (kind == 0)
@@ -1025,8 +1037,7 @@ static int render_impl(rt_ctx* c, const rt_view* views, int32_t nv, const rt_par
                 (float)std::min(1.0, (double)GD_TRAIN_CHUNKS * batch / std::max(1.0, (double)P * 6.0))};
     const int grid = grid_for(c);
     const int depth = prm->max_depth;
-    /* traversal stack capacity: 24 entries when the tree needs no more */
-    const int stack = c->bvh_stack + 1 <= 24 ? 24 : RT_STACK_DEPTH;
+    if (int r = ensure_stack_overflow(c, grid)) return r;
     /* Every pass enqueues the same kernels with the same arguments (the pass
      * index lives in the device counters), so one captured pass can be
      * replayed as a hipGraph - except with training, whose Adam step number
@@ -1064,7 +1075,7 @@ static int render_impl(rt_ctx* c, const rt_view* views, int32_t nv, const rt_par
             int q = dd & 1;
             if (timing) bounce_ev.push_back(ev);
             QCHK(mark());
-            QCHK(launch_extend(c->scene, c->ps, q, c->cfg.count_traversal, stack, grid, c->stream));
+            QCHK(launch_extend(c->scene, c->ps, q, c->cfg.count_traversal, grid, c->stream));
             QCHK(mark());
             if (guided) {
                 /* the last bounce neither samples the guide nor weights against it */
@@ -1078,7 +1089,7 @@ static int render_impl(rt_ctx* c, const rt_view* views, int32_t nv, const rt_par
                 QCHK(launch_shade(c->scene, c->ps, pp, dd, q, grid, c->stream));
             }
             QCHK(mark());
-            QCHK(launch_shadow(c->scene, c->ps, pp, stack, grid, c->stream));
+            QCHK(launch_shadow(c->scene, c->ps, pp, grid, c->stream));
             QCHK(mark());
             QCHK(launch_advance(c->ps, q, dd, c->stream));
         }

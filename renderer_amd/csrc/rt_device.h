@@ -83,7 +83,7 @@ __device__ __forceinline__ U4 rng4(uint32_t seed, uint32_t view, uint32_t pixel,
  * Scene as seen by the kernels (all pointers are device pointers).
  * ------------------------------------------------------------------------- */
 struct DevScene {
-    const float4* nodes;    /* 4 float4 per interior node */
+    const float4* nodes;    /* 8 float4 (128 B) per 4-wide interior node */
     const float4* prims;    /* 3 float4 per primitive record */
     const int32_t* prim_id; /* shading index g -> reference primitive ID */
     const int32_t* mat;     /* g -> material index, or -(light+1) for an area light */
@@ -100,6 +100,7 @@ struct DevScene {
     const float* env_row_cdf;  /* env_h + 1 */
     const float* env_col_cdf;  /* env_h * (env_w + 1) */
     const float* env_texel_p;  /* env_w * env_h */
+    int32_t* stack_ovf;        /* traversal stack overflow: RT_STACK_OVF entries per lane of the launch */
 };
 
 /* ---------------------------------------------------------------------------
@@ -233,36 +234,47 @@ __device__ __forceinline__ void box2(const Trav& T, float lo, float hi, float in
     f = pos ? b : a;
 }
 
-/* Traversal stack of one lane holding LDSN + RT_STACK_REG entries: the
- * first LDSN in LDS (one column per lane, `stride` words apart), the last
- * RT_STACK_REG in registers.  The LDS part bounds how many waves fit on a
- * CU, so it is kept short; the register part gives the rare deep rays room
- * at the cost of a few VGPRs. */
-#define RT_STACK_REG 4
-template <int LDSN>
+/* Traversal stack of one lane, RT_STACK_DEPTH entries: the first
+ * RT_STACK_LDS in LDS (one column per lane, `stride` words apart, so a wave's
+ * accesses hit 64 different banks), the next RT_STACK_REG in registers, the
+ * rest in a global overflow array laid out [entry][lane] over the launch
+ * grid.  The LDS part bounds how many waves fit on a CU (16 entries: 16 KB
+ * per 256-lane block), so it is kept short; only rays that defer more than
+ * RT_STACK_LDS + RT_STACK_REG subtrees at once reach the overflow. */
 struct LaneStack {
     int32_t* lds;
     int stride;
-    int32_t r0 = 0, r1 = 0, r2 = 0, r3 = 0;  /* entries LDSN .. LDSN + 3 (scalars, so they stay in VGPRs) */
-    static constexpr int capacity = LDSN + RT_STACK_REG;
-    __device__ __forceinline__ LaneStack(int32_t* column, int s) : lds(column), stride(s) {}
+    int32_t* ovf;    /* this lane's first overflow entry */
+    size_t ostride;  /* lanes of the launch grid */
+    int32_t r0 = 0, r1 = 0, r2 = 0, r3 = 0;  /* register entries (scalars, so they stay in VGPRs) */
+    static constexpr int capacity = RT_STACK_DEPTH;
+    __device__ __forceinline__ LaneStack(int32_t* column, int s, int32_t* overflow)
+        : lds(column), stride(s), ovf(overflow + (size_t)blockIdx.x * blockDim.x + threadIdx.x),
+          ostride((size_t)gridDim.x * blockDim.x)
+    {
+    }
     __device__ __forceinline__ void put(int sp, int32_t v)
     {
-        if (sp < LDSN) {
+        if (sp < RT_STACK_LDS) {
             lds[sp * stride] = v;
             return;
         }
-        int k = sp - LDSN;
-        r0 = k == 0 ? v : r0;
-        r1 = k == 1 ? v : r1;
-        r2 = k == 2 ? v : r2;
-        r3 = k == 3 ? v : r3;
+        int k = sp - RT_STACK_LDS;
+        if (RT_STACK_REG > 0 && k < RT_STACK_REG) {
+            r0 = k == 0 ? v : r0;
+            r1 = k == 1 ? v : r1;
+            r2 = k == 2 ? v : r2;
+            r3 = k == 3 ? v : r3;
+            return;
+        }
+        ovf[(size_t)(k - RT_STACK_REG) * ostride] = v;
     }
     __device__ __forceinline__ int32_t get(int sp) const
     {
-        if (sp < LDSN) return lds[sp * stride];
-        int k = sp - LDSN;
-        return k == 0 ? r0 : k == 1 ? r1 : k == 2 ? r2 : r3;
+        if (sp < RT_STACK_LDS) return lds[sp * stride];
+        int k = sp - RT_STACK_LDS;
+        if (RT_STACK_REG > 0 && k < RT_STACK_REG) return k == 0 ? r0 : k == 1 ? r1 : k == 2 ? r2 : r3;
+        return ovf[(size_t)(k - RT_STACK_REG) * ostride];
     }
 };
 

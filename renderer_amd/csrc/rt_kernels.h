@@ -29,6 +29,7 @@ struct HostDevScene {
     const float* env_row_cdf;
     const float* env_col_cdf;
     const float* env_texel_p;
+    int32_t* stack_ovf;
 };
 
 /* One view of a render: camera basis computed on the host with the
@@ -110,12 +111,13 @@ hipError_t launch_trace_count(const HostDevScene& s, const float4* o, const floa
 /* Wavefront pipeline kernels. */
 hipError_t launch_generate(const PathState& ps, const DevView* views, const DevTile* tiles, const DevPass* passes,
                            PtParams p, int grid, hipStream_t st);
-hipError_t launch_extend(const HostDevScene& s, const PathState& ps, int qsel, int count, int stack, int grid,
-                         hipStream_t st);
+hipError_t launch_extend(const HostDevScene& s, const PathState& ps, int qsel, int count, int grid, hipStream_t st);
 hipError_t launch_shade(const HostDevScene& s, const PathState& ps, PtParams p, int depth, int qsel, int grid,
                         hipStream_t st);
-hipError_t launch_shadow(const HostDevScene& s, const PathState& ps, PtParams p, int stack, int grid,
-                         hipStream_t st);
+hipError_t launch_shadow(const HostDevScene& s, const PathState& ps, PtParams p, int grid, hipStream_t st);
+/* lanes of a traversal launch of `grid` blocks (the stack overflow array
+ * holds RT_STACK_OVF entries for each) */
+int64_t trav_lanes(int grid);
 hipError_t launch_advance(const PathState& ps, int qsel, int depth, hipStream_t st);
 hipError_t launch_accumulate(const PathState& ps, const DevView* views, const DevTile* tiles,
                              const DevPass* passes, int grid, hipStream_t st);

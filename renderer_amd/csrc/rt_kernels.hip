@@ -30,6 +30,22 @@
 
 static_assert(sizeof(HostDevScene) == sizeof(DevScene), "scene view layouts differ");
 
+/* Waves per SIMD the traversal kernels are compiled for, i.e. a register
+ * budget of 512 / waves VGPRs.  Their latency-bound gathers gain from
+ * occupancy: 6 waves (80 VGPRs, no spills in k_shadow, 36 B in k_extend)
+ * measured 4% faster than the unconstrained 89 VGPRs (5 waves); 7 and 8
+ * waves spill more than they gain. */
+#ifndef RT_TRAV_WAVES
+#define RT_TRAV_WAVES 6
+#endif
+#if RT_TRAV_WAVES > 0
+#define RT_TRAV_OCCUPANCY __attribute__((amdgpu_waves_per_eu(RT_TRAV_WAVES, 8)))
+#else
+#define RT_TRAV_OCCUPANCY
+#endif
+
+int64_t trav_lanes(int grid) { return (int64_t)grid * PT_BLOCK; }
+
 static inline DevScene as_dev(const HostDevScene& s)
 {
     DevScene d;
@@ -114,9 +130,8 @@ __global__ __launch_bounds__(PT_BLOCK) void k_trace(DevScene S, const float4* __
                                                    const float4* __restrict__ d, int64_t n, int32_t* prim, float* tout,
                                                    uint8_t* occ, int32_t* nn, int32_t* np)
 {
-    constexpr int LDSN = RT_STACK_DEPTH - RT_STACK_REG;
-    __shared__ int32_t stack[LDSN * PT_BLOCK];
-    LaneStack<LDSN> my(stack + threadIdx.x, PT_BLOCK);
+    __shared__ int32_t stack[RT_STACK_LDS * PT_BLOCK];
+    LaneStack my(stack + threadIdx.x, PT_BLOCK, S.stack_ovf);
     for (int64_t i = (int64_t)blockIdx.x * PT_BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * PT_BLOCK) {
         float4 a = o[i], b = d[i];
         int cn = 0, cp = 0;
@@ -521,15 +536,13 @@ hipError_t launch_generate(const PathState& ps, const DevView* views, const DevT
  * finished stores its result and takes the next unclaimed ray at once
  * (WaveFeed), instead of idling until the slowest ray of its wave finishes;
  * waves claim rays dynamically, so the launch ends evenly.
- * STACK is the stack capacity (>= the BVH depth + 1), of which all but
- * RT_STACK_REG entries live in LDS; the smaller it is, the more waves fit in
- * a CU's LDS (capacity 24: 20 KB per block, 8 blocks per CU). */
-template <bool COUNT, int STACK>
-__global__ __launch_bounds__(PT_BLOCK) void k_extend(DevScene S, PathState ps, int qsel)
+ * The traversal stack (LaneStack) keeps RT_STACK_LDS entries per lane in
+ * LDS: 16 KB per block, so LDS leaves room for 10 blocks per CU. */
+template <bool COUNT>
+__global__ __launch_bounds__(PT_BLOCK) RT_TRAV_OCCUPANCY void k_extend(DevScene S, PathState ps, int qsel)
 {
-    constexpr int LDSN = STACK - RT_STACK_REG;
-    __shared__ int32_t stack[LDSN * PT_BLOCK];
-    LaneStack<LDSN> my(stack + threadIdx.x, PT_BLOCK);
+    __shared__ int32_t stack[RT_STACK_LDS * PT_BLOCK];
+    LaneStack my(stack + threadIdx.x, PT_BLOCK, S.stack_ovf);
     const int n = ps.cnt->q_count[qsel];
     const int32_t* q = ps.queue[qsel];
     int cn = 0, cp = 0, p = 0;
@@ -563,21 +576,12 @@ __global__ __launch_bounds__(PT_BLOCK) void k_extend(DevScene S, PathState ps, i
     }
 }
 
-hipError_t launch_extend(const HostDevScene& s, const PathState& ps, int qsel, int count, int stack, int grid,
-                         hipStream_t st)
+hipError_t launch_extend(const HostDevScene& s, const PathState& ps, int qsel, int count, int grid, hipStream_t st)
 {
-    if (stack <= 24) {
-        if (count)
-            hipLaunchKernelGGL((k_extend<true, 24>), dim3(grid), dim3(PT_BLOCK), 0, st, as_dev(s), ps, qsel);
-        else
-            hipLaunchKernelGGL((k_extend<false, 24>), dim3(grid), dim3(PT_BLOCK), 0, st, as_dev(s), ps, qsel);
-    } else {
-        if (count)
-            hipLaunchKernelGGL((k_extend<true, RT_STACK_DEPTH>), dim3(grid), dim3(PT_BLOCK), 0, st, as_dev(s), ps, qsel);
-        else
-            hipLaunchKernelGGL((k_extend<false, RT_STACK_DEPTH>), dim3(grid), dim3(PT_BLOCK), 0, st, as_dev(s), ps,
-                               qsel);
-    }
+    if (count)
+        hipLaunchKernelGGL((k_extend<true>), dim3(grid), dim3(PT_BLOCK), 0, st, as_dev(s), ps, qsel);
+    else
+        hipLaunchKernelGGL((k_extend<false>), dim3(grid), dim3(PT_BLOCK), 0, st, as_dev(s), ps, qsel);
     return hipGetLastError();
 }
 
@@ -1063,12 +1067,10 @@ hipError_t launch_shade(const HostDevScene& s, const PathState& ps, PtParams p, 
 /* ===========================================================================
  * shadow: any-hit test of next-event rays
  * ========================================================================= */
-template <int STACK>
-__global__ __launch_bounds__(PT_BLOCK) void k_shadow(DevScene S, PathState ps, int compat)
+__global__ __launch_bounds__(PT_BLOCK) RT_TRAV_OCCUPANCY void k_shadow(DevScene S, PathState ps, int compat)
 {
-    constexpr int LDSN = STACK - RT_STACK_REG;
-    __shared__ int32_t stack[LDSN * PT_BLOCK];
-    LaneStack<LDSN> my(stack + threadIdx.x, PT_BLOCK);
+    __shared__ int32_t stack[RT_STACK_LDS * PT_BLOCK];
+    LaneStack my(stack + threadIdx.x, PT_BLOCK, S.stack_ovf);
     const int n = ps.cnt->shadow_count;
     int cn = 0, cp = 0, item = 0;
     bool live = false, more = true;
@@ -1107,14 +1109,10 @@ __global__ __launch_bounds__(PT_BLOCK) void k_shadow(DevScene S, PathState ps, i
     }
 }
 
-hipError_t launch_shadow(const HostDevScene& s, const PathState& ps, PtParams p, int stack, int grid,
-                         hipStream_t st)
+hipError_t launch_shadow(const HostDevScene& s, const PathState& ps, PtParams p, int grid, hipStream_t st)
 {
     int compat = p.mode == RT_MODE_COMPAT ? 1 : 0;
-    if (stack <= 24)
-        hipLaunchKernelGGL(k_shadow<24>, dim3(grid), dim3(PT_BLOCK), 0, st, as_dev(s), ps, compat);
-    else
-        hipLaunchKernelGGL(k_shadow<RT_STACK_DEPTH>, dim3(grid), dim3(PT_BLOCK), 0, st, as_dev(s), ps, compat);
+    hipLaunchKernelGGL(k_shadow, dim3(grid), dim3(PT_BLOCK), 0, st, as_dev(s), ps, compat);
     return hipGetLastError();
 }
 

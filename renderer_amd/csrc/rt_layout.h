@@ -36,6 +36,14 @@
 
 #define RT_LEAF_MAX 4
 #define RT_STACK_DEPTH 32
+/* traversal stack entries per lane: in LDS, in registers, in global overflow */
+#ifndef RT_STACK_LDS
+#define RT_STACK_LDS 16
+#endif
+#ifndef RT_STACK_REG
+#define RT_STACK_REG 4
+#endif
+#define RT_STACK_OVF (RT_STACK_DEPTH - RT_STACK_LDS - RT_STACK_REG)
 #define RT_NODE_FLOATS 32  /* 128 B per 4-wide node */
 
 /* Material record (64 B).  type: rt_material_type. */
