@@ -205,6 +205,9 @@ struct Trav {
     vec3 o, d, inv;
     float tmin, best, u, v;
     int32_t g, cur, sp;
+    /* per axis, the byte offset (0 or 16) of the near planes within a node's
+     * lo/hi pair: the hi planes when the direction component is negative */
+    uint32_t nx, ny, nz;
 };
 
 __device__ __forceinline__ float safe_inv(float x)
@@ -224,6 +227,17 @@ __device__ __forceinline__ void trav_init(const DevScene& S, Trav& T, vec3 o, ve
     T.g = -1;
     T.sp = 0;
     T.cur = S.n_nodes > 0 && tmin <= tmax ? 0 : -1;
+    T.nx = T.inv.x >= 0.f ? 0u : 16u;
+    T.ny = T.inv.y >= 0.f ? 0u : 16u;
+    T.nz = T.inv.z >= 0.f ? 0u : 16u;
+}
+
+/* Monotone map of a float to uint32 (negative values reversed below the
+ * positive ones), so keys sort with integer min / max. */
+__device__ __forceinline__ uint32_t sortable(float f)
+{
+    uint32_t b = __float_as_uint(f);
+    return b ^ ((uint32_t)((int32_t)b >> 31) | 0x80000000u);
 }
 
 __device__ __forceinline__ void box2(const Trav& T, float lo, float hi, float inv, float o, float& n, float& f)
@@ -288,37 +302,70 @@ struct LaneStack {
 template <bool ANY, bool COUNT, class Stack>
 __device__ __forceinline__ bool trav_step(const DevScene& S, Trav& T, Stack& stack, int& cn, int& cp)
 {
-    const float4* nd = S.nodes + (RT_NODE_FLOATS / 4) * (size_t)T.cur;
-    const float4 lx = nd[0], hx = nd[1], ly = nd[2], hy = nd[3], lz = nd[4], hz = nd[5], cw = nd[6];
+    /* The near and far planes of each axis are fetched directly (the pair
+     * lo/hi sits 16 B apart, so near = base + T.n?, far = that ^ 16): no
+     * per-child selects.  Offsets are 32-bit from the uniform node base, which
+     * bounds the tree at 2^25 nodes (the upload checks it). */
+    const char* nb = reinterpret_cast<const char*>(S.nodes);
+    const uint32_t c = (uint32_t)T.cur << 7;
+    const uint32_t ax = c + T.nx, ay = c + 32u + T.ny, az = c + 64u + T.nz;
+    const float4 nx4 = *reinterpret_cast<const float4*>(nb + ax);
+    const float4 fx4 = *reinterpret_cast<const float4*>(nb + (ax ^ 16u));
+    const float4 ny4 = *reinterpret_cast<const float4*>(nb + ay);
+    const float4 fy4 = *reinterpret_cast<const float4*>(nb + (ay ^ 16u));
+    const float4 nz4 = *reinterpret_cast<const float4*>(nb + az);
+    const float4 fz4 = *reinterpret_cast<const float4*>(nb + (az ^ 16u));
+    const float4 cw = *reinterpret_cast<const float4*>(nb + c + 96u);
     if (COUNT) ++cn;
     const float slack = 9.5367431640625e-07f; /* 2^-20 */
     float bound = T.best;
     bound = bound + fabsf(bound) * slack;
-    const float lo_x[4] = {lx.x, lx.y, lx.z, lx.w}, hi_x[4] = {hx.x, hx.y, hx.z, hx.w};
-    const float lo_y[4] = {ly.x, ly.y, ly.z, ly.w}, hi_y[4] = {hy.x, hy.y, hy.z, hy.w};
-    const float lo_z[4] = {lz.x, lz.y, lz.z, lz.w}, hi_z[4] = {hz.x, hz.y, hz.z, hz.w};
+    const float n_x[4] = {nx4.x, nx4.y, nx4.z, nx4.w}, f_x[4] = {fx4.x, fx4.y, fx4.z, fx4.w};
+    const float n_y[4] = {ny4.x, ny4.y, ny4.z, ny4.w}, f_y[4] = {fy4.x, fy4.y, fy4.z, fy4.w};
+    const float n_z[4] = {nz4.x, nz4.y, nz4.z, nz4.w}, f_z[4] = {fz4.x, fz4.y, fz4.z, fz4.w};
     float key[4];
     int32_t word[4] = {__float_as_int(cw.x), __float_as_int(cw.y), __float_as_int(cw.z), __float_as_int(cw.w)};
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        float nx, ny, nz, fx, fy, fz;
-        box2(T, lo_x[k], hi_x[k], T.inv.x, T.o.x, nx, fx);
-        box2(T, lo_y[k], hi_y[k], T.inv.y, T.o.y, ny, fy);
-        box2(T, lo_z[k], hi_z[k], T.inv.z, T.o.z, nz, fz);
-        float tl = fmaxf(fmaxf(nx, ny), fmaxf(nz, T.tmin));
-        float th = fminf(fminf(fx, fy), fminf(fz, bound));
+        float tl = fmaxf(fmaxf((n_x[k] - T.o.x) * T.inv.x, (n_y[k] - T.o.y) * T.inv.y),
+                         fmaxf((n_z[k] - T.o.z) * T.inv.z, T.tmin));
+        float th = fminf(fminf((f_x[k] - T.o.x) * T.inv.x, (f_y[k] - T.o.y) * T.inv.y),
+                         fminf((f_z[k] - T.o.z) * T.inv.z, bound));
         key[k] = tl <= th + fabsf(th) * slack ? tl : RT_INF;
     }
+    uint32_t uk[4];
     if (!ANY) {
-        /* ascending box entry; misses (key = inf) go last */
-#define RT_CSWAP(i, j)                                  \
-    if (key[j] < key[i]) {                              \
-        float tk = key[i]; key[i] = key[j]; key[j] = tk; \
-        int32_t tw = word[i]; word[i] = word[j]; word[j] = tw; \
+        /* ascending box entry, misses (key = inf) last: the keys as sortable
+         * integers with the slot in the two low bits, through a 5-comparator
+         * network of integer min / max (no branches); then the words follow
+         * their slots.  Clearing the low bits lowers a key by < 4 ulp, so
+         * comparisons against a bound stay conservative. */
+        uint32_t u[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) u[k] = (sortable(key[k]) & ~3u) | (uint32_t)k;
+#define RT_MINMAX(i, j)                 \
+    {                                   \
+        const uint32_t lo_ = min(u[i], u[j]); \
+        u[j] = max(u[i], u[j]);         \
+        u[i] = lo_;                     \
     }
-        RT_CSWAP(0, 1) RT_CSWAP(2, 3) RT_CSWAP(0, 2) RT_CSWAP(1, 3) RT_CSWAP(1, 2)
-#undef RT_CSWAP
+        RT_MINMAX(0, 1) RT_MINMAX(2, 3) RT_MINMAX(0, 2) RT_MINMAX(1, 3) RT_MINMAX(1, 2)
+#undef RT_MINMAX
+        const int32_t w0 = word[0], w1 = word[1], w2 = word[2], w3 = word[3];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const bool b0 = (u[j] & 1u) != 0, b1 = (u[j] & 2u) != 0;
+            const int32_t wl = b0 ? w1 : w0, wh = b0 ? w3 : w2;
+            word[j] = b1 ? wh : wl;
+            uk[j] = u[j] & ~3u;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) uk[k] = sortable(key[k]);
     }
+    /* key comparisons below are on the sortable integers; a missed box has
+     * key = inf, every hit box a finite entry */
+    const uint32_t miss = sortable(RT_INF) & ~3u;
     /* leaves, nearest first: records two at a time, both loads in flight
      * before the first test (a single last record is loaded twice, the
      * second from cache); tests stay in record order within a leaf */
@@ -344,13 +391,14 @@ __device__ __forceinline__ bool trav_step(const DevScene& S, Trav& T, Stack& sta
     uint32_t leaves = 0;
 #pragma unroll
     for (int k = 0; k < 4; ++k)
-        if (word[k] < 0 && key[k] < RT_INF) leaves |= 1u << k;
+        if (word[k] < 0 && uk[k] < miss) leaves |= 1u << k;
     while (leaves) {
         const int k = __builtin_ctz(leaves);
         leaves &= leaves - 1u;
-        const float kk = k == 0 ? key[0] : k == 1 ? key[1] : k == 2 ? key[2] : key[3];
-        const int32_t wk = k == 0 ? word[0] : k == 1 ? word[1] : k == 2 ? word[2] : word[3];
-        if (!(kk <= T.best + fabsf(T.best) * slack)) continue;
+        const bool b0 = (k & 1) != 0, b1 = (k & 2) != 0;
+        const uint32_t kk = b1 ? (b0 ? uk[3] : uk[2]) : (b0 ? uk[1] : uk[0]);
+        const int32_t wk = b1 ? (b0 ? word[3] : word[2]) : (b0 ? word[1] : word[0]);
+        if (!(kk <= sortable(T.best + fabsf(T.best) * slack))) continue;
         uint32_t first, count, kind;
         leaf_decode(wk, first, count, kind);
         for (uint32_t j = 0; j < count; j += 2) {
@@ -366,10 +414,10 @@ __device__ __forceinline__ bool trav_step(const DevScene& S, Trav& T, Stack& sta
     /* interior children, farthest first: each earlier choice is deferred,
      * so the nearest continues and the next nearest is on top */
     int32_t next = -1;
-    const float cut = T.best + fabsf(T.best) * slack;
+    const uint32_t cut = sortable(T.best + fabsf(T.best) * slack);
 #pragma unroll
     for (int k = 3; k >= 0; --k) {
-        if (word[k] < 0 || !(key[k] < RT_INF) || !(key[k] <= cut)) continue;
+        if (word[k] < 0 || !(uk[k] < miss) || !(uk[k] <= cut)) continue;
         if (next >= 0) {
             stack.put(T.sp, next);
             ++T.sp;
