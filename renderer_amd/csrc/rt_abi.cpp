@@ -388,6 +388,9 @@ int rt_scene_upload(rt_ctx* c, const rt_scene_desc* d)
     int br_status = rtb::build(refs, br);
     if (br_status != 0)
         return set_err(c, RT_ERR_BUILD, br_status == -2 ? "BVH deeper than the traversal stack" : "too many primitives");
+    /* node fetches use 32-bit byte offsets from the node base */
+    if (br.nodes.size() * sizeof(float) > 0xffffffffull)
+        return set_err(c, RT_ERR_BUILD, "BVH larger than 4 GiB of nodes");
 
     // primitive records in leaf order
     std::vector<float> rec((size_t)12 * G, 0.f);
