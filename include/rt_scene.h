@@ -29,33 +29,38 @@ extern "C" {
 #endif
 
 /* Material kinds: src/diffusematerial.cpp, src/mirrormaterial.cpp,
- * src/phongmaterial.cpp, src/darkmatter.cpp. */
+ * src/phongmaterial.cpp, src/darkmatter.cpp, src/blinnmaterial.cpp,
+ * src/transparentmaterial.cpp. */
 enum rt_material_type {
     RT_MAT_DIFFUSE = 0,
     RT_MAT_MIRROR = 1,
     RT_MAT_PHONG = 2,
-    RT_MAT_DARK = 3
+    RT_MAT_DARK = 3,
+    RT_MAT_BLINN = 4,
+    RT_MAT_REFRACTIVE = 5
 };
 
-/* Light kinds: src/arealight.cpp, src/pointlight.cpp, src/environmentmap.cpp */
+/* Light kinds: src/arealight.cpp, src/pointlight.cpp, src/environmentmap.cpp,
+ * src/conelight.cpp */
 enum rt_light_type {
     RT_LIGHT_AREA = 0,
     RT_LIGHT_POINT = 1,
-    RT_LIGHT_ENV = 2
+    RT_LIGHT_ENV = 2,
+    RT_LIGHT_CONE = 3
 };
 
 typedef struct rt_material {
     int32_t type;        /* rt_material_type */
-    float kd[3];         /* diffuse colour (RGB); DiffuseMaterial colour */
-    float ks[3];         /* Phong specular colour (RGB) */
-    float exponent;      /* Phong specular exponent */
+    float kd[3];         /* diffuse colour (RGB): Diffuse, Phong, Blinn */
+    float ks[3];         /* specular colour (RGB): Phong, Blinn */
+    float exponent;      /* specular exponent: Phong, Blinn; refraction coefficient: refractive */
 } rt_material;
 
 typedef struct rt_light {
     int32_t type;        /* rt_light_type */
-    float p[3];          /* area: parallelogram origin; point: position */
-    float u[3];          /* area: first edge  (normal = u x v, |u x v| = area) */
-    float v[3];          /* area: second edge */
+    float p[3];          /* area: parallelogram origin; point, cone: position */
+    float u[3];          /* area: first edge  (normal = u x v, |u x v| = area); cone: axis direction */
+    float v[3];          /* area: second edge; cone: v[0] = opening angle in radians */
     float intensity[3];  /* RGB; env: coefficient multiplying the map */
 } rt_light;
 

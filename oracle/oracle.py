@@ -52,6 +52,10 @@ def load():
     lib.oc_env_pdf.restype = ctypes.c_float
     lib.oc_env_pdf.argtypes = [P, P]
     lib.oc_tonemap.argtypes = [P, I64, ctypes.c_float, P]
+    lib.oc_transparent_dir.argtypes = [P, P, ctypes.c_float, ctypes.c_double, P]
+    lib.oc_bsdf.argtypes = [P, ctypes.c_int, P, P, P, P, P]
+    lib.oc_bsdf_sample.restype = ctypes.c_int
+    lib.oc_bsdf_sample.argtypes = [P, ctypes.c_int, P, P, ctypes.c_float, ctypes.c_float, ctypes.c_float, P]
     _lib = lib
     return lib
 
@@ -62,6 +66,15 @@ def philox(ctr, key):
     k = np.ascontiguousarray(key, np.uint32)
     out = np.zeros(4, np.uint32)
     lib.oc_philox4x32_10(c.ctypes.data, k.ctypes.data, out.ctypes.data)
+    return out
+
+
+def transparent_dir(d, n, ior, u):
+    """TransparentMaterial::outDirection for direction d, normal n."""
+    lib = load()
+    a, b = np.ascontiguousarray(d, np.float32), np.ascontiguousarray(n, np.float32)
+    out = np.zeros(3, np.float32)
+    lib.oc_transparent_dir(a.ctypes.data, b.ctypes.data, float(ior), float(u), out.ctypes.data)
     return out
 
 
@@ -144,6 +157,21 @@ class Oracle:
         out = np.zeros(3, np.float32)
         self.lib.oc_render_sample(self.h, ctypes.byref(v), ctypes.byref(p), int(view_id), px, py, s, out.ctypes.data)
         return out
+
+    def bsdf(self, mat, wo, wi, nf):
+        """(value RGB, density) of the unbiased estimators' BSDF of material index mat."""
+        a, b, n = (np.ascontiguousarray(v, np.float32) for v in (wo, wi, nf))
+        f = np.zeros(3, np.float32)
+        pdf = np.zeros(1, np.float32)
+        self.lib.oc_bsdf(self.h, int(mat), a.ctypes.data, b.ctypes.data, n.ctypes.data, f.ctypes.data, pdf.ctypes.data)
+        return f, float(pdf[0])
+
+    def bsdf_sample(self, mat, wo, nf, u0, u1, u2):
+        a, n = (np.ascontiguousarray(v, np.float32) for v in (wo, nf))
+        wi = np.zeros(3, np.float32)
+        ok = self.lib.oc_bsdf_sample(self.h, int(mat), a.ctypes.data, n.ctypes.data, float(u0), float(u1), float(u2),
+                                     wi.ctypes.data)
+        return (wi if ok else None)
 
     def env_sample(self, u0, u1):
         d = np.zeros(3, np.float32)

@@ -460,6 +460,83 @@ struct Phong : Material {
     int kind() const override { return RT_MAT_PHONG; }
 };
 
+inline int sgn_f(float x) { return (0.f < x) - (x < 0.f); }
+
+// src/blinnmaterial.cpp:30-84
+struct Blinn : Material {
+    Col kd, ks;
+    float e;
+    Blinn(Col d, Col s, float ex) : kd(d), ks(s), e(ex) {}
+    float D(V3 wh, V3 n) const { return (float)((e + 2) / (2 * M_PI) * std::pow((double)dot(wh, n), (double)e)); }
+    static float G(V3 wi, V3 wo, V3 n)
+    {
+        V3 wh = normalized(wi + wo);
+        return std::min(1.f, 2 * dot(n, wh) * std::min(dot(n, wo), dot(n, wi)) / dot(wo, wh));
+    }
+    Col brdf(const Hit& hit, V3 direction) const override
+    {
+        V3 n = normalized(hit.n);
+        V3 wo = -normalized(hit.ray.d);
+        V3 wi = -normalized(direction);
+        if (sgn_f(dot(n, wo)) != sgn_f(dot(n, wi))) return Col();
+        if (dot(n, wo) < 0) n = n * -1.f;
+        V3 wh = normalized(wi + wo);
+        return kd * (float)(1 / M_PI) + ks * (D(wh, n) * G(wi, wo, n) / (4 * dot(wo, n) * dot(wi, n)));
+    }
+    V3 out_direction(const Hit& hit, Sample s, float& pdf, Col& b) const override
+    {
+        V3 n = normalized(hit.n);
+        V3 wo = -normalized(hit.ray.d);
+        if (dot(n, wo) < 0) n = n * -1.f;
+        V3 wh = cosine_power_dir(s, n, pdf, e);
+        if (dot(wo, wh) < 0) {
+            pdf = 0;
+            b = Col();
+            return V3();
+        }
+        V3 out = -reflect(wo, wh);
+        pdf /= 4 * std::max(dot(wo, wh), .0001f);
+        b = brdf(hit, out);
+        return out;
+    }
+    int kind() const override { return RT_MAT_BLINN; }
+};
+
+// src/transparentmaterial.cpp:34-72: Fresnel-weighted choice between the
+// mirror and the refracted direction; `u` is the sample's first coordinate.
+V3 transparent_dir(V3 d_in, V3 n_in, float ior, double u)
+{
+    V3 direction = normalized(d_in), normal = normalized(n_in);
+    float ratio;
+    if (dot(-normal, direction) < 0.) {
+        normal = normal * -1.f;
+        ratio = ior;
+    } else {
+        ratio = 1 / ior;
+    }
+    float cosi = dot(-normal, direction);
+    float sino = ratio * std::sqrt(std::max(0., 1. - cosi * cosi));
+    float coso = std::sqrt(std::max(0., 1 - ratio * ratio * std::max(0., 1. - cosi * cosi)));
+    float rperp = (cosi - ratio * coso) / (cosi + ratio * coso);
+    float rpara = (ratio * cosi - coso) / (ratio * cosi + coso);
+    float refl = 1 / 2. * (rperp * rperp + rpara * rpara);
+    if (sino > 1 || u < refl) return reflect(direction, normal);
+    return ratio * direction + (ratio * cosi - sgn_f(cosi) * coso) * normal;
+}
+
+struct Transparent : Material {
+    float ior;
+    explicit Transparent(float c) : ior(c) {}
+    V3 out_direction(const Hit& hit, Sample s, float& pdf, Col& b) const override
+    {
+        pdf = 1;
+        b = Col();
+        return transparent_dir(hit.ray.d, hit.n, ior, s.x);
+    }
+    bool specular() const override { return true; }
+    int kind() const override { return RT_MAT_REFRACTIVE; }
+};
+
 // ---------------------------------------------------------------------------
 // Lights, src/light.h:31-37
 // ---------------------------------------------------------------------------
@@ -552,6 +629,31 @@ struct PointLight : Intersectable, Light {
         ++rays;
         if (sc.intersect(sr).hits()) return Col();
         return I * (1 / length_sq(direction));
+    }
+    Hit intersect(const Ray&) const override { return Hit(); }
+};
+
+// src/conelight.cpp:31-45: a point light lit only inside the cone of the
+// given opening angle about its axis
+struct ConeLight : Intersectable, Light {
+    V3 p, axis;
+    float angle;
+    Col I;
+    ConeLight(V3 p_, V3 a, float ang, Col i) : p(p_), axis(a), angle(ang), I(i) { box.mn = V3(); box.mx = V3(); }
+    bool admits(V3 direction) const
+    {
+        float a = std::acos(dot(normalized(direction), normalized(axis)));
+        return 0 <= a && a <= angle;
+    }
+    Col intensity(const Hit& hit, V3& direction, const Scene& sc, Sample, int64_t& rays) const override
+    {
+        V3 at = hit.p;
+        direction = at - p;
+        Ray sr(at, -normalized(direction), kEps, (float)((double)length(direction) - 0.001));
+        ++rays;
+        Hit sh = sc.intersect(sr);
+        if (admits(direction) && !sh.hits()) return I * (1 / length_sq(direction));
+        return Col();
     }
     Hit intersect(const Ray&) const override { return Hit(); }
 };
@@ -850,6 +952,14 @@ Col bsdf_eval(const Material* m, V3 wo, V3 wi, V3 nf)
         float s = (ph->n + 2) * (0.5f * inv_pi) * std::pow(c, ph->n);
         return ph->kd * inv_pi + ph->ks * s;
     }
+    if (m->kind() == RT_MAT_BLINN) {
+        auto* bl = static_cast<const Blinn*>(m);
+        float co = dot(wo, nf), ci = dot(wi, nf);
+        if (!(co > 0) || !(ci > 0)) return bl->kd * inv_pi;
+        V3 wh = normalized(wo + wi);
+        float s = bl->D(wh, nf) * Blinn::G(wi, wo, nf) / (4 * co * ci);
+        return bl->kd * inv_pi + bl->ks * s;
+    }
     return Col();
 }
 float bsdf_pdf(const Material* m, V3 wo, V3 wi, V3 nf)
@@ -865,6 +975,15 @@ float bsdf_pdf(const Material* m, V3 wo, V3 wi, V3 nf)
         V3 r = reflect(-wo, nf);
         float c = std::max(0.f, dot(r, wi));
         return pd * cs * inv_pi + (1 - pd) * (ph->n + 1) * (0.5f * inv_pi) * std::pow(c, ph->n);
+    }
+    if (m->kind() == RT_MAT_BLINN) {
+        auto* bl = static_cast<const Blinn*>(m);
+        float rd = lum(bl->kd), rs = lum(bl->ks);
+        float pd = rd / (rd + rs);
+        V3 wh = normalized(wo + wi);
+        float ch = std::max(0.f, dot(wh, nf));
+        float ph = (bl->e + 1) * (0.5f * inv_pi) * std::pow(ch, bl->e) / (4 * std::max(dot(wo, wh), 1e-4f));
+        return pd * cs * inv_pi + (1 - pd) * ph;
     }
     return 0;
 }
@@ -882,6 +1001,21 @@ bool bsdf_sample(const Material* m, V3 wo, V3 nf, float u0, float u1, float u2, 
         float pd = rd / (rd + rs);
         if (u2 < pd) wi = lobe_dir(u0, u1, nf, 1.f);
         else wi = lobe_dir(u0, u1, reflect(-wo, nf), ph->n);
+        return dot(wi, nf) > 0;
+    }
+    if (m->kind() == RT_MAT_BLINN) {
+        auto* bl = static_cast<const Blinn*>(m);
+        float rd = lum(bl->kd), rs = lum(bl->ks);
+        if (!(rd + rs > 0)) return false;
+        float pd = rd / (rd + rs);
+        if (u2 < pd) {
+            wi = lobe_dir(u0, u1, nf, 1.f);
+        } else {
+            V3 wh = lobe_dir(u0, u1, nf, bl->e);
+            float c = dot(wo, wh);
+            if (!(c > 0)) return false;
+            wi = wh * (2 * c) - wo;
+        }
         return dot(wi, nf) > 0;
     }
     return false;
@@ -941,6 +1075,12 @@ Col integrate_unbiased(const UnbiasedCtx& c, Ray ray, const SampleKey& k, int64_
             spec = true;
             continue;
         }
+        if (m->kind() == RT_MAT_REFRACTIVE) {
+            U4 gr = philox_uniforms(k.seed, k.view, k.pixel, k.sample, 1 + 2 * depth);
+            ray = Ray(x, transparent_dir(ray.d, n, static_cast<const Transparent*>(m)->ior, gr.u[0]));
+            spec = true;
+            continue;
+        }
         if (m->kind() == RT_MAT_DARK) break;
         V3 wo = -d;
         U4 gb = philox_uniforms(k.seed, k.view, k.pixel, k.sample, 1 + 2 * depth);
@@ -974,6 +1114,13 @@ Col integrate_unbiased(const UnbiasedCtx& c, Ray ray, const SampleKey& k, int64_
                 wi = dv * (1 / dist);
                 le = pt->I * (1 / d2);
                 ok = true;
+            } else if (auto* cl = dynamic_cast<const ConeLight*>(lt)) {
+                V3 dv = cl->p - x;
+                float d2 = dot(dv, dv);
+                dist = std::sqrt(d2);
+                wi = dv * (1 / dist);
+                le = cl->I * (1 / d2);
+                ok = cl->admits(-dv);
             } else if (auto* em = dynamic_cast<const EnvMap*>(lt)) {
                 float pdf;
                 em->sample(gl.u[0], gl.u[1], wi, pdf);
@@ -1038,6 +1185,10 @@ void* oc_scene_create(const rt_scene_desc* d, int accel)
         case RT_MAT_PHONG:
             mm = new Phong(Col(m.kd[0], m.kd[1], m.kd[2]), Col(m.ks[0], m.ks[1], m.ks[2]), m.exponent);
             break;
+        case RT_MAT_BLINN:
+            mm = new Blinn(Col(m.kd[0], m.kd[1], m.kd[2]), Col(m.ks[0], m.ks[1], m.ks[2]), m.exponent);
+            break;
+        case RT_MAT_REFRACTIVE: mm = new Transparent(m.exponent); break;
         default: mm = new DarkMatter(); break;
         }
         sc->mats.emplace_back(mm);
@@ -1064,6 +1215,11 @@ void* oc_scene_create(const rt_scene_desc* d, int accel)
             sc->owned.emplace_back(pl);
             sc->lights.push_back(pl);
             root_items.push_back(pl);
+        } else if (l.type == RT_LIGHT_CONE) {
+            auto* cl = new ConeLight(p, u, l.v[0], I);
+            sc->owned.emplace_back(cl);
+            sc->lights.push_back(cl);
+            root_items.push_back(cl);
         } else {
             auto* em = new EnvMap();
             em->W = d->env_width;
@@ -1244,6 +1400,31 @@ float oc_env_pdf(void* s, const float dir[3])
     const Scene& sc = *static_cast<Scene*>(s);
     if (!sc.env) return 0;
     return sc.env->pdf(V3(dir[0], dir[1], dir[2]));
+}
+
+void oc_transparent_dir(const float d[3], const float n[3], float ior, double u, float out[3])
+{
+    V3 o = transparent_dir(V3(d[0], d[1], d[2]), V3(n[0], n[1], n[2]), ior, u);
+    out[0] = o.x, out[1] = o.y, out[2] = o.z;
+}
+
+void oc_bsdf(void* s, int mat, const float wo[3], const float wi[3], const float nf[3], float f[3], float* pdf)
+{
+    const Scene& sc = *static_cast<Scene*>(s);
+    const Material* m = sc.mats[mat].get();
+    V3 a(wo[0], wo[1], wo[2]), b(wi[0], wi[1], wi[2]), n(nf[0], nf[1], nf[2]);
+    Col c = bsdf_eval(m, a, b, n);
+    f[0] = c.r, f[1] = c.g, f[2] = c.b;
+    *pdf = bsdf_pdf(m, a, b, n);
+}
+
+int oc_bsdf_sample(void* s, int mat, const float wo[3], const float nf[3], float u0, float u1, float u2, float wi[3])
+{
+    const Scene& sc = *static_cast<Scene*>(s);
+    V3 w;
+    bool ok = bsdf_sample(sc.mats[mat].get(), V3(wo[0], wo[1], wo[2]), V3(nf[0], nf[1], nf[2]), u0, u1, u2, w);
+    wi[0] = w.x, wi[1] = w.y, wi[2] = w.z;
+    return ok ? 1 : 0;
 }
 
 // Tonemapper::clamp (src/tonemapper.cpp:31-36): the float argument is

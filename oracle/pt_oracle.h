@@ -78,6 +78,16 @@ float oc_env_pdf(void* scene, const float dir[3]);
  * n bytes. */
 void oc_tonemap(const float* film, int64_t n, float gamma, uint8_t* out);
 
+/* TransparentMaterial::outDirection for ray direction d, surface normal n,
+ * refraction coefficient ior and sample coordinate u. */
+void oc_transparent_dir(const float d[3], const float n[3], float ior, double u, float out[3]);
+
+/* The unbiased estimators' BSDF of scene material `mat`: value and density
+ * of wo -> wi about the viewer-side normal nf, and a sample from uniforms. */
+void oc_bsdf(void* scene, int mat, const float wo[3], const float wi[3], const float nf[3], float f[3], float* pdf);
+int oc_bsdf_sample(void* scene, int mat, const float wo[3], const float nf[3], float u0, float u1, float u2,
+                   float wi[3]);
+
 #ifdef __cplusplus
 }
 #endif

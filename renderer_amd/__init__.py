@@ -2,16 +2,19 @@
 (librt_hip.so, include/rt_abi.h), with a host-side scene model and
 integrator interface named after the reference renderer's classes."""
 from . import _abi
-from ._abi import (LIGHT_AREA, LIGHT_ENV, LIGHT_POINT, MAT_DARK, MAT_DIFFUSE, MAT_MIRROR, MAT_PHONG,
+from ._abi import (LIGHT_AREA, LIGHT_CONE, LIGHT_ENV, LIGHT_POINT, MAT_BLINN, MAT_DARK, MAT_DIFFUSE, MAT_MIRROR,
+                   MAT_PHONG, MAT_REFRACTIVE,
                    MODE_COMPAT, MODE_GUIDED, MODE_UNBIASED, RtError)
 from .integrator import GpuContext, GpuUniDiPathTracingIntegrator, PerPixelRenderer
-from .scene import (AreaLight, Camera, DarkMatter, DiffuseMaterial, EnvironmentMap, IntersectableList, Mesh,
-                    MirrorMaterial, PhongMaterial, PointLight, Quad, Scene, Sphere, Triangle)
+from .scene import (AreaLight, BlinnMaterial, Camera, ConeLight, DarkMatter, DiffuseMaterial, EnvironmentMap,
+                    IntersectableList, Mesh, MirrorMaterial, PhongMaterial, PointLight, Quad, Scene, Sphere,
+                    TransparentMaterial, Triangle)
 from .scene_file import SceneFile
 
 __all__ = [
     "GpuContext", "GpuUniDiPathTracingIntegrator", "PerPixelRenderer", "RtError",
     "AreaLight", "Camera", "DarkMatter", "DiffuseMaterial", "EnvironmentMap", "IntersectableList", "Mesh",
     "MirrorMaterial", "PhongMaterial", "PointLight", "Quad", "Scene", "Sphere", "Triangle", "SceneFile",
+    "BlinnMaterial", "TransparentMaterial", "ConeLight",
     "MODE_COMPAT", "MODE_UNBIASED", "MODE_GUIDED",
 ]

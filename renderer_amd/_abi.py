@@ -88,8 +88,8 @@ class RtLaunchConfig(ctypes.Structure):
                 ("count_traversal", ctypes.c_int32), ("guide_epochs", ctypes.c_int32)]
 
 
-MAT_DIFFUSE, MAT_MIRROR, MAT_PHONG, MAT_DARK = 0, 1, 2, 3
-LIGHT_AREA, LIGHT_POINT, LIGHT_ENV = 0, 1, 2
+MAT_DIFFUSE, MAT_MIRROR, MAT_PHONG, MAT_DARK, MAT_BLINN, MAT_REFRACTIVE = 0, 1, 2, 3, 4, 5
+LIGHT_AREA, LIGHT_POINT, LIGHT_ENV, LIGHT_CONE = 0, 1, 2, 3
 MODE_COMPAT, MODE_UNBIASED, MODE_GUIDED = 0, 1, 2
 
 STATUS = {0: "ok", -1: "invalid argument", -2: "no scene uploaded", -3: "HIP runtime error",

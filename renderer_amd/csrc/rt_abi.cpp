@@ -291,7 +291,7 @@ int rt_scene_upload(rt_ctx* c, const rt_scene_desc* d)
             if (env >= 0 || !d->env_rgb || d->env_width <= 0 || d->env_height <= 0)
                 return set_err(c, RT_ERR_INVALID, "environment light needs exactly one map");
             env = i;
-        } else if (d->lights[i].type != RT_LIGHT_POINT)
+        } else if (d->lights[i].type != RT_LIGHT_POINT && d->lights[i].type != RT_LIGHT_CONE)
             return set_err(c, RT_ERR_INVALID, "unknown light type");
     }
     for (int i = 0; i < nT; ++i) {
@@ -425,6 +425,9 @@ int rt_scene_upload(rt_ctx* c, const rt_scene_desc* d)
             for (int j = 0; j < 3; ++j) qt_normalize(d->normals + 3 * d->tri_vertices[3 * i + j], &vn[9 * i + 3 * j]);
     }
     std::vector<RtMaterial> mats(d->n_materials);
+    for (int i = 0; i < d->n_materials; ++i)
+        if (d->materials[i].type < RT_MAT_DIFFUSE || d->materials[i].type > RT_MAT_REFRACTIVE)
+            return set_err(c, RT_ERR_INVALID, "unknown material type");
     for (int i = 0; i < d->n_materials; ++i) {
         const rt_material& m = d->materials[i];
         RtMaterial& M = mats[i];

@@ -280,6 +280,66 @@ __device__ __forceinline__ vec3 area_intensity(const RtLight& L, vec3 direction)
     return ld3(L.I) * c;
 }
 
+__device__ __forceinline__ int sgn(float x) { return (0.f < x) - (x < 0.f); }
+
+/* BlinnMaterial, src/blinnmaterial.cpp:34-62: normalised Blinn-Phong
+ * distribution D and the Cook-Torrance shadowing term G, about the normal n
+ * (facing wo). */
+__device__ __forceinline__ float blinn_D(float e, vec3 wh, vec3 n) { return (e + 2.f) * RT_INV_2PI * powf(dot(wh, n), e); }
+__device__ __forceinline__ float blinn_G(vec3 wi, vec3 wo, vec3 n)
+{
+    vec3 wh = norm(wi + wo);
+    return fminf(1.f, 2.f * dot(n, wh) * fminf(dot(n, wo), dot(n, wi)) / dot(wo, wh));
+}
+/* BlinnMaterial::brdf(hit, direction): wo towards the viewer, wi = -direction
+ * (the direction handed in points from the light to the surface), zero when
+ * they lie on opposite sides.  For a direction sampled by outDirection this
+ * is zero as well, which ends the throughput of the path there. */
+__device__ __forceinline__ vec3 blinn_ref(const RtMaterial& m, vec3 n, vec3 din, vec3 dir)
+{
+    vec3 wo = -din, wi = -norm(dir);
+    if (sgn(dot(n, wo)) != sgn(dot(n, wi))) return mk(0.f, 0.f, 0.f);
+    if (dot(n, wo) < 0.f) n = -n;
+    vec3 wh = norm(wi + wo);
+    float s = blinn_D(m.exponent, wh, n) * blinn_G(wi, wo, n) / (4.f * dot(wo, n) * dot(wi, n));
+    return ld3(m.kd) * RT_INV_PI + ld3(m.ks) * s;
+}
+
+/* TransparentMaterial::outDirection, src/transparentmaterial.cpp:34-72:
+ * reflection with the unpolarised Fresnel reflectance as probability (and
+ * always under total internal reflection), refraction otherwise; ior is the
+ * refraction coefficient of the inside.  Throughput is unchanged (a
+ * specular vertex).  Types follow the reference (the sines in double). */
+__device__ __forceinline__ vec3 refractive_dir(vec3 d_in, vec3 n_in, float ior, float u)
+{
+    vec3 direction = norm(d_in), normal = norm(n_in);
+    float ratio;
+    if (dot(-normal, direction) < 0.f) {
+        normal = -normal;
+        ratio = ior;
+    } else {
+        ratio = 1.f / ior;
+    }
+    float cosi = dot(-normal, direction);
+    double s2 = fmax(0.0, 1.0 - (double)(cosi * cosi));
+    float sino = (float)((double)ratio * sqrt(s2));
+    float coso = (float)sqrt(fmax(0.0, 1.0 - (double)(ratio * ratio) * s2));
+    float rperp = (cosi - ratio * coso) / (cosi + ratio * coso);
+    float rpara = (ratio * cosi - coso) / (ratio * cosi + coso);
+    float refl = (float)(0.5 * (double)(rperp * rperp + rpara * rpara));
+    if (sino > 1.f || (double)u < (double)refl) return reflect(direction, normal);
+    return direction * ratio + normal * (ratio * cosi - (float)sgn(cosi) * coso);
+}
+
+/* ConeLight::getIntensity, src/conelight.cpp:31-45: lit when the angle
+ * between the light's axis and the direction from the light to the point is
+ * at most the opening angle (v[0]). */
+__device__ __forceinline__ bool cone_admits(const RtLight& L, vec3 dir)
+{
+    float a = acosf(dot(norm(dir), norm(ld3(L.u))));
+    return 0.f <= a && a <= L.v[0];
+}
+
 /* Reference BRDF value, src/diffusematerial.cpp:25-28, src/phongmaterial.cpp:31-49.
  * `n` is the unflipped unit normal, `din` the unit incoming ray direction,
  * `dir` the direction from the light towards the point. */
@@ -292,6 +352,7 @@ __device__ __forceinline__ vec3 brdf_ref(const RtMaterial& m, vec3 n, vec3 din, 
         float spec = (m.exponent + 2.f) * RT_INV_2PI * powf(dp, m.exponent);
         return ld3(m.kd) * RT_INV_PI + ld3(m.ks) * spec;
     }
+    if (m.type == RT_MAT_BLINN) return blinn_ref(m, n, din, dir);
     return mk(0.f, 0.f, 0.f);
 }
 /* Phong specular term alone, src/phongmaterial.cpp:42-49. */
@@ -314,6 +375,9 @@ __device__ __forceinline__ vec3 lobe_dir(float u0, float u1, vec3 axis, float ex
     vec3 t = cross(v, axis);
     return norm(t * (r * cp) + v * (r * sp) + axis * z);
 }
+/* Blinn in the unbiased estimators: the same BRDF, with wo and wi both
+ * leaving the surface; sampled as a luminance-weighted mixture of the
+ * diffuse lobe and the reference's cos^n half-vector lobe. */
 __device__ __forceinline__ vec3 bsdf_eval(const RtMaterial& m, vec3 wo, vec3 wi, vec3 nf)
 {
     if (m.type == RT_MAT_DIFFUSE) return ld3(m.kd) * RT_INV_PI;
@@ -321,6 +385,13 @@ __device__ __forceinline__ vec3 bsdf_eval(const RtMaterial& m, vec3 wo, vec3 wi,
         vec3 r = reflect(-wo, nf);
         float c = fmaxf(0.f, dot(r, wi));
         float s = (m.exponent + 2.f) * RT_INV_2PI * powf(c, m.exponent);
+        return ld3(m.kd) * RT_INV_PI + ld3(m.ks) * s;
+    }
+    if (m.type == RT_MAT_BLINN) {
+        float co = dot(wo, nf), ci = dot(wi, nf);
+        if (!(co > 0.f) || !(ci > 0.f)) return ld3(m.kd) * RT_INV_PI;
+        vec3 wh = norm(wo + wi);
+        float s = blinn_D(m.exponent, wh, nf) * blinn_G(wi, wo, nf) / (4.f * co * ci);
         return ld3(m.kd) * RT_INV_PI + ld3(m.ks) * s;
     }
     return mk(0.f, 0.f, 0.f);
@@ -335,6 +406,13 @@ __device__ __forceinline__ float bsdf_pdf(const RtMaterial& m, vec3 wo, vec3 wi,
         vec3 r = reflect(-wo, nf);
         float c = fmaxf(0.f, dot(r, wi));
         return pd * cs * RT_INV_PI + (1.f - pd) * (m.exponent + 1.f) * RT_INV_2PI * powf(c, m.exponent);
+    }
+    if (m.type == RT_MAT_BLINN) {
+        float pd = m.rho_d / (m.rho_d + m.rho_s);
+        vec3 wh = norm(wo + wi);
+        float ch = fmaxf(0.f, dot(wh, nf));
+        float ph = (m.exponent + 1.f) * RT_INV_2PI * powf(ch, m.exponent) / (4.f * fmaxf(dot(wo, wh), 1e-4f));
+        return pd * cs * RT_INV_PI + (1.f - pd) * ph;
     }
     return 0.f;
 }
@@ -351,10 +429,21 @@ __device__ __forceinline__ bool bsdf_sample(const RtMaterial& m, vec3 wo, vec3 n
         wi = u2 < pd ? lobe_dir(u0, u1, nf, 1.f) : lobe_dir(u0, u1, reflect(-wo, nf), m.exponent);
         return dot(wi, nf) > 0.f;
     }
+    if (m.type == RT_MAT_BLINN) {
+        if (!(m.rho_d + m.rho_s > 0.f)) return false;
+        float pd = m.rho_d / (m.rho_d + m.rho_s);
+        if (u2 < pd) {
+            wi = lobe_dir(u0, u1, nf, 1.f);
+        } else {
+            vec3 wh = lobe_dir(u0, u1, nf, m.exponent);
+            float c = dot(wo, wh);
+            if (!(c > 0.f)) return false;
+            wi = wh * (2.f * c) - wo;
+        }
+        return dot(wi, nf) > 0.f;
+    }
     return false;
 }
-
-__device__ __forceinline__ int sgn(float x) { return (0.f < x) - (x < 0.f); }
 
 /* Packed path bookkeeping in info.w: stored vertices (8 bits), path length
  * (8 bits), specular-chain flag (bit 16). */
@@ -387,8 +476,9 @@ __device__ __forceinline__ bool light_sample(const DevScene& S, vec3 x, const U4
         pl = d2 / (cl * an) / nl;
         return true;
     }
-    if (Lt.type == RT_LIGHT_POINT) {
+    if (Lt.type == RT_LIGHT_POINT || Lt.type == RT_LIGHT_CONE) {
         vec3 dv = ld3(Lt.p) - x;
+        if (Lt.type == RT_LIGHT_CONE && !cone_admits(Lt, -dv)) return false;
         float d2 = dot(dv, dv);
         dist = sqrtf(d2);
         wi = dv * (1.f / dist);
@@ -664,10 +754,10 @@ __device__ void shade_compat(const DevScene& S, const PathState& ps, const PtPar
     vec3 n = hit_normal(S, g, x, h.z, h.w);
     U4 gs = rng4(P.seed, (uint32_t)info.z, (uint32_t)info.x, (uint32_t)info.y, 1u + 2u * depth);
 
-    if (M.type == RT_MAT_MIRROR) {
+    if (M.type == RT_MAT_MIRROR || M.type == RT_MAT_REFRACTIVE) {
         /* specular vertex: not stored; extend the path length once (:87-91) */
         if (depth == L - 1 && L < P.max_depth) ++L;
-        vec3 out = reflect(d, n);
+        vec3 out = M.type == RT_MAT_MIRROR ? reflect(d, n) : refractive_dir(d, n, M.exponent, gs.u[0]);
         if (depth + 1 < L) {
             ps.ray_o[p] = make_float4(x.x, x.y, x.z, RT_EPS);
             ps.ray_d[p] = make_float4(out.x, out.y, out.z, RT_INF);
@@ -703,6 +793,18 @@ __device__ void shade_compat(const DevScene& S, const PathState& ps, const PtPar
             if (dot(out, nrm) < 1.f) pdf = 0.f;
             brdf = phong_spec_ref(M, n, dn, out);
         }
+    } else if (M.type == RT_MAT_BLINN) {
+        /* BlinnMaterial::outDirection, src/blinnmaterial.cpp:64-84 */
+        vec3 wo = -dn;
+        vec3 nrm = dot(n, wo) < 0.f ? -n : n;
+        vec3 wh = cos_pow_dir(gs.u[0], gs.u[1], nrm, M.exponent, pdf);
+        if (dot(wo, wh) < 0.f) {
+            pdf = 0.f;
+        } else {
+            out = -reflect(wo, wh);
+            pdf /= 4.f * fmaxf(dot(wo, wh), 1e-4f);
+            brdf = blinn_ref(M, n, dn, out);
+        }
     }
 
     /* next-event estimate for this stored vertex k (Light::getIntensity) */
@@ -721,14 +823,14 @@ __device__ void shade_compat(const DevScene& S, const PathState& ps, const PtPar
             sd = make_float4(sdir.x, sdir.y, sdir.z, dl - RT_EPS);
             Li = area_intensity(Lt, dir);
             has_ray = true;
-        } else if (Lt.type == RT_LIGHT_POINT) {
+        } else if (Lt.type == RT_LIGHT_POINT || Lt.type == RT_LIGHT_CONE) {
             dir = x - ld3(Lt.p);
             float dl = len(dir);
             so = make_float4(x.x, x.y, x.z, RT_EPS);
             vec3 sdir = -norm(dir);
             sd = make_float4(sdir.x, sdir.y, sdir.z, dl - RT_EPS);
             Li = ld3(Lt.I) * (1.f / dot(dir, dir));
-            has_ray = true;
+            has_ray = Lt.type == RT_LIGHT_POINT || cone_admits(Lt, dir);
         } else {
             /* src/environmentmap.cpp:40-55: cosine sample about the unflipped
              * normal, shadow ray over negative t, value / pdf looked up at
@@ -798,9 +900,13 @@ __device__ void shade_unbiased(const DevScene& S, const PathState& ps, const PtP
     vec3 x = o + h.x * d;
     vec3 n = hit_normal(S, g, x, h.z, h.w);
     vec3 nf = dot(n, d) > 0.f ? -n : n;
-    if (M.type == RT_MAT_MIRROR) {
+    if (M.type == RT_MAT_MIRROR || M.type == RT_MAT_REFRACTIVE) {
         if (depth + 1 < P.max_depth) {
             vec3 out = reflect(d, n);
+            if (M.type == RT_MAT_REFRACTIVE) {
+                U4 gr = rng4(P.seed, (uint32_t)info.z, (uint32_t)info.x, (uint32_t)info.y, 1u + 2u * depth);
+                out = refractive_dir(d, n, M.exponent, gr.u[0]);
+            }
             ps.ray_o[p] = make_float4(x.x, x.y, x.z, RT_EPS);
             ps.ray_d[p] = make_float4(out.x, out.y, out.z, RT_INF);
             ps.info[p] = make_int4(info.x, info.y, info.z, pk(0, 0, 1));
@@ -808,7 +914,7 @@ __device__ void shade_unbiased(const DevScene& S, const PathState& ps, const PtP
         }
         return;
     }
-    if (M.type != RT_MAT_DIFFUSE && M.type != RT_MAT_PHONG) return;
+    if (M.type != RT_MAT_DIFFUSE && M.type != RT_MAT_PHONG && M.type != RT_MAT_BLINN) return;
     vec3 wo = -d;
     const bool last = depth + 1 >= P.max_depth;
     if (S.n_lights > 0) {
@@ -895,9 +1001,13 @@ __device__ void shade_guided(const DevScene& S, const PathState& ps, const Guide
     vec3 x = o + h.x * d;
     vec3 n = hit_normal(S, g, x, h.z, h.w);
     vec3 nf = dot(n, d) > 0.f ? -n : n;
-    if (M.type == RT_MAT_MIRROR) {
+    if (M.type == RT_MAT_MIRROR || M.type == RT_MAT_REFRACTIVE) {
         if (depth + 1 < P.max_depth) {
             vec3 out = reflect(d, n);
+            if (M.type == RT_MAT_REFRACTIVE) {
+                U4 gr = rng4(P.seed, (uint32_t)info.z, (uint32_t)info.x, (uint32_t)info.y, 1u + 2u * depth);
+                out = refractive_dir(d, n, M.exponent, gr.u[0]);
+            }
             ps.ray_o[p] = make_float4(x.x, x.y, x.z, RT_EPS);
             ps.ray_d[p] = make_float4(out.x, out.y, out.z, RT_INF);
             ps.info[p] = make_int4(info.x, info.y, info.z, pk(0, 0, 1));
@@ -905,7 +1015,7 @@ __device__ void shade_guided(const DevScene& S, const PathState& ps, const Guide
         }
         return;
     }
-    if (M.type != RT_MAT_DIFFUSE && M.type != RT_MAT_PHONG) return;
+    if (M.type != RT_MAT_DIFFUSE && M.type != RT_MAT_PHONG && M.type != RT_MAT_BLINN) return;
     vec3 wo = -d;
     const bool last = depth + 1 >= P.max_depth;
     /* the network is not evaluated for the last bounce */

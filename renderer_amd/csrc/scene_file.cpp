@@ -1641,8 +1641,16 @@ private:
             b.m.exponent = m.f;
             break;
         case MatKind::Mirror: b.m.type = RT_MAT_MIRROR; break;
-        case MatKind::Blinn: b.m.type = RT_MAT_DARK; b.unsupported = "blinn material"; break;
-        case MatKind::Refractive: b.m.type = RT_MAT_DARK; b.unsupported = "refractive material"; break;
+        case MatKind::Blinn:
+            b.m.type = RT_MAT_BLINN;
+            b.m.kd[0] = m.a.x, b.m.kd[1] = m.a.y, b.m.kd[2] = m.a.z;
+            b.m.ks[0] = m.b.x, b.m.ks[1] = m.b.y, b.m.ks[2] = m.b.z;
+            b.m.exponent = m.f;
+            break;
+        case MatKind::Refractive:
+            b.m.type = RT_MAT_REFRACTIVE;
+            b.m.exponent = m.f;
+            break;
         case MatKind::Texture:
         case MatKind::ScaledTexture:
             if (!readable(m.s)) {  // TextureMaterial::load failed -> DarkMatter
@@ -1944,7 +1952,13 @@ int rt_scene_file_scene(rt_scene_file* f, rt_scene_desc* desc, rt_camera* camera
                 c.v[0] = L.v.x, c.v[1] = L.v.y, c.v[2] = L.v.z;
                 c.intensity[0] = L.I.x, c.intensity[1] = L.I.y, c.intensity[2] = L.I.z;
                 break;
-            case LightKind::Cone: return fail(RT_ERR_UNSUPPORTED, "conelight is not rendered by the GPU path");
+            case LightKind::Cone:
+                c.type = RT_LIGHT_CONE;
+                c.p[0] = L.p.x, c.p[1] = L.p.y, c.p[2] = L.p.z;
+                c.u[0] = L.u.x, c.u[1] = L.u.y, c.u[2] = L.u.z;
+                c.v[0] = L.angle;
+                c.intensity[0] = L.I.x, c.intensity[1] = L.I.y, c.intensity[2] = L.I.z;
+                break;
             case LightKind::Env:
                 if (readable(L.file))
                     return fail(RT_ERR_UNSUPPORTED, "environmentmap(\"" + L.file +

@@ -56,9 +56,16 @@ public:
     int diffuse(Vec3 rgb) { return material(RT_MAT_DIFFUSE, rgb, Vec3{0, 0, 0}, 0.f); }
     int mirror() { return material(RT_MAT_MIRROR, Vec3{0, 0, 0}, Vec3{0, 0, 0}, 0.f); }
     int phong(Vec3 kd, Vec3 ks, float n) { return material(RT_MAT_PHONG, kd, ks, n); }
+    int blinn(Vec3 kd, Vec3 ks, float n) { return material(RT_MAT_BLINN, kd, ks, n); }
+    int refractive(float ior) { return material(RT_MAT_REFRACTIVE, Vec3{0, 0, 0}, Vec3{0, 0, 0}, ior); }
 
     void areaLight(Vec3 o, Vec3 u, Vec3 v, Vec3 intensity) { light(RT_LIGHT_AREA, o, u, v, intensity); }
     void pointLight(Vec3 p, Vec3 intensity) { light(RT_LIGHT_POINT, p, Vec3{0, 0, 0}, Vec3{0, 0, 0}, intensity); }
+    /* src/conelight.h: opening angle in radians */
+    void coneLight(Vec3 p, Vec3 direction, float angle, Vec3 intensity)
+    {
+        light(RT_LIGHT_CONE, p, direction, Vec3{angle, 0, 0}, intensity);
+    }
     /* latitude-longitude RGB image, row 0 at +y */
     void environmentMap(int w, int h, const std::vector<float>& rgb, Vec3 coefficient)
     {

@@ -1,8 +1,9 @@
 """Host-side scene model.
 
 Mirrors the reference's scene classes (Camera, DiffuseMaterial,
-MirrorMaterial, PhongMaterial, DarkMatter, AreaLight, PointLight,
-EnvironmentMap, Triangle, Quad, Sphere, IntersectableList, Scene) with the
+MirrorMaterial, PhongMaterial, BlinnMaterial, TransparentMaterial,
+DarkMatter, AreaLight, PointLight, ConeLight, EnvironmentMap, Triangle, Quad,
+Sphere, IntersectableList, Scene) with the
 same constructor arguments, and flattens a scene into the pointer-free
 arrays of ``rt_scene_desc`` (include/rt_scene.h).
 
@@ -63,6 +64,25 @@ class DarkMatter(Material):
     kind = _abi.MAT_DARK
 
 
+class BlinnMaterial(Material):
+    """src/blinnmaterial.h: diffuse colour, specular colour, exponent."""
+    kind = _abi.MAT_BLINN
+
+    def __init__(self, color, specular_color, specular_coefficient):
+        self.color = _v3(color)
+        self.specular_color = _v3(specular_color)
+        self.specular_coefficient = float(specular_coefficient)
+
+
+class TransparentMaterial(Material):
+    """src/transparentmaterial.h: refraction coefficient (the scene language's
+    refractive(ior)); a specular reflector / refractor."""
+    kind = _abi.MAT_REFRACTIVE
+
+    def __init__(self, refraction_coefficient):
+        self.refraction_coefficient = float(refraction_coefficient)
+
+
 # ---------------------------------------------------------------------------
 # lights, src/light.h:31-37
 # ---------------------------------------------------------------------------
@@ -82,6 +102,14 @@ class PointLight(Light):
 
     def __init__(self, location, intensity):
         self.location, self.intensity = _v3(location), _v3(intensity)
+
+
+class ConeLight(Light):
+    """src/conelight.h: location, axis direction, opening angle (radians), intensity."""
+
+    def __init__(self, location, direction, opening_angle, intensity):
+        self.location, self.direction = _v3(location), _v3(direction)
+        self.opening_angle, self.intensity = float(opening_angle), _v3(intensity)
 
 
 class EnvironmentMap(Light):
@@ -227,10 +255,12 @@ class Scene:
                 c.type = m.kind
                 if isinstance(m, DiffuseMaterial):
                     c.kd[:] = m.color.tolist()
-                elif isinstance(m, PhongMaterial):
+                elif isinstance(m, (PhongMaterial, BlinnMaterial)):
                     c.kd[:] = m.color.tolist()
                     c.ks[:] = m.specular_color.tolist()
                     c.exponent = m.specular_coefficient
+                elif isinstance(m, TransparentMaterial):
+                    c.exponent = m.refraction_coefficient
                 mat_index[id(m)] = len(mats)
                 mats.append(c)
             return mat_index[id(m)]
@@ -244,6 +274,10 @@ class Scene:
             elif isinstance(L, PointLight):
                 c.type = _abi.LIGHT_POINT
                 c.p[:] = L.location.tolist()
+            elif isinstance(L, ConeLight):
+                c.type = _abi.LIGHT_CONE
+                c.p[:], c.u[:] = L.location.tolist(), L.direction.tolist()
+                c.v[0] = L.opening_angle
             elif isinstance(L, EnvironmentMap):
                 if env is not None:
                     raise ValueError("at most one environment map")

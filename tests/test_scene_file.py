@@ -86,10 +86,12 @@ def test_area_lights_dump_as_the_reference_spells_them():
 def test_obj_import_faces_materials_and_overrides():
     sf = SceneFile.load(os.path.join(GOLDEN, "dump_obj.scene"))
     assert "ObjReader: 3 triangles, 2 faces" in sf.messages
-    # b_mat is Blinn, which the GPU path does not render
-    with pytest.raises(_abi.RtError) as e:
-        sf.arrays()
-    assert e.value.status == _abi.ERR_UNSUPPORTED and "blinn" in str(e.value)
+    # b_mat is the MTL's Blinn material (Ks and Ns given)
+    a = sf.arrays()
+    m = a.materials[a.tri_material[0]]
+    assert (m.type, tuple(np.float32(m.kd)), tuple(np.float32(m.ks)), m.exponent) == (
+        _abi.MAT_BLINN, tuple(np.float32([0.2, 0.4, 0.6])), tuple(np.float32([0.1, 0.1, 0.1])), 20.0)
+    assert a.materials[a.tri_material[2]].type == _abi.MAT_DIFFUSE
     # a later assignment overrides the MTL material of the same name
     base = os.path.join(GOLDEN, "")
     text = open(os.path.join(GOLDEN, "dump_obj.scene")).read()
@@ -209,7 +211,6 @@ def test_unsupported_constructs_parse_and_dump_but_do_not_render():
         "plane([0 1 0 1], diffuse([1 1 1]))": "plane",
         "box([0 0 0],[1 1 1], diffuse([1 1 1]))": "box",
         "union(sphere([0 0 0],1,diffuse([1 1 1])), box([0 0 0],[1 1 1],mirror(1)))": "CSG",
-        "sphere([0 0 0], 1, refractive(1.5))": "refractive",
         "instance(scale([1 2 1]), sphere([0 0 0], 1, diffuse([1 1 1])))": "non-similarity",
     }
     for obj, word in cases.items():
@@ -219,12 +220,18 @@ def test_unsupported_constructs_parse_and_dump_but_do_not_render():
         with pytest.raises(_abi.RtError) as e:
             sf.arrays()
         assert e.value.status == _abi.ERR_UNSUPPORTED and word in str(e.value), str(e.value)
+
+
+def test_cone_light_and_refractive_material_map_to_the_gpu_kinds():
     sf = SceneFile.parse("camera = camera([0 0 5],[0 0 0],[0 1 0],0.5,8,8);\n"
-                         "lights = lights{conelight([0 1 0],[0 -1 0],0.5,[1 1 1])};\n"
-                         "intersectable = intersectables{};")
-    with pytest.raises(_abi.RtError) as e:
-        sf.arrays()
-    assert e.value.status == _abi.ERR_UNSUPPORTED and "conelight" in str(e.value)
+                         "lights = lights{conelight([0 1 0],[0 -1 0],0.5,[1 2 3])};\n"
+                         "intersectable = intersectables{ sphere([0 0 0], 1, refractive(1.5)) };")
+    a = sf.arrays()
+    L = a.lights[0]
+    assert (L.type, tuple(L.p), tuple(L.u), np.float32(L.v[0]), tuple(L.intensity)) == (
+        _abi.LIGHT_CONE, (0, 1, 0), (0, -1, 0), np.float32(0.5), (1, 2, 3))
+    m = a.materials[a.sphere_material[0]]
+    assert (m.type, m.exponent) == (_abi.MAT_REFRACTIVE, 1.5)
 
 
 def test_missing_environment_map_is_a_black_light():
