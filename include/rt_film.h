@@ -49,6 +49,11 @@ int rt_film_write_mat(const char* path, const float* film, int32_t width, int32_
 int64_t rt_unique_filename(const char* path, char* out, int64_t cap);
 const char* rt_film_last_error(void);
 
+/* Radiance RGBE (.hdr) image, as the reference loads environment maps
+ * (src/environmentmap.cpp:20-33, OpenCV imread): width and height, and with
+ * rgb != NULL the height * width * 3 RGB floats (cap = floats available). */
+int rt_image_read_hdr(const char* path, int32_t* width, int32_t* height, float* rgb, int64_t cap);
+
 #ifdef __cplusplus
 }
 #endif

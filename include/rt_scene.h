@@ -88,10 +88,13 @@ typedef struct rt_scene_desc {
     int32_t n_lights;
     const rt_light* lights;
 
-    /* latitude-longitude environment map, row-major, RGB float, row 0 = +y
-     * pole.  Used by an RT_LIGHT_ENV light; NULL when there is none. */
+    /* environment map, row-major, RGB float, used by an RT_LIGHT_ENV light;
+     * NULL when there is none.  env_kind 0: latitude-longitude, row 0 = +y
+     * pole; 1: angular light probe looked up as the reference's
+     * EnvironmentMap::get (src/environmentmap.cpp:57-66). */
     int32_t env_width, env_height;
     const float* env_rgb;
+    int32_t env_kind;
 } rt_scene_desc;
 
 /* Pinhole camera, src/camera.h:34-101: eye, look-at, up, vertical field of

@@ -19,8 +19,8 @@
  * rt_scene_file_dump_bvh writes the --dump-bvh text.
  *
  * What the GPU path does not render (plane, box and CSG objects, loadable
- * texture materials, environment maps loaded from files, spheres under
- * non-similarity instance transforms) still parses and dumps; rt_scene_file_scene then returns
+ * texture materials, spheres under non-similarity instance transforms)
+ * still parses and dumps; rt_scene_file_scene then returns
  * RT_ERR_UNSUPPORTED naming the construct.  Input the reference accepts
  * with a warning on stderr (unknown material variable -> DarkMatter,
  * unreadable OBJ -> empty list, unreadable texture -> DarkMatter,

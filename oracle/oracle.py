@@ -51,6 +51,7 @@ def load():
     lib.oc_env_sample.argtypes = [P, ctypes.c_float, ctypes.c_float, P, P]
     lib.oc_env_pdf.restype = ctypes.c_float
     lib.oc_env_pdf.argtypes = [P, P]
+    lib.oc_env_value.argtypes = [P, P, P]
     lib.oc_tonemap.argtypes = [P, I64, ctypes.c_float, P]
     lib.oc_transparent_dir.argtypes = [P, P, ctypes.c_float, ctypes.c_double, P]
     lib.oc_bsdf.argtypes = [P, ctypes.c_int, P, P, P, P, P]
@@ -172,6 +173,12 @@ class Oracle:
         ok = self.lib.oc_bsdf_sample(self.h, int(mat), a.ctypes.data, n.ctypes.data, float(u0), float(u1), float(u2),
                                      wi.ctypes.data)
         return (wi if ok else None)
+
+    def env_value(self, d):
+        a = np.ascontiguousarray(d, np.float32)
+        out = np.zeros(3, np.float32)
+        self.lib.oc_env_value(self.h, a.ctypes.data, out.ctypes.data)
+        return out
 
     def env_sample(self, u0, u1):
         d = np.zeros(3, np.float32)

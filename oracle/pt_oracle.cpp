@@ -685,10 +685,56 @@ struct EnvMap : Material, Light {
         x = std::min(std::max((int)(u * W), 0), W - 1);
         y = std::min(std::max((int)(v * H), 0), H - 1);
     }
+    // angular light probe (env_kind 1): its own texels for lookups; the
+    // lat-long texels above then hold a proxy for the sampling tables only
+    bool probe = false;
+    int PW = 0, PH = 0;
+    std::vector<float> prgb;
+    // EnvironmentMap::get, src/environmentmap.cpp:35-66: (d.y, -d.z) scaled to
+    // length acos(d.x) / pi; pixelFromNormalizedCoord = ((c + 1) / 2) size - 1,
+    // truncated; row from the second coordinate, column from the first (the
+    // rim index -1 and the undefined axis directions clamped / centred)
+    int probe_index(V3 d) const
+    {
+        float icx = d.y, icy = -d.z;
+        float len = (float)std::sqrt((double)icx * icx + (double)icy * icy);
+        double ang = std::acos((double)std::max(-1.f, std::min(1.f, d.x)));
+        float s = len > 0 ? (float)(ang / ((double)len * M_PI)) : 0.f;
+        icx *= s;
+        icy *= s;
+        int x = (int)(float)(((icx + 1.0) / 2.0) * PW - 1);
+        int y = (int)(float)(((icy + 1.0) / 2.0) * PH - 1);
+        int row = std::min(y, PW - 1), col = std::min(x, PH - 1);
+        row = std::min(std::max(row, 0), PH - 1);
+        col = std::min(std::max(col, 0), PW - 1);
+        return row * PW + col;
+    }
+    void set_probe(const float* img, int w, int h)
+    {
+        probe = true;
+        PW = w;
+        PH = h;
+        prgb.assign(img, img + (size_t)3 * w * h);
+        W = 512;
+        H = 256;
+        rgb.assign((size_t)3 * W * H, 0.f);
+        for (int y = 0; y < H; ++y)
+            for (int x = 0; x < W; ++x) {
+                double th = (y + 0.5) / H * M_PI, ph = ((x + 0.5) / W - 0.5) * 2 * M_PI;
+                V3 d((float)(std::sin(th) * std::sin(ph)), (float)std::cos(th), (float)(-std::sin(th) * std::cos(ph)));
+                int t = probe_index(d);
+                for (int k = 0; k < 3; ++k) rgb[3 * ((size_t)y * W + x) + k] = prgb[3 * (size_t)t + k];
+            }
+    }
     Col get(V3 d) const
     {
+        d = normalized(d);
+        if (probe) {
+            const float* c = &prgb[3 * (size_t)probe_index(d)];
+            return mul(Col(c[0], c[1], c[2]), coef);
+        }
         int x, y;
-        texel_of(normalized(d), x, y);
+        texel_of(d, x, y);
         return mul(texel(x, y), coef);
     }
     void build_sampling()
@@ -1222,9 +1268,13 @@ void* oc_scene_create(const rt_scene_desc* d, int accel)
             root_items.push_back(cl);
         } else {
             auto* em = new EnvMap();
-            em->W = d->env_width;
-            em->H = d->env_height;
-            em->rgb.assign(d->env_rgb, d->env_rgb + (size_t)3 * em->W * em->H);
+            if (d->env_kind == 1) {
+                em->set_probe(d->env_rgb, d->env_width, d->env_height);
+            } else {
+                em->W = d->env_width;
+                em->H = d->env_height;
+                em->rgb.assign(d->env_rgb, d->env_rgb + (size_t)3 * em->W * em->H);
+            }
             em->coef = I;
             em->prim = i;
             em->build_sampling();
@@ -1393,6 +1443,13 @@ void oc_env_sample(void* s, float u0, float u1, float dir[3], float* pdf)
     dir[0] = d.x;
     dir[1] = d.y;
     dir[2] = d.z;
+}
+
+void oc_env_value(void* s, const float dir[3], float rgb[3])
+{
+    const Scene& sc = *static_cast<Scene*>(s);
+    Col c = sc.env ? sc.env->get(V3(dir[0], dir[1], dir[2])) : Col();
+    rgb[0] = c.r, rgb[1] = c.g, rgb[2] = c.b;
 }
 
 float oc_env_pdf(void* s, const float dir[3])

@@ -73,6 +73,8 @@ void oc_render_sample(void* scene, const rt_view* view, const rt_params* params,
  * uniforms (u0, u1); and pdf of a direction. */
 void oc_env_sample(void* scene, float u0, float u1, float dir[3], float* pdf);
 float oc_env_pdf(void* scene, const float dir[3]);
+/* Environment radiance towards a direction (EnvironmentMap::get). */
+void oc_env_value(void* scene, const float dir[3], float rgb[3]);
 
 /* Tonemapper::map per channel value (src/tonemapper.cpp:31-56): n floats to
  * n bytes. */

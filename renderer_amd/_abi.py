@@ -38,6 +38,7 @@ class RtSceneDesc(ctypes.Structure):
         ("env_width", ctypes.c_int32),
         ("env_height", ctypes.c_int32),
         ("env_rgb", ctypes.c_void_p),
+        ("env_kind", ctypes.c_int32),
     ]
 
 
@@ -146,6 +147,7 @@ PROTOTYPES = {
     "rt_film_write_mat": (ctypes.c_int, [ctypes.c_char_p, _P, _I32, _I32]),
     "rt_unique_filename": (_I64, [ctypes.c_char_p, _P, _I64]),
     "rt_film_last_error": (ctypes.c_char_p, []),
+    "rt_image_read_hdr": (ctypes.c_int, [ctypes.c_char_p, _P, _P, _P, _I64]),
 }
 
 _lib = None

@@ -101,9 +101,139 @@ void exr_attr(Out& o, const char* name, const char* type, const Out& value)
 
 }  // namespace
 
+/* Radiance RGBE (.hdr) reader, the format OpenCV's imread decodes for the
+ * reference's environment maps (src/environmentmap.cpp:20-33): a text header
+ * ending in an empty line (FORMAT=32-bit_rle_rgbe), the resolution line
+ * "-Y H +X W" (rows top to bottom), then H scanlines, each flat (4 bytes a
+ * pixel) or run-length encoded per channel (marker 2 2 W/256 W%256; a count
+ * above 128 repeats the next byte count-128 times, otherwise count literal
+ * bytes follow).  A pixel (r, g, b, e) is (r, g, b) * 2^(e - 136), 0 when
+ * e = 0. */
+bool rt_read_rgbe(const std::string& path, int& w, int& h, std::vector<float>& rgb, std::string& err)
+{
+    FILE* fp = std::fopen(path.c_str(), "rb");
+    if (!fp) {
+        err = "cannot open " + path;
+        return false;
+    }
+    std::vector<unsigned char> b;
+    unsigned char buf[1 << 16];
+    size_t r;
+    while ((r = std::fread(buf, 1, sizeof buf, fp)) > 0) b.insert(b.end(), buf, buf + r);
+    std::fclose(fp);
+    size_t i = 0;
+    auto line = [&](std::string& out) {
+        out.clear();
+        while (i < b.size() && b[i] != '\n') out += (char)b[i++];
+        if (i < b.size()) ++i;
+        return true;
+    };
+    std::string ln;
+    line(ln);
+    if (ln.rfind("#?", 0) != 0) {
+        err = path + ": not a Radiance file";
+        return false;
+    }
+    bool rgbe = false;
+    for (;;) {
+        if (i >= b.size()) {
+            err = path + ": truncated header";
+            return false;
+        }
+        line(ln);
+        if (ln.empty()) break;
+        if (ln == "FORMAT=32-bit_rle_rgbe") rgbe = true;
+        if (ln.rfind("FORMAT=", 0) == 0 && !rgbe) {
+            err = path + ": unsupported " + ln;
+            return false;
+        }
+    }
+    line(ln);
+    char ya[3] = {0}, xa[3] = {0};
+    if (std::sscanf(ln.c_str(), "%2s %d %2s %d", ya, &h, xa, &w) != 4 || std::string(ya) != "-Y" ||
+        std::string(xa) != "+X" || w <= 0 || h <= 0 || (int64_t)w * h > (1ll << 28)) {
+        err = path + ": unsupported resolution line \"" + ln + "\"";
+        return false;
+    }
+    rgb.assign((size_t)w * h * 3, 0.f);
+    std::vector<unsigned char> sl((size_t)w * 4);
+    for (int y = 0; y < h; ++y) {
+        if (i + 4 > b.size()) {
+            err = path + ": truncated pixels";
+            return false;
+        }
+        const bool rle = w >= 8 && w < 0x8000 && b[i] == 2 && b[i + 1] == 2 && !(b[i + 2] & 0x80);
+        if (!rle) {
+            if (i + sl.size() > b.size()) {
+                err = path + ": truncated pixels";
+                return false;
+            }
+            std::memcpy(sl.data(), &b[i], sl.size());
+            i += sl.size();
+        } else {
+            if (((int)b[i + 2] << 8 | b[i + 3]) != w) {
+                err = path + ": scanline width mismatch";
+                return false;
+            }
+            i += 4;
+            for (int ch = 0; ch < 4; ++ch) {
+                int x = 0;
+                while (x < w) {
+                    if (i >= b.size()) {
+                        err = path + ": truncated run";
+                        return false;
+                    }
+                    int cnt = b[i++];
+                    if (cnt > 128) {
+                        cnt -= 128;
+                        if (i >= b.size() || x + cnt > w) {
+                            err = path + ": bad run";
+                            return false;
+                        }
+                        unsigned char v = b[i++];
+                        for (int k = 0; k < cnt; ++k) sl[4 * (size_t)(x++) + ch] = v;
+                    } else {
+                        if (cnt == 0 || i + cnt > b.size() || x + cnt > w) {
+                            err = path + ": bad literal run";
+                            return false;
+                        }
+                        for (int k = 0; k < cnt; ++k) sl[4 * (size_t)(x++) + ch] = b[i++];
+                    }
+                }
+            }
+        }
+        for (int x = 0; x < w; ++x) {
+            const unsigned char* px = &sl[4 * (size_t)x];
+            if (px[3] == 0) continue;
+            float f = (float)std::ldexp(1.0, (int)px[3] - (128 + 8));
+            float* o = &rgb[3 * ((size_t)y * w + x)];
+            o[0] = px[0] * f;
+            o[1] = px[1] * f;
+            o[2] = px[2] * f;
+        }
+    }
+    return true;
+}
+
 extern "C" {
 
 const char* rt_film_last_error(void) { return g_film_error.c_str(); }
+
+int rt_image_read_hdr(const char* path, int32_t* width, int32_t* height, float* rgb, int64_t cap)
+{
+    if (!path || !width || !height) return fail(RT_ERR_INVALID, "invalid argument");
+    int w = 0, h = 0;
+    std::vector<float> img;
+    std::string err;
+    if (!rt_read_rgbe(path, w, h, img, err)) return fail(RT_ERR_IO, err);
+    *width = w;
+    *height = h;
+    if (rgb) {
+        if (cap < (int64_t)img.size()) return fail(RT_ERR_INVALID, "buffer too small");
+        std::memcpy(rgb, img.data(), img.size() * sizeof(float));
+    }
+    return RT_OK;
+}
 
 /* OpenEXR 2 single-part scanline file: magic 20000630, version 2, the eight
  * required header attributes, a table of scanline offsets, then one block per

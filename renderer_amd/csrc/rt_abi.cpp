@@ -41,7 +41,7 @@ struct rt_ctx {
     // scene
     bool has_scene = false;
     HostDevScene scene{};
-    DevBuf nodes, prims, prim_id, mat, vnormals, geo, mats, lights, env_rgb, env_row, env_col, env_p;
+    DevBuf nodes, prims, prim_id, mat, vnormals, geo, mats, lights, env_rgb, env_row, env_col, env_p, probe_rgb;
     int bvh_depth = 0;
     int bvh_stack = 0;  /* most stack entries a traversal can need */
 
@@ -215,7 +215,7 @@ int rt_destroy(rt_ctx* c)
     hipSetDevice(c->device);
     hipStreamSynchronize(c->stream);
     DevBuf* all[] = {&c->nodes, &c->prims, &c->prim_id, &c->mat, &c->vnormals, &c->geo, &c->mats, &c->lights,
-                     &c->env_rgb, &c->env_row, &c->env_col, &c->env_p, &c->rb_o, &c->rb_d, &c->rb_out0,
+                     &c->env_rgb, &c->env_row, &c->env_col, &c->env_p, &c->probe_rgb, &c->rb_o, &c->rb_d, &c->rb_out0,
                      &c->rb_out1, &c->stack_ovf, &c->tm_list, &c->tm_count, &c->counters, &c->views, &c->tiles, &c->passes, &c->films, &c->g_theta,
                      &c->g_w16, &c->g_m, &c->g_v, &c->g_grad, &c->g_acc, &c->g_count, &c->g_rec, &c->g_aux,
                      &c->g_open, &c->g_out, &c->g_tmp_rec, &c->g_tmp_y};
@@ -441,12 +441,38 @@ int rt_scene_upload(rt_ctx* c, const rt_scene_desc* d)
         M.rho_d = lum_rgb(m.kd);
         M.rho_s = lum_rgb(m.ks);
     }
-    // environment: texels x coefficient, sampling tables in double precision
-    std::vector<float> erow, ecol, ep, ergb;
-    int W = 0, H = 0;
+    // environment: texels x coefficient, sampling tables in double precision.
+    // A light probe keeps its own texels for lookups; its sampling tables come
+    // from a lat-long proxy (the probe looked up at each proxy texel centre),
+    // so the density stays exact whatever the proxy's resolution.
+    std::vector<float> erow, ecol, ep, ergb, prgb;
+    std::vector<float> proxy;
+    const float* esrc = d->env_rgb;
+    int W = 0, H = 0, PW = 0, PH = 0;
+    const bool probe = env >= 0 && d->env_kind == 1;
+    if (env >= 0 && d->env_kind != 0 && d->env_kind != 1) return set_err(c, RT_ERR_INVALID, "unknown env_kind");
+    if (probe) {
+        PW = d->env_width;
+        PH = d->env_height;
+        const float* coef = d->lights[env].intensity;
+        prgb.resize((size_t)4 * PW * PH);
+        for (size_t i = 0; i < (size_t)PW * PH; ++i)
+            for (int k = 0; k < 3; ++k) prgb[4 * i + k] = d->env_rgb[3 * i + k] * coef[k];
+        proxy.resize((size_t)3 * RT_PROBE_PROXY_W * RT_PROBE_PROXY_H);
+        for (int y = 0; y < RT_PROBE_PROXY_H; ++y)
+            for (int x = 0; x < RT_PROBE_PROXY_W; ++x) {
+                double th = (y + 0.5) / RT_PROBE_PROXY_H * M_PI, ph = ((x + 0.5) / RT_PROBE_PROXY_W - 0.5) * 2 * M_PI;
+                float dx = (float)(std::sin(th) * std::sin(ph)), dy = (float)std::cos(th);
+                float dz = (float)(-std::sin(th) * std::cos(ph));
+                int t = rt_probe_texel(dx, dy, dz, PW, PH);
+                for (int k = 0; k < 3; ++k)
+                    proxy[3 * ((size_t)y * RT_PROBE_PROXY_W + x) + k] = d->env_rgb[3 * (size_t)t + k];
+            }
+        esrc = proxy.data();
+    }
     if (env >= 0) {
-        W = d->env_width;
-        H = d->env_height;
+        W = probe ? RT_PROBE_PROXY_W : d->env_width;
+        H = probe ? RT_PROBE_PROXY_H : d->env_height;
         const float* coef = d->lights[env].intensity;
         ergb.resize((size_t)4 * W * H);
         std::vector<double> tp((size_t)W * H), roww(H, 0.0);
@@ -458,7 +484,7 @@ int rt_scene_upload(rt_ctx* c, const rt_scene_desc* d)
             for (int x = 0; x < W; ++x) {
                 float rgb[3];
                 for (int k = 0; k < 3; ++k) {
-                    rgb[k] = d->env_rgb[3 * ((size_t)y * W + x) + k] * coef[k];
+                    rgb[k] = esrc[3 * ((size_t)y * W + x) + k] * coef[k];
                     ergb[4 * ((size_t)y * W + x) + k] = rgb[k];
                 }
                 double w = (double)lum_rgb(rgb) * st;
@@ -496,7 +522,8 @@ int rt_scene_upload(rt_ctx* c, const rt_scene_desc* d)
         (r = upload(c, c->lights, lights.data(), lights.size() * sizeof(RtLight))) ||
         (r = upload(c, c->env_rgb, ergb.data(), ergb.size() * 4)) ||
         (r = upload(c, c->env_row, erow.data(), erow.size() * 4)) ||
-        (r = upload(c, c->env_col, ecol.data(), ecol.size() * 4)) || (r = upload(c, c->env_p, ep.data(), ep.size() * 4)))
+        (r = upload(c, c->env_col, ecol.data(), ecol.size() * 4)) || (r = upload(c, c->env_p, ep.data(), ep.size() * 4)) ||
+        (r = upload(c, c->probe_rgb, prgb.data(), prgb.size() * 4)))
         return r;
     HostDevScene& s = c->scene;
     s.nodes = c->nodes.p;
@@ -519,6 +546,10 @@ int rt_scene_upload(rt_ctx* c, const rt_scene_desc* d)
     s.env_row_cdf = (const float*)c->env_row.p;
     s.env_col_cdf = (const float*)c->env_col.p;
     s.env_texel_p = (const float*)c->env_p.p;
+    s.env_kind = probe ? 1 : 0;
+    s.probe_w = PW;
+    s.probe_h = PH;
+    s.probe_rgb = c->probe_rgb.p;
     c->has_scene = true;
     c->bvh_depth = br.depth;
     c->bvh_stack = br.stack_need;

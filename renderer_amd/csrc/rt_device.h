@@ -101,6 +101,9 @@ struct DevScene {
     const float* env_col_cdf;  /* env_h * (env_w + 1) */
     const float* env_texel_p;  /* env_w * env_h */
     int32_t* stack_ovf;        /* traversal stack overflow: RT_STACK_OVF entries per lane of the launch */
+    int32_t env_kind;          /* 0: lat-long texels; 1: angular probe (values from probe_rgb) */
+    int32_t probe_w, probe_h;
+    const float4* probe_rgb;   /* probe texels (rgb * coefficient) */
 };
 
 /* ---------------------------------------------------------------------------

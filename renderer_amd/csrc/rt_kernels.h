@@ -30,6 +30,9 @@ struct HostDevScene {
     const float* env_col_cdf;
     const float* env_texel_p;
     int32_t* stack_ovf;
+    int32_t env_kind;
+    int32_t probe_w, probe_h;
+    const void* probe_rgb;
 };
 
 /* One view of a render: camera basis computed on the host with the

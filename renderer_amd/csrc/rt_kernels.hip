@@ -226,7 +226,9 @@ __device__ __forceinline__ int env_texel(const DevScene& S, vec3 d)
 }
 __device__ __forceinline__ vec3 env_value(const DevScene& S, vec3 d)
 {
-    return xyz(S.env_rgb[env_texel(S, norm(d))]);
+    d = norm(d);
+    if (S.env_kind == 1) return xyz(S.probe_rgb[rt_probe_texel(d.x, d.y, d.z, S.probe_w, S.probe_h)]);
+    return xyz(S.env_rgb[env_texel(S, d)]);
 }
 __device__ __forceinline__ int cdf_pick(const float* cdf, int n, float u, float& frac)
 {

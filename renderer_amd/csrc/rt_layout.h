@@ -28,7 +28,40 @@
 #ifndef RT_LAYOUT_H
 #define RT_LAYOUT_H
 
+#include <math.h>
 #include <stdint.h>
+
+#if defined(__HIPCC__)
+#define RT_HD __host__ __device__
+#else
+#define RT_HD
+#endif
+
+/* Angular light-probe lookup of EnvironmentMap::get
+ * (src/environmentmap.cpp:57-66) for a unit direction d: the image point is
+ * (d.y, -d.z) scaled to length acos(d.x) / pi, mapped to pixel
+ * ((c + 1) / 2) * size - 1 and truncated; row from the second coordinate,
+ * column from the first.  Returns row * W + col.  Indices the reference would
+ * read outside the image (-1 on the rim) are clamped; the two directions
+ * along the probe axis (where the scale is undefined) map to the centre. */
+static inline RT_HD int rt_probe_texel(float dx, float dy, float dz, int W, int H)
+{
+    float icx = dy, icy = -dz;
+    float len = (float)sqrt((double)icx * icx + (double)icy * icy);
+    double ax = acos((double)fminf(1.f, fmaxf(-1.f, dx)));
+    float scale = len > 0.f ? (float)(ax / ((double)len * M_PI)) : 0.f;
+    icx *= scale;
+    icy *= scale;
+    int x = (int)(float)(((icx + 1.0) / 2.0) * W - 1);
+    int y = (int)(float)(((icy + 1.0) / 2.0) * H - 1);
+    int row = y < W - 1 ? y : W - 1, col = x < H - 1 ? x : H - 1;
+    row = row < 0 ? 0 : row > H - 1 ? H - 1 : row;
+    col = col < 0 ? 0 : col > W - 1 ? W - 1 : col;
+    return row * W + col;
+}
+/* lat-long proxy of a probe, used only for the importance-sampling tables */
+#define RT_PROBE_PROXY_W 512
+#define RT_PROBE_PROXY_H 256
 
 #define RT_KIND_TRI 0
 #define RT_KIND_SPHERE 1

@@ -39,6 +39,9 @@
 #include "rt_abi.h"
 #include "rt_scene_file.h"
 
+/* film_io.cpp: Radiance RGBE reader */
+bool rt_read_rgbe(const std::string& path, int& w, int& h, std::vector<float>& rgb, std::string& err);
+
 namespace {
 
 thread_local std::string g_last_error;
@@ -1841,6 +1844,8 @@ struct rt_scene_file {
     std::vector<int32_t> tri_vertices, tri_material, tri_prim, sph_material, sph_prim;
     std::vector<rt_material> materials;
     std::vector<rt_light> lights;
+    std::vector<float> env_rgb;  /* light probe of an environmentmap(...) light */
+    int env_w = 0, env_h = 0;
 };
 
 namespace {
@@ -1934,6 +1939,8 @@ int rt_scene_file_scene(rt_scene_file* f, rt_scene_desc* desc, rt_camera* camera
 
     // lights first (src/sceneast.cpp:676-678)
     f->lights.clear();
+    f->env_rgb.clear();
+    f->env_w = f->env_h = 0;
     auto li = ev.lights.find("lights");
     if (li != ev.lights.end()) {
         for (const LightExpr& L : li->second) {
@@ -1959,15 +1966,27 @@ int rt_scene_file_scene(rt_scene_file* f, rt_scene_desc* desc, rt_camera* camera
                 c.v[0] = L.angle;
                 c.intensity[0] = L.I.x, c.intensity[1] = L.I.y, c.intensity[2] = L.I.z;
                 break;
-            case LightKind::Env:
-                if (readable(L.file))
-                    return fail(RT_ERR_UNSUPPORTED, "environmentmap(\"" + L.file +
-                                                        "\"): light-probe files are not rendered by the GPU path; "
-                                                        "pass a latitude-longitude image in rt_scene_desc");
-                // EnvironmentMap::load failed -> BlackLight (src/sceneast.cpp:574-584): no emission
-                f->messages += "Could not load environment map " + L.file + "\n";
-                c.type = RT_LIGHT_POINT;
+            case LightKind::Env: {
+                // EnvironmentMap::load (src/environmentmap.cpp:20-33): an angular
+                // light probe, the path taken as given (relative to the working
+                // directory, as imread takes it); failure -> BlackLight
+                // (src/sceneast.cpp:574-584): no emission
+                std::string err;
+                std::vector<float> img;
+                int w = 0, h = 0;
+                if (!f->env_rgb.empty() || !rt_read_rgbe(L.file, w, h, img, err)) {
+                    f->messages += "Failed to load environment map \"" + L.file + "\"" +
+                                   (err.empty() ? std::string(" (one environment map per scene)") : ": " + err) + "\n";
+                    c.type = RT_LIGHT_POINT;
+                    break;
+                }
+                f->env_rgb.swap(img);
+                f->env_w = w;
+                f->env_h = h;
+                c.type = RT_LIGHT_ENV;
+                c.intensity[0] = L.I.x, c.intensity[1] = L.I.y, c.intensity[2] = L.I.z;
                 break;
+            }
             }
             f->lights.push_back(c);
         }
@@ -2024,6 +2043,12 @@ int rt_scene_file_scene(rt_scene_file* f, rt_scene_desc* desc, rt_camera* camera
     desc->materials = f->materials.empty() ? nullptr : f->materials.data();
     desc->n_lights = nL;
     desc->lights = f->lights.empty() ? nullptr : f->lights.data();
+    if (!f->env_rgb.empty()) {
+        desc->env_width = f->env_w;
+        desc->env_height = f->env_h;
+        desc->env_rgb = f->env_rgb.data();
+        desc->env_kind = 1;
+    }
 
     const CamExpr& c = cam->second;
     std::memset(camera, 0, sizeof *camera);

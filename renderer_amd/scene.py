@@ -113,15 +113,32 @@ class ConeLight(Light):
 
 
 class EnvironmentMap(Light):
-    """Environment light: a latitude-longitude RGB float image (H, W, 3), row 0
-    at +y, times a coefficient (src/environmentmap.h holds an image and a
-    coefficient; its image there is an angular light probe)."""
+    """Environment light: an RGB float image (H, W, 3) times a coefficient
+    (src/environmentmap.h).  mapping "latlong": latitude-longitude, row 0 at
+    +y; "probe": the reference's angular light probe
+    (src/environmentmap.cpp:57-66).  EnvironmentMap.load reads a probe from a
+    Radiance .hdr file, as the reference does."""
 
-    def __init__(self, image, coefficient=(1.0, 1.0, 1.0)):
+    def __init__(self, image, coefficient=(1.0, 1.0, 1.0), mapping="latlong"):
         img = np.ascontiguousarray(image, dtype=np.float32)
         if img.ndim != 3 or img.shape[2] != 3:
             raise ValueError("environment image must be (H, W, 3)")
-        self.image, self.coefficient = img, _v3(coefficient)
+        if mapping not in ("latlong", "probe"):
+            raise ValueError("mapping must be 'latlong' or 'probe'")
+        self.image, self.coefficient, self.mapping = img, _v3(coefficient), mapping
+
+    @classmethod
+    def load(cls, path, coefficient=(1.0, 1.0, 1.0)):
+        lib = _abi.load()
+        w, h = ctypes.c_int32(), ctypes.c_int32()
+        st = lib.rt_image_read_hdr(str(path).encode(), ctypes.byref(w), ctypes.byref(h), None, 0)
+        if st != 0:
+            raise _abi.RtError(st, lib.rt_film_last_error().decode(errors="replace"))
+        img = np.zeros((h.value, w.value, 3), np.float32)
+        st = lib.rt_image_read_hdr(str(path).encode(), ctypes.byref(w), ctypes.byref(h), img.ctypes.data, img.size)
+        if st != 0:
+            raise _abi.RtError(st, lib.rt_film_last_error().decode(errors="replace"))
+        return cls(img, coefficient, mapping="probe")
 
 
 # ---------------------------------------------------------------------------
@@ -194,11 +211,12 @@ class SceneArrays:
     """Flattened scene: numpy arrays + ctypes descriptor (keeps arrays alive)."""
 
     def __init__(self, positions, normals, tri_vertices, tri_material, tri_prim, spheres, sphere_material,
-                 sphere_prim, materials, lights, env):
+                 sphere_prim, materials, lights, env, env_kind=0):
         self.positions, self.normals = positions, normals
         self.tri_vertices, self.tri_material, self.tri_prim = tri_vertices, tri_material, tri_prim
         self.spheres, self.sphere_material, self.sphere_prim = spheres, sphere_material, sphere_prim
         self.materials, self.lights, self.env = materials, lights, env
+        self.env_kind = env_kind
         self._desc = None
 
     @property
@@ -234,6 +252,7 @@ class SceneArrays:
         if self.env is not None:
             d.env_height, d.env_width = self.env.shape[0], self.env.shape[1]
             d.env_rgb = self.env.ctypes.data
+            d.env_kind = self.env_kind
         self._desc = d
         return d
 
@@ -265,7 +284,7 @@ class Scene:
                 mats.append(c)
             return mat_index[id(m)]
 
-        lights, env = [], None
+        lights, env, env_kind = [], None, 0
         for L in self.light:
             c = _abi.RtLight()
             if isinstance(L, AreaLight):
@@ -283,6 +302,7 @@ class Scene:
                     raise ValueError("at most one environment map")
                 c.type = _abi.LIGHT_ENV
                 env = L.image
+                env_kind = 1 if L.mapping == "probe" else 0
             else:
                 raise TypeError(type(L))
             c.intensity[:] = (L.coefficient if isinstance(L, EnvironmentMap) else L.intensity).tolist()
@@ -347,7 +367,7 @@ class Scene:
         tri_prim = np.ascontiguousarray(np.concatenate(tp) if tp else np.zeros(0, np.int32), np.int32)
         spheres = np.ascontiguousarray(np.stack(sph) if sph else np.zeros((0, 4), np.float32), np.float32)
         return SceneArrays(positions, normals, tri_vertices, tri_material, tri_prim, spheres,
-                           np.asarray(sm, np.int32), np.asarray(sp, np.int32), mats, lights, env)
+                           np.asarray(sm, np.int32), np.asarray(sp, np.int32), mats, lights, env, env_kind)
 
 
 def _flat_vertex_normals(P, F):

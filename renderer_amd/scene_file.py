@@ -51,7 +51,10 @@ class FileSceneArrays:
         self.materials = [mats[i] for i in range(d.n_materials)] if mats else []
         lights = ctypes.cast(d.lights, ctypes.POINTER(_abi.RtLight)) if d.lights else None
         self.lights = [lights[i] for i in range(d.n_lights)] if lights else []
-        self.env = None
+        self.env = (np.ctypeslib.as_array(ctypes.cast(d.env_rgb, ctypes.POINTER(ctypes.c_float)),
+                                          shape=(d.env_height * d.env_width * 3,)).reshape(d.env_height, d.env_width, 3)
+                    if d.env_rgb else None)
+        self.env_kind = d.env_kind
         self.camera = Camera(list(camera.eye), list(camera.look_at), list(camera.up), camera.fov, camera.width,
                              camera.height)
 

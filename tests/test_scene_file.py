@@ -239,7 +239,7 @@ def test_missing_environment_map_is_a_black_light():
                          'lights = lights{environmentmap("no/such/probe.hdr", [1 1 1]) pointlight([0 1 0],[1 1 1])};\n'
                          "intersectable = intersectables{ sphere([0 0 0], 1, diffuse([1 1 1])) };")
     a = sf.arrays()
-    assert "Could not load environment map" in sf.messages
+    assert "Failed to load environment map" in sf.messages
     assert [light_tuple(L) for L in a.lights][0] == (_abi.LIGHT_POINT, (0, 0, 0), (0, 0, 0), (0, 0, 0), (0, 0, 0))
     assert list(a.sphere_prim) == [2]
 
