@@ -863,12 +863,25 @@ struct Path {  // src/path.h:31-36
 
 struct SampleKey { uint32_t seed, view, pixel, sample; };
 
+// Where a path's random numbers come from: the Philox streams of one pixel
+// sample (the path-tracing renderer), or a primary-sample vector of
+// 2 * max_depth + 1 groups (MetropolisSample, src/metropolissample.h:47-52:
+// group 0 camera sample + length uniform, 1 + 2i cameraPathSamples[i],
+// 2 + 2k lightSample1[k] + lightIndex uniform).
+struct SampleSource {
+    SampleKey k{};
+    const U4* pss = nullptr;
+    SampleSource(const SampleKey& key) : k(key) {}
+    explicit SampleSource(const U4* v) : pss(v) {}
+    U4 get(int g) const { return pss ? pss[g] : philox_uniforms(k.seed, k.view, k.pixel, k.sample, (uint32_t)g); }
+};
+
 // Renderer::createPath, src/renderer.cpp:47-106, with MAX_DEPTH = max_depth
 // and the negative-binomial length draw made from one uniform by inversion.
-Path create_path(const Scene& sc, const Ray& primary, const SampleKey& k, int max_depth, int rr_start,
+Path create_path(const Scene& sc, const Ray& primary, const SampleSource& k, int max_depth, int rr_start,
                  float p, int64_t& rays)
 {
-    U4 g0 = philox_uniforms(k.seed, k.view, k.pixel, k.sample, 0);
+    U4 g0 = k.get(0);
     int L;
     if (p > 0) {
         int nb = 0;
@@ -888,7 +901,7 @@ Path create_path(const Scene& sc, const Ray& primary, const SampleKey& k, int ma
     Hit hit = sc.intersect(primary);
     for (int i = 0; i < L; ++i) {
         if (!hit.hits()) return res;
-        U4 gs = philox_uniforms(k.seed, k.view, k.pixel, k.sample, 1 + 2 * i);
+        U4 gs = k.get(1 + 2 * i);
         Sample s{gs.u[0], gs.u[1]};
         float pdf = 0;
         Col brdf;
@@ -917,7 +930,7 @@ inline int signum(float x) { return (0.f < x) - (x < 0.f); }
 
 // UniDiPathTracingIntegrator::integrate(Ray) + integrate(Path),
 // src/unidipathtracingintegrator.cpp:34-107
-Col integrate_compat(const Scene& sc, const Ray& ray, const SampleKey& k, int max_depth, int rr_start, float p,
+Col integrate_compat(const Scene& sc, const Ray& ray, const SampleSource& k, int max_depth, int rr_start, float p,
                      int64_t& rays)
 {
     Path path = create_path(sc, ray, k, max_depth, rr_start, p, rays);
@@ -939,7 +952,7 @@ Col integrate_compat(const Scene& sc, const Ray& ray, const SampleKey& k, int ma
             else
                 direction = hit->p - next->p;
         } else if (nl > 0) {
-            U4 gl = philox_uniforms(k.seed, k.view, k.pixel, k.sample, 2 + 2 * i);
+            U4 gl = k.get(2 + 2 * i);
             size_t idx = std::min((size_t)(gl.u[2] * (float)nl), nl - 1);
             li = sc.lights[idx]->intensity(*hit, direction, sc, Sample{gl.u[0], gl.u[1]}, rays);
         }
@@ -1207,6 +1220,287 @@ Col integrate_unbiased(const UnbiasedCtx& c, Ray ray, const SampleKey& k, int64_
 }
 
 }  // namespace
+
+// ---------------------------------------------------------------------------
+// Markov-chain renderers over the sample-driven integrator
+// (src/metropolissample.cpp, src/metropolisrenderer.cpp:56-200,
+// src/energyredistributionrenderer.cpp:38-131).  Every random number is a
+// Philox draw addressed by (seed, stream, index, step, group), the streams
+// of include/rt_mlt.h, so chains here and on the GPU take the same steps.
+// ---------------------------------------------------------------------------
+struct Pss {
+    std::vector<U4> g;
+    explicit Pss(int groups = 0) : g(groups) {}
+};
+
+inline U4 draw(uint32_t seed, uint32_t stream, uint32_t index, uint32_t step, uint32_t group)
+{
+    return philox_uniforms(seed, stream, index, step, group);
+}
+
+// mutate(qreal&), src/metropolissample.cpp:62-78: a step of length
+// b * exp(-log(b / a) * u) between a = 1/1024 and b = 1/64, negative when
+// v < 0.5, wrapped into [0, 1].  The coordinate is a float here.
+float mlt_perturb(float s, float u, float v)
+{
+    const float a = 1.f / 1024.f, b = 1.f / 64.f;
+    const float log_ratio = -std::log(b / a);
+    float delta = (float)(b * std::exp((double)log_ratio * u));
+    if (v < 0.5f) delta = -delta;
+    s += delta;
+    if (s < 0) s += 1;
+    if (s > 1) s -= 1;
+    return s;
+}
+
+// Path through a primary-sample vector: the camera sample scaled to pixels
+// (MetropolisSample::cameraPathFromSample, src/metropolissample.cpp:120-127),
+// or given in pixels (the e_d estimate, src/energyredistributionrenderer.cpp:121-125).
+Col integrate_pss(const Scene& sc, const Camera& cam, const Pss& x, int max_depth, int rr_start, float p,
+                  bool pixel_units, int64_t& rays)
+{
+    const U4& g0 = x.g[0];
+    double px = pixel_units ? (double)g0.u[0] : (double)g0.u[0] * cam.W;
+    double py = pixel_units ? (double)g0.u[1] : (double)g0.u[1] * cam.H;
+    return integrate_compat(sc, cam.ray(px, py), SampleSource(x.g.data()), max_depth, rr_start, p, rays);
+}
+
+// MetropolisSample::getPos, src/metropolissample.cpp:129-136
+inline int mlt_pos(const Pss& x, int W, int H)
+{
+    int px = std::max(0, std::min((int)((double)x.g[0].u[0] * W), W - 1));
+    int py = std::max(0, std::min((int)((double)x.g[0].u[1] * H), H - 1));
+    return py * W + px;
+}
+
+// cv::norm of a Vec3f: the L2 norm in double
+inline double norm2(Col c) { return std::sqrt((double)c.r * c.r + (double)c.g * c.g + (double)c.b * c.b); }
+
+// largeStep (src/metropolissample.cpp:45-59) as fresh draws; smallStep
+// (:86-97) perturbs x and y of every group and keeps z and w (the light
+// index is only redrawn by a large step).
+void mlt_propose(const Pss& cur, Pss& out, uint32_t seed, uint32_t chain, uint32_t step, bool large)
+{
+    for (size_t g = 0; g < cur.g.size(); ++g) {
+        if (large) {
+            out.g[g] = draw(seed, RT_MLT_STREAM_LARGE, chain, step, (uint32_t)g);
+        } else {
+            U4 m = draw(seed, RT_MLT_STREAM_SMALL, chain, step, (uint32_t)g);
+            out.g[g] = cur.g[g];
+            out.g[g].u[0] = mlt_perturb(cur.g[g].u[0], m.u[0], m.u[1]);
+            out.g[g].u[1] = mlt_perturb(cur.g[g].u[1], m.u[2], m.u[3]);
+        }
+    }
+}
+
+// ERPT initial sample i = pixel * spp + s: the camera sample is the pixel
+// plus the jitter of stratum (s / spp_y, s % spp_y) (JitteredSampler,
+// src/jitteredsampler.cpp:23-35) over the image size
+// (src/energyredistributionrenderer.cpp:66-75), the rest a large step.
+void erpt_seed_sample(Pss& x, uint32_t seed, int64_t i, int W, int H, int sx, int sy)
+{
+    const int G = (int)x.g.size(), spp = sx * sy;
+    for (int g = 0; g < G; ++g) x.g[g] = draw(seed, RT_MLT_STREAM_SEED, (uint32_t)i, 0, (uint32_t)g);
+    const int64_t pix = i / spp;
+    const int s = (int)(i % spp), a = s / sy, b = s % sy;
+    double jx = ((double)x.g[0].u[0] + a) / sx, jy = ((double)x.g[0].u[1] + b) / sy;
+    x.g[0].u[0] = (float)(((double)(pix % W) + jx) / W);
+    x.g[0].u[1] = (float)(((double)(pix / W) + jy) / H);
+}
+
+struct MltRun {
+    const Scene* sc;
+    Camera cam;
+    rt_mlt_params p;
+    int G;
+    std::vector<double> film;  // W * H * 3
+    rt_mlt_result res{};
+    int threads;
+    MltRun(const Scene* s, const rt_camera& c, const rt_mlt_params& prm, int thr)
+        : sc(s), cam(c), p(prm), G(2 * prm.max_depth + 1), film((size_t)c.width * c.height * 3, 0.0), threads(thr)
+    {
+    }
+};
+
+int64_t mlt_default_chains(int64_t total)
+{
+    return std::max<int64_t>(1, std::min<int64_t>(total / 16, (int64_t)1 << 22));
+}
+
+// MetropolisRenderer::render with the chains of the GPU: a bootstrap of
+// max(bootstrap, chains) large-step paths at full length (the reference's
+// bootstrap samples keep MetropolisSample's default pathLength = MAX_DEPTH,
+// src/metropolissample.h:35), b = their mean luminance, each chain started
+// from a bootstrap path drawn in proportion to luminance, then `iters`
+// mutations per chain with both deposits of src/metropolisrenderer.cpp:160-175.
+void run_metropolis(MltRun& R)
+{
+    const Scene& sc = *R.sc;
+    const int W = R.cam.W, H = R.cam.H, G = R.G;
+    const int64_t area = (int64_t)W * H;
+    const int64_t total = (int64_t)R.p.mutations * area;
+    const int64_t C = R.p.chains > 0 ? R.p.chains : mlt_default_chains(total);
+    const int64_t iters = std::max<int64_t>(1, (total + C / 2) / C);
+    const int64_t nb = std::max<int64_t>(R.p.bootstrap, C);
+    const uint32_t seed = R.p.seed;
+    std::vector<Col> bval(nb);
+    std::vector<double> blum(nb);
+    int64_t rays = 0;
+#pragma omp parallel for schedule(dynamic, 64) num_threads(R.threads) reduction(+ : rays)
+    for (int64_t i = 0; i < nb; ++i) {
+        Pss x(G);
+        for (int g = 0; g < G; ++g) x.g[g] = draw(seed, RT_MLT_STREAM_BOOTSTRAP, (uint32_t)i, 0, (uint32_t)g);
+        bval[i] = integrate_pss(sc, R.cam, x, R.p.max_depth, R.p.rr_start, 0.f, false, rays);
+        blum[i] = lum(bval[i]);
+    }
+    std::vector<double> cdf(nb);
+    double acc = 0;
+    for (int64_t i = 0; i < nb; ++i) cdf[i] = acc += blum[i];
+    const float b = (float)(acc / (double)nb);
+    R.res.b = b;
+    R.res.chains = C;
+    R.res.seed_paths = nb;
+    const float scale = (float)((double)b * (double)area / ((double)C * (double)iters));
+    const float lp = R.p.large_step_prob;
+    int64_t accepted = 0;
+    std::vector<double>& film = R.film;
+#pragma omp parallel for schedule(dynamic, 16) num_threads(R.threads) reduction(+ : rays, accepted)
+    for (int64_t c = 0; c < C; ++c) {
+        double target = (double)draw(seed, RT_MLT_STREAM_START, (uint32_t)c, 0, 0).u[0] * acc;
+        int64_t idx = std::upper_bound(cdf.begin(), cdf.end(), target) - cdf.begin();
+        idx = std::min(idx, nb - 1);
+        Pss cur(G), prop(G);
+        for (int g = 0; g < G; ++g) cur.g[g] = draw(seed, RT_MLT_STREAM_BOOTSTRAP, (uint32_t)idx, 0, (uint32_t)g);
+        Col cv = bval[idx];
+        float lc = lum(cv);
+        std::vector<std::pair<int, Col>> dep;
+        dep.reserve(2 * iters);
+        for (int64_t m = 0; m < iters; ++m) {
+            U4 ctl = draw(seed, RT_MLT_STREAM_CONTROL, (uint32_t)c, (uint32_t)m, 0);
+            mlt_propose(cur, prop, seed, (uint32_t)c, (uint32_t)m, ctl.u[0] < lp);
+            prop.g[0].u[2] = ctl.u[1];  // mutatePathLength: a fresh length every mutation
+            Col nv = integrate_pss(sc, R.cam, prop, R.p.max_depth, R.p.rr_start, R.p.termination_prob, false, rays);
+            float ln = lum(nv);
+            float a = lc > 0 ? std::min(1.f, ln / lc) : 1.f;
+            if (lc > 0) dep.emplace_back(mlt_pos(cur, W, H), (cv * ((1.f / lc) * scale)) * (1 - a));
+            if (ln > 0) dep.emplace_back(mlt_pos(prop, W, H), (nv * ((1.f / ln) * scale)) * a);
+            if (ctl.u[2] < a) {
+                std::swap(cur, prop);
+                cv = nv;
+                lc = ln;
+                ++accepted;
+            }
+        }
+#pragma omp critical
+        for (auto& d : dep) {
+            film[3 * (size_t)d.first] += d.second.r;
+            film[3 * (size_t)d.first + 1] += d.second.g;
+            film[3 * (size_t)d.first + 2] += d.second.b;
+        }
+    }
+    R.res.mutations = C * iters;
+    R.res.accepted = accepted;
+    R.res.rays = rays;
+}
+
+// EnergyRedistributionRenderer::render.  e_d from one path per pixel centre
+// with termination probability erpt_ed_termination (computeEd,
+// src/energyredistributionrenderer.cpp:115-131); erpt_spp_x * erpt_spp_y
+// jittered initial samples per pixel at full path length; floor(u +
+// |c| / (M e_d)) chains each, which deposit e_d / chains per mutation
+// (src/energyredistributionrenderer.cpp:84-112, with the reference's order:
+// the deposit goes to the pixel of the state after the acceptance test and
+// carries the colour of the state before it).
+void run_erpt(MltRun& R)
+{
+    const Scene& sc = *R.sc;
+    const int W = R.cam.W, H = R.cam.H, G = R.G;
+    const int64_t area = (int64_t)W * H;
+    const int M = R.p.mutations, sx = R.p.erpt_spp_x, sy = R.p.erpt_spp_y, spp = sx * sy;
+    const uint32_t seed = R.p.seed;
+    int64_t rays = 0;
+    std::vector<Col> edv(area);
+#pragma omp parallel for schedule(dynamic, 64) num_threads(R.threads) reduction(+ : rays)
+    for (int64_t i = 0; i < area; ++i) {
+        Pss x(G);
+        for (int g = 0; g < G; ++g) x.g[g] = draw(seed, RT_MLT_STREAM_ED, (uint32_t)i, 0, (uint32_t)g);
+        x.g[0].u[0] = (float)(i % W);
+        x.g[0].u[1] = (float)(i / W);
+        edv[i] = integrate_pss(sc, R.cam, x, R.p.max_depth, R.p.rr_start, R.p.erpt_ed_termination, true, rays);
+    }
+    double er = 0, eg = 0, eb = 0;
+    for (int64_t i = 0; i < area; ++i) er += edv[i].r, eg += edv[i].g, eb += edv[i].b;
+    const float ed = (float)(std::sqrt(er * er + eg * eg + eb * eb) / ((double)area * M));
+    R.res.ed = ed;
+    const int64_t n0 = area * spp;
+    std::vector<Col> cval(n0);
+    std::vector<int32_t> nch(n0);
+#pragma omp parallel for schedule(dynamic, 64) num_threads(R.threads) reduction(+ : rays)
+    for (int64_t i = 0; i < n0; ++i) {
+        Pss x(G);
+        erpt_seed_sample(x, seed, i, W, H, sx, sy);
+        cval[i] = integrate_pss(sc, R.cam, x, R.p.max_depth, R.p.rr_start, 0.f, false, rays);
+        double nr = norm2(cval[i]);
+        nch[i] = ed > 0 ? (int32_t)((double)draw(seed, RT_MLT_STREAM_COUNT, (uint32_t)i, 0, 0).u[0] +
+                                    nr / (double)(M * ed))
+                        : 0;
+    }
+    std::vector<int64_t> first(n0 + 1, 0);
+    for (int64_t i = 0; i < n0; ++i) first[i + 1] = first[i] + nch[i];
+    const int64_t C = first[n0];
+    R.res.chains = C;
+    for (int64_t i = 0; i < n0; ++i) R.res.chain_seeds += nch[i] > 0;
+    R.res.seed_paths = area + n0;
+    const float inv_spp = 1.f / (float)spp;
+    int64_t accepted = 0;
+    std::vector<double>& film = R.film;
+#pragma omp parallel for schedule(dynamic, 16) num_threads(R.threads) reduction(+ : rays, accepted)
+    for (int64_t i = 0; i < n0; ++i) {
+        if (nch[i] == 0) continue;
+        Pss x0(G);
+        erpt_seed_sample(x0, seed, i, W, H, sx, sy);
+        const Col c0 = cval[i];
+        const double n0r = norm2(c0);
+        const double k = (double)ed / n0r / nch[i];
+        const Col dv((float)(c0.r * k), (float)(c0.g * k), (float)(c0.b * k));
+        const double E = norm2(dv);
+        std::vector<std::pair<int, Col>> dep;
+        dep.reserve((size_t)M * nch[i]);
+        for (int64_t ch = first[i]; ch < first[i + 1]; ++ch) {
+            Pss y = x0, z(G);
+            Col yv = c0;
+            float yl = (float)n0r;
+            for (int m = 0; m < M; ++m) {
+                mlt_propose(y, z, seed, (uint32_t)ch, (uint32_t)m, false);
+                Col zv = integrate_pss(sc, R.cam, z, R.p.max_depth, R.p.rr_start, 0.f, false, rays);
+                float zl = (float)norm2(zv);
+                float q = std::min(1.f, zl / yl);
+                const Col before = yv;
+                const double inv_before = 1.0 / (double)yl;
+                if (q > draw(seed, RT_MLT_STREAM_CONTROL, (uint32_t)ch, (uint32_t)m, 0).u[2]) {
+                    std::swap(y, z);
+                    yv = zv;
+                    yl = zl;
+                    ++accepted;
+                }
+                // norm(depVal) * yVal * (1 / norm(yVal)) * (1.f / spp), each product rounded to float
+                Col d((float)(E * before.r), (float)(E * before.g), (float)(E * before.b));
+                d = Col((float)(d.r * inv_before), (float)(d.g * inv_before), (float)(d.b * inv_before));
+                d = d * inv_spp;
+                dep.emplace_back(mlt_pos(y, W, H), d);
+            }
+        }
+#pragma omp critical
+        for (auto& d : dep) {
+            film[3 * (size_t)d.first] += d.second.r;
+            film[3 * (size_t)d.first + 1] += d.second.g;
+            film[3 * (size_t)d.first + 2] += d.second.b;
+        }
+    }
+    R.res.mutations = C * M;
+    R.res.accepted = accepted;
+    R.res.rays = rays;
+}
 
 // ---------------------------------------------------------------------------
 // C API
@@ -1500,6 +1794,44 @@ static int tonemap_clamp(float x)
 void oc_tonemap(const float* film, int64_t n, float gamma, uint8_t* out)
 {
     for (int64_t i = 0; i < n; ++i) out[i] = (uint8_t)(tonemap_clamp(255 * std::pow(film[i], 1. / gamma)) & 0xff);
+}
+
+void oc_mlt_perturb(const float* s, const float* u, const float* v, int64_t n, float* out)
+{
+    for (int64_t i = 0; i < n; ++i) out[i] = mlt_perturb(s[i], u[i], v[i]);
+}
+
+void oc_integrate_samples(void* s, const rt_camera* camera, int max_depth, int rr_start, float p,
+                          const float* samples, int64_t n, float* rgb, int threads)
+{
+    const Scene& sc = *static_cast<Scene*>(s);
+    Camera cam(*camera);
+    const int G = 2 * max_depth + 1;
+#pragma omp parallel for schedule(dynamic, 64) num_threads(threads > 0 ? threads : 1)
+    for (int64_t i = 0; i < n; ++i) {
+        Pss x(G);
+        for (int g = 0; g < G; ++g)
+            for (int k = 0; k < 4; ++k) x.g[g].u[k] = samples[((size_t)i * G + g) * 4 + k];
+        int64_t rays = 0;
+        Col c = integrate_pss(sc, cam, x, max_depth, rr_start, p, false, rays);
+        rgb[3 * i] = c.r, rgb[3 * i + 1] = c.g, rgb[3 * i + 2] = c.b;
+    }
+}
+
+int oc_render_mlt(void* s, const rt_camera* camera, const rt_mlt_params* p, float* film, rt_mlt_result* res,
+                  int threads)
+{
+    if (p->max_depth < 1 || p->mutations < 1) return -1;
+    MltRun R(static_cast<Scene*>(s), *camera, *p, threads > 0 ? threads : 1);
+    if (p->kind == RT_MLT_METROPOLIS)
+        run_metropolis(R);
+    else if (p->kind == RT_MLT_ERPT)
+        run_erpt(R);
+    else
+        return -1;
+    for (size_t i = 0; i < R.film.size(); ++i) film[i] += (float)R.film[i];
+    if (res) *res = R.res;
+    return 0;
 }
 
 }  // extern "C"

@@ -31,6 +31,7 @@
 
 #include <stdint.h>
 #include "../include/rt_scene.h"
+#include "../include/rt_mlt.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -89,6 +90,19 @@ void oc_transparent_dir(const float d[3], const float n[3], float ior, double u,
 void oc_bsdf(void* scene, int mat, const float wo[3], const float wi[3], const float nf[3], float f[3], float* pdf);
 int oc_bsdf_sample(void* scene, int mat, const float wo[3], const float nf[3], float u0, float u1, float u2,
                    float wi[3]);
+
+/* Markov-chain renderers (src/metropolisrenderer.cpp:56-200,
+ * src/energyredistributionrenderer.cpp:38-131) with the chain layout and
+ * Philox streams of include/rt_mlt.h.  film (height * width * 3) is added
+ * to.  Returns 0, or -1 for bad parameters. */
+int oc_render_mlt(void* scene, const rt_camera* camera, const rt_mlt_params* params, float* film,
+                  rt_mlt_result* result, int threads);
+/* The sample-driven integrator (src/unidipathtracingintegrator.cpp:48-107 on
+ * the path of src/renderer.cpp:68-106): samples [n][2*max_depth+1][4]. */
+void oc_integrate_samples(void* scene, const rt_camera* camera, int max_depth, int rr_start, float termination_prob,
+                          const float* samples, int64_t n, float* rgb, int threads);
+/* The primary-sample perturbation (src/metropolissample.cpp:62-78) of n coordinates. */
+void oc_mlt_perturb(const float* s, const float* u, const float* v, int64_t n, float* out);
 
 #ifdef __cplusplus
 }

@@ -3,9 +3,10 @@
 integrator interface named after the reference renderer's classes."""
 from . import _abi
 from ._abi import (LIGHT_AREA, LIGHT_CONE, LIGHT_ENV, LIGHT_POINT, MAT_BLINN, MAT_DARK, MAT_DIFFUSE, MAT_MIRROR,
-                   MAT_PHONG, MAT_REFRACTIVE,
+                   MAT_PHONG, MAT_REFRACTIVE, MLT_ERPT, MLT_METROPOLIS,
                    MODE_COMPAT, MODE_GUIDED, MODE_UNBIASED, RtError)
-from .integrator import GpuContext, GpuUniDiPathTracingIntegrator, PerPixelRenderer
+from .integrator import (EnergyRedistributionRenderer, GpuContext, GpuUniDiPathTracingIntegrator, MetropolisRenderer,
+                         PerPixelRenderer)
 from .scene import (AreaLight, BlinnMaterial, Camera, ConeLight, DarkMatter, DiffuseMaterial, EnvironmentMap,
                     IntersectableList, Mesh, MirrorMaterial, PhongMaterial, PointLight, Quad, Scene, Sphere,
                     TransparentMaterial, Triangle)
@@ -15,6 +16,6 @@ __all__ = [
     "GpuContext", "GpuUniDiPathTracingIntegrator", "PerPixelRenderer", "RtError",
     "AreaLight", "Camera", "DarkMatter", "DiffuseMaterial", "EnvironmentMap", "IntersectableList", "Mesh",
     "MirrorMaterial", "PhongMaterial", "PointLight", "Quad", "Scene", "Sphere", "Triangle", "SceneFile",
-    "BlinnMaterial", "TransparentMaterial", "ConeLight",
-    "MODE_COMPAT", "MODE_UNBIASED", "MODE_GUIDED",
+    "BlinnMaterial", "TransparentMaterial", "ConeLight", "MetropolisRenderer", "EnergyRedistributionRenderer",
+    "MODE_COMPAT", "MODE_UNBIASED", "MODE_GUIDED", "MLT_METROPOLIS", "MLT_ERPT",
 ]

@@ -89,6 +89,22 @@ class RtLaunchConfig(ctypes.Structure):
                 ("count_traversal", ctypes.c_int32), ("guide_epochs", ctypes.c_int32)]
 
 
+class RtMltParams(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("max_depth", ctypes.c_int32), ("rr_start", ctypes.c_int32),
+                ("termination_prob", ctypes.c_float), ("large_step_prob", ctypes.c_float),
+                ("bootstrap", ctypes.c_int32), ("mutations", ctypes.c_int32), ("chains", ctypes.c_int32),
+                ("erpt_spp_x", ctypes.c_int32), ("erpt_spp_y", ctypes.c_int32),
+                ("erpt_ed_termination", ctypes.c_float), ("seed", ctypes.c_uint32)]
+
+
+class RtMltResult(ctypes.Structure):
+    _fields_ = [("b", ctypes.c_double), ("ed", ctypes.c_double), ("chains", ctypes.c_int64),
+                ("mutations", ctypes.c_int64), ("accepted", ctypes.c_int64), ("seed_paths", ctypes.c_int64),
+                ("rays", ctypes.c_int64), ("render_ms", ctypes.c_double),
+                ("chain_seeds", ctypes.c_int64)]
+
+
+MLT_METROPOLIS, MLT_ERPT = 0, 1
 MAT_DIFFUSE, MAT_MIRROR, MAT_PHONG, MAT_DARK, MAT_BLINN, MAT_REFRACTIVE = 0, 1, 2, 3, 4, 5
 LIGHT_AREA, LIGHT_POINT, LIGHT_ENV, LIGHT_CONE = 0, 1, 2, 3
 MODE_COMPAT, MODE_UNBIASED, MODE_GUIDED = 0, 1, 2
@@ -148,6 +164,12 @@ PROTOTYPES = {
     "rt_unique_filename": (_I64, [ctypes.c_char_p, _P, _I64]),
     "rt_film_last_error": (ctypes.c_char_p, []),
     "rt_image_read_hdr": (ctypes.c_int, [ctypes.c_char_p, _P, _P, _P, _I64]),
+    # include/rt_mlt.h
+    "rt_integrate_samples": (ctypes.c_int, [_P, ctypes.POINTER(RtCamera), _I32, _I32, ctypes.c_float, _P, _I64, _P]),
+    "rt_render_mlt": (ctypes.c_int, [_P, ctypes.POINTER(RtCamera), ctypes.POINTER(RtMltParams), _P,
+                                     ctypes.POINTER(RtMltResult)]),
+    "rt_render_mlt_device": (ctypes.c_int, [_P, ctypes.POINTER(RtCamera), ctypes.POINTER(RtMltParams), _P,
+                                            ctypes.POINTER(RtMltResult)]),
 }
 
 _lib = None

@@ -15,6 +15,7 @@
 
 #include "../../include/rt_abi.h"
 #include "../../include/rt_film.h"
+#include "../../include/rt_mlt.h"
 #include "bvh_build.h"
 #include "rt_guide.h"
 #include "rt_kernels.h"
@@ -55,6 +56,8 @@ struct rt_ctx {
     size_t ps_cap = 0;
     std::vector<DevBuf> ps_bufs;
     DevBuf counters, views, tiles, passes, films;
+    // Markov-chain renderers: chain states, proposals, seed-path values, deposits
+    DevBuf mlt_cur, mlt_prop, mlt_val, mlt_energy, mlt_acc, mlt_cnt, mlt_seed, mlt_cdf, mlt_nch, mlt_src, mlt_view;
     /* defaults from a sweep on MI355X (1M-triangle mesh, 1024^2 x 64 spp):
      * 32M path slots per pass (see render_impl), 8 persistent blocks per CU
      * (all resident: the traversal kernels claim work dynamically, so extra
@@ -1234,6 +1237,333 @@ static int render_impl(rt_ctx* c, const rt_view* views, int32_t nv, const rt_par
     return RT_OK;
 }
 
+/* ---------------------------------------------------------------------------
+ * Markov-chain renderers (include/rt_mlt.h)
+ * ------------------------------------------------------------------------- */
+namespace {
+
+/* Trace the n paths whose primary samples are in c->mlt_prop (stride cap):
+ * the compat path kernels, depth bounces, with the given length rule. */
+hipError_t mlt_trace(rt_ctx* c, int n, int cap, int max_depth, int rr_start, float rr_prob, uint32_t seed,
+                     int pixel_units)
+{
+    PtParams pp{RT_MODE_COMPAT, max_depth, rr_start, rr_prob, seed, 0, 0.f, (const float4*)c->mlt_prop.p, cap};
+    const int grid = grid_for(c);
+    hipError_t e;
+    if ((e = launch_generate_pss(c->ps, (const DevView*)c->mlt_view.p, n, pp, pixel_units, grid, c->stream)))
+        return e;
+    for (int dd = 0; dd < max_depth; ++dd) {
+        int q = dd & 1;
+        if ((e = launch_extend(c->scene, c->ps, q, 0, grid, c->stream))) return e;
+        if ((e = launch_shade(c->scene, c->ps, pp, dd, q, grid, c->stream))) return e;
+        if ((e = launch_shadow(c->scene, c->ps, pp, grid, c->stream))) return e;
+        if ((e = launch_advance(c->ps, q, dd, c->stream))) return e;
+    }
+    return hipSuccess;
+}
+
+/* Path slots, proposal and state buffers for cap chains of G groups. */
+int mlt_alloc(rt_ctx* c, int64_t cap, int G, int64_t npix)
+{
+    int r;
+    if ((r = alloc_path_state(c, (size_t)cap))) return r;
+    if ((r = ensure_stack_overflow(c, grid_for(c)))) return r;
+    if ((r = ensure(c, c->mlt_cur, (size_t)G * cap * sizeof(float4))) ||
+        (r = ensure(c, c->mlt_prop, (size_t)G * cap * sizeof(float4))) ||
+        (r = ensure(c, c->mlt_val, (size_t)cap * sizeof(float4))) ||
+        (r = ensure(c, c->mlt_energy, (size_t)cap * sizeof(double))) ||
+        (r = ensure(c, c->mlt_src, (size_t)cap * sizeof(int32_t))) ||
+        (r = ensure(c, c->mlt_acc, (size_t)npix * 3 * sizeof(unsigned long long))) ||
+        (r = ensure(c, c->mlt_cnt, sizeof(unsigned long long))) || (r = ensure(c, c->mlt_view, sizeof(DevView))))
+        return r;
+    return 0;
+}
+
+MltDev mlt_dev(rt_ctx* c, int cap, int G, int W, int H, const rt_mlt_params& p)
+{
+    MltDev m{};
+    m.cur = (float4*)c->mlt_cur.p;
+    m.prop = (float4*)c->mlt_prop.p;
+    m.cur_val = (float4*)c->mlt_val.p;
+    m.energy = (double*)c->mlt_energy.p;
+    m.acc = (unsigned long long*)c->mlt_acc.p;
+    m.accepted = (unsigned long long*)c->mlt_cnt.p;
+    m.cap = cap;
+    m.G = G;
+    m.W = W;
+    m.H = H;
+    m.seed = p.seed;
+    m.kind = p.kind;
+    m.large_prob = p.large_step_prob;
+    m.spp_x = p.erpt_spp_x;
+    m.spp_y = p.erpt_spp_y;
+    return m;
+}
+
+/* Fixed-point unit for deposits of at most `bound` each, n_dep of them in
+ * total: 2^-30 of the bound, coarser only if the sum of n_dep largest
+ * deposits could overflow 63 bits. */
+double mlt_unit(double bound, double n_dep)
+{
+    int frac = 30;
+    while (frac > 0 && std::ldexp(1.0, frac) * (n_dep + 1.0) >= std::ldexp(1.0, 62)) --frac;
+    return bound > 0 ? bound * std::ldexp(1.0, -frac) : 1.0;
+}
+
+#define MLT_TRY(call)                                                                              \
+    do {                                                                                           \
+        hipError_t e_ = (call);                                                                    \
+        if (e_ != hipSuccess) return set_err(c, RT_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+/* MetropolisRenderer::render, src/metropolisrenderer.cpp:56-200, with
+ * chains in lockstep: bootstrap, chain starts drawn by luminance, `iters`
+ * mutations per chain. */
+int run_metropolis(rt_ctx* c, const rt_mlt_params& p, int W, int H, int64_t cap_slots, rt_mlt_result& res,
+                   double& unit)
+{
+    const int G = 2 * p.max_depth + 1;
+    const int64_t area = (int64_t)W * H;
+    const int64_t total = (int64_t)p.mutations * area;
+    const int64_t C = p.chains > 0 ? p.chains : std::max<int64_t>(1, std::min<int64_t>(total / 16, (int64_t)1 << 22));
+    const int64_t iters = std::max<int64_t>(1, (total + C / 2) / C);
+    const int64_t nb = std::max<int64_t>(p.bootstrap, C);
+    if (nb > INT32_MAX || C > INT32_MAX || iters > INT32_MAX) return set_err(c, RT_ERR_INVALID, "too many chains or mutations");
+    const int cap = (int)std::min<int64_t>(cap_slots, std::max(C, nb));
+    if (int r = mlt_alloc(c, cap, G, area)) return r;
+    if (int r = ensure(c, c->mlt_seed, (size_t)nb * sizeof(float4))) return r;
+    MltDev m = mlt_dev(c, cap, G, W, H, p);
+    const int grid = grid_for(c);
+    float4* bval = (float4*)c->mlt_seed.p;
+    /* bootstrap: full-length paths (MetropolisSample's default pathLength =
+     * MAX_DEPTH, src/metropolissample.h:35), no roulette compensation */
+    for (int64_t base = 0; base < nb; base += cap) {
+        const int n = (int)std::min<int64_t>(cap, nb - base);
+        MLT_TRY(launch_mlt_fill(m, RT_MLT_STREAM_BOOTSTRAP, (uint32_t)base, n, 0, grid, c->stream));
+        MLT_TRY(mlt_trace(c, n, cap, p.max_depth, p.rr_start, 0.f, p.seed, 0));
+        MLT_TRY(launch_mlt_values(c->ps, n, bval + base, 0, grid, c->stream));
+    }
+    std::vector<float4> hv((size_t)nb);
+    MLT_TRY(hipMemcpyAsync(hv.data(), bval, (size_t)nb * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
+    MLT_TRY(hipStreamSynchronize(c->stream));
+    std::vector<double> cdf((size_t)nb);
+    double acc = 0;
+    for (int64_t i = 0; i < nb; ++i) cdf[i] = acc += (double)hv[i].w;
+    const float b = (float)(acc / (double)nb);
+    res.b = b;
+    res.chains = C;
+    res.seed_paths = nb;
+    res.mutations = 0;
+    if (!(acc > 0)) return RT_OK;  /* nothing reaches the camera: no deposit */
+    if (int r = upload(c, c->mlt_cdf, cdf.data(), cdf.size() * sizeof(double))) return r;
+    m.scale = (float)((double)b * (double)area / ((double)C * (double)iters));
+    /* a deposit's channel is at most scale / 0.114 (luminance weights) */
+    unit = mlt_unit((double)m.scale * 9.0, 2.0 * (double)C * (double)iters);
+    m.inv_unit = 1.0 / unit;
+    for (int64_t c0 = 0; c0 < C; c0 += cap) {
+        const int n = (int)std::min<int64_t>(cap, C - c0);
+        m.chain0 = (uint32_t)c0;
+        MLT_TRY(launch_mlt_start(m, (const double*)c->mlt_cdf.p, nb, acc, bval, n, grid, c->stream));
+        for (int64_t it = 0; it < iters; ++it) {
+            if (c->stop) return set_err(c, RT_ERR_CANCELLED, "render cancelled");
+            MLT_TRY(launch_mlt_propose(m, (uint32_t)it, n, grid, c->stream));
+            MLT_TRY(mlt_trace(c, n, cap, p.max_depth, p.rr_start, p.termination_prob, p.seed, 0));
+            MLT_TRY(launch_mlt_accept(m, c->ps, (uint32_t)it, n, grid, c->stream));
+        }
+    }
+    res.mutations = C * iters;
+    return RT_OK;
+}
+
+/* EnergyRedistributionRenderer::render, src/energyredistributionrenderer.cpp:38-131 */
+int run_erpt(rt_ctx* c, const rt_mlt_params& p, int W, int H, int64_t cap_slots, rt_mlt_result& res, double& unit)
+{
+    const int G = 2 * p.max_depth + 1;
+    const int64_t area = (int64_t)W * H;
+    const int spp = p.erpt_spp_x * p.erpt_spp_y, M = p.mutations;
+    const int64_t n0 = area * spp;
+    if (n0 > INT32_MAX) return set_err(c, RT_ERR_INVALID, "too many initial samples");
+    const int seed_cap = (int)std::min<int64_t>(cap_slots, std::max(n0, area));
+    if (int r = mlt_alloc(c, seed_cap, G, area)) return r;
+    if (int r = ensure(c, c->mlt_seed, (size_t)std::max(n0, area) * sizeof(float4))) return r;
+    if (int r = ensure(c, c->mlt_nch, (size_t)n0 * sizeof(int32_t))) return r;
+    MltDev m = mlt_dev(c, seed_cap, G, W, H, p);
+    const int grid = grid_for(c);
+    float4* vals = (float4*)c->mlt_seed.p;
+    /* e_d: one path through each pixel corner (computeEd) */
+    for (int64_t base = 0; base < area; base += seed_cap) {
+        const int n = (int)std::min<int64_t>(seed_cap, area - base);
+        MLT_TRY(launch_mlt_fill(m, RT_MLT_STREAM_ED, (uint32_t)base, n, 1, grid, c->stream));
+        MLT_TRY(mlt_trace(c, n, seed_cap, p.max_depth, p.rr_start, p.erpt_ed_termination, p.seed, 1));
+        MLT_TRY(launch_mlt_values(c->ps, n, vals + base, 1, grid, c->stream));
+    }
+    std::vector<float4> hv((size_t)area);
+    MLT_TRY(hipMemcpyAsync(hv.data(), vals, (size_t)area * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
+    MLT_TRY(hipStreamSynchronize(c->stream));
+    double er = 0, eg = 0, eb = 0;
+    for (const float4& v : hv) er += v.x, eg += v.y, eb += v.z;
+    const float ed = (float)(std::sqrt(er * er + eg * eg + eb * eb) / ((double)area * M));
+    res.ed = ed;
+    res.seed_paths = area + n0;
+    /* initial samples and their chain counts */
+    int32_t* nch = (int32_t*)c->mlt_nch.p;
+    for (int64_t base = 0; base < n0; base += seed_cap) {
+        const int n = (int)std::min<int64_t>(seed_cap, n0 - base);
+        MLT_TRY(launch_mlt_fill(m, RT_MLT_STREAM_SEED, (uint32_t)base, n, 2, grid, c->stream));
+        MLT_TRY(mlt_trace(c, n, seed_cap, p.max_depth, p.rr_start, 0.f, p.seed, 0));
+        MLT_TRY(launch_mlt_values(c->ps, n, vals + base, 1, grid, c->stream));
+        MLT_TRY(launch_erpt_count(vals + base, n, (uint32_t)base, p.seed, ed, M, nch + base, grid, c->stream));
+    }
+    std::vector<int32_t> hn((size_t)n0);
+    MLT_TRY(hipMemcpyAsync(hn.data(), nch, (size_t)n0 * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    MLT_TRY(hipStreamSynchronize(c->stream));
+    int64_t C = 0;
+    for (int32_t k : hn) {
+        C += k;
+        res.chain_seeds += k > 0;
+    }
+    res.chains = C;
+    res.mutations = 0;
+    if (C == 0) return RT_OK;
+    if (C > INT32_MAX) return set_err(c, RT_ERR_INVALID, "too many chains");
+    const int cap = (int)std::min<int64_t>(cap_slots, C);
+    if (cap > seed_cap) {
+        if (int r = mlt_alloc(c, cap, G, area)) return r;
+    }
+    const int stride = std::max(cap, seed_cap);
+    m = mlt_dev(c, stride, G, W, H, p);
+    m.scale = 1.f / (float)spp;
+    /* a deposit's channel is at most e_d / spp */
+    unit = mlt_unit((double)ed * m.scale * 1.01, (double)C * M);
+    m.inv_unit = 1.0 / unit;
+    /* chain -> initial sample, chains numbered in initial-sample order; all
+     * of it uploaded before the batches are queued */
+    std::vector<int32_t> src((size_t)C);
+    for (int64_t i = 0, k = 0; i < n0; ++i)
+        for (int32_t r = 0; r < hn[i]; ++r) src[k++] = (int32_t)i;
+    if (int r = upload(c, c->mlt_src, src.data(), src.size() * sizeof(int32_t))) return r;
+    for (int64_t c0 = 0; c0 < C; c0 += cap) {
+        const int n = (int)std::min<int64_t>(cap, C - c0);
+        m.chain0 = (uint32_t)c0;
+        MLT_TRY(launch_erpt_start(m, (const int32_t*)c->mlt_src.p + c0, nch, vals, ed, n, grid, c->stream));
+        for (int it = 0; it < M; ++it) {
+            if (c->stop) return set_err(c, RT_ERR_CANCELLED, "render cancelled");
+            MLT_TRY(launch_mlt_propose(m, (uint32_t)it, n, grid, c->stream));
+            MLT_TRY(mlt_trace(c, n, stride, p.max_depth, p.rr_start, 0.f, p.seed, 0));
+            MLT_TRY(launch_mlt_accept(m, c->ps, (uint32_t)it, n, grid, c->stream));
+        }
+    }
+    res.mutations = C * M;
+    return RT_OK;
+}
+
+int render_mlt_impl(rt_ctx* c, const rt_camera* cam, const rt_mlt_params* p, float* film, bool device_film,
+                    rt_mlt_result* out)
+{
+    if (!c || !cam || !p || !film) return RT_ERR_INVALID;
+    if (int r = check_fault(c)) return r;
+    if (!c->has_scene) return set_err(c, RT_ERR_NO_SCENE, "no scene uploaded");
+    const int W = cam->width, H = cam->height;
+    if (W <= 0 || H <= 0 || (int64_t)W * H > (1 << 26)) return set_err(c, RT_ERR_INVALID, "bad camera");
+    if (p->kind != RT_MLT_METROPOLIS && p->kind != RT_MLT_ERPT) return set_err(c, RT_ERR_INVALID, "unknown kind");
+    if (p->max_depth < 1 || p->max_depth > 64 || p->rr_start < 0 || p->mutations < 1 ||
+        !(p->termination_prob >= 0.f && p->termination_prob < 1.f) ||
+        !(p->large_step_prob >= 0.f && p->large_step_prob <= 1.f) || p->bootstrap < 0 || p->chains < 0 ||
+        (p->kind == RT_MLT_ERPT && (p->erpt_spp_x < 1 || p->erpt_spp_y < 1 ||
+                                    !(p->erpt_ed_termination >= 0.f && p->erpt_ed_termination < 1.f))))
+        return set_err(c, RT_ERR_INVALID, "bad Metropolis parameters");
+    hipSetDevice(c->device);
+    /* chains per batch: 4M path slots unless configured */
+    const int64_t cap = c->cfg.path_slots > 0 ? c->cfg.path_slots : (int64_t)1 << 22;
+    const int64_t area = (int64_t)W * H;
+    rt_view v{*cam, 1, 1};
+    DevView dv = make_view(v, nullptr);
+    rt_mlt_result res{};
+    HIP_TRY(c, hipEventRecord(c->ev_a, c->stream));
+    int r;
+    if ((r = alloc_path_state(c, 256)) || (r = ensure(c, c->mlt_view, sizeof(DevView))) ||
+        (r = ensure(c, c->mlt_cnt, sizeof(unsigned long long))) ||
+        (r = ensure(c, c->mlt_acc, (size_t)area * 3 * sizeof(unsigned long long))))
+        return r;
+    HIP_TRY(c, hipMemcpyAsync(c->mlt_view.p, &dv, sizeof(dv), hipMemcpyHostToDevice, c->stream));
+    if ((r = (launch_reset_counters(c->ps, c->stream) == hipSuccess ? 0 : RT_ERR_HIP))) return set_err(c, r, "reset");
+    MLT_TRY(hipMemsetAsync(c->mlt_acc.p, 0, (size_t)area * 3 * sizeof(unsigned long long), c->stream));
+    MLT_TRY(hipMemsetAsync(c->mlt_cnt.p, 0, sizeof(unsigned long long), c->stream));
+    double unit = 0;
+    r = p->kind == RT_MLT_METROPOLIS ? run_metropolis(c, *p, W, H, cap, res, unit)
+                                     : run_erpt(c, *p, W, H, cap, res, unit);
+    if (r) return r;
+    float* dfilm = film;
+    if (!device_film) {
+        if ((r = ensure(c, c->films, (size_t)area * 3 * 4))) return r;
+        dfilm = (float*)c->films.p;
+        HIP_TRY(c, hipMemcpyAsync(dfilm, film, (size_t)area * 3 * 4, hipMemcpyHostToDevice, c->stream));
+    }
+    if (unit > 0)
+        MLT_TRY(launch_mlt_resolve((const unsigned long long*)c->mlt_acc.p, dfilm, area * 3, unit, grid_for(c),
+                                   c->stream));
+    HIP_TRY(c, hipEventRecord(c->ev_b, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (!device_film) HIP_TRY(c, hipMemcpy(film, dfilm, (size_t)area * 3 * 4, hipMemcpyDeviceToHost));
+    unsigned long long acc_n = 0;
+    HIP_TRY(c, hipMemcpy(&acc_n, c->mlt_cnt.p, sizeof(acc_n), hipMemcpyDeviceToHost));
+    DevCounters hc;
+    HIP_TRY(c, hipMemcpy(&hc, c->ps.cnt, sizeof(hc), hipMemcpyDeviceToHost));
+    float ms = 0;
+    hipEventElapsedTime(&ms, c->ev_a, c->ev_b);
+    res.accepted = (int64_t)acc_n;
+    res.rays = (int64_t)(hc.camera_rays + hc.extension_rays + hc.shadow_rays);
+    res.render_ms = ms;
+    c->stats.render_ms = ms;
+    c->stats.camera_rays = (int64_t)hc.camera_rays;
+    c->stats.extension_rays = (int64_t)hc.extension_rays;
+    c->stats.shadow_rays = (int64_t)hc.shadow_rays;
+    c->stats.extend_ms = 0;
+    c->stats.shadow_ms = 0;
+    c->stats.extend_launches = 0;
+    if (out) *out = res;
+    return RT_OK;
+}
+
+/* Radiance of host-given primary-sample vectors ([n][G][4]). */
+int integrate_samples_impl(rt_ctx* c, const rt_camera* cam, int max_depth, int rr_start, float tp,
+                           const float* samples, int64_t n, float* rgb)
+{
+    if (!c || !cam || (!samples && n > 0) || (!rgb && n > 0) || n < 0) return RT_ERR_INVALID;
+    if (int r = check_fault(c)) return r;
+    if (!c->has_scene) return set_err(c, RT_ERR_NO_SCENE, "no scene uploaded");
+    if (cam->width <= 0 || cam->height <= 0 || max_depth < 1 || max_depth > 64 || rr_start < 0 ||
+        !(tp >= 0.f && tp < 1.f))
+        return set_err(c, RT_ERR_INVALID, "bad parameters");
+    if (n == 0) return RT_OK;
+    hipSetDevice(c->device);
+    const int G = 2 * max_depth + 1;
+    const int cap = (int)std::min<int64_t>(n, c->cfg.path_slots > 0 ? c->cfg.path_slots : (int64_t)1 << 22);
+    int r;
+    if ((r = mlt_alloc(c, cap, G, 1))) return r;
+    rt_view v{*cam, 1, 1};
+    DevView dv = make_view(v, nullptr);
+    HIP_TRY(c, hipMemcpyAsync(c->mlt_view.p, &dv, sizeof(dv), hipMemcpyHostToDevice, c->stream));
+    if (launch_reset_counters(c->ps, c->stream) != hipSuccess) return set_err(c, RT_ERR_HIP, "reset");
+    std::vector<float> soa((size_t)G * cap * 4), out((size_t)cap * 4);
+    for (int64_t base = 0; base < n; base += cap) {
+        const int m = (int)std::min<int64_t>(cap, n - base);
+        for (int j = 0; j < m; ++j)
+            for (int g = 0; g < G; ++g)
+                std::memcpy(&soa[((size_t)g * cap + j) * 4], samples + ((size_t)(base + j) * G + g) * 4, 16);
+        HIP_TRY(c, hipMemcpyAsync(c->mlt_prop.p, soa.data(), soa.size() * 4, hipMemcpyHostToDevice, c->stream));
+        MLT_TRY(mlt_trace(c, m, cap, max_depth, rr_start, tp, 0, 0));
+        MLT_TRY(launch_mlt_values(c->ps, m, (float4*)c->mlt_val.p, 0, grid_for(c), c->stream));
+        HIP_TRY(c, hipMemcpyAsync(out.data(), c->mlt_val.p, (size_t)m * 16, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        for (int j = 0; j < m; ++j)
+            for (int k = 0; k < 3; ++k) rgb[(base + j) * 3 + k] = out[(size_t)j * 4 + k];
+    }
+    return RT_OK;
+}
+
+}  // namespace
+
 int rt_render(rt_ctx* c, const rt_view* views, int32_t nv, const rt_params* p, float* const* films)
 {
     return render_impl(c, views, nv, p, films, false);
@@ -1242,6 +1572,23 @@ int rt_render(rt_ctx* c, const rt_view* views, int32_t nv, const rt_params* p, f
 int rt_render_device(rt_ctx* c, const rt_view* views, int32_t nv, const rt_params* p, float* const* films)
 {
     return render_impl(c, views, nv, p, films, true);
+}
+
+int rt_integrate_samples(rt_ctx* c, const rt_camera* camera, int32_t max_depth, int32_t rr_start,
+                         float termination_prob, const float* samples, int64_t n, float* rgb)
+{
+    return integrate_samples_impl(c, camera, max_depth, rr_start, termination_prob, samples, n, rgb);
+}
+
+int rt_render_mlt(rt_ctx* c, const rt_camera* camera, const rt_mlt_params* p, float* film, rt_mlt_result* res)
+{
+    return render_mlt_impl(c, camera, p, film, false, res);
+}
+
+int rt_render_mlt_device(rt_ctx* c, const rt_camera* camera, const rt_mlt_params* p, float* film,
+                         rt_mlt_result* res)
+{
+    return render_mlt_impl(c, camera, p, film, true, res);
 }
 
 }  // extern "C"

@@ -97,7 +97,52 @@ struct PtParams {
     uint32_t seed;
     int32_t guide_train;  /* record training samples */
     float rec_prob;       /* probability of keeping a training record */
+    /* Compat paths driven by primary samples instead of Philox streams
+     * (Markov-chain renderers): group g of the path in slot p is
+     * pss[g * pss_stride + p].  Null for the path-tracing renderer. */
+    const float4* pss;
+    int32_t pss_stride;
 };
+
+/* Chains of the Markov-chain renderers (rt_mlt.hip), one path slot each. */
+struct MltDev {
+    float4* cur;        /* G x cap: primary samples of the current states (group-major) */
+    float4* prop;       /* G x cap: proposals (PtParams::pss of the traced paths) */
+    float4* cur_val;    /* cap: radiance of the current state, w = its luminance (Metropolis) or norm (ERPT) */
+    double* energy;     /* cap (ERPT): norm of the chain's deposit value */
+    unsigned long long* acc;       /* W * H * 3 deposits in fixed point */
+    unsigned long long* accepted;  /* accepted proposals */
+    int32_t cap, G, W, H;
+    uint32_t seed;
+    int32_t kind;       /* rt_mlt_kind */
+    float large_prob;
+    float scale;        /* Metropolis: b * pixels / mutations; ERPT: 1 / samples per pixel */
+    double inv_unit;    /* fixed-point steps per unit of radiance */
+    uint32_t chain0;    /* chain number of slot 0 */
+    int32_t spp_x, spp_y;  /* ERPT initial samples per pixel */
+};
+
+/* Fill the proposal vectors of n paths with fresh draws of `stream` for
+ * items index0 .. index0 + n - 1; mode 0: as drawn, 1: group 0 (x, y) = the
+ * pixel's corner in pixels (ERPT e_d), 2: ERPT initial sample. */
+hipError_t launch_mlt_fill(const MltDev& m, uint32_t stream, uint32_t index0, int n, int mode, int grid,
+                           hipStream_t st);
+/* out[j] = (radiance of path j, its luminance (metric 0) or norm (1)). */
+hipError_t launch_mlt_values(const PathState& ps, int n, float4* out, int metric, int grid, hipStream_t st);
+/* Metropolis chain starts: bootstrap path drawn by luminance from cdf. */
+hipError_t launch_mlt_start(const MltDev& m, const double* cdf, int64_t nb, double total, const float4* bval,
+                            int n, int grid, hipStream_t st);
+/* ERPT chain counts of initial samples index0 ..: floor(u + |c| / (M e_d)). */
+hipError_t launch_erpt_count(const float4* val, int n, uint32_t index0, uint32_t seed, float ed, int M,
+                             int32_t* nch, int grid, hipStream_t st);
+/* ERPT chain starts: chain j begins at initial sample src[j]. */
+hipError_t launch_erpt_start(const MltDev& m, const int32_t* src, const int32_t* nch, const float4* val, float ed,
+                             int n, int grid, hipStream_t st);
+hipError_t launch_mlt_propose(const MltDev& m, uint32_t step, int n, int grid, hipStream_t st);
+hipError_t launch_mlt_accept(const MltDev& m, const PathState& ps, uint32_t step, int n, int grid, hipStream_t st);
+/* film[i] += acc[i] * unit */
+hipError_t launch_mlt_resolve(const unsigned long long* acc, float* film, int64_t n, double unit, int grid,
+                              hipStream_t st);
 
 struct GuideState;
 hipError_t launch_shade_guided(const HostDevScene& s, const PathState& ps, const GuideState& g, PtParams p,
@@ -114,6 +159,11 @@ hipError_t launch_trace_count(const HostDevScene& s, const float4* o, const floa
 /* Wavefront pipeline kernels. */
 hipError_t launch_generate(const PathState& ps, const DevView* views, const DevTile* tiles, const DevPass* passes,
                            PtParams p, int grid, hipStream_t st);
+/* Camera rays of n paths from their primary samples (group 0: image point,
+ * scaled by the resolution unless pixel_units, and the length uniform);
+ * path slot j = sample j.  View: the camera of `view` (film unused). */
+hipError_t launch_generate_pss(const PathState& ps, const DevView* view, int n, PtParams p, int pixel_units,
+                               int grid, hipStream_t st);
 hipError_t launch_extend(const HostDevScene& s, const PathState& ps, int qsel, int count, int grid, hipStream_t st);
 hipError_t launch_shade(const HostDevScene& s, const PathState& ps, PtParams p, int depth, int qsel, int grid,
                         hipStream_t st);

@@ -43,6 +43,18 @@ static_assert(sizeof(HostDevScene) == sizeof(DevScene), "scene view layouts diff
 #else
 #define RT_TRAV_OCCUPANCY
 #endif
+/* Register budget of k_shade (waves per SIMD; 0 = the compiler's choice).
+ * Left alone the unbiased shader takes 145 VGPRs (3 waves); 4 waves (128
+ * VGPRs, 8 bytes of spill) shade a 1024^2 x 64 spp render 10% faster, 5
+ * waves spill 108 bytes and gain nothing more. */
+#ifndef RT_SHADE_WAVES
+#define RT_SHADE_WAVES 4
+#endif
+#if RT_SHADE_WAVES > 0
+#define RT_SHADE_OCCUPANCY __attribute__((amdgpu_waves_per_eu(RT_SHADE_WAVES, 8)))
+#else
+#define RT_SHADE_OCCUPANCY
+#endif
 
 int64_t trav_lanes(int grid) { return (int64_t)grid * PT_BLOCK; }
 
@@ -451,6 +463,23 @@ __device__ __forceinline__ bool bsdf_sample(const RtMaterial& m, vec3 wo, vec3 n
  * (8 bits), specular-chain flag (bit 16). */
 __device__ __forceinline__ int pk(int k, int L, int spec) { return k | (L << 8) | (spec << 16); }
 
+/* Random numbers of group g of the path in slot p: its pixel sample's
+ * Philox stream, or its primary-sample vector when the paths are driven by
+ * Markov chains. */
+__device__ __forceinline__ U4 path_u4(const PtParams& P, int4 info, int p, uint32_t g)
+{
+    if (P.pss) {
+        const float4 v = P.pss[(size_t)g * P.pss_stride + p];
+        U4 r;
+        r.u[0] = v.x;
+        r.u[1] = v.y;
+        r.u[2] = v.z;
+        r.u[3] = v.w;
+        return r;
+    }
+    return rng4(P.seed, (uint32_t)info.z, (uint32_t)info.x, (uint32_t)info.y, g);
+}
+
 /* Next-event sample of the unbiased estimators: picks light floor(u2 N) and a
  * direction towards it.  Returns false when nothing can arrive.  `le` is the
  * light's contribution over the direction density (without the 1/N of the
@@ -562,6 +591,28 @@ __device__ __forceinline__ int tile_pixel(const DevTile& T, const DevView& V, in
 /* ===========================================================================
  * generate
  * ========================================================================= */
+/* Camera::getRay (src/camera.cpp:52-57) with the reference's arithmetic: the
+ * image-plane point in double (QPointF), rounded to float, then the
+ * direction x u + y v - w with every product and sum rounded on its own. */
+__device__ __forceinline__ vec3 camera_dir(const DevView& V, double px, double py)
+{
+    const float x = (float)__dadd_rn((double)V.l, __ddiv_rn(__dmul_rn((double)(V.r - V.l), px + 0.5), (double)V.W));
+    const float y = (float)__dadd_rn((double)V.t, __ddiv_rn(__dmul_rn((double)(V.b - V.t), py + 0.5), (double)V.H));
+    vec3 d;
+    d.x = __fadd_rn(__fadd_rn(__fmul_rn(x, V.u[0]), __fmul_rn(y, V.v[0])), -V.w[0]);
+    d.y = __fadd_rn(__fadd_rn(__fmul_rn(x, V.u[1]), __fmul_rn(y, V.v[1])), -V.w[1]);
+    d.z = __fadd_rn(__fadd_rn(__fmul_rn(x, V.u[2]), __fmul_rn(y, V.v[2])), -V.w[2]);
+    return d;
+}
+
+/* QVector3D::normalized(): the length in double, components rounded to float */
+__device__ __forceinline__ vec3 norm_qt(vec3 a)
+{
+    const double l = __dadd_rn(__dadd_rn(__dmul_rn(a.x, a.x), __dmul_rn(a.y, a.y)), __dmul_rn(a.z, a.z));
+    if (l == 0.0) return mk(0.f, 0.f, 0.f);
+    const double r = sqrt(l);
+    return mk((float)(a.x / r), (float)(a.y / r), (float)(a.z / r));
+}
 __global__ __launch_bounds__(PT_BLOCK) void k_generate(PathState ps, const DevView* __restrict__ views,
                                                       const DevTile* __restrict__ tiles,
                                                       const DevPass* __restrict__ passes, PtParams P)
@@ -589,12 +640,11 @@ __global__ __launch_bounds__(PT_BLOCK) void k_generate(PathState ps, const DevVi
         int s = T.s0 + j / T.npix;
         U4 g0 = rng4(P.seed, (uint32_t)T.view, (uint32_t)pixel, (uint32_t)s, 0u);
         int a = s / V.spp_y, b = s % V.spp_y;
-        float sx = (g0.u[0] + (float)a) / (float)V.spp_x, sy = (g0.u[1] + (float)b) / (float)V.spp_y;
-        float px = (float)(pixel % V.W) + sx, py = (float)(pixel / V.W) + sy;
-        float x = V.l + (V.r - V.l) * (px + 0.5f) / (float)V.W;
-        float y = V.t + (V.b - V.t) * (py + 0.5f) / (float)V.H;
-        vec3 d = ld3(V.u) * x + ld3(V.v) * y - ld3(V.w);
-        if (P.mode != RT_MODE_COMPAT) d = norm(d);
+        /* src/jitteredsampler.cpp:23-35 and src/perpixelrenderer.cpp:79-84 in
+         * double (QPointF) */
+        double sx = ((double)g0.u[0] + a) / V.spp_x, sy = ((double)g0.u[1] + b) / V.spp_y;
+        vec3 d = camera_dir(V, (double)(pixel % V.W) + sx, (double)(pixel / V.W) + sy);
+        if (P.mode != RT_MODE_COMPAT) d = norm_qt(d);
         int L = P.max_depth;
         if (P.mode == RT_MODE_COMPAT && P.rr_prob > 0.f) {
             int nb = 0;
@@ -618,6 +668,57 @@ hipError_t launch_generate(const PathState& ps, const DevView* views, const DevT
                            PtParams p, int grid, hipStream_t st)
 {
     hipLaunchKernelGGL(k_generate, dim3(grid), dim3(PT_BLOCK), 0, st, ps, views, tiles, passes, p);
+    return hipGetLastError();
+}
+
+/* Camera paths from primary samples (MetropolisSample::cameraPathFromSample,
+ * src/metropolissample.cpp:120-127): the image point is group 0's (x, y)
+ * times the resolution, the path length is drawn from its z as in
+ * k_generate.  The pixel recorded in info is MetropolisSample::getPos. */
+__global__ __launch_bounds__(PT_BLOCK) void k_generate_pss(PathState ps, const DevView* __restrict__ view, int n,
+                                                          PtParams P, int pixel_units)
+{
+    const DevView V = *view;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ps.cnt->q_count[0] = n;
+        ps.cnt->q_count[1] = 0;
+        ps.cnt->shadow_count = 0;
+        ps.cnt->ext_next = 0;
+        ps.cnt->sh_next = 0;
+        ps.cnt->camera_rays += (unsigned long long)n;
+    }
+    for (int slot = blockIdx.x * PT_BLOCK + threadIdx.x; slot < n; slot += gridDim.x * PT_BLOCK) {
+        const float4 g0 = P.pss[slot];
+        double px = pixel_units ? (double)g0.x : (double)g0.x * V.W;
+        double py = pixel_units ? (double)g0.y : (double)g0.y * V.H;
+        vec3 d = camera_dir(V, px, py);
+        int L = P.max_depth;
+        if (P.rr_prob > 0.f) {
+            int nb = 0;
+            if (P.rr_prob < 1.f) {
+                float q = floorf(logf(1.f - g0.z) / logf(1.f - P.rr_prob));
+                nb = q > 1000.f ? 1000 : (int)q;
+            }
+            L = min(P.rr_start + nb, P.max_depth);
+        }
+        int ix = pixel_units ? (int)g0.x : (int)((double)g0.x * V.W);
+        int iy = pixel_units ? (int)g0.y : (int)((double)g0.y * V.H);
+        ix = max(0, min(ix, V.W - 1));
+        iy = max(0, min(iy, V.H - 1));
+        ps.ray_o[slot] = make_float4(V.eye[0], V.eye[1], V.eye[2], RT_EPS);
+        ps.ray_d[slot] = make_float4(d.x, d.y, d.z, RT_INF);
+        ps.alpha[slot] = make_float4(1.f, 1.f, 1.f, 0.f);
+        ps.radiance[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
+        ps.pend_c[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
+        ps.info[slot] = make_int4(iy * V.W + ix, 0, 0, pk(0, L, 1));
+        ps.queue[0][slot] = slot;
+    }
+}
+
+hipError_t launch_generate_pss(const PathState& ps, const DevView* view, int n, PtParams p, int pixel_units,
+                               int grid, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_generate_pss, dim3(grid), dim3(PT_BLOCK), 0, st, ps, view, n, p, pixel_units);
     return hipGetLastError();
 }
 
@@ -754,7 +855,7 @@ __device__ void shade_compat(const DevScene& S, const PathState& ps, const PtPar
     const RtMaterial& M = S.mats[S.mat[g]];
     vec3 x = o + h.x * d;
     vec3 n = hit_normal(S, g, x, h.z, h.w);
-    U4 gs = rng4(P.seed, (uint32_t)info.z, (uint32_t)info.x, (uint32_t)info.y, 1u + 2u * depth);
+    U4 gs = path_u4(P, info, p, 1u + 2u * depth);
 
     if (M.type == RT_MAT_MIRROR || M.type == RT_MAT_REFRACTIVE) {
         /* specular vertex: not stored; extend the path length once (:87-91) */
@@ -811,7 +912,7 @@ __device__ void shade_compat(const DevScene& S, const PathState& ps, const PtPar
 
     /* next-event estimate for this stored vertex k (Light::getIntensity) */
     if (S.n_lights > 0) {
-        U4 gl = rng4(P.seed, (uint32_t)info.z, (uint32_t)info.x, (uint32_t)info.y, 2u + 2u * k);
+        U4 gl = path_u4(P, info, p, 2u + 2u * k);
         int li = min((int)(gl.u[2] * (float)S.n_lights), S.n_lights - 1);
         const RtLight& Lt = S.lights[li];
         vec3 dir, Li = mk(0.f, 0.f, 0.f);
@@ -1135,7 +1236,7 @@ hipError_t launch_shade_guided(const HostDevScene& s, const PathState& ps, const
 }
 
 template <int MODE>
-__global__ __launch_bounds__(PT_BLOCK) void k_shade(DevScene S, PathState ps, PtParams P, int depth, int qsel)
+__global__ __launch_bounds__(PT_BLOCK) RT_SHADE_OCCUPANCY void k_shade(DevScene S, PathState ps, PtParams P, int depth, int qsel)
 {
     const int n = ps.cnt->q_count[qsel];
     const int32_t* q = ps.queue[qsel];

@@ -127,6 +127,53 @@ class GpuContext:
         p = self.make_params(**params)
         _check(self.lib, self.h, self.lib.rt_render_device(self.h, va, len(views), ctypes.byref(p), ptrs))
 
+    # -- Markov-chain renderers (include/rt_mlt.h) --------------------------------
+    @staticmethod
+    def make_mlt_params(kind=_abi.MLT_METROPOLIS, max_depth=5, rr_start=3, termination_prob=0.5,
+                        large_step_prob=0.1, bootstrap=10000, mutations=16, chains=0, erpt_spp=(2, 2),
+                        erpt_ed_termination=0.1, seed=0):
+        p = _abi.RtMltParams()
+        p.kind, p.max_depth, p.rr_start = int(kind), int(max_depth), int(rr_start)
+        p.termination_prob, p.large_step_prob = float(termination_prob), float(large_step_prob)
+        p.bootstrap, p.mutations, p.chains = int(bootstrap), int(mutations), int(chains)
+        p.erpt_spp_x, p.erpt_spp_y = int(erpt_spp[0]), int(erpt_spp[1])
+        p.erpt_ed_termination, p.seed = float(erpt_ed_termination), int(seed)
+        return p
+
+    @staticmethod
+    def _mlt_result(r):
+        return {k: getattr(r, k) for k, _ in _abi.RtMltResult._fields_}
+
+    def render_mlt(self, camera, film=None, **params):
+        """Metropolis or ERPT render (make_mlt_params keywords) added into a
+        host film (height, width, 3); returns (film, result dict)."""
+        if film is None:
+            film = np.zeros((camera.height, camera.width, 3), np.float32)
+        assert film.dtype == np.float32 and film.flags.c_contiguous and film.shape == (camera.height, camera.width, 3)
+        cam = camera.to_c()
+        p, res = self.make_mlt_params(**params), _abi.RtMltResult()
+        _check(self.lib, self.h, self.lib.rt_render_mlt(self.h, ctypes.byref(cam), ctypes.byref(p), film.ctypes.data,
+                                                        ctypes.byref(res)))
+        return film, self._mlt_result(res)
+
+    def render_mlt_device(self, camera, film_ptr, **params):
+        cam = camera.to_c()
+        p, res = self.make_mlt_params(**params), _abi.RtMltResult()
+        _check(self.lib, self.h, self.lib.rt_render_mlt_device(self.h, ctypes.byref(cam), ctypes.byref(p),
+                                                               ctypes.c_void_p(film_ptr), ctypes.byref(res)))
+        return self._mlt_result(res)
+
+    def integrate_samples(self, camera, samples, max_depth=5, rr_start=3, termination_prob=0.0):
+        """Radiance of paths given by primary samples (n, 2*max_depth+1, 4) float32."""
+        x = np.ascontiguousarray(samples, np.float32)
+        assert x.ndim == 3 and x.shape[1:] == (2 * max_depth + 1, 4)
+        out = np.zeros((len(x), 3), np.float32)
+        cam = camera.to_c()
+        _check(self.lib, self.h, self.lib.rt_integrate_samples(self.h, ctypes.byref(cam), int(max_depth), int(rr_start),
+                                                               float(termination_prob), x.ctypes.data, len(x),
+                                                               out.ctypes.data))
+        return out
+
     def set_launch_config(self, path_slots=0, block=0, blocks_per_cu=0, use_graph=0, count_traversal=0,
                           guide_epochs=0):
         c = _abi.RtLaunchConfig(int(path_slots), int(block), int(blocks_per_cu), int(use_graph), int(count_traversal),
@@ -239,3 +286,39 @@ class PerPixelRenderer:
     def render(self, scene):
         cam = scene.camera
         return self.integrator.render_views(scene, [(cam, self.x_samples, self.y_samples)], seed=self.seed)[0]
+
+
+class MetropolisRenderer:
+    """MetropolisRenderer (src/metropolisrenderer.cpp:56-212) on the GPU: the
+    met-* options of the reference's command line, chains run side by side."""
+
+    name = "metropolis"
+
+    def __init__(self, large_step_prob=0.1, termination_prob=0.5, bootstrap=10000, mutations=16, chains=0,
+                 seed=0, max_depth=5, context=None):
+        self.params = dict(kind=_abi.MLT_METROPOLIS, large_step_prob=large_step_prob,
+                           termination_prob=termination_prob, bootstrap=bootstrap, mutations=mutations,
+                           chains=chains, seed=seed, max_depth=max_depth)
+        self.ctx = context or GpuContext()
+        self._scene = None
+        self.result = None
+
+    def render(self, scene):
+        if scene is not self._scene:
+            self.ctx.upload(scene)
+            self._scene = scene
+        film, self.result = self.ctx.render_mlt(scene.camera, **self.params)
+        return film
+
+
+class EnergyRedistributionRenderer(MetropolisRenderer):
+    """EnergyRedistributionRenderer (src/energyredistributionrenderer.cpp:38-147)
+    on the GPU: erpt-mutations chains steps, erpt-x/y-samples initial samples
+    per pixel."""
+
+    name = "energyredist"
+
+    def __init__(self, mutations=100, x_samples=2, y_samples=2, seed=0, max_depth=5, context=None):
+        super().__init__(seed=seed, max_depth=max_depth, context=context)
+        self.params = dict(kind=_abi.MLT_ERPT, mutations=mutations, erpt_spp=(x_samples, y_samples), seed=seed,
+                           max_depth=max_depth)
