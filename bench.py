@@ -49,6 +49,7 @@ def parse():
     ap.add_argument("--no-batched", action="store_true", help="skip the 32-view batched render")
     ap.add_argument("--no-scene-file", action="store_true", help="skip the scene-file front-end timing")
     ap.add_argument("--no-film", action="store_true", help="skip the tonemapper and film-writer timing")
+    ap.add_argument("--no-mlt", action="store_true", help="skip the Metropolis and ERPT renders")
     ap.add_argument("--room-res", type=str, default="480x270")
     ap.add_argument("--room-subdiv", type=int, default=64, help="octahedron segments per edge of the 8 objects")
     ap.add_argument("--room-seconds", type=float, default=2.0, help="wall-time budget of each estimator")
@@ -362,6 +363,51 @@ def film_outputs(ctx, film):
     }
 
 
+def markov_chains(ctx, scene, arrays, cpu=True):
+    """The Markov-chain renderers with the reference's default options on the
+    benchmark scene (MAX_DEPTH 5): Metropolis with 16 mutations per pixel and
+    ERPT with 100 mutations per chain from 2x2 initial samples per pixel.
+    Throughput = proposals (path evaluations) per second; the oracle runs a
+    bounded sample of the same Metropolis workload on the host cores."""
+    import torch
+
+    from renderer_amd import MLT_ERPT, MLT_METROPOLIS
+
+    cam = scene.camera
+    film = torch.zeros((cam.height, cam.width, 3), dtype=torch.float32, device="cuda:0")
+    out = {}
+    for name, kw in (("metropolis", dict(kind=MLT_METROPOLIS, mutations=16, bootstrap=10000, seed=1)),
+                     ("erpt", dict(kind=MLT_ERPT, mutations=100, erpt_spp=(2, 2), seed=1))):
+        ctx.render_mlt_device(cam, film.data_ptr(), **kw)  # warm-up
+        film.zero_()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = ctx.render_mlt_device(cam, film.data_ptr(), **kw)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        paths = r["mutations"] + r["seed_paths"]
+        out[name] = {"mutations": r["mutations"], "chains": r["chains"], "seed_paths": r["seed_paths"],
+                     "acceptance": round(r["accepted"] / max(1, r["mutations"]), 4), "seconds": round(wall, 3),
+                     "mutations_per_s": round(r["mutations"] / wall, 1), "paths_per_s": round(paths / wall, 1),
+                     "mrays_per_s": round(r["rays"] / wall / 1e6, 1)}
+        log(f"{name}: {r['mutations']} mutations in {wall:.2f} s, {r['rays'] / wall / 1e6:.0f} Mrays/s")
+    del film
+    if cpu:
+        from oracle.oracle import Oracle
+        from renderer_amd import Camera
+        threads = os.cpu_count() or 1
+        orc = Oracle(arrays, accel=1)
+        small = Camera(cam.cop, cam.look_at, cam.up, cam.fov, 128, 128)
+        t0 = time.time()
+        _, r = orc.render_mlt(small, threads=threads, kind=MLT_METROPOLIS, mutations=16, bootstrap=10000, seed=1)
+        dt = time.time() - t0
+        orc.close()
+        out["metropolis"]["cpu_oracle"] = {"mutations_per_s": round(r["mutations"] / dt, 1), "cores": threads,
+                                           "sample": f"128x128, 16 mutations per pixel ({r['mutations']} "
+                                                     f"mutations, {r['seed_paths']} bootstrap paths) in {dt:.1f} s"}
+    return out
+
+
 def cpu_baseline(arrays, scene, budget_s):
     """The oracle (the reference's algorithms: leaf-1 BVH visiting both
     children, per-sample critical section) rendering whole frames of the same
@@ -459,6 +505,11 @@ def main():
         log("copy calibration: 1 GiB read + 1 GiB written")
         del a, b
 
+    mlt = None
+    if not args.no_mlt:
+        log("Metropolis and ERPT")
+        mlt = markov_chains(ctx, scene, arrays, cpu=not args.no_cpu)
+
     fout = None
     if not args.no_film:
         log("film outputs")
@@ -533,6 +584,7 @@ def main():
         "batched_views": batched,
         "scene_file": sfile,
         "film_outputs": fout,
+        "markov_chains": mlt,
         "equal_time_guided_vs_plain": guided,
         "equal_time_guided_vs_plain_env": env,
     }

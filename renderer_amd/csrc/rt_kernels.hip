@@ -50,6 +50,14 @@ static_assert(sizeof(HostDevScene) == sizeof(DevScene), "scene view layouts diff
 #ifndef RT_SHADE_WAVES
 #define RT_SHADE_WAVES 4
 #endif
+#ifndef RT_SHADE_GUIDED_WAVES
+#define RT_SHADE_GUIDED_WAVES 0
+#endif
+#if RT_SHADE_GUIDED_WAVES > 0
+#define RT_SHADE_GUIDED_OCCUPANCY __attribute__((amdgpu_waves_per_eu(RT_SHADE_GUIDED_WAVES, 8)))
+#else
+#define RT_SHADE_GUIDED_OCCUPANCY
+#endif
 #if RT_SHADE_WAVES > 0
 #define RT_SHADE_OCCUPANCY __attribute__((amdgpu_waves_per_eu(RT_SHADE_WAVES, 8)))
 #else
@@ -1185,7 +1193,7 @@ __device__ void shade_guided(const DevScene& S, const PathState& ps, const Guide
     }
 }
 
-__global__ __launch_bounds__(PT_BLOCK) void k_shade_guided(DevScene S, PathState ps, GuideState G, PtParams P,
+__global__ __launch_bounds__(PT_BLOCK) RT_SHADE_GUIDED_OCCUPANCY void k_shade_guided(DevScene S, PathState ps, GuideState G, PtParams P,
                                                           int depth, int qsel)
 {
     const int n = ps.cnt->q_count[qsel];

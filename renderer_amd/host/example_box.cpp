@@ -5,10 +5,12 @@
  *
  *   example_box <out.raw> [width height spp_x spp_y mode max_depth seed]
  *   example_box <out.raw> --scene <file.scene> [spp_x spp_y mode max_depth seed]
+ *   example_box <out.raw> --mlt <metropolis|energyredist> [width height mutations seed]
  *
  * The second form reads the scene and its camera from a scene file
- * (SceneFile, the reference's scene language).  Writes height * width * 3
- * float32 RGB and prints the mean radiance.
+ * (SceneFile, the reference's scene language); the third renders the box
+ * with MetropolisRenderer or EnergyRedistributionRenderer.  Writes
+ * height * width * 3 float32 RGB and prints the mean radiance.
  */
 #include <cmath>
 #include <cstdio>
@@ -33,6 +35,24 @@ static int write_film(const char* path, const std::vector<float>& film)
     return 0;
 }
 
+/* the box of renderer_amd/scenes.py box_scene */
+static rtgpu::Camera box(rtgpu::SceneBuilder& s, int w, int h)
+{
+    int grey = s.diffuse({0.75f, 0.75f, 0.75f});
+    int red = s.diffuse({0.63f, 0.06f, 0.05f});
+    int green = s.diffuse({0.14f, 0.45f, 0.09f});
+    int mirror = s.mirror();
+    s.areaLight({0.375f, 0.999f, 0.375f}, {0.25f, 0, 0}, {0, 0, 0.25f}, {17, 12, 4});
+    s.quad({0, 0, 0}, {1, 0, 0}, {1, 0, 1}, {0, 0, 1}, grey);
+    s.quad({0, 1, 0}, {0, 1, 1}, {1, 1, 1}, {1, 1, 0}, grey);
+    s.quad({0, 0, 0}, {0, 1, 0}, {1, 1, 0}, {1, 0, 0}, grey);
+    s.quad({0, 0, 0}, {0, 0, 1}, {0, 1, 1}, {0, 1, 0}, red);
+    s.quad({1, 0, 0}, {1, 1, 0}, {1, 1, 1}, {1, 0, 1}, green);
+    s.sphere({0.3f, 0.2f, 0.62f}, 0.2f, grey);
+    s.sphere({0.7f, 0.2f, 0.35f}, 0.2f, mirror);
+    return rtgpu::Camera{{0.5f, 0.5f, 2.4f}, {0.5f, 0.5f, 0.f}, {0, 1, 0}, (float)(40.0 * M_PI / 180.0), w, h};
+}
+
 int main(int argc, char** argv)
 {
     if (argc < 2) {
@@ -55,6 +75,33 @@ int main(int argc, char** argv)
             return 1;
         }
     }
+    if (argc > 3 && std::string(argv[2]) == "--mlt") {
+        const std::string kind = argv[3];
+        int w = argc > 4 ? std::atoi(argv[4]) : 64, h = argc > 5 ? std::atoi(argv[5]) : 64;
+        int muts = argc > 6 ? std::atoi(argv[6]) : 16;
+        unsigned long seed = argc > 7 ? std::strtoul(argv[7], nullptr, 10) : 3;
+        try {
+            rtgpu::SceneBuilder s;
+            rtgpu::Camera cam = box(s, w, h);
+            std::vector<float> film;
+            if (kind == "metropolis") {
+                rtgpu::MetropolisRenderer r(0.1f, 0.5f, 10000, muts, seed);
+                r.setScene(s);
+                film = r.render(cam);
+            } else if (kind == "energyredist") {
+                rtgpu::EnergyRedistributionRenderer r(muts, 2, 2, seed);
+                r.setScene(s);
+                film = r.render(cam);
+            } else {
+                std::fprintf(stderr, "unknown renderer %s\n", kind.c_str());
+                return 2;
+            }
+            return write_film(argv[1], film);
+        } catch (const rtgpu::RtError& e) {
+            std::fprintf(stderr, "error %d: %s\n", e.status, e.what());
+            return 1;
+        }
+    }
     int w = argc > 2 ? std::atoi(argv[2]) : 64, h = argc > 3 ? std::atoi(argv[3]) : 64;
     int sx = argc > 4 ? std::atoi(argv[4]) : 2, sy = argc > 5 ? std::atoi(argv[5]) : 2;
     int mode = argc > 6 ? std::atoi(argv[6]) : RT_MODE_UNBIASED, depth = argc > 7 ? std::atoi(argv[7]) : 8;
@@ -62,19 +109,7 @@ int main(int argc, char** argv)
     using rtgpu::Vec3;
     try {
         rtgpu::SceneBuilder s;
-        int grey = s.diffuse({0.75f, 0.75f, 0.75f});
-        int red = s.diffuse({0.63f, 0.06f, 0.05f});
-        int green = s.diffuse({0.14f, 0.45f, 0.09f});
-        int mirror = s.mirror();
-        s.areaLight({0.375f, 0.999f, 0.375f}, {0.25f, 0, 0}, {0, 0, 0.25f}, {17, 12, 4});
-        s.quad({0, 0, 0}, {1, 0, 0}, {1, 0, 1}, {0, 0, 1}, grey);
-        s.quad({0, 1, 0}, {0, 1, 1}, {1, 1, 1}, {1, 1, 0}, grey);
-        s.quad({0, 0, 0}, {0, 1, 0}, {1, 1, 0}, {1, 0, 0}, grey);
-        s.quad({0, 0, 0}, {0, 0, 1}, {0, 1, 1}, {0, 1, 0}, red);
-        s.quad({1, 0, 0}, {1, 1, 0}, {1, 1, 1}, {1, 0, 1}, green);
-        s.sphere({0.3f, 0.2f, 0.62f}, 0.2f, grey);
-        s.sphere({0.7f, 0.2f, 0.35f}, 0.2f, mirror);
-        rtgpu::Camera cam{{0.5f, 0.5f, 2.4f}, {0.5f, 0.5f, 0.f}, {0, 1, 0}, (float)(40.0 * M_PI / 180.0), w, h};
+        rtgpu::Camera cam = box(s, w, h);
 
         rtgpu::GpuUniDiPathTracingIntegrator integrator(0.5f, mode, depth);
         integrator.setScene(s);

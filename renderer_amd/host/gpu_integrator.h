@@ -17,6 +17,12 @@
  *   SceneFile                     a scene file read by the library's front end
  *                                 (SceneGrammar::parse + resolveVars + buildScene,
  *                                 src/main.cpp:150-182; --dump-bvh)
+ *   MetropolisRenderer            MetropolisRenderer::render
+ *                                 (src/metropolisrenderer.cpp:56-212) with the
+ *                                 met-* options, chains side by side on the GPU
+ *   EnergyRedistributionRenderer  EnergyRedistributionRenderer::render
+ *                                 (src/energyredistributionrenderer.cpp:38-147)
+ *                                 with the erpt-* options
  *
  * Header-only; link with -lrt_hip.  Errors throw RtError (C++ callers) while
  * the library itself only returns status codes.
@@ -30,6 +36,7 @@
 #include <vector>
 
 #include "../../include/rt_abi.h"
+#include "../../include/rt_mlt.h"
 #include "../../include/rt_scene_file.h"
 
 namespace rtgpu {
@@ -291,6 +298,74 @@ private:
     Integrator& integ_;
     int sx_, sy_;
     unsigned long seed_;
+};
+
+/* The Markov-chain renderers share a context and the uploaded scene. */
+class MarkovChainRenderer {
+public:
+    explicit MarkovChainRenderer(int device = 0) { check(nullptr, rt_create(device, &ctx_), "rt_create"); }
+    virtual ~MarkovChainRenderer() { rt_destroy(ctx_); }
+    MarkovChainRenderer(const MarkovChainRenderer&) = delete;
+    MarkovChainRenderer& operator=(const MarkovChainRenderer&) = delete;
+    void setScene(const SceneBuilder& scene)
+    {
+        rt_scene_desc d = scene.desc();
+        check(ctx_, rt_scene_upload(ctx_, &d), "rt_scene_upload");
+    }
+    void setScene(const rt_scene_desc& d) { check(ctx_, rt_scene_upload(ctx_, &d), "rt_scene_upload"); }
+    /* a new film (height * width * 3 RGB floats) */
+    std::vector<float> render(const Camera& cam)
+    {
+        std::vector<float> film((size_t)cam.width * cam.height * 3, 0.f);
+        rt_camera c = cam.c();
+        check(ctx_, rt_render_mlt(ctx_, &c, &p_, film.data(), &last_), "rt_render_mlt");
+        return film;
+    }
+    const rt_mlt_result& lastResult() const { return last_; }
+    void stop() { rt_request_stop(ctx_, 1); }
+    rt_ctx* context() { return ctx_; }
+
+protected:
+    rt_ctx* ctx_ = nullptr;
+    rt_mlt_params p_{};
+    rt_mlt_result last_{};
+};
+
+/* met-large-step-prob, met-termination-prob, met-bootstrap, met-mutations, seed */
+class MetropolisRenderer : public MarkovChainRenderer {
+public:
+    MetropolisRenderer(float largeStepProb = 0.1f, float terminationProb = 0.5f, int bootstrap = 10000,
+                       int mutations = 16, unsigned long seed = 0, int chains = 0, int device = 0)
+        : MarkovChainRenderer(device)
+    {
+        p_.kind = RT_MLT_METROPOLIS;
+        p_.max_depth = 5;
+        p_.rr_start = 3;
+        p_.large_step_prob = largeStepProb;
+        p_.termination_prob = terminationProb;
+        p_.bootstrap = bootstrap;
+        p_.mutations = mutations;
+        p_.chains = chains;
+        p_.seed = (uint32_t)seed;
+    }
+};
+
+/* erpt-mutations, erpt-x-samples, erpt-y-samples, seed */
+class EnergyRedistributionRenderer : public MarkovChainRenderer {
+public:
+    EnergyRedistributionRenderer(int mutations = 100, int xSamples = 2, int ySamples = 2, unsigned long seed = 0,
+                                 int device = 0)
+        : MarkovChainRenderer(device)
+    {
+        p_.kind = RT_MLT_ERPT;
+        p_.max_depth = 5;
+        p_.rr_start = 3;
+        p_.mutations = mutations;
+        p_.erpt_spp_x = xSamples;
+        p_.erpt_spp_y = ySamples;
+        p_.erpt_ed_termination = 0.1f;
+        p_.seed = (uint32_t)seed;
+    }
 };
 
 }  // namespace rtgpu

@@ -45,3 +45,27 @@ def test_cpp_scene_file_matches_python_host(gpu_ctx, tmp_path):
     gpu_ctx.upload(sc.flatten())
     py = gpu_ctx.render([(sc.camera, 1, 2)], mode=MODE_UNBIASED, max_depth=8, rr_start=3, rr_prob=0.5, seed=7)[0]
     assert np.array_equal(cpp, py)
+
+
+@pytest.mark.parametrize("kind", ["metropolis", "energyredist"])
+def test_cpp_markov_chain_renderers_match_python_host(gpu_ctx, tmp_path, kind):
+    """example_box --mlt: MetropolisRenderer / EnergyRedistributionRenderer of
+    the C++ host render the box exactly as the Python host does."""
+    from renderer_amd import EnergyRedistributionRenderer, MetropolisRenderer
+
+    if not os.path.exists(EXAMPLE):
+        pytest.fail("renderer_amd/host/example_box is not built (run __graft_entry__.build())")
+    out = tmp_path / "film.raw"
+    muts = 16 if kind == "metropolis" else 12
+    r = subprocess.run([EXAMPLE, str(out), "--mlt", kind, "40", "32", str(muts), "9"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    cpp = np.fromfile(out, np.float32).reshape(32, 40, 3)
+    sc = scenes.box_scene(40, 32)
+    if kind == "metropolis":
+        rend = MetropolisRenderer(mutations=muts, seed=9, context=gpu_ctx)
+    else:
+        rend = EnergyRedistributionRenderer(mutations=muts, seed=9, context=gpu_ctx)
+    py = rend.render(sc)
+    assert np.array_equal(cpp, py)
+    assert cpp.mean() > 0
