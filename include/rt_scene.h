@@ -49,6 +49,28 @@ enum rt_light_type {
     RT_LIGHT_CONE = 3
 };
 
+/* Analytic objects (src/plane.cpp, src/axisalignedbox.cpp, src/sphere.cpp,
+ * src/csgoperation.cpp): planes, boxes and CSG trees over spheres and
+ * boxes, each given as a postfix program of nodes.  A leaf pushes its
+ * interval list (the reference's getCSGIntersection), an operation pops two
+ * (left below right) and pushes their combination; the program of a plane is
+ * the plane alone. */
+enum rt_csg_op {
+    RT_CSG_SPHERE = 0,  /* p: centre, radius */
+    RT_CSG_BOX = 1,     /* p: min corner, max corner */
+    RT_CSG_PLANE = 2,   /* p: (a, b, c, d), points x with a x + b y + c z + d = 0; normal (a, b, c) */
+    RT_CSG_ISECT = 3,   /* isect(l, r) */
+    RT_CSG_UNION = 4,   /* union(l, r) */
+    RT_CSG_DIFF = 5,    /* diff(l, r) */
+    RT_CSG_XOR = 6      /* xor(l, r) */
+};
+
+typedef struct rt_csg_node {
+    int32_t op;         /* rt_csg_op */
+    int32_t material;   /* leaves: material index */
+    float p[6];
+} rt_csg_node;
+
 typedef struct rt_material {
     int32_t type;        /* rt_material_type */
     float kd[3];         /* diffuse colour (RGB): Diffuse, Phong, Blinn */
@@ -95,7 +117,18 @@ typedef struct rt_scene_desc {
     int32_t env_width, env_height;
     const float* env_rgb;
     int32_t env_kind;
+
+    /* analytic objects: object i is the program csg_nodes[object_first[i] ..
+     * object_first[i + 1]) (object_first has n_objects + 1 entries) with
+     * primitive ID object_prim_id[i]; a CSG program has at most
+     * RT_CSG_MAX_LEAVES leaves.  n_objects = 0 when there are none. */
+    int32_t n_objects;
+    const rt_csg_node* csg_nodes;
+    const int32_t* object_first;
+    const int32_t* object_prim_id;
 } rt_scene_desc;
+
+#define RT_CSG_MAX_LEAVES 16
 
 /* Pinhole camera, src/camera.h:34-101: eye, look-at, up, vertical field of
  * view in radians, resolution. */

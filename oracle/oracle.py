@@ -41,6 +41,7 @@ def load():
     lib.oc_scene_destroy.argtypes = [P]
     lib.oc_trace_closest.argtypes = [P, I64, P, P, P, P, ctypes.c_int]
     lib.oc_trace_any.argtypes = [P, I64, P, P, P, ctypes.c_int]
+    lib.oc_hit_normals.argtypes = [P, I64, P, P, P]
     lib.oc_intersect_prim.restype = ctypes.c_float
     lib.oc_intersect_prim.argtypes = [P, ctypes.c_int32, P, P]
     lib.oc_render.restype = I64
@@ -122,6 +123,13 @@ class Oracle:
         t = np.empty(n, np.float32)
         self.lib.oc_trace_closest(self.h, n, o.ctypes.data, d.ctypes.data, prim.ctypes.data, t.ctypes.data, threads)
         return prim, t
+
+    def hit_normals(self, o_tmin, d_tmax):
+        o = np.ascontiguousarray(o_tmin, np.float32)
+        d = np.ascontiguousarray(d_tmax, np.float32)
+        out = np.zeros((len(o), 3), np.float32)
+        self.lib.oc_hit_normals(self.h, len(o), o.ctypes.data, d.ctypes.data, out.ctypes.data)
+        return out
 
     def trace_any(self, o_tmin, d_tmax, threads=8):
         o = np.ascontiguousarray(o_tmin, np.float32)

@@ -308,6 +308,159 @@ struct Sphere : Intersectable {
     }
 };
 
+// src/plane.cpp:27-40: t = (P . (o, 1)) / -(d . n), accepted when
+// from <= t <= to; the bounding box is infinite (kept out of the BVH here).
+struct Plane : Intersectable {
+    float P[4];
+    const Material* m;
+    int prim;
+    Plane(const float* p, const Material* mat, int id) : m(mat), prim(id)
+    {
+        for (int k = 0; k < 4; ++k) P[k] = p[k];
+    }
+    Hit intersect(const Ray& ray) const override
+    {
+        float d = -(ray.d.x * P[0] + ray.d.y * P[1] + ray.d.z * P[2]);
+        float u = ray.o.x * P[0] + ray.o.y * P[1] + ray.o.z * P[2] + 1.f * P[3];
+        float t = u / d;
+        if (!ray.in_range(t)) return Hit();
+        return make_hit(t, ray, m, V3(P[0], P[1], P[2]), prim);
+    }
+};
+
+// CSG objects: interval lists (getCSGIntersection) of spheres, boxes and
+// boolean operations, src/sphere.cpp:29-56, src/axisalignedbox.cpp:30-71,
+// src/csgoperation.cpp:26-91, resolved by src/csgobject.cpp:25-44.
+struct CsgParam {
+    float t;
+    V3 n;
+    const Material* m;
+};
+struct CsgNode {
+    Box box;
+    virtual ~CsgNode() {}
+    virtual std::vector<CsgParam> params(const Ray& ray) const = 0;
+};
+struct CsgSphere : CsgNode {
+    V3 c;
+    float r;
+    const Material* m;
+    CsgSphere(V3 c_, float r_, const Material* mat) : c(c_), r(r_), m(mat)
+    {
+        V3 rv(r, r, r);
+        box.mn = c - rv;
+        box.mx = c + rv;
+    }
+    std::vector<CsgParam> params(const Ray& ray) const override
+    {
+        std::vector<CsgParam> out;
+        V3 diff = ray.o - c;
+        float a = length_sq(ray.d);
+        float b = 2 * dot(ray.d, diff);
+        float cc = length_sq(diff) - r * r;
+        float disc = b * b - 4 * a * cc;
+        if (disc >= 0) {
+            float root = std::sqrt(disc);
+            for (float t : {(-b - root) / (2 * a), (-b + root) / (2 * a)})
+                out.push_back({t, (ray.o + t * ray.d) - c, m});
+        }
+        return out;
+    }
+};
+struct CsgBox : CsgNode {
+    const Material* m;
+    CsgBox(V3 mn, V3 mx, const Material* mat) : m(mat)
+    {
+        box.mn = mn;
+        box.mx = mx;
+    }
+    std::vector<CsgParam> params(const Ray& ray) const override
+    {
+        std::vector<CsgParam> out;
+        float tmin = -kInf, tmax = kInf;
+        V3 normal;
+        const float mn[3] = {box.mn.x, box.mn.y, box.mn.z}, mx[3] = {box.mx.x, box.mx.y, box.mx.z};
+        const float o[3] = {ray.o.x, ray.o.y, ray.o.z}, dv[3] = {ray.d.x, ray.d.y, ray.d.z};
+        for (int i = 0; i < 3; ++i) {
+            float a = 1 / dv[i];
+            float n[3] = {0, 0, 0}, t0, t1;
+            if (a >= 0) {
+                t0 = mn[i] - o[i];
+                t1 = mx[i] - o[i];
+                n[i] = -1;
+            } else {
+                t0 = mx[i] - o[i];
+                t1 = mn[i] - o[i];
+                n[i] = 1;
+            }
+            t0 *= a;
+            t1 *= a;
+            if (t0 > tmin) {
+                tmin = t0;
+                normal = V3(n[0], n[1], n[2]);
+            }
+            if (t1 < tmax) tmax = t1;
+            if (tmin > tmax) return out;
+        }
+        out.push_back({tmin, normal, m});
+        out.push_back({tmax, normal, m});
+        return out;
+    }
+};
+struct CsgOp : CsgNode {
+    int op;
+    std::unique_ptr<CsgNode> l, r;
+    CsgOp(int o, std::unique_ptr<CsgNode> a, std::unique_ptr<CsgNode> b) : op(o), l(std::move(a)), r(std::move(b))
+    {
+        box.mn = V3(std::min(l->box.mn.x, r->box.mn.x), std::min(l->box.mn.y, r->box.mn.y),
+                    std::min(l->box.mn.z, r->box.mn.z));
+        box.mx = V3(std::max(l->box.mx.x, r->box.mx.x), std::max(l->box.mx.y, r->box.mx.y),
+                    std::max(l->box.mx.z, r->box.mx.z));
+    }
+    bool inside(bool a, bool b) const
+    {
+        switch (op) {
+        case RT_CSG_ISECT: return a && b;
+        case RT_CSG_UNION: return a || b;
+        case RT_CSG_DIFF: return a && !b;
+        default: return a != b;
+        }
+    }
+    std::vector<CsgParam> params(const Ray& ray) const override
+    {
+        std::vector<CsgParam> L = l->params(ray), R = r->params(ray), out;
+        bool li = false, ri = false;
+        size_t a = 0, b = 0;
+        while (a < L.size() || b < R.size()) {
+            bool left = a < L.size() && (b == R.size() || L[a].t <= R[b].t);
+            bool was = inside(li, ri);
+            if (left) li = !li;
+            else ri = !ri;
+            if (was != inside(li, ri)) out.push_back(left ? L[a] : R[b]);
+            if (left) ++a;
+            else ++b;
+        }
+        return out;
+    }
+};
+struct CsgObject : Intersectable {
+    std::unique_ptr<CsgNode> root;
+    int prim;
+    CsgObject(std::unique_ptr<CsgNode> r, int id) : root(std::move(r)), prim(id) { box = root->box; }
+    Hit intersect(const Ray& ray) const override
+    {
+        std::vector<CsgParam> ps = root->params(ray);
+        if (ps.empty() || ray.from > ps.back().t) return Hit();
+        for (const CsgParam& p : ps) {
+            if (p.t >= ray.from) {
+                if (p.t > ray.to) return Hit();
+                return make_hit(p.t, ray, p.m, p.n, prim);
+            }
+        }
+        return Hit();
+    }
+};
+
 // src/intersectablelist.cpp:30-48: linear scan with ray shortening; ties go
 // to the later component.
 struct List : Intersectable {
@@ -1537,6 +1690,7 @@ void* oc_scene_create(const rt_scene_desc* d, int accel)
     int max_prim = d->n_lights - 1;
     for (int i = 0; i < d->n_triangles; ++i) max_prim = std::max(max_prim, d->tri_prim_id[i]);
     for (int i = 0; i < d->n_spheres; ++i) max_prim = std::max(max_prim, d->sphere_prim_id[i]);
+    for (int i = 0; i < d->n_objects; ++i) max_prim = std::max(max_prim, d->object_prim_id[i]);
     sc->prims.assign(max_prim + 1, nullptr);
 
     std::vector<Intersectable*> root_items;
@@ -1612,6 +1766,37 @@ void* oc_scene_create(const rt_scene_desc* d, int accel)
         sc->owned.emplace_back(sp);
         items.push_back({d->sphere_prim_id[i], sp});
     }
+    // analytic objects from their postfix programs
+    std::vector<Intersectable*> planes;
+    for (int i = 0; i < d->n_objects; ++i) {
+        const int f = d->object_first[i], e = d->object_first[i + 1];
+        const int id = d->object_prim_id[i];
+        if (e - f == 1 && d->csg_nodes[f].op == RT_CSG_PLANE) {
+            auto* pl = new Plane(d->csg_nodes[f].p, mats[d->csg_nodes[f].material], id);
+            sc->owned.emplace_back(pl);
+            sc->prims[id] = pl;
+            planes.push_back(pl);
+            continue;
+        }
+        std::vector<std::unique_ptr<CsgNode>> st;
+        for (int k = f; k < e; ++k) {
+            const rt_csg_node& n = d->csg_nodes[k];
+            if (n.op == RT_CSG_SPHERE) {
+                st.emplace_back(new CsgSphere(V3(n.p[0], n.p[1], n.p[2]), n.p[3], mats[n.material]));
+            } else if (n.op == RT_CSG_BOX) {
+                st.emplace_back(new CsgBox(V3(n.p[0], n.p[1], n.p[2]), V3(n.p[3], n.p[4], n.p[5]), mats[n.material]));
+            } else {
+                auto r = std::move(st.back());
+                st.pop_back();
+                auto l = std::move(st.back());
+                st.pop_back();
+                st.emplace_back(new CsgOp(n.op, std::move(l), std::move(r)));
+            }
+        }
+        auto* obj = new CsgObject(std::move(st.back()), id);
+        sc->owned.emplace_back(obj);
+        items.push_back({id, obj});
+    }
     std::stable_sort(items.begin(), items.end(), [](const Item& a, const Item& b) { return a.id < b.id; });
     for (auto& it : items) {
         sc->prims[it.id] = it.obj;
@@ -1626,6 +1811,15 @@ void* oc_scene_create(const rt_scene_desc* d, int accel)
         sc->root = l;
     } else {
         sc->root = build_bvh(root_items, sc->owned);
+    }
+    // planes have infinite bounds: tested after the rest, in a list (ties
+    // with the rest go to the plane)
+    if (!planes.empty()) {
+        auto* l = new List();
+        if (sc->root) l->items.push_back(sc->root);
+        for (auto* p : planes) l->items.push_back(p);
+        sc->owned.emplace_back(l);
+        sc->root = l;
     }
     return sc;
 }
@@ -1656,6 +1850,16 @@ void oc_trace_any(void* s, int64_t n, const float* o, const float* d, uint8_t* o
     for (int64_t i = 0; i < n; ++i) {
         Hit h = sc.root ? sc.root->intersect(ray_from(o, d, i)) : Hit();
         occ[i] = h.hits() ? 1 : 0;
+    }
+}
+
+void oc_hit_normals(void* s, int64_t n, const float* o, const float* d, float* normals)
+{
+    const Scene& sc = *static_cast<Scene*>(s);
+    for (int64_t i = 0; i < n; ++i) {
+        Hit h = sc.root ? sc.root->intersect(ray_from(o, d, i)) : Hit();
+        V3 v = h.hits() ? h.n : V3();
+        normals[3 * i] = v.x, normals[3 * i + 1] = v.y, normals[3 * i + 2] = v.z;
     }
 }
 

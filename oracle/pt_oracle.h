@@ -55,6 +55,10 @@ void oc_trace_closest(void* scene, int64_t n, const float* o_tmin, const float* 
 void oc_trace_any(void* scene, int64_t n, const float* o_tmin, const float* d_tmax,
                   uint8_t* occluded, int threads);
 
+/* Surface normal (normalised, as HitRecord keeps it) of each ray's closest
+ * hit, 0 for a miss. */
+void oc_hit_normals(void* scene, int64_t n, const float* o_tmin, const float* d_tmax, float* normals);
+
 /* Single-primitive test with the reference arithmetic, used to judge hit
  * mismatches at shared edges: returns t or NaN. */
 float oc_intersect_prim(void* scene, int32_t prim, const float* o_tmin, const float* d_tmax);

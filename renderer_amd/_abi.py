@@ -39,7 +39,19 @@ class RtSceneDesc(ctypes.Structure):
         ("env_height", ctypes.c_int32),
         ("env_rgb", ctypes.c_void_p),
         ("env_kind", ctypes.c_int32),
+        ("n_objects", ctypes.c_int32),
+        ("csg_nodes", ctypes.c_void_p),
+        ("object_first", ctypes.c_void_p),
+        ("object_prim_id", ctypes.c_void_p),
     ]
+
+
+class RtCsgNode(ctypes.Structure):
+    _fields_ = [("op", ctypes.c_int32), ("material", ctypes.c_int32), ("p", ctypes.c_float * 6)]
+
+
+CSG_SPHERE, CSG_BOX, CSG_PLANE, CSG_ISECT, CSG_UNION, CSG_DIFF, CSG_XOR = range(7)
+CSG_MAX_LEAVES = 16
 
 
 class RtCamera(ctypes.Structure):

@@ -43,6 +43,7 @@ struct rt_ctx {
     bool has_scene = false;
     HostDevScene scene{};
     DevBuf nodes, prims, prim_id, mat, vnormals, geo, mats, lights, env_rgb, env_row, env_col, env_p, probe_rgb;
+    DevBuf obj_nodes, obj_first;
     int bvh_depth = 0;
     int bvh_stack = 0;  /* most stack entries a traversal can need */
 
@@ -221,7 +222,9 @@ int rt_destroy(rt_ctx* c)
                      &c->env_rgb, &c->env_row, &c->env_col, &c->env_p, &c->probe_rgb, &c->rb_o, &c->rb_d, &c->rb_out0,
                      &c->rb_out1, &c->stack_ovf, &c->tm_list, &c->tm_count, &c->counters, &c->views, &c->tiles, &c->passes, &c->films, &c->g_theta,
                      &c->g_w16, &c->g_m, &c->g_v, &c->g_grad, &c->g_acc, &c->g_count, &c->g_rec, &c->g_aux,
-                     &c->g_open, &c->g_out, &c->g_tmp_rec, &c->g_tmp_y};
+                     &c->g_open, &c->g_out, &c->g_tmp_rec, &c->g_tmp_y, &c->obj_nodes, &c->obj_first,
+                     &c->mlt_cur, &c->mlt_prop, &c->mlt_val, &c->mlt_energy, &c->mlt_acc, &c->mlt_cnt,
+                     &c->mlt_seed, &c->mlt_cdf, &c->mlt_nch, &c->mlt_src, &c->mlt_view};
     for (DevBuf* b : all) release(*b);
     for (auto& b : c->ps_bufs) release(b);
     for (auto e : c->ev_pool) hipEventDestroy(e);
@@ -387,6 +390,75 @@ int rt_scene_upload(rt_ctx* c, const rt_scene_desc* d)
             mat[g] = -(i + 1);
         }
     }
+    /* analytic objects: validated postfix programs; every leaf gets a shading
+     * index after the BVH primitives */
+    std::vector<float> onodes;
+    std::vector<int32_t> ofirst;
+    int n_leaves = 0;
+    if (d->n_objects < 0) return set_err(c, RT_ERR_INVALID, "negative object count");
+    if (d->n_objects > 0) {
+        if (!d->csg_nodes || !d->object_first || !d->object_prim_id)
+            return set_err(c, RT_ERR_INVALID, "missing object array");
+        if (d->object_first[0] != 0) return set_err(c, RT_ERR_INVALID, "object_first[0] must be 0");
+        for (int i = 0; i < d->n_objects; ++i) {
+            const int f = d->object_first[i], e = d->object_first[i + 1];
+            if (e <= f) return set_err(c, RT_ERR_INVALID, "empty object program");
+            int depth = 0, leaves = 0;
+            for (int k = f; k < e; ++k) {
+                const rt_csg_node& nd = d->csg_nodes[k];
+                if (nd.op == RT_CSG_SPHERE || nd.op == RT_CSG_BOX || nd.op == RT_CSG_PLANE) {
+                    if (nd.material < 0 || nd.material >= d->n_materials)
+                        return set_err(c, RT_ERR_INVALID, "object material out of range");
+                    if (nd.op == RT_CSG_PLANE && e - f != 1)
+                        return set_err(c, RT_ERR_INVALID, "a plane is an object of its own, not a CSG operand");
+                    if (nd.op == RT_CSG_SPHERE && !(nd.p[3] > 0.f))
+                        return set_err(c, RT_ERR_INVALID, "sphere radius must be positive");
+                    ++depth;
+                    ++leaves;
+                } else if (nd.op >= RT_CSG_ISECT && nd.op <= RT_CSG_XOR) {
+                    if (depth < 2) return set_err(c, RT_ERR_INVALID, "CSG operation without two operands");
+                    --depth;
+                } else {
+                    return set_err(c, RT_ERR_INVALID, "unknown CSG node");
+                }
+            }
+            if (depth != 1) return set_err(c, RT_ERR_INVALID, "CSG program does not reduce to one object");
+            if (leaves > RT_CSG_MAX_LEAVES) return set_err(c, RT_ERR_UNSUPPORTED, "CSG object with too many leaves");
+            n_leaves += leaves;
+        }
+        const int n_nodes = d->object_first[d->n_objects];
+        onodes.assign((size_t)8 * n_nodes, 0.f);
+        ofirst.assign(d->object_first, d->object_first + d->n_objects + 1);
+        geo.resize((size_t)4 * (G + n_leaves), 0.f);
+        pid.resize(G + n_leaves);
+        mat.resize(G + n_leaves);
+        int g = G;
+        for (int i = 0; i < d->n_objects; ++i)
+            for (int k = d->object_first[i]; k < d->object_first[i + 1]; ++k) {
+                const rt_csg_node& nd = d->csg_nodes[k];
+                float* o = &onodes[8 * (size_t)k];
+                for (int j = 0; j < 6; ++j) o[j] = nd.p[j];
+                std::memcpy(&o[6], &nd.op, 4);
+                int32_t leaf_g = -1;
+                if (nd.op <= RT_CSG_PLANE) {
+                    leaf_g = g++;
+                    pid[leaf_g] = d->object_prim_id[i];
+                    mat[leaf_g] = nd.material;
+                    float* q = &geo[4 * (size_t)leaf_g];
+                    if (nd.op == RT_CSG_SPHERE) {
+                        for (int j = 0; j < 4; ++j) q[j] = nd.p[j];
+                    } else if (nd.op == RT_CSG_PLANE) {
+                        /* HitRecord normalises the normal (src/hitrecord.h:74) */
+                        qt_normalize(nd.p, q);
+                        q[3] = -2.f;
+                    } else {
+                        q[3] = -1.f;
+                    }
+                }
+                std::memcpy(&o[7], &leaf_g, 4);
+            }
+    }
+
     rtb::BuildResult br;
     int br_status = rtb::build(refs, br);
     if (br_status != 0)
@@ -521,6 +593,8 @@ int rt_scene_upload(rt_ctx* c, const rt_scene_desc* d)
         (r = upload(c, c->prim_id, pid.data(), pid.size() * 4)) || (r = upload(c, c->mat, mat.data(), mat.size() * 4)) ||
         (r = upload(c, c->geo, geo.data(), geo.size() * 4)) ||
         (r = upload(c, c->vnormals, vn.data(), vn.size() * 4)) ||
+        (r = upload(c, c->obj_nodes, onodes.data(), onodes.size() * 4)) ||
+        (r = upload(c, c->obj_first, ofirst.data(), ofirst.size() * 4)) ||
         (r = upload(c, c->mats, mats.data(), mats.size() * sizeof(RtMaterial))) ||
         (r = upload(c, c->lights, lights.data(), lights.size() * sizeof(RtLight))) ||
         (r = upload(c, c->env_rgb, ergb.data(), ergb.size() * 4)) ||
@@ -553,6 +627,10 @@ int rt_scene_upload(rt_ctx* c, const rt_scene_desc* d)
     s.probe_w = PW;
     s.probe_h = PH;
     s.probe_rgb = c->probe_rgb.p;
+    s.obj_nodes = onodes.empty() ? nullptr : c->obj_nodes.p;
+    s.obj_first = ofirst.empty() ? nullptr : (const int32_t*)c->obj_first.p;
+    s.n_obj = d->n_objects;
+    s.g_obj = G;
     c->has_scene = true;
     c->bvh_depth = br.depth;
     c->bvh_stack = br.stack_need;

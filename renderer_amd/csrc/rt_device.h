@@ -104,6 +104,12 @@ struct DevScene {
     int32_t env_kind;          /* 0: lat-long texels; 1: angular probe (values from probe_rgb) */
     int32_t probe_w, probe_h;
     const float4* probe_rgb;   /* probe texels (rgb * coefficient) */
+    /* analytic objects outside the BVH: postfix programs of 2-float4 nodes
+     * (p0..p3), (p4, p5, op, leaf g); object i = nodes [obj_first[i],
+     * obj_first[i + 1]); the leaves have shading indices from g_obj on */
+    const float4* obj_nodes;
+    const int32_t* obj_first;
+    int32_t n_obj, g_obj;
 };
 
 /* ---------------------------------------------------------------------------
@@ -171,7 +177,166 @@ __device__ __forceinline__ bool hit_sphere(vec3 o, vec3 d, float4 r0, float4 r1,
     t = t0 >= tmin ? t0 : t1;
     return t <= tmax && t < best;
 }
+/* Plane::intersect, src/plane.cpp:27-40: t = (P . (o, 1)) / -(d . n),
+ * accepted when from <= t <= to (t = +inf for a parallel ray counts, as in
+ * the reference). */
+__device__ __forceinline__ bool hit_plane(vec3 o, vec3 d, float4 P, float tmin, float tmax, float& t)
+{
+    float dd = -(d.x * P.x + d.y * P.y + d.z * P.z);
+    float u = o.x * P.x + o.y * P.y + o.z * P.z + 1.f * P.w;
+    t = u / dd;
+    return tmin <= t && t <= tmax;
+}
+
+/* AxisAlignedBox::getCSGIntersection, src/axisalignedbox.cpp:30-71: entry
+ * and exit parameters; both carry the normal of the entry slab (code
+ * axis * 2 + (normal > 0)). */
+__device__ __forceinline__ bool box_params(vec3 o, vec3 d, float4 lo, float4 hi, float& t0, float& t1, int& code)
+{
+    const float mn[3] = {lo.x, lo.y, lo.z}, mx[3] = {lo.w, hi.x, hi.y};
+    const float oo[3] = {o.x, o.y, o.z}, dv[3] = {d.x, d.y, d.z};
+    float tn = -RT_INF, tf = RT_INF;
+    code = -1;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        float a = 1.f / dv[i];
+        float x0, x1;
+        int c;
+        if (a >= 0.f) {
+            x0 = mn[i] - oo[i];
+            x1 = mx[i] - oo[i];
+            c = 2 * i;
+        } else {
+            x0 = mx[i] - oo[i];
+            x1 = mn[i] - oo[i];
+            c = 2 * i + 1;
+        }
+        x0 *= a;
+        x1 *= a;
+        if (x0 > tn) {
+            tn = x0;
+            code = c;
+        }
+        if (x1 < tf) tf = x1;
+        if (tn > tf) return false;
+    }
+    t0 = tn;
+    t1 = tf;
+    return true;
+}
+
+/* Sphere::getCSGIntersection, src/sphere.cpp:29-56: both roots */
+__device__ __forceinline__ bool sphere_params(vec3 o, vec3 d, float4 c, float& t0, float& t1)
+{
+    vec3 diff = o - xyz(c);
+    float a = dot_x(d, d);
+    float b = 2.f * dot_x(d, diff);
+    float cc = dot_x(diff, diff) - c.w * c.w;
+    float disc = b * b - 4.f * a * cc;
+    if (!(disc >= 0.f)) return false;
+    float root = sqrtf(disc);
+    t0 = (-b - root) / (2.f * a);
+    t1 = (-b + root) / (2.f * a);
+    return true;
+}
 #pragma clang fp contract(fast)
+
+/* Value of a CSG program for the leaves whose bits are set in `inside`
+ * (CSGIntersection / CSGUnion / CSGDifference / CSGXor::isInside,
+ * src/csgoperation.cpp:73-91). */
+__device__ __forceinline__ bool csg_value(const float4* __restrict__ nodes, int first, int count, uint32_t inside)
+{
+    uint32_t st = 0;
+    int leaf = 0;
+    for (int i = 0; i < count; ++i) {
+        int op = __float_as_int(nodes[2 * (first + i) + 1].z);
+        if (op <= RT_CSG_PLANE) {
+            st = (st << 1) | ((inside >> leaf) & 1u);
+            ++leaf;
+        } else {
+            uint32_t r = st & 1u, l = (st >> 1) & 1u, v;
+            if (op == RT_CSG_ISECT) v = l & r;
+            else if (op == RT_CSG_UNION) v = l | r;
+            else if (op == RT_CSG_DIFF) v = l & (r ^ 1u);
+            else v = l ^ r;
+            st = ((st >> 2) << 1) | v;
+        }
+    }
+    return st & 1u;
+}
+
+/* First hit of object `obj` in [tmin, tmax] (CSGObject::intersect,
+ * src/csgobject.cpp:25-44, on the interval list of CSGOperation::
+ * getCSGIntersection, src/csgoperation.cpp:39-63).  The nested pairwise
+ * merges of the reference (left first on equal t) give the same list as one
+ * sweep over all leaf events ordered by t and then by leaf order, keeping
+ * the events at which the program's value changes: every operation maps
+ * (outside, outside) to outside, so a parent's view of a child is the
+ * child's value.  Returns the leaf's shading index and the box normal code. */
+__device__ __forceinline__ bool obj_hit(const float4* __restrict__ nd, const int32_t* __restrict__ obj_first, int obj,
+                                     vec3 o, vec3 d, float tmin, float tmax, float& t, int& g, float& code)
+{
+    const int first = obj_first[obj], count = obj_first[obj + 1] - first;
+    if (count == 1 && __float_as_int(nd[2 * first + 1].z) == RT_CSG_PLANE) {
+        if (!hit_plane(o, d, nd[2 * first], tmin, tmax, t)) return false;
+        g = __float_as_int(nd[2 * first + 1].w);
+        code = -1.f;
+        return true;
+    }
+    float et[2 * RT_CSG_MAX_LEAVES];
+    int8_t el[2 * RT_CSG_MAX_LEAVES], ec[2 * RT_CSG_MAX_LEAVES];
+    int ne = 0, leaf = 0;
+    for (int i = 0; i < count; ++i) {
+        float4 a = nd[2 * (first + i)], b = nd[2 * (first + i) + 1];
+        int op = __float_as_int(b.z);
+        if (op > RT_CSG_PLANE) continue;
+        float t0, t1;
+        int c = -1;
+        bool any = op == RT_CSG_SPHERE ? sphere_params(o, d, a, t0, t1) : box_params(o, d, a, b, t0, t1, c);
+        if (any) {
+            /* insertion after the events of equal t: stable in leaf order */
+            for (int k = 0; k < 2; ++k) {
+                float tv = k ? t1 : t0;
+                int j = ne++;
+                while (j > 0 && et[j - 1] > tv) {
+                    et[j] = et[j - 1];
+                    el[j] = el[j - 1];
+                    ec[j] = ec[j - 1];
+                    --j;
+                }
+                et[j] = tv;
+                el[j] = (int8_t)leaf;
+                ec[j] = (int8_t)c;
+            }
+        }
+        ++leaf;
+    }
+    uint32_t inside = 0;
+    bool val = false;
+    for (int j = 0; j < ne; ++j) {
+        inside ^= 1u << el[j];
+        bool nv = csg_value(nd, first, count, inside);
+        if (nv == val) continue;
+        val = nv;
+        if (!(et[j] >= tmin)) continue;
+        if (et[j] > tmax) return false;
+        t = et[j];
+        /* the leaf's shading index: leaf k of the program */
+        int k = 0;
+        for (int i = 0; i < count; ++i) {
+            float4 b = nd[2 * (first + i) + 1];
+            if (__float_as_int(b.z) > RT_CSG_PLANE) continue;
+            if (k++ == el[j]) {
+                g = __float_as_int(b.w);
+                break;
+            }
+        }
+        code = (float)ec[j];
+        return true;
+    }
+    return false;
+}
+
 
 /* Leaf word decoding (rt_layout.h). */
 __device__ __forceinline__ void leaf_decode(int32_t w, uint32_t& first, uint32_t& count, uint32_t& kind)
@@ -233,6 +398,35 @@ __device__ __forceinline__ void trav_init(const DevScene& S, Trav& T, vec3 o, ve
     T.nx = T.inv.x >= 0.f ? 0u : 16u;
     T.ny = T.inv.y >= 0.f ? 0u : 16u;
     T.nz = T.inv.z >= 0.f ? 0u : 16u;
+}
+
+/* The analytic objects against a freshly initialised ray (tmax: its range
+ * end): the closest object hit becomes the current hit, which the BVH
+ * traversal then only replaces with a strictly closer one. */
+__device__ __forceinline__ void obj_closest(const DevScene& S, Trav& T, float tmax)
+{
+    for (int i = 0; i < S.n_obj; ++i) {
+        float t, code;
+        int g;
+        if (obj_hit(S.obj_nodes, S.obj_first, i, T.o, T.d, T.tmin, T.g >= 0 ? T.best : tmax, t, g, code) &&
+            (T.g < 0 || t < T.best)) {
+            T.best = t;
+            T.g = g;
+            T.u = code;
+            T.v = 0.f;
+        }
+    }
+}
+
+/* Any object hit in [tmin, tmax] (shadow rays). */
+__device__ __forceinline__ bool obj_any(const DevScene& S, vec3 o, vec3 d, float tmin, float tmax)
+{
+    for (int i = 0; i < S.n_obj; ++i) {
+        float t, code;
+        int g;
+        if (obj_hit(S.obj_nodes, S.obj_first, i, o, d, tmin, tmax, t, g, code)) return true;
+    }
+    return false;
 }
 
 /* Monotone map of a float to uint32 (negative values reversed below the
@@ -445,6 +639,16 @@ __device__ __forceinline__ TraceOut traverse(const DevScene& S, vec3 o, vec3 d, 
     Trav T;
     trav_init(S, T, o, d, tmin, tmax);
     int cn = 0, cp = 0;
+    if (S.n_obj > 0 && tmin <= tmax) {
+        if (ANY) {
+            if (obj_any(S, o, d, tmin, tmax)) {
+                T.g = 0;
+                T.cur = -1;
+            }
+        } else {
+            obj_closest(S, T, tmax);
+        }
+    }
     if (T.cur >= 0)
         while (!trav_step<ANY, COUNT>(S, T, stack, cn, cp)) {
         }
@@ -476,7 +680,14 @@ __device__ __forceinline__ vec3 hit_normal(const DevScene& S, int32_t g, vec3 p,
         return xyz(gg);
     }
     if (g < S.n_tri + S.n_sphere) return norm(p - xyz(gg));
-    return xyz(gg);
+    if (g < S.g_obj) return xyz(gg);
+    /* object leaves: sphere (centre, radius), plane (unit normal, -2), box
+     * (-1: the normal of the entry slab, coded in u) */
+    if (gg.w > 0.f) return norm(p - xyz(gg));
+    if (gg.w == -2.f) return xyz(gg);
+    const int code = (int)u, axis = code >> 1;
+    const float sgn = (code & 1) ? 1.f : -1.f;
+    return mk(axis == 0 ? sgn : 0.f, axis == 1 ? sgn : 0.f, axis == 2 ? sgn : 0.f);
 }
 
 #endif

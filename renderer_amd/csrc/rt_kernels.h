@@ -33,6 +33,9 @@ struct HostDevScene {
     int32_t env_kind;
     int32_t probe_w, probe_h;
     const void* probe_rgb;
+    const void* obj_nodes;
+    const int32_t* obj_first;
+    int32_t n_obj, g_obj;
 };
 
 /* One view of a render: camera basis computed on the host with the

@@ -739,7 +739,7 @@ hipError_t launch_generate_pss(const PathState& ps, const DevView* view, int n, 
  * waves claim rays dynamically, so the launch ends evenly.
  * The traversal stack (LaneStack) keeps RT_STACK_LDS entries per lane in
  * LDS: 16 KB per block, so LDS leaves room for 10 blocks per CU. */
-template <bool COUNT>
+template <bool COUNT, bool OBJ>
 __global__ __launch_bounds__(PT_BLOCK) RT_TRAV_OCCUPANCY void k_extend(DevScene S, PathState ps, int qsel)
 {
     __shared__ int32_t stack[RT_STACK_LDS * PT_BLOCK];
@@ -759,6 +759,7 @@ __global__ __launch_bounds__(PT_BLOCK) RT_TRAV_OCCUPANCY void k_extend(DevScene 
                     p = q[i];
                     float4 a = ps.ray_o[p], b = ps.ray_d[p];
                     trav_init(S, T, xyz(a), xyz(b), a.w, b.w);
+                    if (OBJ && a.w <= b.w) obj_closest(S, T, b.w);
                     live = true;
                 } else {
                     more = false;
@@ -779,10 +780,16 @@ __global__ __launch_bounds__(PT_BLOCK) RT_TRAV_OCCUPANCY void k_extend(DevScene 
 
 hipError_t launch_extend(const HostDevScene& s, const PathState& ps, int qsel, int count, int grid, hipStream_t st)
 {
-    if (count)
-        hipLaunchKernelGGL((k_extend<true>), dim3(grid), dim3(PT_BLOCK), 0, st, as_dev(s), ps, qsel);
+    /* scenes without analytic objects run the variant without their test */
+    const bool obj = s.n_obj > 0;
+    if (count && obj)
+        hipLaunchKernelGGL((k_extend<true, true>), dim3(grid), dim3(PT_BLOCK), 0, st, as_dev(s), ps, qsel);
+    else if (count)
+        hipLaunchKernelGGL((k_extend<true, false>), dim3(grid), dim3(PT_BLOCK), 0, st, as_dev(s), ps, qsel);
+    else if (obj)
+        hipLaunchKernelGGL((k_extend<false, true>), dim3(grid), dim3(PT_BLOCK), 0, st, as_dev(s), ps, qsel);
     else
-        hipLaunchKernelGGL((k_extend<false>), dim3(grid), dim3(PT_BLOCK), 0, st, as_dev(s), ps, qsel);
+        hipLaunchKernelGGL((k_extend<false, false>), dim3(grid), dim3(PT_BLOCK), 0, st, as_dev(s), ps, qsel);
     return hipGetLastError();
 }
 
@@ -1288,6 +1295,7 @@ hipError_t launch_shade(const HostDevScene& s, const PathState& ps, PtParams p, 
 /* ===========================================================================
  * shadow: any-hit test of next-event rays
  * ========================================================================= */
+template <bool OBJ>
 __global__ __launch_bounds__(PT_BLOCK) RT_TRAV_OCCUPANCY void k_shadow(DevScene S, PathState ps, int compat)
 {
     __shared__ int32_t stack[RT_STACK_LDS * PT_BLOCK];
@@ -1306,6 +1314,10 @@ __global__ __launch_bounds__(PT_BLOCK) RT_TRAV_OCCUPANCY void k_shadow(DevScene 
                     float4 a = ps.sh_o[i], b = ps.sh_d[i];
                     item = i;
                     trav_init(S, T, xyz(a), xyz(b), a.w, b.w);
+                    if (OBJ && a.w <= b.w && obj_any(S, T.o, T.d, a.w, b.w)) {
+                        T.g = 0;  /* occluded by an object: no traversal */
+                        T.cur = -1;
+                    }
                     live = true;
                 } else {
                     more = false;
@@ -1333,7 +1345,10 @@ __global__ __launch_bounds__(PT_BLOCK) RT_TRAV_OCCUPANCY void k_shadow(DevScene 
 hipError_t launch_shadow(const HostDevScene& s, const PathState& ps, PtParams p, int grid, hipStream_t st)
 {
     int compat = p.mode == RT_MODE_COMPAT ? 1 : 0;
-    hipLaunchKernelGGL(k_shadow, dim3(grid), dim3(PT_BLOCK), 0, st, as_dev(s), ps, compat);
+    if (s.n_obj > 0)
+        hipLaunchKernelGGL(k_shadow<true>, dim3(grid), dim3(PT_BLOCK), 0, st, as_dev(s), ps, compat);
+    else
+        hipLaunchKernelGGL(k_shadow<false>, dim3(grid), dim3(PT_BLOCK), 0, st, as_dev(s), ps, compat);
     return hipGetLastError();
 }
 

@@ -3,7 +3,8 @@
 Mirrors the reference's scene classes (Camera, DiffuseMaterial,
 MirrorMaterial, PhongMaterial, BlinnMaterial, TransparentMaterial,
 DarkMatter, AreaLight, PointLight, ConeLight, EnvironmentMap, Triangle, Quad,
-Sphere, IntersectableList, Scene) with the
+Sphere, Plane, Box, CSGIntersection, CSGUnion, CSGDifference, CSGXor, IntersectableList,
+Scene) with the
 same constructor arguments, and flattens a scene into the pointer-free
 arrays of ``rt_scene_desc`` (include/rt_scene.h).
 
@@ -172,6 +173,51 @@ class Sphere(Intersectable):
         self.center, self.radius, self.material = _v3(center), float(radius), material
 
 
+class Plane(Intersectable):
+    """src/plane.h: the plane a x + b y + c z + d = 0 given as [a b c d], with
+    normal (a, b, c)."""
+
+    def __init__(self, plane, material):
+        self.plane = np.asarray(plane, np.float32).reshape(4)
+        self.material = material
+
+
+class Box(Intersectable):
+    """src/axisalignedbox.h as an intersectable: min and max corners, material."""
+
+    def __init__(self, min_corner, max_corner, material):
+        self.min, self.max, self.material = _v3(min_corner), _v3(max_corner), material
+
+
+class CSGOperation(Intersectable):
+    """src/csgoperation.h: a boolean combination of two CSG objects (spheres,
+    boxes or operations)."""
+
+    op = None
+
+    def __init__(self, left, right):
+        for x in (left, right):
+            if not isinstance(x, (Sphere, Box, CSGOperation)):
+                raise TypeError("CSG operands are spheres, boxes and CSG operations")
+        self.left, self.right = left, right
+
+
+class CSGIntersection(CSGOperation):
+    op = _abi.CSG_ISECT
+
+
+class CSGUnion(CSGOperation):
+    op = _abi.CSG_UNION
+
+
+class CSGDifference(CSGOperation):
+    op = _abi.CSG_DIFF
+
+
+class CSGXor(CSGOperation):
+    op = _abi.CSG_XOR
+
+
 class Mesh(Intersectable):
     """Indexed triangle mesh (what the reference's obj(...) produces): one
     primitive per triangle, one material for all or one per face."""
@@ -211,12 +257,16 @@ class SceneArrays:
     """Flattened scene: numpy arrays + ctypes descriptor (keeps arrays alive)."""
 
     def __init__(self, positions, normals, tri_vertices, tri_material, tri_prim, spheres, sphere_material,
-                 sphere_prim, materials, lights, env, env_kind=0):
+                 sphere_prim, materials, lights, env, env_kind=0, csg_nodes=(), object_first=None,
+                 object_prim=None):
         self.positions, self.normals = positions, normals
         self.tri_vertices, self.tri_material, self.tri_prim = tri_vertices, tri_material, tri_prim
         self.spheres, self.sphere_material, self.sphere_prim = spheres, sphere_material, sphere_prim
         self.materials, self.lights, self.env = materials, lights, env
         self.env_kind = env_kind
+        self.csg_nodes = list(csg_nodes)
+        self.object_first = np.zeros(1, np.int32) if object_first is None else np.asarray(object_first, np.int32)
+        self.object_prim = np.zeros(0, np.int32) if object_prim is None else np.asarray(object_prim, np.int32)
         self._desc = None
 
     @property
@@ -253,6 +303,12 @@ class SceneArrays:
             d.env_height, d.env_width = self.env.shape[0], self.env.shape[1]
             d.env_rgb = self.env.ctypes.data
             d.env_kind = self.env_kind
+        if len(self.object_prim):
+            self._nodes_c = (_abi.RtCsgNode * len(self.csg_nodes))(*self.csg_nodes)
+            d.n_objects = len(self.object_prim)
+            d.csg_nodes = ctypes.addressof(self._nodes_c)
+            d.object_first = self.object_first.ctypes.data
+            d.object_prim_id = self.object_prim.ctypes.data
         self._desc = d
         return d
 
@@ -310,6 +366,26 @@ class Scene:
 
         pos, nrm, tv, tm, tp = [], [], [], [], []
         sph, sm, sp = [], [], []
+        nodes, ofirst, oprim = [], [0], []
+
+        def csg_program(o):
+            """postfix nodes of a sphere / box / operation tree"""
+            if isinstance(o, CSGOperation):
+                csg_program(o.left)
+                csg_program(o.right)
+                n = _abi.RtCsgNode()
+                n.op = o.op
+                nodes.append(n)
+                return
+            n = _abi.RtCsgNode()
+            n.material = mid(o.material)
+            if isinstance(o, Sphere):
+                n.op = _abi.CSG_SPHERE
+                n.p[:4] = [*o.center.tolist(), o.radius]
+            else:
+                n.op = _abi.CSG_BOX
+                n.p[:] = [*o.min.tolist(), *o.max.tolist()]
+            nodes.append(n)
         nv = [0]
         has_normals = [False]
         next_id = [len(lights)]
@@ -335,6 +411,19 @@ class Scene:
             elif isinstance(o, Quad):
                 add_tris(o.p, np.array([[0, 1, 3], [3, 1, 2]], np.int32), None,
                          np.full(2, mid(o.material), np.int32), np.full(2, next_id[0], np.int32))
+                next_id[0] += 1
+            elif isinstance(o, (Box, CSGOperation)):
+                csg_program(o)
+                ofirst.append(len(nodes))
+                oprim.append(next_id[0])
+                next_id[0] += 1
+            elif isinstance(o, Plane):
+                n = _abi.RtCsgNode()
+                n.op, n.material = _abi.CSG_PLANE, mid(o.material)
+                n.p[:4] = o.plane.tolist()
+                nodes.append(n)
+                ofirst.append(len(nodes))
+                oprim.append(next_id[0])
                 next_id[0] += 1
             elif isinstance(o, Sphere):
                 sph.append(np.array([*o.center.tolist(), o.radius], np.float32))
@@ -367,7 +456,8 @@ class Scene:
         tri_prim = np.ascontiguousarray(np.concatenate(tp) if tp else np.zeros(0, np.int32), np.int32)
         spheres = np.ascontiguousarray(np.stack(sph) if sph else np.zeros((0, 4), np.float32), np.float32)
         return SceneArrays(positions, normals, tri_vertices, tri_material, tri_prim, spheres,
-                           np.asarray(sm, np.int32), np.asarray(sp, np.int32), mats, lights, env, env_kind)
+                           np.asarray(sm, np.int32), np.asarray(sp, np.int32), mats, lights, env, env_kind,
+                           nodes, np.asarray(ofirst, np.int32), np.asarray(oprim, np.int32))
 
 
 def _flat_vertex_normals(P, F):
