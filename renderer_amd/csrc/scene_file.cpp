@@ -1545,6 +1545,8 @@ struct Geometry {
     std::vector<int32_t> tri_mat, tri_grp;
     std::vector<float> sph;     // 4 per sphere
     std::vector<int32_t> sph_mat, sph_grp;
+    std::vector<rt_csg_node> csg;           // postfix programs of planes, boxes, CSG trees
+    std::vector<int32_t> obj_first{0}, obj_grp;
     int32_t groups = 0;
     std::string unsupported;
 };
@@ -1792,22 +1794,143 @@ private:
             build(n.kids[0], &W, g, id);
             break;
         }
-        case NodeKind::Plane:
-            if (g.unsupported.empty()) g.unsupported = "plane";
+        case NodeKind::Plane: {  // src/sceneast.cpp plane(...): one object
+            int id = grp >= 0 ? grp : g.groups++;
+            rt_csg_node nd{};
+            nd.op = RT_CSG_PLANE;
+            nd.material = material_of(n, g);
+            for (int k = 0; k < 4; ++k) nd.p[k] = n.f[k];
+            if (xf) {
+                // object-space plane P: the world plane is P M^-1 (its normal is
+                // the normal matrix applied to P's, as the instance reports it)
+                double inv[4][4];
+                if (!invert(*xf, inv)) {
+                    if (g.unsupported.empty()) g.unsupported = "plane under a singular instance transform";
+                    break;
+                }
+                for (int j = 0; j < 4; ++j) {
+                    double v = 0;
+                    for (int i = 0; i < 4; ++i) v += (double)n.f[i] * inv[i][j];
+                    nd.p[j] = (float)v;
+                }
+            }
+            g.csg.push_back(nd);
+            g.obj_first.push_back((int32_t)g.csg.size());
+            g.obj_grp.push_back(id);
             break;
+        }
         case NodeKind::Box:
-            if (g.unsupported.empty()) g.unsupported = "box";
-            break;
         case NodeKind::Isect:
         case NodeKind::Union:
         case NodeKind::Diff:
-        case NodeKind::Xor:
-            if (g.unsupported.empty()) g.unsupported = "CSG object";
+        case NodeKind::Xor: {
+            int id = grp >= 0 ? grp : g.groups++;
+            const size_t start = g.csg.size();
+            int leaves = 0;
+            std::string why;
+            if (!csg_program(n, xf, g, leaves, why) || leaves > RT_CSG_MAX_LEAVES) {
+                g.csg.resize(start);
+                if (why.empty()) why = "CSG object with more than 16 spheres and boxes";
+                if (g.unsupported.empty()) g.unsupported = why;
+                break;
+            }
+            g.obj_first.push_back((int32_t)g.csg.size());
+            g.obj_grp.push_back(id);
             break;
+        }
         case NodeKind::Obj:
             if (g.unsupported.empty()) g.unsupported = "unresolved obj";
             break;
         }
+    }
+
+    // Postfix program of a CSG tree (src/sceneast.cpp csg builders): leaves
+    // spheres and boxes, then the operation.  Under an instance transform a
+    // sphere needs a similarity and a box a positive axis scale plus
+    // translation, so that it stays a sphere or an axis-aligned box.
+    bool csg_program(const Node& n, const M4* xf, Geometry& g, int& leaves, std::string& why)
+    {
+        rt_csg_node nd{};
+        switch (n.kind) {
+        case NodeKind::Sphere: {
+            nd.op = RT_CSG_SPHERE;
+            nd.material = material_of(n, g);
+            float c[3] = {n.f[0], n.f[1], n.f[2]}, r = n.f[3];
+            if (xf) {
+                float sc;
+                if (!similarity(*xf, sc)) {
+                    why = "sphere under a non-similarity instance transform";
+                    return false;
+                }
+                V3 q = xf->map(V3{c[0], c[1], c[2]});
+                c[0] = q.x, c[1] = q.y, c[2] = q.z;
+                r *= sc;
+            }
+            nd.p[0] = c[0], nd.p[1] = c[1], nd.p[2] = c[2], nd.p[3] = r;
+            ++leaves;
+            break;
+        }
+        case NodeKind::Box: {
+            nd.op = RT_CSG_BOX;
+            nd.material = material_of(n, g);
+            for (int k = 0; k < 6; ++k) nd.p[k] = n.f[k];
+            if (xf) {
+                const M4& M = *xf;
+                bool ok = M.m[3][0] == 0 && M.m[3][1] == 0 && M.m[3][2] == 0 && M.m[3][3] == 1;
+                for (int i = 0; i < 3 && ok; ++i)
+                    for (int j = 0; j < 3 && ok; ++j) ok = i == j ? M.m[i][j] > 0 : M.m[i][j] == 0;
+                if (!ok) {
+                    why = "box under an instance transform other than scale and translation";
+                    return false;
+                }
+                V3 lo = M.map(V3{n.f[0], n.f[1], n.f[2]}), hi = M.map(V3{n.f[3], n.f[4], n.f[5]});
+                nd.p[0] = lo.x, nd.p[1] = lo.y, nd.p[2] = lo.z, nd.p[3] = hi.x, nd.p[4] = hi.y, nd.p[5] = hi.z;
+            }
+            ++leaves;
+            break;
+        }
+        case NodeKind::Isect:
+        case NodeKind::Union:
+        case NodeKind::Diff:
+        case NodeKind::Xor:
+            if (!csg_program(n.kids[0], xf, g, leaves, why) || !csg_program(n.kids[1], xf, g, leaves, why))
+                return false;
+            nd.op = n.kind == NodeKind::Isect ? RT_CSG_ISECT
+                    : n.kind == NodeKind::Union ? RT_CSG_UNION
+                    : n.kind == NodeKind::Diff  ? RT_CSG_DIFF
+                                                : RT_CSG_XOR;
+            break;
+        default:
+            why = "CSG operand that is not a sphere, box or CSG operation";
+            return false;
+        }
+        g.csg.push_back(nd);
+        return true;
+    }
+
+    static bool invert(const M4& M, double inv[4][4])
+    {
+        double a[4][8];
+        for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < 8; ++j) a[i][j] = j < 4 ? (double)M.m[i][j] : (j - 4 == i ? 1.0 : 0.0);
+        for (int c = 0; c < 4; ++c) {
+            int piv = c;
+            for (int r = c + 1; r < 4; ++r)
+                if (std::fabs(a[r][c]) > std::fabs(a[piv][c])) piv = r;
+            if (a[piv][c] == 0) return false;
+            if (piv != c)
+                for (int j = 0; j < 8; ++j) std::swap(a[c][j], a[piv][j]);
+            double d = a[c][c];
+            for (int j = 0; j < 8; ++j) a[c][j] /= d;
+            for (int r = 0; r < 4; ++r)
+                if (r != c) {
+                    double f = a[r][c];
+                    for (int j = 0; j < 8; ++j) a[r][j] -= f * a[c][j];
+                }
+        }
+        for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < 4; ++j) inv[i][j] = a[i][4 + j];
+        return true;
     }
 
     // upper 3x3 = s * rotation, no projective row
@@ -1842,6 +1965,8 @@ struct rt_scene_file {
     std::string messages;  // load_log + the warnings of the last evaluation
     std::vector<float> positions, normals, spheres;
     std::vector<int32_t> tri_vertices, tri_material, tri_prim, sph_material, sph_prim;
+    std::vector<rt_csg_node> csg_nodes;
+    std::vector<int32_t> object_first, object_prim;
     std::vector<rt_material> materials;
     std::vector<rt_light> lights;
     std::vector<float> env_rgb;  /* light probe of an environmentmap(...) light */
@@ -2024,6 +2149,11 @@ int rt_scene_file_scene(rt_scene_file* f, rt_scene_desc* desc, rt_camera* camera
     f->sph_prim.resize(g.sph_grp.size());
     for (size_t i = 0; i < g.sph_grp.size(); ++i) f->sph_prim[i] = nL + g.sph_grp[i];
 
+    f->csg_nodes = g.csg;
+    f->object_first = g.obj_first;
+    f->object_prim.resize(g.obj_grp.size());
+    for (size_t i = 0; i < g.obj_grp.size(); ++i) f->object_prim[i] = nL + g.obj_grp[i];
+
     if (3 * nT > (size_t)std::numeric_limits<int32_t>::max())
         return fail(RT_ERR_INVALID, "too many triangles for one scene description");
 
@@ -2043,6 +2173,12 @@ int rt_scene_file_scene(rt_scene_file* f, rt_scene_desc* desc, rt_camera* camera
     desc->materials = f->materials.empty() ? nullptr : f->materials.data();
     desc->n_lights = nL;
     desc->lights = f->lights.empty() ? nullptr : f->lights.data();
+    if (!f->object_prim.empty()) {
+        desc->n_objects = (int32_t)f->object_prim.size();
+        desc->csg_nodes = f->csg_nodes.data();
+        desc->object_first = f->object_first.data();
+        desc->object_prim_id = f->object_prim.data();
+    }
     if (!f->env_rgb.empty()) {
         desc->env_width = f->env_w;
         desc->env_height = f->env_h;

@@ -55,12 +55,18 @@ class FileSceneArrays:
                                           shape=(d.env_height * d.env_width * 3,)).reshape(d.env_height, d.env_width, 3)
                     if d.env_rgb else None)
         self.env_kind = d.env_kind
+        nodes = ctypes.cast(d.csg_nodes, ctypes.POINTER(_abi.RtCsgNode)) if d.csg_nodes else None
+        self.object_first = view(d.object_first, d.n_objects + 1, np.int32) if d.n_objects else np.zeros(1, np.int32)
+        self.object_prim = view(d.object_prim_id, d.n_objects, np.int32)
+        self.csg_nodes = [nodes[i] for i in range(int(self.object_first[-1]))] if nodes else []
         self.camera = Camera(list(camera.eye), list(camera.look_at), list(camera.up), camera.fov, camera.width,
                              camera.height)
 
     @property
     def n_prims(self):
         ids = [len(self.lights) - 1]
+        if len(self.object_prim):
+            ids.append(int(self.object_prim.max()))
         if len(self.tri_prim):
             ids.append(int(self.tri_prim.max()))
         if len(self.sphere_prim):

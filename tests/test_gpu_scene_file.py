@@ -69,3 +69,48 @@ def test_obj_mesh_traces_like_the_oracle_and_renders_like_a_mesh(gpu_ctx, tmp_pa
     img_host = gpu_ctx.render([view], mode=MODE_UNBIASED, max_depth=6, seed=9)[0]
     assert img_file.mean() > 0
     assert np.array_equal(img_file, img_host)
+
+
+CSG_SCENE = """
+camera = camera([0.3 0.8 4.5], [0 -0.3 0], [0 1 0], 0.75, 48, 48);
+lights = lights{ arealight([-0.5 2.9 -0.5], [1 0 0], [0 0 1], [9 9 8]) pointlight([2 2 2], [2 2 2]) };
+grey = diffuse([0.7 0.7 0.7]);
+red = diffuse([0.1 0.2 0.7]);
+intersectable = intersectables{
+  plane([0 1 0 1], grey)
+  box([-2 -1 -1.5], [-0.9 0.2 -0.4], red)
+  diff(sphere([0.6 -0.2 -0.6], 0.8, phong([0.2 0.4 0.4], [0.3 0.3 0.3], 30)), box([0.1 -0.3 -0.1], [1.5 1 0.5], grey))
+  instance(translate([0 0.5 0]) * scale([0.5 0.5 0.5]),
+           xor(sphere([-3 1 1], 0.9, grey), sphere([-2.4 1 1], 0.9, red)))
+};
+"""
+
+
+def test_csg_scene_file_renders_like_the_host_model_and_the_oracle(gpu_ctx):
+    from renderer_amd import MODE_COMPAT
+    from renderer_amd.scene import (AreaLight, Box, Camera, CSGDifference, CSGXor, Plane, Sphere)
+    a = SceneFile.parse(CSG_SCENE).arrays()
+    grey, red = DiffuseMaterial((0.7, 0.7, 0.7)), DiffuseMaterial((0.1, 0.2, 0.7))
+    shiny = PhongMaterial((0.2, 0.4, 0.4), (0.3, 0.3, 0.3), 30.0)
+    host = Scene(Camera((0.3, 0.8, 4.5), (0, -0.3, 0), (0, 1, 0), 0.75, 48, 48),
+                 [AreaLight((-0.5, 2.9, -0.5), (1, 0, 0), (0, 0, 1), (9, 9, 8)), PointLight((2, 2, 2), (2, 2, 2))],
+                 IntersectableList([
+                     Plane([0, 1, 0, 1], grey),
+                     Box((-2, -1, -1.5), (-0.9, 0.2, -0.4), red),
+                     CSGDifference(Sphere((0.6, -0.2, -0.6), 0.8, shiny), Box((0.1, -0.3, -0.1), (1.5, 1, 0.5), grey)),
+                     CSGXor(Sphere((-1.5, 1.0, 0.5), 0.45, grey), Sphere((-1.2, 1.0, 0.5), 0.45, red)),
+                 ])).flatten()
+    views = [(a.camera, 2, 2)]
+    print("CSG scene file: 48x48, 4 spp", flush=True)
+    gpu_ctx.upload(a)
+    img_file = gpu_ctx.render(views, mode=MODE_UNBIASED, max_depth=6, seed=9)[0]
+    gpu_ctx.upload(host)
+    img_host = gpu_ctx.render(views, mode=MODE_UNBIASED, max_depth=6, seed=9)[0]
+    assert img_file.mean() > 0
+    assert np.array_equal(img_file, img_host)
+    gpu_ctx.upload(a)
+    img = gpu_ctx.render(views, mode=MODE_COMPAT, max_depth=5, seed=4)[0]
+    ref, _ = Oracle(a).render(views[0], mode=MODE_COMPAT, max_depth=5, seed=4)
+    agree = np.all(np.abs(img - ref) <= 1e-3 * np.abs(ref) + 1e-4, axis=2).mean()
+    print(f"scene file vs oracle agreement {agree:.4f}", flush=True)
+    assert agree > 0.97

@@ -208,10 +208,13 @@ def test_parse_errors_name_the_position():
 
 def test_unsupported_constructs_parse_and_dump_but_do_not_render():
     cases = {
-        "plane([0 1 0 1], diffuse([1 1 1]))": "plane",
-        "box([0 0 0],[1 1 1], diffuse([1 1 1]))": "box",
-        "union(sphere([0 0 0],1,diffuse([1 1 1])), box([0 0 0],[1 1 1],mirror(1)))": "CSG",
+        "instance(rotate(30, [0 1 0]), box([0 0 0],[1 1 1], diffuse([1 1 1])))": "box under",
+        "instance(rotate(30, [0 1 0]), diff(box([0 0 0],[1 1 1], diffuse([1 1 1])), sphere([0 0 0],1,mirror(1))))":
+            "box under",
         "instance(scale([1 2 1]), sphere([0 0 0], 1, diffuse([1 1 1])))": "non-similarity",
+        "instance(scale([1 2 1]), union(sphere([0 0 0],1,diffuse([1 1 1])), box([0 0 0],[1 1 1],mirror(1))))":
+            "non-similarity",
+        "instance(scale([1 0 1]), plane([0 1 0 1], diffuse([1 1 1])))": "singular",
     }
     for obj, word in cases.items():
         sf = SceneFile.parse("camera = camera([0 0 5],[0 0 0],[0 1 0],0.5,8,8);\n"
@@ -220,6 +223,31 @@ def test_unsupported_constructs_parse_and_dump_but_do_not_render():
         with pytest.raises(_abi.RtError) as e:
             sf.arrays()
         assert e.value.status == _abi.ERR_UNSUPPORTED and word in str(e.value), str(e.value)
+
+
+def test_planes_boxes_and_csg_map_to_object_programs():
+    sf = SceneFile.parse(
+        "camera = camera([0 0 5],[0 0 0],[0 1 0],0.5,8,8);\n"
+        "lights = lights{pointlight([0 5 0],[1 1 1])};\n"
+        "intersectable = intersectables{ plane([0 1 0 1], diffuse([.5 .5 .5]))"
+        " box([0 0 0],[1 1 1], mirror(1))"
+        " diff(sphere([0 0 0],1,diffuse([1 0 0])), box([0 0 0],[2 2 2],diffuse([0 1 0])))"
+        " instance(translate([1 2 3]) * scale([2 2 2]),"
+        "          union(sphere([0 0 0],1,diffuse([1 1 1])), box([-1 -1 -1],[0 0 0],diffuse([1 1 1]))))"
+        " instance(translate([0 1 0]), plane([0 1 0 0], diffuse([1 1 1]))) };")
+    a = sf.arrays()
+    from renderer_amd._abi import CSG_BOX, CSG_DIFF, CSG_PLANE, CSG_SPHERE, CSG_UNION
+    assert list(a.object_first) == [0, 1, 2, 5, 8, 9]
+    assert list(a.object_prim) == [1, 2, 3, 4, 5]          # the point light is primitive 0
+    ops = [n.op for n in a.csg_nodes]
+    assert ops == [CSG_PLANE, CSG_BOX, CSG_SPHERE, CSG_BOX, CSG_DIFF, CSG_SPHERE, CSG_BOX, CSG_UNION, CSG_PLANE]
+    p = [list(n.p) for n in a.csg_nodes]
+    assert p[0][:4] == [0, 1, 0, 1]
+    assert p[5][:4] == [1, 2, 3, 2]                         # centre moved, radius scaled
+    assert p[6] == [-1, 0, 1, 1, 2, 3]                      # box corners mapped
+    assert p[8][:4] == [0, 1, 0, -1]                        # the plane y = 1
+    mats = a.materials
+    assert mats[a.csg_nodes[1].material].type == _abi.MAT_MIRROR
 
 
 def test_cone_light_and_refractive_material_map_to_the_gpu_kinds():
