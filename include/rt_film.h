@@ -54,6 +54,11 @@ const char* rt_film_last_error(void);
  * rgb != NULL the height * width * 3 RGB floats (cap = floats available). */
 int rt_image_read_hdr(const char* path, int32_t* width, int32_t* height, float* rgb, int64_t cap);
 
+/* 8-bit image as the reference reads textures (cv::imread, then /255,
+ * src/texturematerial.cpp:46-60): PNG, binary PPM / PGM or BMP, as
+ * height * width * 3 RGB floats byte / 255, row 0 at the top. */
+int rt_image_read(const char* path, int32_t* width, int32_t* height, float* rgb, int64_t cap);
+
 #ifdef __cplusplus
 }
 #endif

@@ -30,14 +30,15 @@ extern "C" {
 
 /* Material kinds: src/diffusematerial.cpp, src/mirrormaterial.cpp,
  * src/phongmaterial.cpp, src/darkmatter.cpp, src/blinnmaterial.cpp,
- * src/transparentmaterial.cpp. */
+ * src/transparentmaterial.cpp, src/texturematerial.cpp. */
 enum rt_material_type {
     RT_MAT_DIFFUSE = 0,
     RT_MAT_MIRROR = 1,
     RT_MAT_PHONG = 2,
     RT_MAT_DARK = 3,
     RT_MAT_BLINN = 4,
-    RT_MAT_REFRACTIVE = 5
+    RT_MAT_REFRACTIVE = 5,
+    RT_MAT_TEXTURE = 6   /* src/texturematerial.cpp: kd = coefficient, exponent = gamma, texture = image index */
 };
 
 /* Light kinds: src/arealight.cpp, src/pointlight.cpp, src/environmentmap.cpp,
@@ -75,8 +76,16 @@ typedef struct rt_material {
     int32_t type;        /* rt_material_type */
     float kd[3];         /* diffuse colour (RGB): Diffuse, Phong, Blinn */
     float ks[3];         /* specular colour (RGB): Phong, Blinn */
-    float exponent;      /* specular exponent: Phong, Blinn; refraction coefficient: refractive */
+    float exponent;      /* specular exponent: Phong, Blinn; refraction coefficient: refractive; gamma: texture */
+    int32_t texture;     /* texture: index into rt_scene_desc.textures */
 } rt_material;
+
+/* A texture image: height * width * 3 RGB floats (byte / 255 as the
+ * reference's convertTo, src/texturematerial.cpp:46-60), row 0 at the top. */
+typedef struct rt_texture {
+    int32_t width, height;
+    const float* rgb;
+} rt_texture;
 
 typedef struct rt_light {
     int32_t type;        /* rt_light_type */
@@ -126,6 +135,13 @@ typedef struct rt_scene_desc {
     const rt_csg_node* csg_nodes;
     const int32_t* object_first;
     const int32_t* object_prim_id;
+
+    /* texture images of RT_MAT_TEXTURE materials, and texture coordinates of
+     * the triangles (u, v at p1, p2, p3: 6 floats per triangle; NULL = the
+     * reference's default (0, 0), (0, 1), (1, 0), src/triangle.h:35-36) */
+    int32_t n_textures;
+    const rt_texture* textures;
+    const float* tri_texcoords;
 } rt_scene_desc;
 
 #define RT_CSG_MAX_LEAVES 16

@@ -18,14 +18,15 @@
  * rt_scene_upload takes (include/rt_scene.h), with the camera variable.
  * rt_scene_file_dump_bvh writes the --dump-bvh text.
  *
- * Planes, boxes and CSG trees become the analytic objects of rt_scene_desc.
- * What the GPU path does not render (loadable texture materials, spheres
- * under non-similarity instance transforms, boxes under instance transforms
- * other than axis scale and translation) still parses and dumps;
+ * Planes, boxes and CSG trees become the analytic objects of rt_scene_desc,
+ * texture materials its textures (PNG, PPM or BMP images).  What the GPU path
+ * does not render (spheres under non-similarity instance transforms, boxes
+ * under instance transforms other than axis scale and translation) still
+ * parses and dumps;
  * rt_scene_file_scene then returns RT_ERR_UNSUPPORTED naming the
  * construct.  Input the reference accepts
  * with a warning on stderr (unknown material variable -> DarkMatter,
- * unreadable OBJ -> empty list, unreadable texture -> DarkMatter,
+ * unreadable OBJ -> empty list, unreadable or undecodable texture -> DarkMatter,
  * unreadable environment map -> BlackLight) is accepted the same way, the
  * warning going to rt_scene_file_messages.
  */

@@ -42,6 +42,9 @@ def load():
     lib.oc_trace_closest.argtypes = [P, I64, P, P, P, P, ctypes.c_int]
     lib.oc_trace_any.argtypes = [P, I64, P, P, P, ctypes.c_int]
     lib.oc_hit_normals.argtypes = [P, I64, P, P, P]
+    lib.oc_hit_texcoords.argtypes = [P, I64, P, P, P]
+    lib.oc_texture_albedo.restype = ctypes.c_int
+    lib.oc_texture_albedo.argtypes = [P, ctypes.c_int, ctypes.c_float, ctypes.c_float, P]
     lib.oc_intersect_prim.restype = ctypes.c_float
     lib.oc_intersect_prim.argtypes = [P, ctypes.c_int32, P, P]
     lib.oc_render.restype = I64
@@ -129,6 +132,19 @@ class Oracle:
         d = np.ascontiguousarray(d_tmax, np.float32)
         out = np.zeros((len(o), 3), np.float32)
         self.lib.oc_hit_normals(self.h, len(o), o.ctypes.data, d.ctypes.data, out.ctypes.data)
+        return out
+
+    def hit_texcoords(self, o_tmin, d_tmax):
+        o = np.ascontiguousarray(o_tmin, np.float32)
+        d = np.ascontiguousarray(d_tmax, np.float32)
+        out = np.zeros((len(o), 2), np.float32)
+        self.lib.oc_hit_texcoords(self.h, len(o), o.ctypes.data, d.ctypes.data, out.ctypes.data)
+        return out
+
+    def texture_albedo(self, mat, s, t):
+        out = np.zeros(3, np.float32)
+        if self.lib.oc_texture_albedo(self.h, int(mat), float(s), float(t), out.ctypes.data) != 0:
+            raise ValueError("not a texture material")
         return out
 
     def trace_any(self, o_tmin, d_tmax, threads=8):

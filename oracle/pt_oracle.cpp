@@ -145,11 +145,14 @@ struct Hit {
     Ray ray;
     const Material* mat = nullptr;
     int prim = -1;
+    float tc[2] = {0, 0};  // texture coordinates (cv::Point2f, default (0, 0))
     bool hits() const { return !std::isnan(t); }
 };
-Hit make_hit(float t, const Ray& ray, const Material* m, V3 normal, int prim)
+Hit make_hit(float t, const Ray& ray, const Material* m, V3 normal, int prim, float ts = 0, float tt = 0)
 {
     Hit h;
+    h.tc[0] = ts;
+    h.tc[1] = tt;
     h.t = t;
     h.ray = ray;
     h.mat = m;
@@ -235,6 +238,7 @@ struct Intersectable {
 // src/triangle.cpp:30-56, ctor src/triangle.h:35-38
 struct Triangle : Intersectable {
     V3 p1, p2, p3, n1, n2, n3;
+    float t1[2] = {0, 0}, t2[2] = {0, 1}, t3[2] = {1, 0};  // default texcoords, src/triangle.h:35-36
     const Material* m;
     int prim;
     Triangle(V3 a, V3 b, V3 c, const Material* mat, int id) : p1(a), p2(b), p3(c), m(mat), prim(id)
@@ -258,7 +262,8 @@ struct Triangle : Intersectable {
         if (v < 0 || u + v > 1) return Hit();
         V3 normal = (1 - u - v) * n1 + u * n2 + v * n3;
         if (is_zero(normal)) return Hit();
-        return make_hit(t, ray, m, normal, prim);
+        const float bw = 1 - u - v;  // (1 - u - v) * t1 + u * t2 + v * t3, src/triangle.cpp:54
+        return make_hit(t, ray, m, normal, prim, bw * t1[0] + u * t2[0] + v * t3[0], bw * t1[1] + u * t2[1] + v * t3[1]);
     }
 };
 
@@ -272,6 +277,10 @@ struct Quad : Intersectable {
         return h.hits() ? h : t2.intersect(ray);
     }
 };
+
+// src/sphere.cpp:47: texture coordinates of the unnormalised normal n
+inline float sphere_tc_s(V3 n, float r) { return (float)(std::acos(n.z / r) / M_PI); }
+inline float sphere_tc_t(V3 n) { return (float)(std::atan2(n.x, n.y) / M_PI / 2 + 0.5f); }
 
 // src/sphere.cpp:29-56 + src/csgobject.cpp:25-44
 struct Sphere : Intersectable {
@@ -301,7 +310,7 @@ struct Sphere : Intersectable {
             if (t >= ray.from) {
                 if (t > ray.to) return Hit();
                 V3 n = (ray.o + t * ray.d) - c;
-                return make_hit(t, ray, m, n, prim);
+                return make_hit(t, ray, m, n, prim, sphere_tc_s(n, r), sphere_tc_t(n));
             }
         }
         return Hit();
@@ -335,6 +344,7 @@ struct CsgParam {
     float t;
     V3 n;
     const Material* m;
+    float ts = 0, tt = 0;
 };
 struct CsgNode {
     Box box;
@@ -361,8 +371,10 @@ struct CsgSphere : CsgNode {
         float disc = b * b - 4 * a * cc;
         if (disc >= 0) {
             float root = std::sqrt(disc);
-            for (float t : {(-b - root) / (2 * a), (-b + root) / (2 * a)})
-                out.push_back({t, (ray.o + t * ray.d) - c, m});
+            for (float t : {(-b - root) / (2 * a), (-b + root) / (2 * a)}) {
+                V3 n = (ray.o + t * ray.d) - c;
+                out.push_back({t, n, m, sphere_tc_s(n, r), sphere_tc_t(n)});
+            }
         }
         return out;
     }
@@ -454,7 +466,7 @@ struct CsgObject : Intersectable {
         for (const CsgParam& p : ps) {
             if (p.t >= ray.from) {
                 if (p.t > ray.to) return Hit();
-                return make_hit(p.t, ray, p.m, p.n, prim);
+                return make_hit(p.t, ray, p.m, p.n, prim, p.ts, p.tt);
             }
         }
         return Hit();
@@ -555,6 +567,49 @@ struct Diffuse : Material {
         return out;
     }
     int kind() const override { return RT_MAT_DIFFUSE; }
+};
+
+// src/texturematerial.cpp:33-72 (an AbstractDiffuseMaterial): brdf =
+// pow(get(texcoord), gamma) / pi, get = the image sampled bilinearly at
+// (clamp(s) (W - 1), clamp(1 - t) (H - 1)) with the border replicated
+// (cv::getRectSubPix of a 1x1 patch) times the coefficient; clamp wraps into
+// [0, 1) with fmod.
+struct TextureMat : Material {
+    int W = 0, H = 0;
+    std::vector<float> rgb;
+    Col coef;
+    float gamma = 1;
+    static float wrap(float x)
+    {
+        float f = std::fmod(x, 1.f);
+        return f < 0 ? f + 1 : f;
+    }
+    Col albedo(const Hit& h) const
+    {
+        const float cx = wrap(h.tc[0]) * (W - 1), cy = wrap(1 - h.tc[1]) * (H - 1);
+        const float fx = std::floor(cx), fy = std::floor(cy), a = cx - fx, b = cy - fy;
+        const int x0 = std::min(std::max((int)fx, 0), W - 1), y0 = std::min(std::max((int)fy, 0), H - 1);
+        const int x1 = std::min(x0 + 1, W - 1), y1 = std::min(y0 + 1, H - 1);
+        auto px = [&](int x, int y, int k) { return rgb[3 * ((size_t)y * W + x) + k]; };
+        const float w00 = (1 - a) * (1 - b), w01 = a * (1 - b), w10 = (1 - a) * b, w11 = a * b;
+        float c[3];
+        const float cf[3] = {coef.r, coef.g, coef.b};
+        for (int k = 0; k < 3; ++k) {
+            float v = px(x0, y0, k) * w00 + px(x1, y0, k) * w01 + px(x0, y1, k) * w10 + px(x1, y1, k) * w11;
+            c[k] = std::pow(v * cf[k], gamma);
+        }
+        return Col(c[0], c[1], c[2]);
+    }
+    Col brdf(const Hit& h, V3) const override { return albedo(h) * (float)(1 / M_PI); }
+    V3 out_direction(const Hit& hit, Sample s, float& pdf, Col& b) const override
+    {
+        V3 n = hit.n;
+        if (dot(normalized(n), normalized(hit.ray.d)) > 0) n = n * -1.f;
+        V3 out = cosine_dir(s, n, pdf);
+        b = brdf(hit, out);
+        return out;
+    }
+    int kind() const override { return RT_MAT_TEXTURE; }
 };
 
 // src/mirrormaterial.cpp:31-41
@@ -1278,6 +1333,11 @@ Col integrate_unbiased(const UnbiasedCtx& c, Ray ray, const SampleKey& k, int64_
             break;
         }
         const Material* m = hit.mat;
+        Diffuse tex_here(Col(0, 0, 0));  // a texture is a diffuse surface with the albedo at the hit
+        if (m->kind() == RT_MAT_TEXTURE) {
+            tex_here.kd = static_cast<const TextureMat*>(m)->albedo(hit);
+            m = &tex_here;
+        }
         V3 n = hit.n;
         V3 d = normalized(ray.d);
         V3 nf = dot(n, d) > 0 ? -n : n;
@@ -1682,6 +1742,17 @@ void* oc_scene_create(const rt_scene_desc* d, int accel)
             mm = new Blinn(Col(m.kd[0], m.kd[1], m.kd[2]), Col(m.ks[0], m.ks[1], m.ks[2]), m.exponent);
             break;
         case RT_MAT_REFRACTIVE: mm = new Transparent(m.exponent); break;
+        case RT_MAT_TEXTURE: {
+            auto* tm = new TextureMat();
+            const rt_texture& t = d->textures[m.texture];
+            tm->W = t.width;
+            tm->H = t.height;
+            tm->rgb.assign(t.rgb, t.rgb + (size_t)3 * t.width * t.height);
+            tm->coef = Col(m.kd[0], m.kd[1], m.kd[2]);
+            tm->gamma = m.exponent;
+            mm = tm;
+            break;
+        }
         default: mm = new DarkMatter(); break;
         }
         sc->mats.emplace_back(mm);
@@ -1741,6 +1812,10 @@ void* oc_scene_create(const rt_scene_desc* d, int accel)
         const int32_t* tv = d->tri_vertices + 3 * i;
         Triangle t(vert(tv[0]), vert(tv[1]), vert(tv[2]), mats[d->tri_material[i]], d->tri_prim_id[i]);
         if (d->normals) t.set_normals(nrm(tv[0]), nrm(tv[1]), nrm(tv[2]));
+        if (d->tri_texcoords) {
+            const float* q = d->tri_texcoords + 6 * (size_t)i;
+            t.t1[0] = q[0], t.t1[1] = q[1], t.t2[0] = q[2], t.t2[1] = q[3], t.t3[0] = q[4], t.t3[1] = q[5];
+        }
         return t;
     };
     // Components in primitive-ID order; two consecutive triangles with one ID form a quad.
@@ -1851,6 +1926,28 @@ void oc_trace_any(void* s, int64_t n, const float* o, const float* d, uint8_t* o
         Hit h = sc.root ? sc.root->intersect(ray_from(o, d, i)) : Hit();
         occ[i] = h.hits() ? 1 : 0;
     }
+}
+
+void oc_hit_texcoords(void* s, int64_t n, const float* o, const float* d, float* tc)
+{
+    const Scene& sc = *static_cast<Scene*>(s);
+    for (int64_t i = 0; i < n; ++i) {
+        Hit h = sc.root ? sc.root->intersect(ray_from(o, d, i)) : Hit();
+        tc[2 * i] = h.hits() ? h.tc[0] : kNaN;
+        tc[2 * i + 1] = h.hits() ? h.tc[1] : kNaN;
+    }
+}
+
+int oc_texture_albedo(void* s, int mat, float ts, float tt, float rgb[3])
+{
+    const Scene& sc = *static_cast<Scene*>(s);
+    if (mat < 0 || mat >= (int)sc.mats.size() || sc.mats[mat]->kind() != RT_MAT_TEXTURE) return -1;
+    Hit h;
+    h.tc[0] = ts;
+    h.tc[1] = tt;
+    Col c = static_cast<const TextureMat*>(sc.mats[mat].get())->albedo(h);
+    rgb[0] = c.r, rgb[1] = c.g, rgb[2] = c.b;
+    return 0;
 }
 
 void oc_hit_normals(void* s, int64_t n, const float* o, const float* d, float* normals)

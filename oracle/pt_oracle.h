@@ -55,6 +55,12 @@ void oc_trace_closest(void* scene, int64_t n, const float* o_tmin, const float* 
 void oc_trace_any(void* scene, int64_t n, const float* o_tmin, const float* d_tmax,
                   uint8_t* occluded, int threads);
 
+/* Texture coordinates of each ray's closest hit (NaN for a miss), and the
+ * albedo pow(image(s, t) * coefficient, gamma) of texture material `mat`
+ * (returns -1 when it is not a texture). */
+void oc_hit_texcoords(void* scene, int64_t n, const float* o_tmin, const float* d_tmax, float* tc);
+int oc_texture_albedo(void* scene, int mat, float s, float t, float rgb[3]);
+
 /* Surface normal (normalised, as HitRecord keeps it) of each ray's closest
  * hit, 0 for a miss. */
 void oc_hit_normals(void* scene, int64_t n, const float* o_tmin, const float* d_tmax, float* normals);

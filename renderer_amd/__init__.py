@@ -9,13 +9,13 @@ from .integrator import (EnergyRedistributionRenderer, GpuContext, GpuUniDiPathT
                          PerPixelRenderer)
 from .scene import (AreaLight, BlinnMaterial, Camera, ConeLight, DarkMatter, DiffuseMaterial, EnvironmentMap,
                     IntersectableList, Mesh, MirrorMaterial, PhongMaterial, PointLight, Quad, Scene, Sphere,
-                    TransparentMaterial, Triangle)
+                    TextureMaterial, TransparentMaterial, Triangle)
 from .scene_file import SceneFile
 
 __all__ = [
     "GpuContext", "GpuUniDiPathTracingIntegrator", "PerPixelRenderer", "RtError",
     "AreaLight", "Camera", "DarkMatter", "DiffuseMaterial", "EnvironmentMap", "IntersectableList", "Mesh",
     "MirrorMaterial", "PhongMaterial", "PointLight", "Quad", "Scene", "Sphere", "Triangle", "SceneFile",
-    "BlinnMaterial", "TransparentMaterial", "ConeLight", "MetropolisRenderer", "EnergyRedistributionRenderer",
+    "BlinnMaterial", "TransparentMaterial", "TextureMaterial", "ConeLight", "MetropolisRenderer", "EnergyRedistributionRenderer",
     "MODE_COMPAT", "MODE_UNBIASED", "MODE_GUIDED", "MLT_METROPOLIS", "MLT_ERPT",
 ]

@@ -10,7 +10,8 @@ c_float3 = ctypes.c_float * 3
 
 
 class RtMaterial(ctypes.Structure):
-    _fields_ = [("type", ctypes.c_int32), ("kd", c_float3), ("ks", c_float3), ("exponent", ctypes.c_float)]
+    _fields_ = [("type", ctypes.c_int32), ("kd", c_float3), ("ks", c_float3), ("exponent", ctypes.c_float),
+                ("texture", ctypes.c_int32)]
 
 
 class RtLight(ctypes.Structure):
@@ -43,7 +44,14 @@ class RtSceneDesc(ctypes.Structure):
         ("csg_nodes", ctypes.c_void_p),
         ("object_first", ctypes.c_void_p),
         ("object_prim_id", ctypes.c_void_p),
+        ("n_textures", ctypes.c_int32),
+        ("textures", ctypes.c_void_p),
+        ("tri_texcoords", ctypes.c_void_p),
     ]
+
+
+class RtTexture(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_int32), ("height", ctypes.c_int32), ("rgb", ctypes.c_void_p)]
 
 
 class RtCsgNode(ctypes.Structure):
@@ -117,7 +125,7 @@ class RtMltResult(ctypes.Structure):
 
 
 MLT_METROPOLIS, MLT_ERPT = 0, 1
-MAT_DIFFUSE, MAT_MIRROR, MAT_PHONG, MAT_DARK, MAT_BLINN, MAT_REFRACTIVE = 0, 1, 2, 3, 4, 5
+MAT_DIFFUSE, MAT_MIRROR, MAT_PHONG, MAT_DARK, MAT_BLINN, MAT_REFRACTIVE, MAT_TEXTURE = 0, 1, 2, 3, 4, 5, 6
 LIGHT_AREA, LIGHT_POINT, LIGHT_ENV, LIGHT_CONE = 0, 1, 2, 3
 MODE_COMPAT, MODE_UNBIASED, MODE_GUIDED = 0, 1, 2
 
@@ -176,6 +184,7 @@ PROTOTYPES = {
     "rt_unique_filename": (_I64, [ctypes.c_char_p, _P, _I64]),
     "rt_film_last_error": (ctypes.c_char_p, []),
     "rt_image_read_hdr": (ctypes.c_int, [ctypes.c_char_p, _P, _P, _P, _I64]),
+    "rt_image_read": (ctypes.c_int, [ctypes.c_char_p, _P, _P, _P, _I64]),
     # include/rt_mlt.h
     "rt_integrate_samples": (ctypes.c_int, [_P, ctypes.POINTER(RtCamera), _I32, _I32, ctypes.c_float, _P, _I64, _P]),
     "rt_render_mlt": (ctypes.c_int, [_P, ctypes.POINTER(RtCamera), ctypes.POINTER(RtMltParams), _P,

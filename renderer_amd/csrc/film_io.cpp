@@ -8,6 +8,8 @@
  */
 #include <sys/stat.h>
 #include <zlib.h>
+#include <cctype>
+#include <cstdlib>
 
 #include <cmath>
 #include <cstdio>
@@ -215,9 +217,227 @@ bool rt_read_rgbe(const std::string& path, int& w, int& h, std::vector<float>& r
     return true;
 }
 
+namespace {
+
+bool read_file(const std::string& path, std::vector<uint8_t>& buf)
+{
+    FILE* f = std::fopen(path.c_str(), "rb");
+    if (!f) return false;
+    uint8_t tmp[1 << 16];
+    size_t n;
+    while ((n = std::fread(tmp, 1, sizeof tmp, f)) > 0) buf.insert(buf.end(), tmp, tmp + n);
+    std::fclose(f);
+    return true;
+}
+
+uint32_t be32(const uint8_t* p) { return (uint32_t)p[0] << 24 | (uint32_t)p[1] << 16 | (uint32_t)p[2] << 8 | p[3]; }
+uint32_t le32(const uint8_t* p) { return (uint32_t)p[3] << 24 | (uint32_t)p[2] << 16 | (uint32_t)p[1] << 8 | p[0]; }
+uint16_t le16(const uint8_t* p) { return (uint16_t)(p[1] << 8 | p[0]); }
+
+/* PNG (ISO/IEC 15948): IHDR, optional PLTE, IDAT stream inflated by zlib,
+ * per-scanline filters None / Sub / Up / Average / Paeth; bit depth 8 or 16
+ * (the high byte kept) and 1, 2, 4 for grey (scaled to 8 bits) and palette;
+ * colour types grey, RGB, palette, grey + alpha, RGBA; alpha dropped, grey
+ * replicated; no interlacing. */
+bool decode_png(const std::vector<uint8_t>& b, int& w, int& h, std::vector<uint8_t>& rgb, std::string& err)
+{
+    static const uint8_t sig[8] = {137, 80, 78, 71, 13, 10, 26, 10};
+    if (b.size() < 8 || std::memcmp(b.data(), sig, 8) != 0) return err = "not a PNG file", false;
+    size_t pos = 8;
+    int depth = 0, ctype = -1, interlace = 0;
+    std::vector<uint8_t> idat, pal;
+    while (pos + 8 <= b.size()) {
+        uint32_t len = be32(&b[pos]);
+        std::string type((const char*)&b[pos + 4], 4);
+        if (pos + 12 + (size_t)len > b.size()) return err = "truncated PNG chunk", false;
+        const uint8_t* d = &b[pos + 8];
+        if (type == "IHDR") {
+            if (len < 13) return err = "bad IHDR", false;
+            w = (int)be32(d);
+            h = (int)be32(d + 4);
+            depth = d[8];
+            ctype = d[9];
+            interlace = d[12];
+        } else if (type == "PLTE") {
+            pal.assign(d, d + len);
+        } else if (type == "IDAT") {
+            idat.insert(idat.end(), d, d + len);
+        } else if (type == "IEND") {
+            break;
+        }
+        pos += 12 + len;
+    }
+    if (w <= 0 || h <= 0 || (int64_t)w * h > (1 << 28)) return err = "bad PNG size", false;
+    if (interlace) return err = "interlaced PNG is not supported", false;
+    int ch = ctype == 0 ? 1 : ctype == 2 ? 3 : ctype == 3 ? 1 : ctype == 4 ? 2 : ctype == 6 ? 4 : 0;
+    const bool sub = depth < 8;  // 1, 2, 4 bits: grey or palette only
+    if (!ch || !(depth == 8 || (depth == 16 && ctype != 3) ||
+                 (sub && (depth == 1 || depth == 2 || depth == 4) && (ctype == 0 || ctype == 3))))
+        return err = "unsupported PNG pixel format", false;
+    const size_t bpp = sub ? 1 : (size_t)ch * (depth / 8);
+    const size_t stride = sub ? ((size_t)w * depth + 7) / 8 : bpp * w;
+    std::vector<uint8_t> raw((stride + 1) * h);
+    uLongf rn = (uLongf)raw.size();
+    if (uncompress(raw.data(), &rn, idat.data(), (uLong)idat.size()) != Z_OK || rn != raw.size())
+        return err = "corrupt PNG image data", false;
+    std::vector<uint8_t> px(stride * h), prev(stride, 0);
+    for (int y = 0; y < h; ++y) {
+        const uint8_t f = raw[y * (stride + 1)];
+        const uint8_t* in = &raw[y * (stride + 1) + 1];
+        uint8_t* out = &px[y * stride];
+        for (size_t i = 0; i < stride; ++i) {
+            int a = i >= bpp ? out[i - bpp] : 0, up = prev[i], c = i >= bpp ? prev[i - bpp] : 0, v;
+            switch (f) {
+            case 0: v = in[i]; break;
+            case 1: v = in[i] + a; break;
+            case 2: v = in[i] + up; break;
+            case 3: v = in[i] + ((a + up) >> 1); break;
+            case 4: {
+                int p = a + up - c, pa = std::abs(p - a), pb = std::abs(p - up), pc = std::abs(p - c);
+                v = in[i] + (pa <= pb && pa <= pc ? a : pb <= pc ? up : c);
+                break;
+            }
+            default: return err = "bad PNG filter", false;
+            }
+            out[i] = (uint8_t)v;
+        }
+        std::memcpy(prev.data(), out, stride);
+    }
+    rgb.resize((size_t)3 * w * h);
+    if (sub) {  // unpack: grey scaled to 8 bits, palette indices kept
+        std::vector<uint8_t> un((size_t)w * h);
+        const int per = 8 / depth, mask = (1 << depth) - 1;
+        for (int y = 0; y < h; ++y)
+            for (int x = 0; x < w; ++x) {
+                int v = (px[(size_t)y * stride + x / per] >> ((per - 1 - x % per) * depth)) & mask;
+                un[(size_t)y * w + x] = (uint8_t)(ctype == 0 ? v * 255 / mask : v);
+            }
+        px.swap(un);
+    }
+    const size_t step = sub ? 1 : depth / 8, pbpp = sub ? 1 : bpp;
+    for (size_t i = 0; i < (size_t)w * h; ++i) {
+        const uint8_t* q = &px[i * pbpp];
+        auto sample = [&](int k) { return q[k * step]; };  // high byte of 16-bit samples
+        uint8_t r, g, bl;
+        if (ctype == 3) {
+            size_t k = (size_t)q[0] * 3;
+            if (k + 2 >= pal.size()) return err = "PNG palette index out of range", false;
+            r = pal[k], g = pal[k + 1], bl = pal[k + 2];
+        } else if (ctype == 0 || ctype == 4) {
+            r = g = bl = sample(0);
+        } else {
+            r = sample(0), g = sample(1), bl = sample(2);
+        }
+        rgb[3 * i] = r, rgb[3 * i + 1] = g, rgb[3 * i + 2] = bl;
+    }
+    return true;
+}
+
+/* Binary PPM / PGM (P6, P5), maxval up to 65535 (high byte kept above 255). */
+bool decode_pnm(const std::vector<uint8_t>& b, int& w, int& h, std::vector<uint8_t>& rgb, std::string& err)
+{
+    size_t pos = 2;
+    auto field = [&](int& v) {
+        while (pos < b.size()) {
+            if (b[pos] == '#') {
+                while (pos < b.size() && b[pos] != '\n') ++pos;
+            } else if (std::isspace(b[pos])) {
+                ++pos;
+            } else {
+                break;
+            }
+        }
+        if (pos >= b.size() || !std::isdigit(b[pos])) return false;
+        v = 0;
+        while (pos < b.size() && std::isdigit(b[pos])) v = v * 10 + (b[pos++] - '0');
+        return true;
+    };
+    const bool color = b[1] == '6';
+    int maxv = 0;
+    if (!field(w) || !field(h) || !field(maxv) || w <= 0 || h <= 0 || maxv <= 0 || maxv > 65535)
+        return err = "bad PNM header", false;
+    ++pos;  // one whitespace byte after maxval
+    const int ch = color ? 3 : 1, bytes = maxv > 255 ? 2 : 1;
+    if (pos + (size_t)w * h * ch * bytes > b.size()) return err = "truncated PNM data", false;
+    rgb.resize((size_t)3 * w * h);
+    for (size_t i = 0; i < (size_t)w * h; ++i)
+        for (int k = 0; k < 3; ++k) {
+            size_t at = pos + (i * ch + (color ? k : 0)) * bytes;
+            rgb[3 * i + k] = b[at];
+        }
+    return true;
+}
+
+/* BMP: BITMAPINFOHEADER or later, uncompressed 24 or 32 bits, rows bottom-up
+ * (or top-down for a negative height), each padded to 4 bytes, BGR(A). */
+bool decode_bmp(const std::vector<uint8_t>& b, int& w, int& h, std::vector<uint8_t>& rgb, std::string& err)
+{
+    if (b.size() < 54) return err = "truncated BMP", false;
+    const uint32_t off = le32(&b[10]);
+    w = (int32_t)le32(&b[18]);
+    int32_t hh = (int32_t)le32(&b[22]);
+    const int bits = le16(&b[28]);
+    const uint32_t comp = le32(&b[30]);
+    if (w <= 0 || hh == 0 || !(bits == 24 || bits == 32) || !(comp == 0 || (comp == 3 && bits == 32)))
+        return err = "unsupported BMP format", false;
+    h = hh < 0 ? -hh : hh;
+    const size_t bpp = bits / 8, row = ((size_t)w * bpp + 3) & ~(size_t)3;
+    if (off + row * h > b.size()) return err = "truncated BMP data", false;
+    rgb.resize((size_t)3 * w * h);
+    for (int y = 0; y < h; ++y) {
+        const uint8_t* src = &b[off + row * (hh < 0 ? y : h - 1 - y)];
+        for (int x = 0; x < w; ++x) {
+            uint8_t* o = &rgb[3 * ((size_t)y * w + x)];
+            o[0] = src[x * bpp + 2], o[1] = src[x * bpp + 1], o[2] = src[x * bpp];
+        }
+    }
+    return true;
+}
+
+}  // namespace
+
+/* An 8-bit image as the reference reads textures (cv::imread with the
+ * default flags, then convertTo(CV_32F, 1./255.), src/texturematerial.cpp:46-60):
+ * RGB floats byte / 255, row 0 at the top.  PNG, binary PPM / PGM and BMP. */
+bool rt_read_image8(const std::string& path, int& w, int& h, std::vector<float>& rgb, std::string& err)
+{
+    std::vector<uint8_t> b;
+    if (!read_file(path, b)) return err = "cannot open " + path, false;
+    std::vector<uint8_t> px;
+    bool ok;
+    if (b.size() >= 8 && b[0] == 137 && b[1] == 'P' && b[2] == 'N' && b[3] == 'G')
+        ok = decode_png(b, w, h, px, err);
+    else if (b.size() >= 3 && b[0] == 'P' && (b[1] == '6' || b[1] == '5'))
+        ok = decode_pnm(b, w, h, px, err);
+    else if (b.size() >= 2 && b[0] == 'B' && b[1] == 'M')
+        ok = decode_bmp(b, w, h, px, err);
+    else
+        return err = "unrecognised image format: " + path, false;
+    if (!ok) return false;
+    rgb.resize(px.size());
+    for (size_t i = 0; i < px.size(); ++i) rgb[i] = (float)(px[i] * (1.0 / 255.0));
+    return true;
+}
+
 extern "C" {
 
 const char* rt_film_last_error(void) { return g_film_error.c_str(); }
+
+int rt_image_read(const char* path, int32_t* width, int32_t* height, float* rgb, int64_t cap)
+{
+    if (!path || !width || !height) return fail(RT_ERR_INVALID, "invalid argument");
+    int w = 0, h = 0;
+    std::vector<float> img;
+    std::string err;
+    if (!rt_read_image8(path, w, h, img, err)) return fail(RT_ERR_IO, err);
+    *width = w;
+    *height = h;
+    if (rgb) {
+        if (cap < (int64_t)img.size()) return fail(RT_ERR_INVALID, "buffer too small");
+        std::memcpy(rgb, img.data(), img.size() * sizeof(float));
+    }
+    return RT_OK;
+}
 
 int rt_image_read_hdr(const char* path, int32_t* width, int32_t* height, float* rgb, int64_t cap)
 {

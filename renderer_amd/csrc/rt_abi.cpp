@@ -43,7 +43,7 @@ struct rt_ctx {
     bool has_scene = false;
     HostDevScene scene{};
     DevBuf nodes, prims, prim_id, mat, vnormals, geo, mats, lights, env_rgb, env_row, env_col, env_p, probe_rgb;
-    DevBuf obj_nodes, obj_first;
+    DevBuf obj_nodes, obj_first, tex_rgb, tex_info, tri_tc;
     int bvh_depth = 0;
     int bvh_stack = 0;  /* most stack entries a traversal can need */
 
@@ -223,6 +223,7 @@ int rt_destroy(rt_ctx* c)
                      &c->rb_out1, &c->stack_ovf, &c->tm_list, &c->tm_count, &c->counters, &c->views, &c->tiles, &c->passes, &c->films, &c->g_theta,
                      &c->g_w16, &c->g_m, &c->g_v, &c->g_grad, &c->g_acc, &c->g_count, &c->g_rec, &c->g_aux,
                      &c->g_open, &c->g_out, &c->g_tmp_rec, &c->g_tmp_y, &c->obj_nodes, &c->obj_first,
+                     &c->tex_rgb, &c->tex_info, &c->tri_tc,
                      &c->mlt_cur, &c->mlt_prop, &c->mlt_val, &c->mlt_energy, &c->mlt_acc, &c->mlt_cnt,
                      &c->mlt_seed, &c->mlt_cdf, &c->mlt_nch, &c->mlt_src, &c->mlt_view};
     for (DevBuf* b : all) release(*b);
@@ -501,8 +502,25 @@ int rt_scene_upload(rt_ctx* c, const rt_scene_desc* d)
     }
     std::vector<RtMaterial> mats(d->n_materials);
     for (int i = 0; i < d->n_materials; ++i)
-        if (d->materials[i].type < RT_MAT_DIFFUSE || d->materials[i].type > RT_MAT_REFRACTIVE)
+        if (d->materials[i].type < RT_MAT_DIFFUSE || d->materials[i].type > RT_MAT_TEXTURE)
             return set_err(c, RT_ERR_INVALID, "unknown material type");
+        else if (d->materials[i].type == RT_MAT_TEXTURE &&
+                 (d->materials[i].texture < 0 || d->materials[i].texture >= d->n_textures))
+            return set_err(c, RT_ERR_INVALID, "texture index out of range");
+    /* textures: all texels in one (rgb, 0) array, a table of (first, w, h) */
+    std::vector<float> tex;
+    std::vector<int32_t> tex_info;
+    if (d->n_textures < 0 || (d->n_textures > 0 && !d->textures)) return set_err(c, RT_ERR_INVALID, "bad textures");
+    for (int i = 0; i < d->n_textures; ++i) {
+        const rt_texture& t = d->textures[i];
+        if (t.width <= 0 || t.height <= 0 || !t.rgb) return set_err(c, RT_ERR_INVALID, "bad texture");
+        const size_t first = tex.size() / 4;
+        tex_info.insert(tex_info.end(), {(int32_t)first, t.width, t.height, 0});
+        for (size_t k = 0; k < (size_t)t.width * t.height; ++k)
+            tex.insert(tex.end(), {t.rgb[3 * k], t.rgb[3 * k + 1], t.rgb[3 * k + 2], 0.f});
+    }
+    std::vector<float> tri_tc;
+    if (d->tri_texcoords && nT) tri_tc.assign(d->tri_texcoords, d->tri_texcoords + (size_t)6 * nT);
     for (int i = 0; i < d->n_materials; ++i) {
         const rt_material& m = d->materials[i];
         RtMaterial& M = mats[i];
@@ -513,6 +531,7 @@ int rt_scene_upload(rt_ctx* c, const rt_scene_desc* d)
             M.ks[k] = m.ks[k];
         }
         M.exponent = m.exponent;
+        M.texture = m.type == RT_MAT_TEXTURE ? m.texture : -1;
         M.rho_d = lum_rgb(m.kd);
         M.rho_s = lum_rgb(m.ks);
     }
@@ -594,6 +613,9 @@ int rt_scene_upload(rt_ctx* c, const rt_scene_desc* d)
         (r = upload(c, c->geo, geo.data(), geo.size() * 4)) ||
         (r = upload(c, c->vnormals, vn.data(), vn.size() * 4)) ||
         (r = upload(c, c->obj_nodes, onodes.data(), onodes.size() * 4)) ||
+        (r = upload(c, c->tex_rgb, tex.data(), tex.size() * 4)) ||
+        (r = upload(c, c->tex_info, tex_info.data(), tex_info.size() * 4)) ||
+        (r = upload(c, c->tri_tc, tri_tc.data(), tri_tc.size() * 4)) ||
         (r = upload(c, c->obj_first, ofirst.data(), ofirst.size() * 4)) ||
         (r = upload(c, c->mats, mats.data(), mats.size() * sizeof(RtMaterial))) ||
         (r = upload(c, c->lights, lights.data(), lights.size() * sizeof(RtLight))) ||
@@ -631,6 +653,9 @@ int rt_scene_upload(rt_ctx* c, const rt_scene_desc* d)
     s.obj_first = ofirst.empty() ? nullptr : (const int32_t*)c->obj_first.p;
     s.n_obj = d->n_objects;
     s.g_obj = G;
+    s.tex_rgb = tex.empty() ? nullptr : c->tex_rgb.p;
+    s.tex_info = tex_info.empty() ? nullptr : c->tex_info.p;
+    s.tri_tc = tri_tc.empty() ? nullptr : (const float*)c->tri_tc.p;
     c->has_scene = true;
     c->bvh_depth = br.depth;
     c->bvh_stack = br.stack_need;

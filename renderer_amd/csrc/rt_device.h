@@ -110,6 +110,12 @@ struct DevScene {
     const float4* obj_nodes;
     const int32_t* obj_first;
     int32_t n_obj, g_obj;
+    /* textures: texels (rgb, 0) of all images; per image (first texel,
+     * width, height, 0); texture coordinates of the triangles (6 floats per
+     * triangle) or null for the default ones */
+    const float4* tex_rgb;
+    const int4* tex_info;
+    const float* tri_tc;
 };
 
 /* ---------------------------------------------------------------------------
@@ -688,6 +694,76 @@ __device__ __forceinline__ vec3 hit_normal(const DevScene& S, int32_t g, vec3 p,
     const int code = (int)u, axis = code >> 1;
     const float sgn = (code & 1) ? 1.f : -1.f;
     return mk(axis == 0 ? sgn : 0.f, axis == 1 ? sgn : 0.f, axis == 2 ? sgn : 0.f);
+}
+
+/* Texture coordinates of a hit: triangles interpolate their vertices'
+ * (src/triangle.cpp:54), spheres map the unnormalised normal p - c
+ * (src/sphere.cpp:47), everything else reports (0, 0). */
+__device__ __forceinline__ void hit_texcoord(const DevScene& S, int32_t g, vec3 p, float u, float v, float& s,
+                                             float& t)
+{
+    s = t = 0.f;
+    if (g < S.n_tri) {
+        float t1x = 0.f, t1y = 0.f, t2x = 0.f, t2y = 1.f, t3x = 1.f, t3y = 0.f;
+        if (S.tri_tc) {
+            const float* q = S.tri_tc + 6 * (size_t)g;
+            t1x = q[0], t1y = q[1], t2x = q[2], t2y = q[3], t3x = q[4], t3y = q[5];
+        }
+        const float w = 1.f - u - v;
+        s = w * t1x + u * t2x + v * t3x;
+        t = w * t1y + u * t2y + v * t3y;
+        return;
+    }
+    const float4 gg = S.geo[g];
+    const bool sphere = g < S.n_tri + S.n_sphere || (g >= S.g_obj && gg.w > 0.f);
+    if (!sphere) return;
+    const vec3 n = p - xyz(gg);
+    s = (float)((double)acosf(n.z / gg.w) / M_PI);
+    t = (float)((double)atan2f(n.x, n.y) / M_PI / 2.0 + 0.5);
+}
+
+/* TextureMaterial::get (src/texturematerial.cpp:62-72) and the gamma of
+ * its brdf: the coordinate wrapped into [0, 1) (clamp), the point
+ * (s (W - 1), (1 - t) (H - 1)) sampled bilinearly with the border
+ * replicated (cv::getRectSubPix of a 1x1 patch), times the coefficient,
+ * raised to gamma.  The diffuse albedo of the hit (the brdf is it / pi). */
+__device__ __forceinline__ vec3 texture_albedo(const DevScene& S, const RtMaterial& M, int32_t g, vec3 p, float u,
+                                               float v)
+{
+    float s, t;
+    hit_texcoord(S, g, p, u, v, s, t);
+    auto wrap = [](float x) {
+        float f = fmodf(x, 1.f);
+        return f < 0.f ? f + 1.f : f;
+    };
+    const int4 info = S.tex_info[M.texture];
+    const int W = info.y, H = info.z;
+    const float cx = wrap(s) * (float)(W - 1), cy = wrap(1.f - t) * (float)(H - 1);
+    const float fx = floorf(cx), fy = floorf(cy), a = cx - fx, b = cy - fy;
+    const int x0 = min(max((int)fx, 0), W - 1), y0 = min(max((int)fy, 0), H - 1);
+    const int x1 = min(x0 + 1, W - 1), y1 = min(y0 + 1, H - 1);
+    const float4* img = S.tex_rgb + info.x;
+    const float4 p00 = img[(size_t)y0 * W + x0], p01 = img[(size_t)y0 * W + x1];
+    const float4 p10 = img[(size_t)y1 * W + x0], p11 = img[(size_t)y1 * W + x1];
+    const float w00 = (1.f - a) * (1.f - b), w01 = a * (1.f - b), w10 = (1.f - a) * b, w11 = a * b;
+    vec3 c = mk(p00.x * w00 + p01.x * w01 + p10.x * w10 + p11.x * w11,
+                p00.y * w00 + p01.y * w01 + p10.y * w10 + p11.y * w11,
+                p00.z * w00 + p01.z * w01 + p10.z * w10 + p11.z * w11);
+    const float gamma = M.exponent;
+    return mk(powf(c.x * M.kd[0], gamma), powf(c.y * M.kd[1], gamma), powf(c.z * M.kd[2], gamma));
+}
+
+/* The material of a hit as the shading code sees it: a texture becomes a
+ * diffuse material with the texture's albedo at the hit. */
+__device__ __forceinline__ RtMaterial hit_material(const DevScene& S, int32_t g, vec3 p, float u, float v)
+{
+    RtMaterial M = S.mats[S.mat[g]];
+    if (M.type == RT_MAT_TEXTURE) {
+        const vec3 a = texture_albedo(S, M, g, p, u, v);
+        M.type = RT_MAT_DIFFUSE;
+        M.kd[0] = a.x, M.kd[1] = a.y, M.kd[2] = a.z;
+    }
+    return M;
 }
 
 #endif

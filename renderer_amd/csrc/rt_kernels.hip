@@ -853,8 +853,13 @@ __device__ void shade_compat(const DevScene& S, const PathState& ps, const PtPar
         }
         /* replace the pending estimate of the previous stored vertex (:70-82) */
         float4 px = ps.pend_x[p];
-        vec3 pn = xyz(ps.pend_n[p]), pd = xyz(ps.pend_d[p]), pa = xyz(ps.pend_a[p]);
-        const RtMaterial& pm = S.mats[__float_as_int(px.w)];
+        float4 pn4 = ps.pend_n[p], pd4 = ps.pend_d[p], pa4 = ps.pend_a[p];
+        vec3 pn = xyz(pn4), pd = xyz(pd4), pa = xyz(pa4);
+        RtMaterial pm = S.mats[__float_as_int(px.w)];
+        if (pm.type == RT_MAT_TEXTURE) {  /* the albedo at that vertex, kept in the w components */
+            pm.type = RT_MAT_DIFFUSE;
+            pm.kd[0] = pn4.w, pm.kd[1] = pd4.w, pm.kd[2] = pa4.w;
+        }
         vec3 f = brdf_ref(pm, pn, pd, edir);
         float in_cos = dot(-norm(edir), pn), out_cos = dot(-pd, pn);
         float4 rad = ps.radiance[p];
@@ -867,8 +872,8 @@ __device__ void shade_compat(const DevScene& S, const PathState& ps, const PtPar
         return;
     }
 
-    const RtMaterial& M = S.mats[S.mat[g]];
     vec3 x = o + h.x * d;
+    const RtMaterial M = hit_material(S, g, x, h.z, h.w);
     vec3 n = hit_normal(S, g, x, h.z, h.w);
     U4 gs = path_u4(P, info, p, 1u + 2u * depth);
 
@@ -972,9 +977,9 @@ __device__ void shade_compat(const DevScene& S, const PathState& ps, const PtPar
         }
     }
     ps.pend_x[p] = make_float4(x.x, x.y, x.z, __int_as_float(S.mat[g]));
-    ps.pend_n[p] = make_float4(n.x, n.y, n.z, 0.f);
-    ps.pend_d[p] = make_float4(dn.x, dn.y, dn.z, 0.f);
-    ps.pend_a[p] = al;
+    ps.pend_n[p] = make_float4(n.x, n.y, n.z, M.kd[0]);
+    ps.pend_d[p] = make_float4(dn.x, dn.y, dn.z, M.kd[1]);
+    ps.pend_a[p] = make_float4(al.x, al.y, al.z, M.kd[2]);
     ++k;
     if (pdf == 0.f || is_zero(alpha)) {
         ps.info[p] = make_int4(info.x, info.y, info.z, pk(k, L, 0));
@@ -1014,8 +1019,8 @@ __device__ void shade_unbiased(const DevScene& S, const PathState& ps, const PtP
     float4 al = ps.alpha[p];
     vec3 alpha = xyz(al);
     if (add_emission(S, ps, p, g, spec, h.x, d, al)) return;
-    const RtMaterial& M = S.mats[S.mat[g]];
     vec3 x = o + h.x * d;
+    const RtMaterial M = hit_material(S, g, x, h.z, h.w);
     vec3 n = hit_normal(S, g, x, h.z, h.w);
     vec3 nf = dot(n, d) > 0.f ? -n : n;
     if (M.type == RT_MAT_MIRROR || M.type == RT_MAT_REFRACTIVE) {
@@ -1115,8 +1120,8 @@ __device__ void shade_guided(const DevScene& S, const PathState& ps, const Guide
     float4 al = ps.alpha[p];
     vec3 alpha = xyz(al);
     if (add_emission(S, ps, p, g, spec, h.x, d, al)) return;
-    const RtMaterial& M = S.mats[S.mat[g]];
     vec3 x = o + h.x * d;
+    const RtMaterial M = hit_material(S, g, x, h.z, h.w);
     vec3 n = hit_normal(S, g, x, h.z, h.w);
     vec3 nf = dot(n, d) > 0.f ? -n : n;
     if (M.type == RT_MAT_MIRROR || M.type == RT_MAT_REFRACTIVE) {

@@ -86,7 +86,8 @@ struct RtMaterial {
     float ks[3];
     float exponent;
     float rho_d, rho_s;  /* luminance of kd / ks (Phong lobe pick) */
-    float pad[6];
+    int32_t texture;     /* RT_MAT_TEXTURE: texture index (kd = coefficient, exponent = gamma) */
+    float pad[5];
 };
 
 /* Light record (64 B).  type: rt_light_type.  n = u x v (not normalised,

@@ -41,6 +41,7 @@
 
 /* film_io.cpp: Radiance RGBE reader */
 bool rt_read_rgbe(const std::string& path, int& w, int& h, std::vector<float>& rgb, std::string& err);
+bool rt_read_image8(const std::string& path, int& w, int& h, std::vector<float>& rgb, std::string& err);
 
 namespace {
 
@@ -626,14 +627,6 @@ bool read_file(const std::string& path, std::string& out)
     char buf[1 << 16];
     size_t r;
     while ((r = std::fread(buf, 1, sizeof buf, fp)) > 0) out.append(buf, r);
-    std::fclose(fp);
-    return true;
-}
-
-bool readable(const std::string& path)
-{
-    FILE* fp = std::fopen(path.c_str(), "rb");
-    if (!fp) return false;
     std::fclose(fp);
     return true;
 }
@@ -1543,6 +1536,7 @@ struct Geometry {
     std::vector<float> tri_n;   // 9 per triangle (valid where tri_has_n)
     std::vector<uint8_t> tri_has_n;
     std::vector<int32_t> tri_mat, tri_grp;
+    std::vector<float> tri_tc;  // 6 per triangle: texture coordinates at p1, p2, p3
     std::vector<float> sph;     // 4 per sphere
     std::vector<int32_t> sph_mat, sph_grp;
     std::vector<rt_csg_node> csg;           // postfix programs of planes, boxes, CSG trees
@@ -1657,14 +1651,30 @@ private:
             b.m.exponent = m.f;
             break;
         case MatKind::Texture:
-        case MatKind::ScaledTexture:
-            if (!readable(m.s)) {  // TextureMaterial::load failed -> DarkMatter
-                log_ += "Could not load texture " + m.s + "\n";
+        case MatKind::ScaledTexture: {
+            // TextureMaterial(coefficient, gamma) + load (src/sceneast.cpp:345-365,
+            // src/texturematerial.cpp:46-60); an unreadable image is DarkMatter
+            int w = 0, h = 0;
+            std::vector<float> img;
+            std::string err;
+            if (!rt_read_image8(m.s, w, h, img, err)) {
+                log_ += "Error loading texture \"" + m.s + "\"\n";
                 return dark_matter();
             }
-            b.m.type = RT_MAT_DARK;
-            b.unsupported = "texture material (" + m.s + ")";
+            b.m.type = RT_MAT_TEXTURE;
+            if (m.kind == MatKind::ScaledTexture) {
+                b.m.kd[0] = m.a.x, b.m.kd[1] = m.a.y, b.m.kd[2] = m.a.z;
+                b.m.exponent = m.g;
+            } else {
+                b.m.kd[0] = b.m.kd[1] = b.m.kd[2] = 1.f;
+                b.m.exponent = 1.f;
+            }
+            b.m.texture = (int32_t)tex_images.size();
+            tex_images.push_back(std::move(img));
+            tex_w.push_back(w);
+            tex_h.push_back(h);
             break;
+        }
         case MatKind::Ident: return resolve(m);
         }
         int idx = (int)built.size();
@@ -1698,11 +1708,20 @@ private:
         return idx;
     }
     std::map<int, int> cache_;
-    std::set<std::string> missing_;  // the reference prints once per use; here once per name
+    std::set<std::string> missing_;
+  public:
+    std::vector<std::vector<float>> tex_images;  // texture images in material order
+    std::vector<int> tex_w, tex_h;
+  private:  // the reference prints once per use; here once per name
 
     void emit_tri(Geometry& g, const float* p1, const float* p2, const float* p3, const float* n, bool has_n, int mat,
-                  int grp, const M4* xf)
+                  int grp, const M4* xf, const float* tc = nullptr)
     {
+        // the Triangle constructors without texture coordinates use (0, 0), (0, 1),
+        // (1, 0) (src/triangle.h:35-36); t(...) and OBJ faces pass theirs
+        static const float def_tc[6] = {0, 0, 0, 1, 1, 0};
+        const float* t = tc ? tc : def_tc;
+        g.tri_tc.insert(g.tri_tc.end(), t, t + 6);
         const float* P[3] = {p1, p2, p3};
         float q[9];
         for (int k = 0; k < 3; ++k) {
@@ -1758,7 +1777,7 @@ private:
             int mat = material_of(n, g);
             // t(...) with a null normal is the flat-normal constructor (src/sceneast.cpp:510-520)
             bool has_n = !(is_null(n.f + 9) || is_null(n.f + 12) || is_null(n.f + 15));
-            emit_tri(g, n.f, n.f + 3, n.f + 6, n.f + 9, has_n, mat, id, xf);
+            emit_tri(g, n.f, n.f + 3, n.f + 6, n.f + 9, has_n, mat, id, xf, n.f + 18);
             break;
         }
         case NodeKind::Quad: {  // Triangle(p1,p2,p4), Triangle(p4,p2,p3)
@@ -1967,6 +1986,9 @@ struct rt_scene_file {
     std::vector<int32_t> tri_vertices, tri_material, tri_prim, sph_material, sph_prim;
     std::vector<rt_csg_node> csg_nodes;
     std::vector<int32_t> object_first, object_prim;
+    std::vector<float> tri_tc;
+    std::vector<std::vector<float>> tex_rgb;  // images of texture materials
+    std::vector<rt_texture> textures;
     std::vector<rt_material> materials;
     std::vector<rt_light> lights;
     std::vector<float> env_rgb;  /* light probe of an environmentmap(...) light */
@@ -2149,6 +2171,7 @@ int rt_scene_file_scene(rt_scene_file* f, rt_scene_desc* desc, rt_camera* camera
     f->sph_prim.resize(g.sph_grp.size());
     for (size_t i = 0; i < g.sph_grp.size(); ++i) f->sph_prim[i] = nL + g.sph_grp[i];
 
+    f->tri_tc = g.tri_tc;
     f->csg_nodes = g.csg;
     f->object_first = g.obj_first;
     f->object_prim.resize(g.obj_grp.size());
@@ -2173,6 +2196,19 @@ int rt_scene_file_scene(rt_scene_file* f, rt_scene_desc* desc, rt_camera* camera
     desc->materials = f->materials.empty() ? nullptr : f->materials.data();
     desc->n_lights = nL;
     desc->lights = f->lights.empty() ? nullptr : f->lights.data();
+    desc->tri_texcoords = nT ? f->tri_tc.data() : nullptr;
+    f->textures.clear();
+    for (size_t i = 0; i < ev.tex_images.size(); ++i) {
+        rt_texture t{};
+        t.width = ev.tex_w[i];
+        t.height = ev.tex_h[i];
+        t.rgb = ev.tex_images[i].data();
+        f->textures.push_back(t);
+    }
+    f->tex_rgb = std::move(ev.tex_images);
+    for (size_t i = 0; i < f->textures.size(); ++i) f->textures[i].rgb = f->tex_rgb[i].data();
+    desc->n_textures = (int32_t)f->textures.size();
+    desc->textures = f->textures.empty() ? nullptr : f->textures.data();
     if (!f->object_prim.empty()) {
         desc->n_objects = (int32_t)f->object_prim.size();
         desc->csg_nodes = f->csg_nodes.data();

@@ -1,7 +1,7 @@
 """Host-side scene model.
 
 Mirrors the reference's scene classes (Camera, DiffuseMaterial,
-MirrorMaterial, PhongMaterial, BlinnMaterial, TransparentMaterial,
+MirrorMaterial, PhongMaterial, BlinnMaterial, TransparentMaterial, TextureMaterial,
 DarkMatter, AreaLight, PointLight, ConeLight, EnvironmentMap, Triangle, Quad,
 Sphere, Plane, Box, CSGIntersection, CSGUnion, CSGDifference, CSGXor, IntersectableList,
 Scene) with the
@@ -84,6 +84,35 @@ class TransparentMaterial(Material):
         self.refraction_coefficient = float(refraction_coefficient)
 
 
+class TextureMaterial(Material):
+    """src/texturematerial.h: a diffuse surface whose albedo is an image,
+    pow(image(texcoord) * coefficient, gamma); image (H, W, 3) float RGB in
+    [0, 1] (byte / 255, as the reference converts it), row 0 at the top.
+    TextureMaterial.load reads a PNG, PPM or BMP file as the reference's
+    imread does."""
+
+    kind = _abi.MAT_TEXTURE
+
+    def __init__(self, image, coefficient=(1.0, 1.0, 1.0), gamma=1.0):
+        img = np.ascontiguousarray(image, dtype=np.float32)
+        if img.ndim != 3 or img.shape[2] != 3:
+            raise ValueError("texture image must be (H, W, 3)")
+        self.image, self.coefficient, self.gamma = img, _v3(coefficient), float(gamma)
+
+    @classmethod
+    def load(cls, path, coefficient=(1.0, 1.0, 1.0), gamma=1.0):
+        lib = _abi.load()
+        w, h = ctypes.c_int32(), ctypes.c_int32()
+        st = lib.rt_image_read(str(path).encode(), ctypes.byref(w), ctypes.byref(h), None, 0)
+        if st != 0:
+            raise _abi.RtError(st, lib.rt_film_last_error().decode(errors="replace"))
+        img = np.zeros((h.value, w.value, 3), np.float32)
+        st = lib.rt_image_read(str(path).encode(), ctypes.byref(w), ctypes.byref(h), img.ctypes.data, img.size)
+        if st != 0:
+            raise _abi.RtError(st, lib.rt_film_last_error().decode(errors="replace"))
+        return cls(img, coefficient, gamma)
+
+
 # ---------------------------------------------------------------------------
 # lights, src/light.h:31-37
 # ---------------------------------------------------------------------------
@@ -152,9 +181,11 @@ class Intersectable:
 class Triangle(Intersectable):
     """src/triangle.h: three points, optional vertex normals, material."""
 
-    def __init__(self, p1, p2, p3, material, normals=None):
+    def __init__(self, p1, p2, p3, material, normals=None, texcoords=None):
         self.p = np.stack([_v3(p1), _v3(p2), _v3(p3)])
         self.normals = None if normals is None else np.asarray(normals, np.float32).reshape(3, 3)
+        # (u, v) at p1, p2, p3; None = the reference's default (0, 0), (0, 1), (1, 0)
+        self.texcoords = None if texcoords is None else np.asarray(texcoords, np.float32).reshape(3, 2)
         self.material = material
 
 
@@ -222,7 +253,8 @@ class Mesh(Intersectable):
     """Indexed triangle mesh (what the reference's obj(...) produces): one
     primitive per triangle, one material for all or one per face."""
 
-    def __init__(self, positions, faces, material, normals=None, face_materials=None):
+    def __init__(self, positions, faces, material, normals=None, face_materials=None, texcoords=None):
+        self.texcoords = None if texcoords is None else np.ascontiguousarray(texcoords, dtype=np.float32)  # per vertex
         self.positions = np.ascontiguousarray(positions, dtype=np.float32)
         self.faces = np.ascontiguousarray(faces, dtype=np.int32)
         self.normals = None if normals is None else np.ascontiguousarray(normals, dtype=np.float32)
@@ -258,7 +290,7 @@ class SceneArrays:
 
     def __init__(self, positions, normals, tri_vertices, tri_material, tri_prim, spheres, sphere_material,
                  sphere_prim, materials, lights, env, env_kind=0, csg_nodes=(), object_first=None,
-                 object_prim=None):
+                 object_prim=None, textures=(), tri_texcoords=None):
         self.positions, self.normals = positions, normals
         self.tri_vertices, self.tri_material, self.tri_prim = tri_vertices, tri_material, tri_prim
         self.spheres, self.sphere_material, self.sphere_prim = spheres, sphere_material, sphere_prim
@@ -267,6 +299,8 @@ class SceneArrays:
         self.csg_nodes = list(csg_nodes)
         self.object_first = np.zeros(1, np.int32) if object_first is None else np.asarray(object_first, np.int32)
         self.object_prim = np.zeros(0, np.int32) if object_prim is None else np.asarray(object_prim, np.int32)
+        self.textures = list(textures)
+        self.tri_texcoords = tri_texcoords
         self._desc = None
 
     @property
@@ -309,6 +343,15 @@ class SceneArrays:
             d.csg_nodes = ctypes.addressof(self._nodes_c)
             d.object_first = self.object_first.ctypes.data
             d.object_prim_id = self.object_prim.ctypes.data
+        if self.textures:
+            self._tex_c = (_abi.RtTexture * len(self.textures))()
+            for i, img in enumerate(self.textures):
+                self._tex_c[i].height, self._tex_c[i].width = img.shape[0], img.shape[1]
+                self._tex_c[i].rgb = img.ctypes.data
+            d.n_textures = len(self.textures)
+            d.textures = ctypes.addressof(self._tex_c)
+        if self.tri_texcoords is not None:
+            d.tri_texcoords = self.tri_texcoords.ctypes.data
         self._desc = d
         return d
 
@@ -322,7 +365,7 @@ class Scene:
         self.object = intersectable
 
     def flatten(self):
-        mats, mat_index = [], {}
+        mats, mat_index, textures = [], {}, []
 
         def mid(m):
             if id(m) not in mat_index:
@@ -336,6 +379,11 @@ class Scene:
                     c.exponent = m.specular_coefficient
                 elif isinstance(m, TransparentMaterial):
                     c.exponent = m.refraction_coefficient
+                elif isinstance(m, TextureMaterial):
+                    c.kd[:] = m.coefficient.tolist()
+                    c.exponent = m.gamma
+                    c.texture = len(textures)
+                    textures.append(m.image)
                 mat_index[id(m)] = len(mats)
                 mats.append(c)
             return mat_index[id(m)]
@@ -364,7 +412,7 @@ class Scene:
             c.intensity[:] = (L.coefficient if isinstance(L, EnvironmentMap) else L.intensity).tolist()
             lights.append(c)
 
-        pos, nrm, tv, tm, tp = [], [], [], [], []
+        pos, nrm, tv, tm, tp, tcs = [], [], [], [], [], []
         sph, sm, sp = [], [], []
         nodes, ofirst, oprim = [], [0], []
 
@@ -390,7 +438,8 @@ class Scene:
         has_normals = [False]
         next_id = [len(lights)]
 
-        def add_tris(P, F, N, mats_per_face, ids):
+        def add_tris(P, F, N, mats_per_face, ids, tc=None):
+            tcs.append(tc)
             pos.append(P)
             nrm.append(N)
             if N is not None:
@@ -406,7 +455,7 @@ class Scene:
                     walk(it)
             elif isinstance(o, Triangle):
                 add_tris(o.p, np.array([[0, 1, 2]], np.int32), o.normals, np.array([mid(o.material)], np.int32),
-                         np.array([next_id[0]], np.int32))
+                         np.array([next_id[0]], np.int32), None if o.texcoords is None else o.texcoords.reshape(1, 6))
                 next_id[0] += 1
             elif isinstance(o, Quad):
                 add_tris(o.p, np.array([[0, 1, 3], [3, 1, 2]], np.int32), None,
@@ -439,7 +488,8 @@ class Scene:
                 else:
                     fm = np.full(n, mid(o.material), np.int32)
                 add_tris(o.positions, o.faces, o.normals, fm.astype(np.int32),
-                         np.arange(next_id[0], next_id[0] + n, dtype=np.int32))
+                         np.arange(next_id[0], next_id[0] + n, dtype=np.int32),
+                         None if o.texcoords is None else o.texcoords[o.faces].reshape(n, 6))
                 next_id[0] += n
             else:
                 raise TypeError(type(o))
@@ -455,9 +505,14 @@ class Scene:
         tri_material = np.ascontiguousarray(np.concatenate(tm) if tm else np.zeros(0, np.int32), np.int32)
         tri_prim = np.ascontiguousarray(np.concatenate(tp) if tp else np.zeros(0, np.int32), np.int32)
         spheres = np.ascontiguousarray(np.stack(sph) if sph else np.zeros((0, 4), np.float32), np.float32)
+        tri_tc = None
+        if any(t is not None for t in tcs):
+            default = np.array([0, 0, 0, 1, 1, 0], np.float32)
+            tri_tc = np.ascontiguousarray(np.concatenate(
+                [t if t is not None else np.tile(default, (len(F), 1)) for t, F in zip(tcs, tv)]), np.float32)
         return SceneArrays(positions, normals, tri_vertices, tri_material, tri_prim, spheres,
                            np.asarray(sm, np.int32), np.asarray(sp, np.int32), mats, lights, env, env_kind,
-                           nodes, np.asarray(ofirst, np.int32), np.asarray(oprim, np.int32))
+                           nodes, np.asarray(ofirst, np.int32), np.asarray(oprim, np.int32), textures, tri_tc)
 
 
 def _flat_vertex_normals(P, F):

@@ -1,0 +1,124 @@
+"""Texture materials (src/texturematerial.cpp, src/triangle.cpp:54,
+src/sphere.cpp:47): the image reader against files written by PIL and by
+the library's own writers, and the oracle's texture lookup pinned at texel
+centres (the texel itself), between texels (the bilinear mean), under
+wrap-around and gamma, plus the texture coordinates of triangle and sphere
+hits at closed-form points; a constant texture renders like the diffuse
+material of the same albedo."""
+import math
+
+import numpy as np
+import pytest
+from PIL import Image
+
+from oracle.oracle import Oracle
+from renderer_amd import _abi
+from renderer_amd.film import write_image
+from renderer_amd.scene import (Camera, DiffuseMaterial, IntersectableList, PointLight, Quad, Scene, Sphere,
+                                TextureMaterial, Triangle)
+
+
+def _rgb8(rng, h, w):
+    return rng.integers(0, 256, (h, w, 3), dtype=np.uint8)
+
+
+@pytest.mark.parametrize("fmt", ["png_rgb", "png_rgba", "png_gray", "png_palette", "png16", "bmp", "ppm", "own_png",
+                                 "own_bmp"])
+def test_image_reader(tmp_path, fmt):
+    rng = np.random.default_rng(3)
+    px = _rgb8(rng, 7, 11)
+    path = tmp_path / ("img." + ("bmp" if "bmp" in fmt else "ppm" if fmt == "ppm" else "png"))
+    want = px.astype(np.float64)
+    if fmt == "png_rgb" or fmt == "bmp" or fmt == "ppm":
+        Image.fromarray(px, "RGB").save(path)
+    elif fmt == "png_rgba":
+        Image.fromarray(np.dstack([px, rng.integers(0, 256, (7, 11), dtype=np.uint8)]), "RGBA").save(path)
+    elif fmt == "png_gray":
+        g = px[..., 0]
+        Image.fromarray(g, "L").save(path)
+        want = np.repeat(g[..., None], 3, axis=2).astype(np.float64)
+    elif fmt == "png_palette":
+        im = Image.fromarray(px, "RGB").convert("P", palette=Image.ADAPTIVE, colors=16)
+        im.save(path)
+        want = np.asarray(im.convert("RGB")).astype(np.float64)
+    elif fmt == "png16":
+        g16 = (px[..., 0].astype(np.uint16) << 8) | 0x5A
+        Image.frombytes("I;16", (11, 7), g16.astype("<u2").tobytes()).save(path)
+        want = np.repeat(px[..., :1], 3, axis=2).astype(np.float64)   # the high byte
+    elif fmt in ("own_png", "own_bmp"):
+        write_image(str(path), px)
+    tex = TextureMaterial.load(path)
+    assert tex.image.shape == (7, 11, 3)
+    assert np.array_equal(tex.image, (want / 255.0).astype(np.float32))
+
+
+def test_image_reader_rejects_what_it_cannot_read(tmp_path):
+    bad = tmp_path / "x.png"
+    bad.write_bytes(b"\x89PNG\r\n\x1a\n" + b"\x00" * 20)
+    with pytest.raises(_abi.RtError) as e:
+        TextureMaterial.load(bad)
+    assert e.value.status == _abi.ERR_IO
+    with pytest.raises(_abi.RtError):
+        TextureMaterial.load(tmp_path / "missing.png")
+
+
+def _scene(objs):
+    return Scene(Camera((0, 0, 3), (0, 0, 0), (0, 1, 0), 0.6, 16, 16), [PointLight((0, 0, 3), (4, 4, 4))],
+                 IntersectableList(objs))
+
+
+def test_texture_lookup_closed_forms():
+    img = np.zeros((3, 4, 3), np.float32)
+    img[..., 0] = np.arange(4)[None, :] / 4.0          # red: column / 4
+    img[..., 1] = np.arange(3)[:, None] / 3.0          # green: row / 3
+    img[..., 2] = 0.5
+    tex = TextureMaterial(img, (1.0, 1.0, 2.0), 1.0)
+    orc = Oracle(_scene([Quad((-1, -1, 0), (1, -1, 0), (1, 1, 0), (-1, 1, 0), tex)]).flatten())
+    W, H = 4, 3
+    for x in range(W - 1):
+        for y in range(H - 1):
+            # texel (x, y) sits at s = x / (W - 1), t = 1 - y / (H - 1)
+            v = orc.texture_albedo(0, x / (W - 1), 1 - y / (H - 1))
+            assert v == pytest.approx([x / 4.0, y / 3.0, 1.0], abs=1e-6)
+    # clamp() wraps with fmod, so s = 1 is s = 0 and t = 0 (1 - t = 1) is the top row
+    assert orc.texture_albedo(0, 1.0, 0.0) == pytest.approx([0.0, 0.0, 1.0], abs=1e-6)
+    # the last column and row are reached from below
+    assert orc.texture_albedo(0, 1 - 1e-7, 1e-7) == pytest.approx([0.75, 2 / 3, 1.0], abs=1e-5)
+    mid = orc.texture_albedo(0, 0.5 / (W - 1), 1 - 0.5 / (H - 1))   # between four texels
+    assert mid == pytest.approx([0.125, 1 / 6, 1.0], abs=1e-6)
+    assert orc.texture_albedo(0, 1.25, 0.5) == pytest.approx(orc.texture_albedo(0, 0.25, 0.5), abs=1e-6)
+    assert orc.texture_albedo(0, -0.25, 0.5) == pytest.approx(orc.texture_albedo(0, 0.75, 0.5), abs=1e-6)
+    g = Oracle(_scene([Quad((-1, -1, 0), (1, -1, 0), (1, 1, 0), (-1, 1, 0),
+                            TextureMaterial(img, (1, 1, 1), 2.2))]).flatten())
+    assert g.texture_albedo(0, 2 / 3, 0.5) == pytest.approx(np.array([0.5, 1 / 3, 0.5]) ** 2.2, rel=1e-5)
+
+
+def test_texture_coordinates_of_hits():
+    tri = Triangle((0, 0, 0), (1, 0, 0), (0, 1, 0), DiffuseMaterial((1, 1, 1)),
+                   texcoords=[[0.1, 0.2], [0.9, 0.3], [0.4, 0.8]])
+    sph = Sphere((3, 0, 0), 0.5, DiffuseMaterial((1, 1, 1)))
+    orc = Oracle(_scene([tri, sph]).flatten())
+    o = np.float32([[0.25, 0.5, 1, 1e-3], [3, 0, 2, 1e-3], [3.5 + 1, 0, 0, 1e-3]])
+    d = np.float32([[0, 0, -1, np.inf], [0, 0, -1, np.inf], [-1, 0, 0, np.inf]])
+    tc = orc.hit_texcoords(o, d)
+    # barycentrics of (0.25, 0.5): u = 0.25 along p2 - p1, v = 0.5 along p3 - p1
+    w = 1 - 0.25 - 0.5
+    assert tc[0] == pytest.approx([w * 0.1 + 0.25 * 0.9 + 0.5 * 0.4, w * 0.2 + 0.25 * 0.3 + 0.5 * 0.8], abs=1e-6)
+    # sphere: the top (n = (0, 0, r)) has s = acos(1) / pi = 0, t = atan2(0, 0) / 2pi + 1/2
+    assert tc[1] == pytest.approx([0.0, 0.5], abs=1e-6)
+    # the +x point: n = (r, 0, 0): s = acos(0) / pi = 1/2, t = atan2(r, 0) / 2pi + 1/2 = 3/4
+    assert tc[2] == pytest.approx([0.5, 0.75], abs=1e-6)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_constant_texture_renders_like_diffuse(mode):
+    c, k, gamma = np.float32([0.6, 0.4, 0.2]), np.float32([1.0, 0.9, 0.8]), 1.0
+    tex = TextureMaterial(np.tile(c, (5, 6, 1)), k, gamma)
+    dif = DiffuseMaterial(c * k)
+    a = _scene([Quad((-1, -1, 0), (1, -1, 0), (1, 1, 0), (-1, 1, 0), tex), Sphere((0, 0, 0.5), 0.3, tex)])
+    b = _scene([Quad((-1, -1, 0), (1, -1, 0), (1, 1, 0), (-1, 1, 0), dif), Sphere((0, 0, 0.5), 0.3, dif)])
+    ia, _ = Oracle(a.flatten()).render((a.camera, 2, 2), mode=mode, max_depth=4, seed=2)
+    ib, _ = Oracle(b.flatten()).render((b.camera, 2, 2), mode=mode, max_depth=4, seed=2)
+    assert ia.mean() > 0
+    # equal up to the rounding of the four bilinear weights' sum
+    assert np.allclose(ia, ib, rtol=1e-5, atol=1e-7)
