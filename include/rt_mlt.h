@@ -16,7 +16,13 @@
  *                          RT_MLT_ERPT: EnergyRedistributionRenderer::render
  *                          (src/energyredistributionrenderer.cpp:38-131),
  *                          energy redistribution path tracing (Cline,
- *                          Talbot, Egbert 2005)
+ *                          Talbot, Egbert 2005);
+ *                          RT_MLT_FILTERED: MetropolisFltRenderer::render
+ *                          (src/metropolisfltrenderer.cpp:60-320): passes of
+ *                          two Metropolis films driven by an importance map
+ *                          from the relative variance of the previous pass,
+ *                          each pass cross-filtered by the symmetric filter
+ *                          (src/symmetricfilter.cpp:40-137)
  *
  * Both use the reference estimator (RT_MODE_COMPAT semantics, quirks
  * included).  The scene is the one uploaded with rt_scene_upload.
@@ -51,7 +57,15 @@
 extern "C" {
 #endif
 
-enum rt_mlt_kind { RT_MLT_METROPOLIS = 0, RT_MLT_ERPT = 1 };
+enum rt_mlt_kind { RT_MLT_METROPOLIS = 0, RT_MLT_ERPT = 1, RT_MLT_FILTERED = 2 };
+
+/* RT_MLT_FILTERED options (MetropolisFltRenderer, --metflt-*) */
+enum rt_mlt_filter_flags {
+    RT_FLT_NO_FILTER = 1,              /* --metflt-no-filter */
+    RT_FLT_NO_CROSSFILTERING = 2,      /* --metflt-no-crossfiltering */
+    RT_FLT_NO_VARIANCE_FILTER = 4,     /* --metflt-no-variance-filter */
+    RT_FLT_SIMPLE_VARIANCE_FILTER = 8  /* --metflt-simple-variance-filter */
+};
 
 /* Philox-4x32-10 streams of the chains: key (seed, stream), counter
  * (index, step, group, 0), four uniforms (x >> 8) * 2^-24 per draw - the
@@ -95,6 +109,8 @@ typedef struct rt_mlt_params {
     float erpt_ed_termination; /* ERPT: termination probability of the paths that estimate e_d
                                   (UniDiPathTracingIntegrator(0.1), src/energyredistributionrenderer.cpp:118) */
     uint32_t seed;             /* --seed */
+    int32_t flt_passes;        /* FILTERED: --metflt-num-passes (3) */
+    int32_t flt_flags;         /* FILTERED: rt_mlt_filter_flags */
 } rt_mlt_params;
 
 typedef struct rt_mlt_result {
@@ -122,6 +138,13 @@ int rt_render_mlt(rt_ctx* ctx, const rt_camera* camera, const rt_mlt_params* par
                   rt_mlt_result* result);
 int rt_render_mlt_device(rt_ctx* ctx, const rt_camera* camera, const rt_mlt_params* params, float* film,
                          rt_mlt_result* result);
+
+/* SymmetricFilter::filter (src/symmetricfilter.cpp:72-137), the filter of
+ * RT_MLT_FILTERED: image filtered with weights from patch distances of
+ * guide less the per-pixel variance pixvar, over a (2 radius + 1)^2 window
+ * (patch odd).  All arrays height * width * 3 floats in host memory. */
+int rt_symmetric_filter(rt_ctx* ctx, const float* image, const float* guide, const float* pixvar, int32_t width,
+                        int32_t height, int32_t patch, int32_t radius, float* out);
 
 #ifdef __cplusplus
 }

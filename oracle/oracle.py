@@ -65,6 +65,7 @@ def load():
     lib.oc_integrate_samples.argtypes = [P, ctypes.POINTER(_abi.RtCamera), ctypes.c_int, ctypes.c_int, ctypes.c_float,
                                          P, I64, P, ctypes.c_int]
     lib.oc_mlt_perturb.argtypes = [P, P, P, I64, P]
+    lib.oc_sym_filter.argtypes = [P, P, P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, P]
     lib.oc_bsdf_sample.restype = ctypes.c_int
     lib.oc_bsdf_sample.argtypes = [P, ctypes.c_int, P, P, ctypes.c_float, ctypes.c_float, ctypes.c_float, P]
     _lib = lib
@@ -255,4 +256,14 @@ def mlt_perturb(s, u, v):
     s, u, v = (np.ascontiguousarray(a, np.float32) for a in (s, u, v))
     out = np.zeros_like(s)
     lib.oc_mlt_perturb(s.ctypes.data, u.ctypes.data, v.ctypes.data, len(s), out.ctypes.data)
+    return out
+
+
+def sym_filter(image, guide, pixvar, patch=7, radius=10):
+    """SymmetricFilter(patch, radius).filter(image, guide, pixvar) on (H, W, 3) float32 images."""
+    lib = load()
+    a, g, v = (np.ascontiguousarray(x, np.float32) for x in (image, guide, pixvar))
+    out = np.zeros_like(a)
+    lib.oc_sym_filter(a.ctypes.data, g.ctypes.data, v.ctypes.data, a.shape[1], a.shape[0], int(patch), int(radius),
+                      out.ctypes.data)
     return out

@@ -1716,6 +1716,348 @@ void run_erpt(MltRun& R)
 }
 
 // ---------------------------------------------------------------------------
+// MetropolisFltRenderer (src/metropolisfltrenderer.cpp:60-320) and the
+// symmetric filter (src/symmetricfilter.cpp:40-137), on images of H x W x C
+// floats.  OpenCV's blur (a normalised box filter, BORDER_REFLECT) and
+// element-wise Mat arithmetic are restated as loops; their float summation
+// order is this restatement's own.
+// ---------------------------------------------------------------------------
+struct Img {
+    int W = 0, H = 0, C = 1;
+    std::vector<float> v;
+    Img() = default;
+    Img(int w, int h, int c, float x = 0) : W(w), H(h), C(c), v((size_t)w * h * c, x) {}
+    float& at(int y, int x, int c = 0) { return v[((size_t)y * W + x) * C + c]; }
+    float at(int y, int x, int c = 0) const { return v[((size_t)y * W + x) * C + c]; }
+};
+
+// mathhelper var() of two images: mean = (a + b) / 2, var = max((a^2 + b^2 - sum mean) / 2, 0)
+void var2(const Img& a, const Img& b, Img& mean, Img& var)
+{
+    mean = Img(a.W, a.H, a.C);
+    var = Img(a.W, a.H, a.C);
+    for (size_t i = 0; i < a.v.size(); ++i) {
+        float sum = a.v[i] + b.v[i], sq = a.v[i] * a.v[i] + b.v[i] * b.v[i];
+        float m = sum * 0.5f;
+        mean.v[i] = m;
+        var.v[i] = std::max((sq - sum * m) / 2.f, 0.f);
+    }
+}
+
+inline float chan_mean3(float a, float b, float c) { return (a / 3.f + b / 3.f) + c / 3.f; }
+
+// cv::blur(.., Size(p, p), Point(-1, -1), BORDER_REFLECT) of a one-channel image
+Img box_blur(const Img& in, int p)
+{
+    if (p <= 1) return in;
+    const int r = p / 2, W = in.W, H = in.H;
+    auto refl = [](int i, int n) {
+        while (i < 0 || i >= n) i = i < 0 ? -i - 1 : 2 * n - i - 1;
+        return i;
+    };
+    Img t(W, H, 1), out(W, H, 1);
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            float s = 0;
+            for (int k = -r; k <= r; ++k) s += in.at(y, refl(x + k, W));
+            t.at(y, x) = s;
+        }
+    const float inv = 1.f / (float)(p * p);
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            float s = 0;
+            for (int k = -r; k <= r; ++k) s += t.at(refl(y + k, H), x);
+            out.at(y, x) = s * inv;
+        }
+    return out;
+}
+
+// SymmetricFilter::filter (src/symmetricfilter.cpp:72-137) with computeWeights
+// (:46-70): pairs of offsets +-d (half the window; d = 0 once), weights from
+// the patch distance of the guide less the variance, the symmetric average
+// when it beats both one-sided weights; outside the image everything is 0.
+Img sym_filter(const Img& image, const Img& guide, const Img& pixvar, int patch, int R)
+{
+    const int W = image.W, H = image.H;
+    const float h2 = 0.4f * 0.4f;
+    Img area(W, H, 1), acc(W, H, 3);
+    auto get = [&](const Img& m, int y, int x, int c) { return y < 0 || y >= H || x < 0 || x >= W ? 0.f : m.at(y, x, c); };
+    for (int dx = -R; dx <= 0; ++dx)
+        for (int dy = -R; dy <= R; ++dy) {
+            if (dx == 0 && dy == 0) {
+                for (int y = 0; y < H; ++y)
+                    for (int x = 0; x < W; ++x) {
+                        area.at(y, x) += 1.f;
+                        for (int c = 0; c < 3; ++c) acc.at(y, x, c) += image.at(y, x, c);
+                    }
+                break;
+            }
+            Img d2[3] = {Img(W, H, 1), Img(W, H, 1), Img(W, H, 1)}, den[3] = {Img(W, H, 1), Img(W, H, 1), Img(W, H, 1)};
+            for (int y = 0; y < H; ++y)
+                for (int x = 0; x < W; ++x) {
+                    float dd[3][3], vv[3][3];
+                    for (int c = 0; c < 3; ++c) {
+                        float s1 = get(guide, y - dy, x - dx, c), s2 = get(guide, y + dy, x + dx, c);
+                        float v1 = get(pixvar, y - dy, x - dx, c), v2 = get(pixvar, y + dy, x + dx, c);
+                        float sS = (s1 + s2) / 2.f, vS = (v1 + v2) / 4.f;
+                        float t = guide.at(y, x, c), v = pixvar.at(y, x, c);
+                        const float src[3] = {s1, s2, sS}, sv[3] = {v1, v2, vS};
+                        for (int k = 0; k < 3; ++k) {
+                            float df = src[k] - t;
+                            dd[k][c] = df * df - (v + std::min(sv[k], v));
+                            vv[k][c] = v + sv[k];
+                        }
+                    }
+                    for (int k = 0; k < 3; ++k) {
+                        d2[k].at(y, x) = chan_mean3(dd[k][0], dd[k][1], dd[k][2]);
+                        den[k].at(y, x) = 1e-8f + h2 * chan_mean3(vv[k][0], vv[k][1], vv[k][2]);
+                    }
+                }
+            Img w[3];
+            for (int k = 0; k < 3; ++k) {
+                Img b = box_blur(d2[k], patch), raw(W, H, 1);
+                for (size_t i = 0; i < raw.v.size(); ++i) raw.v[i] = std::exp(-std::max(b.v[i], 0.f) / den[k].v[i]);
+                w[k] = box_blur(raw, patch);
+                for (float& x : w[k].v) {
+                    x = std::max(x, 0.f);
+                    if (x < 0.05f) x = 0.f;
+                }
+            }
+            for (int y = 0; y < H; ++y)
+                for (int x = 0; x < W; ++x) {
+                    float w1 = w[0].at(y, x), w2 = w[1].at(y, x), wS = w[2].at(y, x);
+                    if (wS > std::max(w1, w2)) w1 = w2 = wS;
+                    area.at(y, x) += w1 + w2;
+                    for (int c = 0; c < 3; ++c)
+                        acc.at(y, x, c) += w1 * get(image, y - dy, x - dx, c) + w2 * get(image, y + dy, x + dx, c);
+                }
+        }
+    Img out(W, H, 3);
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x)
+            for (int c = 0; c < 3; ++c) out.at(y, x, c) = acc.at(y, x, c) / area.at(y, x);
+    return out;
+}
+
+// The chains of one pass of MetropolisFltRenderer::renderStep
+// (src/metropolisfltrenderer.cpp:86-205) with the GPU's chain layout: per
+// film, `chains` chains started from bootstrap draws in proportion to
+// norm x importance, norm-based acceptance weighted by the importance map,
+// deposits weighted by 1 / (summed importance) into the film and the
+// weighted running mean / variance of addSample (:54-75).
+struct FltState {
+    Img film[2], mean[2], m2[2], sw[2];
+};
+
+void flt_pass(MltRun& R, int pass, const Img& imp, const Img& newsum, float b, FltState& st, int64_t& rays,
+              int64_t& accepted, int64_t& mutations)
+{
+    const Scene& sc = *R.sc;
+    const int W = R.cam.W, H = R.cam.H, G = R.G, P = R.p.flt_passes;
+    const int64_t area = (int64_t)W * H;
+    const int64_t per_film = std::max<int64_t>(1, (int64_t)R.p.mutations * area / P / 2);
+    const int64_t C = R.p.chains > 0 ? R.p.chains : mlt_default_chains(per_film);
+    const int64_t iters = std::max<int64_t>(1, (per_film + C / 2) / C);
+    const int64_t nb = std::max<int64_t>(R.p.bootstrap, C);
+    const uint32_t seed = R.p.seed;
+    // bootstrap of this pass: full-length large steps, weighted by norm x importance
+    std::vector<Col> bval(nb);
+    std::vector<double> cdf(nb);
+#pragma omp parallel for schedule(dynamic, 64) num_threads(R.threads) reduction(+ : rays)
+    for (int64_t i = 0; i < nb; ++i) {
+        Pss x(G);
+        for (int g = 0; g < G; ++g) x.g[g] = draw(seed, RT_MLT_STREAM_BOOTSTRAP, (uint32_t)i, (uint32_t)pass, (uint32_t)g);
+        bval[i] = integrate_pss(sc, R.cam, x, R.p.max_depth, R.p.rr_start, 0.f, false, rays);
+        const int pos = mlt_pos(x, W, H);
+        cdf[i] = (float)(norm2(bval[i]) * imp.v[pos]);
+    }
+    double acc = 0;
+    for (int64_t i = 0; i < nb; ++i) cdf[i] = acc += cdf[i];
+    R.res.chains += 2 * C;
+    R.res.seed_paths += nb;
+    if (!(acc > 0)) return;
+    const float scale = (float)((double)b * P / R.p.mutations);
+    for (int n = 0; n < 2; ++n) {
+        std::vector<std::vector<std::pair<int, std::pair<float, Col>>>> deps(C);
+#pragma omp parallel for schedule(dynamic, 16) num_threads(R.threads) reduction(+ : rays, accepted)
+        for (int64_t c = 0; c < C; ++c) {
+            const uint32_t id = (uint32_t)(((int64_t)pass * 2 + n) * C + c);
+            double target = (double)draw(seed, RT_MLT_STREAM_START, id, 0, 0).u[0] * acc;
+            int64_t idx = std::min<int64_t>(std::upper_bound(cdf.begin(), cdf.end(), target) - cdf.begin(), nb - 1);
+            Pss cur(G), prop(G);
+            for (int g = 0; g < G; ++g)
+                cur.g[g] = draw(seed, RT_MLT_STREAM_BOOTSTRAP, (uint32_t)idx, (uint32_t)pass, (uint32_t)g);
+            Col cv = bval[idx];
+            float cn = (float)norm2(cv);
+            auto& dep = deps[c];
+            for (int64_t m = 0; m < iters; ++m) {
+                U4 ctl = draw(seed, RT_MLT_STREAM_CONTROL, id, (uint32_t)m, 0);
+                mlt_propose(cur, prop, seed, id, (uint32_t)m, ctl.u[0] < R.p.large_step_prob);
+                prop.g[0].u[2] = ctl.u[1];
+                Col nv = integrate_pss(sc, R.cam, prop, R.p.max_depth, R.p.rr_start, R.p.termination_prob, false, rays);
+                double zn = norm2(nv);
+                const int cp = mlt_pos(cur, W, H), np = mlt_pos(prop, W, H);
+                float a = (float)std::min(1., zn * imp.v[np] / ((double)cn * imp.v[cp]));
+                if (cn > 0 && a < 1) {
+                    const double k = 1 / (double)cn * scale;
+                    dep.push_back({cp, {(1 - a) / newsum.v[cp], Col((float)(cv.r * k), (float)(cv.g * k), (float)(cv.b * k))}});
+                }
+                if (zn > 0 && a > 0) {
+                    const double k = 1 / zn * scale;
+                    dep.push_back({np, {a / newsum.v[np], Col((float)(nv.r * k), (float)(nv.g * k), (float)(nv.b * k))}});
+                }
+                if (ctl.u[2] < a) {
+                    std::swap(cur, prop);
+                    cv = nv;
+                    cn = (float)zn;
+                    ++accepted;
+                }
+            }
+        }
+        // addSample, in chain order
+        for (auto& dep : deps)
+            for (auto& d : dep) {
+                const int pos = d.first;
+                const float w = d.second.first;
+                const Col v = d.second.second;
+                const float ow = st.sw[n].v[pos], nw = w + ow;
+                const float vc[3] = {v.r, v.g, v.b};
+                for (int c = 0; c < 3; ++c) {
+                    float delta = vc[c] - st.mean[n].at(pos / W, pos % W, c);
+                    float Rr = delta * (w / nw);
+                    st.mean[n].at(pos / W, pos % W, c) += Rr;
+                    st.m2[n].at(pos / W, pos % W, c) += ow * (delta * Rr);
+                    st.film[n].at(pos / W, pos % W, c) += w * vc[c];
+                }
+                st.sw[n].v[pos] = nw;
+            }
+        mutations += C * iters;
+    }
+}
+
+// MetropolisFltRenderer::render (src/metropolisfltrenderer.cpp:207-320)
+void run_filtered(MltRun& R)
+{
+    const int W = R.cam.W, H = R.cam.H, P = R.p.flt_passes, F = R.p.flt_flags;
+    const int64_t area = (int64_t)W * H;
+    FltState st;
+    for (int n = 0; n < 2; ++n) {
+        st.film[n] = Img(W, H, 3);
+        st.mean[n] = Img(W, H, 3);
+        st.m2[n] = Img(W, H, 3);
+        st.sw[n] = Img(W, H, 1);
+    }
+    Img sumimp(W, H, 1), varOfFiltered, filteredMean;
+    int64_t rays = 0, accepted = 0, mutations = 0;
+    for (int i = 0; i < P; ++i) {
+        Img imp(W, H, 1, 1.f);
+        if (i > 0) {
+            // relative variance -> importance (:231-239), normalised to mean 1
+            double sum = 0;
+            for (int y = 0; y < H; ++y)
+                for (int x = 0; x < W; ++x) {
+                    float vm = chan_mean3(varOfFiltered.at(y, x, 0), varOfFiltered.at(y, x, 1), varOfFiltered.at(y, x, 2));
+                    float mm = chan_mean3(filteredMean.at(y, x, 0), filteredMean.at(y, x, 1), filteredMean.at(y, x, 2));
+                    float rel = vm / (mm * mm + 1e-2f) + 1e-8f;
+                    float e = std::exp(-rel);
+                    imp.at(y, x) = -20 * (e - 1) + 1;
+                    sum += imp.at(y, x);
+                }
+            const float k = (float)(area / sum);
+            for (float& x : imp.v) x *= k;
+        }
+        // b: the bootstrap's mean norm (first pass) or the first film's (:118-124)
+        float b;
+        if (i == 0) {
+            const int64_t per_film = std::max<int64_t>(1, (int64_t)R.p.mutations * area / P / 2);
+            const int64_t C = R.p.chains > 0 ? R.p.chains : mlt_default_chains(per_film);
+            const int64_t nb = std::max<int64_t>(R.p.bootstrap, C);
+            // the same paths as the pass's bootstrap, whose rays flt_pass counts
+            double acc = 0;
+            int64_t unused_rays = 0;
+#pragma omp parallel for schedule(dynamic, 64) num_threads(R.threads) reduction(+ : unused_rays, acc)
+            for (int64_t j = 0; j < nb; ++j) {
+                Pss x(R.G);
+                for (int g = 0; g < R.G; ++g) x.g[g] = draw(R.p.seed, RT_MLT_STREAM_BOOTSTRAP, (uint32_t)j, 0, (uint32_t)g);
+                acc += norm2(integrate_pss(*R.sc, R.cam, x, R.p.max_depth, R.p.rr_start, 0.f, false, unused_rays));
+            }
+            b = (float)(acc / (double)nb);
+        } else {
+            double acc = 0;
+            for (int y = 0; y < H; ++y)
+                for (int x = 0; x < W; ++x) {
+                    float r = st.film[0].at(y, x, 0), g = st.film[0].at(y, x, 1), bb = st.film[0].at(y, x, 2);
+                    acc += (double)(std::sqrt(r * r + g * g + bb * bb) * imp.at(y, x));
+                }
+            b = (float)(acc / (double)area);
+        }
+        if (i == 0) R.res.b = b;
+        Img newsum(W, H, 1);
+        for (size_t k = 0; k < newsum.v.size(); ++k) newsum.v[k] = sumimp.v[k] + imp.v[k];
+        for (int n = 0; n < 2; ++n)
+            for (int64_t k = 0; k < area; ++k) {
+                const float r = sumimp.v[k] / newsum.v[k];
+                for (int c = 0; c < 3; ++c) {
+                    st.film[n].v[3 * k + c] = r * st.film[n].v[3 * k + c];
+                    st.m2[n].v[3 * k + c] = r * (r * st.m2[n].v[3 * k + c]);
+                }
+            }
+        flt_pass(R, i, imp, newsum, b, st, rays, accepted, mutations);
+        // filtering (:252-300)
+        Img bv[2];
+        for (int n = 0; n < 2; ++n) {
+            bv[n] = Img(W, H, 3);
+            for (int64_t k = 0; k < area; ++k)
+                for (int c = 0; c < 3; ++c) bv[n].v[3 * k + c] = st.m2[n].v[3 * k + c] / (st.sw[n].v[k] + 1e-8f);
+        }
+        Img newOut, noisy, meanvar, varvar, filteredVar;
+        var2(st.film[0], st.film[1], newOut, noisy);
+        var2(bv[0], bv[1], meanvar, varvar);
+        if ((F & RT_FLT_NO_FILTER) || (F & RT_FLT_NO_VARIANCE_FILTER)) {
+            filteredVar = noisy;
+        } else if (F & RT_FLT_SIMPLE_VARIANCE_FILTER) {
+            // sepFilter2D with [.1 .8 .1] both ways (BORDER_REFLECT_101)
+            auto r101 = [](int i, int n) { return i < 0 ? -i : i >= n ? 2 * n - i - 2 : i; };
+            Img t(W, H, 3);
+            filteredVar = Img(W, H, 3);
+            const float k3[3] = {.1f, .8f, .1f};
+            for (int y = 0; y < H; ++y)
+                for (int x = 0; x < W; ++x)
+                    for (int c = 0; c < 3; ++c) {
+                        float s = 0;
+                        for (int j = -1; j <= 1; ++j) s += k3[j + 1] * noisy.at(y, std::min(std::max(r101(x + j, W), 0), W - 1), c);
+                        t.at(y, x, c) = s;
+                    }
+            for (int y = 0; y < H; ++y)
+                for (int x = 0; x < W; ++x)
+                    for (int c = 0; c < 3; ++c) {
+                        float s = 0;
+                        for (int j = -1; j <= 1; ++j) s += k3[j + 1] * t.at(std::min(std::max(r101(y + j, H), 0), H - 1), x, c);
+                        filteredVar.at(y, x, c) = s;
+                    }
+        } else {
+            filteredVar = sym_filter(noisy, meanvar, varvar, 1, 2);
+        }
+        if (F & RT_FLT_NO_FILTER) {
+            varOfFiltered = filteredVar;
+            filteredMean = newOut;
+        } else if (F & RT_FLT_NO_CROSSFILTERING) {
+            varOfFiltered = filteredVar;
+            filteredMean = sym_filter(newOut, newOut, filteredVar, 7, 10);
+        } else {
+            Img f0 = sym_filter(st.film[0], st.film[1], filteredVar, 7, 10);
+            Img f1 = sym_filter(st.film[1], st.film[0], filteredVar, 7, 10);
+            var2(f0, f1, filteredMean, varOfFiltered);
+        }
+        sumimp = newsum;
+    }
+    for (size_t k = 0; k < R.film.size(); ++k) R.film[k] = filteredMean.v[k];
+    R.res.mutations = mutations;
+    R.res.accepted = accepted;
+    R.res.rays = rays;
+}
+
+// ---------------------------------------------------------------------------
 // C API
 // ---------------------------------------------------------------------------
 extern "C" {
@@ -2097,6 +2439,17 @@ void oc_tonemap(const float* film, int64_t n, float gamma, uint8_t* out)
     for (int64_t i = 0; i < n; ++i) out[i] = (uint8_t)(tonemap_clamp(255 * std::pow(film[i], 1. / gamma)) & 0xff);
 }
 
+void oc_sym_filter(const float* image, const float* guide, const float* pixvar, int w, int h, int patch, int radius,
+                   float* out)
+{
+    Img a(w, h, 3), g(w, h, 3), v(w, h, 3);
+    std::memcpy(a.v.data(), image, a.v.size() * 4);
+    std::memcpy(g.v.data(), guide, g.v.size() * 4);
+    std::memcpy(v.v.data(), pixvar, v.v.size() * 4);
+    Img o = sym_filter(a, g, v, patch, radius);
+    std::memcpy(out, o.v.data(), o.v.size() * 4);
+}
+
 void oc_mlt_perturb(const float* s, const float* u, const float* v, int64_t n, float* out)
 {
     for (int64_t i = 0; i < n; ++i) out[i] = mlt_perturb(s[i], u[i], v[i]);
@@ -2128,6 +2481,8 @@ int oc_render_mlt(void* s, const rt_camera* camera, const rt_mlt_params* p, floa
         run_metropolis(R);
     else if (p->kind == RT_MLT_ERPT)
         run_erpt(R);
+    else if (p->kind == RT_MLT_FILTERED && p->flt_passes >= 1)
+        run_filtered(R);
     else
         return -1;
     for (size_t i = 0; i < R.film.size(); ++i) film[i] += (float)R.film[i];

@@ -111,6 +111,10 @@ int oc_render_mlt(void* scene, const rt_camera* camera, const rt_mlt_params* par
  * the path of src/renderer.cpp:68-106): samples [n][2*max_depth+1][4]. */
 void oc_integrate_samples(void* scene, const rt_camera* camera, int max_depth, int rr_start, float termination_prob,
                           const float* samples, int64_t n, float* rgb, int threads);
+/* SymmetricFilter(patch, radius).filter(image, guide, pixvar)
+ * (src/symmetricfilter.cpp:72-137) on h * w * 3 float images. */
+void oc_sym_filter(const float* image, const float* guide, const float* pixvar, int w, int h, int patch, int radius,
+                   float* out);
 /* The primary-sample perturbation (src/metropolissample.cpp:62-78) of n coordinates. */
 void oc_mlt_perturb(const float* s, const float* u, const float* v, int64_t n, float* out);
 

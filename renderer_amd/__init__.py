@@ -3,10 +3,11 @@
 integrator interface named after the reference renderer's classes."""
 from . import _abi
 from ._abi import (LIGHT_AREA, LIGHT_CONE, LIGHT_ENV, LIGHT_POINT, MAT_BLINN, MAT_DARK, MAT_DIFFUSE, MAT_MIRROR,
-                   MAT_PHONG, MAT_REFRACTIVE, MLT_ERPT, MLT_METROPOLIS,
+                   MAT_PHONG, MAT_REFRACTIVE, MLT_ERPT, MLT_FILTERED, MLT_METROPOLIS,
+                   FLT_NO_CROSSFILTERING, FLT_NO_FILTER, FLT_NO_VARIANCE_FILTER, FLT_SIMPLE_VARIANCE_FILTER,
                    MODE_COMPAT, MODE_GUIDED, MODE_UNBIASED, RtError)
-from .integrator import (EnergyRedistributionRenderer, GpuContext, GpuUniDiPathTracingIntegrator, MetropolisRenderer,
-                         PerPixelRenderer)
+from .integrator import (EnergyRedistributionRenderer, GpuContext, GpuUniDiPathTracingIntegrator, MetropolisFltRenderer,
+                         MetropolisRenderer, PerPixelRenderer)
 from .scene import (AreaLight, BlinnMaterial, Camera, ConeLight, DarkMatter, DiffuseMaterial, EnvironmentMap,
                     IntersectableList, Mesh, MirrorMaterial, PhongMaterial, PointLight, Quad, Scene, Sphere,
                     TextureMaterial, TransparentMaterial, Triangle)
@@ -17,5 +18,7 @@ __all__ = [
     "AreaLight", "Camera", "DarkMatter", "DiffuseMaterial", "EnvironmentMap", "IntersectableList", "Mesh",
     "MirrorMaterial", "PhongMaterial", "PointLight", "Quad", "Scene", "Sphere", "Triangle", "SceneFile",
     "BlinnMaterial", "TransparentMaterial", "TextureMaterial", "ConeLight", "MetropolisRenderer", "EnergyRedistributionRenderer",
+    "MetropolisFltRenderer", "MLT_FILTERED", "FLT_NO_FILTER", "FLT_NO_CROSSFILTERING", "FLT_NO_VARIANCE_FILTER",
+    "FLT_SIMPLE_VARIANCE_FILTER",
     "MODE_COMPAT", "MODE_UNBIASED", "MODE_GUIDED", "MLT_METROPOLIS", "MLT_ERPT",
 ]

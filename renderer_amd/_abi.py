@@ -114,7 +114,8 @@ class RtMltParams(ctypes.Structure):
                 ("termination_prob", ctypes.c_float), ("large_step_prob", ctypes.c_float),
                 ("bootstrap", ctypes.c_int32), ("mutations", ctypes.c_int32), ("chains", ctypes.c_int32),
                 ("erpt_spp_x", ctypes.c_int32), ("erpt_spp_y", ctypes.c_int32),
-                ("erpt_ed_termination", ctypes.c_float), ("seed", ctypes.c_uint32)]
+                ("erpt_ed_termination", ctypes.c_float), ("seed", ctypes.c_uint32),
+                ("flt_passes", ctypes.c_int32), ("flt_flags", ctypes.c_int32)]
 
 
 class RtMltResult(ctypes.Structure):
@@ -124,7 +125,8 @@ class RtMltResult(ctypes.Structure):
                 ("chain_seeds", ctypes.c_int64)]
 
 
-MLT_METROPOLIS, MLT_ERPT = 0, 1
+MLT_METROPOLIS, MLT_ERPT, MLT_FILTERED = 0, 1, 2
+FLT_NO_FILTER, FLT_NO_CROSSFILTERING, FLT_NO_VARIANCE_FILTER, FLT_SIMPLE_VARIANCE_FILTER = 1, 2, 4, 8
 MAT_DIFFUSE, MAT_MIRROR, MAT_PHONG, MAT_DARK, MAT_BLINN, MAT_REFRACTIVE, MAT_TEXTURE = 0, 1, 2, 3, 4, 5, 6
 LIGHT_AREA, LIGHT_POINT, LIGHT_ENV, LIGHT_CONE = 0, 1, 2, 3
 MODE_COMPAT, MODE_UNBIASED, MODE_GUIDED = 0, 1, 2
@@ -191,6 +193,7 @@ PROTOTYPES = {
                                      ctypes.POINTER(RtMltResult)]),
     "rt_render_mlt_device": (ctypes.c_int, [_P, ctypes.POINTER(RtCamera), ctypes.POINTER(RtMltParams), _P,
                                             ctypes.POINTER(RtMltResult)]),
+    "rt_symmetric_filter": (ctypes.c_int, [_P, _P, _P, _P, _I32, _I32, _I32, _I32, _P]),
 }
 
 _lib = None

@@ -11,7 +11,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "librt_hip.so")
 
-SOURCES = ["rt_kernels.hip", "rt_guide.hip", "rt_film.hip", "rt_mlt.hip", "rt_abi.cpp", "bvh_build.cpp", "scene_file.cpp",
+SOURCES = ["rt_kernels.hip", "rt_guide.hip", "rt_film.hip", "rt_mlt.hip", "rt_filter.hip", "rt_abi.cpp", "bvh_build.cpp", "scene_file.cpp",
            "film_io.cpp"]
 
 # -packed-fp32-ops: keep the compiler from forming packed FP32 instructions.

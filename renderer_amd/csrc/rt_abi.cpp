@@ -59,6 +59,7 @@ struct rt_ctx {
     DevBuf counters, views, tiles, passes, films;
     // Markov-chain renderers: chain states, proposals, seed-path values, deposits
     DevBuf mlt_cur, mlt_prop, mlt_val, mlt_energy, mlt_acc, mlt_cnt, mlt_seed, mlt_cdf, mlt_nch, mlt_src, mlt_view;
+    DevBuf flt_img, flt_acc;  /* filtered Metropolis: per-pixel images, fixed-point pass sums */
     /* defaults from a sweep on MI355X (1M-triangle mesh, 1024^2 x 64 spp):
      * 32M path slots per pass (see render_impl), 8 persistent blocks per CU
      * (all resident: the traversal kernels claim work dynamically, so extra
@@ -225,7 +226,7 @@ int rt_destroy(rt_ctx* c)
                      &c->g_open, &c->g_out, &c->g_tmp_rec, &c->g_tmp_y, &c->obj_nodes, &c->obj_first,
                      &c->tex_rgb, &c->tex_info, &c->tri_tc,
                      &c->mlt_cur, &c->mlt_prop, &c->mlt_val, &c->mlt_energy, &c->mlt_acc, &c->mlt_cnt,
-                     &c->mlt_seed, &c->mlt_cdf, &c->mlt_nch, &c->mlt_src, &c->mlt_view};
+                     &c->mlt_seed, &c->mlt_cdf, &c->mlt_nch, &c->mlt_src, &c->mlt_view, &c->flt_img, &c->flt_acc};
     for (DevBuf* b : all) release(*b);
     for (auto& b : c->ps_bufs) release(b);
     for (auto e : c->ev_pool) hipEventDestroy(e);
@@ -1560,6 +1561,166 @@ int run_erpt(rt_ctx* c, const rt_mlt_params& p, int W, int H, int64_t cap_slots,
     return RT_OK;
 }
 
+/* MetropolisFltRenderer::render, src/metropolisfltrenderer.cpp:60-320:
+ * flt_passes passes; each rescales the two films by the summed importance,
+ * runs `chains` chains per film with norm x importance acceptance, merges the
+ * deposits into the films and the running variance, then filters.  The
+ * result (the filtered mean of the films) is added to film_out (W * H * 3,
+ * device memory). */
+int run_filtered(rt_ctx* c, const rt_mlt_params& p, int W, int H, int64_t cap_slots, rt_mlt_result& res, float* film_out)
+{
+    const int G = 2 * p.max_depth + 1, P = p.flt_passes, F = p.flt_flags;
+    const int64_t area = (int64_t)W * H;
+    const int64_t per_film = std::max<int64_t>(1, (int64_t)p.mutations * area / P / 2);
+    const int64_t C = p.chains > 0 ? p.chains : std::max<int64_t>(1, std::min<int64_t>(per_film / 16, (int64_t)1 << 22));
+    const int64_t iters = std::max<int64_t>(1, (per_film + C / 2) / C);
+    const int64_t nb = std::max<int64_t>(p.bootstrap, C);
+    if (nb > INT32_MAX || (int64_t)P * 2 * C > INT32_MAX || iters > INT32_MAX)
+        return set_err(c, RT_ERR_INVALID, "too many chains or mutations");
+    const int cap = (int)std::min<int64_t>(cap_slots, std::max(C, nb));
+    if (int r = mlt_alloc(c, cap, G, area)) return r;
+    if (int r = ensure(c, c->mlt_seed, (size_t)nb * sizeof(float4) * 2)) return r;
+    /* images: films, means, m2 (3 each per film), sw (1 per film), sumimp,
+     * imp, newsum, per-pixel scratch (1), newOut, noisy, bvar x2, meanvar,
+     * varvar, filteredVar, filteredMean, varOfFiltered, filtered films x2,
+     * filter scratch (15) */
+    const size_t n3 = (size_t)area * 3;
+    float* im = nullptr;
+    const size_t total = n3 * 6 + (size_t)area * 2 + (size_t)area * 4 + n3 * 11 + (size_t)area * 15;
+    if (int r = ensure(c, c->flt_img, total * 4)) return r;
+    if (int r = ensure(c, c->flt_acc, (size_t)area * 4 * sizeof(unsigned long long))) return r;
+    im = (float*)c->flt_img.p;
+    float *film[2] = {im, im + n3}, *mean[2] = {im + 2 * n3, im + 3 * n3}, *m2[2] = {im + 4 * n3, im + 5 * n3};
+    float* q = im + 6 * n3;
+    float *sw[2] = {q, q + area}, *sumimp = q + 2 * area, *imp = q + 3 * area, *newsum = q + 4 * area,
+          *pix = q + 5 * area;
+    q += 6 * area;
+    float *newOut = q, *noisy = q + n3, *bv[2] = {q + 2 * n3, q + 3 * n3}, *meanvar = q + 4 * n3, *varvar = q + 5 * n3,
+          *fvar = q + 6 * n3, *fmean = q + 7 * n3, *vof = q + 8 * n3, *ff[2] = {q + 9 * n3, q + 10 * n3};
+    float* scratch = q + 11 * n3;
+    unsigned long long* aw = (unsigned long long*)c->flt_acc.p;
+    unsigned long long* aq = aw + area;
+    unsigned long long* ax = (unsigned long long*)c->mlt_acc.p;
+    const int grid = grid_for(c);
+    MLT_TRY(hipMemsetAsync(im, 0, (size_t)(n3 * 6 + area * 3) * 4, c->stream));  /* films, stats, sumimp */
+    float4* bval = (float4*)c->mlt_seed.p;
+    float4* bw = bval + nb;
+    MltDev m = mlt_dev(c, cap, G, W, H, p);
+    m.imp = imp;
+    m.newsum = newsum;
+    m.acc_w = aw;
+    m.acc_q = aq;
+    std::vector<float> hbuf((size_t)area);
+    std::vector<float4> hv((size_t)nb);
+    int64_t mutations = 0;
+    for (int pass = 0; pass < P; ++pass) {
+        if (c->stop) return set_err(c, RT_ERR_CANCELLED, "render cancelled");
+        /* importance map */
+        if (pass == 0) {
+            std::fill(hbuf.begin(), hbuf.end(), 1.f);
+            MLT_TRY(hipMemcpyAsync(imp, hbuf.data(), area * 4, hipMemcpyHostToDevice, c->stream));
+        } else {
+            MLT_TRY(launch_flt_importance(vof, fmean, imp, area, grid, c->stream));
+            MLT_TRY(hipMemcpyAsync(hbuf.data(), imp, area * 4, hipMemcpyDeviceToHost, c->stream));
+            MLT_TRY(hipStreamSynchronize(c->stream));
+            double sum = 0;
+            for (float v : hbuf) sum += v;
+            MLT_TRY(launch_scale(imp, (float)(area / sum), area, grid, c->stream));
+        }
+        /* bootstrap of this pass (full-length large steps), weights norm x importance */
+        m.step = (uint32_t)pass;
+        for (int64_t base = 0; base < nb; base += cap) {
+            const int n = (int)std::min<int64_t>(cap, nb - base);
+            MLT_TRY(launch_mlt_fill(m, RT_MLT_STREAM_BOOTSTRAP, (uint32_t)base, n, 0, grid, c->stream));
+            MLT_TRY(mlt_trace(c, n, cap, p.max_depth, p.rr_start, 0.f, p.seed, 0));
+            MLT_TRY(launch_mlt_values(c->ps, n, bval + base, 1, grid, c->stream));
+            MLT_TRY(launch_flt_boot_weight(m, bval + base, bw + base, n, grid, c->stream));
+        }
+        MLT_TRY(hipMemcpyAsync(hv.data(), bw, (size_t)nb * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
+        /* b: the bootstrap's mean norm, or the first film's norm x importance (:109-124) */
+        float b;
+        if (pass == 0) {
+            std::vector<float4> hb((size_t)nb);
+            MLT_TRY(hipMemcpyAsync(hb.data(), bval, (size_t)nb * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
+            MLT_TRY(hipStreamSynchronize(c->stream));
+            double acc = 0;
+            for (const float4& v : hb) acc += std::sqrt((double)v.x * v.x + (double)v.y * v.y + (double)v.z * v.z);
+            b = (float)(acc / (double)nb);
+            res.b = b;
+        } else {
+            MLT_TRY(launch_flt_bnorm(film[0], imp, pix, area, grid, c->stream));
+            MLT_TRY(hipMemcpyAsync(hbuf.data(), pix, area * 4, hipMemcpyDeviceToHost, c->stream));
+            MLT_TRY(hipStreamSynchronize(c->stream));
+            double acc = 0;
+            for (float v : hbuf) acc += (double)v;
+            b = (float)(acc / (double)area);
+        }
+        std::vector<double> cdf((size_t)nb);
+        double tot = 0;
+        for (int64_t i = 0; i < nb; ++i) cdf[i] = tot += (double)hv[i].w;
+        MLT_TRY(launch_flt_rescale(sumimp, imp, newsum, film[0], film[1], m2[0], m2[1], area, grid, c->stream));
+        if (tot > 0) {
+            if (int r = upload(c, c->mlt_cdf, cdf.data(), cdf.size() * sizeof(double))) return r;
+            /* fixed point: a deposit is at most (1 / min newsum) * scale per channel */
+            MLT_TRY(hipMemcpyAsync(hbuf.data(), newsum, area * 4, hipMemcpyDeviceToHost, c->stream));
+            MLT_TRY(hipStreamSynchronize(c->stream));
+            float mn = INFINITY;
+            for (float v : hbuf) mn = std::min(mn, v);
+            m.scale = (float)((double)b * P / p.mutations);
+            const double wmax = 1.0 / mn, deps = 2.0 * (double)C * (double)iters;
+            const double ux = mlt_unit(wmax * m.scale * 1.01, deps), uw = mlt_unit(wmax * 1.01, deps),
+                         uq = mlt_unit(wmax * (double)m.scale * m.scale * 1.01, deps);
+            m.inv_unit = 1.0 / ux;
+            m.inv_unit_w = 1.0 / uw;
+            m.inv_unit_q = 1.0 / uq;
+            for (int n = 0; n < 2; ++n) {
+                MLT_TRY(hipMemsetAsync(ax, 0, n3 * 8, c->stream));
+                MLT_TRY(hipMemsetAsync(aw, 0, (size_t)area * 4 * 8, c->stream));
+                for (int64_t c0 = 0; c0 < C; c0 += cap) {
+                    const int nn = (int)std::min<int64_t>(cap, C - c0);
+                    m.chain0 = (uint32_t)(((int64_t)pass * 2 + n) * C + c0);
+                    MLT_TRY(launch_mlt_start(m, (const double*)c->mlt_cdf.p, nb, tot, bval, nn, grid, c->stream));
+                    for (int64_t it = 0; it < iters; ++it) {
+                        MLT_TRY(launch_mlt_propose(m, (uint32_t)it, nn, grid, c->stream));
+                        MLT_TRY(mlt_trace(c, nn, cap, p.max_depth, p.rr_start, p.termination_prob, p.seed, 0));
+                        MLT_TRY(launch_mlt_accept(m, c->ps, (uint32_t)it, nn, grid, c->stream));
+                    }
+                }
+                MLT_TRY(launch_flt_merge(aw, ax, aq, uw, ux, uq, film[n], mean[n], m2[n], sw[n], area, grid,
+                                         c->stream));
+                mutations += C * iters;
+            }
+        }
+        /* filtering (:252-300) */
+        for (int n = 0; n < 2; ++n) MLT_TRY(launch_flt_bvar(m2[n], sw[n], bv[n], area, grid, c->stream));
+        MLT_TRY(launch_var2(film[0], film[1], newOut, noisy, (int64_t)n3, grid, c->stream));
+        MLT_TRY(launch_var2(bv[0], bv[1], meanvar, varvar, (int64_t)n3, grid, c->stream));
+        if ((F & RT_FLT_NO_FILTER) || (F & RT_FLT_NO_VARIANCE_FILTER))
+            MLT_TRY(hipMemcpyAsync(fvar, noisy, n3 * 4, hipMemcpyDeviceToDevice, c->stream));
+        else if (F & RT_FLT_SIMPLE_VARIANCE_FILTER)
+            MLT_TRY(launch_sep3(noisy, scratch, fvar, W, H, grid, c->stream));
+        else
+            MLT_TRY(launch_sym_filter(noisy, meanvar, varvar, W, H, 1, 2, scratch, fvar, grid, c->stream));
+        if (F & RT_FLT_NO_FILTER) {
+            MLT_TRY(hipMemcpyAsync(vof, fvar, n3 * 4, hipMemcpyDeviceToDevice, c->stream));
+            MLT_TRY(hipMemcpyAsync(fmean, newOut, n3 * 4, hipMemcpyDeviceToDevice, c->stream));
+        } else if (F & RT_FLT_NO_CROSSFILTERING) {
+            MLT_TRY(hipMemcpyAsync(vof, fvar, n3 * 4, hipMemcpyDeviceToDevice, c->stream));
+            MLT_TRY(launch_sym_filter(newOut, newOut, fvar, W, H, 7, 10, scratch, fmean, grid, c->stream));
+        } else {
+            MLT_TRY(launch_sym_filter(film[0], film[1], fvar, W, H, 7, 10, scratch, ff[0], grid, c->stream));
+            MLT_TRY(launch_sym_filter(film[1], film[0], fvar, W, H, 7, 10, scratch, ff[1], grid, c->stream));
+            MLT_TRY(launch_var2(ff[0], ff[1], fmean, vof, (int64_t)n3, grid, c->stream));
+        }
+        MLT_TRY(hipMemcpyAsync(sumimp, newsum, area * 4, hipMemcpyDeviceToDevice, c->stream));
+    }
+    MLT_TRY(launch_add(film_out, fmean, (int64_t)n3, grid, c->stream));
+    res.chains = 2 * C * P;
+    res.seed_paths = nb * P;
+    res.mutations = mutations;
+    return RT_OK;
+}
+
 int render_mlt_impl(rt_ctx* c, const rt_camera* cam, const rt_mlt_params* p, float* film, bool device_film,
                     rt_mlt_result* out)
 {
@@ -1568,7 +1729,10 @@ int render_mlt_impl(rt_ctx* c, const rt_camera* cam, const rt_mlt_params* p, flo
     if (!c->has_scene) return set_err(c, RT_ERR_NO_SCENE, "no scene uploaded");
     const int W = cam->width, H = cam->height;
     if (W <= 0 || H <= 0 || (int64_t)W * H > (1 << 26)) return set_err(c, RT_ERR_INVALID, "bad camera");
-    if (p->kind != RT_MLT_METROPOLIS && p->kind != RT_MLT_ERPT) return set_err(c, RT_ERR_INVALID, "unknown kind");
+    if (p->kind != RT_MLT_METROPOLIS && p->kind != RT_MLT_ERPT && p->kind != RT_MLT_FILTERED)
+        return set_err(c, RT_ERR_INVALID, "unknown kind");
+    if (p->kind == RT_MLT_FILTERED && (p->flt_passes < 1 || p->flt_passes > 64 || p->flt_flags < 0 || p->flt_flags > 15))
+        return set_err(c, RT_ERR_INVALID, "bad filtered Metropolis parameters");
     if (p->max_depth < 1 || p->max_depth > 64 || p->rr_start < 0 || p->mutations < 1 ||
         !(p->termination_prob >= 0.f && p->termination_prob < 1.f) ||
         !(p->large_step_prob >= 0.f && p->large_step_prob <= 1.f) || p->bootstrap < 0 || p->chains < 0 ||
@@ -1593,14 +1757,18 @@ int render_mlt_impl(rt_ctx* c, const rt_camera* cam, const rt_mlt_params* p, flo
     MLT_TRY(hipMemsetAsync(c->mlt_acc.p, 0, (size_t)area * 3 * sizeof(unsigned long long), c->stream));
     MLT_TRY(hipMemsetAsync(c->mlt_cnt.p, 0, sizeof(unsigned long long), c->stream));
     double unit = 0;
-    r = p->kind == RT_MLT_METROPOLIS ? run_metropolis(c, *p, W, H, cap, res, unit)
-                                     : run_erpt(c, *p, W, H, cap, res, unit);
-    if (r) return r;
     float* dfilm = film;
     if (!device_film) {
         if ((r = ensure(c, c->films, (size_t)area * 3 * 4))) return r;
         dfilm = (float*)c->films.p;
         HIP_TRY(c, hipMemcpyAsync(dfilm, film, (size_t)area * 3 * 4, hipMemcpyHostToDevice, c->stream));
+    }
+    if (p->kind == RT_MLT_FILTERED) {
+        if ((r = run_filtered(c, *p, W, H, cap, res, dfilm))) return r;
+    } else {
+        r = p->kind == RT_MLT_METROPOLIS ? run_metropolis(c, *p, W, H, cap, res, unit)
+                                         : run_erpt(c, *p, W, H, cap, res, unit);
+        if (r) return r;
     }
     if (unit > 0)
         MLT_TRY(launch_mlt_resolve((const unsigned long long*)c->mlt_acc.p, dfilm, area * 3, unit, grid_for(c),
@@ -1692,6 +1860,28 @@ int rt_render_mlt_device(rt_ctx* c, const rt_camera* camera, const rt_mlt_params
                          rt_mlt_result* res)
 {
     return render_mlt_impl(c, camera, p, film, true, res);
+}
+
+int rt_symmetric_filter(rt_ctx* c, const float* image, const float* guide, const float* pixvar, int32_t width,
+                        int32_t height, int32_t patch, int32_t radius, float* out)
+{
+    if (!c || !image || !guide || !pixvar || !out) return RT_ERR_INVALID;
+    if (int r = check_fault(c)) return r;
+    if (width <= 0 || height <= 0 || (int64_t)width * height > (1 << 26) || patch < 1 || patch % 2 == 0 ||
+        patch > 63 || radius < 0 || radius > 63)
+        return set_err(c, RT_ERR_INVALID, "bad filter size");
+    hipSetDevice(c->device);
+    const size_t n3 = (size_t)width * height * 3, n = (size_t)width * height;
+    if (int r = ensure(c, c->flt_img, (n3 * 4 + n * 15) * 4)) return r;
+    float* d = (float*)c->flt_img.p;
+    HIP_TRY(c, hipMemcpyAsync(d, image, n3 * 4, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(d + n3, guide, n3 * 4, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(d + 2 * n3, pixvar, n3 * 4, hipMemcpyHostToDevice, c->stream));
+    MLT_TRY(launch_sym_filter(d, d + n3, d + 2 * n3, width, height, patch, radius, d + 4 * n3, d + 3 * n3,
+                              grid_for(c), c->stream));
+    HIP_TRY(c, hipMemcpyAsync(out, d + 3 * n3, n3 * 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return RT_OK;
 }
 
 }  // extern "C"

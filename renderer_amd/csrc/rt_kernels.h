@@ -126,6 +126,14 @@ struct MltDev {
     double inv_unit;    /* fixed-point steps per unit of radiance */
     uint32_t chain0;    /* chain number of slot 0 */
     int32_t spp_x, spp_y;  /* ERPT initial samples per pixel */
+    /* filtered Metropolis: importance map and summed importance per pixel;
+     * sums of deposit weights and of weight * value^2 in fixed point */
+    const float* imp;
+    const float* newsum;
+    unsigned long long* acc_w;
+    unsigned long long* acc_q;
+    double inv_unit_w, inv_unit_q;
+    uint32_t step;      /* bootstrap draws: the pass number (0 outside the filtered renderer) */
 };
 
 /* Fill the proposal vectors of n paths with fresh draws of `stream` for
@@ -146,6 +154,26 @@ hipError_t launch_erpt_start(const MltDev& m, const int32_t* src, const int32_t*
                              int n, int grid, hipStream_t st);
 hipError_t launch_mlt_propose(const MltDev& m, uint32_t step, int n, int grid, hipStream_t st);
 hipError_t launch_mlt_accept(const MltDev& m, const PathState& ps, uint32_t step, int n, int grid, hipStream_t st);
+/* filtered Metropolis: bootstrap weights norm x importance at the path's
+ * pixel into out[j].w (groups 0 of the proposals give the pixel) */
+hipError_t launch_flt_boot_weight(const MltDev& m, const float4* val, float4* out, int n, int grid, hipStream_t st);
+/* image passes (rt_filter.hip) */
+hipError_t launch_var2(const float* a, const float* b, float* mean, float* var, int64_t n, int grid, hipStream_t st);
+hipError_t launch_sym_filter(const float* image, const float* guide, const float* pixvar, int W, int H, int patch,
+                             int radius, float* tmp, float* out, int grid, hipStream_t st);
+hipError_t launch_sep3(const float* in, float* tmp, float* out, int W, int H, int grid, hipStream_t st);
+hipError_t launch_flt_importance(const float* vof, const float* fmean, float* imp, int64_t n, int grid, hipStream_t st);
+hipError_t launch_scale(float* x, float k, int64_t n, int grid, hipStream_t st);
+hipError_t launch_flt_bnorm(const float* film, const float* imp, float* out, int64_t n, int grid, hipStream_t st);
+hipError_t launch_flt_rescale(const float* sumimp, const float* imp, float* newsum, float* f0, float* f1, float* m0,
+                              float* m1, int64_t n, int grid, hipStream_t st);
+hipError_t launch_flt_merge(const unsigned long long* aw, const unsigned long long* ax, const unsigned long long* aq,
+                            double uw, double ux, double uq, float* film, float* mean, float* m2, float* sw, int64_t n,
+                            int grid, hipStream_t st);
+hipError_t launch_flt_bvar(const float* m2, const float* sw, float* out, int64_t n, int grid, hipStream_t st);
+/* dst[i] += src[i] */
+hipError_t launch_add(float* dst, const float* src, int64_t n, int grid, hipStream_t st);
+
 /* film[i] += acc[i] * unit */
 hipError_t launch_mlt_resolve(const unsigned long long* acc, float* film, int64_t n, double unit, int grid,
                               hipStream_t st);

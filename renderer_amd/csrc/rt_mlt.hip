@@ -99,6 +99,29 @@ __device__ __forceinline__ void deposit(unsigned long long* acc, int pix, float 
     if (b > 0.f) atomicAdd(a + 2, __double2ull_rn((double)b * inv_unit));
 }
 
+/* addSample's film deposit w * value and the weight sums of the running
+ * variance, in fixed point */
+__device__ __forceinline__ void flt_deposit(const MltDev& m, int pix, float w, float x, float y, float z)
+{
+    atomicAdd(m.acc_w + pix, __double2ull_rn((double)w * m.inv_unit_w));
+    const float v[3] = {x, y, z};
+    for (int c = 0; c < 3; ++c) {
+        if (!(v[c] > 0.f)) continue;
+        atomicAdd(m.acc + 3 * (size_t)pix + c, __double2ull_rn((double)(w * v[c]) * m.inv_unit));
+        atomicAdd(m.acc_q + 3 * (size_t)pix + c, __double2ull_rn((double)w * v[c] * v[c] * m.inv_unit_q));
+    }
+}
+
+__global__ __launch_bounds__(MLT_BLOCK) void k_flt_boot_weight(MltDev m, const float4* __restrict__ val,
+                                                              float4* out, int n)
+{
+    for (int j = blockIdx.x * MLT_BLOCK + threadIdx.x; j < n; j += gridDim.x * MLT_BLOCK) {
+        const float4 v = val[j];
+        const int pos = pixel_of(m.prop[j], m.W, m.H);
+        out[j] = make_float4(v.x, v.y, v.z, (float)(norm_ref(v.x, v.y, v.z) * (double)m.imp[pos]));
+    }
+}
+
 __global__ __launch_bounds__(MLT_BLOCK) void k_mlt_fill(MltDev m, uint32_t stream, uint32_t index0, int n, int mode)
 {
     for (int j = blockIdx.x * MLT_BLOCK + threadIdx.x; j < n; j += gridDim.x * MLT_BLOCK) {
@@ -108,7 +131,7 @@ __global__ __launch_bounds__(MLT_BLOCK) void k_mlt_fill(MltDev m, uint32_t strea
             if (mode == 2) {
                 v = erpt_group(m.seed, i, g, m.W, m.H, m.spp_x, m.spp_y);
             } else {
-                v = as_f4(draw(m.seed, stream, i, 0, (uint32_t)g));
+                v = as_f4(draw(m.seed, stream, i, m.step, (uint32_t)g));
                 if (mode == 1 && g == 0) {  /* the pixel in pixel units (camera.getRay(pixelCoord)) */
                     v.x = (float)(i % (uint32_t)m.W);
                     v.y = (float)(i / (uint32_t)m.W);
@@ -145,8 +168,10 @@ __global__ __launch_bounds__(MLT_BLOCK) void k_mlt_start(MltDev m, const double*
         }
         const int64_t i = lo < nb ? lo : nb - 1;
         for (int g = 0; g < m.G; ++g)
-            m.cur[(size_t)g * m.cap + j] = as_f4(draw(m.seed, RT_MLT_STREAM_BOOTSTRAP, (uint32_t)i, 0, (uint32_t)g));
-        m.cur_val[j] = bval[i];
+            m.cur[(size_t)g * m.cap + j] = as_f4(draw(m.seed, RT_MLT_STREAM_BOOTSTRAP, (uint32_t)i, m.step, (uint32_t)g));
+        float4 v = bval[i];
+        if (m.kind == RT_MLT_FILTERED) v.w = (float)norm_ref(v.x, v.y, v.z);
+        m.cur_val[j] = v;
     }
 }
 
@@ -189,7 +214,7 @@ __global__ __launch_bounds__(MLT_BLOCK) void k_mlt_propose(MltDev m, uint32_t st
     for (int j = blockIdx.x * MLT_BLOCK + threadIdx.x; j < n; j += gridDim.x * MLT_BLOCK) {
         const uint32_t c = m.chain0 + (uint32_t)j;
         const U4 ctl = draw(m.seed, RT_MLT_STREAM_CONTROL, c, step, 0);
-        const bool large = m.kind == RT_MLT_METROPOLIS && ctl.u[0] < m.large_prob;
+        const bool large = m.kind != RT_MLT_ERPT && ctl.u[0] < m.large_prob;
         for (int g = 0; g < m.G; ++g) {
             float4 v;
             if (large) {
@@ -200,7 +225,7 @@ __global__ __launch_bounds__(MLT_BLOCK) void k_mlt_propose(MltDev m, uint32_t st
                 v.x = perturb(v.x, d.u[0], d.u[1]);
                 v.y = perturb(v.y, d.u[2], d.u[3]);
             }
-            if (g == 0 && m.kind == RT_MLT_METROPOLIS) v.z = ctl.u[1];
+            if (g == 0 && m.kind != RT_MLT_ERPT) v.z = ctl.u[1];
             m.prop[(size_t)g * m.cap + j] = v;
         }
     }
@@ -233,6 +258,23 @@ __global__ __launch_bounds__(MLT_BLOCK) void k_mlt_accept(MltDev m, PathState ps
             }
             move = u < a;
             if (move) m.cur_val[j] = make_float4(r, g, b, ln);
+        } else if (m.kind == RT_MLT_FILTERED) {
+            /* src/metropolisfltrenderer.cpp:166-186: norm x importance */
+            const double zn = norm_ref(r, g, b), cn = (double)cv.w;
+            const int cp = pixel_of(g0c, m.W, m.H), np = pixel_of(g0p, m.W, m.H);
+            const float a = (float)fmin(1.0, zn * (double)m.imp[np] / (cn * (double)m.imp[cp]));
+            if (cn > 0.0 && a < 1.f) {
+                const float w = (1.f - a) / m.newsum[cp];
+                const double k = 1.0 / cn * (double)m.scale;
+                flt_deposit(m, cp, w, (float)(cv.x * k), (float)(cv.y * k), (float)(cv.z * k));
+            }
+            if (zn > 0.0 && a > 0.f) {
+                const float w = a / m.newsum[np];
+                const double k = 1.0 / zn * (double)m.scale;
+                flt_deposit(m, np, w, (float)(r * k), (float)(g * k), (float)(b * k));
+            }
+            move = u < a;
+            if (move) m.cur_val[j] = make_float4(r, g, b, (float)zn);
         } else {
             /* src/energyredistributionrenderer.cpp:92-111 */
             const float zl = (float)norm_ref(r, g, b), yl = cv.w;
@@ -277,6 +319,12 @@ hipError_t launch_mlt_fill(const MltDev& m, uint32_t stream, uint32_t index0, in
                            hipStream_t st)
 {
     hipLaunchKernelGGL(k_mlt_fill, dim3(blocks_for(n, grid)), dim3(MLT_BLOCK), 0, st, m, stream, index0, n, mode);
+    return hipGetLastError();
+}
+
+hipError_t launch_flt_boot_weight(const MltDev& m, const float4* val, float4* out, int n, int grid, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_flt_boot_weight, dim3(blocks_for(n, grid)), dim3(MLT_BLOCK), 0, st, m, val, out, n);
     return hipGetLastError();
 }
 

@@ -5,7 +5,7 @@
  *
  *   example_box <out.raw> [width height spp_x spp_y mode max_depth seed]
  *   example_box <out.raw> --scene <file.scene> [spp_x spp_y mode max_depth seed]
- *   example_box <out.raw> --mlt <metropolis|energyredist> [width height mutations seed]
+ *   example_box <out.raw> --mlt <metropolis|metropolisflt|energyredist> [width height mutations seed]
  *
  * The second form reads the scene and its camera from a scene file
  * (SceneFile, the reference's scene language); the third renders the box
@@ -86,6 +86,10 @@ int main(int argc, char** argv)
             std::vector<float> film;
             if (kind == "metropolis") {
                 rtgpu::MetropolisRenderer r(0.1f, 0.5f, 10000, muts, seed);
+                r.setScene(s);
+                film = r.render(cam);
+            } else if (kind == "metropolisflt") {
+                rtgpu::MetropolisFltRenderer r(3, 0, 0.1f, 0.5f, 10000, muts, seed);
                 r.setScene(s);
                 film = r.render(cam);
             } else if (kind == "energyredist") {

@@ -350,6 +350,21 @@ public:
     }
 };
 
+/* metflt-num-passes, the --metflt-* switches (rt_mlt_filter_flags), and the
+ * met-* options of the chains */
+class MetropolisFltRenderer : public MetropolisRenderer {
+public:
+    MetropolisFltRenderer(int passes = 3, int flags = 0, float largeStepProb = 0.1f, float terminationProb = 0.5f,
+                          int bootstrap = 10000, int mutations = 16, unsigned long seed = 0, int chains = 0,
+                          int device = 0)
+        : MetropolisRenderer(largeStepProb, terminationProb, bootstrap, mutations, seed, chains, device)
+    {
+        p_.kind = RT_MLT_FILTERED;
+        p_.flt_passes = passes;
+        p_.flt_flags = flags;
+    }
+};
+
 /* erpt-mutations, erpt-x-samples, erpt-y-samples, seed */
 class EnergyRedistributionRenderer : public MarkovChainRenderer {
 public:

@@ -131,13 +131,14 @@ class GpuContext:
     @staticmethod
     def make_mlt_params(kind=_abi.MLT_METROPOLIS, max_depth=5, rr_start=3, termination_prob=0.5,
                         large_step_prob=0.1, bootstrap=10000, mutations=16, chains=0, erpt_spp=(2, 2),
-                        erpt_ed_termination=0.1, seed=0):
+                        erpt_ed_termination=0.1, seed=0, flt_passes=3, flt_flags=0):
         p = _abi.RtMltParams()
         p.kind, p.max_depth, p.rr_start = int(kind), int(max_depth), int(rr_start)
         p.termination_prob, p.large_step_prob = float(termination_prob), float(large_step_prob)
         p.bootstrap, p.mutations, p.chains = int(bootstrap), int(mutations), int(chains)
         p.erpt_spp_x, p.erpt_spp_y = int(erpt_spp[0]), int(erpt_spp[1])
         p.erpt_ed_termination, p.seed = float(erpt_ed_termination), int(seed)
+        p.flt_passes, p.flt_flags = int(flt_passes), int(flt_flags)
         return p
 
     @staticmethod
@@ -162,6 +163,16 @@ class GpuContext:
         _check(self.lib, self.h, self.lib.rt_render_mlt_device(self.h, ctypes.byref(cam), ctypes.byref(p),
                                                                ctypes.c_void_p(film_ptr), ctypes.byref(res)))
         return self._mlt_result(res)
+
+    def symmetric_filter(self, image, guide, pixvar, patch=7, radius=10):
+        """SymmetricFilter::filter on (height, width, 3) float32 arrays."""
+        a, g, v = (np.ascontiguousarray(x, np.float32) for x in (image, guide, pixvar))
+        assert a.ndim == 3 and a.shape[2] == 3 and g.shape == a.shape and v.shape == a.shape
+        out = np.empty_like(a)
+        _check(self.lib, self.h, self.lib.rt_symmetric_filter(self.h, a.ctypes.data, g.ctypes.data, v.ctypes.data,
+                                                              a.shape[1], a.shape[0], int(patch), int(radius),
+                                                              out.ctypes.data))
+        return out
 
     def integrate_samples(self, camera, samples, max_depth=5, rr_start=3, termination_prob=0.0):
         """Radiance of paths given by primary samples (n, 2*max_depth+1, 4) float32."""
@@ -309,6 +320,22 @@ class MetropolisRenderer:
             self._scene = scene
         film, self.result = self.ctx.render_mlt(scene.camera, **self.params)
         return film
+
+
+class MetropolisFltRenderer(MetropolisRenderer):
+    """MetropolisFltRenderer (src/metropolisfltrenderer.cpp:60-320) on the
+    GPU: metflt-num-passes passes of two Metropolis films, an importance map
+    from the previous pass's relative variance, and the symmetric
+    cross-filter (src/symmetricfilter.cpp:40-137); `flags` combines
+    FLT_NO_FILTER, FLT_NO_CROSSFILTERING, FLT_NO_VARIANCE_FILTER and
+    FLT_SIMPLE_VARIANCE_FILTER (the --metflt-* switches)."""
+
+    name = "metropolisflt"
+
+    def __init__(self, passes=3, flags=0, large_step_prob=0.1, termination_prob=0.5, bootstrap=10000, mutations=16,
+                 chains=0, seed=0, max_depth=5, context=None):
+        super().__init__(large_step_prob, termination_prob, bootstrap, mutations, chains, seed, max_depth, context)
+        self.params.update(kind=_abi.MLT_FILTERED, flt_passes=passes, flt_flags=flags)
 
 
 class EnergyRedistributionRenderer(MetropolisRenderer):

@@ -1,0 +1,77 @@
+"""The filtered Metropolis renderer (src/metropolisfltrenderer.cpp) and its
+symmetric filter (src/symmetricfilter.cpp) on the GPU against the oracle:
+the filter on the same images (weights below 0.05 are cut, so a rounding
+difference can drop one weight: agreement is counted per pixel), one pass
+without filtering chain by chain with its norm conservation, and the
+default three passes by their image mean."""
+import numpy as np
+import pytest
+
+from oracle.oracle import Oracle, sym_filter
+from renderer_amd import FLT_NO_FILTER, MLT_FILTERED
+from renderer_amd.scene import DiffuseMaterial
+from tests.test_oracle_mlt import _cornell
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("patch,radius", [(7, 10), (1, 2), (3, 4)])
+def test_symmetric_filter_matches_oracle(gpu_ctx, patch, radius):
+    rng = np.random.default_rng(patch)
+    H, W = 37, 45
+    base = np.zeros((H, W, 3), np.float32)
+    base[:, W // 3:] = (0.8, 0.5, 0.2)
+    img = (base + rng.normal(0, 0.05, (H, W, 3))).astype(np.float32)
+    guide = (base + rng.normal(0, 0.05, (H, W, 3))).astype(np.float32)
+    var = rng.uniform(0, 0.005, (H, W, 3)).astype(np.float32)
+    print(f"symmetric filter {W}x{H} patch {patch} radius {radius}", flush=True)
+    got = gpu_ctx.symmetric_filter(img, guide, var, patch, radius)
+    want = sym_filter(img, guide, var, patch, radius)
+    close = np.all(np.abs(got - want) <= 1e-4 * np.abs(want) + 1e-6, axis=2).mean()
+    print(f"agreement {close:.4f}", flush=True)
+    assert close > 0.99
+    assert np.abs(got - want).max() < 0.05
+
+
+def test_filtered_one_pass_matches_oracle(gpu_ctx):
+    sc = _cornell(24, eye=(0.46, 0.53, 2.2), look_at=(0.52, 0.48, 0.0))
+    for q in sc.object.items:
+        q.material = DiffuseMaterial((0.6, 0.6, 0.6))
+    arr = sc.flatten()
+    gpu_ctx.upload(arr)
+    kw = dict(kind=MLT_FILTERED, mutations=16, chains=1024, bootstrap=8000, seed=5, flt_passes=1,
+              flt_flags=FLT_NO_FILTER)
+    print("filtered metropolis, one pass, no filter: 24x24", flush=True)
+    img, res = gpu_ctx.render_mlt(sc.camera, **kw)
+    ref, rref = Oracle(arr).render_mlt(sc.camera, **kw)
+    print("gpu", res, "\noracle", rref, flush=True)
+    for k in ("chains", "mutations", "seed_paths"):
+        assert res[k] == rref[k], k
+    assert res["b"] == pytest.approx(rref["b"], rel=1e-5)
+    assert abs(res["accepted"] - rref["accepted"]) <= 0.01 * rref["accepted"]
+    d = np.abs(img - ref)
+    agree = np.all(d <= 1e-3 * np.abs(ref) + 1e-6, axis=2).mean()
+    print(f"pixel agreement {agree:.4f}", flush=True)
+    assert agree > 0.95
+    # grey scene: norms add; each of the two films holds half the mutations,
+    # each depositing weight 1 of a unit-norm colour times b / 16 (mutations
+    # per pixel), and the output is the films' mean
+    total = np.sqrt((img.astype(np.float64) ** 2).sum(axis=2)).sum()
+    assert total == pytest.approx(res["mutations"] / 2 * res["b"] / 16, rel=1e-4)
+    again, _ = gpu_ctx.render_mlt(sc.camera, **kw)
+    assert np.array_equal(img, again)
+
+
+def test_filtered_default_passes_match_oracle_mean(gpu_ctx):
+    sc = _cornell(24)
+    arr = sc.flatten()
+    gpu_ctx.upload(arr)
+    kw = dict(kind=MLT_FILTERED, mutations=32, chains=1024, bootstrap=8000, seed=9)
+    print("filtered metropolis, three passes: 24x24", flush=True)
+    img, res = gpu_ctx.render_mlt(sc.camera, **kw)
+    ref, rref = Oracle(arr).render_mlt(sc.camera, **kw)
+    print("gpu", res, "\noracle", rref, flush=True)
+    assert res["mutations"] == rref["mutations"] and res["chains"] == rref["chains"]
+    assert abs(res["accepted"] - rref["accepted"]) <= 0.01 * rref["accepted"]
+    assert np.all(np.isfinite(img))
+    assert img.mean() == pytest.approx(ref.mean(), rel=0.05)

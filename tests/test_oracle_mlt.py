@@ -145,3 +145,51 @@ def test_erpt_deposits_e_d_per_chain_step():
     assert res["mutations"] == res["chains"] * M
     total = np.sqrt((film.astype(np.float64) ** 2).sum(axis=2)).sum()
     assert total == pytest.approx(res["chain_seeds"] * M * res["ed"] / spp, rel=1e-4)
+
+
+def test_symmetric_filter_keeps_flat_regions_and_edges():
+    """SymmetricFilter (src/symmetricfilter.cpp): with zero variance a flat
+    region is kept, and so is a step edge whose two sides differ far more
+    than the noise (every weight across it vanishes), away from the border
+    where the zero padding enters the patch distances."""
+    from oracle.oracle import sym_filter
+    H, W, R, P = 40, 44, 10, 7
+    m = R + P  # interior margin
+    img = np.full((H, W, 3), 0.7, np.float32)
+    img[:, W // 2:] = (0.1, 0.2, 0.3)
+    zero = np.zeros_like(img)
+    out = sym_filter(img, img, zero, P, R)
+    assert np.allclose(out[m:H - m, m:W - m], img[m:H - m, m:W - m], rtol=1e-5, atol=0)   # summation rounding
+    # noise of known variance: the filter lowers it where the guide is flat
+    rng = np.random.default_rng(1)
+    noisy = (0.5 + rng.normal(0, 0.05, (H, W, 3))).astype(np.float32)
+    var = np.full_like(noisy, 0.05 ** 2)
+    f = sym_filter(noisy, noisy, var, P, R)
+    assert f[m:H - m, m:W - m].var() < 0.3 * noisy[m:H - m, m:W - m].var()
+    assert f[m:H - m, m:W - m].mean() == pytest.approx(0.5, abs=0.01)
+
+
+def test_filtered_metropolis_conserves_norm_per_film():
+    """MetropolisFltRenderer without filtering, one pass: every mutation
+    deposits weight 1 (importance 1, summed importance 1) of a unit-norm
+    colour times b * passes / mutations-per-pixel, and each film gets half of
+    the mutations, so in a grey scene (norms add) the output, the mean of the
+    two films, has summed norm b * pixels / 2 - the reference's normalisation."""
+    from renderer_amd._abi import FLT_NO_FILTER, MLT_FILTERED
+    sc = _cornell(16)
+    for q in sc.object.items:
+        q.material = DiffuseMaterial((0.6, 0.6, 0.6))
+    orc = Oracle(sc.flatten())
+    film, res = orc.render_mlt(sc.camera, kind=MLT_FILTERED, mutations=16, chains=256, bootstrap=4000, seed=2,
+                               flt_passes=1, flt_flags=FLT_NO_FILTER)
+    total = np.sqrt((film.astype(np.float64) ** 2).sum(axis=2)).sum()
+    assert res["mutations"] == 2 * 256 * 8
+    assert total == pytest.approx(res["b"] * 16 * 16 / 2, rel=1e-4)
+
+
+def test_filtered_metropolis_default_passes_run():
+    from renderer_amd._abi import MLT_FILTERED
+    sc = _cornell(24)
+    film, res = Oracle(sc.flatten()).render_mlt(sc.camera, kind=MLT_FILTERED, mutations=16, bootstrap=1000, seed=3)
+    assert np.all(np.isfinite(film)) and film.mean() > 0
+    assert res["accepted"] > 0
