@@ -365,19 +365,22 @@ def film_outputs(ctx, film):
 
 def markov_chains(ctx, scene, arrays, cpu=True):
     """The Markov-chain renderers with the reference's default options on the
-    benchmark scene (MAX_DEPTH 5): Metropolis with 16 mutations per pixel and
-    ERPT with 100 mutations per chain from 2x2 initial samples per pixel.
+    benchmark scene (MAX_DEPTH 5): Metropolis with 16 mutations per pixel,
+    ERPT with 100 mutations per chain from 2x2 initial samples per pixel, and
+    the filtered Metropolis renderer (3 passes of two films, cross-filtered;
+    its time includes the filtering).
     Throughput = proposals (path evaluations) per second; the oracle runs a
     bounded sample of the same Metropolis workload on the host cores."""
     import torch
 
-    from renderer_amd import MLT_ERPT, MLT_METROPOLIS
+    from renderer_amd import MLT_ERPT, MLT_FILTERED, MLT_METROPOLIS
 
     cam = scene.camera
     film = torch.zeros((cam.height, cam.width, 3), dtype=torch.float32, device="cuda:0")
     out = {}
     for name, kw in (("metropolis", dict(kind=MLT_METROPOLIS, mutations=16, bootstrap=10000, seed=1)),
-                     ("erpt", dict(kind=MLT_ERPT, mutations=100, erpt_spp=(2, 2), seed=1))):
+                     ("erpt", dict(kind=MLT_ERPT, mutations=100, erpt_spp=(2, 2), seed=1)),
+                     ("metropolisflt", dict(kind=MLT_FILTERED, mutations=16, bootstrap=10000, seed=1, flt_passes=3))):
         ctx.render_mlt_device(cam, film.data_ptr(), **kw)  # warm-up
         film.zero_()
         torch.cuda.synchronize()

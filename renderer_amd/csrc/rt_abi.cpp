@@ -1583,10 +1583,10 @@ int run_filtered(rt_ctx* c, const rt_mlt_params& p, int W, int H, int64_t cap_sl
     /* images: films, means, m2 (3 each per film), sw (1 per film), sumimp,
      * imp, newsum, per-pixel scratch (1), newOut, noisy, bvar x2, meanvar,
      * varvar, filteredVar, filteredMean, varOfFiltered, filtered films x2,
-     * filter scratch (15) */
+     * filter scratch (RT_SYM_FILTER_TMP per pixel, last) */
     const size_t n3 = (size_t)area * 3;
     float* im = nullptr;
-    const size_t total = n3 * 6 + (size_t)area * 2 + (size_t)area * 4 + n3 * 11 + (size_t)area * 15;
+    const size_t total = n3 * 6 + (size_t)area * 6 + n3 * 11 + (size_t)area * RT_SYM_FILTER_TMP;
     if (int r = ensure(c, c->flt_img, total * 4)) return r;
     if (int r = ensure(c, c->flt_acc, (size_t)area * 4 * sizeof(unsigned long long))) return r;
     im = (float*)c->flt_img.p;
@@ -1872,7 +1872,7 @@ int rt_symmetric_filter(rt_ctx* c, const float* image, const float* guide, const
         return set_err(c, RT_ERR_INVALID, "bad filter size");
     hipSetDevice(c->device);
     const size_t n3 = (size_t)width * height * 3, n = (size_t)width * height;
-    if (int r = ensure(c, c->flt_img, (n3 * 4 + n * 15) * 4)) return r;
+    if (int r = ensure(c, c->flt_img, (n3 * 4 + n * RT_SYM_FILTER_TMP) * 4)) return r;
     float* d = (float*)c->flt_img.p;
     HIP_TRY(c, hipMemcpyAsync(d, image, n3 * 4, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(c, hipMemcpyAsync(d + n3, guide, n3 * 4, hipMemcpyHostToDevice, c->stream));

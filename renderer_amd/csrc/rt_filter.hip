@@ -279,15 +279,16 @@ hipError_t launch_var2(const float* a, const float* b, float* mean, float* var, 
 }
 
 /* SymmetricFilter(patch, radius).filter(image, guide, pixvar) into out.
- * tmp: 15 * W * H floats of scratch (three distance planes, three
- * denominators, two blur buffers of three planes, the weight sums of one
- * channel and the accumulated values of three). */
+ * tmp: RT_SYM_FILTER_TMP * W * H floats of scratch (three distance planes,
+ * three denominators, two blur buffers of three planes, the weight sums of
+ * one channel and the accumulated values of three). */
 hipError_t launch_sym_filter(const float* image, const float* guide, const float* pixvar, int W, int H, int patch,
                              int radius, float* tmp, float* out, int grid, hipStream_t st)
 {
     const int64_t n = (int64_t)W * H;
     float *d2 = tmp, *den = tmp + 3 * n, *b0 = tmp + 6 * n, *b1 = tmp + 9 * n, *area = tmp + 12 * n,
-          *acc = tmp + 13 * n;
+          *acc = tmp + 13 * n;  /* acc ends at 16 n = RT_SYM_FILTER_TMP * n */
+    static_assert(RT_SYM_FILTER_TMP == 16, "scratch layout");
     const int g1 = blocks(n, grid), g3 = blocks(3 * n, grid);
     hipLaunchKernelGGL(k_sf_init, dim3(g1), dim3(FB), 0, st, area, acc, n);
     for (int dx = -radius; dx <= 0; ++dx)

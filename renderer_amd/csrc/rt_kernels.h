@@ -159,6 +159,10 @@ hipError_t launch_mlt_accept(const MltDev& m, const PathState& ps, uint32_t step
 hipError_t launch_flt_boot_weight(const MltDev& m, const float4* val, float4* out, int n, int grid, hipStream_t st);
 /* image passes (rt_filter.hip) */
 hipError_t launch_var2(const float* a, const float* b, float* mean, float* var, int64_t n, int grid, hipStream_t st);
+/* scratch of launch_sym_filter in floats per pixel: distance and denominator
+ * planes (3 + 3), two blur buffers (3 + 3), weight sums (1), accumulated
+ * colours (3) */
+constexpr int64_t RT_SYM_FILTER_TMP = 16;
 hipError_t launch_sym_filter(const float* image, const float* guide, const float* pixvar, int W, int H, int patch,
                              int radius, float* tmp, float* out, int grid, hipStream_t st);
 hipError_t launch_sep3(const float* in, float* tmp, float* out, int W, int H, int grid, hipStream_t st);

@@ -15,10 +15,11 @@ from tests.test_oracle_mlt import _cornell
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("patch,radius", [(7, 10), (1, 2), (3, 4)])
-def test_symmetric_filter_matches_oracle(gpu_ctx, patch, radius):
+@pytest.mark.parametrize("patch,radius,H,W", [(7, 10, 37, 45), (1, 2, 37, 45), (3, 4, 37, 45), (7, 10, 220, 300)])
+def test_symmetric_filter_matches_oracle(gpu_ctx, patch, radius, H, W):
+    # 300 x 220: the filter's scratch (16 floats per pixel) is the last
+    # region of its buffer, so a short scratch would write 256 KiB past it
     rng = np.random.default_rng(patch)
-    H, W = 37, 45
     base = np.zeros((H, W, 3), np.float32)
     base[:, W // 3:] = (0.8, 0.5, 0.2)
     img = (base + rng.normal(0, 0.05, (H, W, 3))).astype(np.float32)
