@@ -1577,9 +1577,11 @@ int run_filtered(rt_ctx* c, const rt_mlt_params& p, int W, int H, int64_t cap_sl
     const int64_t nb = std::max<int64_t>(p.bootstrap, C);
     if (nb > INT32_MAX || (int64_t)P * 2 * C > INT32_MAX || iters > INT32_MAX)
         return set_err(c, RT_ERR_INVALID, "too many chains or mutations");
-    const int cap = (int)std::min<int64_t>(cap_slots, std::max(C, nb));
+    /* both films' chains in one batch: chain ids (pass * 2 + film) * C + j */
+    const int cap = (int)std::min<int64_t>(cap_slots, std::max(2 * C, nb));
     if (int r = mlt_alloc(c, cap, G, area)) return r;
     if (int r = ensure(c, c->mlt_seed, (size_t)nb * sizeof(float4) * 2)) return r;
+    if (int r = ensure(c, c->mlt_acc, (size_t)area * 6 * sizeof(unsigned long long))) return r;
     /* images: films, means, m2 (3 each per film), sw (1 per film), sumimp,
      * imp, newsum, per-pixel scratch (1), newOut, noisy, bvar x2, meanvar,
      * varvar, filteredVar, filteredMean, varOfFiltered, filtered films x2,
@@ -1588,7 +1590,7 @@ int run_filtered(rt_ctx* c, const rt_mlt_params& p, int W, int H, int64_t cap_sl
     float* im = nullptr;
     const size_t total = n3 * 6 + (size_t)area * 6 + n3 * 11 + (size_t)area * RT_SYM_FILTER_TMP;
     if (int r = ensure(c, c->flt_img, total * 4)) return r;
-    if (int r = ensure(c, c->flt_acc, (size_t)area * 4 * sizeof(unsigned long long))) return r;
+    if (int r = ensure(c, c->flt_acc, (size_t)area * 8 * sizeof(unsigned long long))) return r;
     im = (float*)c->flt_img.p;
     float *film[2] = {im, im + n3}, *mean[2] = {im + 2 * n3, im + 3 * n3}, *m2[2] = {im + 4 * n3, im + 5 * n3};
     float* q = im + 6 * n3;
@@ -1598,8 +1600,8 @@ int run_filtered(rt_ctx* c, const rt_mlt_params& p, int W, int H, int64_t cap_sl
     float *newOut = q, *noisy = q + n3, *bv[2] = {q + 2 * n3, q + 3 * n3}, *meanvar = q + 4 * n3, *varvar = q + 5 * n3,
           *fvar = q + 6 * n3, *fmean = q + 7 * n3, *vof = q + 8 * n3, *ff[2] = {q + 9 * n3, q + 10 * n3};
     float* scratch = q + 11 * n3;
-    unsigned long long* aw = (unsigned long long*)c->flt_acc.p;
-    unsigned long long* aq = aw + area;
+    unsigned long long* aw = (unsigned long long*)c->flt_acc.p;  /* [film][pixel] */
+    unsigned long long* aq = aw + 2 * area;                        /* [film][pixel][3] */
     unsigned long long* ax = (unsigned long long*)c->mlt_acc.p;
     const int grid = grid_for(c);
     MLT_TRY(hipMemsetAsync(im, 0, (size_t)(n3 * 6 + area * 3) * 4, c->stream));  /* films, stats, sumimp */
@@ -1610,6 +1612,7 @@ int run_filtered(rt_ctx* c, const rt_mlt_params& p, int W, int H, int64_t cap_sl
     m.newsum = newsum;
     m.acc_w = aw;
     m.acc_q = aq;
+    m.film_chains = (uint32_t)C;
     std::vector<float> hbuf((size_t)area);
     std::vector<float4> hv((size_t)nb);
     int64_t mutations = 0;
@@ -1673,23 +1676,23 @@ int run_filtered(rt_ctx* c, const rt_mlt_params& p, int W, int H, int64_t cap_sl
             m.inv_unit = 1.0 / ux;
             m.inv_unit_w = 1.0 / uw;
             m.inv_unit_q = 1.0 / uq;
-            for (int n = 0; n < 2; ++n) {
-                MLT_TRY(hipMemsetAsync(ax, 0, n3 * 8, c->stream));
-                MLT_TRY(hipMemsetAsync(aw, 0, (size_t)area * 4 * 8, c->stream));
-                for (int64_t c0 = 0; c0 < C; c0 += cap) {
-                    const int nn = (int)std::min<int64_t>(cap, C - c0);
-                    m.chain0 = (uint32_t)(((int64_t)pass * 2 + n) * C + c0);
-                    MLT_TRY(launch_mlt_start(m, (const double*)c->mlt_cdf.p, nb, tot, bval, nn, grid, c->stream));
-                    for (int64_t it = 0; it < iters; ++it) {
-                        MLT_TRY(launch_mlt_propose(m, (uint32_t)it, nn, grid, c->stream));
-                        MLT_TRY(mlt_trace(c, nn, cap, p.max_depth, p.rr_start, p.termination_prob, p.seed, 0));
-                        MLT_TRY(launch_mlt_accept(m, c->ps, (uint32_t)it, nn, grid, c->stream));
-                    }
+            MLT_TRY(hipMemsetAsync(ax, 0, n3 * 2 * 8, c->stream));
+            MLT_TRY(hipMemsetAsync(aw, 0, (size_t)area * 8 * 8, c->stream));
+            for (int64_t c0 = 0; c0 < 2 * C; c0 += cap) {
+                const int nn = (int)std::min<int64_t>(cap, 2 * C - c0);
+                m.chain0 = (uint32_t)((int64_t)pass * 2 * C + c0);
+                MLT_TRY(launch_mlt_start(m, (const double*)c->mlt_cdf.p, nb, tot, bval, nn, grid, c->stream));
+                for (int64_t it = 0; it < iters; ++it) {
+                    if (c->stop) return set_err(c, RT_ERR_CANCELLED, "render cancelled");
+                    MLT_TRY(launch_mlt_propose(m, (uint32_t)it, nn, grid, c->stream));
+                    MLT_TRY(mlt_trace(c, nn, cap, p.max_depth, p.rr_start, p.termination_prob, p.seed, 0));
+                    MLT_TRY(launch_mlt_accept(m, c->ps, (uint32_t)it, nn, grid, c->stream));
                 }
-                MLT_TRY(launch_flt_merge(aw, ax, aq, uw, ux, uq, film[n], mean[n], m2[n], sw[n], area, grid,
-                                         c->stream));
-                mutations += C * iters;
             }
+            for (int n = 0; n < 2; ++n)
+                MLT_TRY(launch_flt_merge(aw + n * area, ax + n * n3, aq + n * n3, uw, ux, uq, film[n], mean[n], m2[n],
+                                         sw[n], area, grid, c->stream));
+            mutations += 2 * C * iters;
         }
         /* filtering (:252-300) */
         for (int n = 0; n < 2; ++n) MLT_TRY(launch_flt_bvar(m2[n], sw[n], bv[n], area, grid, c->stream));

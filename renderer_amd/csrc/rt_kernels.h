@@ -134,6 +134,8 @@ struct MltDev {
     unsigned long long* acc_q;
     double inv_unit_w, inv_unit_q;
     uint32_t step;      /* bootstrap draws: the pass number (0 outside the filtered renderer) */
+    uint32_t film_chains;  /* filtered: chains per film; chain id / film_chains odd = the second film,
+                              whose sums follow the first film's (W * H weights, W * H * 3 values) */
 };
 
 /* Fill the proposal vectors of n paths with fresh draws of `stream` for

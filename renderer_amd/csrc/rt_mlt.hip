@@ -101,14 +101,15 @@ __device__ __forceinline__ void deposit(unsigned long long* acc, int pix, float 
 
 /* addSample's film deposit w * value and the weight sums of the running
  * variance, in fixed point */
-__device__ __forceinline__ void flt_deposit(const MltDev& m, int pix, float w, float x, float y, float z)
+__device__ __forceinline__ void flt_deposit(const MltDev& m, int film, int pix, float w, float x, float y, float z)
 {
-    atomicAdd(m.acc_w + pix, __double2ull_rn((double)w * m.inv_unit_w));
+    const size_t pw = (size_t)film * m.W * m.H + pix;
+    atomicAdd(m.acc_w + pw, __double2ull_rn((double)w * m.inv_unit_w));
     const float v[3] = {x, y, z};
     for (int c = 0; c < 3; ++c) {
         if (!(v[c] > 0.f)) continue;
-        atomicAdd(m.acc + 3 * (size_t)pix + c, __double2ull_rn((double)(w * v[c]) * m.inv_unit));
-        atomicAdd(m.acc_q + 3 * (size_t)pix + c, __double2ull_rn((double)w * v[c] * v[c] * m.inv_unit_q));
+        atomicAdd(m.acc + 3 * pw + c, __double2ull_rn((double)(w * v[c]) * m.inv_unit));
+        atomicAdd(m.acc_q + 3 * pw + c, __double2ull_rn((double)w * v[c] * v[c] * m.inv_unit_q));
     }
 }
 
@@ -262,16 +263,17 @@ __global__ __launch_bounds__(MLT_BLOCK) void k_mlt_accept(MltDev m, PathState ps
             /* src/metropolisfltrenderer.cpp:166-186: norm x importance */
             const double zn = norm_ref(r, g, b), cn = (double)cv.w;
             const int cp = pixel_of(g0c, m.W, m.H), np = pixel_of(g0p, m.W, m.H);
+            const int film = (int)((c / m.film_chains) & 1u);
             const float a = (float)fmin(1.0, zn * (double)m.imp[np] / (cn * (double)m.imp[cp]));
             if (cn > 0.0 && a < 1.f) {
                 const float w = (1.f - a) / m.newsum[cp];
                 const double k = 1.0 / cn * (double)m.scale;
-                flt_deposit(m, cp, w, (float)(cv.x * k), (float)(cv.y * k), (float)(cv.z * k));
+                flt_deposit(m, film, cp, w, (float)(cv.x * k), (float)(cv.y * k), (float)(cv.z * k));
             }
             if (zn > 0.0 && a > 0.f) {
                 const float w = a / m.newsum[np];
                 const double k = 1.0 / zn * (double)m.scale;
-                flt_deposit(m, np, w, (float)(r * k), (float)(g * k), (float)(b * k));
+                flt_deposit(m, film, np, w, (float)(r * k), (float)(g * k), (float)(b * k));
             }
             move = u < a;
             if (move) m.cur_val[j] = make_float4(r, g, b, (float)zn);
