@@ -18,6 +18,12 @@
  * counter-based Philox-4x32-10 stream that the GPU path uses; the layout of
  * the stream is fixed by philox_uniforms() below and is the contract that
  * lets the GPU image be compared pixel by pixel with this one.
+ *
+ * Parity unpinned (no reference vectors and no closed form checks them):
+ * the filtered Metropolis renderer's importance map of later passes, its
+ * weighted running variance, and the border arithmetic of the OpenCV calls
+ * it restates (blur, sepFilter2D); the symmetric filter itself and one pass
+ * without filtering are pinned by the tests of test_oracle_mlt.py.
  */
 #include "pt_oracle.h"
 
