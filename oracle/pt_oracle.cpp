@@ -1882,7 +1882,10 @@ void flt_pass(MltRun& R, int pass, const Img& imp, const Img& newsum, float b, F
     R.res.chains += 2 * C;
     R.res.seed_paths += nb;
     if (!(acc > 0)) return;
-    const float scale = (float)((double)b * P / R.p.mutations);
+    // b * area / 2 per film and pass, over the C * iters mutations that run
+    // (src/metropolisfltrenderer.cpp:139,158-165 aim at that total with
+    // b * passes / mutations-per-pixel per deposit)
+    const float scale = (float)((double)b * (double)area / (2.0 * (double)C * (double)iters));
     for (int n = 0; n < 2; ++n) {
         std::vector<std::vector<std::pair<int, std::pair<float, Col>>>> deps(C);
 #pragma omp parallel for schedule(dynamic, 16) num_threads(R.threads) reduction(+ : rays, accepted)

@@ -1669,7 +1669,12 @@ int run_filtered(rt_ctx* c, const rt_mlt_params& p, int W, int H, int64_t cap_sl
             MLT_TRY(hipStreamSynchronize(c->stream));
             float mn = INFINITY;
             for (float v : hbuf) mn = std::min(mn, v);
-            m.scale = (float)((double)b * P / p.mutations);
+            /* the reference scales each deposit by b * passes / mutations-per-pixel
+             * over (mutations-per-pixel * area / passes / 2) mutations per film
+             * (src/metropolisfltrenderer.cpp:139,158-165): b * area / 2 per film
+             * and pass.  The chains here run C * iters mutations per film, so
+             * the scale that keeps that total is b * area / (2 C iters). */
+            m.scale = (float)((double)b * (double)area / (2.0 * (double)C * (double)iters));
             const double wmax = 1.0 / mn, deps = 2.0 * (double)C * (double)iters;
             const double ux = mlt_unit(wmax * m.scale * 1.01, deps), uw = mlt_unit(wmax * 1.01, deps),
                          uq = mlt_unit(wmax * (double)m.scale * m.scale * 1.01, deps);

@@ -8,7 +8,8 @@ import numpy as np
 import pytest
 
 from oracle.oracle import Oracle, sym_filter
-from renderer_amd import FLT_NO_FILTER, MLT_FILTERED
+from renderer_amd import (FLT_NO_CROSSFILTERING, FLT_NO_FILTER, FLT_NO_VARIANCE_FILTER, FLT_SIMPLE_VARIANCE_FILTER,
+                          MLT_FILTERED)
 from renderer_amd.scene import DiffuseMaterial
 from tests.test_oracle_mlt import _cornell
 
@@ -54,11 +55,12 @@ def test_filtered_one_pass_matches_oracle(gpu_ctx):
     agree = np.all(d <= 1e-3 * np.abs(ref) + 1e-6, axis=2).mean()
     print(f"pixel agreement {agree:.4f}", flush=True)
     assert agree > 0.95
-    # grey scene: norms add; each of the two films holds half the mutations,
-    # each depositing weight 1 of a unit-norm colour times b / 16 (mutations
-    # per pixel), and the output is the films' mean
+    # grey scene: norms add; each film receives b * pixels / 2 per pass (the
+    # reference's normalisation), however many steps the chains take: 1024
+    # chains run 5 steps each, 5120 mutations per film for a nominal 4608
     total = np.sqrt((img.astype(np.float64) ** 2).sum(axis=2)).sum()
-    assert total == pytest.approx(res["mutations"] / 2 * res["b"] / 16, rel=1e-4)
+    assert res["mutations"] == 2 * 1024 * 5
+    assert total == pytest.approx(res["b"] * 24 * 24 / 2, rel=1e-4)
     again, _ = gpu_ctx.render_mlt(sc.camera, **kw)
     assert np.array_equal(img, again)
 
@@ -76,3 +78,54 @@ def test_filtered_default_passes_match_oracle_mean(gpu_ctx):
     assert abs(res["accepted"] - rref["accepted"]) <= 0.01 * rref["accepted"]
     assert np.all(np.isfinite(img))
     assert img.mean() == pytest.approx(ref.mean(), rel=0.05)
+
+
+def _agreement(img, ref, rtol=1e-3):
+    return np.all(np.abs(img - ref) <= rtol * np.abs(ref) + 1e-6, axis=2).mean()
+
+
+def test_filtered_three_passes_without_filter_match_oracle_pixels(gpu_ctx):
+    """Three passes with FLT_NO_FILTER: the importance map of the previous
+    pass's relative variance and its normalisation, the rescale of both films
+    by summed / new summed importance, b from the first film, and the running
+    mean and variance merge, compared pixel by pixel (no weight cut of the
+    filter is involved)."""
+    sc = _cornell(24, eye=(0.46, 0.53, 2.2), look_at=(0.52, 0.48, 0.0))
+    arr = sc.flatten()
+    gpu_ctx.upload(arr)
+    kw = dict(kind=MLT_FILTERED, mutations=24, chains=1000, bootstrap=8000, seed=11, flt_passes=3,
+              flt_flags=FLT_NO_FILTER)
+    print("filtered metropolis, three passes, no filter: 24x24", flush=True)
+    img, res = gpu_ctx.render_mlt(sc.camera, **kw)
+    ref, rref = Oracle(arr).render_mlt(sc.camera, **kw)
+    print("gpu", res, "\noracle", rref, flush=True)
+    for k in ("chains", "mutations", "seed_paths"):
+        assert res[k] == rref[k], k
+    assert res["b"] == pytest.approx(rref["b"], rel=1e-5)
+    assert abs(res["accepted"] - rref["accepted"]) <= 0.01 * rref["accepted"]
+    agree = _agreement(img, ref)
+    print(f"pixel agreement {agree:.4f}", flush=True)
+    assert agree > 0.95
+    assert img.sum() == pytest.approx(ref.sum(), rel=2e-3)
+
+
+@pytest.mark.parametrize("flag", [FLT_NO_CROSSFILTERING, FLT_NO_VARIANCE_FILTER, FLT_SIMPLE_VARIANCE_FILTER])
+def test_filtered_flags_match_oracle(gpu_ctx, flag):
+    """Each remaining --metflt-* switch, two passes: the filter paths it
+    selects (FLT_SIMPLE_VARIANCE_FILTER runs the separable 3-tap variance
+    blur, k_sep3) against the oracle's."""
+    sc = _cornell(24, eye=(0.46, 0.53, 2.2), look_at=(0.52, 0.48, 0.0))
+    arr = sc.flatten()
+    gpu_ctx.upload(arr)
+    kw = dict(kind=MLT_FILTERED, mutations=16, chains=1000, bootstrap=8000, seed=13, flt_passes=2, flt_flags=flag)
+    print(f"filtered metropolis, flags {flag}: 24x24", flush=True)
+    img, res = gpu_ctx.render_mlt(sc.camera, **kw)
+    ref, rref = Oracle(arr).render_mlt(sc.camera, **kw)
+    print("gpu", res, "\noracle", rref, flush=True)
+    assert res["mutations"] == rref["mutations"]
+    assert abs(res["accepted"] - rref["accepted"]) <= 0.01 * rref["accepted"]
+    assert np.all(np.isfinite(img))
+    agree = _agreement(img, ref, 1e-2)
+    print(f"pixel agreement {agree:.4f}", flush=True)
+    assert agree > 0.9
+    assert img.mean() == pytest.approx(ref.mean(), rel=0.01)
