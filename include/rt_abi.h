@@ -65,7 +65,7 @@ typedef struct rt_stats {
     int32_t bvh_depth;        /* depth of the uploaded BVH */
     int32_t paths_per_pass;   /* path slots used by the last render */
     int32_t passes;           /* passes of the last render */
-    int32_t reserved;
+    int32_t bvh_stack_need;   /* most traversal-stack entries a ray of the uploaded BVH can hold */
     int64_t node_visits;      /* interior nodes fetched by closest-hit traversal */
     int64_t prim_tests;       /* primitive records tested by closest-hit traversal */
     /* RT_MODE_GUIDED only (0 otherwise): network time and work of the last render */
@@ -103,6 +103,19 @@ int rt_trace_closest_device(rt_ctx* ctx, const float* o_tmin, const float* d_tma
 int rt_trace_any(rt_ctx* ctx, const float* o_tmin, const float* d_tmax, int64_t n, uint8_t* occluded);
 int rt_trace_any_device(rt_ctx* ctx, const float* o_tmin, const float* d_tmax, int64_t n,
                         uint8_t* occluded);
+
+/* rt_trace_closest_device for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  It calls no hipMalloc or
+ * hipFree: the traversal-stack overflow is a static array of 24 MiB per GPU
+ * (held from the library's first use on that GPU until the process ends),
+ * whose address rt_create looks up.  Same arguments and results as
+ * rt_trace_closest_device; n above RT_NOALLOC_MAX_RAYS is RT_ERR_INVALID.
+ * Calls that overlap on one GPU (from several contexts or threads) share the
+ * array and race. */
+#define RT_NOALLOC_MAX_RAYS ((int64_t)1 << 31)
+int rt_trace_closest_device_noalloc(rt_ctx* ctx, const float* o_tmin, const float* d_tmax, int64_t n,
+                                    int32_t* prim, float* t);
 
 /* Traversal statistics of a ray batch (device pointers): interior nodes
  * visited and primitives tested per ray by the closest-hit traversal. */

@@ -91,7 +91,7 @@ class RtStats(ctypes.Structure):
         ("bvh_depth", ctypes.c_int32),
         ("paths_per_pass", ctypes.c_int32),
         ("passes", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("bvh_stack_need", ctypes.c_int32),
         ("node_visits", ctypes.c_int64),
         ("prim_tests", ctypes.c_int64),
         ("guide_infer_ms", ctypes.c_double),
@@ -153,6 +153,7 @@ PROTOTYPES = {
     "rt_trace_any": (ctypes.c_int, [_P, _P, _P, _I64, _P]),
     "rt_trace_any_device": (ctypes.c_int, [_P, _P, _P, _I64, _P]),
     "rt_trace_count_device": (ctypes.c_int, [_P, _P, _P, _I64, _P, _P]),
+    "rt_trace_closest_device_noalloc": (ctypes.c_int, [_P, _P, _P, _I64, _P, _P]),
     "rt_render": (ctypes.c_int, [_P, ctypes.POINTER(RtView), _I32, ctypes.POINTER(RtParams), _P]),
     "rt_render_device": (ctypes.c_int, [_P, ctypes.POINTER(RtView), _I32, ctypes.POINTER(RtParams), _P]),
     "rt_set_launch_config": (ctypes.c_int, [_P, ctypes.POINTER(RtLaunchConfig)]),

@@ -208,6 +208,12 @@ int rt_create(int device, rt_ctx** out)
     }
     int cu = 0;
     if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cu > 0) c->n_cu = cu;
+    /* the allocation-free trace's static scratch: its address on this device */
+    if (noalloc_scratch_init(device) != hipSuccess) {
+        hipStreamDestroy(c->stream);
+        delete c;
+        return RT_ERR_HIP;
+    }
     hipEventCreate(&c->ev_a);
     hipEventCreate(&c->ev_b);
     *out = c;
@@ -663,6 +669,7 @@ int rt_scene_upload(rt_ctx* c, const rt_scene_desc* d)
     c->stats.bvh_nodes = s.n_nodes;
     c->stats.bvh_prims = G;
     c->stats.bvh_depth = br.depth;
+    c->stats.bvh_stack_need = br.stack_need;
     // scene box (finite primitives) for the guiding network's position encoding
     {
         float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
@@ -733,6 +740,38 @@ int rt_trace_count_device(rt_ctx* c, const float* o, const float* d, int64_t n, 
     if (!c || n < 0 || (n && (!o || !d || !nodes || !prims))) return RT_ERR_INVALID;
     if (int r = check_fault(c)) return r;
     return trace_device(c, 2, o, d, n, nodes, prims);
+}
+
+/* rt_trace_closest_device without allocation: the launch is capped at
+ * RT_NOALLOC_BLOCKS blocks (any ray count is covered by the kernel's
+ * grid-stride loop) and its stack overflow is the static scratch array whose
+ * address rt_create looked up on this device. */
+int rt_trace_closest_device_noalloc(rt_ctx* c, const float* o, const float* d, int64_t n, int32_t* prim, float* t)
+{
+    if (!c || n < 0 || (n && (!o || !d || !prim || !t))) return RT_ERR_INVALID;
+    if (n > RT_NOALLOC_MAX_RAYS)
+        return set_err(c, RT_ERR_INVALID, "rt_trace_closest_device_noalloc: more than RT_NOALLOC_MAX_RAYS rays");
+    if (int r = check_fault(c)) return r;
+    if (!c->has_scene) return set_err(c, RT_ERR_NO_SCENE, "no scene uploaded");
+    if (n == 0) return RT_OK;
+    hipSetDevice(c->device);
+    int32_t* ovf = noalloc_scratch(c->device);
+    if (!ovf) return set_err(c, RT_ERR_HIP, "allocation-free trace: no scratch address on this device");
+    const int grid = (int)std::min<int64_t>((n + 255) / 256, RT_NOALLOC_BLOCKS);
+    if (trav_lanes(grid) * RT_STACK_OVF > noalloc_scratch_entries())
+        return set_err(c, RT_ERR_INVALID, "allocation-free trace: launch exceeds the scratch array");
+    HostDevScene s = c->scene;
+    s.stack_ovf = ovf;
+    HIP_TRY(c, hipEventRecord(c->ev_a, c->stream));
+    hipError_t e = launch_trace_closest(s, (const float4*)o, (const float4*)d, n, prim, t, 256, grid, c->stream);
+    if (e != hipSuccess) return set_err(c, RT_ERR_HIP, std::string("trace launch: ") + hipGetErrorString(e));
+    HIP_TRY(c, hipEventRecord(c->ev_b, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    float ms = 0;
+    hipEventElapsedTime(&ms, c->ev_a, c->ev_b);
+    c->stats.extend_ms = ms;
+    c->stats.extend_launches = 1;
+    return RT_OK;
 }
 
 static int trace_host(rt_ctx* c, int kind, const float* o, const float* d, int64_t n, void* out0, void* out1)

@@ -196,6 +196,17 @@ hipError_t launch_trace_any(const HostDevScene& s, const float4* o, const float4
 hipError_t launch_trace_count(const HostDevScene& s, const float4* o, const float4* d, int64_t n, int32_t* nodes,
                               int32_t* prims, int block, int grid, hipStream_t st);
 
+/* Allocation-free closest hit.  The traversal-stack overflow of launches of
+ * at most RT_NOALLOC_BLOCKS blocks in a static device array (RT_STACK_OVF
+ * entries per lane): noalloc_scratch_init(device), with that device current,
+ * looks up its address once per device; noalloc_scratch(device) returns it
+ * (nullptr before the lookup). */
+#define RT_NOALLOC_BLOCKS 2048
+#define RT_NOALLOC_MAX_DEVICES 64
+hipError_t noalloc_scratch_init(int device);
+int32_t* noalloc_scratch(int device);
+int64_t noalloc_scratch_entries();
+
 /* Wavefront pipeline kernels. */
 hipError_t launch_generate(const PathState& ps, const DevView* views, const DevTile* tiles, const DevPass* passes,
                            PtParams p, int grid, hipStream_t st);

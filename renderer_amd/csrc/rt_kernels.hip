@@ -22,6 +22,7 @@
 #include "rt_kernels.h"
 
 #include <cstring>
+#include <mutex>
 
 #define PT_BLOCK 256
 #define RT_PI 3.14159265358979f
@@ -193,6 +194,43 @@ hipError_t launch_trace_count(const HostDevScene& s, const float4* o, const floa
                        (int32_t*)nullptr, (float*)nullptr, (uint8_t*)nullptr, nodes, prims);
     return hipGetLastError();
 }
+
+/* ---------------------------------------------------------------------------
+ * Allocation-free closest hit (rt_trace_closest_device_noalloc): the stack
+ * overflow of a launch of at most RT_NOALLOC_BLOCKS blocks lives in this
+ * file-scope array instead of memory the context allocates.  Each GPU holds
+ * its own copy (24 MiB) from the first use of the library on it until the
+ * process ends.  Host code never passes `scratch` itself to a kernel: that
+ * name is a host-side stand-in, not a device address.  rt_create looks up the
+ * array's address on its device once (noalloc_scratch_init) and every launch
+ * takes that pointer.  Nothing in it needs clearing: a stack entry is written
+ * before it is read.  Calls that overlap on one GPU share the array and race.
+ * ------------------------------------------------------------------------- */
+static __device__ __align__(256) int32_t scratch[(size_t)RT_NOALLOC_BLOCKS * PT_BLOCK * RT_STACK_OVF];
+
+static std::mutex g_scratch_mu;
+static int32_t* g_scratch_ptr[RT_NOALLOC_MAX_DEVICES];
+
+hipError_t noalloc_scratch_init(int device)
+{
+    if (device < 0 || device >= RT_NOALLOC_MAX_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return hipSuccess;
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[device] = (int32_t*)p;
+    return hipSuccess;
+}
+
+int32_t* noalloc_scratch(int device)
+{
+    if (device < 0 || device >= RT_NOALLOC_MAX_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return g_scratch_ptr[device];
+}
+
+int64_t noalloc_scratch_entries() { return (int64_t)RT_NOALLOC_BLOCKS * PT_BLOCK * RT_STACK_OVF; }
 
 /* ===========================================================================
  * Sampling and material helpers

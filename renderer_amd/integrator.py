@@ -216,7 +216,7 @@ class GpuContext:
     def stats(self):
         s = _abi.RtStats()
         _check(self.lib, self.h, self.lib.rt_get_stats(self.h, ctypes.byref(s)))
-        return {f: getattr(s, f) for f, _ in s._fields_ if f != "reserved"}
+        return {f: getattr(s, f) for f, _ in s._fields_}
 
     def request_stop(self, flag=True):
         _check(self.lib, self.h, self.lib.rt_request_stop(self.h, 1 if flag else 0))
