@@ -9,6 +9,9 @@
 #include <sys/stat.h>
 #include <zlib.h>
 #include <cctype>
+#include <climits>
+#include <cstdint>
+#include <exception>
 #include <cstdlib>
 
 #include <cmath>
@@ -349,16 +352,22 @@ bool decode_pnm(const std::vector<uint8_t>& b, int& w, int& h, std::vector<uint8
         }
         if (pos >= b.size() || !std::isdigit(b[pos])) return false;
         v = 0;
-        while (pos < b.size() && std::isdigit(b[pos])) v = v * 10 + (b[pos++] - '0');
+        while (pos < b.size() && std::isdigit(b[pos])) {
+            const int dgt = b[pos++] - '0';
+            if (v > (INT_MAX - dgt) / 10) return false;  /* the number does not fit an int */
+            v = v * 10 + dgt;
+        }
         return true;
     };
     const bool color = b[1] == '6';
     int maxv = 0;
-    if (!field(w) || !field(h) || !field(maxv) || w <= 0 || h <= 0 || maxv <= 0 || maxv > 65535)
+    if (!field(w) || !field(h) || !field(maxv) || w <= 0 || h <= 0 || maxv <= 0 || maxv > 65535 ||
+        (int64_t)w * h > (1ll << 28))
         return err = "bad PNM header", false;
     ++pos;  // one whitespace byte after maxval
     const int ch = color ? 3 : 1, bytes = maxv > 255 ? 2 : 1;
-    if (pos + (size_t)w * h * ch * bytes > b.size()) return err = "truncated PNM data", false;
+    /* w * h <= 2^28, so the byte count below stays far from overflow */
+    if (pos > b.size() || (uint64_t)w * h * ch * bytes > b.size() - pos) return err = "truncated PNM data", false;
     rgb.resize((size_t)3 * w * h);
     for (size_t i = 0; i < (size_t)w * h; ++i)
         for (int k = 0; k < 3; ++k) {
@@ -378,11 +387,13 @@ bool decode_bmp(const std::vector<uint8_t>& b, int& w, int& h, std::vector<uint8
     int32_t hh = (int32_t)le32(&b[22]);
     const int bits = le16(&b[28]);
     const uint32_t comp = le32(&b[30]);
-    if (w <= 0 || hh == 0 || !(bits == 24 || bits == 32) || !(comp == 0 || (comp == 3 && bits == 32)))
+    if (w <= 0 || hh == 0 || hh == INT32_MIN || !(bits == 24 || bits == 32) || !(comp == 0 || (comp == 3 && bits == 32)))
         return err = "unsupported BMP format", false;
     h = hh < 0 ? -hh : hh;
+    if ((int64_t)w * h > (1ll << 28)) return err = "bad BMP size", false;
     const size_t bpp = bits / 8, row = ((size_t)w * bpp + 3) & ~(size_t)3;
-    if (off + row * h > b.size()) return err = "truncated BMP data", false;
+    /* w * h <= 2^28: row * h is below 2^31 * 4 and cannot wrap */
+    if (off > b.size() || row * (size_t)h > b.size() - off) return err = "truncated BMP data", false;
     rgb.resize((size_t)3 * w * h);
     for (int y = 0; y < h; ++y) {
         const uint8_t* src = &b[off + row * (hh < 0 ? y : h - 1 - y)];
@@ -405,16 +416,20 @@ bool rt_read_image8(const std::string& path, int& w, int& h, std::vector<float>&
     if (!read_file(path, b)) return err = "cannot open " + path, false;
     std::vector<uint8_t> px;
     bool ok;
-    if (b.size() >= 8 && b[0] == 137 && b[1] == 'P' && b[2] == 'N' && b[3] == 'G')
-        ok = decode_png(b, w, h, px, err);
-    else if (b.size() >= 3 && b[0] == 'P' && (b[1] == '6' || b[1] == '5'))
-        ok = decode_pnm(b, w, h, px, err);
-    else if (b.size() >= 2 && b[0] == 'B' && b[1] == 'M')
-        ok = decode_bmp(b, w, h, px, err);
-    else
-        return err = "unrecognised image format: " + path, false;
-    if (!ok) return false;
-    rgb.resize(px.size());
+    try {  /* no exception (bad_alloc of a large image) may leave through the C ABI */
+        if (b.size() >= 8 && b[0] == 137 && b[1] == 'P' && b[2] == 'N' && b[3] == 'G')
+            ok = decode_png(b, w, h, px, err);
+        else if (b.size() >= 3 && b[0] == 'P' && (b[1] == '6' || b[1] == '5'))
+            ok = decode_pnm(b, w, h, px, err);
+        else if (b.size() >= 2 && b[0] == 'B' && b[1] == 'M')
+            ok = decode_bmp(b, w, h, px, err);
+        else
+            return err = "unrecognised image format: " + path, false;
+        if (!ok) return false;
+        rgb.resize(px.size());
+    } catch (const std::exception& e) {
+        return err = std::string("cannot decode ") + path + ": " + e.what(), false;
+    }
     for (size_t i = 0; i < px.size(); ++i) rgb[i] = (float)(px[i] * (1.0 / 255.0));
     return true;
 }

@@ -77,6 +77,7 @@ struct rt_ctx {
     int timing = 1;
     hipEvent_t ev_a = nullptr, ev_b = nullptr;
     std::vector<hipEvent_t> ev_pool;
+    std::vector<hipEvent_t> ev_flight;  /* ends of the units of work in flight (InFlight) */
     rt_stats stats{};
 };
 
@@ -159,6 +160,43 @@ float lum_rgb(const float c[3]) { return (float)(.299 * c[0] + .587 * c[1] + .11
 
 int grid_for(rt_ctx* c) { return c->n_cu * std::max(1, c->cfg.blocks_per_cu); }
 
+/* Bounds the units of work (render passes, Markov-chain mutations) queued on
+ * the context's stream ahead of the GPU: after each unit an event is
+ * recorded, and before recording the unit `depth` places later the host
+ * waits for it.  The GPU always has the next units queued, and a stop
+ * request (rt_request_stop from another thread) is seen within about
+ * `depth` units instead of after everything was queued. */
+struct InFlight {
+    rt_ctx* c;
+    int depth;
+    size_t n = 0;
+    InFlight(rt_ctx* ctx, int d) : c(ctx), depth(d) {}
+    hipError_t unit_done()
+    {
+        while ((int)c->ev_flight.size() < depth) {
+            hipEvent_t e;
+            hipError_t r = hipEventCreateWithFlags(&e, hipEventDisableTiming);
+            if (r != hipSuccess) return r;
+            c->ev_flight.push_back(e);
+        }
+        hipEvent_t slot = c->ev_flight[n % (size_t)depth];
+        if (n >= (size_t)depth) {
+            hipError_t r = hipEventSynchronize(slot);  /* the unit `depth` before this one */
+            if (r != hipSuccess) return r;
+        }
+        ++n;
+        return hipEventRecord(slot, c->stream);
+    }
+};
+
+/* A stop request seen between units: wait for the queued work, so that no
+ * kernel writes into the caller's memory after the call returns. */
+int cancel(rt_ctx* c)
+{
+    hipStreamSynchronize(c->stream);
+    return set_err(c, RT_ERR_CANCELLED, "render cancelled");
+}
+
 /* The overflow part of the traversal stacks for a launch of `grid` blocks:
  * RT_STACK_OVF int32 entries per lane.  Grows, never shrinks. */
 int ensure_stack_overflow(rt_ctx* c, int grid)
@@ -236,6 +274,7 @@ int rt_destroy(rt_ctx* c)
     for (DevBuf* b : all) release(*b);
     for (auto& b : c->ps_bufs) release(b);
     for (auto e : c->ev_pool) hipEventDestroy(e);
+    for (auto e : c->ev_flight) hipEventDestroy(e);
     if (c->ev_a) hipEventDestroy(c->ev_a);
     if (c->ev_b) hipEventDestroy(c->ev_b);
     hipStreamDestroy(c->stream);
@@ -1318,12 +1357,14 @@ static int render_impl(rt_ctx* c, const rt_view* views, int32_t nv, const rt_par
         if (gr) hipGraphDestroy(gr);
         if (e != hipSuccess) return set_err(c, RT_ERR_HIP, std::string("pass graph: ") + hipGetErrorString(e));
     }
+    InFlight flight(c, 2);
     for (size_t pi = 0; pi < passes.size(); ++pi) {
         if (c->stop) {
             cancelled = true;
             break;
         }
         hipError_t e = graph ? hipGraphLaunch(gexec, c->stream) : enqueue_pass();
+        if (e == hipSuccess) e = flight.unit_done();
         if (e != hipSuccess) {
             if (gexec) hipGraphExecDestroy(gexec);
             return set_err(c, RT_ERR_HIP, std::string("render pass: ") + hipGetErrorString(e));
@@ -1477,6 +1518,7 @@ int run_metropolis(rt_ctx* c, const rt_mlt_params& p, int W, int H, int64_t cap_
     if (int r = ensure(c, c->mlt_seed, (size_t)nb * sizeof(float4))) return r;
     MltDev m = mlt_dev(c, cap, G, W, H, p);
     const int grid = grid_for(c);
+    InFlight flight(c, 4);
     float4* bval = (float4*)c->mlt_seed.p;
     /* bootstrap: full-length paths (MetropolisSample's default pathLength =
      * MAX_DEPTH, src/metropolissample.h:35), no roulette compensation */
@@ -1508,10 +1550,11 @@ int run_metropolis(rt_ctx* c, const rt_mlt_params& p, int W, int H, int64_t cap_
         m.chain0 = (uint32_t)c0;
         MLT_TRY(launch_mlt_start(m, (const double*)c->mlt_cdf.p, nb, acc, bval, n, grid, c->stream));
         for (int64_t it = 0; it < iters; ++it) {
-            if (c->stop) return set_err(c, RT_ERR_CANCELLED, "render cancelled");
+            if (c->stop) return cancel(c);
             MLT_TRY(launch_mlt_propose(m, (uint32_t)it, n, grid, c->stream));
             MLT_TRY(mlt_trace(c, n, cap, p.max_depth, p.rr_start, p.termination_prob, p.seed, 0));
             MLT_TRY(launch_mlt_accept(m, c->ps, (uint32_t)it, n, grid, c->stream));
+            MLT_TRY(flight.unit_done());
         }
     }
     res.mutations = C * iters;
@@ -1532,6 +1575,7 @@ int run_erpt(rt_ctx* c, const rt_mlt_params& p, int W, int H, int64_t cap_slots,
     if (int r = ensure(c, c->mlt_nch, (size_t)n0 * sizeof(int32_t))) return r;
     MltDev m = mlt_dev(c, seed_cap, G, W, H, p);
     const int grid = grid_for(c);
+    InFlight flight(c, 4);
     float4* vals = (float4*)c->mlt_seed.p;
     /* e_d: one path through each pixel corner (computeEd) */
     for (int64_t base = 0; base < area; base += seed_cap) {
@@ -1590,10 +1634,11 @@ int run_erpt(rt_ctx* c, const rt_mlt_params& p, int W, int H, int64_t cap_slots,
         m.chain0 = (uint32_t)c0;
         MLT_TRY(launch_erpt_start(m, (const int32_t*)c->mlt_src.p + c0, nch, vals, ed, n, grid, c->stream));
         for (int it = 0; it < M; ++it) {
-            if (c->stop) return set_err(c, RT_ERR_CANCELLED, "render cancelled");
+            if (c->stop) return cancel(c);
             MLT_TRY(launch_mlt_propose(m, (uint32_t)it, n, grid, c->stream));
             MLT_TRY(mlt_trace(c, n, stride, p.max_depth, p.rr_start, 0.f, p.seed, 0));
             MLT_TRY(launch_mlt_accept(m, c->ps, (uint32_t)it, n, grid, c->stream));
+            MLT_TRY(flight.unit_done());
         }
     }
     res.mutations = C * M;
@@ -1643,6 +1688,7 @@ int run_filtered(rt_ctx* c, const rt_mlt_params& p, int W, int H, int64_t cap_sl
     unsigned long long* aq = aw + 2 * area;                        /* [film][pixel][3] */
     unsigned long long* ax = (unsigned long long*)c->mlt_acc.p;
     const int grid = grid_for(c);
+    InFlight flight(c, 4);
     MLT_TRY(hipMemsetAsync(im, 0, (size_t)(n3 * 6 + area * 3) * 4, c->stream));  /* films, stats, sumimp */
     float4* bval = (float4*)c->mlt_seed.p;
     float4* bw = bval + nb;
@@ -1656,7 +1702,7 @@ int run_filtered(rt_ctx* c, const rt_mlt_params& p, int W, int H, int64_t cap_sl
     std::vector<float4> hv((size_t)nb);
     int64_t mutations = 0;
     for (int pass = 0; pass < P; ++pass) {
-        if (c->stop) return set_err(c, RT_ERR_CANCELLED, "render cancelled");
+        if (c->stop) return cancel(c);
         /* importance map */
         if (pass == 0) {
             std::fill(hbuf.begin(), hbuf.end(), 1.f);
@@ -1727,10 +1773,11 @@ int run_filtered(rt_ctx* c, const rt_mlt_params& p, int W, int H, int64_t cap_sl
                 m.chain0 = (uint32_t)((int64_t)pass * 2 * C + c0);
                 MLT_TRY(launch_mlt_start(m, (const double*)c->mlt_cdf.p, nb, tot, bval, nn, grid, c->stream));
                 for (int64_t it = 0; it < iters; ++it) {
-                    if (c->stop) return set_err(c, RT_ERR_CANCELLED, "render cancelled");
+                    if (c->stop) return cancel(c);
                     MLT_TRY(launch_mlt_propose(m, (uint32_t)it, nn, grid, c->stream));
                     MLT_TRY(mlt_trace(c, nn, cap, p.max_depth, p.rr_start, p.termination_prob, p.seed, 0));
                     MLT_TRY(launch_mlt_accept(m, c->ps, (uint32_t)it, nn, grid, c->stream));
+                    MLT_TRY(flight.unit_done());
                 }
             }
             for (int n = 0; n < 2; ++n)

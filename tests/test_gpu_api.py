@@ -1,5 +1,7 @@
 """Error handling, cancellation, the fault hook, the autotuner and the
 environment light through the C ABI on the GPU."""
+import time
+
 import numpy as np
 import pytest
 
@@ -32,6 +34,70 @@ def test_cancel_between_passes(gpu_ctx):
     finally:
         gpu_ctx.request_stop(False)
     assert gpu_ctx.render([(sc.camera, 1, 1)], mode=MODE_UNBIASED)[0].mean() > 0
+
+
+def _render_in_thread(fn):
+    import threading
+    out = {}
+
+    def run():
+        t0 = time.perf_counter()
+        try:
+            fn()
+            out["status"] = 0
+        except RtError as e:
+            out["status"] = e.status
+        out["seconds"] = time.perf_counter() - t0
+
+    th = threading.Thread(target=run)
+    th.start()
+    return th, out
+
+
+def test_stop_from_another_thread_ends_a_render_early():
+    """rt_request_stop from a second thread while rt_render runs: passes are
+    queued at most two ahead of the GPU, so the call returns
+    RT_ERR_CANCELLED long before the render would have finished."""
+    sc = scenes.box_scene(256, 256)
+    with GpuContext(0) as ctx:
+        ctx.upload(sc.flatten())
+        ctx.set_launch_config(path_slots=1 << 20)  # 64 passes of 1M paths
+        view = [(sc.camera, 32, 32)]
+        ctx.render(view, mode=MODE_UNBIASED, seed=1)  # warm-up
+        t0 = time.perf_counter()
+        ctx.render(view, mode=MODE_UNBIASED, seed=1)
+        full = time.perf_counter() - t0
+        print(f"full render {full:.3f} s, {ctx.stats()['passes']} passes", flush=True)
+        th, out = _render_in_thread(lambda: ctx.render(view, mode=MODE_UNBIASED, seed=2))
+        time.sleep(0.2 * full)
+        ctx.request_stop(True)
+        th.join(60)
+        ctx.request_stop(False)
+        print(f"cancelled render: status {out.get('status')} after {out.get('seconds', 0):.3f} s", flush=True)
+        assert out["status"] == -5
+        assert out["seconds"] < 0.6 * full
+        assert ctx.render([(sc.camera, 1, 1)], mode=MODE_UNBIASED)[0].mean() > 0
+
+
+def test_stop_from_another_thread_ends_metropolis_early():
+    from renderer_amd import MLT_METROPOLIS
+
+    sc = scenes.box_scene(128, 128)
+    with GpuContext(0) as ctx:
+        ctx.upload(sc.flatten())
+        kw = dict(kind=MLT_METROPOLIS, mutations=256, chains=1 << 16, bootstrap=1 << 16, seed=3)
+        t0 = time.perf_counter()
+        ctx.render_mlt(sc.camera, **kw)
+        full = time.perf_counter() - t0
+        print(f"full Metropolis render {full:.3f} s", flush=True)
+        th, out = _render_in_thread(lambda: ctx.render_mlt(sc.camera, **kw))
+        time.sleep(0.2 * full)
+        ctx.request_stop(True)
+        th.join(60)
+        ctx.request_stop(False)
+        print(f"cancelled: status {out.get('status')} after {out.get('seconds', 0):.3f} s", flush=True)
+        assert out["status"] == -5
+        assert out["seconds"] < 0.6 * full
 
 
 def test_invalid_arguments_are_rejected(gpu_ctx):

@@ -62,6 +62,31 @@ def test_image_reader_rejects_what_it_cannot_read(tmp_path):
         TextureMaterial.load(tmp_path / "missing.png")
 
 
+def _bmp_header(w, h, bits=24, off=54, size=54):
+    import struct
+    return (b"BM" + struct.pack("<IHHI", size, 0, 0, off) +
+            struct.pack("<IiiHHIIiiII", 40, w, h, 1, bits, 0, 0, 2835, 2835, 0, 0))
+
+
+@pytest.mark.parametrize("name,data", [
+    ("pnm_overflowing_width", b"P6 99999999999999999999 2 255\n" + b"\0" * 64),
+    ("pnm_huge", b"P6 65536 65536 255\n" + b"\0" * 64),
+    ("pnm_truncated", b"P6 4 4 255\n" + b"\0" * 10),
+    ("pnm_header_only", b"P5 2 2 255"),
+    ("bmp_int_min_height", _bmp_header(2, -2 ** 31)),
+    ("bmp_huge", _bmp_header(1 << 20, 1 << 20)),
+    ("bmp_offset_past_end", _bmp_header(2, 2, off=1 << 30)),
+])
+def test_image_reader_rejects_crafted_headers(tmp_path, name, data):
+    """Headers whose sizes overflow an int, exceed 2^28 pixels or point past
+    the file end are read errors, not allocations or out-of-bounds reads."""
+    path = tmp_path / (name + (".bmp" if name.startswith("bmp") else ".ppm"))
+    path.write_bytes(data)
+    with pytest.raises(_abi.RtError) as e:
+        TextureMaterial.load(path)
+    assert e.value.status == _abi.ERR_IO
+
+
 def _scene(objs):
     return Scene(Camera((0, 0, 3), (0, 0, 0), (0, 1, 0), 0.6, 16, 16), [PointLight((0, 0, 3), (4, 4, 4))],
                  IntersectableList(objs))
