@@ -85,7 +85,7 @@ def test_stop_from_another_thread_ends_metropolis_early():
     sc = scenes.box_scene(128, 128)
     with GpuContext(0) as ctx:
         ctx.upload(sc.flatten())
-        kw = dict(kind=MLT_METROPOLIS, mutations=256, chains=1 << 16, bootstrap=1 << 16, seed=3)
+        kw = dict(kind=MLT_METROPOLIS, mutations=2048, chains=1 << 16, bootstrap=1 << 16, seed=3)
         t0 = time.perf_counter()
         ctx.render_mlt(sc.camera, **kw)
         full = time.perf_counter() - t0
