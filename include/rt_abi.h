@@ -180,8 +180,15 @@ int rt_guide_init(rt_ctx* ctx, uint32_t seed);
 int rt_guide_set_params(rt_ctx* ctx, const float* theta, int32_t n);
 int rt_guide_get_params(rt_ctx* ctx, float* theta, int32_t n);
 int rt_guide_set_bounds(rt_ctx* ctx, const float* lo, const float* hi);
-/* raw network outputs y[n][48] for n records (host arrays) */
+/* raw network outputs y[n][48] for n records (host arrays); outputs 41..47,
+ * which the head does not use, are returned as 0 */
 int rt_guide_forward(rt_ctx* ctx, const float* records, int32_t n, float* y);
+/* head outputs out[n][48] for n records, as the guided shading kernel
+ * receives them (stored as FP16): lobe j at [5j .. 5j+4] = (mean direction,
+ * concentration, mixture weight), [40] the probability of sampling the
+ * guide; the directions are read renormalised and the weights divided by
+ * their sum */
+int rt_guide_head(rt_ctx* ctx, const float* records, int32_t n, float* out);
 /* gradient of L = -sum_i w_i log p_mix_i / sum_i w_i over n records, and L */
 int rt_guide_gradient(rt_ctx* ctx, const float* records, int32_t n, float* grad, float* loss);
 /* `steps` Adam steps on n records with learning rate lr */

@@ -167,6 +167,7 @@ PROTOTYPES = {
     "rt_guide_get_params": (ctypes.c_int, [_P, _P, _I32]),
     "rt_guide_set_bounds": (ctypes.c_int, [_P, _P, _P]),
     "rt_guide_forward": (ctypes.c_int, [_P, _P, _I32, _P]),
+    "rt_guide_head": (ctypes.c_int, [_P, _P, _I32, _P]),
     "rt_guide_gradient": (ctypes.c_int, [_P, _P, _I32, _P, _P]),
     "rt_guide_train": (ctypes.c_int, [_P, _P, _I32, _I32, ctypes.c_float]),
     # include/rt_scene_file.h

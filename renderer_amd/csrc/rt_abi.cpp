@@ -72,7 +72,7 @@ struct rt_ctx {
     bool guide_bounds_set = false;
     int guide_step = 0;
     size_t guide_slots = 0;
-    DevBuf g_theta, g_w16, g_m, g_v, g_grad, g_acc, g_count, g_rec, g_aux, g_open, g_out, g_tmp_rec, g_tmp_y;
+    DevBuf g_theta, g_w16, g_m, g_v, g_grad, g_acc, g_count, g_rec, g_aux, g_open, g_out, g_in, g_tmp_rec, g_tmp_y;
     float scene_lo[3] = {0, 0, 0}, scene_hi[3] = {1, 1, 1};
     int timing = 1;
     hipEvent_t ev_a = nullptr, ev_b = nullptr;
@@ -267,7 +267,7 @@ int rt_destroy(rt_ctx* c)
                      &c->env_rgb, &c->env_row, &c->env_col, &c->env_p, &c->probe_rgb, &c->rb_o, &c->rb_d, &c->rb_out0,
                      &c->rb_out1, &c->stack_ovf, &c->tm_list, &c->tm_count, &c->counters, &c->views, &c->tiles, &c->passes, &c->films, &c->g_theta,
                      &c->g_w16, &c->g_m, &c->g_v, &c->g_grad, &c->g_acc, &c->g_count, &c->g_rec, &c->g_aux,
-                     &c->g_open, &c->g_out, &c->g_tmp_rec, &c->g_tmp_y, &c->obj_nodes, &c->obj_first,
+                     &c->g_open, &c->g_out, &c->g_in, &c->g_tmp_rec, &c->g_tmp_y, &c->obj_nodes, &c->obj_first,
                      &c->tex_rgb, &c->tex_info, &c->tri_tc,
                      &c->mlt_cur, &c->mlt_prop, &c->mlt_val, &c->mlt_energy, &c->mlt_acc, &c->mlt_cnt,
                      &c->mlt_seed, &c->mlt_cdf, &c->mlt_nch, &c->mlt_src, &c->mlt_view, &c->flt_img, &c->flt_acc};
@@ -960,13 +960,14 @@ int alloc_path_state(rt_ctx* c, size_t cap)
 static int guide_alloc_base(rt_ctx* c)
 {
     int r;
-    if ((r = ensure(c, c->g_theta, GD_NPARAM * 4)) || (r = ensure(c, c->g_w16, GD_NW16 * 2)) ||
+    if ((r = ensure(c, c->g_theta, GD_NPARAM * 4)) || (r = ensure(c, c->g_w16, (GD_NW16 + GD_NW16I) * 2)) ||
         (r = ensure(c, c->g_m, GD_NPARAM * 4)) || (r = ensure(c, c->g_v, GD_NPARAM * 4)) ||
         (r = ensure(c, c->g_grad, GD_NPARAM * 4)) || (r = ensure(c, c->g_acc, 16)) || (r = ensure(c, c->g_count, 32)))
         return r;
     GuideState& g = c->guide;
     g.theta = (float*)c->g_theta.p;
     g.w16 = (_Float16*)c->g_w16.p;
+    g.w16i = g.w16 + GD_NW16;
     g.adam_m = (float*)c->g_m.p;
     g.adam_v = (float*)c->g_v.p;
     g.grad = (float*)c->g_grad.p;
@@ -982,13 +983,16 @@ static int guide_alloc_slots(rt_ctx* c, size_t slots, int batch)
     int r;
     size_t cap = (size_t)GD_TRAIN_CHUNKS * std::max(batch, 1 << 14);
     if ((r = ensure(c, c->g_rec, cap * GD_REC * 4)) || (r = ensure(c, c->g_aux, cap * 16)) ||
-        (r = ensure(c, c->g_open, slots * 4)) || (r = ensure(c, c->g_out, slots * GD_PARAMS * 4)))
+        (r = ensure(c, c->g_open, slots * 4)) || (r = ensure(c, c->g_out, slots * GD_PARAMS * sizeof(_Float16))) ||
+        (r = ensure(c, c->g_in, slots * (sizeof(float4) + sizeof(float2)))))
         return r;
     GuideState& g = c->guide;
     g.rec = (float*)c->g_rec.p;
     g.rec_aux = (float4*)c->g_aux.p;
     g.open_rec = (int32_t*)c->g_open.p;
-    g.out = (float*)c->g_out.p;
+    g.out = (_Float16*)c->g_out.p;
+    g.in_a = (float4*)c->g_in.p;
+    g.in_b = (float2*)((char*)c->g_in.p + slots * sizeof(float4));
     g.rec_cap = (int32_t)std::min<size_t>(cap, c->g_rec.bytes / (GD_REC * 4));
     if (c->guide_slots < slots) {
         HIP_TRY(c, launch_guide_reset_records(g, (int)slots, c->stream));
@@ -1001,15 +1005,18 @@ static int guide_alloc_slots(rt_ctx* c, size_t slots, int batch)
 
 static int guide_upload_params(rt_ctx* c, const float* theta)
 {
-    std::vector<_Float16> w16(GD_NW16);
+    /* training tiles, then inference tiles (whose padding stays zero) */
+    std::vector<_Float16> w16(GD_NW16 + GD_NW16I, (_Float16)0.f);
     for (int i = 0; i < GD_NPARAM; ++i) {
         int fw, bw;
         gd_w16_slots(i, &fw, &bw);
         if (fw >= 0) w16[fw] = (_Float16)theta[i];
         if (bw >= 0) w16[bw] = (_Float16)theta[i];
+        const int iw = gd_w16i_slot(i);
+        if (iw >= 0) w16[GD_NW16 + iw] = (_Float16)theta[i];
     }
     HIP_TRY(c, hipMemcpy(c->guide.theta, theta, GD_NPARAM * 4, hipMemcpyHostToDevice));
-    HIP_TRY(c, hipMemcpy(c->guide.w16, w16.data(), GD_NW16 * 2, hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(c->guide.w16, w16.data(), w16.size() * 2, hipMemcpyHostToDevice));
     HIP_TRY(c, hipMemset(c->guide.adam_m, 0, GD_NPARAM * 4));
     HIP_TRY(c, hipMemset(c->guide.adam_v, 0, GD_NPARAM * 4));
     HIP_TRY(c, hipMemset(c->guide.grad, 0, GD_NPARAM * 4));
@@ -1098,10 +1105,30 @@ int rt_guide_forward(rt_ctx* c, const float* rec, int32_t n, float* y)
     hipSetDevice(c->device);
     int r;
     if ((r = guide_records_to_device(c, rec, n)) || (r = ensure(c, c->g_tmp_y, (size_t)n * GD_OUT * 4))) return r;
-    int grid = std::max(1, std::min(grid_for(c), (n + 63) / 64));
+    int grid = std::max(1, std::min(c->n_cu * GD_INFER_BLOCKS_PER_CU, (n + 255) / 256));
+    HIP_TRY(c, hipMemsetAsync(c->g_tmp_y.p, 0, (size_t)n * GD_OUT * 4, c->stream));
     HIP_TRY(c, launch_guide_forward(c->guide, (const float*)c->g_tmp_rec.p, n, (float*)c->g_tmp_y.p, grid, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     HIP_TRY(c, hipMemcpy(y, c->g_tmp_y.p, (size_t)n * GD_OUT * 4, hipMemcpyDeviceToHost));
+    return RT_OK;
+}
+
+int rt_guide_head(rt_ctx* c, const float* rec, int32_t n, float* out)
+{
+    if (!c || n < 0 || (n && (!rec || !out))) return RT_ERR_INVALID;
+    if (int r = check_fault(c)) return r;
+    if (!c->guide_ready) return set_err(c, RT_ERR_NO_GUIDE, "no guiding network");
+    if (n == 0) return RT_OK;
+    hipSetDevice(c->device);
+    int r;
+    if ((r = guide_records_to_device(c, rec, n)) || (r = ensure(c, c->g_tmp_y, (size_t)n * GD_PARAMS * 2))) return r;
+    int grid = std::max(1, std::min(c->n_cu * GD_INFER_BLOCKS_PER_CU, (n + 255) / 256));
+    HIP_TRY(c, hipMemsetAsync(c->g_tmp_y.p, 0, (size_t)n * GD_PARAMS * 2, c->stream));
+    HIP_TRY(c, launch_guide_head(c->guide, (const float*)c->g_tmp_rec.p, n, (_Float16*)c->g_tmp_y.p, grid, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    std::vector<_Float16> h((size_t)n * GD_PARAMS);
+    HIP_TRY(c, hipMemcpy(h.data(), c->g_tmp_y.p, h.size() * 2, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < h.size(); ++i) out[i] = (float)h[i];
     return RT_OK;
 }
 

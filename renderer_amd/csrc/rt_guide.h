@@ -6,7 +6,7 @@
  * -> 48 outputs = 8 von Mises-Fisher lobes (mean direction, log
  * concentration, mixture logit) + the logit of the probability of sampling
  * the guide instead of the BSDF.  FP16 weights and activations, FP32
- * accumulation (v_mfma_f32_16x16x16_f16), FP32 master weights and Adam.
+ * accumulation, FP32 master weights and Adam.
  *
  * Flat parameter layout (FP32 master, 16,624 values):
  *   W1[64][80] b1[64] W2[64][64] b2[64] W3[64][64] b3[64] W4[48][64] b4[48]
@@ -15,6 +15,19 @@
  * conflicts): W1..W4 for the forward pass (tile (ot, kb), lane m + 16 q
  * holds W[16 ot + m][16 kb + 4 q .. +3]), then W2..W4 transposed for the
  * backward pass (tile (it, kb), lane m + 16 q holds W[16 kb + 4 q .. +3][16 it + m]).
+ * These 16x16x16 tiles serve training (v_mfma_f32_16x16x16_f16).
+ *
+ * Inference copy (17,408 values, v_mfma_f32_16x16x32_f16 A operands of
+ * 16 rows x 32 inputs, 64 lanes x 8 halves per tile, tiles (ot, kb)):
+ * lane m + 16 q, slot j holds W[row(ot, m)][32 kb + col(q, j)].
+ *   W1 (inputs padded to 96): col = 8 q + j, the feature order itself.
+ *   W2..W4: col = 4 q + j for j < 4, 16 + 4 q + (j - 4) for j >= 4, the
+ *   order in which a lane holds the previous layer's outputs (its D tiles
+ *   2 kb and 2 kb + 1), so a layer's result is the next one's B operand.
+ *   W1..W3: row = 16 ot + m.  W4: row m of tile ot is the output that lane
+ *   group m / 4 keeps in slot 4 ot + m % 4: slots 0..9 are lobes 2 g and
+ *   2 g + 1 of lane group g (outputs 10 g .. 10 g + 9), slot 10 of group 0
+ *   is the guide probability logit (output 40); the other slots are zero.
  */
 #ifndef RT_GUIDE_H
 #define RT_GUIDE_H
@@ -44,14 +57,19 @@
 #define GD_W16T_W2 16384
 #define GD_W16T_W3 20480
 #define GD_W16T_W4 24576
+#define GD_NW16I 17408   /* inference tiles (16x16x32 layout) */
+#define GD_W16I_W1 0
+#define GD_W16I_W2 6144
+#define GD_W16I_W3 10240
+#define GD_W16I_W4 14336
 #define GD_KLO -3.0f
 #define GD_KHI 10.0f
 #define GD_REC 16        /* floats per training record */
-#define GD_PARAMS 48     /* floats of head output per path */
+#define GD_PARAMS 48     /* halves of head output per path vertex */
 #define GD_WCLAMP 100.0f /* cap of a normalised sample weight */
 #define GD_TRAIN_CHUNKS 4 /* training steps per render pass (record capacity = 4 batches) */
 /* Persistent grids of the network kernels: the blocks one CU holds at once
- * (LDS: 46 KB per inference block, 138 KB per training block).  More blocks
+ * (LDS: 45 KB per inference block, 138 KB per training block).  More blocks
  * would only reload the weights into LDS and, for training, add one round
  * of gradient atomics per block. */
 #define GD_INFER_BLOCKS_PER_CU 3
@@ -65,7 +83,8 @@
  * closed (rgb) and the path slot (int bits). */
 struct GuideState {
     float* theta;       /* FP32 master parameters */
-    _Float16* w16;      /* FP16 weights */
+    _Float16* w16;      /* FP16 weights, training tiles */
+    _Float16* w16i;     /* FP16 weights, inference tiles */
     float* adam_m;
     float* adam_v;
     float* grad;
@@ -76,7 +95,9 @@ struct GuideState {
     float* rec;         /* capacity x GD_REC */
     float4* rec_aux;    /* capacity */
     int32_t* open_rec;  /* per path slot: index of its open record or -1 */
-    float* out;         /* per path slot: GD_PARAMS head outputs */
+    _Float16* out;      /* per queue position of the bounce: GD_PARAMS head outputs */
+    float4* in_a;       /* per queue position: position in [0,1]^3, normal.xy (FP16 pair) (k_guide_gather) */
+    float2* in_b;       /* per queue position: (normal.z, wo.x), (wo.y, wo.z) as FP16 pairs */
     int32_t rec_cap;
     float lo[3], inv_ext[3];
 };
@@ -99,11 +120,57 @@ __host__ __device__ inline void gd_w16_slots(int i, int* fwd, int* bwd)
         *bwd = bb + ((((c >> 4) * (O >> 4) + (o >> 4)) * 64 + (c & 15) + 16 * ((o & 15) >> 2)) << 2) + (o & 3);
 }
 
+/* Output of W4 that inference lane group g keeps in slot s (0..11), or -1. */
+__host__ __device__ inline int gd_head_output(int g, int s)
+{
+    return s < 10 ? 10 * g + s : (s == 10 && g == 0 ? 40 : -1);
+}
+
+/* Position of master parameter i in the inference copy (-1: biases, and the
+ * unused outputs 41..47 of W4). */
+__host__ __device__ inline int gd_w16i_slot(int i)
+{
+    int off, I, base, KB, o, c;
+    if (i < GD_OFF_B1) { off = GD_OFF_W1; I = 80; base = GD_W16I_W1; KB = 3; }
+    else if (i >= GD_OFF_W2 && i < GD_OFF_B2) { off = GD_OFF_W2; I = 64; base = GD_W16I_W2; KB = 2; }
+    else if (i >= GD_OFF_W3 && i < GD_OFF_B3) { off = GD_OFF_W3; I = 64; base = GD_W16I_W3; KB = 2; }
+    else if (i >= GD_OFF_W4 && i < GD_OFF_B4) { off = GD_OFF_W4; I = 64; base = GD_W16I_W4; KB = 2; }
+    else return -1;
+    o = (i - off) / I;
+    c = (i - off) % I;
+    int ot, m;
+    if (off == GD_OFF_W4) {
+        int g, s;
+        if (o < 40) { g = o / 10; s = o % 10; }
+        else if (o == 40) { g = 0; s = 10; }
+        else return -1;
+        ot = s >> 2;
+        m = 4 * g + (s & 3);
+    } else {
+        ot = o >> 4;
+        m = o & 15;
+    }
+    const int kb = c >> 5, r = c & 31;
+    int q, j;
+    if (off == GD_OFF_W1) {
+        q = r >> 3;
+        j = r & 7;
+    } else if (r < 16) {
+        q = r >> 2;
+        j = r & 3;
+    } else {
+        q = (r - 16) >> 2;
+        j = 4 + ((r - 16) & 3);
+    }
+    return base + (((ot * KB + kb) * 64 + m + 16 * q) << 3) + j;
+}
+
 struct GuideHost;  /* forward declaration for the ABI layer */
 
 hipError_t launch_guide_infer(const struct HostDevScene& s, const struct PathState& ps, const GuideState& g,
                               int qsel, int grid, hipStream_t st);
 hipError_t launch_guide_forward(const GuideState& g, const float* rec, int n, float* y, int grid, hipStream_t st);
+hipError_t launch_guide_head(const GuideState& g, const float* rec, int n, _Float16* out, int grid, hipStream_t st);
 /* Training works on chunks of records [off, off + max_n) of the stored ones. */
 hipError_t launch_guide_weights(const GuideState& g, const struct PathState& ps, int off, int max_n, int grid,
                                 hipStream_t st);

@@ -1,15 +1,20 @@
 /*
  * rt_guide.hip — gfx950 kernels of the path-guiding network.
  *
- * Matrix products use v_mfma_f32_16x16x16_f16 in the transposed form
- * Z[out x samples] = W[out x in] * X[in x samples]: a wave carries 16
- * samples; lane l holds sample l % 16 and the feature quad 4 (l / 16)..+3.
- * With that orientation the D tile of one layer (4 consecutive outputs of
- * one sample per lane) is exactly the B operand of the next layer's K block,
- * so activations never leave registers between layers.  Weight gradients
- * (a reduction over samples) read activations and deltas back from LDS in
- * [neuron][sample] order, which gives both MFMA operands as contiguous
- * 8-byte reads.
+ * Inference (k_guide_net) uses v_mfma_f32_16x16x32_f16 in the transposed
+ * form Z[out x samples] = W[out x in] * X[in x samples].  A wave carries 64
+ * samples as four tiles of 16, so every weight tile read from LDS feeds four
+ * MFMAs.  Lane l holds sample l % 16 of a tile; the D tile of a layer gives
+ * it outputs 4 (l / 16) .. +3, and two D tiles are exactly the lane's B
+ * operand of the next layer once the next layer's weight columns are stored
+ * in that order (rt_guide.h), so activations never leave registers.  The
+ * last layer's rows are ordered so that each lane ends with two whole
+ * lobes, and the head is evaluated in registers too.
+ *
+ * Training (k_guide_train) uses v_mfma_f32_16x16x16_f16 on 16 samples per
+ * wave.  Weight gradients (a reduction over samples) read activations and
+ * deltas back from LDS in [neuron][sample] order, which gives both MFMA
+ * operands as contiguous 8-byte reads.
  */
 #include "rt_device.h"
 #include "rt_guide.h"
@@ -18,7 +23,10 @@
 
 #include <cstring>
 
+typedef _Float16 h2 __attribute__((ext_vector_type(2)));
 typedef _Float16 h4 __attribute__((ext_vector_type(4)));
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+typedef float f2 __attribute__((ext_vector_type(2)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
 #define GB 256  /* threads per block: 4 waves */
@@ -33,6 +41,10 @@ static inline DevScene as_dev_g(const HostDevScene& s)
 __device__ __forceinline__ f4 mfma16(h4 a, h4 b, f4 c)
 {
     return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ f4 mfma32(h8 a, h8 b, f4 c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
 }
 
 /* FP16 weights (the first NW values of the tiled copy) and FP32 biases in LDS */
@@ -172,6 +184,15 @@ __device__ __forceinline__ void lobe_head(const float* r, vec3& mu, float& mlen,
     kappa = expf(fminf(fmaxf(r[3], GD_KLO), GD_KHI));
 }
 
+/* The same head with the hardware's reciprocal square root and exponent:
+ * its outputs are stored as FP16, far coarser than these approximations. */
+__device__ __forceinline__ void lobe_head_fast(const float* r, vec3& mu, float& kappa)
+{
+    const vec3 m = mk(r[0], r[1], r[2]);
+    mu = m * __builtin_amdgcn_rsqf(fmaxf(dot(m, m), 1e-12f));
+    kappa = __expf(fminf(fmaxf(r[3], GD_KLO), GD_KHI));
+}
+
 __device__ __forceinline__ float quad_max(float v)
 {
     v = fmaxf(v, __shfl_xor(v, 16));
@@ -184,95 +205,338 @@ __device__ __forceinline__ float quad_sum(float v)
 }
 
 /* ===========================================================================
- * inference: head outputs for every live path at a surface hit
+ * Inference: the network on 64 samples per wave
  * ========================================================================= */
-__global__ __launch_bounds__(GB) void k_guide_infer(DevScene S, PathState ps, GuideState g, int qsel)
+#define GD_INS 9  /* inputs staged per sample: position in [0,1]^3, normal, outgoing direction */
+/* waves per SIMD the network kernel is compiled for (0: the compiler's choice) */
+#ifndef RT_GUIDE_NET_WAVES
+#define RT_GUIDE_NET_WAVES 3
+#endif
+#if RT_GUIDE_NET_WAVES > 0
+#define RT_GUIDE_NET_OCCUPANCY __attribute__((amdgpu_waves_per_eu(RT_GUIDE_NET_WAVES, 8)))
+#else
+#define RT_GUIDE_NET_OCCUPANCY
+#endif
+
+/* FP16 inference tiles, FP32 biases in D-tile order (b4 in the row order of
+ * the inference W4), and per wave the staged inputs of its 64 samples (an
+ * odd stride keeps the 16 samples a tile reads on distinct banks). */
+struct alignas(16) InferLds {
+    _Float16 w[GD_NW16I];
+    float b[4 * GD_H];
+    float in[GB / 64][64][GD_INS];
+};
+
+__device__ __forceinline__ void load_infer_weights(InferLds& L, const GuideState& g)
 {
-    __shared__ WLds<GD_NW16F> L;
-    __shared__ float Y[4][GD_OUT][16];
-    load_weights(L, g);
-    const int lane = __lane_id(), n = lane & 15, q = lane >> 4, w = threadIdx.x >> 6;
+    const uint4* src = reinterpret_cast<const uint4*>(g.w16i);
+    uint4* dst = reinterpret_cast<uint4*>(L.w);
+    for (int i = threadIdx.x; i < GD_NW16I / 8; i += blockDim.x) dst[i] = src[i];
+    for (int i = threadIdx.x; i < 4 * GD_H; i += blockDim.x) {
+        float v = 0.f;
+        if (i < 3 * GD_H) {
+            v = g.theta[(i < 64 ? GD_OFF_B1 : i < 128 ? GD_OFF_B2 - 64 : GD_OFF_B3 - 128) + i];
+        } else if (i < 3 * GD_H + GD_OUT) {
+            const int r = i - 3 * GD_H, ot = r >> 4, m = r & 15;
+            const int o = gd_head_output(m >> 2, 4 * ot + (m & 3));
+            if (o >= 0) v = g.theta[GD_OFF_B4 + o];
+        }
+        L.b[i] = v;
+    }
+    __syncthreads();
+}
+
+/* LDS written by some lanes of a wave and read by others: a wave's LDS
+ * operations execute in order, so only the compiler must not move them. */
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+/* sin and cos of 2 pi x for x in [0, 1) (the hardware's sine and cosine
+ * take revolutions) */
+__device__ __forceinline__ void sincos_rev(float x, float& s, float& c)
+{
+    s = __builtin_amdgcn_sinf(x);
+    c = __builtin_amdgcn_cosf(x);
+}
+
+/* Eight features of one coordinate p in [0, 1]: (sin, cos) of 2^k pi p for
+ * k = k0 .. k0 + 3.  The base angle is reduced exactly (p 2^(k0-1) has at
+ * most 7 integer bits, so its fractional part is exact in FP32), evaluated
+ * in revolutions, then doubled three times (sin 2a = 2 sin a cos a,
+ * cos 2a = (cos a - sin a)(cos a + sin a)). */
+__device__ __forceinline__ h8 freq_group(float p, int k0)
+{
+    const float x = ldexpf(p, k0 - 1);
+    float sn, cs;
+    sincos_rev(x - floorf(x), sn, cs);
+    h8 r;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        r[2 * j] = (_Float16)sn;
+        r[2 * j + 1] = (_Float16)cs;
+        const float s2 = 2.f * sn * cs, c2 = (cs - sn) * (cs + sn);
+        sn = s2;
+        cs = c2;
+    }
+    return r;
+}
+
+/* Features 32 kb + 8 q .. +7 of one sample, i.e. group g = 4 kb + q of the
+ * encoding: g < 9 is coordinate g / 3 at frequencies 4 (g % 3) .. +3, g = 9
+ * the normal and outgoing direction, g = 10, 11 the zero padding.  v: the
+ * sample's staged inputs. */
+__device__ __forceinline__ h8 feature_group(int kb, int q, const float* v)
+{
+    if (kb == 0) return freq_group(q < 3 ? v[0] : v[1], 4 * (q < 3 ? q : 0));
+    if (kb == 1) return freq_group(q < 2 ? v[1] : v[2], 4 * ((q + 1) % 3));
+    if (q == 0) return freq_group(v[2], 8);
+    h8 r = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (q == 1)
+#pragma unroll
+        for (int j = 0; j < 6; ++j) r[j] = (_Float16)v[3 + j];
+    return r;
+}
+
+/* bias of D tile ot in every sample tile */
+template <int OT>
+__device__ __forceinline__ void bias_init(const float* b, f4 (&out)[4][OT])
+{
+    const int q = __lane_id() >> 4;
+#pragma unroll
+    for (int ot = 0; ot < OT; ++ot) {
+        const f4 bias = {b[16 * ot + 4 * q], b[16 * ot + 4 * q + 1], b[16 * ot + 4 * q + 2], b[16 * ot + 4 * q + 3]};
+#pragma unroll
+        for (int st = 0; st < 4; ++st) out[st][ot] = bias;
+    }
+}
+
+/* out[st][ot] += W tile (ot, kb) x in[st] for every output tile: one weight
+ * read from LDS per tile, four MFMAs on it */
+template <int KB, int OT>
+__device__ __forceinline__ void dense32_k(const _Float16* W, int kb, const h8 (&in)[4], f4 (&out)[4][OT])
+{
+    const int lane = __lane_id();
+#pragma unroll
+    for (int ot = 0; ot < OT; ++ot) {
+        const h8 a = *reinterpret_cast<const h8*>(W + ((((ot * KB + kb) << 6) + lane) << 3));
+#pragma unroll
+        for (int st = 0; st < 4; ++st) out[st][ot] = mfma32(a, in[st], out[st][ot]);
+    }
+}
+
+/* out = W x in + b for a hidden or output layer whose input is two B operands */
+template <int OT>
+__device__ __forceinline__ void dense32(const _Float16* W, const float* b, const h8 (&in)[4][2], f4 (&out)[4][OT])
+{
+    bias_init<OT>(b, out);
+#pragma unroll
+    for (int kb = 0; kb < 2; ++kb) {
+        const h8 x[4] = {in[0][kb], in[1][kb], in[2][kb], in[3][kb]};
+        dense32_k<2, OT>(W, kb, x, out);
+    }
+}
+
+/* ReLU of four D tiles into the two B operands of the next layer: pairs
+ * converted to FP16 first, then clamped at zero as pairs (rounding keeps the
+ * sign, so this is the ReLU of the FP32 value, rounded) */
+__device__ __forceinline__ void relu_pack(const f4 (&z)[4][4], h8 (&h)[4][2])
+{
+    const h2 zero = {(_Float16)0.f, (_Float16)0.f};
+#pragma unroll
+    for (int st = 0; st < 4; ++st)
+#pragma unroll
+        for (int kb = 0; kb < 2; ++kb)
+#pragma unroll
+            for (int j = 0; j < 8; j += 2) {
+                const f4& src = z[st][2 * kb + (j >> 2)];
+                const f2 pair = {src[j & 3], src[(j & 3) + 1]};
+                h2 v = __builtin_convertvector(pair, h2);
+                v = __builtin_elementwise_max(v, zero);
+                h[st][kb][j] = v[0];
+                h[st][kb][j + 1] = v[1];
+            }
+}
+
+/* The network's inputs of every path vertex of the bounce, by queue
+ * position: position in the scene box (FP32, the features need its full
+ * precision), viewer-side normal and outgoing direction (FP16, as the
+ * network reads them), 24 bytes.  A separate kernel with few registers and
+ * many waves hides the latency of the dependent loads (queue -> slot ->
+ * hit -> shading normal), and the network kernel then reads its inputs
+ * coalesced. */
+__global__ __launch_bounds__(GB) void k_guide_gather(DevScene S, PathState ps, GuideState g, int qsel)
+{
     const int cnt = ps.cnt->q_count[qsel];
-    if (blockIdx.x == 0 && threadIdx.x == 0 && cnt > 0) atomicAdd(g.inferred, (unsigned long long)cnt);
-    for (int base = blockIdx.x * 64; base < cnt; base += gridDim.x * 64) {
-        int i = base + 16 * w + n;
-        bool valid = i < cnt;
-        int p = valid ? ps.queue[qsel][i] : 0;
-        float p01[3] = {0.f, 0.f, 0.f};
-        vec3 nf = mk(0.f, 0.f, 0.f), wo = mk(0.f, 0.f, 0.f);
-        if (valid) {
-            float4 h = ps.hit[p];
-            int gg = __float_as_int(h.y);
-            if (gg >= 0) {
-                vec3 o = xyz(ps.ray_o[p]), d = xyz(ps.ray_d[p]);
-                vec3 x = o + h.x * d;
-                vec3 nn = hit_normal(S, gg, x, h.z, h.w);
-                nf = dot(nn, d) > 0.f ? -nn : nn;
-                wo = -d;
-                p01[0] = (x.x - g.lo[0]) * g.inv_ext[0];
-                p01[1] = (x.y - g.lo[1]) * g.inv_ext[1];
-                p01[2] = (x.z - g.lo[2]) * g.inv_ext[2];
+    for (int i = blockIdx.x * GB + threadIdx.x; i < cnt; i += gridDim.x * GB) {
+        const int p = ps.queue[qsel][i];
+        const float4 h = ps.hit[p];
+        const int gg = __float_as_int(h.y);
+        float4 a = make_float4(0.f, 0.f, 0.f, 0.f);
+        float2 b = make_float2(0.f, 0.f);
+        if (gg >= 0) {
+            const vec3 o = xyz(ps.ray_o[p]), d = xyz(ps.ray_d[p]);
+            const vec3 x = o + h.x * d;
+            const vec3 nn = hit_normal(S, gg, x, h.z, h.w);
+            const vec3 nf = dot(nn, d) > 0.f ? -nn : nn;
+            const h2 n01 = {(_Float16)nf.x, (_Float16)nf.y}, n2w0 = {(_Float16)nf.z, (_Float16)-d.x},
+                     w12 = {(_Float16)-d.y, (_Float16)-d.z};
+            a = make_float4((x.x - g.lo[0]) * g.inv_ext[0], (x.y - g.lo[1]) * g.inv_ext[1],
+                            (x.z - g.lo[2]) * g.inv_ext[2], __builtin_bit_cast(float, n01));
+            b = make_float2(__builtin_bit_cast(float, n2w0), __builtin_bit_cast(float, w12));
+        }
+        g.in_a[i] = a;
+        g.in_b[i] = b;
+    }
+}
+
+__device__ __forceinline__ void unpack_inputs(float4 a, float2 b, float* v)
+{
+    const h2 n01 = __builtin_bit_cast(h2, a.w), n2w0 = __builtin_bit_cast(h2, b.x), w12 = __builtin_bit_cast(h2, b.y);
+    v[0] = a.x, v[1] = a.y, v[2] = a.z;
+    v[3] = (float)n01[0], v[4] = (float)n01[1], v[5] = (float)n2w0[0];
+    v[6] = (float)n2w0[1], v[7] = (float)w12[0], v[8] = (float)w12[1];
+}
+
+/* Head outputs of the path vertex (or record) at queue position i: 48
+ * halves, lobe j at [5j .. 5j+4] = (mu.xyz, kappa, weight), [40] the
+ * probability of sampling the guide.  RAW: the 48 raw network outputs as
+ * floats instead (records only, the parity surface).  Path vertices take
+ * their inputs from k_guide_gather's compact records, the next chunk's read
+ * while this one is computed. */
+template <bool FROM_REC, bool RAW>
+__global__ __launch_bounds__(GB) RT_GUIDE_NET_OCCUPANCY void k_guide_net(PathState ps, GuideState g, int qsel,
+                                                                        const float* rec, int nrec, _Float16* hout,
+                                                                        float* yout)
+{
+    __shared__ InferLds L;
+    load_infer_weights(L, g);
+    const int lane = __lane_id(), n = lane & 15, q = lane >> 4, w = threadIdx.x >> 6;
+    const int cnt = FROM_REC ? nrec : ps.cnt->q_count[qsel];
+    if (!FROM_REC && blockIdx.x == 0 && threadIdx.x == 0 && cnt > 0) atomicAdd(g.inferred, (unsigned long long)cnt);
+    float (*IN)[GD_INS] = L.in[w];
+    const int stride = gridDim.x * (GB / 64) * 64;
+    int base = (blockIdx.x * (GB / 64) + w) * 64;
+    float4 na = make_float4(0.f, 0.f, 0.f, 0.f);
+    float2 nb = make_float2(0.f, 0.f);
+    if (!FROM_REC && base + lane < cnt) {
+        na = g.in_a[base + lane];
+        nb = g.in_b[base + lane];
+    }
+    for (; base < cnt; base += stride) {
+        /* stage: lane l holds sample base + l */
+        {
+            const int i = base + lane;
+            float v[GD_INS];
+#pragma unroll
+            for (int c = 0; c < GD_INS; ++c) v[c] = 0.f;
+            if (FROM_REC) {
+                if (i < cnt) {
+                    const float* r = rec + (size_t)i * GD_REC;
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) v[k] = (r[k] - g.lo[k]) * g.inv_ext[k];
+#pragma unroll
+                    for (int k = 3; k < 9; ++k) v[k] = r[k];
+                }
             } else {
-                valid = false;
+                unpack_inputs(na, nb, v);
+                /* the next chunk's inputs, in flight during this chunk's work */
+                const int j = i + stride;
+                na = make_float4(0.f, 0.f, 0.f, 0.f);
+                nb = make_float2(0.f, 0.f);
+                if (j < cnt) {
+                    na = g.in_a[j];
+                    nb = g.in_b[j];
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < GD_INS; ++c) IN[lane][c] = v[c];
+        }
+        wave_lds_sync();
+        /* layer 1, one K block of features at a time */
+        f4 z[4][4];
+        bias_init<4>(L.b, z);
+#pragma unroll
+        for (int kb = 0; kb < 3; ++kb) {
+            h8 x[4];
+#pragma unroll
+            for (int st = 0; st < 4; ++st) x[st] = feature_group(kb, q, IN[16 * st + n]);
+            dense32_k<3, 4>(L.w + GD_W16I_W1, kb, x, z);
+        }
+        wave_lds_sync();
+        h8 h[4][2];
+        relu_pack(z, h);
+        dense32<4>(L.w + GD_W16I_W2, L.b + GD_H, h, z);
+        relu_pack(z, h);
+        dense32<4>(L.w + GD_W16I_W3, L.b + 2 * GD_H, h, z);
+        relu_pack(z, h);
+        f4 y[4][3];
+        dense32<3>(L.w + GD_W16I_W4, L.b + 3 * GD_H, h, y);
+#pragma unroll
+        for (int st = 0; st < 4; ++st) {
+            const int i = base + 16 * st + n;
+            const bool valid = i < cnt;
+            float r[12];
+#pragma unroll
+            for (int ot = 0; ot < 3; ++ot)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) r[4 * ot + e] = y[st][ot][e];
+            if (RAW) {
+#pragma unroll
+                for (int sl = 0; sl < 12; ++sl) {
+                    const int o = gd_head_output(q, sl);
+                    if (valid && o >= 0) yout[(size_t)i * GD_OUT + o] = r[sl];
+                }
+                continue;
+            }
+            vec3 mu0, mu1;
+            float k0, k1;
+            lobe_head_fast(r, mu0, k0);
+            lobe_head_fast(r + 5, mu1, k1);
+            const float mx = quad_max(fmaxf(r[4], r[9]));
+            const float e0 = __expf(r[4] - mx), e1 = __expf(r[9] - mx);
+            const float inv = __builtin_amdgcn_rcpf(quad_sum(e0 + e1));
+            if (valid) {
+                h2* o = reinterpret_cast<h2*>(hout + (size_t)i * GD_PARAMS + 10 * q);
+                o[0] = h2{(_Float16)mu0.x, (_Float16)mu0.y};
+                o[1] = h2{(_Float16)mu0.z, (_Float16)k0};
+                o[2] = h2{(_Float16)(e0 * inv), (_Float16)mu1.x};
+                o[3] = h2{(_Float16)mu1.y, (_Float16)mu1.z};
+                o[4] = h2{(_Float16)k1, (_Float16)(e1 * inv)};
+                if (q == 0) hout[(size_t)i * GD_PARAMS + 40] = (_Float16)(0.05f + 0.9f * __builtin_amdgcn_rcpf(1.f + __expf(-r[10])));
             }
         }
-        h4 x[5], h1[4], h2[4], h3[4];
-        f4 y[3];
-        /* the wave's feature rows share its output staging area */
-        _Float16 (*F)[GD_FROW] = reinterpret_cast<_Float16 (*)[GD_FROW]>(&Y[w][0][0]);
-        features_to_lds(F, n, q, p01, nf, wo);
-        __syncthreads();
-        features_from_lds(F, n, q, x);
-        forward16(L.w, L.b, x, h1, h2, h3, y);
-#pragma unroll
-        for (int t = 0; t < 3; ++t)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) Y[w][16 * t + 4 * q + r][n] = y[t][r];
-        __syncthreads();
-        float a[2];
-        vec3 mu[2];
-        float kap[2];
-#pragma unroll
-        for (int s = 0; s < 2; ++s) {
-            int j = 2 * q + s;
-            float r[5];
-#pragma unroll
-            for (int c = 0; c < 5; ++c) r[c] = Y[w][5 * j + c][n];
-            float mlen;
-            bool kin;
-            lobe_head(r, mu[s], mlen, kap[s], kin);
-            a[s] = r[4];
-        }
-        float mx = quad_max(fmaxf(a[0], a[1]));
-        float e0 = expf(a[0] - mx), e1 = expf(a[1] - mx);
-        float inv = 1.f / quad_sum(e0 + e1);
-        if (valid) {
-            float* o = g.out + (size_t)p * GD_PARAMS;
-#pragma unroll
-            for (int s = 0; s < 2; ++s) {
-                int j = 2 * q + s;
-                o[5 * j] = mu[s].x;
-                o[5 * j + 1] = mu[s].y;
-                o[5 * j + 2] = mu[s].z;
-                o[5 * j + 3] = kap[s];
-                o[5 * j + 4] = (s ? e1 : e0) * inv;
-            }
-            if (q == 0) o[40] = 0.05f + 0.9f / (1.f + expf(-Y[w][40][n]));
-        }
-        __syncthreads();
     }
 }
 
 hipError_t launch_guide_infer(const HostDevScene& s, const PathState& ps, const GuideState& g, int qsel, int grid,
                               hipStream_t st)
 {
-    hipLaunchKernelGGL(k_guide_infer, dim3(grid), dim3(GB), 0, st, as_dev_g(s), ps, g, qsel);
+    hipLaunchKernelGGL(k_guide_gather, dim3(grid * 4), dim3(GB), 0, st, as_dev_g(s), ps, g, qsel);
+    hipLaunchKernelGGL((k_guide_net<false, false>), dim3(grid), dim3(GB), 0, st, ps, g, qsel, (const float*)nullptr, 0,
+                       g.out, (float*)nullptr);
     return hipGetLastError();
 }
 
-/* ===========================================================================
- * forward only on records (parity surface): raw outputs y [n x 48]
- * ========================================================================= */
+/* the network on records (parity surface): raw outputs y [n x 48] */
+hipError_t launch_guide_forward(const GuideState& g, const float* rec, int n, float* y, int grid, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_guide_net<true, true>), dim3(grid), dim3(GB), 0, st, PathState{}, g, 0, rec, n,
+                       (_Float16*)nullptr, y);
+    return hipGetLastError();
+}
+
+/* head outputs of records as the shading kernel receives them [n x 48 halves] */
+hipError_t launch_guide_head(const GuideState& g, const float* rec, int n, _Float16* out, int grid, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_guide_net<true, false>), dim3(grid), dim3(GB), 0, st, PathState{}, g, 0, rec, n, out,
+                       (float*)nullptr);
+    return hipGetLastError();
+}
+
 __device__ __forceinline__ void record_inputs(const GuideState& g, const float* r, float* p01, vec3& nf, vec3& wo)
 {
     p01[0] = (r[0] - g.lo[0]) * g.inv_ext[0];
@@ -280,38 +544,6 @@ __device__ __forceinline__ void record_inputs(const GuideState& g, const float* 
     p01[2] = (r[2] - g.lo[2]) * g.inv_ext[2];
     nf = mk(r[3], r[4], r[5]);
     wo = mk(r[6], r[7], r[8]);
-}
-
-__global__ __launch_bounds__(GB) void k_guide_forward(GuideState g, const float* rec, int N, float* yout)
-{
-    __shared__ WLds<GD_NW16F> L;
-    __shared__ _Float16 F[4][16][GD_FROW];
-    load_weights(L, g);
-    const int lane = __lane_id(), n = lane & 15, q = lane >> 4, w = threadIdx.x >> 6;
-    for (int base = blockIdx.x * 64; base < N; base += gridDim.x * 64) {
-        int i = base + 16 * w + n;
-        bool valid = i < N;
-        float p01[3] = {0.f, 0.f, 0.f};
-        vec3 nf = mk(0.f, 0.f, 0.f), wo = nf;
-        if (valid) record_inputs(g, rec + (size_t)i * GD_REC, p01, nf, wo);
-        h4 x[5], h1[4], h2[4], h3[4];
-        f4 y[3];
-        features_to_lds(F[w], n, q, p01, nf, wo);
-        __syncthreads();
-        features_from_lds(F[w], n, q, x);
-        forward16(L.w, L.b, x, h1, h2, h3, y);
-        if (valid)
-#pragma unroll
-            for (int t = 0; t < 3; ++t)
-#pragma unroll
-                for (int r = 0; r < 4; ++r) yout[(size_t)i * GD_OUT + 16 * t + 4 * q + r] = y[t][r];
-    }
-}
-
-hipError_t launch_guide_forward(const GuideState& g, const float* rec, int n, float* y, int grid, hipStream_t st)
-{
-    hipLaunchKernelGGL(k_guide_forward, dim3(grid), dim3(GB), 0, st, g, rec, n, y);
-    return hipGetLastError();
 }
 
 /* ===========================================================================
@@ -616,6 +848,8 @@ __global__ __launch_bounds__(GB) void k_guide_adam(GuideState g, float lr, int s
         gd_w16_slots(i, &fw, &bw);
         if (fw >= 0) g.w16[fw] = (_Float16)th;
         if (bw >= 0) g.w16[bw] = (_Float16)th;
+        const int iw = gd_w16i_slot(i);
+        if (iw >= 0) g.w16i[iw] = (_Float16)th;
     }
 }
 

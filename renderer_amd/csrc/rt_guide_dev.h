@@ -38,27 +38,50 @@ __device__ __forceinline__ vec3 vmf_sample(vec3 mu, float kappa, float u0, float
     return norm(t * (r * cp) + v * (r * sp) + mu * w);
 }
 
-/* Head output record of one path: lobe j at [5j .. 5j+4] = (mu.xyz, kappa,
- * weight), [40] = probability of sampling the guide. */
-__device__ __forceinline__ float guide_pdf(const float* o, vec3 w)
+/* Head output of one path vertex: 48 halves, lobe j at [5j .. 5j+4] =
+ * (mu.xyz, kappa, weight), [40] the probability of sampling the guide.  The
+ * lobes are read as the distribution the rounded halves define: mu
+ * renormalised and the weights divided by their sum, the same values in
+ * guide_sample and guide_pdf, so that the density is exactly that of the
+ * samples. */
+__device__ __forceinline__ void guide_lobe(const _Float16* o, int j, vec3& mu, float& kappa, float& w)
 {
-    float q = 0.f;
-#pragma unroll
-    for (int j = 0; j < GD_LOBES; ++j)
-        q += o[5 * j + 4] * vmf_pdf(mk(o[5 * j], o[5 * j + 1], o[5 * j + 2]), o[5 * j + 3], w);
-    return q;
+    mu = norm(mk((float)o[5 * j], (float)o[5 * j + 1], (float)o[5 * j + 2]));
+    kappa = (float)o[5 * j + 3];
+    w = (float)o[5 * j + 4];
 }
 
-__device__ __forceinline__ vec3 guide_sample(const float* o, float ul, float u0, float u1)
+__device__ __forceinline__ float guide_pdf(const _Float16* o, vec3 w)
 {
+    float q = 0.f, ws = 0.f;
+#pragma unroll
+    for (int j = 0; j < GD_LOBES; ++j) {
+        vec3 mu;
+        float k, wt;
+        guide_lobe(o, j, mu, k, wt);
+        q += wt * vmf_pdf(mu, k, w);
+        ws += wt;
+    }
+    return q / ws;
+}
+
+__device__ __forceinline__ vec3 guide_sample(const _Float16* o, float ul, float u0, float u1)
+{
+    float ws = 0.f;
+#pragma unroll
+    for (int j = 0; j < GD_LOBES; ++j) ws += (float)o[5 * j + 4];
+    const float t = ul * ws;
     int j = 0;
     float acc = 0.f;
 #pragma unroll
     for (int k = 0; k < GD_LOBES - 1; ++k) {
-        acc += o[5 * k + 4];
-        if (ul >= acc) j = k + 1;
+        acc += (float)o[5 * k + 4];
+        if (t >= acc) j = k + 1;
     }
-    return vmf_sample(mk(o[5 * j], o[5 * j + 1], o[5 * j + 2]), o[5 * j + 3], u0, u1);
+    vec3 mu;
+    float kappa, wt;
+    guide_lobe(o, j, mu, kappa, wt);
+    return vmf_sample(mu, kappa, u0, u1);
 }
 
 #endif

@@ -1134,7 +1134,7 @@ struct GuideRecord {
 };
 
 __device__ void shade_guided(const DevScene& S, const PathState& ps, const GuideState& G, const PtParams& P,
-                             int depth, int p, bool& cont, bool& shadow, float4& so, float4& sd, float4& sc,
+                             int depth, int p, int qi, bool& cont, bool& shadow, float4& so, float4& sd, float4& sc,
                              bool& want_rec, GuideRecord& rec)
 {
     cont = false;
@@ -1179,9 +1179,10 @@ __device__ void shade_guided(const DevScene& S, const PathState& ps, const Guide
     if (M.type != RT_MAT_DIFFUSE && M.type != RT_MAT_PHONG && M.type != RT_MAT_BLINN) return;
     vec3 wo = -d;
     const bool last = depth + 1 >= P.max_depth;
-    /* the network is not evaluated for the last bounce */
-    const float* go = G.out + (size_t)p * GD_PARAMS;
-    float ag = last ? 0.f : go[40];
+    /* the network is not evaluated for the last bounce; its outputs are
+     * stored by queue position */
+    const _Float16* go = G.out + (size_t)qi * GD_PARAMS;
+    float ag = last ? 0.f : (float)go[40];
     if (S.n_lights > 0) {
         U4 gl = rng4(P.seed, (uint32_t)info.z, (uint32_t)info.x, (uint32_t)info.y, 2u + 2u * depth);
         vec3 wi, le;
@@ -1257,7 +1258,7 @@ __global__ __launch_bounds__(PT_BLOCK) RT_SHADE_GUIDED_OCCUPANCY void k_shade_gu
         int p = -1;
         if (i < n) {
             p = q[i];
-            shade_guided(S, ps, G, P, depth, p, cont, shadow, so, sd, sc, want, rec);
+            shade_guided(S, ps, G, P, depth, p, i, cont, shadow, so, sd, sc, want, rec);
         }
         const bool pred[3] = {cont, shadow, want && P.guide_train};
         int32_t* const ctr[3] = {&ps.cnt->q_count[qsel ^ 1], &ps.cnt->shadow_count, G.rec_count};

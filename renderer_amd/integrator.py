@@ -251,6 +251,13 @@ class GpuContext:
         _check(self.lib, self.h, self.lib.rt_guide_forward(self.h, rec.ctypes.data, len(rec), y.ctypes.data))
         return y
 
+    def guide_head(self, records):
+        """Head outputs (n, 48) as the guided shader receives them."""
+        rec = _f32(records, 16)
+        out = np.empty((len(rec), 48), np.float32)
+        _check(self.lib, self.h, self.lib.rt_guide_head(self.h, rec.ctypes.data, len(rec), out.ctypes.data))
+        return out
+
     def guide_gradient(self, records):
         rec = _f32(records, 16)
         g = np.empty(self.guide_param_count(), np.float32)

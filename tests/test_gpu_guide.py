@@ -71,11 +71,40 @@ def test_forward_matches_oracle(guide_ctx):
     y = guide_ctx.guide_forward(rec)
     x, _, _, _ = _oracle_inputs(rec)
     yr, _ = G.forward(_fp16_weights(th), x, fp16_acts=True)
+    # outputs 41..47 feed nothing and are not computed
+    assert np.all(y[:, 41:] == 0)
+    y, yr = y[:, :41], yr[:, :41]
     err = np.abs(y - yr)
     scale = np.abs(yr).max()
     print(f"forward max err {err.max():.3e} scale {scale:.3f}", flush=True)
     assert err.max() <= 2e-2 * scale
     assert np.sqrt((err ** 2).mean()) <= 2e-3 * scale
+
+
+def test_head_matches_oracle(guide_ctx):
+    """The values the guided shader samples from and evaluates: mean
+    directions (renormalised), concentrations, mixture weights (divided by
+    their sum) and the guide probability, against the oracle's head of its
+    FP64 forward pass.  The outputs are stored as FP16."""
+    th = _theta(7)
+    guide_ctx.guide_set_params(th)
+    rec = _records(3000, 8)
+    out = guide_ctx.guide_head(rec).astype(np.float64)
+    x, _, _, _ = _oracle_inputs(rec)
+    yr, _ = G.forward(_fp16_weights(th), x, fp16_acts=True)
+    hd = G.head(yr)
+    lob = out[:, :40].reshape(-1, 8, 5)
+    mu = lob[:, :, :3] / np.linalg.norm(lob[:, :, :3], axis=2, keepdims=True)
+    ang = np.arccos(np.clip((mu * hd["mu"]).sum(2), -1, 1))
+    wts = lob[:, :, 4] / lob[:, :, 4].sum(1, keepdims=True)
+    print(f"head: direction err max {ang.max():.2e} rad, log-kappa err max "
+          f"{np.abs(np.log(lob[:, :, 3]) - np.log(hd['kappa'])).max():.2e}, weight err max "
+          f"{np.abs(wts - hd['pi']).max():.2e}, alpha err max {np.abs(out[:, 40] - hd['alpha']).max():.2e}", flush=True)
+    assert np.all(out[:, 41:] == 0)
+    assert np.percentile(ang, 99) < 0.02 and ang.max() < 0.1
+    assert np.abs(np.log(lob[:, :, 3]) - np.log(hd["kappa"])).max() < 0.05
+    assert np.abs(wts - hd["pi"]).max() < 0.01
+    assert np.abs(out[:, 40] - hd["alpha"]).max() < 0.005
 
 
 def test_gradient_matches_oracle(guide_ctx):
