@@ -24,6 +24,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 RAY_BYTES = 32 + 16      # ray (origin, tmin, direction, tmax) + hit (t, primitive, u, v)
+SHADOW_RAY_BYTES = 32 + 16  # ray + its contribution record (rgb, path slot)
 NODE_BYTES = 128         # one 4-wide node: four child boxes + child words
 PRIM_BYTES = 48          # one primitive record
 
@@ -50,21 +51,22 @@ def parse():
     ap.add_argument("--no-scene-file", action="store_true", help="skip the scene-file front-end timing")
     ap.add_argument("--no-film", action="store_true", help="skip the tonemapper and film-writer timing")
     ap.add_argument("--no-mlt", action="store_true", help="skip the Metropolis and ERPT renders")
-    ap.add_argument("--room-res", type=str, default="480x270")
-    ap.add_argument("--room-subdiv", type=int, default=64, help="octahedron segments per edge of the 8 objects")
-    ap.add_argument("--room-seconds", type=float, default=2.0, help="wall-time budget of each estimator")
-    ap.add_argument("--room-ref-spp", type=int, default=16384)
+    ap.add_argument("--room-res", type=str, default="1920x1080")
+    ap.add_argument("--room-subdiv", type=int, default=256,
+                    help="octahedron segments per edge of the 8 objects (256: 4,194,304 triangles)")
+    ap.add_argument("--room-seconds", type=float, default=20.0, help="wall-time budget of each estimator")
+    ap.add_argument("--room-ref-spp", type=int, default=4096)
     ap.add_argument("--room-pass-spp", type=int, default=16)
-    ap.add_argument("--room-train-passes", type=int, default=24, help="guided passes that train the network")
+    ap.add_argument("--room-train-passes", type=int, default=16, help="guided passes that train the network")
     ap.add_argument("--room-epochs", type=int, default=4, help="training epochs over each pass's records")
-    ap.add_argument("--room-lr", type=float, default=5e-3)
+    ap.add_argument("--room-lr", type=float, default=1e-3)
     ap.add_argument("--env-res", type=int, default=1024)
     ap.add_argument("--env-seconds", type=float, default=1.5)
     ap.add_argument("--env-ref-spp", type=int, default=4096)
     ap.add_argument("--env-pass-spp", type=int, default=16)
     ap.add_argument("--env-train-passes", type=int, default=8)
     ap.add_argument("--env-epochs", type=int, default=4)
-    ap.add_argument("--env-lr", type=float, default=5e-3)
+    ap.add_argument("--env-lr", type=float, default=1e-3)
     return ap.parse_args()
 
 
@@ -418,7 +420,10 @@ def cpu_baseline(arrays, scene, budget_s):
     spent.  Returns Mrays/s over the frames rendered."""
     from oracle.oracle import Oracle
 
-    threads = os.cpu_count() or 1
+    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    if os.environ.get("OMP_NUM_THREADS", "").isdigit():
+        threads = min(threads, int(os.environ["OMP_NUM_THREADS"]))
+    cpu_model, cpu_mhz = host_cpu()
     t0 = time.time()
     orc = Oracle(arrays, accel=1)
     build_s = time.time() - t0
@@ -434,8 +439,26 @@ def cpu_baseline(arrays, scene, budget_s):
         frames += 1
     orc.close()
     return {"value": round(rays / t_render / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "cpu": cpu_model, "cpu_mhz": cpu_mhz,
             "sample": f"{frames} frames of {cam.width}x{cam.height} at 1 spp (seeds 0..{frames - 1}), {rays} rays "
                       f"in {t_render:.1f} s on {threads} threads; reference BVH build {build_s:.1f} s not timed"}
+
+
+def host_cpu():
+    """Model name and current clock (MHz, mean over the cores) from /proc/cpuinfo."""
+    model, mhz = None, []
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                k, _, v = line.partition(":")
+                k = k.strip()
+                if k == "model name" and model is None:
+                    model = v.strip()
+                elif k == "cpu MHz":
+                    mhz.append(float(v))
+    except OSError:
+        pass
+    return model, (round(sum(mhz) / len(mhz), 1) if mhz else None)
 
 
 def main():
@@ -469,6 +492,7 @@ def main():
     torch.cuda.synchronize()
     rays = 0
     ext_ms = 0.0
+    sh_ms = 0.0
     ext_launches = 0
     t0 = time.perf_counter()
     for i in range(args.steps):
@@ -476,6 +500,7 @@ def main():
         s = ctx.stats()
         rays += s["camera_rays"] + s["extension_rays"] + s["shadow_rays"]
         ext_ms += s["extend_ms"]
+        sh_ms += s["shadow_ms"]
         ext_launches += s["extend_launches"]
         log(f"step {i}: {s['render_ms']:.1f} ms, {(s['camera_rays'] + s['extension_rays'] + s['shadow_rays']) / 1e6:.1f} Mrays")
     torch.cuda.synchronize()
@@ -493,11 +518,23 @@ def main():
     per_launch_bytes = alg_bytes / max(1, cs["extend_launches"])
     avg_launch_ms = ext_ms / max(1, ext_launches)
     achieved = per_launch_bytes / (avg_launch_ms * 1e-3) / 1e9 if avg_launch_ms > 0 else None
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles", "extend_traffic.json")
+    # the same for the any-hit launches (one per bounce, like the closest-hit ones)
+    sh_bytes = (cs["shadow_rays"] * SHADOW_RAY_BYTES + cs["shadow_node_visits"] * NODE_BYTES +
+                cs["shadow_prim_tests"] * PRIM_BYTES)
+    sh_per_launch = sh_bytes / max(1, cs["extend_launches"])
+    sh_avg_ms = sh_ms / max(1, ext_launches)
+    sh_achieved = sh_per_launch / (sh_avg_ms * 1e-3) / 1e9 if sh_avg_ms > 0 else None
+    # measured L2-to-fabric traffic and L2 hit rates of these kernels (rocprofv3
+    # counters of the same workload, committed under profiles/)
+    traffic = sh_traffic = l2 = sh_l2 = None
+    tfile = os.path.join(ROOT, "profiles", "traversal_traffic.json")
     if os.path.exists(tfile):
         with open(tfile) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+            tr = json.load(f)
+        traffic = tr.get("k_extend", {}).get("hbm_bytes_per_launch")
+        sh_traffic = tr.get("k_shadow", {}).get("hbm_bytes_per_launch")
+        l2 = tr.get("k_extend", {}).get("l2_hit_rate")
+        sh_l2 = tr.get("k_shadow", {}).get("l2_hit_rate")
 
     if args.copy_calibration:
         a = torch.ones(1 << 28, dtype=torch.float32, device="cuda:0")  # 1 GiB
@@ -582,6 +619,21 @@ def main():
             "alg_bytes_per_launch": int(per_launch_bytes),
             "nodes_per_ray": round(cs["node_visits"] / max(1, ext_rays), 2),
             "prims_per_ray": round(cs["prim_tests"] / max(1, ext_rays), 2),
+            "l2_hit_rate": l2,
+        },
+        "roofline_shadow": {
+            "bound": "hbm",
+            "kernel": "k_shadow (any-hit BVH traversal of next-event rays)",
+            "achieved": round(sh_achieved, 1) if sh_achieved else None,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(sh_achieved / HBM_PEAK_GBS, 4) if sh_achieved else None,
+            "traffic": sh_traffic,
+            "avg_launch_ms": round(sh_avg_ms, 4),
+            "alg_bytes_per_launch": int(sh_per_launch),
+            "nodes_per_ray": round(cs["shadow_node_visits"] / max(1, cs["shadow_rays"]), 2),
+            "prims_per_ray": round(cs["shadow_prim_tests"] / max(1, cs["shadow_rays"]), 2),
+            "l2_hit_rate": sh_l2,
         },
         "cpu_baseline": cpu,
         "batched_views": batched,

@@ -75,6 +75,8 @@ typedef struct rt_stats {
     int64_t guide_train_samples;  /* records summed over all training steps */
     int64_t guide_train_steps;    /* Adam steps */
     double tonemap_ms;            /* duration of the last rt_film_tonemap kernel (GPU events) */
+    int64_t shadow_node_visits;   /* interior nodes fetched by any-hit traversal (count_traversal) */
+    int64_t shadow_prim_tests;    /* primitive records tested by any-hit traversal (count_traversal) */
 } rt_stats;
 
 /* Library-wide. */
@@ -143,7 +145,7 @@ typedef struct rt_launch_config {
     int32_t use_graph;     /* 1 = capture one pass's kernel sequence once and replay it as a
                               hipGraph (ignored while the guide trains); the per-kernel
                               times in rt_stats are then 0 */
-    int32_t count_traversal; /* 1 = closest-hit kernels also count nodes and primitives */
+    int32_t count_traversal; /* 1 = closest-hit and any-hit kernels also count nodes and primitives */
     int32_t guide_epochs;  /* GUIDED + training: passes over each render pass's records (0 = 1) */
 } rt_launch_config;
 int rt_set_launch_config(rt_ctx* ctx, const rt_launch_config* cfg);

@@ -100,6 +100,8 @@ class RtStats(ctypes.Structure):
         ("guide_train_samples", ctypes.c_int64),
         ("guide_train_steps", ctypes.c_int64),
         ("tonemap_ms", ctypes.c_double),
+        ("shadow_node_visits", ctypes.c_int64),
+        ("shadow_prim_tests", ctypes.c_int64),
     ]
 
 

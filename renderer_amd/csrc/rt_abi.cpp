@@ -1346,7 +1346,7 @@ static int render_impl(rt_ctx* c, const rt_view* views, int32_t nv, const rt_par
                 QCHK(launch_shade(c->scene, c->ps, pp, dd, q, grid, c->stream));
             }
             QCHK(mark());
-            QCHK(launch_shadow(c->scene, c->ps, pp, grid, c->stream));
+            QCHK(launch_shadow(c->scene, c->ps, pp, c->cfg.count_traversal, grid, c->stream));
             QCHK(mark());
             QCHK(launch_advance(c->ps, q, dd, c->stream));
         }
@@ -1436,6 +1436,8 @@ static int render_impl(rt_ctx* c, const rt_view* views, int32_t nv, const rt_par
     c->stats.shadow_rays = (int64_t)hc.shadow_rays;
     c->stats.node_visits = (int64_t)hc.node_visits;
     c->stats.prim_tests = (int64_t)hc.prim_tests;
+    c->stats.shadow_node_visits = (int64_t)hc.sh_node_visits;
+    c->stats.shadow_prim_tests = (int64_t)hc.sh_prim_tests;
     c->stats.paths_per_pass = (int32_t)P;
     c->stats.passes = (int32_t)passes.size();
     if (!device_films) {
@@ -1467,7 +1469,7 @@ hipError_t mlt_trace(rt_ctx* c, int n, int cap, int max_depth, int rr_start, flo
         int q = dd & 1;
         if ((e = launch_extend(c->scene, c->ps, q, 0, grid, c->stream))) return e;
         if ((e = launch_shade(c->scene, c->ps, pp, dd, q, grid, c->stream))) return e;
-        if ((e = launch_shadow(c->scene, c->ps, pp, grid, c->stream))) return e;
+        if ((e = launch_shadow(c->scene, c->ps, pp, 0, grid, c->stream))) return e;
         if ((e = launch_advance(c->ps, q, dd, c->stream))) return e;
     }
     return hipSuccess;

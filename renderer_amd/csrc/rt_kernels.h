@@ -75,6 +75,7 @@ struct DevCounters {
     int32_t sh_next;         /* the next unclaimed queue position (zero before each launch) */
     unsigned long long camera_rays, extension_rays, shadow_rays;
     unsigned long long node_visits, prim_tests;
+    unsigned long long sh_node_visits, sh_prim_tests;  /* any-hit traversal (count_traversal) */
 };
 
 /* Path state in HBM (structure of arrays, capacity = path slots). */
@@ -218,7 +219,7 @@ hipError_t launch_generate_pss(const PathState& ps, const DevView* view, int n, 
 hipError_t launch_extend(const HostDevScene& s, const PathState& ps, int qsel, int count, int grid, hipStream_t st);
 hipError_t launch_shade(const HostDevScene& s, const PathState& ps, PtParams p, int depth, int qsel, int grid,
                         hipStream_t st);
-hipError_t launch_shadow(const HostDevScene& s, const PathState& ps, PtParams p, int grid, hipStream_t st);
+hipError_t launch_shadow(const HostDevScene& s, const PathState& ps, PtParams p, int count, int grid, hipStream_t st);
 /* lanes of a traversal launch of `grid` blocks (the stack overflow array
  * holds RT_STACK_OVF entries for each) */
 int64_t trav_lanes(int grid);

@@ -1339,7 +1339,7 @@ hipError_t launch_shade(const HostDevScene& s, const PathState& ps, PtParams p, 
 /* ===========================================================================
  * shadow: any-hit test of next-event rays
  * ========================================================================= */
-template <bool OBJ>
+template <bool OBJ, bool COUNT>
 __global__ __launch_bounds__(PT_BLOCK) RT_TRAV_OCCUPANCY void k_shadow(DevScene S, PathState ps, int compat)
 {
     __shared__ int32_t stack[RT_STACK_LDS * PT_BLOCK];
@@ -1369,7 +1369,7 @@ __global__ __launch_bounds__(PT_BLOCK) RT_TRAV_OCCUPANCY void k_shadow(DevScene 
             }
         }
         if (!__ballot(live)) break;  /* queue drained and every lane idle */
-        if (live && (T.cur < 0 || trav_step<true, false>(S, T, my, cn, cp))) {
+        if (live && (T.cur < 0 || trav_step<true, COUNT>(S, T, my, cn, cp))) {
             if (T.g < 0) {
                 /* unoccluded: add the contribution to its path */
                 float4 c = ps.sh_c[item];
@@ -1384,15 +1384,24 @@ __global__ __launch_bounds__(PT_BLOCK) RT_TRAV_OCCUPANCY void k_shadow(DevScene 
             live = false;
         }
     }
+    if (COUNT && (cn || cp)) {
+        atomicAdd(&ps.cnt->sh_node_visits, (unsigned long long)cn);
+        atomicAdd(&ps.cnt->sh_prim_tests, (unsigned long long)cp);
+    }
 }
 
-hipError_t launch_shadow(const HostDevScene& s, const PathState& ps, PtParams p, int grid, hipStream_t st)
+hipError_t launch_shadow(const HostDevScene& s, const PathState& ps, PtParams p, int count, int grid, hipStream_t st)
 {
     int compat = p.mode == RT_MODE_COMPAT ? 1 : 0;
-    if (s.n_obj > 0)
-        hipLaunchKernelGGL(k_shadow<true>, dim3(grid), dim3(PT_BLOCK), 0, st, as_dev(s), ps, compat);
+    const bool obj = s.n_obj > 0;
+    if (count && obj)
+        hipLaunchKernelGGL((k_shadow<true, true>), dim3(grid), dim3(PT_BLOCK), 0, st, as_dev(s), ps, compat);
+    else if (count)
+        hipLaunchKernelGGL((k_shadow<false, true>), dim3(grid), dim3(PT_BLOCK), 0, st, as_dev(s), ps, compat);
+    else if (obj)
+        hipLaunchKernelGGL((k_shadow<true, false>), dim3(grid), dim3(PT_BLOCK), 0, st, as_dev(s), ps, compat);
     else
-        hipLaunchKernelGGL(k_shadow<false>, dim3(grid), dim3(PT_BLOCK), 0, st, as_dev(s), ps, compat);
+        hipLaunchKernelGGL((k_shadow<false, false>), dim3(grid), dim3(PT_BLOCK), 0, st, as_dev(s), ps, compat);
     return hipGetLastError();
 }
 
@@ -1473,6 +1482,7 @@ __global__ void k_reset(PathState ps)
     c->ext_next = c->sh_next = 0;
     c->camera_rays = c->extension_rays = c->shadow_rays = 0ull;
     c->node_visits = c->prim_tests = 0ull;
+    c->sh_node_visits = c->sh_prim_tests = 0ull;
 }
 hipError_t launch_reset_counters(const PathState& ps, hipStream_t st)
 {
