@@ -7,6 +7,7 @@ axis at the centre, directions perpendicular to it on the circle of radius
 one half), white furnaces under a constant probe in both estimators, and
 the sampling density built from the probe integrating to one."""
 import math
+import os
 
 import numpy as np
 import pytest
@@ -115,3 +116,61 @@ def test_probe_sampling_density_integrates_to_one():
     v /= np.linalg.norm(v, axis=1, keepdims=True)
     est = np.mean([orc.env_pdf(d) for d in v]) * 4 * math.pi
     assert est == pytest.approx(1.0, abs=0.05)
+
+
+CAMPUS_PROBE = "/root/reference/environmentmaps/campus_probe.hdr"
+
+
+def _decode_rgbe_rle(data):
+    """Independent reader of Radiance files with new-style run-length
+    scanlines (the format of the reference's probe): header lines up to an
+    empty line, '-Y h +X w', then per scanline the marker (2, 2, w >> 8,
+    w & 255) and the four channels each as runs (count > 128: count - 128
+    copies of one byte) and literals (count bytes).  Values byte * 2^(e - 136)."""
+    end = data.index(b"\n\n") + 2
+    res_end = data.index(b"\n", end)
+    ysign, h, xsign, w = data[end:res_end].split()
+    assert ysign == b"-Y" and xsign == b"+X"
+    h, w = int(h), int(w)
+    pos = res_end + 1
+    rgbe = np.zeros((h, w, 4), np.uint8)
+    buf = np.frombuffer(data, np.uint8)
+    for y in range(h):
+        assert tuple(buf[pos:pos + 4]) == (2, 2, w >> 8, w & 255)
+        pos += 4
+        for ch in range(4):
+            x = 0
+            while x < w:
+                c = int(buf[pos])
+                pos += 1
+                if c > 128:
+                    rgbe[y, x:x + c - 128, ch] = buf[pos]
+                    pos += 1
+                    x += c - 128
+                else:
+                    rgbe[y, x:x + c, ch] = buf[pos:pos + c]
+                    pos += c
+                    x += c
+    e = rgbe[..., 3].astype(np.int32)
+    scale = np.where(e == 0, 0.0, np.ldexp(1.0, e - 136))
+    return (rgbe[..., :3].astype(np.float64) * scale[..., None]).astype(np.float32)
+
+
+@pytest.mark.skipif(not os.path.exists(CAMPUS_PROBE), reason="the reference tree is not present")
+def test_reference_campus_probe_decodes():
+    """The reference's one data asset, the 640 x 640 angular light probe that
+    src/environmentmap.cpp:20-33 loads: the library's reader agrees bit for
+    bit with the independent decoder above, the values are finite and not
+    negative, and the probe's disk holds almost all of the energy (an
+    angular map uses only the inscribed circle; its corners are nearly
+    black)."""
+    env = EnvironmentMap.load(CAMPUS_PROBE)
+    with open(CAMPUS_PROBE, "rb") as f:
+        want = _decode_rgbe_rle(f.read())
+    assert env.image.shape == (640, 640, 3)
+    assert np.array_equal(env.image.view(np.uint32), want.view(np.uint32))
+    assert np.all(np.isfinite(env.image)) and env.image.min() >= 0
+    c = (np.arange(640) + 0.5) / 320.0 - 1.0
+    r = np.hypot(c[None, :], c[:, None])
+    assert env.image[r <= 1].sum() > 0.99 * env.image.sum()
+    assert env.image[r < 0.98].mean() > 0
