@@ -133,6 +133,24 @@ int rt_render(rt_ctx* ctx, const rt_view* views, int32_t n_views, const rt_param
 int rt_render_device(rt_ctx* ctx, const rt_view* views, int32_t n_views, const rt_params* params,
                      float* const* films);
 
+/* UniDiPathTracingIntegrator::integrate(const Ray&, const Intersectable&,
+ * std::vector<const Light*>, gsl_rng*) (src/integrator.h:35,
+ * src/unidipathtracingintegrator.cpp:34-46) for n caller rays: rgb[3i..3i+2]
+ * = the estimate of params->mode (RT_MODE_COMPAT or RT_MODE_UNBIASED; the
+ * view fields and the guide fields of params are not used) along ray i,
+ * given as for rt_trace_closest (origin, range start; direction, range end).
+ * The reference estimator takes the ray as given (its direction length
+ * enters the emission term as in src/arealight.cpp:102-106); the unbiased
+ * one normalises the direction and scales the range by its length.  Ray i
+ * draws the random numbers of the Philox stream (params->seed, view 0,
+ * index key0 + i, sample 0), the groups a camera sample of rt_render draws
+ * (group 0: only its path-length uniform), so a ray and key give the same
+ * value on every call. */
+int rt_integrate_rays(rt_ctx* ctx, const float* o_tmin, const float* d_tmax, int64_t n, const rt_params* params,
+                      uint32_t key0, float* rgb);
+int rt_integrate_rays_device(rt_ctx* ctx, const float* o_tmin, const float* d_tmax, int64_t n,
+                             const rt_params* params, uint32_t key0, float* rgb);
+
 /* Launch configuration used by rt_render; 0 leaves path_slots, block,
  * blocks_per_cu and guide_epochs unchanged, while the two flags are set as
  * given.  path_slots: paths in flight per pass; block: threads per block;

@@ -50,6 +50,7 @@ def load():
     lib.oc_render.restype = I64
     lib.oc_render.argtypes = [P, ctypes.POINTER(_abi.RtView), ctypes.POINTER(_abi.RtParams), ctypes.c_uint32, P,
                               ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    lib.oc_integrate_rays.argtypes = [P, I64, P, P, ctypes.POINTER(_abi.RtParams), ctypes.c_uint32, P, ctypes.c_int]
     lib.oc_render_sample.argtypes = [P, ctypes.POINTER(_abi.RtView), ctypes.POINTER(_abi.RtParams),
                                      ctypes.c_uint32, ctypes.c_int, ctypes.c_int, ctypes.c_int, P]
     lib.oc_env_sample.argtypes = [P, ctypes.c_float, ctypes.c_float, P, P]
@@ -183,6 +184,19 @@ class Oracle:
         rays = self.lib.oc_render(self.h, ctypes.byref(v), ctypes.byref(p), int(view_id), film.ctypes.data, r0, r1,
                                   threads)
         return film, int(rays)
+
+    def integrate_rays(self, o_tmin, d_tmax, mode=1, max_depth=8, rr_start=3, rr_prob=0.5, seed=0, key0=0,
+                       threads=8):
+        """UniDiPathTracingIntegrator::integrate(const Ray&, ...) per ray: (n, 3)."""
+        from renderer_amd import _abi
+        o = np.ascontiguousarray(o_tmin, np.float32)
+        d = np.ascontiguousarray(d_tmax, np.float32)
+        p = _abi.RtParams()
+        p.mode, p.max_depth, p.rr_start, p.rr_prob, p.seed = int(mode), int(max_depth), int(rr_start), float(rr_prob), int(seed)
+        out = np.zeros((len(o), 3), np.float32)
+        self.lib.oc_integrate_rays(self.h, len(o), o.ctypes.data, d.ctypes.data, ctypes.byref(p), int(key0) & 0xffffffff,
+                                   out.ctypes.data, threads)
+        return out
 
     def render_sample(self, view, px, py, s, mode=1, max_depth=8, rr_start=3, rr_prob=0.5, seed=0, view_id=0):
         v, p = self._view_params(view, mode, max_depth, rr_start, rr_prob, seed)

@@ -2318,6 +2318,37 @@ float oc_intersect_prim(void* s, int32_t prim, const float* o, const float* d)
     return sc.prims[prim]->intersect(ray_from(o, d, 0)).t;
 }
 
+// UniDiPathTracingIntegrator::integrate(const Ray&, ...),
+// src/unidipathtracingintegrator.cpp:34-46, for caller rays: ray i draws the
+// Philox stream (seed, view 0, index key0 + i, sample 0).  The unbiased
+// estimator normalises the direction (Qt: length in double, components
+// rounded to float) and scales the range by that length.
+void oc_integrate_rays(void* s, int64_t n, const float* o, const float* d, const rt_params* p, uint32_t key0,
+                       float* rgb, int threads)
+{
+    const Scene& sc = *static_cast<Scene*>(s);
+#pragma omp parallel for schedule(dynamic, 64) num_threads(threads > 0 ? threads : 1)
+    for (int64_t i = 0; i < n; ++i) {
+        Ray ray = ray_from(o, d, i);
+        SampleKey k{p->seed, 0u, key0 + (uint32_t)i, 0u};
+        int64_t rays = 0;
+        Col c;
+        if (p->mode == RT_MODE_COMPAT) {
+            c = integrate_compat(sc, ray, k, p->max_depth, p->rr_start, p->rr_prob, rays);
+        } else {
+            const double l = std::sqrt((double)ray.d.x * ray.d.x + (double)ray.d.y * ray.d.y + (double)ray.d.z * ray.d.z);
+            if (l > 0) {
+                ray.d = V3((float)(ray.d.x / l), (float)(ray.d.y / l), (float)(ray.d.z / l));
+                ray.from = (float)(ray.from * l);
+                if (!std::isinf(ray.to)) ray.to = (float)(ray.to * l);
+            }
+            UnbiasedCtx ctx{&sc, p->max_depth, p->rr_start, p->rr_prob};
+            c = integrate_unbiased(ctx, ray, k, rays);
+        }
+        rgb[3 * i] = c.r, rgb[3 * i + 1] = c.g, rgb[3 * i + 2] = c.b;
+    }
+}
+
 static Col render_one(const Scene& sc, const Camera& cam, const rt_view* view, const rt_params* p,
                       uint32_t view_id, int px, int py, int s, int64_t& rays)
 {

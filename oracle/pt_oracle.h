@@ -77,6 +77,11 @@ int64_t oc_render(void* scene, const rt_view* view, const rt_params* params, uin
 
 /* Radiance of one camera sample (pixel (px,py), sample index s) — the
  * per-sample Integrator::integrate entry of the reference. */
+/* UniDiPathTracingIntegrator::integrate(const Ray&, ...) for n rays (the
+ * layout of oc_trace_closest); ray i draws the stream (seed, view 0,
+ * key0 + i, sample 0); rgb[3i..3i+2] */
+void oc_integrate_rays(void* scene, int64_t n, const float* o, const float* d, const rt_params* params,
+                       uint32_t key0, float* rgb, int threads);
 void oc_render_sample(void* scene, const rt_view* view, const rt_params* params, uint32_t view_id,
                       int px, int py, int s, float out_rgb[3]);
 

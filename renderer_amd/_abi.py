@@ -158,6 +158,8 @@ PROTOTYPES = {
     "rt_trace_closest_device_noalloc": (ctypes.c_int, [_P, _P, _P, _I64, _P, _P]),
     "rt_render": (ctypes.c_int, [_P, ctypes.POINTER(RtView), _I32, ctypes.POINTER(RtParams), _P]),
     "rt_render_device": (ctypes.c_int, [_P, ctypes.POINTER(RtView), _I32, ctypes.POINTER(RtParams), _P]),
+    "rt_integrate_rays": (ctypes.c_int, [_P, _P, _P, _I64, ctypes.POINTER(RtParams), ctypes.c_uint32, _P]),
+    "rt_integrate_rays_device": (ctypes.c_int, [_P, _P, _P, _I64, ctypes.POINTER(RtParams), ctypes.c_uint32, _P]),
     "rt_set_launch_config": (ctypes.c_int, [_P, ctypes.POINTER(RtLaunchConfig)]),
     "rt_get_launch_config": (ctypes.c_int, [_P, ctypes.POINTER(RtLaunchConfig)]),
     "rt_get_stats": (ctypes.c_int, [_P, ctypes.POINTER(RtStats)]),

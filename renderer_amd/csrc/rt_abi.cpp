@@ -1958,6 +1958,85 @@ int integrate_samples_impl(rt_ctx* c, const rt_camera* cam, int max_depth, int r
 
 }  // namespace
 
+/* UniDiPathTracingIntegrator::integrate(const Ray&, ...) for a batch of rays
+ * in device memory: slots of at most the configured path-slot count, each
+ * chunk traced through the bounce pipeline of rt_render. */
+static int integrate_rays_impl(rt_ctx* c, const float* o, const float* d, int64_t n, const rt_params* p,
+                               uint32_t key0, float* rgb)
+{
+    if (!c->has_scene) return set_err(c, RT_ERR_NO_SCENE, "no scene uploaded");
+    if (p->mode != RT_MODE_COMPAT && p->mode != RT_MODE_UNBIASED)
+        return set_err(c, RT_ERR_INVALID, "rt_integrate_rays: mode must be RT_MODE_COMPAT or RT_MODE_UNBIASED");
+    if (p->max_depth < 1 || p->max_depth > 64 || p->rr_prob < 0.f || p->rr_prob >= 1.f || p->rr_start < 0)
+        return set_err(c, RT_ERR_INVALID, "bad render parameters");
+    if (n > (int64_t)UINT32_MAX) return set_err(c, RT_ERR_INVALID, "too many rays");
+    if (n == 0) return RT_OK;
+    hipSetDevice(c->device);
+    const int64_t cap = std::min<int64_t>(n, c->cfg.path_slots > 0 ? c->cfg.path_slots : (int64_t)1 << 25);
+    if (int r = alloc_path_state(c, (size_t)cap)) return r;
+    const int grid = grid_for(c);
+    if (int r = ensure_stack_overflow(c, grid)) return r;
+    PtParams pp{p->mode, p->max_depth, p->rr_start, p->rr_prob, p->seed, 0, 0.f};
+    HIP_TRY(c, launch_reset_counters(c->ps, c->stream));
+    HIP_TRY(c, hipEventRecord(c->ev_a, c->stream));
+    InFlight flight(c, 2);
+    for (int64_t base = 0; base < n; base += cap) {
+        if (c->stop) return cancel(c);
+        const int m = (int)std::min<int64_t>(cap, n - base);
+        HIP_TRY(c, launch_generate_rays(c->ps, (const float4*)o + base, (const float4*)d + base, m, pp,
+                                        key0 + (uint32_t)base, grid, c->stream));
+        for (int dd = 0; dd < p->max_depth; ++dd) {
+            const int q = dd & 1;
+            HIP_TRY(c, launch_extend(c->scene, c->ps, q, 0, grid, c->stream));
+            HIP_TRY(c, launch_shade(c->scene, c->ps, pp, dd, q, grid, c->stream));
+            HIP_TRY(c, launch_shadow(c->scene, c->ps, pp, 0, grid, c->stream));
+            HIP_TRY(c, launch_advance(c->ps, q, dd, c->stream));
+        }
+        HIP_TRY(c, launch_ray_values(c->ps, m, rgb + 3 * base, grid, c->stream));
+        HIP_TRY(c, flight.unit_done());
+    }
+    HIP_TRY(c, hipEventRecord(c->ev_b, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    float ms = 0;
+    hipEventElapsedTime(&ms, c->ev_a, c->ev_b);
+    c->stats.render_ms = ms;
+    DevCounters hc;
+    HIP_TRY(c, hipMemcpy(&hc, c->ps.cnt, sizeof(hc), hipMemcpyDeviceToHost));
+    c->stats.camera_rays = (int64_t)hc.camera_rays;
+    c->stats.extension_rays = (int64_t)hc.extension_rays;
+    c->stats.shadow_rays = (int64_t)hc.shadow_rays;
+    return RT_OK;
+}
+
+int rt_integrate_rays_device(rt_ctx* c, const float* o, const float* d, int64_t n, const rt_params* p, uint32_t key0,
+                             float* rgb)
+{
+    if (!c || !p || n < 0 || (n && (!o || !d || !rgb))) return RT_ERR_INVALID;
+    if (int r = check_fault(c)) return r;
+    return integrate_rays_impl(c, o, d, n, p, key0, rgb);
+}
+
+int rt_integrate_rays(rt_ctx* c, const float* o, const float* d, int64_t n, const rt_params* p, uint32_t key0,
+                      float* rgb)
+{
+    if (!c || !p || n < 0 || (n && (!o || !d || !rgb))) return RT_ERR_INVALID;
+    if (int r = check_fault(c)) return r;
+    if (!c->has_scene) return set_err(c, RT_ERR_NO_SCENE, "no scene uploaded");
+    if (n == 0) return RT_OK;
+    hipSetDevice(c->device);
+    int r;
+    if ((r = ensure(c, c->rb_o, (size_t)n * 16)) || (r = ensure(c, c->rb_d, (size_t)n * 16)) ||
+        (r = ensure(c, c->rb_out0, (size_t)n * 12)))
+        return r;
+    HIP_TRY(c, hipMemcpyAsync(c->rb_o.p, o, (size_t)n * 16, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->rb_d.p, d, (size_t)n * 16, hipMemcpyHostToDevice, c->stream));
+    if ((r = integrate_rays_impl(c, (const float*)c->rb_o.p, (const float*)c->rb_d.p, n, p, key0,
+                                 (float*)c->rb_out0.p)))
+        return r;
+    HIP_TRY(c, hipMemcpy(rgb, c->rb_out0.p, (size_t)n * 12, hipMemcpyDeviceToHost));
+    return RT_OK;
+}
+
 int rt_render(rt_ctx* c, const rt_view* views, int32_t nv, const rt_params* p, float* const* films)
 {
     return render_impl(c, views, nv, p, films, false);

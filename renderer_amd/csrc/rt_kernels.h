@@ -216,6 +216,11 @@ hipError_t launch_generate(const PathState& ps, const DevView* views, const DevT
  * path slot j = sample j.  View: the camera of `view` (film unused). */
 hipError_t launch_generate_pss(const PathState& ps, const DevView* view, int n, PtParams p, int pixel_units,
                                int grid, hipStream_t st);
+/* Paths from caller rays: slot j = ray j, random numbers keyed by key0 + j. */
+hipError_t launch_generate_rays(const PathState& ps, const float4* o, const float4* d, int n, PtParams p,
+                                uint32_t key0, int grid, hipStream_t st);
+/* rgb[3 j .. 3 j + 2] = radiance of path slot j, j < n */
+hipError_t launch_ray_values(const PathState& ps, int n, float* rgb, int grid, hipStream_t st);
 hipError_t launch_extend(const HostDevScene& s, const PathState& ps, int qsel, int count, int grid, hipStream_t st);
 hipError_t launch_shade(const HostDevScene& s, const PathState& ps, PtParams p, int depth, int qsel, int grid,
                         hipStream_t st);

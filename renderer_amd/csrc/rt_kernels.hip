@@ -768,6 +768,81 @@ hipError_t launch_generate_pss(const PathState& ps, const DevView* view, int n, 
     return hipGetLastError();
 }
 
+/* Paths that start with caller rays (UniDiPathTracingIntegrator::integrate(
+ * const Ray&, ...), src/unidipathtracingintegrator.cpp:34-46): ray j of the
+ * batch fills slot j; its random numbers are the Philox stream of (seed,
+ * view 0, index key0 + base + j, sample 0), group 0 giving the path-length
+ * uniform as for a camera sample.  The reference estimator takes the ray as
+ * given; the unbiased one normalises the direction (Qt arithmetic) and
+ * scales the range by the same length, so the range covers the same points. */
+__global__ __launch_bounds__(PT_BLOCK) void k_generate_rays(PathState ps, const float4* __restrict__ o,
+                                                           const float4* __restrict__ d, int n, PtParams P,
+                                                           uint32_t key0)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ps.cnt->q_count[0] = n;
+        ps.cnt->q_count[1] = 0;
+        ps.cnt->shadow_count = 0;
+        ps.cnt->ext_next = 0;
+        ps.cnt->sh_next = 0;
+        ps.cnt->camera_rays += (unsigned long long)n;
+    }
+    for (int slot = blockIdx.x * PT_BLOCK + threadIdx.x; slot < n; slot += gridDim.x * PT_BLOCK) {
+        float4 a = o[slot], b = d[slot];
+        const uint32_t key = key0 + (uint32_t)slot;
+        int L = P.max_depth;
+        if (P.mode == RT_MODE_COMPAT && P.rr_prob > 0.f) {
+            const U4 g0 = rng4(P.seed, 0u, key, 0u, 0u);
+            int nb = 0;
+            if (P.rr_prob < 1.f) {
+                float q = floorf(logf(1.f - g0.u[2]) / logf(1.f - P.rr_prob));
+                nb = q > 1000.f ? 1000 : (int)q;
+            }
+            L = min(P.rr_start + nb, P.max_depth);
+        }
+        if (P.mode != RT_MODE_COMPAT) {
+            const double l = sqrt(__dadd_rn(__dadd_rn(__dmul_rn(b.x, b.x), __dmul_rn(b.y, b.y)), __dmul_rn(b.z, b.z)));
+            if (l > 0.0) {
+                b = make_float4((float)(b.x / l), (float)(b.y / l), (float)(b.z / l),
+                                b.w == RT_INF ? RT_INF : (float)(b.w * l));
+                a.w = (float)(a.w * l);
+            }
+        }
+        ps.ray_o[slot] = a;
+        ps.ray_d[slot] = b;
+        ps.alpha[slot] = make_float4(1.f, 1.f, 1.f, 0.f);
+        ps.radiance[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
+        ps.pend_c[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
+        ps.info[slot] = make_int4((int)key, 0, 0, pk(0, L, 1));
+        ps.queue[0][slot] = slot;
+    }
+}
+
+hipError_t launch_generate_rays(const PathState& ps, const float4* o, const float4* d, int n, PtParams p,
+                                uint32_t key0, int grid, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_generate_rays, dim3(grid), dim3(PT_BLOCK), 0, st, ps, o, d, n, p, key0);
+    return hipGetLastError();
+}
+
+/* rgb[3 j ..] = the radiance of path slot j (with a pending next-event
+ * estimate of the reference estimator committed) */
+__global__ __launch_bounds__(PT_BLOCK) void k_ray_values(PathState ps, int n, float* __restrict__ rgb)
+{
+    for (int j = blockIdx.x * PT_BLOCK + threadIdx.x; j < n; j += gridDim.x * PT_BLOCK) {
+        const float4 L = ps.radiance[j], c = ps.pend_c[j];
+        rgb[3 * (size_t)j] = L.x + c.x;
+        rgb[3 * (size_t)j + 1] = L.y + c.y;
+        rgb[3 * (size_t)j + 2] = L.z + c.z;
+    }
+}
+
+hipError_t launch_ray_values(const PathState& ps, int n, float* rgb, int grid, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_ray_values, dim3(grid), dim3(PT_BLOCK), 0, st, ps, n, rgb);
+    return hipGetLastError();
+}
+
 /* ===========================================================================
  * extend: closest hit for every live path
  * ========================================================================= */

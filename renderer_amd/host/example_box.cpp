@@ -6,11 +6,15 @@
  *   example_box <out.raw> [width height spp_x spp_y mode max_depth seed]
  *   example_box <out.raw> --scene <file.scene> [spp_x spp_y mode max_depth seed]
  *   example_box <out.raw> --mlt <metropolis|metropolisflt|energyredist> [width height mutations seed]
+ *   example_box <out.raw> --rays <rays.raw> [mode max_depth key seed]
  *
  * The second form reads the scene and its camera from a scene file
  * (SceneFile, the reference's scene language); the third renders the box
  * with MetropolisRenderer or EnergyRedistributionRenderer.  Writes
- * height * width * 3 float32 RGB and prints the mean radiance.
+ * height * width * 3 float32 RGB and prints the mean radiance.  The fourth
+ * integrates the rays of rays.raw (8 float32 per ray: origin, range start,
+ * direction, range end) in the box one by one through the integrator's
+ * per-ray entry and writes 3 float32 per ray.
  */
 #include <cmath>
 #include <cstdio>
@@ -72,6 +76,38 @@ int main(int argc, char** argv)
             return write_film(argv[1], renderer.render(file.camera()));
         } catch (const rtgpu::RtError& e) {
             std::fprintf(stderr, "error %d: %s\n", e.status, e.what());
+            return 1;
+        }
+    }
+    if (argc > 3 && std::string(argv[2]) == "--rays") {
+        int mode = argc > 4 ? std::atoi(argv[4]) : RT_MODE_COMPAT, depth = argc > 5 ? std::atoi(argv[5]) : 8;
+        uint32_t key = argc > 6 ? (uint32_t)std::strtoul(argv[6], nullptr, 10) : 0;
+        unsigned long seed = argc > 7 ? std::strtoul(argv[7], nullptr, 10) : 3;
+        std::vector<float> raw;
+        if (FILE* f = std::fopen(argv[3], "rb")) {
+            float v[8];
+            while (std::fread(v, sizeof(float), 8, f) == 8) raw.insert(raw.end(), v, v + 8);
+            std::fclose(f);
+        }
+        try {
+            rtgpu::SceneBuilder s;
+            box(s, 8, 8);
+            rtgpu::GpuUniDiPathTracingIntegrator integ(0.5f, mode, depth);
+            integ.setScene(s);
+            std::vector<float> out;
+            for (size_t i = 0; i < raw.size() / 8; ++i) {
+                const float* v = &raw[8 * i];
+                rtgpu::Ray r;
+                r.o = {v[0], v[1], v[2]};
+                r.from = v[3];
+                r.d = {v[4], v[5], v[6]};
+                r.to = v[7];
+                rtgpu::Vec3 c = integ.integrate(r, key + (uint32_t)i, seed);
+                out.insert(out.end(), {c.x, c.y, c.z});
+            }
+            return write_film(argv[1], out);
+        } catch (const rtgpu::RtError& e) {
+            std::fprintf(stderr, "%s\n", e.what());
             return 1;
         }
     }

@@ -11,7 +11,9 @@
  *   Integrator                    the integrator interface, batch form
  *                                 (src/integrator.h:32-37)
  *   GpuUniDiPathTracingIntegrator UniDiPathTracingIntegrator(terminationProb)
- *                                 (src/unidipathtracingintegrator.h:30) on the GPU
+ *                                 (src/unidipathtracingintegrator.h:30) on the GPU:
+ *                                 films, and integrate(ray) per ray
+ *                                 (src/unidipathtracingintegrator.cpp:34-46)
  *   PerPixelRenderer              PerPixelRenderer::render (src/perpixelrenderer.cpp:43-101)
  *                                 with pt-x-samples / pt-y-samples / seed
  *   SceneFile                     a scene file read by the library's front end
@@ -54,6 +56,12 @@ inline void check(rt_ctx* ctx, int status, const char* call)
 
 struct Vec3 {
     float x, y, z;
+};
+
+/* src/ray.h:36-111: origin, direction, parameter range [from, to] */
+struct Ray {
+    Vec3 o, d;
+    float from = 0.001f, to = __builtin_inff();
 };
 
 /* Scene objects in file order; primitive IDs follow the reference's root
@@ -270,6 +278,33 @@ public:
         p.seed = (uint32_t)seed;
         float* films[1] = {film};
         check(ctx_, rt_render(ctx_, &v, 1, &p, films), "rt_render");
+    }
+    /* Integrator::integrate(const Ray&, scene, lights, rng): the radiance
+     * along each ray (RGB, 3 floats per ray).  Ray i draws the random numbers
+     * keyed by key + i, so a caller holding the reference's gsl_rng passes
+     * gsl_rng_get(rng) as the key. */
+    std::vector<float> integrate(const std::vector<Ray>& rays, uint32_t key, unsigned long seed)
+    {
+        std::vector<float> o(rays.size() * 4), d(rays.size() * 4), rgb(rays.size() * 3);
+        for (size_t i = 0; i < rays.size(); ++i) {
+            const Ray& r = rays[i];
+            o[4 * i] = r.o.x, o[4 * i + 1] = r.o.y, o[4 * i + 2] = r.o.z, o[4 * i + 3] = r.from;
+            d[4 * i] = r.d.x, d[4 * i + 1] = r.d.y, d[4 * i + 2] = r.d.z, d[4 * i + 3] = r.to;
+        }
+        rt_params p{};
+        p.mode = mode_;
+        p.max_depth = depth_;
+        p.rr_start = 3;
+        p.rr_prob = p_;
+        p.seed = (uint32_t)seed;
+        check(ctx_, rt_integrate_rays(ctx_, o.data(), d.data(), (int64_t)rays.size(), &p, key, rgb.data()),
+              "rt_integrate_rays");
+        return rgb;
+    }
+    Vec3 integrate(const Ray& ray, uint32_t key, unsigned long seed)
+    {
+        std::vector<float> c = integrate(std::vector<Ray>{ray}, key, seed);
+        return Vec3{c[0], c[1], c[2]};
     }
     void stop() { rt_request_stop(ctx_, 1); }
     rt_ctx* context() { return ctx_; }

@@ -127,6 +127,22 @@ class GpuContext:
         p = self.make_params(**params)
         _check(self.lib, self.h, self.lib.rt_render_device(self.h, va, len(views), ctypes.byref(p), ptrs))
 
+    def integrate_rays(self, o_tmin, d_tmax, key0=0, **params):
+        """Integrator::integrate(const Ray&, ...) for each ray (n x 4 origin +
+        range start, n x 4 direction + range end): (n, 3) RGB."""
+        o, d = _f32(o_tmin), _f32(d_tmax)
+        n = len(o)
+        out = np.empty((n, 3), np.float32)
+        p = self.make_params(**params)
+        _check(self.lib, self.h, self.lib.rt_integrate_rays(self.h, o.ctypes.data, d.ctypes.data, n, ctypes.byref(p),
+                                                            int(key0) & 0xffffffff, out.ctypes.data))
+        return out
+
+    def integrate_rays_device(self, o_ptr, d_ptr, n, rgb_ptr, key0=0, **params):
+        p = self.make_params(**params)
+        _check(self.lib, self.h, self.lib.rt_integrate_rays_device(self.h, o_ptr, d_ptr, int(n), ctypes.byref(p),
+                                                                   int(key0) & 0xffffffff, rgb_ptr))
+
     # -- Markov-chain renderers (include/rt_mlt.h) --------------------------------
     @staticmethod
     def make_mlt_params(kind=_abi.MLT_METROPOLIS, max_depth=5, rr_start=3, termination_prob=0.5,

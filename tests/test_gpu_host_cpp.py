@@ -69,3 +69,26 @@ def test_cpp_markov_chain_renderers_match_python_host(gpu_ctx, tmp_path, kind):
     py = rend.render(sc)
     assert np.array_equal(cpp, py)
     assert cpp.mean() > 0
+
+
+@pytest.mark.parametrize("mode", [MODE_COMPAT, MODE_UNBIASED])
+def test_cpp_per_ray_integrate_matches_python_batch(gpu_ctx, tmp_path, mode):
+    """example_box --rays: GpuUniDiPathTracingIntegrator::integrate(ray, key)
+    called once per ray from C++ gives, ray for ray, the values of one
+    batched rt_integrate_rays call with keys key0 + i (the same stream)."""
+    from tests.helpers import random_rays
+
+    if not os.path.exists(EXAMPLE):
+        pytest.fail("renderer_amd/host/example_box is not built (run __graft_entry__.build())")
+    o4, d4 = random_rays(64, (0.05, 0.05, 0.05), (0.95, 0.95, 1.5), seed=8, normalize=False)
+    rays = tmp_path / "rays.raw"
+    np.concatenate([o4, d4], 1).astype(np.float32).tofile(rays)
+    out = tmp_path / "rgb.raw"
+    r = subprocess.run([EXAMPLE, str(out), "--rays", str(rays), str(mode), "8", "500", "13"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    cpp = np.fromfile(out, np.float32).reshape(-1, 3)
+    gpu_ctx.upload(scenes.box_scene(8, 8).flatten())
+    py = gpu_ctx.integrate_rays(o4, d4, mode=mode, max_depth=8, rr_start=3, rr_prob=0.5, seed=13, key0=500)
+    assert np.array_equal(cpp, py)
+    assert py.sum() > 0
