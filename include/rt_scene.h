@@ -142,6 +142,17 @@ typedef struct rt_scene_desc {
     int32_t n_textures;
     const rt_texture* textures;
     const float* tri_texcoords;
+
+    /* analytic objects under an instance transform (IntersectableInstance,
+     * src/intersectableinstance.cpp:24-30): 12 floats per object, the
+     * world-to-object affine map [A | b] row-major (object point =
+     * A x + b), or NULL when every object is given in world space.  A ray is
+     * mapped into object space before the object's program is evaluated
+     * (its parameter t is unchanged by an affine map) and the object-space
+     * normal n comes back as A^T n (the normal matrix of the instance).  An
+     * identity row set (A = I, b = 0) leaves the object in world space.
+     * Planes are always given in world space. */
+    const float* object_xform;
 } rt_scene_desc;
 
 #define RT_CSG_MAX_LEAVES 16

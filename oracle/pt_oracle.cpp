@@ -479,6 +479,69 @@ struct CsgObject : Intersectable {
     }
 };
 
+// IntersectableInstance::intersect, src/intersectableinstance.cpp:24-30: the
+// ray mapped into object space by the inverse transform (points A x + b,
+// directions A d; the range keeps its parameters), the object intersected
+// there, and the hit reported in world space: the normal through the
+// normal matrix A^T (src/hitrecord.h:79-84), the point o + t d of the world
+// ray, the object-space texture coordinates.  The box is the transformed
+// object box's 8 corners (createBoundingBox, :33-47).
+struct Instance : Intersectable {
+    Intersectable* inner;
+    float r[12];  // rows (A row, b) of the world-to-object map
+    Instance(Intersectable* obj, const float* w2o) : inner(obj)
+    {
+        std::copy(w2o, w2o + 12, r);
+        // object-to-world: the inverse of [A | b], in double
+        double a[3][4];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 4; ++j) a[i][j] = r[4 * i + j];
+        const double det = a[0][0] * (a[1][1] * a[2][2] - a[1][2] * a[2][1]) -
+                           a[0][1] * (a[1][0] * a[2][2] - a[1][2] * a[2][0]) +
+                           a[0][2] * (a[1][0] * a[2][1] - a[1][1] * a[2][0]);
+        double m[3][3];
+        m[0][0] = (a[1][1] * a[2][2] - a[1][2] * a[2][1]) / det;
+        m[0][1] = (a[0][2] * a[2][1] - a[0][1] * a[2][2]) / det;
+        m[0][2] = (a[0][1] * a[1][2] - a[0][2] * a[1][1]) / det;
+        m[1][0] = (a[1][2] * a[2][0] - a[1][0] * a[2][2]) / det;
+        m[1][1] = (a[0][0] * a[2][2] - a[0][2] * a[2][0]) / det;
+        m[1][2] = (a[0][2] * a[1][0] - a[0][0] * a[1][2]) / det;
+        m[2][0] = (a[1][0] * a[2][1] - a[1][1] * a[2][0]) / det;
+        m[2][1] = (a[0][1] * a[2][0] - a[0][0] * a[2][1]) / det;
+        m[2][2] = (a[0][0] * a[1][1] - a[0][1] * a[1][0]) / det;
+        const Box& ib = inner->box;
+        box = Box();
+        for (int c = 0; c < 8; ++c) {
+            const double x[3] = {(c & 1) ? ib.mx.x : ib.mn.x, (c & 2) ? ib.mx.y : ib.mn.y, (c & 4) ? ib.mx.z : ib.mn.z};
+            double w[3];
+            for (int i = 0; i < 3; ++i)
+                w[i] = m[i][0] * (x[0] - a[0][3]) + m[i][1] * (x[1] - a[1][3]) + m[i][2] * (x[2] - a[2][3]);
+            include_point(box, V3((float)w[0], (float)w[1], (float)w[2]));
+        }
+    }
+    V3 point(V3 x) const
+    {
+        return V3(((r[0] * x.x + r[1] * x.y) + r[2] * x.z) + r[3], ((r[4] * x.x + r[5] * x.y) + r[6] * x.z) + r[7],
+                  ((r[8] * x.x + r[9] * x.y) + r[10] * x.z) + r[11]);
+    }
+    V3 dir(V3 x) const
+    {
+        return V3((r[0] * x.x + r[1] * x.y) + r[2] * x.z, (r[4] * x.x + r[5] * x.y) + r[6] * x.z,
+                  (r[8] * x.x + r[9] * x.y) + r[10] * x.z);
+    }
+    V3 normal(V3 n) const
+    {
+        return V3((r[0] * n.x + r[4] * n.y) + r[8] * n.z, (r[1] * n.x + r[5] * n.y) + r[9] * n.z,
+                  (r[2] * n.x + r[6] * n.y) + r[10] * n.z);
+    }
+    Hit intersect(const Ray& ray) const override
+    {
+        Hit h = inner->intersect(Ray(point(ray.o), dir(ray.d), ray.from, ray.to));
+        if (!h.hits()) return h;
+        return make_hit(h.t, ray, h.mat, normal(h.n), h.prim, h.tc[0], h.tc[1]);
+    }
+};
+
 // src/intersectablelist.cpp:30-48: linear scan with ray shortening; ties go
 // to the later component.
 struct List : Intersectable {
@@ -2221,7 +2284,14 @@ void* oc_scene_create(const rt_scene_desc* d, int accel)
         }
         auto* obj = new CsgObject(std::move(st.back()), id);
         sc->owned.emplace_back(obj);
-        items.push_back({id, obj});
+        static const float identity[12] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0};
+        if (d->object_xform && !std::equal(identity, identity + 12, d->object_xform + 12 * (size_t)i)) {
+            auto* inst = new Instance(obj, d->object_xform + 12 * (size_t)i);
+            sc->owned.emplace_back(inst);
+            items.push_back({id, inst});
+        } else {
+            items.push_back({id, obj});
+        }
     }
     std::stable_sort(items.begin(), items.end(), [](const Item& a, const Item& b) { return a.id < b.id; });
     for (auto& it : items) {

@@ -47,6 +47,7 @@ class RtSceneDesc(ctypes.Structure):
         ("n_textures", ctypes.c_int32),
         ("textures", ctypes.c_void_p),
         ("tri_texcoords", ctypes.c_void_p),
+        ("object_xform", ctypes.c_void_p),
     ]
 
 

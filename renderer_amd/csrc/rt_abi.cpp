@@ -43,7 +43,7 @@ struct rt_ctx {
     bool has_scene = false;
     HostDevScene scene{};
     DevBuf nodes, prims, prim_id, mat, vnormals, geo, mats, lights, env_rgb, env_row, env_col, env_p, probe_rgb;
-    DevBuf obj_nodes, obj_first, tex_rgb, tex_info, tri_tc;
+    DevBuf obj_nodes, obj_first, obj_xf, leaf_obj, tex_rgb, tex_info, tri_tc;
     int bvh_depth = 0;
     int bvh_stack = 0;  /* most stack entries a traversal can need */
 
@@ -267,7 +267,7 @@ int rt_destroy(rt_ctx* c)
                      &c->env_rgb, &c->env_row, &c->env_col, &c->env_p, &c->probe_rgb, &c->rb_o, &c->rb_d, &c->rb_out0,
                      &c->rb_out1, &c->stack_ovf, &c->tm_list, &c->tm_count, &c->counters, &c->views, &c->tiles, &c->passes, &c->films, &c->g_theta,
                      &c->g_w16, &c->g_m, &c->g_v, &c->g_grad, &c->g_acc, &c->g_count, &c->g_rec, &c->g_aux,
-                     &c->g_open, &c->g_out, &c->g_in, &c->g_tmp_rec, &c->g_tmp_y, &c->obj_nodes, &c->obj_first,
+                     &c->g_open, &c->g_out, &c->g_in, &c->g_tmp_rec, &c->g_tmp_y, &c->obj_nodes, &c->obj_first, &c->obj_xf, &c->leaf_obj,
                      &c->tex_rgb, &c->tex_info, &c->tri_tc,
                      &c->mlt_cur, &c->mlt_prop, &c->mlt_val, &c->mlt_energy, &c->mlt_acc, &c->mlt_cnt,
                      &c->mlt_seed, &c->mlt_cdf, &c->mlt_nch, &c->mlt_src, &c->mlt_view, &c->flt_img, &c->flt_acc};
@@ -439,8 +439,8 @@ int rt_scene_upload(rt_ctx* c, const rt_scene_desc* d)
     }
     /* analytic objects: validated postfix programs; every leaf gets a shading
      * index after the BVH primitives */
-    std::vector<float> onodes;
-    std::vector<int32_t> ofirst;
+    std::vector<float> onodes, oxf;
+    std::vector<int32_t> ofirst, lobj;
     int n_leaves = 0;
     if (d->n_objects < 0) return set_err(c, RT_ERR_INVALID, "negative object count");
     if (d->n_objects > 0) {
@@ -504,6 +504,32 @@ int rt_scene_upload(rt_ctx* c, const rt_scene_desc* d)
                 }
                 std::memcpy(&o[7], &leaf_g, 4);
             }
+        /* instance transforms of objects (world-to-object rows), and the
+         * object of every leaf of a transformed object */
+        if (d->object_xform) {
+            static const float identity[12] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0};
+            bool any = false;
+            for (int i = 0; i < d->n_objects && !any; ++i) {
+                const float* x = d->object_xform + 12 * (size_t)i;
+                if (std::memcmp(x, identity, sizeof identity) == 0) continue;
+                if (d->csg_nodes[d->object_first[i]].op == RT_CSG_PLANE)
+                    return set_err(c, RT_ERR_INVALID, "a plane is given in world space, without a transform");
+                any = true;
+            }
+            if (any) {
+                oxf.assign(d->object_xform, d->object_xform + 12 * (size_t)d->n_objects);
+                lobj.assign(n_leaves, -1);
+                int leaf = 0;
+                for (int i = 0; i < d->n_objects; ++i) {
+                    const bool mapped = std::memcmp(d->object_xform + 12 * (size_t)i, identity, sizeof identity) != 0;
+                    for (int k = d->object_first[i]; k < d->object_first[i + 1]; ++k)
+                        if (d->csg_nodes[k].op <= RT_CSG_PLANE) {
+                            if (mapped) lobj[leaf] = i;
+                            ++leaf;
+                        }
+                }
+            }
+        }
     }
 
     rtb::BuildResult br;
@@ -663,6 +689,8 @@ int rt_scene_upload(rt_ctx* c, const rt_scene_desc* d)
         (r = upload(c, c->tex_info, tex_info.data(), tex_info.size() * 4)) ||
         (r = upload(c, c->tri_tc, tri_tc.data(), tri_tc.size() * 4)) ||
         (r = upload(c, c->obj_first, ofirst.data(), ofirst.size() * 4)) ||
+        (r = upload(c, c->obj_xf, oxf.data(), oxf.size() * 4)) ||
+        (r = upload(c, c->leaf_obj, lobj.data(), lobj.size() * 4)) ||
         (r = upload(c, c->mats, mats.data(), mats.size() * sizeof(RtMaterial))) ||
         (r = upload(c, c->lights, lights.data(), lights.size() * sizeof(RtLight))) ||
         (r = upload(c, c->env_rgb, ergb.data(), ergb.size() * 4)) ||
@@ -699,6 +727,8 @@ int rt_scene_upload(rt_ctx* c, const rt_scene_desc* d)
     s.obj_first = ofirst.empty() ? nullptr : (const int32_t*)c->obj_first.p;
     s.n_obj = d->n_objects;
     s.g_obj = G;
+    s.obj_xf = oxf.empty() ? nullptr : c->obj_xf.p;
+    s.leaf_obj = lobj.empty() ? nullptr : (const int32_t*)c->leaf_obj.p;
     s.tex_rgb = tex.empty() ? nullptr : c->tex_rgb.p;
     s.tex_info = tex_info.empty() ? nullptr : c->tex_info.p;
     s.tri_tc = tri_tc.empty() ? nullptr : (const float*)c->tri_tc.p;

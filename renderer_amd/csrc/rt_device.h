@@ -110,6 +110,12 @@ struct DevScene {
     const float4* obj_nodes;
     const int32_t* obj_first;
     int32_t n_obj, g_obj;
+    /* objects under an instance transform: per object three rows (A row,
+     * b) of the world-to-object map (null when no object has one), and per
+     * object leaf (shading index - g_obj) its object, or -1 for a leaf of an
+     * object in world space */
+    const float4* obj_xf;
+    const int32_t* leaf_obj;
     /* textures: texels (rgb, 0) of all images; per image (first texel,
      * width, height, 0); texture coordinates of the triangles (6 floats per
      * triangle) or null for the default ones */
@@ -245,6 +251,26 @@ __device__ __forceinline__ bool sphere_params(vec3 o, vec3 d, float4 c, float& t
     t1 = (-b + root) / (2.f * a);
     return true;
 }
+/* The world-to-object map of an instance, rows (A row, b): points A x + b,
+ * directions A d, and normals back to world space as A^T n (the normal
+ * matrix of the instance transform, src/hitrecord.h:79-84).  Written out so
+ * that the oracle's evaluation in the same order gives the same bits. */
+__device__ __forceinline__ vec3 xf_point(const float4* r, vec3 x)
+{
+    return mk(((r[0].x * x.x + r[0].y * x.y) + r[0].z * x.z) + r[0].w,
+              ((r[1].x * x.x + r[1].y * x.y) + r[1].z * x.z) + r[1].w,
+              ((r[2].x * x.x + r[2].y * x.y) + r[2].z * x.z) + r[2].w);
+}
+__device__ __forceinline__ vec3 xf_dir(const float4* r, vec3 x)
+{
+    return mk((r[0].x * x.x + r[0].y * x.y) + r[0].z * x.z, (r[1].x * x.x + r[1].y * x.y) + r[1].z * x.z,
+              (r[2].x * x.x + r[2].y * x.y) + r[2].z * x.z);
+}
+__device__ __forceinline__ vec3 xf_normal(const float4* r, vec3 n)
+{
+    return mk((r[0].x * n.x + r[1].x * n.y) + r[2].x * n.z, (r[0].y * n.x + r[1].y * n.y) + r[2].y * n.z,
+              (r[0].z * n.x + r[1].z * n.y) + r[2].z * n.z);
+}
 #pragma clang fp contract(fast)
 
 /* Value of a CSG program for the leaves whose bits are set in `inside`
@@ -280,9 +306,16 @@ __device__ __forceinline__ bool csg_value(const float4* __restrict__ nodes, int 
  * (outside, outside) to outside, so a parent's view of a child is the
  * child's value.  Returns the leaf's shading index and the box normal code. */
 __device__ __forceinline__ bool obj_hit(const float4* __restrict__ nd, const int32_t* __restrict__ obj_first, int obj,
-                                     vec3 o, vec3 d, float tmin, float tmax, float& t, int& g, float& code)
+                                     const float4* xf, vec3 o, vec3 d, float tmin, float tmax, float& t, int& g,
+                                     float& code)
 {
     const int first = obj_first[obj], count = obj_first[obj + 1] - first;
+    /* an instance (src/intersectableinstance.cpp:24-30): the program in
+     * object space, the ray mapped there; t is the same on both sides */
+    if (xf) {
+        o = xf_point(xf, o);
+        d = xf_dir(xf, d);
+    }
     if (count == 1 && __float_as_int(nd[2 * first + 1].z) == RT_CSG_PLANE) {
         if (!hit_plane(o, d, nd[2 * first], tmin, tmax, t)) return false;
         g = __float_as_int(nd[2 * first + 1].w);
@@ -414,7 +447,8 @@ __device__ __forceinline__ void obj_closest(const DevScene& S, Trav& T, float tm
     for (int i = 0; i < S.n_obj; ++i) {
         float t, code;
         int g;
-        if (obj_hit(S.obj_nodes, S.obj_first, i, T.o, T.d, T.tmin, T.g >= 0 ? T.best : tmax, t, g, code) &&
+        const float4* xf = S.obj_xf ? S.obj_xf + 3 * i : nullptr;
+        if (obj_hit(S.obj_nodes, S.obj_first, i, xf, T.o, T.d, T.tmin, T.g >= 0 ? T.best : tmax, t, g, code) &&
             (T.g < 0 || t < T.best)) {
             T.best = t;
             T.g = g;
@@ -430,7 +464,8 @@ __device__ __forceinline__ bool obj_any(const DevScene& S, vec3 o, vec3 d, float
     for (int i = 0; i < S.n_obj; ++i) {
         float t, code;
         int g;
-        if (obj_hit(S.obj_nodes, S.obj_first, i, o, d, tmin, tmax, t, g, code)) return true;
+        const float4* xf = S.obj_xf ? S.obj_xf + 3 * i : nullptr;
+        if (obj_hit(S.obj_nodes, S.obj_first, i, xf, o, d, tmin, tmax, t, g, code)) return true;
     }
     return false;
 }
@@ -688,12 +723,21 @@ __device__ __forceinline__ vec3 hit_normal(const DevScene& S, int32_t g, vec3 p,
     if (g < S.n_tri + S.n_sphere) return norm(p - xyz(gg));
     if (g < S.g_obj) return xyz(gg);
     /* object leaves: sphere (centre, radius), plane (unit normal, -2), box
-     * (-1: the normal of the entry slab, coded in u) */
-    if (gg.w > 0.f) return norm(p - xyz(gg));
+     * (-1: the normal of the entry slab, coded in u); a leaf of an instance
+     * has them in object space */
     if (gg.w == -2.f) return xyz(gg);
-    const int code = (int)u, axis = code >> 1;
-    const float sgn = (code & 1) ? 1.f : -1.f;
-    return mk(axis == 0 ? sgn : 0.f, axis == 1 ? sgn : 0.f, axis == 2 ? sgn : 0.f);
+    const int lo = S.leaf_obj ? S.leaf_obj[g - S.g_obj] : -1;
+    const float4* xf = lo >= 0 ? S.obj_xf + 3 * lo : nullptr;
+    vec3 n;
+    if (gg.w > 0.f) {
+        n = (xf ? xf_point(xf, p) : p) - xyz(gg);
+    } else {
+        const int code = (int)u, axis = code >> 1;
+        const float sgn = (code & 1) ? 1.f : -1.f;
+        n = mk(axis == 0 ? sgn : 0.f, axis == 1 ? sgn : 0.f, axis == 2 ? sgn : 0.f);
+        if (!xf) return n;
+    }
+    return norm(xf ? xf_normal(xf, n) : n);
 }
 
 /* Texture coordinates of a hit: triangles interpolate their vertices'
@@ -717,7 +761,9 @@ __device__ __forceinline__ void hit_texcoord(const DevScene& S, int32_t g, vec3 
     const float4 gg = S.geo[g];
     const bool sphere = g < S.n_tri + S.n_sphere || (g >= S.g_obj && gg.w > 0.f);
     if (!sphere) return;
-    const vec3 n = p - xyz(gg);
+    /* an instance reports the object-space coordinates of its hit */
+    const int lo = g >= S.g_obj && S.leaf_obj ? S.leaf_obj[g - S.g_obj] : -1;
+    const vec3 n = (lo >= 0 ? xf_point(S.obj_xf + 3 * lo, p) : p) - xyz(gg);
     s = (float)((double)acosf(n.z / gg.w) / M_PI);
     t = (float)((double)atan2f(n.x, n.y) / M_PI / 2.0 + 0.5);
 }

@@ -36,6 +36,8 @@ struct HostDevScene {
     const void* obj_nodes;
     const int32_t* obj_first;
     int32_t n_obj, g_obj;
+    const void* obj_xf;
+    const int32_t* leaf_obj;
     const void* tex_rgb;
     const void* tex_info;
     const float* tri_tc;

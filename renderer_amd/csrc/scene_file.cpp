@@ -1541,6 +1541,8 @@ struct Geometry {
     std::vector<int32_t> sph_mat, sph_grp;
     std::vector<rt_csg_node> csg;           // postfix programs of planes, boxes, CSG trees
     std::vector<int32_t> obj_first{0}, obj_grp;
+    std::vector<float> obj_xf;              // 12 per object: world-to-object [A | b], identity if none
+    bool any_obj_xf = false;
     int32_t groups = 0;
     std::string unsupported;
 };
@@ -1794,7 +1796,19 @@ private:
             if (xf) {
                 float s;
                 if (!similarity(*xf, s)) {
-                    if (g.unsupported.empty()) g.unsupported = "sphere under a non-similarity instance transform";
+                    /* an ellipsoid: the sphere as an analytic object in its
+                     * own space, the ray mapped there */
+                    float w2o[12];
+                    if (!world_to_object(*xf, w2o)) {
+                        if (g.unsupported.empty()) g.unsupported = "sphere under a singular or projective instance transform";
+                        break;
+                    }
+                    rt_csg_node nd{};
+                    nd.op = RT_CSG_SPHERE;
+                    nd.material = mat;
+                    nd.p[0] = c[0], nd.p[1] = c[1], nd.p[2] = c[2], nd.p[3] = r;
+                    g.csg.push_back(nd);
+                    add_object(g, id, w2o);
                     break;
                 }
                 V3 q = xf->map(V3{c[0], c[1], c[2]});
@@ -1834,8 +1848,7 @@ private:
                 }
             }
             g.csg.push_back(nd);
-            g.obj_first.push_back((int32_t)g.csg.size());
-            g.obj_grp.push_back(id);
+            add_object(g, id, nullptr);
             break;
         }
         case NodeKind::Box:
@@ -1847,14 +1860,30 @@ private:
             const size_t start = g.csg.size();
             int leaves = 0;
             std::string why;
-            if (!csg_program(n, xf, g, leaves, why) || leaves > RT_CSG_MAX_LEAVES) {
+            bool ok = csg_program(n, xf, g, leaves, why);
+            float w2o[12];
+            bool mapped = false;
+            if (!ok && xf && why.find("instance transform") != std::string::npos) {
+                /* leaves the transform cannot be folded into (a sphere under a
+                 * non-similarity, a rotated or sheared box): the program in
+                 * object space and the ray mapped into it */
+                g.csg.resize(start);
+                leaves = 0;
+                why.clear();
+                if (!world_to_object(*xf, w2o)) {
+                    why = "CSG object under a singular or projective instance transform";
+                } else {
+                    ok = csg_program(n, nullptr, g, leaves, why);
+                    mapped = ok;
+                }
+            }
+            if (!ok || leaves > RT_CSG_MAX_LEAVES) {
                 g.csg.resize(start);
                 if (why.empty()) why = "CSG object with more than 16 spheres and boxes";
                 if (g.unsupported.empty()) g.unsupported = why;
                 break;
             }
-            g.obj_first.push_back((int32_t)g.csg.size());
-            g.obj_grp.push_back(id);
+            add_object(g, id, mapped ? w2o : nullptr);
             break;
         }
         case NodeKind::Obj:
@@ -1927,6 +1956,29 @@ private:
         return true;
     }
 
+    // Close the object just appended to g.csg, with its world-to-object map
+    // (nullptr: the object is in world space).
+    static void add_object(Geometry& g, int id, const float* w2o)
+    {
+        static const float identity[12] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0};
+        g.obj_first.push_back((int32_t)g.csg.size());
+        g.obj_grp.push_back(id);
+        g.obj_xf.insert(g.obj_xf.end(), w2o ? w2o : identity, (w2o ? w2o : identity) + 12);
+        g.any_obj_xf |= w2o != nullptr;
+    }
+
+    // The inverse of an affine instance matrix as 3 x 4 rows [A | b]; false
+    // for a projective or singular matrix.
+    static bool world_to_object(const M4& M, float out[12])
+    {
+        if (M.m[3][0] != 0 || M.m[3][1] != 0 || M.m[3][2] != 0 || M.m[3][3] != 1) return false;
+        double inv[4][4];
+        if (!invert(M, inv)) return false;
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 4; ++j) out[4 * i + j] = (float)inv[i][j];
+        return true;
+    }
+
     static bool invert(const M4& M, double inv[4][4])
     {
         double a[4][8];
@@ -1986,6 +2038,7 @@ struct rt_scene_file {
     std::vector<int32_t> tri_vertices, tri_material, tri_prim, sph_material, sph_prim;
     std::vector<rt_csg_node> csg_nodes;
     std::vector<int32_t> object_first, object_prim;
+    std::vector<float> object_xform;
     std::vector<float> tri_tc;
     std::vector<std::vector<float>> tex_rgb;  // images of texture materials
     std::vector<rt_texture> textures;
@@ -2174,6 +2227,7 @@ int rt_scene_file_scene(rt_scene_file* f, rt_scene_desc* desc, rt_camera* camera
     f->tri_tc = g.tri_tc;
     f->csg_nodes = g.csg;
     f->object_first = g.obj_first;
+    f->object_xform = g.obj_xf;
     f->object_prim.resize(g.obj_grp.size());
     for (size_t i = 0; i < g.obj_grp.size(); ++i) f->object_prim[i] = nL + g.obj_grp[i];
 
@@ -2214,6 +2268,7 @@ int rt_scene_file_scene(rt_scene_file* f, rt_scene_desc* desc, rt_camera* camera
         desc->csg_nodes = f->csg_nodes.data();
         desc->object_first = f->object_first.data();
         desc->object_prim_id = f->object_prim.data();
+        desc->object_xform = g.any_obj_xf ? f->object_xform.data() : nullptr;
     }
     if (!f->env_rgb.empty()) {
         desc->env_width = f->env_w;

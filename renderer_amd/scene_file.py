@@ -59,6 +59,9 @@ class FileSceneArrays:
         self.object_first = view(d.object_first, d.n_objects + 1, np.int32) if d.n_objects else np.zeros(1, np.int32)
         self.object_prim = view(d.object_prim_id, d.n_objects, np.int32)
         self.csg_nodes = [nodes[i] for i in range(int(self.object_first[-1]))] if nodes else []
+        # world-to-object maps of instanced objects (n_objects x 3 x 4), or None
+        self.object_xform = (view(d.object_xform, d.n_objects, np.float32, 12).reshape(-1, 3, 4)
+                             if d.object_xform else None)
         self.camera = Camera(list(camera.eye), list(camera.look_at), list(camera.up), camera.fov, camera.width,
                              camera.height)
 

@@ -89,3 +89,50 @@ def test_mixed_scene_renders_like_the_oracle(gpu_ctx, mode):
     print(f"agreement {agree:.4f}, means {img.mean():.5f} {ref.mean():.5f}", flush=True)
     assert agree > 0.97
     assert abs(img.mean() - ref.mean()) <= 0.01 * ref.mean()
+
+
+INSTANCED = """camera = camera([0.3 0.8 4.5],[0 -0.3 0],[0 1 0],0.75,64,64);
+lights = lights{ arealight([-0.5 2.9 -0.5],[1 0 0],[0 0 1],[9 9 8]) pointlight([2 2 2],[2 2 2]) };
+intersectable = intersectables{
+  plane([0 1 0 1], diffuse([.7 .7 .7]))
+  instance(translate([-1.2 -0.3 -0.6]) * scale([1.5 0.6 0.9]), sphere([0 0 0], 0.6, phong([.4 .4 .2],[.3 .3 .3],30)))
+  instance(translate([0.8 -0.4 -0.4]) * rotate(35, [0 1 1]), box([-0.4 -0.4 -0.4],[0.4 0.4 0.4], diffuse([.7 .2 .1])))
+  instance(translate([0 -0.3 1]) * rotate(25, [1 0 0]) * scale([1 1.5 0.7]),
+           diff(box([-0.5 -0.5 -0.5],[0.5 0.5 0.5], diffuse([.1 .2 .7])), sphere([0 0 0], 0.6, mirror(1))))
+  quad([-3 -1 -3],[3 -1 -3],[3 3 -3],[-3 3 -3], diffuse([.7 .7 .7]))
+};
+"""
+
+
+def test_instanced_objects_hit_like_the_oracle(gpu_ctx):
+    """An ellipsoid, a rotated box and a rotated, scaled CSG difference (the
+    ray mapped into object space on both sides with the same arithmetic, so
+    t agrees bit for bit)."""
+    from renderer_amd.scene_file import SceneFile
+
+    sf = SceneFile.parse(INSTANCED)
+    arr = sf.arrays()
+    assert arr.object_xform is not None
+    gpu_ctx.upload(arr)
+    o, d = _random_rays(np.random.default_rng(5), 100000)
+    closest, anyhit, frac = _compare_hits(gpu_ctx, arr, o, d)
+    print(f"instanced objects: closest agreement {closest:.6f}, any {anyhit:.6f}, hit fraction {frac:.3f}",
+          flush=True)
+    assert closest > 0.9995 and anyhit > 0.9995 and frac > 0.5
+
+
+@pytest.mark.parametrize("mode", [MODE_COMPAT, MODE_UNBIASED])
+def test_instanced_objects_render_like_the_oracle(gpu_ctx, mode):
+    from renderer_amd.scene_file import SceneFile
+
+    sf = SceneFile.parse(INSTANCED)
+    arr = sf.arrays()
+    gpu_ctx.upload(arr)
+    view = (arr.camera, 4, 4)
+    print(f"instanced scene render, mode {mode}", flush=True)
+    img = gpu_ctx.render([view], mode=mode, max_depth=6, rr_start=3, rr_prob=0.5, seed=23)[0]
+    ref, _ = Oracle(arr).render(view, mode=mode, max_depth=6, rr_start=3, rr_prob=0.5, seed=23)
+    agree = np.all(np.abs(img - ref) <= 1e-3 * np.abs(ref) + 1e-4, axis=2).mean()
+    print(f"agreement {agree:.4f}, means {img.mean():.5f} {ref.mean():.5f}", flush=True)
+    assert agree > 0.97
+    assert abs(img.mean() - ref.mean()) <= 0.01 * ref.mean()

@@ -207,14 +207,12 @@ def test_parse_errors_name_the_position():
 
 
 def test_unsupported_constructs_parse_and_dump_but_do_not_render():
+    """Singular instance transforms have no object space to map rays into."""
     cases = {
-        "instance(rotate(30, [0 1 0]), box([0 0 0],[1 1 1], diffuse([1 1 1])))": "box under",
-        "instance(rotate(30, [0 1 0]), diff(box([0 0 0],[1 1 1], diffuse([1 1 1])), sphere([0 0 0],1,mirror(1))))":
-            "box under",
-        "instance(scale([1 2 1]), sphere([0 0 0], 1, diffuse([1 1 1])))": "non-similarity",
-        "instance(scale([1 2 1]), union(sphere([0 0 0],1,diffuse([1 1 1])), box([0 0 0],[1 1 1],mirror(1))))":
-            "non-similarity",
         "instance(scale([1 0 1]), plane([0 1 0 1], diffuse([1 1 1])))": "singular",
+        "instance(scale([1 0 1]), sphere([0 0 0], 1, diffuse([1 1 1])))": "singular",
+        "instance(scale([0 1 1]), diff(box([0 0 0],[1 1 1], diffuse([1 1 1])), sphere([0 0 0],1,mirror(1))))":
+            "singular",
     }
     for obj, word in cases.items():
         sf = SceneFile.parse("camera = camera([0 0 5],[0 0 0],[0 1 0],0.5,8,8);\n"
