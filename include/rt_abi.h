@@ -119,10 +119,35 @@ int rt_trace_any_device(rt_ctx* ctx, const float* o_tmin, const float* d_tmax, i
 int rt_trace_closest_device_noalloc(rt_ctx* ctx, const float* o_tmin, const float* d_tmax, int64_t n,
                                     int32_t* prim, float* t);
 
+/* The 4-wide BVH that rt_scene_upload builds for a description, built on
+ * the host (no context, no device), over its triangles, spheres and area
+ * lights (analytic objects are tested outside the BVH).  Node i (root 0):
+ * 32 floats = lo.x[4] hi.x[4] lo.y[4] hi.y[4] lo.z[4] hi.z[4] of its child
+ * boxes, 4 child words (int bits), 4 zeros; a word >= 0 is an interior node,
+ * a word < 0 a leaf, ~word = first record << 5 | kind << 3 | count (kind 0
+ * triangle, 1 sphere, 2 parallelogram; an unused slot has an empty box and
+ * count 0).  Record (12 floats, leaf order): (p, g bits) (e1, 0) (e2, 0) for
+ * a triangle (vertex 0, edges to vertices 1 and 2), (c, g bits) (r, 0, 0, 0)
+ * (0) for a sphere, (corner, g bits) (u, 0) (v, 0) for a parallelogram; g is
+ * the primitive's index: triangles, then spheres, then area lights in light
+ * order.  On entry *n_nodes and *n_records are the capacities of
+ * the buffers in nodes and records, on return the counts; null buffers ask
+ * for the counts only, and a buffer below its count is RT_ERR_INVALID.
+ * stack_need (may be null): most stack entries a traversal holds at once.
+ * No counterpart in the reference (its BVH is private to the CPU renderer,
+ * src/bvh.cpp): it lets a CPU traversal of the same layout count what the
+ * device traversal visits. */
+int rt_bvh_export(const rt_scene_desc* desc, float* nodes, int64_t* n_nodes, float* records, int64_t* n_records,
+                  int32_t* stack_need);
+
 /* Traversal statistics of a ray batch (device pointers): interior nodes
  * visited and primitives tested per ray by the closest-hit traversal. */
 int rt_trace_count_device(rt_ctx* ctx, const float* o_tmin, const float* d_tmax, int64_t n,
                           int32_t* nodes, int32_t* prims);
+/* The same for the any-hit traversal of shadow rays (stops at the first hit,
+ * children unordered). */
+int rt_trace_any_count_device(rt_ctx* ctx, const float* o_tmin, const float* d_tmax, int64_t n,
+                              int32_t* nodes, int32_t* prims);
 
 /* Render.  films[v] receives view v's image, height*width*3 RGB float,
  * accumulated as film += sample / spp in sample order (the caller zeroes it

@@ -41,6 +41,7 @@ def load():
     lib.oc_scene_destroy.argtypes = [P]
     lib.oc_trace_closest.argtypes = [P, I64, P, P, P, P, ctypes.c_int]
     lib.oc_trace_any.argtypes = [P, I64, P, P, P, ctypes.c_int]
+    lib.oc_flat_trace.argtypes = [P, I64, P, I64, P, P, ctypes.c_int, P, P, P, P, ctypes.c_int]
     lib.oc_hit_normals.argtypes = [P, I64, P, P, P]
     lib.oc_hit_texcoords.argtypes = [P, I64, P, P, P]
     lib.oc_texture_albedo.restype = ctypes.c_int
@@ -98,6 +99,28 @@ def tonemap(film, gamma=2.2):
     out = np.zeros(f.shape, np.uint8)
     lib.oc_tonemap(f.ctypes.data, f.size, float(gamma), out.ctypes.data)
     return out
+
+
+def flat_trace(nodes, records, o_tmin, d_tmax, any_hit=False, threads=8):
+    """Traversal of an exported 4-wide BVH in the device traversal's order.
+    Closest hit: (g, t, visits, tests) with g the primitive index (-1: miss);
+    any hit: (occluded, visits, tests)."""
+    lib = load()
+    nodes = np.ascontiguousarray(nodes, np.float32)
+    records = np.ascontiguousarray(records, np.float32)
+    o = np.ascontiguousarray(o_tmin, np.float32)
+    d = np.ascontiguousarray(d_tmax, np.float32)
+    n = len(o)
+    g = np.zeros(n, np.int32)
+    t = np.zeros(n, np.float32)
+    visits = np.zeros(n, np.int32)
+    tests = np.zeros(n, np.int32)
+    lib.oc_flat_trace(nodes.ctypes.data, len(nodes), records.ctypes.data, n, o.ctypes.data, d.ctypes.data,
+                      int(bool(any_hit)), g.ctypes.data, t.ctypes.data, visits.ctypes.data, tests.ctypes.data,
+                      int(threads))
+    if any_hit:
+        return g.astype(np.uint8), visits, tests
+    return g, t, visits, tests
 
 
 class Oracle:

@@ -2601,3 +2601,193 @@ int oc_render_mlt(void* s, const rt_camera* camera, const rt_mlt_params* p, floa
 }
 
 }  // extern "C"
+
+/* ---------------------------------------------------------------------------
+ * Traversal of the flattened 4-wide BVH (the layout rt_bvh_export writes),
+ * restating the device traversal's visiting order (trav_step and traverse in
+ * renderer_amd/csrc/rt_device.h) so that the nodes it fetches and the
+ * records it tests can be counted on the CPU.  The reference has no such
+ * tree (its BVH is binary, leaf = 1 primitive, both children visited,
+ * src/bvh.cpp); the primitive tests are the reference's formulas
+ * (src/triangle.cpp:30-56 on the un-normalised normal, the parallelogram of
+ * src/arealight.cpp:73-90 as a triangle whose v runs to 1, the sphere of
+ * src/sphere.cpp:29-56), evaluated in the written order, as on the device.
+ * ------------------------------------------------------------------------- */
+namespace flat {
+
+enum { KIND_TRI = 0, KIND_SPHERE = 1, KIND_PARA = 2 }; /* leaf word kinds */
+
+struct F3 {
+    float x, y, z;
+};
+inline F3 sub(F3 a, F3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline float dotf(F3 a, F3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline F3 crossf(F3 a, F3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+
+/* Accepts tmin <= t <= tmax and t < best (the device's exact test after its
+ * widened reciprocal prefilter, which never drops an accepted hit). */
+inline bool tri(F3 o, F3 d, const float* r, float tmin, float tmax, float best, bool para, float& t)
+{
+    F3 p1{r[0], r[1], r[2]}, e1{r[4], r[5], r[6]}, e2{r[8], r[9], r[10]};
+    F3 ad = sub(o, p1);
+    F3 n = crossf(e1, e2);
+    float w = -dotf(n, d);
+    float tn = dotf(n, ad);
+    F3 adxde = crossf(ad, d);
+    float un = dotf(adxde, e2);
+    float vn = -dotf(adxde, e1);
+    if (w == 0.f) return false;
+    t = tn / w;
+    float u = un / w, v = vn / w;
+    bool ok = (t >= tmin) && (t <= tmax) && (t < best) && (u >= 0.f) && (u <= 1.f) && (v >= 0.f);
+    return ok && (para ? (v <= 1.f) : (u + v <= 1.f));
+}
+
+inline bool sphere(F3 o, F3 d, const float* r, float tmin, float tmax, float best, float& t)
+{
+    F3 c{r[0], r[1], r[2]};
+    float rad = r[4];
+    F3 diff = sub(o, c);
+    float a = dotf(d, d);
+    float b = 2.f * dotf(d, diff);
+    float cc = dotf(diff, diff) - rad * rad;
+    float disc = b * b - 4.f * a * cc;
+    if (!(disc >= 0.f)) return false;
+    float root = std::sqrt(disc);
+    float t0 = (-b - root) / (2.f * a), t1 = (-b + root) / (2.f * a);
+    if (tmin > t1) return false;
+    t = t0 >= tmin ? t0 : t1;
+    return t <= tmax && t < best;
+}
+
+/* Order-preserving map of a float onto uint32 (negatives reversed below the
+ * positives). */
+inline uint32_t sortable(float f)
+{
+    uint32_t b;
+    std::memcpy(&b, &f, 4);
+    return b ^ ((uint32_t)((int32_t)b >> 31) | 0x80000000u);
+}
+
+inline float safe_inv(float x)
+{
+    const float tiny = 1e-30f;
+    return 1.0f / (std::fabs(x) < tiny ? std::copysign(tiny, x) : x);
+}
+
+struct Result {
+    float t;
+    int32_t g;
+    int visits, tests;
+};
+
+Result trace(const float* nodes, int64_t n_nodes, const float* recs, F3 o, F3 d, float tmin, float tmax, bool any)
+{
+    const float inf = std::numeric_limits<float>::infinity();
+    const float slack = 9.5367431640625e-07f; /* 2^-20 */
+    Result R{inf, -1, 0, 0};
+    const F3 inv{safe_inv(d.x), safe_inv(d.y), safe_inv(d.z)};
+    /* best starts one ulp above tmax: "t < best" then also enforces t <= tmax */
+    float best = std::nextafter(tmax, inf);
+    int64_t cur = n_nodes > 0 && tmin <= tmax ? 0 : -1;
+    std::vector<int32_t> stack;
+    const int near_x = inv.x >= 0.f ? 0 : 4, near_y = inv.y >= 0.f ? 0 : 4, near_z = inv.z >= 0.f ? 0 : 4;
+    while (cur >= 0) {
+        const float* nd = nodes + 32 * cur;
+        ++R.visits;
+        const float bound = best + std::fabs(best) * slack;
+        float key[4];
+        int32_t word[4];
+        for (int k = 0; k < 4; ++k) {
+            const float nx = nd[near_x + k], fx = nd[(4 - near_x) + k];
+            const float ny = nd[8 + near_y + k], fy = nd[8 + (4 - near_y) + k];
+            const float nz = nd[16 + near_z + k], fz = nd[16 + (4 - near_z) + k];
+            float tl = std::fmax(std::fmax((nx - o.x) * inv.x, (ny - o.y) * inv.y), std::fmax((nz - o.z) * inv.z, tmin));
+            float th = std::fmin(std::fmin((fx - o.x) * inv.x, (fy - o.y) * inv.y), std::fmin((fz - o.z) * inv.z, bound));
+            key[k] = tl <= th + std::fabs(th) * slack ? tl : inf;
+            std::memcpy(&word[k], &nd[24 + k], 4);
+        }
+        uint32_t uk[4];
+        if (!any) {
+            /* ascending entry distance, slot in the two low bits as the tie
+             * break; the child words follow their slots */
+            uint32_t u[4];
+            for (int k = 0; k < 4; ++k) u[k] = (sortable(key[k]) & ~3u) | (uint32_t)k;
+            std::sort(u, u + 4);
+            int32_t w[4];
+            for (int j = 0; j < 4; ++j) {
+                w[j] = word[u[j] & 3u];
+                uk[j] = u[j] & ~3u;
+            }
+            std::memcpy(word, w, sizeof w);
+        } else {
+            for (int k = 0; k < 4; ++k) uk[k] = sortable(key[k]);
+        }
+        const uint32_t miss = sortable(inf) & ~3u;
+        bool done = false;
+        /* leaves, nearest first; records in leaf order */
+        for (int k = 0; k < 4 && !done; ++k) {
+            if (!(word[k] < 0 && uk[k] < miss)) continue;
+            if (!(uk[k] <= sortable(best + std::fabs(best) * slack))) continue;
+            const uint32_t x = ~(uint32_t)word[k];
+            const uint32_t count = x & 7u, kind = (x >> 3) & 3u, first = x >> 5;
+            for (uint32_t j = 0; j < count; ++j) {
+                const float* r = recs + 12 * (size_t)(first + j);
+                ++R.tests;
+                float t = 0.f;
+                bool hit = kind == KIND_SPHERE ? sphere(o, d, r, tmin, inf, best, t)
+                                               : tri(o, d, r, tmin, inf, best, kind == KIND_PARA, t);
+                if (hit) {
+                    best = t;
+                    std::memcpy(&R.g, &r[3], 4);
+                    if (any) {
+                        done = true;
+                        break;
+                    }
+                }
+            }
+        }
+        if (done) break;
+        /* interior children: the nearest continues, the others are deferred
+         * with the next nearest on top */
+        int32_t next = -1;
+        const uint32_t cut = sortable(best + std::fabs(best) * slack);
+        for (int k = 3; k >= 0; --k) {
+            if (word[k] < 0 || !(uk[k] < miss) || !(uk[k] <= cut)) continue;
+            if (next >= 0) stack.push_back(next);
+            next = word[k];
+        }
+        if (next >= 0) {
+            cur = next;
+        } else if (stack.empty()) {
+            cur = -1;
+        } else {
+            cur = stack.back();
+            stack.pop_back();
+        }
+    }
+    if (R.g >= 0) R.t = best;
+    return R;
+}
+
+}  // namespace flat
+
+extern "C" void oc_flat_trace(const float* nodes, int64_t n_nodes, const float* records, int64_t n,
+                              const float* o_tmin, const float* d_tmax, int any, int32_t* g, float* t,
+                              int32_t* visits, int32_t* tests, int threads)
+{
+#pragma omp parallel for schedule(dynamic, 256) num_threads(threads > 0 ? threads : 1)
+    for (int64_t i = 0; i < n; ++i) {
+        const float* a = o_tmin + 4 * i;
+        const float* b = d_tmax + 4 * i;
+        flat::Result R = flat::trace(nodes, n_nodes, records, {a[0], a[1], a[2]}, {b[0], b[1], b[2]}, a[3], b[3], any != 0);
+        if (any) {
+            g[i] = R.g >= 0 ? 1 : 0;
+        } else {
+            g[i] = R.g;
+            t[i] = R.g >= 0 ? R.t : std::numeric_limits<float>::quiet_NaN();
+        }
+        if (visits) visits[i] = R.visits;
+        if (tests) tests[i] = R.tests;
+    }
+}

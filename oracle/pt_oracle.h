@@ -55,6 +55,15 @@ void oc_trace_closest(void* scene, int64_t n, const float* o_tmin, const float* 
 void oc_trace_any(void* scene, int64_t n, const float* o_tmin, const float* d_tmax,
                   uint8_t* occluded, int threads);
 
+/* Traversal of a flattened 4-wide BVH as rt_bvh_export writes it (nodes,
+ * records), in the order the device traversal visits it: closest hit
+ * (any = 0: g[i] = record's primitive index or -1, t[i] = its parameter or
+ * NaN) or any hit (any = 1: g[i] = 1 occluded, 0 not; t untouched).
+ * visits[i] / tests[i]: interior nodes fetched and primitive records tested
+ * by ray i.  Rays as for oc_trace_closest. */
+void oc_flat_trace(const float* nodes, int64_t n_nodes, const float* records, int64_t n, const float* o_tmin,
+                   const float* d_tmax, int any, int32_t* g, float* t, int32_t* visits, int32_t* tests, int threads);
+
 /* Texture coordinates of each ray's closest hit (NaN for a miss), and the
  * albedo pow(image(s, t) * coefficient, gamma) of texture material `mat`
  * (returns -1 when it is not a texture). */

@@ -156,6 +156,9 @@ PROTOTYPES = {
     "rt_trace_any": (ctypes.c_int, [_P, _P, _P, _I64, _P]),
     "rt_trace_any_device": (ctypes.c_int, [_P, _P, _P, _I64, _P]),
     "rt_trace_count_device": (ctypes.c_int, [_P, _P, _P, _I64, _P, _P]),
+    "rt_trace_any_count_device": (ctypes.c_int, [_P, _P, _P, _I64, _P, _P]),
+    "rt_bvh_export": (ctypes.c_int, [ctypes.POINTER(RtSceneDesc), _P, ctypes.POINTER(ctypes.c_int64), _P,
+                                     ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int32)]),
     "rt_trace_closest_device_noalloc": (ctypes.c_int, [_P, _P, _P, _I64, _P, _P]),
     "rt_render": (ctypes.c_int, [_P, ctypes.POINTER(RtView), _I32, ctypes.POINTER(RtParams), _P]),
     "rt_render_device": (ctypes.c_int, [_P, ctypes.POINTER(RtView), _I32, ctypes.POINTER(RtParams), _P]),

@@ -325,26 +325,25 @@ int rt_get_stats(rt_ctx* c, rt_stats* out)
     return RT_OK;
 }
 
-int rt_scene_upload(rt_ctx* c, const rt_scene_desc* d)
+/* Checks the primitive, material and light arrays of a scene description;
+ * *env_out receives the index of the environment light or -1. */
+static int check_prims(rt_ctx* c, const rt_scene_desc* d, int* env_out)
 {
-    if (!c || !d) return RT_ERR_INVALID;
-    if (int r = check_fault(c)) return r;
     if (d->n_vertices < 0 || d->n_triangles < 0 || d->n_spheres < 0 || d->n_materials < 0 || d->n_lights < 0)
         return set_err(c, RT_ERR_INVALID, "negative count");
     if ((d->n_triangles && (!d->tri_vertices || !d->tri_material || !d->tri_prim_id || !d->positions)) ||
         (d->n_spheres && (!d->spheres || !d->sphere_material || !d->sphere_prim_id)) ||
         (d->n_materials && !d->materials) || (d->n_lights && !d->lights))
         return set_err(c, RT_ERR_INVALID, "missing array");
-    hipSetDevice(c->device);
     const int nT = d->n_triangles, nS = d->n_spheres;
-    int nP = 0, env = -1;
+    int env = -1;
     for (int i = 0; i < d->n_lights; ++i) {
-        if (d->lights[i].type == RT_LIGHT_AREA) ++nP;
-        else if (d->lights[i].type == RT_LIGHT_ENV) {
+        if (d->lights[i].type == RT_LIGHT_ENV) {
             if (env >= 0 || !d->env_rgb || d->env_width <= 0 || d->env_height <= 0)
                 return set_err(c, RT_ERR_INVALID, "environment light needs exactly one map");
             env = i;
-        } else if (d->lights[i].type != RT_LIGHT_POINT && d->lights[i].type != RT_LIGHT_CONE)
+        } else if (d->lights[i].type != RT_LIGHT_AREA && d->lights[i].type != RT_LIGHT_POINT &&
+                   d->lights[i].type != RT_LIGHT_CONE)
             return set_err(c, RT_ERR_INVALID, "unknown light type");
     }
     for (int i = 0; i < nT; ++i) {
@@ -357,12 +356,23 @@ int rt_scene_upload(rt_ctx* c, const rt_scene_desc* d)
     for (int i = 0; i < nS; ++i)
         if (d->sphere_material[i] < 0 || d->sphere_material[i] >= d->n_materials)
             return set_err(c, RT_ERR_INVALID, "sphere material out of range");
-    const int G = nT + nS + nP;
+    *env_out = env;
+    return RT_OK;
+}
 
+/* The 4-wide BVH over a checked description's triangles, spheres and area
+ * lights (primitive index g: triangles, then spheres, then area lights in
+ * light order), and its primitive records in leaf order: 12 floats each,
+ * (p, g bits) (e1 or radius, 0) (e2, 0) for triangles (p = vertex 0,
+ * e = edges from it), spheres (centre, radius) and parallelograms (corner,
+ * u, v).  lo/hi: the union of the primitive boxes (+inf/-inf when empty). */
+static int build_bvh(rt_ctx* c, const rt_scene_desc* d, rtb::BuildResult& br, std::vector<float>& rec, float lo[3],
+                     float hi[3])
+{
+    const int nT = d->n_triangles, nS = d->n_spheres;
     std::vector<rtb::PrimRef> refs;
-    refs.reserve(G);
-    std::vector<float> geo((size_t)4 * G, 0.f);
-    std::vector<int32_t> pid(G), mat(G);
+    std::vector<int> light_of;  /* light index of area light j */
+    refs.reserve(nT + nS);
     auto P = [d](int v, int k) { return d->positions[3 * v + k]; };
     for (int i = 0; i < nT; ++i) {
         const int32_t* tv = d->tri_vertices + 3 * i;
@@ -374,6 +384,115 @@ int rt_scene_upload(rt_ctx* c, const rt_scene_desc* d)
         r.kind = RT_KIND_TRI;
         r.g = i;
         refs.push_back(r);
+    }
+    for (int i = 0; i < nS; ++i) {
+        const float* s = d->spheres + 4 * i;
+        rtb::PrimRef r;
+        for (int k = 0; k < 3; ++k) {
+            r.lo[k] = s[k] - s[3];
+            r.hi[k] = s[k] + s[3];
+        }
+        r.kind = RT_KIND_SPHERE;
+        r.g = nT + i;
+        refs.push_back(r);
+    }
+    for (int i = 0; i < d->n_lights; ++i) {
+        const rt_light& l = d->lights[i];
+        if (l.type != RT_LIGHT_AREA) continue;
+        rtb::PrimRef r;
+        for (int k = 0; k < 3; ++k) {
+            float c0 = l.p[k], c1 = l.p[k] + l.u[k], c2 = l.p[k] + l.v[k], c3 = l.p[k] + l.u[k] + l.v[k];
+            r.lo[k] = std::min(std::min(c0, c1), std::min(c2, c3));
+            r.hi[k] = std::max(std::max(c0, c1), std::max(c2, c3));
+        }
+        r.kind = RT_KIND_PARA;
+        r.g = nT + nS + (int)light_of.size();
+        light_of.push_back(i);
+        refs.push_back(r);
+    }
+    const int G = (int)refs.size();
+    for (int k = 0; k < 3; ++k) {
+        lo[k] = INFINITY;
+        hi[k] = -INFINITY;
+    }
+    for (const auto& r : refs)
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = std::min(lo[k], r.lo[k]);
+            hi[k] = std::max(hi[k], r.hi[k]);
+        }
+    int br_status = rtb::build(refs, br);
+    if (br_status != 0)
+        return set_err(c, RT_ERR_BUILD, br_status == -2 ? "BVH deeper than the traversal stack" : "too many primitives");
+    /* node fetches use 32-bit byte offsets from the node base */
+    if (br.nodes.size() * sizeof(float) > 0xffffffffull)
+        return set_err(c, RT_ERR_BUILD, "BVH larger than 4 GiB of nodes");
+    rec.assign((size_t)12 * G, 0.f);
+    for (size_t i = 0; i < refs.size(); ++i) {
+        int g = refs[i].g;
+        float* o = &rec[12 * i];
+        std::memcpy(&o[3], &g, 4);
+        if (g < nT) {
+            const int32_t* tv = d->tri_vertices + 3 * g;
+            for (int k = 0; k < 3; ++k) {
+                o[k] = P(tv[0], k);
+                o[4 + k] = P(tv[1], k) - P(tv[0], k);
+                o[8 + k] = P(tv[2], k) - P(tv[0], k);
+            }
+        } else if (g < nT + nS) {
+            const float* s = d->spheres + 4 * (g - nT);
+            for (int k = 0; k < 3; ++k) o[k] = s[k];
+            o[4] = s[3];
+        } else {
+            const rt_light& L = d->lights[light_of[g - nT - nS]];
+            for (int k = 0; k < 3; ++k) {
+                o[k] = L.p[k];
+                o[4 + k] = L.u[k];
+                o[8 + k] = L.v[k];
+            }
+        }
+    }
+    return RT_OK;
+}
+
+int rt_bvh_export(const rt_scene_desc* d, float* nodes, int64_t* n_nodes, float* records, int64_t* n_records,
+                  int32_t* stack_need)
+{
+    if (!d || !n_nodes || !n_records) return RT_ERR_INVALID;
+    int env = -1;
+    if (int r = check_prims(nullptr, d, &env)) return r;
+    rtb::BuildResult br;
+    std::vector<float> rec;
+    float lo[3], hi[3];
+    if (int r = build_bvh(nullptr, d, br, rec, lo, hi)) return r;
+    const int64_t nn = (int64_t)(br.nodes.size() / RT_NODE_FLOATS), nr = (int64_t)(rec.size() / 12);
+    const int64_t cap_n = *n_nodes, cap_r = *n_records;
+    *n_nodes = nn;
+    *n_records = nr;
+    if (stack_need) *stack_need = br.stack_need;
+    if ((nodes && cap_n < nn) || (records && cap_r < nr)) return RT_ERR_INVALID;
+    if (nodes) std::memcpy(nodes, br.nodes.data(), br.nodes.size() * sizeof(float));
+    if (records) std::memcpy(records, rec.data(), rec.size() * sizeof(float));
+    return RT_OK;
+}
+
+int rt_scene_upload(rt_ctx* c, const rt_scene_desc* d)
+{
+    if (!c || !d) return RT_ERR_INVALID;
+    if (int r = check_fault(c)) return r;
+    int env0 = -1;
+    if (int r = check_prims(c, d, &env0)) return r;
+    hipSetDevice(c->device);
+    const int nT = d->n_triangles, nS = d->n_spheres;
+    int nP = 0;
+    const int env = env0;
+    for (int i = 0; i < d->n_lights; ++i) nP += d->lights[i].type == RT_LIGHT_AREA;
+    const int G = nT + nS + nP;
+
+    std::vector<float> geo((size_t)4 * G, 0.f);
+    std::vector<int32_t> pid(G), mat(G);
+    auto P = [d](int v, int k) { return d->positions[3 * v + k]; };
+    for (int i = 0; i < nT; ++i) {
+        const int32_t* tv = d->tri_vertices + 3 * i;
         float a[3], b[3], n[3], nn[3];
         for (int k = 0; k < 3; ++k) {
             a[k] = P(tv[0], k) - P(tv[1], k);
@@ -389,14 +508,6 @@ int rt_scene_upload(rt_ctx* c, const rt_scene_desc* d)
     }
     for (int i = 0; i < nS; ++i) {
         const float* s = d->spheres + 4 * i;
-        rtb::PrimRef r;
-        for (int k = 0; k < 3; ++k) {
-            r.lo[k] = s[k] - s[3];
-            r.hi[k] = s[k] + s[3];
-        }
-        r.kind = RT_KIND_SPHERE;
-        r.g = nT + i;
-        refs.push_back(r);
         for (int k = 0; k < 4; ++k) geo[4 * (nT + i) + k] = s[k];
         pid[nT + i] = d->sphere_prim_id[i];
         mat[nT + i] = d->sphere_material[i];
@@ -421,15 +532,6 @@ int rt_scene_upload(rt_ctx* c, const rt_scene_desc* d)
             if (l.type != RT_LIGHT_AREA) continue;
             int g = nT + nS + j++;
             L.g = g;
-            rtb::PrimRef r;
-            for (int k = 0; k < 3; ++k) {
-                float c0 = l.p[k], c1 = l.p[k] + l.u[k], c2 = l.p[k] + l.v[k], c3 = l.p[k] + l.u[k] + l.v[k];
-                r.lo[k] = std::min(std::min(c0, c1), std::min(c2, c3));
-                r.hi[k] = std::max(std::max(c0, c1), std::max(c2, c3));
-            }
-            r.kind = RT_KIND_PARA;
-            r.g = g;
-            refs.push_back(r);
             float nn[3];
             qt_normalize(L.n, nn);
             for (int k = 0; k < 3; ++k) geo[4 * g + k] = nn[k];
@@ -533,39 +635,9 @@ int rt_scene_upload(rt_ctx* c, const rt_scene_desc* d)
     }
 
     rtb::BuildResult br;
-    int br_status = rtb::build(refs, br);
-    if (br_status != 0)
-        return set_err(c, RT_ERR_BUILD, br_status == -2 ? "BVH deeper than the traversal stack" : "too many primitives");
-    /* node fetches use 32-bit byte offsets from the node base */
-    if (br.nodes.size() * sizeof(float) > 0xffffffffull)
-        return set_err(c, RT_ERR_BUILD, "BVH larger than 4 GiB of nodes");
-
-    // primitive records in leaf order
-    std::vector<float> rec((size_t)12 * G, 0.f);
-    for (size_t i = 0; i < refs.size(); ++i) {
-        int g = refs[i].g;
-        float* o = &rec[12 * i];
-        std::memcpy(&o[3], &g, 4);
-        if (g < nT) {
-            const int32_t* tv = d->tri_vertices + 3 * g;
-            for (int k = 0; k < 3; ++k) {
-                o[k] = P(tv[0], k);
-                o[4 + k] = P(tv[1], k) - P(tv[0], k);
-                o[8 + k] = P(tv[2], k) - P(tv[0], k);
-            }
-        } else if (g < nT + nS) {
-            const float* s = d->spheres + 4 * (g - nT);
-            for (int k = 0; k < 3; ++k) o[k] = s[k];
-            o[4] = s[3];
-        } else {
-            const RtLight& L = lights[pid[g]];
-            for (int k = 0; k < 3; ++k) {
-                o[k] = L.p[k];
-                o[4 + k] = L.u[k];
-                o[8 + k] = L.v[k];
-            }
-        }
-    }
+    std::vector<float> rec;
+    float box_lo[3], box_hi[3];
+    if (int r = build_bvh(c, d, br, rec, box_lo, box_hi)) return r;
     std::vector<float> vn;
     if (d->normals && nT) {
         vn.resize((size_t)9 * nT);
@@ -741,12 +813,7 @@ int rt_scene_upload(rt_ctx* c, const rt_scene_desc* d)
     c->stats.bvh_stack_need = br.stack_need;
     // scene box (finite primitives) for the guiding network's position encoding
     {
-        float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-        for (const auto& r : refs)
-            for (int k = 0; k < 3; ++k) {
-                lo[k] = std::min(lo[k], r.lo[k]);
-                hi[k] = std::max(hi[k], r.hi[k]);
-            }
+        float lo[3] = {box_lo[0], box_lo[1], box_lo[2]}, hi[3] = {box_hi[0], box_hi[1], box_hi[2]};
         for (int k = 0; k < 3; ++k) {
             if (!(lo[k] <= hi[k])) { lo[k] = 0.f; hi[k] = 1.f; }
             float pad = 1e-3f * std::max(1e-3f, hi[k] - lo[k]);
@@ -779,7 +846,7 @@ static int trace_device(rt_ctx* c, int kind, const float* o, const float* d, int
         e = launch_trace_any(c->scene, (const float4*)o, (const float4*)d, n, (uint8_t*)out0, 256, grid, c->stream);
     else
         e = launch_trace_count(c->scene, (const float4*)o, (const float4*)d, n, (int32_t*)out0, (int32_t*)out1, 256,
-                               grid, c->stream);
+                               grid, c->stream, kind == 3);
     if (e != hipSuccess) return set_err(c, RT_ERR_HIP, std::string("trace launch: ") + hipGetErrorString(e));
     HIP_TRY(c, hipEventRecord(c->ev_b, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
@@ -809,6 +876,13 @@ int rt_trace_count_device(rt_ctx* c, const float* o, const float* d, int64_t n, 
     if (!c || n < 0 || (n && (!o || !d || !nodes || !prims))) return RT_ERR_INVALID;
     if (int r = check_fault(c)) return r;
     return trace_device(c, 2, o, d, n, nodes, prims);
+}
+
+int rt_trace_any_count_device(rt_ctx* c, const float* o, const float* d, int64_t n, int32_t* nodes, int32_t* prims)
+{
+    if (!c || n < 0 || (n && (!o || !d || !nodes || !prims))) return RT_ERR_INVALID;
+    if (int r = check_fault(c)) return r;
+    return trace_device(c, 3, o, d, n, nodes, prims);
 }
 
 /* rt_trace_closest_device without allocation: the launch is capped at

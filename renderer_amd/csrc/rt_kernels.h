@@ -196,8 +196,10 @@ hipError_t launch_trace_closest(const HostDevScene& s, const float4* o, const fl
                                 float* t, int block, int grid, hipStream_t st);
 hipError_t launch_trace_any(const HostDevScene& s, const float4* o, const float4* d, int64_t n, uint8_t* occ,
                             int block, int grid, hipStream_t st);
+/* Nodes fetched and records tested per ray, by the closest-hit (any = false)
+ * or the any-hit traversal. */
 hipError_t launch_trace_count(const HostDevScene& s, const float4* o, const float4* d, int64_t n, int32_t* nodes,
-                              int32_t* prims, int block, int grid, hipStream_t st);
+                              int32_t* prims, int block, int grid, hipStream_t st, bool any);
 
 /* Allocation-free closest hit.  The traversal-stack overflow of launches of
  * at most RT_NOALLOC_BLOCKS blocks in a static device array (RT_STACK_OVF

@@ -157,15 +157,15 @@ __global__ __launch_bounds__(PT_BLOCK) void k_trace(DevScene S, const float4* __
         float4 a = o[i], b = d[i];
         int cn = 0, cp = 0;
         TraceOut h = traverse<ANY, COUNT>(S, xyz(a), xyz(b), a.w, b.w, my, &cn, &cp);
-        if (ANY) {
+        /* counting launches write only the counts (the hit outputs are null) */
+        if (COUNT) {
+            nn[i] = cn;
+            np[i] = cp;
+        } else if (ANY) {
             occ[i] = h.g >= 0 ? 1 : 0;
         } else {
             prim[i] = h.g >= 0 ? S.prim_id[h.g] : -1;
             tout[i] = h.g >= 0 ? h.t : __builtin_nanf("");
-        }
-        if (COUNT) {
-            nn[i] = cn;
-            np[i] = cp;
         }
     }
 }
@@ -187,11 +187,15 @@ hipError_t launch_trace_any(const HostDevScene& s, const float4* o, const float4
     return hipGetLastError();
 }
 hipError_t launch_trace_count(const HostDevScene& s, const float4* o, const float4* d, int64_t n, int32_t* nodes,
-                              int32_t* prims, int block, int grid, hipStream_t st)
+                              int32_t* prims, int block, int grid, hipStream_t st, bool any)
 {
     (void)block;
-    hipLaunchKernelGGL((k_trace<false, true>), dim3(grid), dim3(PT_BLOCK), 0, st, as_dev(s), o, d, n,
-                       (int32_t*)nullptr, (float*)nullptr, (uint8_t*)nullptr, nodes, prims);
+    if (any)
+        hipLaunchKernelGGL((k_trace<true, true>), dim3(grid), dim3(PT_BLOCK), 0, st, as_dev(s), o, d, n,
+                           (int32_t*)nullptr, (float*)nullptr, (uint8_t*)nullptr, nodes, prims);
+    else
+        hipLaunchKernelGGL((k_trace<false, true>), dim3(grid), dim3(PT_BLOCK), 0, st, as_dev(s), o, d, n,
+                           (int32_t*)nullptr, (float*)nullptr, (uint8_t*)nullptr, nodes, prims);
     return hipGetLastError();
 }
 

@@ -31,6 +31,23 @@ def _f32(a, shape_last=4):
     return a
 
 
+def bvh_export(scene):
+    """The 4-wide BVH that GpuContext.upload builds for a scene, built on the
+    host with no device: (nodes float32 (n, 32), records float32 (m, 12),
+    stack_need).  Layout as rt_bvh_export in rt_abi.h states."""
+    lib = _abi.load()
+    arrays = scene.flatten() if isinstance(scene, Scene) else scene
+    desc = arrays.desc()
+    nn, nr, need = ctypes.c_int64(0), ctypes.c_int64(0), ctypes.c_int32(0)
+    _check(lib, None, lib.rt_bvh_export(ctypes.byref(desc), None, ctypes.byref(nn), None, ctypes.byref(nr),
+                                        ctypes.byref(need)))
+    nodes = np.zeros((max(nn.value, 1), 32), np.float32)
+    recs = np.zeros((max(nr.value, 1), 12), np.float32)
+    _check(lib, None, lib.rt_bvh_export(ctypes.byref(desc), nodes.ctypes.data, ctypes.byref(nn), recs.ctypes.data,
+                                        ctypes.byref(nr), ctypes.byref(need)))
+    return nodes[:nn.value], recs[:nr.value], need.value
+
+
 class GpuContext:
     def __init__(self, device=0):
         self.lib = _abi.load()
@@ -91,6 +108,10 @@ class GpuContext:
 
     def trace_count_device(self, o_ptr, d_ptr, n, nodes_ptr, prims_ptr):
         _check(self.lib, self.h, self.lib.rt_trace_count_device(self.h, o_ptr, d_ptr, int(n), nodes_ptr, prims_ptr))
+
+    def trace_any_count_device(self, o_ptr, d_ptr, n, nodes_ptr, prims_ptr):
+        _check(self.lib, self.h,
+               self.lib.rt_trace_any_count_device(self.h, o_ptr, d_ptr, int(n), nodes_ptr, prims_ptr))
 
     # -- render ------------------------------------------------------------------
     @staticmethod
