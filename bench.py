@@ -47,6 +47,7 @@ def parse():
                     help="also run a 1 GiB device-to-device copy, the reference point for HBM counters")
     ap.add_argument("--no-guided", action="store_true", help="skip the equal-time guided vs plain comparisons")
     ap.add_argument("--no-env", action="store_true", help="skip the env-lit mesh comparison")
+    ap.add_argument("--no-room", action="store_true", help="skip the occluded-light room comparison")
     ap.add_argument("--no-batched", action="store_true", help="skip the 32-view batched render")
     ap.add_argument("--no-scene-file", action="store_true", help="skip the scene-file front-end timing")
     ap.add_argument("--no-film", action="store_true", help="skip the tonemapper and film-writer timing")
@@ -101,6 +102,19 @@ def batched_views(args, scene, arrays):
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     st = ctx.stats()
+    # the same render with each pass replayed from one captured hipGraph
+    eager = [f.clone() for f in films]
+    for f in films:
+        f.zero_()
+    torch.cuda.synchronize()
+    ctx.set_launch_config(path_slots=tune["path_slots"], blocks_per_cu=tune["blocks_per_cu"], use_graph=1)
+    ctx.render_device(views, [f.data_ptr() for f in films], seed=1, **params)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ctx.render_device(views, [f.data_ptr() for f in films], seed=2, **params)
+    torch.cuda.synchronize()
+    dt_graph = time.perf_counter() - t0
+    same = all(bool(torch.equal(a, b)) for a, b in zip(eager, films))
     ctx.close()
     rays = st["camera_rays"] + st["extension_rays"] + st["shadow_rays"]
     return {
@@ -113,6 +127,8 @@ def batched_views(args, scene, arrays):
         "passes": st["passes"],
         "autotuned": {"path_slots": tune["path_slots"], "blocks_per_cu": tune["blocks_per_cu"],
                       "trials": len(tune["trials"]), "tuning_seconds": round(tune_s, 1)},
+        "graph": {"seconds": round(dt_graph, 3), "mrays_per_s": round(rays / dt_graph / 1e6, 2),
+                  "films_equal_eager": same},
     }
 
 
@@ -573,8 +589,9 @@ def main():
 
     guided = env = None
     if not args.no_guided:
-        log("equal-time guided vs plain: occluded-light room")
-        guided = equal_time_room(args)
+        if not args.no_room:
+            log("equal-time guided vs plain: occluded-light room")
+            guided = equal_time_room(args)
         if not args.no_env:
             log("equal-time guided vs plain: env-lit mesh")
             env = equal_time_env(args)

@@ -72,7 +72,8 @@ struct rt_ctx {
     bool guide_bounds_set = false;
     int guide_step = 0;
     size_t guide_slots = 0;
-    DevBuf g_theta, g_w16, g_m, g_v, g_grad, g_acc, g_count, g_rec, g_aux, g_open, g_out, g_in, g_tmp_rec, g_tmp_y;
+    int64_t guide_keep_slots = 0; /* pass size (path slots) the device keep probability was set for */
+    DevBuf g_theta, g_w16, g_m, g_v, g_grad, g_part, g_acc, g_count, g_rec, g_aux, g_open, g_out, g_in, g_tmp_rec, g_tmp_y;
     float scene_lo[3] = {0, 0, 0}, scene_hi[3] = {1, 1, 1};
     int timing = 1;
     hipEvent_t ev_a = nullptr, ev_b = nullptr;
@@ -266,7 +267,7 @@ int rt_destroy(rt_ctx* c)
     DevBuf* all[] = {&c->nodes, &c->prims, &c->prim_id, &c->mat, &c->vnormals, &c->geo, &c->mats, &c->lights,
                      &c->env_rgb, &c->env_row, &c->env_col, &c->env_p, &c->probe_rgb, &c->rb_o, &c->rb_d, &c->rb_out0,
                      &c->rb_out1, &c->stack_ovf, &c->tm_list, &c->tm_count, &c->counters, &c->views, &c->tiles, &c->passes, &c->films, &c->g_theta,
-                     &c->g_w16, &c->g_m, &c->g_v, &c->g_grad, &c->g_acc, &c->g_count, &c->g_rec, &c->g_aux,
+                     &c->g_w16, &c->g_m, &c->g_v, &c->g_grad, &c->g_part, &c->g_acc, &c->g_count, &c->g_rec, &c->g_aux,
                      &c->g_open, &c->g_out, &c->g_in, &c->g_tmp_rec, &c->g_tmp_y, &c->obj_nodes, &c->obj_first, &c->obj_xf, &c->leaf_obj,
                      &c->tex_rgb, &c->tex_info, &c->tri_tc,
                      &c->mlt_cur, &c->mlt_prop, &c->mlt_val, &c->mlt_energy, &c->mlt_acc, &c->mlt_cnt,
@@ -1064,18 +1065,22 @@ int alloc_path_state(rt_ctx* c, size_t cap)
 static int guide_alloc_base(rt_ctx* c)
 {
     int r;
-    if ((r = ensure(c, c->g_theta, GD_NPARAM * 4)) || (r = ensure(c, c->g_w16, (GD_NW16 + GD_NW16I) * 2)) ||
+    if ((r = ensure(c, c->g_theta, GD_NPARAM * 4)) || (r = ensure(c, c->g_w16, GD_NW16 * 2)) ||
         (r = ensure(c, c->g_m, GD_NPARAM * 4)) || (r = ensure(c, c->g_v, GD_NPARAM * 4)) ||
-        (r = ensure(c, c->g_grad, GD_NPARAM * 4)) || (r = ensure(c, c->g_acc, 16)) || (r = ensure(c, c->g_count, 32)))
+        (r = ensure(c, c->g_grad, GD_NPARAM * 4)) || (r = ensure(c, c->g_acc, 32)) || (r = ensure(c, c->g_count, 32)) ||
+        (r = ensure(c, c->g_part, (size_t)c->n_cu * GD_TRAIN_BLOCKS_PER_CU * GD_NPARAM * 4)))
         return r;
     GuideState& g = c->guide;
     g.theta = (float*)c->g_theta.p;
-    g.w16 = (_Float16*)c->g_w16.p;
-    g.w16i = g.w16 + GD_NW16;
+    g.w16i = (_Float16*)c->g_w16.p;
+    g.w16b = g.w16i + GD_NW16I;
+    g.part = (float*)c->g_part.p;
+    g.part_blocks = c->n_cu * GD_TRAIN_BLOCKS_PER_CU;
     g.adam_m = (float*)c->g_m.p;
     g.adam_v = (float*)c->g_v.p;
     g.grad = (float*)c->g_grad.p;
     g.acc = (float*)c->g_acc.p;
+    g.keep = g.acc + 4;
     g.rec_count = (int32_t*)c->g_count.p;
     g.trained = (unsigned long long*)((char*)c->g_count.p + 8);
     g.inferred = (unsigned long long*)((char*)c->g_count.p + 16);
@@ -1109,18 +1114,15 @@ static int guide_alloc_slots(rt_ctx* c, size_t slots, int batch)
 
 static int guide_upload_params(rt_ctx* c, const float* theta)
 {
-    /* training tiles, then inference tiles (whose padding stays zero) */
-    std::vector<_Float16> w16(GD_NW16 + GD_NW16I, (_Float16)0.f);
+    /* forward tiles, then backward tiles (their padding stays zero) */
+    std::vector<_Float16> w16(GD_NW16, (_Float16)0.f);
     for (int i = 0; i < GD_NPARAM; ++i) {
-        int fw, bw;
-        gd_w16_slots(i, &fw, &bw);
-        if (fw >= 0) w16[fw] = (_Float16)theta[i];
-        if (bw >= 0) w16[bw] = (_Float16)theta[i];
-        const int iw = gd_w16i_slot(i);
-        if (iw >= 0) w16[GD_NW16 + iw] = (_Float16)theta[i];
+        const int iw = gd_w16i_slot(i), ib = gd_w16b_slot(i);
+        if (iw >= 0) w16[iw] = (_Float16)theta[i];
+        if (ib >= 0) w16[GD_NW16I + ib] = (_Float16)theta[i];
     }
     HIP_TRY(c, hipMemcpy(c->guide.theta, theta, GD_NPARAM * 4, hipMemcpyHostToDevice));
-    HIP_TRY(c, hipMemcpy(c->guide.w16, w16.data(), w16.size() * 2, hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(c->guide.w16i, w16.data(), w16.size() * 2, hipMemcpyHostToDevice));
     HIP_TRY(c, hipMemset(c->guide.adam_m, 0, GD_NPARAM * 4));
     HIP_TRY(c, hipMemset(c->guide.adam_v, 0, GD_NPARAM * 4));
     HIP_TRY(c, hipMemset(c->guide.grad, 0, GD_NPARAM * 4));
@@ -1392,10 +1394,24 @@ static int render_impl(rt_ctx* c, const rt_view* views, int32_t nv, const rt_par
     const DevTile* d_tiles = (const DevTile*)c->tiles.p;
     const DevPass* d_passes = (const DevPass*)c->passes.p;
 
-    PtParams pp{prm->mode, prm->max_depth, prm->rr_start, prm->rr_prob, prm->seed, train ? 1 : 0,
-                /* keep about cap / 6 records per path slot: deep vertices are
-                 * represented as well as first hits (paths average < 6 vertices) */
-                (float)std::min(1.0, (double)GD_TRAIN_CHUNKS * batch / std::max(1.0, (double)P * 6.0))};
+    PtParams pp{prm->mode, prm->max_depth, prm->rr_start, prm->rr_prob, prm->seed, train ? 1 : 0};
+    if (train) {
+        /* Records are kept with a probability that spreads the record buffer
+         * over all path depths (a full buffer would keep only the first
+         * vertices appended, i.e. the shallow ones).  The first training pass
+         * of a context assumes 6 vertices per path slot; every training pass
+         * then sets it from the vertices it offered (k_guide_keep), so that
+         * the next fills about 90% of the buffer: GD_TRAIN_CHUNKS steps of
+         * `batch` records each.  The value carries over to later renders,
+         * scaled by the ratio of their pass sizes. */
+        if (c->guide_keep_slots == 0) {
+            const float keep0 = (float)std::min(1.0, (double)c->guide.rec_cap / std::max(1.0, (double)P * 6.0));
+            HIP_TRY(c, hipMemcpyAsync(c->guide.keep, &keep0, sizeof(float), hipMemcpyHostToDevice, c->stream));
+        } else if (c->guide_keep_slots != P) {
+            HIP_TRY(c, launch_guide_keep(c->guide, (float)((double)c->guide_keep_slots / (double)P), c->stream));
+        }
+        c->guide_keep_slots = P;
+    }
     const int grid = grid_for(c);
     const int depth = prm->max_depth;
     if (int r = ensure_stack_overflow(c, grid)) return r;
@@ -1469,6 +1485,7 @@ static int render_impl(rt_ctx* c, const rt_view* views, int32_t nv, const rt_par
                 QCHK(launch_guide_adam(c->guide, lr, c->guide_step, c->guide.rec_count, off, batch, c->stream));
             }
             QCHK(mark());
+            QCHK(launch_guide_keep(c->guide, 0.f, c->stream));
             QCHK(launch_guide_reset_records(c->guide, (int)P, c->stream));
         }
         QCHK(launch_pass_end(c->ps, c->stream));
@@ -1564,7 +1581,7 @@ namespace {
 hipError_t mlt_trace(rt_ctx* c, int n, int cap, int max_depth, int rr_start, float rr_prob, uint32_t seed,
                      int pixel_units)
 {
-    PtParams pp{RT_MODE_COMPAT, max_depth, rr_start, rr_prob, seed, 0, 0.f, (const float4*)c->mlt_prop.p, cap};
+    PtParams pp{RT_MODE_COMPAT, max_depth, rr_start, rr_prob, seed, 0, (const float4*)c->mlt_prop.p, cap};
     const int grid = grid_for(c);
     hipError_t e;
     if ((e = launch_generate_pss(c->ps, (const DevView*)c->mlt_view.p, n, pp, pixel_units, grid, c->stream)))
@@ -2080,7 +2097,7 @@ static int integrate_rays_impl(rt_ctx* c, const float* o, const float* d, int64_
     if (int r = alloc_path_state(c, (size_t)cap)) return r;
     const int grid = grid_for(c);
     if (int r = ensure_stack_overflow(c, grid)) return r;
-    PtParams pp{p->mode, p->max_depth, p->rr_start, p->rr_prob, p->seed, 0, 0.f};
+    PtParams pp{p->mode, p->max_depth, p->rr_start, p->rr_prob, p->seed, 0};
     HIP_TRY(c, launch_reset_counters(c->ps, c->stream));
     HIP_TRY(c, hipEventRecord(c->ev_a, c->stream));
     InFlight flight(c, 2);

@@ -83,25 +83,39 @@ __device__ __forceinline__ bool defer(bool need, int64_t idx, const Deferred& d,
     return true;
 }
 
+/* Each block iteration covers TM_UNROLL x 256 consecutive float4: a lane
+ * issues its TM_UNROLL loads (256 float4 apart, so every load instruction of
+ * a wave is one contiguous 1 KB) before converting any, which keeps
+ * TM_UNROLL x 16 B per lane in flight. */
+#define TM_UNROLL 4
 __global__ void __launch_bounds__(256) k_tonemap4(const float4* __restrict__ in, uint32_t* __restrict__ out,
                                                   int64_t n4, Gamma g, Deferred d)
 {
     __shared__ unsigned int cnt;
     if (threadIdx.x == 0) cnt = 0;
     __syncthreads();
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride) {
-        float4 c = in[i];
-        const float cc[4] = {c.x, c.y, c.z, c.w};
-        uint32_t word = 0;
+    const int64_t stride = (int64_t)gridDim.x * 256 * TM_UNROLL;
+    for (int64_t i0 = (int64_t)blockIdx.x * 256 * TM_UNROLL + threadIdx.x; i0 < n4; i0 += stride) {
+        float4 c[TM_UNROLL];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            uint32_t b;
-            bool ok = tone_fast(cc[k], g, b);
-            if (!defer(!ok, 4 * i + k, d, &cnt)) b = tone_exact(cc[k], g);
-            word |= b << (8 * k);
+        for (int u = 0; u < TM_UNROLL; ++u) {
+            const int64_t i = i0 + 256 * u;
+            c[u] = i < n4 ? in[i] : make_float4(0.f, 0.f, 0.f, 0.f);
         }
-        out[i] = word;
+#pragma unroll
+        for (int u = 0; u < TM_UNROLL; ++u) {
+            const int64_t i = i0 + 256 * u;
+            const float cc[4] = {c[u].x, c[u].y, c[u].z, c[u].w};
+            uint32_t word = 0;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                uint32_t b;
+                const bool ok = tone_fast(cc[k], g, b) || i >= n4;
+                if (!defer(!ok, 4 * i + k, d, &cnt)) b = tone_exact(cc[k], g);
+                word |= b << (8 * k);
+            }
+            if (i < n4) out[i] = word;
+        }
     }
     __syncthreads();
     if (threadIdx.x == 0) d.count[blockIdx.x] = min(cnt, (unsigned int)d.seg);
@@ -129,7 +143,10 @@ __global__ void __launch_bounds__(256) k_tonemap_deferred(const float* __restric
 
 }  // namespace
 
-int tonemap_grid(int64_t n, int grid) { return (int)std::max<int64_t>(1, std::min<int64_t>(grid, (n / 4 + 255) / 256)); }
+int tonemap_grid(int64_t n, int grid)
+{
+    return (int)std::max<int64_t>(1, std::min<int64_t>(grid, (n / 4 + 256 * TM_UNROLL - 1) / (256 * TM_UNROLL)));
+}
 
 hipError_t launch_tonemap(const float* film, uint8_t* out, int64_t n, float gamma, int64_t* list,
                           unsigned int* count, int64_t cap, int grid, hipStream_t st)

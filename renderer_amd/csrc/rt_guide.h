@@ -10,15 +10,12 @@
  *
  * Flat parameter layout (FP32 master, 16,624 values):
  *   W1[64][80] b1[64] W2[64][64] b2[64] W3[64][64] b3[64] W4[48][64] b4[48]
- * FP16 weight copy (27,648 values), pre-tiled as MFMA A operands so that a
- * wave reads each 16x16 operand as 64 contiguous 8-byte words (no LDS bank
- * conflicts): W1..W4 for the forward pass (tile (ot, kb), lane m + 16 q
- * holds W[16 ot + m][16 kb + 4 q .. +3]), then W2..W4 transposed for the
- * backward pass (tile (it, kb), lane m + 16 q holds W[16 kb + 4 q .. +3][16 it + m]).
- * These 16x16x16 tiles serve training (v_mfma_f32_16x16x16_f16).
  *
- * Inference copy (17,408 values, v_mfma_f32_16x16x32_f16 A operands of
- * 16 rows x 32 inputs, 64 lanes x 8 halves per tile, tiles (ot, kb)):
+ * FP16 copies of the weights, pre-tiled as A operands of
+ * v_mfma_f32_16x16x32_f16 (16 rows x 32 K, 64 lanes x 8 halves per tile), so
+ * a wave reads each operand as 64 contiguous 16-byte words:
+ *
+ * Forward tiles (17,408 values; inference and training), tiles (ot, kb):
  * lane m + 16 q, slot j holds W[row(ot, m)][32 kb + col(q, j)].
  *   W1 (inputs padded to 96): col = 8 q + j, the feature order itself.
  *   W2..W4: col = 4 q + j for j < 4, 16 + 4 q + (j - 4) for j >= 4, the
@@ -28,6 +25,14 @@
  *   group m / 4 keeps in slot 4 ot + m % 4: slots 0..9 are lobes 2 g and
  *   2 g + 1 of lane group g (outputs 10 g .. 10 g + 9), slot 10 of group 0
  *   is the guide probability logit (output 40); the other slots are zero.
+ *
+ * Backward tiles (12,288 values; training), the transposed weights
+ * W2^T, W3^T, W4^T, tiles (it, kb): lane m + 16 q, slot j holds
+ * W[out(q, j, kb)][16 it + m].  W2, W3: out = 32 kb + col(q, j) as above, so
+ * a layer's output gradients, held like its outputs, are the B operand.  W4:
+ * out = the output of lane group q, slot 8 kb + j (zero for kb = 1, j >= 4
+ * and for slots without an output), so the head's gradients, held by the
+ * lanes that evaluated the head, are the B operand.
  */
 #ifndef RT_GUIDE_H
 #define RT_GUIDE_H
@@ -40,8 +45,6 @@
 #define GD_OUT 48
 #define GD_LOBES 8
 #define GD_NPARAM 16624
-#define GD_NW16F 16384  /* forward tiles */
-#define GD_NW16 27648   /* forward + transposed tiles */
 #define GD_OFF_W1 0
 #define GD_OFF_B1 5120
 #define GD_OFF_W2 5184
@@ -50,18 +53,16 @@
 #define GD_OFF_B3 13440
 #define GD_OFF_W4 13504
 #define GD_OFF_B4 16576
-#define GD_W16_W1 0
-#define GD_W16_W2 5120
-#define GD_W16_W3 9216
-#define GD_W16_W4 13312
-#define GD_W16T_W2 16384
-#define GD_W16T_W3 20480
-#define GD_W16T_W4 24576
-#define GD_NW16I 17408   /* inference tiles (16x16x32 layout) */
+#define GD_NW16I 17408   /* forward tiles */
 #define GD_W16I_W1 0
 #define GD_W16I_W2 6144
 #define GD_W16I_W3 10240
 #define GD_W16I_W4 14336
+#define GD_NW16B 12288   /* backward tiles */
+#define GD_W16B_W2 0
+#define GD_W16B_W3 4096
+#define GD_W16B_W4 8192
+#define GD_NW16 (GD_NW16I + GD_NW16B)
 #define GD_KLO -3.0f
 #define GD_KHI 10.0f
 #define GD_REC 16        /* floats per training record */
@@ -69,11 +70,12 @@
 #define GD_WCLAMP 100.0f /* cap of a normalised sample weight */
 #define GD_TRAIN_CHUNKS 4 /* training steps per render pass (record capacity = 4 batches) */
 /* Persistent grids of the network kernels: the blocks one CU holds at once
- * (LDS: 45 KB per inference block, 138 KB per training block).  More blocks
+ * (LDS: 45 KB per inference block, 155 KB per training block).  More blocks
  * would only reload the weights into LDS and, for training, add one round
  * of gradient atomics per block. */
 #define GD_INFER_BLOCKS_PER_CU 3
 #define GD_TRAIN_BLOCKS_PER_CU 1
+#define GD_TRAIN_THREADS 256   /* 4 waves: one per SIMD, its gradient sums in the accumulation registers */
 
 /* Training record (16 floats):
  *   0..2 position, 3..5 viewer-side normal, 6..8 outgoing direction,
@@ -83,12 +85,15 @@
  * closed (rgb) and the path slot (int bits). */
 struct GuideState {
     float* theta;       /* FP32 master parameters */
-    _Float16* w16;      /* FP16 weights, training tiles */
-    _Float16* w16i;     /* FP16 weights, inference tiles */
+    _Float16* w16i;     /* FP16 weights: forward tiles, then (w16b) backward tiles */
+    _Float16* w16b;
     float* adam_m;
     float* adam_v;
     float* grad;
+    float* part;        /* per training block: its gradient sums, GD_NPARAM each */
+    int32_t part_blocks;
     float* acc;         /* [0] weight sum, [1] loss sum, [2] records trained */
+    float* keep;        /* probability that a render keeps a path vertex as a training record */
     int32_t* rec_count;
     unsigned long long* trained;  /* running total of records over all Adam steps */
     unsigned long long* inferred; /* running total of path vertices evaluated by k_guide_infer */
@@ -101,24 +106,6 @@ struct GuideState {
     int32_t rec_cap;
     float lo[3], inv_ext[3];
 };
-
-/* Positions of master parameter i in the FP16 copy: forward tile slot and
- * transposed tile slot (-1 where none: biases, and W1 has no transposed copy). */
-__host__ __device__ inline void gd_w16_slots(int i, int* fwd, int* bwd)
-{
-    *fwd = -1;
-    *bwd = -1;
-    int off, O, I, bf, bb;
-    if (i < GD_OFF_B1) { off = GD_OFF_W1; O = 64; I = 80; bf = GD_W16_W1; bb = -1; }
-    else if (i >= GD_OFF_W2 && i < GD_OFF_B2) { off = GD_OFF_W2; O = 64; I = 64; bf = GD_W16_W2; bb = GD_W16T_W2; }
-    else if (i >= GD_OFF_W3 && i < GD_OFF_B3) { off = GD_OFF_W3; O = 64; I = 64; bf = GD_W16_W3; bb = GD_W16T_W3; }
-    else if (i >= GD_OFF_W4 && i < GD_OFF_B4) { off = GD_OFF_W4; O = 48; I = 64; bf = GD_W16_W4; bb = GD_W16T_W4; }
-    else return;
-    int r = i - off, o = r / I, c = r % I;
-    *fwd = bf + ((((o >> 4) * (I >> 4) + (c >> 4)) * 64 + (o & 15) + 16 * ((c & 15) >> 2)) << 2) + (c & 3);
-    if (bb >= 0)
-        *bwd = bb + ((((c >> 4) * (O >> 4) + (o >> 4)) * 64 + (c & 15) + 16 * ((o & 15) >> 2)) << 2) + (o & 3);
-}
 
 /* Output of W4 that inference lane group g keeps in slot s (0..11), or -1. */
 __host__ __device__ inline int gd_head_output(int g, int s)
@@ -165,6 +152,50 @@ __host__ __device__ inline int gd_w16i_slot(int i)
     return base + (((ot * KB + kb) * 64 + m + 16 * q) << 3) + j;
 }
 
+/* Lane group and slot (0..11) in which the head holds output o, or false. */
+__host__ __device__ inline bool gd_head_slot(int o, int* g, int* s)
+{
+    if (o < 40) { *g = o / 10; *s = o % 10; return true; }
+    if (o == 40) { *g = 0; *s = 10; return true; }
+    return false;
+}
+
+/* Position of master parameter i in the backward tiles (-1: W1, biases, and
+ * the unused outputs 41..47 of W4). */
+__host__ __device__ inline int gd_w16b_slot(int i)
+{
+    int off, base;
+    if (i >= GD_OFF_W2 && i < GD_OFF_B2) { off = GD_OFF_W2; base = GD_W16B_W2; }
+    else if (i >= GD_OFF_W3 && i < GD_OFF_B3) { off = GD_OFF_W3; base = GD_W16B_W3; }
+    else if (i >= GD_OFF_W4 && i < GD_OFF_B4) { off = GD_OFF_W4; base = GD_W16B_W4; }
+    else return -1;
+    const int o = (i - off) / GD_H, c = (i - off) % GD_H;  /* output row, input column */
+    const int it = c >> 4, m = c & 15;
+    int kb, q, j;
+    if (off == GD_OFF_W4) {
+        int g, s;
+        if (!gd_head_slot(o, &g, &s)) return -1;
+        kb = s >> 3;
+        q = g;
+        j = s & 7;
+    } else {
+        kb = o >> 5;
+        const int r = o & 31;
+        q = (r & 15) >> 2;
+        j = (r & 3) + (r >= 16 ? 4 : 0);
+    }
+    return base + (((it * 2 + kb) * 64 + m + 16 * q) << 3) + j;
+}
+
+/* Whether training computes a gradient for master parameter i (all but the
+ * unused outputs 41..47 of W4 and their biases). */
+__host__ __device__ inline bool gd_trained(int i)
+{
+    if (i >= GD_OFF_W4 && i < GD_OFF_B4) return (i - GD_OFF_W4) / GD_H <= 40;
+    if (i >= GD_OFF_B4) return i - GD_OFF_B4 <= 40;
+    return true;
+}
+
 struct GuideHost;  /* forward declaration for the ABI layer */
 
 hipError_t launch_guide_infer(const struct HostDevScene& s, const struct PathState& ps, const GuideState& g,
@@ -180,5 +211,10 @@ hipError_t launch_guide_train(const GuideState& g, const float* rec, const int32
 hipError_t launch_guide_adam(const GuideState& g, float lr, int step, const int32_t* count, int off, int max_n,
                              hipStream_t st);
 hipError_t launch_guide_reset_records(const GuideState& g, int slots, hipStream_t st);
+/* scale = 0, after a training pass: the keep probability for the next pass
+ * from the vertices this one offered (records counted / keep), aiming at 90%
+ * of the record capacity.  scale > 0: the probability times scale (passes of
+ * another size), at most 1. */
+hipError_t launch_guide_keep(const GuideState& g, float scale, hipStream_t st);
 
 #endif

@@ -11,10 +11,16 @@
  * last layer's rows are ordered so that each lane ends with two whole
  * lobes, and the head is evaluated in registers too.
  *
- * Training (k_guide_train) uses v_mfma_f32_16x16x16_f16 on 16 samples per
- * wave.  Weight gradients (a reduction over samples) read activations and
- * deltas back from LDS in [neuron][sample] order, which gives both MFMA
- * operands as contiguous 8-byte reads.
+ * Training (k_guide_train) uses the same instruction and forward pass on 64
+ * samples per wave, back-propagates with transposed weight tiles laid out
+ * so that a layer's output gradients are the next B operand, and forms the
+ * weight gradients (a reduction over samples) with samples along K.  The
+ * transposition that this needs - samples from lanes into the registers of
+ * one lane - is itself an MFMA: a B operand (8 features of one sample per
+ * lane) used as the A operand of a product with an identity selection gives
+ * a D tile holding 4 samples of one feature per lane.  Everything stays in
+ * registers; the gradients are summed over the block's waves with LDS
+ * atomics and written once per block.
  */
 #include "rt_device.h"
 #include "rt_guide.h"
@@ -38,140 +44,15 @@ static inline DevScene as_dev_g(const HostDevScene& s)
     return d;
 }
 
-__device__ __forceinline__ f4 mfma16(h4 a, h4 b, f4 c)
-{
-    return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0);
-}
 __device__ __forceinline__ f4 mfma32(h8 a, h8 b, f4 c)
 {
     return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
-}
-
-/* FP16 weights (the first NW values of the tiled copy) and FP32 biases in LDS */
-template <int NW>
-struct alignas(16) WLds {
-    _Float16 w[NW];
-    float b[3 * GD_H + GD_OUT];
-};
-
-template <int NW>
-__device__ __forceinline__ void load_weights(WLds<NW>& L, const GuideState& g)
-{
-    const uint4* src = reinterpret_cast<const uint4*>(g.w16);
-    uint4* dst = reinterpret_cast<uint4*>(L.w);
-    for (int i = threadIdx.x; i < NW / 8; i += blockDim.x) dst[i] = src[i];
-    for (int i = threadIdx.x; i < 3 * GD_H + GD_OUT; i += blockDim.x) {
-        int off = i < 64 ? GD_OFF_B1 + i : i < 128 ? GD_OFF_B2 + i - 64 : i < 192 ? GD_OFF_B3 + i - 128 : GD_OFF_B4 + i - 192;
-        L.b[i] = g.theta[off];
-    }
-    __syncthreads();
 }
 
 /* conversion of a gradient term to FP16 that saturates instead of overflowing */
 __device__ __forceinline__ _Float16 to_h_sat(float v)
 {
     return (_Float16)fminf(fmaxf(v, -65000.f), 65000.f);
-}
-
-/* Input features of a wave's 16 samples, staged through LDS rows of
- * GD_FROW halves (the padding keeps the 8-byte reads of 16 rows on distinct
- * banks).  Lane (n, q) evaluates sin/cos(2^k pi p_d) for k = 3q .. 3q+2 and
- * the three coordinates of sample n: one sincospi at k = 3q, then two angle
- * doublings (sin 2a = 2 sin a cos a, cos 2a = (cos a - sin a)(cos a + sin a));
- * lane q = 3 adds the normal, the outgoing direction and the zero padding.
- * After a barrier, x[kb] (features 16 kb + 4 q .. +3 of sample n, the MFMA
- * B-operand quad) is one 8-byte read. */
-#define GD_FROW (GD_IN + 4)
-__device__ __forceinline__ void features_to_lds(_Float16 (*F)[GD_FROW], int n, int q, const float* p01, vec3 nf,
-                                                vec3 wo)
-{
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        float s, c;
-        sincospif(ldexpf(p01[d], 3 * q), &s, &c);
-#pragma unroll
-        for (int j = 0; j < 3; ++j) {
-            int f = 24 * d + 2 * (3 * q + j);
-            F[n][f] = (_Float16)s;
-            F[n][f + 1] = (_Float16)c;
-            float s2 = 2.f * s * c, c2 = (c - s) * (c + s);
-            s = s2;
-            c = c2;
-        }
-    }
-    if (q == 3) {
-        F[n][72] = (_Float16)nf.x;
-        F[n][73] = (_Float16)nf.y;
-        F[n][74] = (_Float16)nf.z;
-        F[n][75] = (_Float16)wo.x;
-        F[n][76] = (_Float16)wo.y;
-        F[n][77] = (_Float16)wo.z;
-        F[n][78] = (_Float16)0.f;
-        F[n][79] = (_Float16)0.f;
-    }
-}
-__device__ __forceinline__ void features_from_lds(const _Float16 (*F)[GD_FROW], int n, int q, h4* x)
-{
-#pragma unroll
-    for (int kb = 0; kb < 5; ++kb) x[kb] = *reinterpret_cast<const h4*>(&F[n][16 * kb + 4 * q]);
-}
-
-/* out[ot] = W[16 ot .. +15][:] * in (+ bias), W as forward tiles (ot, kb). */
-template <int KB, int OT>
-__device__ __forceinline__ void dense(const _Float16* W, const float* b, const h4* in, f4* out)
-{
-    const int lane = __lane_id(), q = lane >> 4;
-#pragma unroll
-    for (int ot = 0; ot < OT; ++ot) {
-        f4 acc = {b[16 * ot + 4 * q], b[16 * ot + 4 * q + 1], b[16 * ot + 4 * q + 2], b[16 * ot + 4 * q + 3]};
-#pragma unroll
-        for (int kb = 0; kb < KB; ++kb) {
-            h4 a = *reinterpret_cast<const h4*>(W + (((ot * KB + kb) << 6) + lane) * 4);
-            acc = mfma16(a, in[kb], acc);
-        }
-        out[ot] = acc;
-    }
-}
-
-/* out[it] = W^T[16 it .. +15][:] * in, W^T as transposed tiles (it, kb)
- * (back-propagation through a layer with KB * 16 outputs). */
-template <int KB, int IT>
-__device__ __forceinline__ void dense_t(const _Float16* WT, const h4* in, f4* out)
-{
-    const int lane = __lane_id();
-#pragma unroll
-    for (int it = 0; it < IT; ++it) {
-        f4 acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int kb = 0; kb < KB; ++kb) {
-            h4 a = *reinterpret_cast<const h4*>(WT + (((it * KB + kb) << 6) + lane) * 4);
-            acc = mfma16(a, in[kb], acc);
-        }
-        out[it] = acc;
-    }
-}
-
-template <int T>
-__device__ __forceinline__ void relu_h(const f4* z, h4* h)
-{
-#pragma unroll
-    for (int t = 0; t < T; ++t)
-#pragma unroll
-        for (int i = 0; i < 4; ++i) h[t][i] = (_Float16)fmaxf(z[t][i], 0.f);
-}
-
-/* Forward pass of 16 samples held by one wave.  Returns y tiles (3 x f4). */
-__device__ __forceinline__ void forward16(const _Float16* w, const float* b, const h4* x, h4* h1, h4* h2, h4* h3,
-                                          f4* y)
-{
-    f4 z[4];
-    dense<5, 4>(w + GD_W16_W1, b, x, z);
-    relu_h<4>(z, h1);
-    dense<4, 4>(w + GD_W16_W2, b + 64, h1, z);
-    relu_h<4>(z, h2);
-    dense<4, 4>(w + GD_W16_W3, b + 128, h2, z);
-    relu_h<4>(z, h3);
-    dense<4, 3>(w + GD_W16_W4, b + 192, h3, y);
 }
 
 /* Head of one lobe from its five raw outputs. */
@@ -227,22 +108,25 @@ struct alignas(16) InferLds {
     float in[GB / 64][64][GD_INS];
 };
 
+/* Bias i of the D-tile order: b1, b2, b3, then b4 in the row order of the
+ * forward W4 tiles (zero for rows without an output). */
+__device__ __forceinline__ float bias_value(const GuideState& g, int i)
+{
+    if (i < 3 * GD_H) return g.theta[(i < 64 ? GD_OFF_B1 : i < 128 ? GD_OFF_B2 - 64 : GD_OFF_B3 - 128) + i];
+    if (i < 3 * GD_H + GD_OUT) {
+        const int r = i - 3 * GD_H, ot = r >> 4, m = r & 15;
+        const int o = gd_head_output(m >> 2, 4 * ot + (m & 3));
+        if (o >= 0) return g.theta[GD_OFF_B4 + o];
+    }
+    return 0.f;
+}
+
 __device__ __forceinline__ void load_infer_weights(InferLds& L, const GuideState& g)
 {
     const uint4* src = reinterpret_cast<const uint4*>(g.w16i);
     uint4* dst = reinterpret_cast<uint4*>(L.w);
     for (int i = threadIdx.x; i < GD_NW16I / 8; i += blockDim.x) dst[i] = src[i];
-    for (int i = threadIdx.x; i < 4 * GD_H; i += blockDim.x) {
-        float v = 0.f;
-        if (i < 3 * GD_H) {
-            v = g.theta[(i < 64 ? GD_OFF_B1 : i < 128 ? GD_OFF_B2 - 64 : GD_OFF_B3 - 128) + i];
-        } else if (i < 3 * GD_H + GD_OUT) {
-            const int r = i - 3 * GD_H, ot = r >> 4, m = r & 15;
-            const int o = gd_head_output(m >> 2, 4 * ot + (m & 3));
-            if (o >= 0) v = g.theta[GD_OFF_B4 + o];
-        }
-        L.b[i] = v;
-    }
+    for (int i = threadIdx.x; i < 4 * GD_H; i += blockDim.x) L.b[i] = bias_value(g, i);
     __syncthreads();
 }
 
@@ -537,15 +421,6 @@ hipError_t launch_guide_head(const GuideState& g, const float* rec, int n, _Floa
     return hipGetLastError();
 }
 
-__device__ __forceinline__ void record_inputs(const GuideState& g, const float* r, float* p01, vec3& nf, vec3& wo)
-{
-    p01[0] = (r[0] - g.lo[0]) * g.inv_ext[0];
-    p01[1] = (r[1] - g.lo[1]) * g.inv_ext[1];
-    p01[2] = (r[2] - g.lo[2]) * g.inv_ext[2];
-    nf = mk(r[3], r[4], r[5]);
-    wo = mk(r[6], r[7], r[8]);
-}
-
 /* ===========================================================================
  * record weights: w = lum(radiance_final - radiance_at_close) / lum(alpha),
  * and the weight sum of the records a training step will use
@@ -612,213 +487,426 @@ hipError_t launch_guide_zero_acc(const GuideState& g, hipStream_t st)
  * training: loss -sum s_i log p_mix_i with s_i = w_i N / sum w (capped),
  * gradients summed over the records into g.grad (Adam divides by N)
  * ========================================================================= */
-/* Rows of 64 samples padded by 4 halves: a 16-lane group reading 8 bytes
- * from 16 consecutive rows then touches 32 distinct 4-byte banks. */
-#define GD_ROW (64 + 4)
-struct TrainLds {
-    WLds<GD_NW16> wl;
-    float Y[4][GD_OUT][16];
-    _Float16 act[GD_IN + 3 * GD_H][GD_ROW];   /* x, h1, h2, h3 of the block's 64 samples */
-    _Float16 dz[3 * GD_H + GD_OUT][GD_ROW];   /* dz1, dz2, dz3, dy */
+#define GD_TW (GD_TRAIN_THREADS / 64) /* waves per training block */
+#define GD_TIN 15                      /* staged floats per sample (odd stride: distinct banks) */
+#define GD_NTILE 64                    /* weight-gradient tiles: W1 20, W2 16, W3 16, W4 12 */
+#define GD_NBT 15                      /* bias-gradient tiles: b1..b3 4 each, b4 3 */
+
+#define GD_NT1 20  /* gradient tiles of W1 */
+/* FP16 forward and backward tiles (after the last chunk: the block's sums
+ * of the W2..W4 gradient tiles, [tile][lane][e]), FP32 biases in D-tile
+ * order, and per wave the staged inputs of its 64 samples (network inputs
+ * (9), sampled direction (3), BSDF pdf, normalised weight), its bias-gradient
+ * sums ([tile][e][q]) and its W1 gradient sums ([tile][lane][e]).  The W2..W4
+ * sums of a wave stay in its accumulation registers. */
+struct alignas(16) TrainLds {
+    union {
+        _Float16 w[GD_NW16];
+        float acc[(GD_NTILE - GD_NT1) * 256];
+    };
+    float b[4 * GD_H];
+    float in[GD_TW][64][GD_TIN];
+    float bacc[GD_TW][GD_NBT * 16];
+    float acc1[GD_TW][GD_NT1 * 256];
 };
 
-/* tile t of the block's 64 weight-gradient tiles -> (layer, out tile, in tile) */
-__device__ __forceinline__ void grad_tile(int t, int& layer, int& ot, int& it)
+/* Identity selection of one 16-feature tile of a K block, as the B operand
+ * of a transposing product (lane c + 16 q).  Hidden layers hold their
+ * features in the permuted order of the forward tiles (feature 16 t + c of
+ * the block at slot c - 4 q + 4 t of lane group c / 4); the network inputs in
+ * natural order (slot 16 t + c - 8 q of lane group (16 t + c) / 8). */
+__device__ __forceinline__ h8 sel_op(int t, bool natural)
 {
-    if (t < 20) { layer = 0; ot = t / 5; it = t % 5; }
-    else if (t < 36) { layer = 1; ot = (t - 20) / 4; it = (t - 20) % 4; }
-    else if (t < 52) { layer = 2; ot = (t - 36) / 4; it = (t - 36) % 4; }
-    else { layer = 3; ot = (t - 52) / 4; it = (t - 52) % 4; }
+    const int lane = __lane_id(), c = lane & 15, q = lane >> 4;
+    const int d = c - 4 * q;
+    const int j = natural ? 16 * t + c - 8 * q : (d >= 0 && d < 4 ? d + 4 * t : -1);
+    h8 r;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) r[k] = (_Float16)(k == j ? 1.f : 0.f);
+    return r;
 }
 
-__global__ __launch_bounds__(GB) void k_guide_train(GuideState g, const float* rec0, const int32_t* count, int off,
-                                                   int max_n)
+__device__ __forceinline__ h8 pack8(f4 a, f4 b)
 {
-    __shared__ TrainLds S;
-    load_weights(S.wl, g);
+    const h4 lo = __builtin_convertvector(a, h4), hi = __builtin_convertvector(b, h4);
+    return __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7);
+}
+
+/* One 16-feature tile of a matrix held as B operands M[st] (the K block
+ * that contains it; sel picks it) with samples along K: S[ks] holds, in
+ * lane c + 16 q, feature c of the tile for the samples
+ * 16 (2 ks + j / 4) + 4 q + j % 4, j = 0..7.  FP16 values pass the product
+ * exactly. */
+template <int T, bool NATURAL>
+__device__ __forceinline__ void to_sil(const h8 (&M)[4], h8 (&S)[2])
+{
+    const h8 sel = sel_op(T, NATURAL);  /* rematerialised at each use: fewer live registers */
+    const f4 zero = {0.f, 0.f, 0.f, 0.f};
+    f4 d[4];
+#pragma unroll
+    for (int st = 0; st < 4; ++st) d[st] = mfma32(M[st], sel, zero);
+    S[0] = pack8(d[0], d[1]);
+    S[1] = pack8(d[2], d[3]);
+}
+
+/* Weight-gradient tile: the wave's 64 samples of output gradients (rows)
+ * times inputs (columns), added to the wave's register sums. */
+__device__ __forceinline__ void grad_tile(f4& acc, const h8 (&dz)[2], const h8 (&a)[2])
+{
+    acc = mfma32(dz[0], a[0], acc);
+    acc = mfma32(dz[1], a[1], acc);
+}
+
+/* Bias-gradient tile: the output gradients summed over the samples (a
+ * product with ones, every column the same sum), added by the lanes of
+ * column 0 to the wave's own sums in LDS (no other wave writes them). */
+__device__ __forceinline__ void bias_tile(float* bacc, int tile, const h8 (&dz)[2], h8 ones)
+{
+    f4 d = {0.f, 0.f, 0.f, 0.f};
+    d = mfma32(dz[0], ones, d);
+    d = mfma32(dz[1], ones, d);
+    const int lane = __lane_id();
+    if ((lane & 15) == 0)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) bacc[(tile * 4 + e) * 4 + (lane >> 4)] += d[e];
+}
+
+/* Output gradients of the layer below (ReLU activations h, B operands) from
+ * this layer's output gradients `in`: W^T (backward tiles (it, kb)) x in,
+ * masked by h > 0 and packed as B operands (slot j of K block kb is input
+ * 16 (2 kb + j / 4) + 4 q + j % 4), one K block of the result at a time. */
+__device__ __forceinline__ void back_relu(const _Float16* WT, const h8 (&in)[4][2], const h8 (&h)[4][2],
+                                          h8 (&out)[4][2])
+{
+    const int lane = __lane_id();
+#pragma unroll
+    for (int ko = 0; ko < 2; ++ko) {
+        f4 d[4][2];
+#pragma unroll
+        for (int st = 0; st < 4; ++st) d[st][0] = d[st][1] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kb = 0; kb < 2; ++kb)
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                const int it = 2 * ko + t;
+                const h8 a = *reinterpret_cast<const h8*>(WT + ((((it * 2 + kb) << 6) + lane) << 3));
+#pragma unroll
+                for (int st = 0; st < 4; ++st) d[st][t] = mfma32(a, in[st][kb], d[st][t]);
+            }
+#pragma unroll
+        for (int st = 0; st < 4; ++st)
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+                out[st][ko][j] = h[st][ko][j] > (_Float16)0.f ? to_h_sat(d[st][j >> 2][j & 3]) : (_Float16)0.f;
+    }
+}
+
+/* One hidden layer of the backward pass: the gradients of its weights
+ * (tiles tile0 + 4 ot + it) and biases (tiles btile0 + ot) from its output
+ * gradients dz and inputs a, then dz <- the output gradients of the layer
+ * below (through W^T and that layer's ReLU, whose activations a are). */
+template <int TILE0, int BTILE0>
+__device__ __forceinline__ void hidden_back(f4 (&acc)[GD_NTILE], float* bacc, const _Float16* WT,
+                                            const h8 (&a)[4][2], h8 (&dz)[4][2], h8 ones)
+{
+    h8 zS[4][2];
+#pragma unroll
+    for (int ot = 0; ot < 4; ++ot) {
+        const h8 m[4] = {dz[0][ot >> 1], dz[1][ot >> 1], dz[2][ot >> 1], dz[3][ot >> 1]};
+        if (ot & 1)
+                to_sil<1, false>(m, zS[ot]);
+            else
+                to_sil<0, false>(m, zS[ot]);
+        bias_tile(bacc, BTILE0 + ot, zS[ot], ones);
+    }
+#pragma unroll
+    for (int it = 0; it < 4; ++it) {
+        const h8 ma[4] = {a[0][it >> 1], a[1][it >> 1], a[2][it >> 1], a[3][it >> 1]};
+        h8 aS[2];
+        if (it & 1)
+                to_sil<1, false>(ma, aS);
+            else
+                to_sil<0, false>(ma, aS);
+#pragma unroll
+        for (int ot = 0; ot < 4; ++ot) grad_tile(acc[TILE0 + 4 * ot + it], zS[ot], aS);
+    }
+    h8 below[4][2];
+    back_relu(WT, dz, a, below);
+#pragma unroll
+    for (int st = 0; st < 4; ++st) dz[st][0] = below[st][0], dz[st][1] = below[st][1];
+}
+
+/* Master parameter of gradient tile `tile`, element e, lane (-1: none). */
+__device__ __forceinline__ int grad_param(int tile, int e, int lane)
+{
+    const int n = lane & 15, q = lane >> 4;
+    if (tile < 20) return GD_OFF_W1 + (16 * (tile / 5) + 4 * q + e) * GD_IN + 16 * (tile % 5) + n;
+    if (tile < 52) {
+        const int t = tile < 36 ? tile - 20 : tile - 36;
+        return (tile < 36 ? GD_OFF_W2 : GD_OFF_W3) + (16 * (t >> 2) + 4 * q + e) * GD_H + 16 * (t & 3) + n;
+    }
+    const int t = tile - 52, o = gd_head_output(q, 4 * (t >> 2) + e);
+    return o < 0 ? -1 : GD_OFF_W4 + o * GD_H + 16 * (t & 3) + n;
+}
+__device__ __forceinline__ int bias_param(int tile, int e, int q)
+{
+    if (tile < 12) return (tile < 4 ? GD_OFF_B1 : tile < 8 ? GD_OFF_B2 : GD_OFF_B3) + 16 * (tile & 3) + 4 * q + e;
+    const int o = gd_head_output(q, 4 * (tile - 12) + e);
+    return o < 0 ? -1 : GD_OFF_B4 + o;
+}
+
+__global__ __launch_bounds__(GD_TRAIN_THREADS) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_guide_train(GuideState g, const float* rec0, const int32_t* count,
+                                                                  int off, int max_n)
+{
+    __shared__ TrainLds L;
+    {
+        const uint4* src = reinterpret_cast<const uint4*>(g.w16i);
+        uint4* dst = reinterpret_cast<uint4*>(L.w);
+        for (int i = threadIdx.x; i < GD_NW16 / 8; i += blockDim.x) dst[i] = src[i];
+        for (int i = threadIdx.x; i < 4 * GD_H; i += blockDim.x) L.b[i] = bias_value(g, i);
+        for (int i = threadIdx.x; i < GD_TW * GD_NT1 * 256; i += blockDim.x) (&L.acc1[0][0])[i] = 0.f;
+        for (int i = threadIdx.x; i < GD_TW * GD_NBT * 16; i += blockDim.x) (&L.bacc[0][0])[i] = 0.f;
+    }
+    __syncthreads();
+    const _Float16* WF = L.w;
+    const _Float16* WB = L.w + GD_NW16I;
     const int lane = __lane_id(), n = lane & 15, q = lane >> 4, w = threadIdx.x >> 6;
     const int N = chunk_len(g, count, off, max_n);
     const float* rec = rec0 + (size_t)off * GD_REC;
     const float wsum = g.acc[0];
     const float wscale = wsum > 0.f ? (float)N / wsum : 0.f;
-    f4 gacc[16];
+    float (*IN)[GD_TIN] = L.in[w];
+    h8 ones;
 #pragma unroll
-    for (int t = 0; t < 16; ++t) gacc[t] = f4{0.f, 0.f, 0.f, 0.f};
-    float bacc = 0.f, lacc = 0.f;
-    const int col = 16 * w + n;
-    for (int base = blockIdx.x * 64; base < N; base += gridDim.x * 64) {
-        int i = base + col;
-        bool valid = i < N;
-        const float* r = rec + (size_t)(valid ? i : 0) * GD_REC;
-        float p01[3] = {0.f, 0.f, 0.f};
-        vec3 nf = mk(0.f, 0.f, 0.f), wo = nf, om = mk(0.f, 0.f, 1.f);
-        float pb = 1.f, sw = 0.f;
-        if (valid) {
-            record_inputs(g, r, p01, nf, wo);
-            om = mk(r[9], r[10], r[11]);
-            pb = r[12];
-            sw = fminf(r[13] * wscale, GD_WCLAMP);
-        }
-        h4 x[5], h1[4], h2[4], h3[4];
-        f4 y[3];
-        _Float16 (*F)[GD_FROW] = reinterpret_cast<_Float16 (*)[GD_FROW]>(&S.Y[w][0][0]);
-        features_to_lds(F, n, q, p01, nf, wo);
-        __syncthreads();
-        features_from_lds(F, n, q, x);
-        forward16(S.wl.w, S.wl.b, x, h1, h2, h3, y);
-        /* activations to LDS, [neuron][sample] */
+    for (int k = 0; k < 8; ++k) ones[k] = (_Float16)1.f;
+    float lacc = 0.f;
+    float* bacc = L.bacc[w];
+    f4* acc1 = reinterpret_cast<f4*>(L.acc1[w]) + lane;  /* tile t at acc1[64 t] */
+    f4 gacc[GD_NTILE];  /* W2..W4 tiles (20..63) used */
 #pragma unroll
-        for (int kb = 0; kb < 5; ++kb)
+    for (int t = GD_NT1; t < GD_NTILE; ++t) gacc[t] = f4{0.f, 0.f, 0.f, 0.f};
+    const int stride = gridDim.x * GD_TW * 64;
+    for (int base = (blockIdx.x * GD_TW + w) * 64; base < N; base += stride) {
+        /* stage: lane l holds record base + l */
+        {
+            const int i = base + lane;
+            float v[GD_TIN];
 #pragma unroll
-            for (int e = 0; e < 4; ++e) S.act[16 * kb + 4 * q + e][col] = x[kb][e];
+            for (int c = 0; c < GD_TIN; ++c) v[c] = 0.f;
+            if (i < N) {
+                const float* r = rec + (size_t)i * GD_REC;
 #pragma unroll
-        for (int t = 0; t < 4; ++t)
+                for (int k = 0; k < 3; ++k) v[k] = (r[k] - g.lo[k]) * g.inv_ext[k];
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                S.act[GD_IN + 16 * t + 4 * q + e][col] = h1[t][e];
-                S.act[GD_IN + 64 + 16 * t + 4 * q + e][col] = h2[t][e];
-                S.act[GD_IN + 128 + 16 * t + 4 * q + e][col] = h3[t][e];
+                for (int k = 3; k < 13; ++k) v[k] = r[k];
+                v[13] = fminf(r[13] * wscale, GD_WCLAMP);
             }
 #pragma unroll
-        for (int t = 0; t < 3; ++t)
-#pragma unroll
-            for (int e = 0; e < 4; ++e) S.Y[w][16 * t + 4 * q + e][n] = y[t][e];
-        __syncthreads();
+            for (int c = 0; c < GD_TIN; ++c) IN[lane][c] = v[c];
+        }
+        wave_lds_sync();
 
-        /* head: loss and dL/dy of lobes 2q, 2q+1 (and alpha on q == 0) */
-        float rr[2][5];
-        vec3 mu[2];
-        float mlen[2], kap[2], v[2];
-        bool kin[2];
+        /* forward (as inference), keeping the activations as B operands */
+        h8 h1[4][2], h2[4][2], h3[4][2];
+        f4 y[4][3];
+        {
+            f4 z[4][4];
+            bias_init<4>(L.b, z);
 #pragma unroll
-        for (int s = 0; s < 2; ++s) {
-            int j = 2 * q + s;
+            for (int kb = 0; kb < 3; ++kb) {
+                h8 x[4];
 #pragma unroll
-            for (int c = 0; c < 5; ++c) rr[s][c] = S.Y[w][5 * j + c][n];
-            lobe_head(rr[s], mu[s], mlen[s], kap[s], kin[s]);
-            v[s] = vmf_pdf(mu[s], kap[s], om);
+                for (int st = 0; st < 4; ++st) x[st] = feature_group(kb, q, IN[16 * st + n]);
+                dense32_k<3, 4>(WF + GD_W16I_W1, kb, x, z);
+            }
+            relu_pack(z, h1);
+            dense32<4>(WF + GD_W16I_W2, L.b + GD_H, h1, z);
+            relu_pack(z, h2);
+            dense32<4>(WF + GD_W16I_W3, L.b + 2 * GD_H, h2, z);
+            relu_pack(z, h3);
+            dense32<3>(WF + GD_W16I_W4, L.b + 3 * GD_H, h3, y);
         }
-        float mx = quad_max(fmaxf(rr[0][4], rr[1][4]));
-        float e0 = expf(rr[0][4] - mx), e1 = expf(rr[1][4] - mx);
-        float inv = 1.f / quad_sum(e0 + e1);
-        float pi0 = e0 * inv, pi1 = e1 * inv;
-        float qd = quad_sum(pi0 * v[0] + pi1 * v[1]);
-        float sg = 1.f / (1.f + expf(-S.Y[w][40][n]));
-        float alpha = 0.05f + 0.9f * sg;
-        float pm = alpha * qd + (1.f - alpha) * pb;
-        float c = valid ? -sw / pm : 0.f;
-        float dq = c * alpha;
-        if (valid && q == 0) lacc += -sw * logf(pm);
-#pragma unroll
-        for (int s = 0; s < 2; ++s) {
-            int j = 2 * q + s;
-            float pij = s ? pi1 : pi0;
-            float dv = dq * pij;
-            float cw = dot(mu[s], om);
-            float em = expf(-2.f * kap[s]);
-            float dlogc = 1.f - 2.f * kap[s] * em / (-expm1f(-2.f * kap[s]));
-            float dk = kin[s] ? dv * v[s] * (dlogc + kap[s] * (cw - 1.f)) : 0.f;
-            float gm = dv * v[s] * kap[s] / mlen[s];
-            vec3 dm = (om - mu[s] * cw) * gm;
-            float da = dq * pij * (v[s] - qd);
-            S.dz[3 * GD_H + 5 * j][col] = to_h_sat(dm.x);
-            S.dz[3 * GD_H + 5 * j + 1][col] = to_h_sat(dm.y);
-            S.dz[3 * GD_H + 5 * j + 2][col] = to_h_sat(dm.z);
-            S.dz[3 * GD_H + 5 * j + 3][col] = to_h_sat(dk);
-            S.dz[3 * GD_H + 5 * j + 4][col] = to_h_sat(da);
-        }
-        if (q == 0) S.dz[3 * GD_H + 40][col] = to_h_sat(c * (qd - pb) * 0.9f * sg * (1.f - sg));
-        if (q == 1)
-            for (int o = 41; o < 48; ++o) S.dz[3 * GD_H + o][col] = (_Float16)0.f;
-        __syncthreads();
 
-        /* back-propagate: dy -> dz3 -> dz2 -> dz1 (B-operand layout in registers) */
-        h4 dy[3], d3[4], d2[4], d1[4];
-        f4 z[4];
+        /* head: loss and its gradient in the lanes that hold the outputs
+         * (lane group q: lobes 2q, 2q+1; group 0 also the guide logit) */
+        h8 dy[4][2];
 #pragma unroll
-        for (int t = 0; t < 3; ++t)
+        for (int st = 0; st < 4; ++st) {
+            const int i = base + 16 * st + n;
+            const bool valid = i < N;
+            const float* v = IN[16 * st + n];
+            const vec3 om = mk(v[9], v[10], v[11]);
+            const float pb = v[12], sw = v[13];
+            float r[12];
 #pragma unroll
-            for (int e = 0; e < 4; ++e) dy[t][e] = S.dz[3 * GD_H + 16 * t + 4 * q + e][col];
-        dense_t<3, 4>(S.wl.w + GD_W16T_W4, dy, z);
+            for (int ot = 0; ot < 3; ++ot)
 #pragma unroll
-        for (int t = 0; t < 4; ++t)
+                for (int e = 0; e < 4; ++e) r[4 * ot + e] = y[st][ot][e];
+            vec3 mu[2];
+            float mlen[2], kap[2], vv[2];
+            bool kin[2];
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                d3[t][e] = h3[t][e] > (_Float16)0.f ? to_h_sat(z[t][e]) : (_Float16)0.f;
-                S.dz[2 * GD_H + 16 * t + 4 * q + e][col] = d3[t][e];
+            for (int s = 0; s < 2; ++s) {
+                lobe_head(r + 5 * s, mu[s], mlen[s], kap[s], kin[s]);
+                vv[s] = vmf_pdf(mu[s], kap[s], om);
             }
-        dense_t<4, 4>(S.wl.w + GD_W16T_W3, d3, z);
+            const float mx = quad_max(fmaxf(r[4], r[9]));
+            const float e0 = expf(r[4] - mx), e1 = expf(r[9] - mx);
+            const float inv = 1.f / quad_sum(e0 + e1);
+            const float pi[2] = {e0 * inv, e1 * inv};
+            const float qd = quad_sum(pi[0] * vv[0] + pi[1] * vv[1]);
+            const float sg = 1.f / (1.f + expf(-__shfl(r[10], n)));
+            const float alpha = 0.05f + 0.9f * sg;
+            const float pm = alpha * qd + (1.f - alpha) * pb;
+            const float c = valid ? -sw / pm : 0.f;
+            const float dq = c * alpha;
+            if (valid && q == 0) lacc += -sw * logf(pm);
+            float dr[12];
 #pragma unroll
-        for (int t = 0; t < 4; ++t)
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                d2[t][e] = h2[t][e] > (_Float16)0.f ? to_h_sat(z[t][e]) : (_Float16)0.f;
-                S.dz[GD_H + 16 * t + 4 * q + e][col] = d2[t][e];
+            for (int s = 0; s < 2; ++s) {
+                const float dv = dq * pi[s];
+                const float cw = dot(mu[s], om);
+                const float em = expf(-2.f * kap[s]);
+                const float dlogc = 1.f - 2.f * kap[s] * em / (-expm1f(-2.f * kap[s]));
+                const float dk = kin[s] ? dv * vv[s] * (dlogc + kap[s] * (cw - 1.f)) : 0.f;
+                const float gm = dv * vv[s] * kap[s] / mlen[s];
+                const vec3 dm = (om - mu[s] * cw) * gm;
+                dr[5 * s] = dm.x;
+                dr[5 * s + 1] = dm.y;
+                dr[5 * s + 2] = dm.z;
+                dr[5 * s + 3] = dk;
+                dr[5 * s + 4] = dq * pi[s] * (vv[s] - qd);
             }
-        dense_t<4, 4>(S.wl.w + GD_W16T_W2, d2, z);
+            dr[10] = q == 0 ? c * (qd - pb) * 0.9f * sg * (1.f - sg) : 0.f;
+            dr[11] = 0.f;
 #pragma unroll
-        for (int t = 0; t < 4; ++t)
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                d1[t][e] = h1[t][e] > (_Float16)0.f ? to_h_sat(z[t][e]) : (_Float16)0.f;
-                S.dz[16 * t + 4 * q + e][col] = d1[t][e];
+            for (int j = 0; j < 8; ++j) {
+                dy[st][0][j] = to_h_sat(dr[j]);
+                dy[st][1][j] = j < 4 ? to_h_sat(dr[8 + j]) : (_Float16)0.f;
             }
-        __syncthreads();
+        }
 
-        /* weight gradients: wave w owns tiles 16 w .. 16 w + 15, K = 64 samples */
+        /* W4: gradients (rows: the head's slots as the forward W4 tile rows) */
+        {
+            h8 dyS[3][2];
+            {
+                const h8 m0[4] = {dy[0][0], dy[1][0], dy[2][0], dy[3][0]};
+                const h8 m1[4] = {dy[0][1], dy[1][1], dy[2][1], dy[3][1]};
+                to_sil<0, false>(m0, dyS[0]);
+                to_sil<1, false>(m0, dyS[1]);
+                to_sil<0, false>(m1, dyS[2]);
+            }
 #pragma unroll
-        for (int tt = 0; tt < 16; ++tt) {
-            int layer, ot, it;
-            grad_tile(16 * w + tt, layer, ot, it);
-            int zrow = layer == 3 ? 3 * GD_H : layer * GD_H;   /* delta rows */
-            int arow = layer == 0 ? 0 : GD_IN + (layer - 1) * GD_H; /* input rows */
+            for (int it = 0; it < 4; ++it) {
+                const h8 m[4] = {h3[0][it >> 1], h3[1][it >> 1], h3[2][it >> 1], h3[3][it >> 1]};
+                h8 aS[2];
+                if (it & 1)
+                to_sil<1, false>(m, aS);
+            else
+                to_sil<0, false>(m, aS);
 #pragma unroll
-            for (int ks = 0; ks < 4; ++ks) {
-                h4 a = *reinterpret_cast<const h4*>(&S.dz[zrow + 16 * ot + n][16 * ks + 4 * q]);
-                h4 b = *reinterpret_cast<const h4*>(&S.act[arow + 16 * it + n][16 * ks + 4 * q]);
-                gacc[tt] = mfma16(a, b, gacc[tt]);
+                for (int ot = 0; ot < 3; ++ot) grad_tile(gacc[52 + 4 * ot + it], dyS[ot], aS);
+            }
+#pragma unroll
+            for (int ot = 0; ot < 3; ++ot) bias_tile(bacc, 12 + ot, dyS[ot], ones);
+        }
+        /* back through W4, W3, W2; each layer's gradients as soon as its
+         * output gradients exist */
+        h8 dz[4][2];
+        back_relu(WB + GD_W16B_W4, dy, h3, dz);
+        hidden_back<36, 8>(gacc, bacc, WB + GD_W16B_W3, h2, dz, ones);
+        hidden_back<20, 4>(gacc, bacc, WB + GD_W16B_W2, h1, dz, ones);
+        /* W1: output gradients dz (layer 1), inputs recomputed */
+        {
+            h8 zS[4][2];
+#pragma unroll
+            for (int ot = 0; ot < 4; ++ot) {
+                const h8 m[4] = {dz[0][ot >> 1], dz[1][ot >> 1], dz[2][ot >> 1], dz[3][ot >> 1]};
+                if (ot & 1)
+                to_sil<1, false>(m, zS[ot]);
+            else
+                to_sil<0, false>(m, zS[ot]);
+                bias_tile(bacc, ot, zS[ot], ones);
+            }
+#pragma unroll
+            for (int kb = 0; kb < 3; ++kb) {
+                h8 x[4];
+#pragma unroll
+                for (int st = 0; st < 4; ++st) x[st] = feature_group(kb, q, IN[16 * st + n]);
+#pragma unroll
+                for (int t = 0; t < 2; ++t) {
+                    const int it = 2 * kb + t;
+                    if (it >= 5) continue;  /* inputs 80..95 are padding */
+                    h8 xS[2];
+                    if (t)
+                to_sil<1, true>(x, xS);
+            else
+                to_sil<0, true>(x, xS);
+#pragma unroll
+                    for (int ot = 0; ot < 4; ++ot) {
+                        f4 v = acc1[64 * (5 * ot + it)];
+                        grad_tile(v, zS[ot], xS);
+                        acc1[64 * (5 * ot + it)] = v;
+                    }
+                }
             }
         }
-        /* bias gradients: thread t < 240 sums one delta row */
-        if (threadIdx.x < 3 * GD_H + GD_OUT) {
-            int row = threadIdx.x;  /* dz1 0..63, dz2 64..127, dz3 128..191, dy 192..239 */
-            float s = 0.f;
+        wave_lds_sync();
+    }
+    /* the block's sums: the waves add their register sums in turn (into the
+     * weights' space, free once every wave is past its last chunk) */
+    __syncthreads();
+    for (int turn = 0; turn < GD_TW; ++turn) {
+        if (w == turn)
 #pragma unroll
-            for (int k = 0; k < 64; k += 4) {
-                h4 v = *reinterpret_cast<const h4*>(&S.dz[row][k]);
-                s += ((float)v[0] + (float)v[1]) + ((float)v[2] + (float)v[3]);
+            for (int t = GD_NT1; t < GD_NTILE; ++t) {
+                f4* dst = reinterpret_cast<f4*>(&L.acc[((t - GD_NT1) * 64 + lane) * 4]);
+                *dst = turn == 0 ? gacc[t] : *dst + gacc[t];
             }
-            bacc += s;
-        }
         __syncthreads();
     }
-    /* flush */
+    float* part = g.part + (size_t)blockIdx.x * GD_NPARAM;
+    for (int k = threadIdx.x; k < GD_NTILE * 256; k += blockDim.x) {
+        const int t = k >> 8, pi = grad_param(t, k & 3, (k >> 2) & 63);
+        float v;
+        if (t < GD_NT1) {
+            v = 0.f;
 #pragma unroll
-    for (int tt = 0; tt < 16; ++tt) {
-        int layer, ot, it;
-        grad_tile(16 * w + tt, layer, ot, it);
-        int in_dim = layer == 0 ? GD_IN : GD_H;
-        int off = layer == 0 ? GD_OFF_W1 : layer == 1 ? GD_OFF_W2 : layer == 2 ? GD_OFF_W3 : GD_OFF_W4;
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            int o = 16 * ot + 4 * q + e, ii = 16 * it + n;
-            float val = gacc[tt][e];
-            if (val != 0.f) atomicAdd(&g.grad[off + o * in_dim + ii], val);
+            for (int ww = 0; ww < GD_TW; ++ww) v += L.acc1[ww][k];
+        } else {
+            v = L.acc[k - GD_NT1 * 256];
         }
+        if (pi >= 0) part[pi] = v;
     }
-    if (threadIdx.x < 3 * GD_H + GD_OUT && bacc != 0.f) {
-        int row = threadIdx.x;
-        int off = row < 64 ? GD_OFF_B1 + row : row < 128 ? GD_OFF_B2 + row - 64 : row < 192 ? GD_OFF_B3 + row - 128 : GD_OFF_B4 + row - 192;
-        atomicAdd(&g.grad[off], bacc);
+    for (int k = threadIdx.x; k < GD_NBT * 16; k += blockDim.x) {
+        const int pi = bias_param(k >> 4, (k >> 2) & 3, k & 3);
+        float v = 0.f;
+#pragma unroll
+        for (int ww = 0; ww < GD_TW; ++ww) v += L.bacc[ww][k];
+        if (pi >= 0) part[pi] = v;
     }
     for (int o = 32; o > 0; o >>= 1) lacc += __shfl_xor(lacc, o);
     if (lane == 0 && lacc != 0.f) atomicAdd(&g.acc[1], lacc);
 }
 
+/* grad += the sum of the training blocks' partial sums; blockIdx.y takes a
+ * slice of the blocks */
+__global__ __launch_bounds__(GB) void k_guide_reduce(GuideState g, int nb)
+{
+    const int i = blockIdx.x * GB + threadIdx.x;
+    if (i >= GD_NPARAM || !gd_trained(i)) return;
+    const int per = (nb + gridDim.y - 1) / gridDim.y, b0 = blockIdx.y * per, b1 = min(nb, b0 + per);
+    float s = 0.f;
+    for (int b = b0; b < b1; ++b) s += g.part[(size_t)b * GD_NPARAM + i];
+    if (s != 0.f) atomicAdd(&g.grad[i], s);
+}
+
 hipError_t launch_guide_train(const GuideState& g, const float* rec, const int32_t* count, int off, int max_n,
                               int grid, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_guide_train, dim3(grid), dim3(GB), 0, st, g, rec, count, off, max_n);
+    if (grid > g.part_blocks) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_guide_train, dim3(grid), dim3(GD_TRAIN_THREADS), 0, st, g, rec, count, off, max_n);
+    hipLaunchKernelGGL(k_guide_reduce, dim3((GD_NPARAM + GB - 1) / GB, 8), dim3(GB), 0, st, g, grid);
     return hipGetLastError();
 }
 
@@ -844,12 +932,9 @@ __global__ __launch_bounds__(GB) void k_guide_adam(GuideState g, float lr, int s
         g.adam_v[i] = v;
         float th = g.theta[i] - lr * (m / c1) / (sqrtf(v / c2) + eps);
         g.theta[i] = th;
-        int fw, bw;
-        gd_w16_slots(i, &fw, &bw);
-        if (fw >= 0) g.w16[fw] = (_Float16)th;
-        if (bw >= 0) g.w16[bw] = (_Float16)th;
-        const int iw = gd_w16i_slot(i);
+        const int iw = gd_w16i_slot(i), ib = gd_w16b_slot(i);
         if (iw >= 0) g.w16i[iw] = (_Float16)th;
+        if (ib >= 0) g.w16b[ib] = (_Float16)th;
     }
 }
 
@@ -857,6 +942,25 @@ hipError_t launch_guide_adam(const GuideState& g, float lr, int step, const int3
                              hipStream_t st)
 {
     hipLaunchKernelGGL(k_guide_adam, dim3(32), dim3(GB), 0, st, g, lr, step, count, off, max_n);
+    return hipGetLastError();
+}
+
+__global__ void k_guide_keep(GuideState g, float scale)
+{
+    if (threadIdx.x != 0) return;
+    const int n = *g.rec_count;
+    const float k = *g.keep;
+    if (scale > 0.f) {
+        *g.keep = fminf(1.f, fmaxf(k * scale, 1e-7f));
+    } else if (n > 0 && k > 0.f) {
+        const float offered = (float)n / k;
+        *g.keep = fminf(1.f, fmaxf(0.9f * (float)g.rec_cap / offered, 1e-7f));
+    }
+}
+
+hipError_t launch_guide_keep(const GuideState& g, float scale, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_guide_keep, dim3(1), dim3(64), 0, st, g, scale);
     return hipGetLastError();
 }
 

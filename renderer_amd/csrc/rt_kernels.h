@@ -104,8 +104,7 @@ struct PtParams {
     int32_t mode, max_depth, rr_start;
     float rr_prob;
     uint32_t seed;
-    int32_t guide_train;  /* record training samples */
-    float rec_prob;       /* probability of keeping a training record */
+    int32_t guide_train;  /* record training samples (kept with probability *GuideState::keep) */
     /* Compat paths driven by primary samples instead of Philox streams
      * (Markov-chain renderers): group g of the path in slot p is
      * pss[g * pss_stride + p].  Null for the path-tracing renderer. */

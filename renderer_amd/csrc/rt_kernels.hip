@@ -1312,7 +1312,7 @@ __device__ void shade_guided(const DevScene& S, const PathState& ps, const Guide
     ps.ray_d[p] = make_float4(wi.x, wi.y, wi.z, RT_INF);
     ps.info[p] = make_int4(info.x, info.y, info.z, pk(0, 0, 0));
     cont = true;
-    if (P.guide_train && gx.u[2] < P.rec_prob && la > 0.f) {
+    if (P.guide_train && gx.u[2] < *G.keep && la > 0.f) {
         want_rec = true;
         rec.x = x;
         rec.nf = nf;
