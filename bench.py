@@ -48,6 +48,11 @@ def parse():
     ap.add_argument("--no-guided", action="store_true", help="skip the equal-time guided vs plain comparisons")
     ap.add_argument("--no-env", action="store_true", help="skip the env-lit mesh comparison")
     ap.add_argument("--no-room", action="store_true", help="skip the occluded-light room comparison")
+    ap.add_argument("--guide-bound", action="store_true",
+                    help="also price the first guided bounce in the room against ideal proposals")
+    ap.add_argument("--bound-points", type=int, default=64)
+    ap.add_argument("--bound-grid", type=int, default=32, help="hemisphere bins per axis")
+    ap.add_argument("--bound-rays", type=int, default=128, help="rays per bin")
     ap.add_argument("--no-batched", action="store_true", help="skip the 32-view batched render")
     ap.add_argument("--no-scene-file", action="store_true", help="skip the scene-file front-end timing")
     ap.add_argument("--no-film", action="store_true", help="skip the tonemapper and film-writer timing")
@@ -265,6 +270,196 @@ def equal_time_room(args):
             f"light outside behind a 5% door gap")
     return equal_time(scene, desc, 16, args.room_seconds, args.room_ref_spp, args.room_pass_spp,
                       args.room_train_passes, args.room_epochs, args.room_lr)
+
+
+def _vmf(mu, kappa, w):
+    """vMF densities of directions w (..., 3) for lobes mu (L, 3), kappa (L):
+    (..., L)."""
+    import numpy as np
+
+    k = np.maximum(kappa, 1e-6)
+    c = k / (2 * np.pi * -np.expm1(-2 * k))
+    return c * np.exp(k * (w @ mu.T - 1.0))
+
+
+def _fit_vmf(dirs, q, lobes=8, iters=60, seed=0):
+    """Weighted EM for a mixture of `lobes` vMF lobes on directions `dirs`
+    with masses q (Banerjee et al. 2005, kappa by their approximation):
+    the best an 8-lobe mixture does on a tabulated target."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    q = q / q.sum()
+    mu = dirs[rng.choice(len(dirs), lobes, replace=False, p=q)]
+    kappa = np.full(lobes, 10.0)
+    w = np.full(lobes, 1.0 / lobes)
+    for _ in range(iters):
+        r = w * _vmf(mu, kappa, dirs) + 1e-300
+        r /= r.sum(1, keepdims=True)
+        rq = r * q[:, None]
+        w = rq.sum(0) + 1e-12
+        R = rq.T @ dirs / w[:, None]
+        rl = np.clip(np.linalg.norm(R, axis=1), 1e-6, 1 - 1e-6)
+        mu = R / rl[:, None]
+        kappa = np.clip(rl * (3 - rl * rl) / (1 - rl * rl), 0.05, 2e4)
+        w /= w.sum()
+    return mu, kappa, w
+
+
+def guide_bound(args):
+    """How much better the first bounce could be sampled in the occluded-light
+    room, at camera-visible points of its walls, floor and ceiling (diffuse,
+    albedo 0.75): the incoming radiance over a 32 x 32 grid of
+    cosine-weighted hemisphere bins, 128 jittered rays per bin (plain PT from
+    there, rt_integrate_rays), gives the second moment of a one-sample
+    estimate of the reflected radiance for any proposal that is piecewise
+    cosine-shaped over the bins.  Reported as the gain in relMSE (variance /
+    mean^2, summed over the points) over cosine sampling, for: the
+    variance-optimal proposal (bin mass ~ sqrt of the second moment, fitted on
+    half the rays, scored on the other half); it mixed with 5% cosine (the
+    guided estimator's floor); the radiance-proportional proposal the
+    training loss aims at, mixed the same way; the best 8-lobe vMF mixture
+    of that target (weighted EM), mixed; and the trained network itself at
+    the same points (its own mixing weight).  The network is trained as in
+    the equal-time comparison.  The rest of the path stays plain PT, so this
+    prices the first guided bounce only."""
+    import numpy as np
+    import torch
+
+    from renderer_amd import LIGHT_AREA, MODE_GUIDED, MODE_UNBIASED, GpuContext, scenes
+
+    W, H = (int(x) for x in args.room_res.split("x"))
+    scene = scenes.room_scene(W, H, n=args.room_subdiv, seed=3)
+    arrays = scene.flatten()
+    ctx = GpuContext(0)
+    ctx.upload(arrays)
+    ctx.set_launch_config(guide_epochs=args.room_epochs)
+    ctx.guide_init(7)
+    common = dict(max_depth=16, rr_start=3, rr_prob=0.5)
+    film = torch.zeros((H, W, 3), dtype=torch.float32, device="cuda:0")
+    t0 = time.time()
+    for k in range(args.room_train_passes):
+        torch.cuda.synchronize()
+        ctx.render_device([(scene.camera, 4, 4)], [film.data_ptr()], seed=k, mode=MODE_GUIDED, guide_train=1,
+                          guide_lr=args.room_lr, **common)
+    torch.cuda.synchronize()
+    log(f"guide trained in {time.time() - t0:.1f} s")
+
+    # camera rays of seeded pixels; keep hits on the 12 room quads
+    cam = scene.camera
+    rng = np.random.default_rng(11)
+    w_ = cam.cop - cam.look_at
+    w_ = w_ / np.linalg.norm(w_)
+    u_ = np.cross(cam.up, w_)
+    u_ = u_ / np.linalg.norm(u_)
+    v_ = np.cross(w_, u_)
+    th = np.tan(cam.fov / 2)
+    m = 4096
+    px, py = rng.random(m) * W, rng.random(m) * H
+    dcam = ((-W / H * th + 2 * W / H * th * px / W)[:, None] * u_ + (th - 2 * th * py / H)[:, None] * v_ - w_)
+    dcam /= np.linalg.norm(dcam, axis=1, keepdims=True)
+    o4 = np.concatenate([np.tile(cam.cop, (m, 1)), np.full((m, 1), 1e-3)], 1).astype(np.float32)
+    d4 = np.concatenate([dcam, np.full((m, 1), np.inf)], 1).astype(np.float32)
+    prim, t = ctx.trace_closest(o4, d4)
+    n_lights = sum(1 for l in arrays.lights if l.type == LIGHT_AREA)
+    quad_ids = set(int(p) for p in np.unique(arrays.tri_prim)[:12])
+    P = np.asarray(arrays.positions, np.float64).reshape(-1, 3)
+    TV = np.asarray(arrays.tri_vertices).reshape(-1, 3)
+    normal_of = {}
+    for pid in quad_ids:
+        tri = TV[np.nonzero(arrays.tri_prim == pid)[0][0]]
+        nn = np.cross(P[tri[1]] - P[tri[0]], P[tri[2]] - P[tri[0]])
+        normal_of[pid] = nn / np.linalg.norm(nn)
+    pts = [i for i in range(m) if int(prim[i]) in quad_ids][: args.bound_points]
+    del n_lights
+
+    G, S = args.bound_grid, args.bound_rays  # bins per axis, rays per bin
+    B = G * G
+    gu, gv = np.meshgrid(np.arange(G), np.arange(G), indexing="ij")
+    ju = (gu.reshape(-1, 1) + rng.random((B, S))) / G
+    jv = (gv.reshape(-1, 1) + rng.random((B, S))) / G
+    sin_t = np.sqrt(ju)                # cosine-weighted: equal cosine measure per bin
+    cos_t = np.sqrt(1 - ju)
+    phi = 2 * np.pi * jv
+    local = np.stack([sin_t * np.cos(phi), sin_t * np.sin(phi), cos_t], -1)  # (B, S, 3)
+    rho = 0.75
+    names = ["ideal", "ideal_mix5", "radiance_mix5", "vmf8_mix5", "network"]
+    num = dict.fromkeys(["cosine"] + names, 0.0)
+    per_point = []
+    recs = np.zeros((len(pts), 16), np.float32)
+    frames = []
+    for j, i in enumerate(pts):
+        x = o4[i, :3].astype(np.float64) + t[i] * dcam[i]
+        nrm = normal_of[int(prim[i])]
+        if nrm @ dcam[i] > 0:
+            nrm = -nrm
+        a = np.array([1.0, 0.0, 0.0]) if abs(nrm[0]) < 0.9 else np.array([0.0, 1.0, 0.0])
+        tx = np.cross(a, nrm)
+        tx /= np.linalg.norm(tx)
+        ty = np.cross(nrm, tx)
+        frames.append((x, nrm, tx, ty))
+        recs[j, 0:3], recs[j, 3:6], recs[j, 6:9] = x, nrm, -dcam[i]
+    head = ctx.guide_head(recs) if len(pts) else np.zeros((0, 48), np.float32)
+    t0 = time.time()
+    for j, (x, nrm, tx, ty) in enumerate(frames):
+        dirs = local[..., 0:1] * tx + local[..., 1:2] * ty + local[..., 2:3] * nrm  # (B, S, 3)
+        rays_o = np.concatenate([np.tile(x + 1e-4 * nrm, (B * S, 1)), np.full((B * S, 1), 1e-4)], 1)
+        rays_d = np.concatenate([dirs.reshape(-1, 3), np.full((B * S, 1), np.inf)], 1)
+        rgb = ctx.integrate_rays(rays_o.astype(np.float32), rays_d.astype(np.float32), key0=j * B * S,
+                                 mode=MODE_UNBIASED, seed=99, max_depth=15, rr_start=2, rr_prob=0.5)
+        L = (rgb @ np.array([0.299, 0.587, 0.114])).reshape(B, S).astype(np.float64)
+        mean = L.mean(1)
+        m2a, m2b = (L[:, 0::2] ** 2).mean(1), (L[:, 1::2] ** 2).mean(1)
+        I = rho * mean.mean()
+        if not I > 0:
+            continue
+
+        def rel_var(q):
+            # one-sample second moment: sum_b q_b (rho L / (B q_b))^2
+            return (rho * rho / (B * B) * np.sum(m2b / np.maximum(q, 1e-30)) - I * I) / (I * I)
+
+        cosine = np.full(B, 1.0 / B)
+        ideal = np.sqrt(m2a) + 1e-9 * np.sqrt(m2a).mean()
+        ideal /= ideal.sum()
+        mean_a = L[:, 0::2].mean(1)
+        target = np.maximum(mean_a, 0) + 1e-9 * mean_a.mean()
+        target /= target.sum()
+        cdir = local.mean(1)
+        cdir = cdir[:, 0:1] * tx + cdir[:, 1:2] * ty + cdir[:, 2:3] * nrm
+        cdir /= np.linalg.norm(cdir, axis=1, keepdims=True)
+        mu, kap, wt = _fit_vmf(cdir, target, seed=j)
+        # bin mass of a density: sum over the bin's rays of pdf x solid angle
+        omega = np.pi / (B * S) / np.maximum(local[..., 2], 1e-6)  # solid angle per ray
+        vmf_q = (_vmf(mu, kap, dirs.reshape(-1, 3)) @ wt).reshape(B, S)
+        vmf_q = (vmf_q * omega).sum(1)
+        h = head[j]
+        mu_n = h[0:40].reshape(8, 5)[:, 0:3].astype(np.float64)
+        mu_n /= np.maximum(np.linalg.norm(mu_n, axis=1, keepdims=True), 1e-12)
+        k_n = h[0:40].reshape(8, 5)[:, 3].astype(np.float64)
+        w_n = h[0:40].reshape(8, 5)[:, 4].astype(np.float64)
+        w_n = w_n / max(w_n.sum(), 1e-12)
+        alpha = float(h[40])
+        net_q = ((_vmf(mu_n, k_n, dirs.reshape(-1, 3)) @ w_n).reshape(B, S) * omega).sum(1)
+        qs = {"cosine": cosine, "ideal": ideal, "ideal_mix5": 0.05 * cosine + 0.95 * ideal,
+              "radiance_mix5": 0.05 * cosine + 0.95 * target, "vmf8_mix5": 0.05 * cosine + 0.95 * vmf_q,
+              "network": (1 - alpha) * cosine + alpha * net_q}
+        rv = {k: rel_var(q) for k, q in qs.items()}
+        for k in num:
+            num[k] += rv[k]
+        per_point.append({"relvar_cosine": round(rv["cosine"], 4), "alpha": round(alpha, 3),
+                          **{f"gain_{k}": round(rv["cosine"] / rv[k], 3) for k in names}})
+    ctx.close()
+    log(f"guide bound: {len(per_point)} points in {time.time() - t0:.1f} s")
+    return {
+        "scene": f"occluded-light room {W}x{H}, {8 * 8 * args.room_subdiv ** 2 + 24} triangles",
+        "points": len(per_point),
+        "bins": B, "rays_per_bin": S,
+        "training": {"passes": args.room_train_passes, "spp_per_pass": 16, "epochs": args.room_epochs,
+                     "lr": args.room_lr},
+        "relmse_gain_over_cosine": {k: round(num["cosine"] / num[k], 3) for k in names},
+        "median_point_gain": {k: round(float(np.median([pp[f"gain_{k}"] for pp in per_point])), 3) for k in names},
+        "per_point": per_point[:8],
+    }
 
 
 def equal_time_env(args):
@@ -587,11 +782,14 @@ def main():
         log("scene-file front end")
         sfile = scene_file_load(args)
 
-    guided = env = None
+    guided = env = bound = None
     if not args.no_guided:
         if not args.no_room:
             log("equal-time guided vs plain: occluded-light room")
             guided = equal_time_room(args)
+        if args.guide_bound:
+            log("first-bounce proposals in the room")
+            bound = guide_bound(args)
         if not args.no_env:
             log("equal-time guided vs plain: env-lit mesh")
             env = equal_time_env(args)
@@ -658,6 +856,7 @@ def main():
         "film_outputs": fout,
         "markov_chains": mlt,
         "equal_time_guided_vs_plain": guided,
+        "guide_first_bounce_bound": bound,
         "equal_time_guided_vs_plain_env": env,
     }
     print(json.dumps(out), flush=True)

@@ -68,6 +68,9 @@
 #define GD_REC 16        /* floats per training record */
 #define GD_PARAMS 48     /* halves of head output per path vertex */
 #define GD_WCLAMP 100.0f /* cap of a normalised sample weight */
+/* probability of sampling the guide: GD_ALPHA_LO + GD_ALPHA_SPAN sigmoid(output 40) */
+#define GD_ALPHA_LO 0.05f
+#define GD_ALPHA_SPAN 0.9f
 #define GD_TRAIN_CHUNKS 4 /* training steps per render pass (record capacity = 4 batches) */
 /* Persistent grids of the network kernels: the blocks one CU holds at once
  * (LDS: 45 KB per inference block, 155 KB per training block).  More blocks

@@ -390,7 +390,7 @@ __global__ __launch_bounds__(GB) RT_GUIDE_NET_OCCUPANCY void k_guide_net(PathSta
                 o[2] = h2{(_Float16)(e0 * inv), (_Float16)mu1.x};
                 o[3] = h2{(_Float16)mu1.y, (_Float16)mu1.z};
                 o[4] = h2{(_Float16)k1, (_Float16)(e1 * inv)};
-                if (q == 0) hout[(size_t)i * GD_PARAMS + 40] = (_Float16)(0.05f + 0.9f * __builtin_amdgcn_rcpf(1.f + __expf(-r[10])));
+                if (q == 0) hout[(size_t)i * GD_PARAMS + 40] = (_Float16)(GD_ALPHA_LO + GD_ALPHA_SPAN * __builtin_amdgcn_rcpf(1.f + __expf(-r[10])));
             }
         }
     }
@@ -756,7 +756,7 @@ __global__ __launch_bounds__(GD_TRAIN_THREADS) __attribute__((amdgpu_waves_per_e
             const float pi[2] = {e0 * inv, e1 * inv};
             const float qd = quad_sum(pi[0] * vv[0] + pi[1] * vv[1]);
             const float sg = 1.f / (1.f + expf(-__shfl(r[10], n)));
-            const float alpha = 0.05f + 0.9f * sg;
+            const float alpha = GD_ALPHA_LO + GD_ALPHA_SPAN * sg;
             const float pm = alpha * qd + (1.f - alpha) * pb;
             const float c = valid ? -sw / pm : 0.f;
             const float dq = c * alpha;
@@ -777,7 +777,7 @@ __global__ __launch_bounds__(GD_TRAIN_THREADS) __attribute__((amdgpu_waves_per_e
                 dr[5 * s + 3] = dk;
                 dr[5 * s + 4] = dq * pi[s] * (vv[s] - qd);
             }
-            dr[10] = q == 0 ? c * (qd - pb) * 0.9f * sg * (1.f - sg) : 0.f;
+            dr[10] = q == 0 ? c * (qd - pb) * GD_ALPHA_SPAN * sg * (1.f - sg) : 0.f;
             dr[11] = 0.f;
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
