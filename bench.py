@@ -48,6 +48,7 @@ def parse():
     ap.add_argument("--no-guided", action="store_true", help="skip the equal-time guided vs plain comparisons")
     ap.add_argument("--no-env", action="store_true", help="skip the env-lit mesh comparison")
     ap.add_argument("--no-room", action="store_true", help="skip the occluded-light room comparison")
+    ap.add_argument("--no-sah", action="store_true", help="skip the surface-area-heuristic tree run")
     ap.add_argument("--guide-bound", action="store_true",
                     help="also price the first guided bounce in the room against ideal proposals")
     ap.add_argument("--bound-points", type=int, default=64)
@@ -270,6 +271,57 @@ def equal_time_room(args):
             f"light outside behind a 5% door gap")
     return equal_time(scene, desc, 16, args.room_seconds, args.room_ref_spp, args.room_pass_spp,
                       args.room_train_passes, args.room_epochs, args.room_lr)
+
+
+def sah_tree(ctx, arrays, views, params, film, args):
+    """The headline workload (its median-split tree is the one the workload
+    names) on a surface-area-heuristic tree of the same scene: same rays and
+    image up to ties, less traversal work.  Untimed warm-up, then the same
+    number of timed renders; nodes and records per ray from a counting
+    render.  The context goes back to the median tree afterwards."""
+    import torch
+
+    from renderer_amd import BVH_MEDIAN, BVH_SAH
+
+    t0 = time.time()
+    arrays.bvh_build = BVH_SAH
+    ctx.upload(arrays)
+    st = ctx.stats()
+    build_s = time.time() - t0
+    for i in range(args.warmup):
+        ctx.render_device(views, [film.data_ptr()], seed=1000 + i, **params)
+    torch.cuda.synchronize()
+    rays = ext_ms = sh_ms = launches = 0
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        ctx.render_device(views, [film.data_ptr()], seed=i, **params)
+        s = ctx.stats()
+        rays += s["camera_rays"] + s["extension_rays"] + s["shadow_rays"]
+        ext_ms += s["extend_ms"]
+        sh_ms += s["shadow_ms"]
+        launches += s["extend_launches"]
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ctx.set_launch_config(count_traversal=1)
+    ctx.render_device(views, [film.data_ptr()], seed=0, **params)
+    cs = ctx.stats()
+    ctx.set_launch_config(count_traversal=0)
+    ext_rays = cs["camera_rays"] + cs["extension_rays"]
+    out = {
+        "mrays_per_s": round(rays / dt / 1e6, 2),
+        "ms_per_step": round(dt / args.steps * 1e3, 3),
+        "bvh_nodes": st["bvh_nodes"], "bvh_depth": st["bvh_depth"], "stack_need": st["bvh_stack_need"],
+        "build_s": round(build_s, 2),
+        "extend_avg_launch_ms": round(ext_ms / max(1, launches), 4),
+        "shadow_avg_launch_ms": round(sh_ms / max(1, launches), 4),
+        "nodes_per_ray": round(cs["node_visits"] / max(1, ext_rays), 2),
+        "prims_per_ray": round(cs["prim_tests"] / max(1, ext_rays), 2),
+        "shadow_nodes_per_ray": round(cs["shadow_node_visits"] / max(1, cs["shadow_rays"]), 2),
+        "shadow_prims_per_ray": round(cs["shadow_prim_tests"] / max(1, cs["shadow_rays"]), 2),
+    }
+    arrays.bvh_build = BVH_MEDIAN
+    ctx.upload(arrays)
+    return out
 
 
 def _vmf(mu, kappa, w):
@@ -747,6 +799,11 @@ def main():
         l2 = tr.get("k_extend", {}).get("l2_hit_rate")
         sh_l2 = tr.get("k_shadow", {}).get("l2_hit_rate")
 
+    sah = None
+    if not args.no_sah:
+        log("the same workload on the surface-area-heuristic tree")
+        sah = sah_tree(ctx, arrays, views, params, film, args)
+
     if args.copy_calibration:
         a = torch.ones(1 << 28, dtype=torch.float32, device="cuda:0")  # 1 GiB
         b = torch.empty_like(a)
@@ -855,6 +912,7 @@ def main():
         "scene_file": sfile,
         "film_outputs": fout,
         "markov_chains": mlt,
+        "sah_tree": sah,
         "equal_time_guided_vs_plain": guided,
         "guide_first_bounce_bound": bound,
         "equal_time_guided_vs_plain_env": env,

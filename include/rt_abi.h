@@ -109,7 +109,7 @@ int rt_trace_any_device(rt_ctx* ctx, const float* o_tmin, const float* d_tmax, i
 /* rt_trace_closest_device for callers that cannot allocate device memory
  * (code under HIP graph capture, tight serving loops, harnesses that forbid
  * hipMalloc).  No counterpart in the reference.  It calls no hipMalloc or
- * hipFree: the traversal-stack overflow is a static array of 24 MiB per GPU
+ * hipFree: the traversal-stack overflow is a static array of 28 MiB per GPU
  * (held from the library's first use on that GPU until the process ends),
  * whose address rt_create looks up.  Same arguments and results as
  * rt_trace_closest_device; n above RT_NOALLOC_MAX_RAYS is RT_ERR_INVALID.

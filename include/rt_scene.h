@@ -153,7 +153,16 @@ typedef struct rt_scene_desc {
      * identity row set (A = I, b = 0) leaves the object in world space.
      * Planes are always given in world space. */
     const float* object_xform;
+
+    /* Split rule of the BVH that rt_scene_upload builds: RT_BVH_MEDIAN, the
+     * reference's (BVHNode::create, src/bvh.cpp:67-94: median of the box
+     * centres along the longest axis; leaves of up to 4 records here), or
+     * RT_BVH_SAH, the surface-area heuristic.  Hits do not depend on it
+     * (up to ties in t); the traversal work per ray does. */
+    int32_t bvh_build;
 } rt_scene_desc;
+
+enum { RT_BVH_MEDIAN = 0, RT_BVH_SAH = 1 };
 
 #define RT_CSG_MAX_LEAVES 16
 

@@ -5,7 +5,7 @@ from . import _abi
 from ._abi import (LIGHT_AREA, LIGHT_CONE, LIGHT_ENV, LIGHT_POINT, MAT_BLINN, MAT_DARK, MAT_DIFFUSE, MAT_MIRROR,
                    MAT_PHONG, MAT_REFRACTIVE, MLT_ERPT, MLT_FILTERED, MLT_METROPOLIS,
                    FLT_NO_CROSSFILTERING, FLT_NO_FILTER, FLT_NO_VARIANCE_FILTER, FLT_SIMPLE_VARIANCE_FILTER,
-                   MODE_COMPAT, MODE_GUIDED, MODE_UNBIASED, RtError)
+                   MODE_COMPAT, MODE_GUIDED, MODE_UNBIASED, BVH_MEDIAN, BVH_SAH, RtError)
 from .integrator import (EnergyRedistributionRenderer, GpuContext, GpuUniDiPathTracingIntegrator, MetropolisFltRenderer,
                          MetropolisRenderer, PerPixelRenderer)
 from .scene import (AreaLight, BlinnMaterial, Camera, ConeLight, DarkMatter, DiffuseMaterial, EnvironmentMap,

@@ -48,6 +48,7 @@ class RtSceneDesc(ctypes.Structure):
         ("textures", ctypes.c_void_p),
         ("tri_texcoords", ctypes.c_void_p),
         ("object_xform", ctypes.c_void_p),
+        ("bvh_build", ctypes.c_int32),
     ]
 
 
@@ -133,6 +134,7 @@ FLT_NO_FILTER, FLT_NO_CROSSFILTERING, FLT_NO_VARIANCE_FILTER, FLT_SIMPLE_VARIANC
 MAT_DIFFUSE, MAT_MIRROR, MAT_PHONG, MAT_DARK, MAT_BLINN, MAT_REFRACTIVE, MAT_TEXTURE = 0, 1, 2, 3, 4, 5, 6
 LIGHT_AREA, LIGHT_POINT, LIGHT_ENV, LIGHT_CONE = 0, 1, 2, 3
 MODE_COMPAT, MODE_UNBIASED, MODE_GUIDED = 0, 1, 2
+BVH_MEDIAN, BVH_SAH = 0, 1
 
 STATUS = {0: "ok", -1: "invalid argument", -2: "no scene uploaded", -3: "HIP runtime error",
           -4: "BVH build failed", -5: "cancelled", -6: "injected fault", -7: "no guiding network",

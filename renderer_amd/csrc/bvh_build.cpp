@@ -5,16 +5,21 @@
  * The reference builds a pointer tree with one primitive per leaf by
  * sorting the whole list at every level (BVHNode::create, src/bvh.cpp:67-94:
  * longest axis of the list's box, std::sort by box centre, split at n/2).
- * This builder keeps the median split but:
- *   - partitions with nth_element (O(n) per level instead of a full sort),
- *   - uses the longest axis of the box-centre bounds,
- *   - stops at leaves of at most 4 primitives of one kind,
- *   - runs the top levels on several threads; the result does not depend
- *     on the thread schedule (each range is partitioned by one thread and
- *     the layout pass below is sequential),
- *   - writes nodes depth-first with both child boxes in the parent.
+ * This builder:
+ *   - splits by the surface-area heuristic over 32 bins of the box centres
+ *     per axis (Wald 2007), falling back to the centre median where no plane
+ *     separates the centres, and to the median for every node below a
+ *     binary level cap when the SAH tree would need more traversal stack
+ *     than RT_STACK_DEPTH (build_best lowers the cap until it fits);
+ *   - can also keep the reference's median split (nth_element, O(n) per
+ *     level instead of a full sort, longest axis of the centre bounds);
+ *   - stops at leaves of at most 4 primitives of one kind;
+ *   - builds the top levels on several threads; the result does not depend
+ *     on the thread schedule (each subtree is built by one thread into its
+ *     own list, appended in a fixed order);
+ *   - collapses the binary tree into 4-wide nodes written depth-first.
  * Closest-hit results do not depend on the tree shape except for exact
- * ties in t.
+ * ties in t; only the work per ray does.
  */
 #include "bvh_build.h"
 
@@ -48,58 +53,143 @@ bool is_leaf(const std::vector<PrimRef>& refs, size_t b, size_t e)
     return true;
 }
 
-void partition(std::vector<PrimRef>& refs, size_t b, size_t e, int level)
-{
-    if (e - b <= 1 || is_leaf(refs, b, e)) return;
-    float clo[3] = {INFINITY, INFINITY, INFINITY}, chi[3] = {-INFINITY, -INFINITY, -INFINITY};
-    for (size_t i = b; i < e; ++i)
-        for (int k = 0; k < 3; ++k) {
-            clo[k] = std::min(clo[k], refs[i].c[k]);
-            chi[k] = std::max(chi[k], refs[i].c[k]);
-        }
-    float dx = chi[0] - clo[0], dy = chi[1] - clo[1], dz = chi[2] - clo[2];
-    int axis = dx > dy && dx > dz ? 0 : (dy > dz ? 1 : 2);
-    size_t mid = b + (e - b) / 2;
-    std::nth_element(refs.begin() + b, refs.begin() + mid, refs.begin() + e,
-                     [axis](const PrimRef& a, const PrimRef& c) {
-                         if (a.c[axis] != c.c[axis]) return a.c[axis] < c.c[axis];
-                         return a.g < c.g;
-                     });
-    if (level < 4 && e - b > 65536) {
-        std::thread t([&refs, b, mid, level] { partition(refs, b, mid, level + 1); });
-        partition(refs, mid, e, level + 1);
-        t.join();
-    } else {
-        partition(refs, b, mid, level + 1);
-        partition(refs, mid, e, level + 1);
-    }
-}
-
-/* Binary tree of the median splits (intermediate form). */
+/* Binary tree (intermediate form). */
 struct BinNode {
     Box3 box[2];
     int32_t child[2];  // >= 0: binary interior node, < 0: leaf word
 };
 
-struct Binary {
-    const std::vector<PrimRef>& refs;
-    std::vector<BinNode> nodes;
+float area(const Box3& b)
+{
+    const float x = b.hi[0] - b.lo[0], y = b.hi[1] - b.lo[1], z = b.hi[2] - b.lo[2];
+    return x < 0.f ? 0.f : x * y + y * z + z * x;
+}
 
-    // Returns the child word for range [b, e) and its box.
-    int32_t emit(size_t b, size_t e, Box3& box)
+/* Split point of [b, e) by the median of the box centres along the
+ * longest axis of their bounds (ties by shading index). */
+size_t split_median(std::vector<PrimRef>& refs, size_t b, size_t e, const float* clo, const float* chi)
+{
+    const float dx = chi[0] - clo[0], dy = chi[1] - clo[1], dz = chi[2] - clo[2];
+    const int axis = dx > dy && dx > dz ? 0 : (dy > dz ? 1 : 2);
+    const size_t mid = b + (e - b) / 2;
+    std::nth_element(refs.begin() + b, refs.begin() + mid, refs.begin() + e,
+                     [axis](const PrimRef& a, const PrimRef& c) {
+                         if (a.c[axis] != c.c[axis]) return a.c[axis] < c.c[axis];
+                         return a.g < c.g;
+                     });
+    return mid;
+}
+
+/* Split point of [b, e) by the surface-area heuristic over SAH_BINS bins of
+ * the centre bounds per axis (Wald 2007): the plane minimising
+ * area(left) * ceil(n_left / 4) + area(right) * ceil(n_right / 4) (leaves
+ * hold up to four records, so a record count costs in steps of four).
+ * Records keep their relative order within each side.  Returns b when no
+ * plane separates the centres. */
+constexpr int SAH_BINS = 32;
+size_t split_sah(std::vector<PrimRef>& refs, size_t b, size_t e, const float* clo, const float* chi)
+{
+    float best = INFINITY;
+    int best_axis = -1, best_plane = 0;
+    for (int axis = 0; axis < 3; ++axis) {
+        const float ext = chi[axis] - clo[axis];
+        if (!(ext > 0.f)) continue;
+        const float scale = SAH_BINS / ext;
+        Box3 bin[SAH_BINS];
+        size_t cnt[SAH_BINS] = {};
+        for (size_t i = b; i < e; ++i) {
+            int k = std::min(SAH_BINS - 1, (int)((refs[i].c[axis] - clo[axis]) * scale));
+            bin[k].grow(refs[i].lo, refs[i].hi);
+            ++cnt[k];
+        }
+        float right_area[SAH_BINS];
+        size_t right_cnt[SAH_BINS];
+        Box3 acc;
+        size_t n = 0;
+        for (int k = SAH_BINS - 1; k > 0; --k) {
+            acc.grow(bin[k].lo, bin[k].hi);
+            n += cnt[k];
+            right_area[k] = area(acc);
+            right_cnt[k] = n;
+        }
+        Box3 left;
+        size_t nl = 0;
+        for (int k = 0; k < SAH_BINS - 1; ++k) {
+            left.grow(bin[k].lo, bin[k].hi);
+            nl += cnt[k];
+            const size_t nr = right_cnt[k + 1];
+            if (nl == 0 || nr == 0) continue;
+            const float cost = area(left) * (float)((nl + 3) / 4) + right_area[k + 1] * (float)((nr + 3) / 4);
+            if (cost < best) {
+                best = cost;
+                best_axis = axis;
+                best_plane = k + 1;
+            }
+        }
+    }
+    if (best_axis < 0) return b;
+    const int axis = best_axis;
+    const float lo = clo[axis], scale = SAH_BINS / (chi[axis] - clo[axis]);
+    auto it = std::stable_partition(refs.begin() + b, refs.begin() + e, [&](const PrimRef& r) {
+        return std::min(SAH_BINS - 1, (int)((r.c[axis] - lo) * scale)) < best_plane;
+    });
+    return (size_t)(it - refs.begin());
+}
+
+/* Builds the binary tree of [b, e) into `out` (depth-first, parent first)
+ * and returns its child word and box.  The first levels of large ranges
+ * build their second half on another thread into a list of their own,
+ * appended after the first half's nodes, so the result does not depend on
+ * the thread schedule. */
+struct Builder {
+    std::vector<PrimRef>& refs;
+    int method, sah_levels;
+
+    int32_t emit(size_t b, size_t e, int level, std::vector<BinNode>& out, Box3& box)
     {
         if (is_leaf(refs, b, e)) {
             for (size_t i = b; i < e; ++i) box.grow(refs[i].lo, refs[i].hi);
             uint32_t x = ((uint32_t)b << 5) | ((uint32_t)refs[b].kind << 3) | (uint32_t)(e - b);
             return (int32_t)~x;
         }
-        size_t idx = nodes.size();
-        nodes.emplace_back();
-        size_t mid = b + (e - b) / 2;
+        float clo[3] = {INFINITY, INFINITY, INFINITY}, chi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (size_t i = b; i < e; ++i)
+            for (int k = 0; k < 3; ++k) {
+                clo[k] = std::min(clo[k], refs[i].c[k]);
+                chi[k] = std::max(chi[k], refs[i].c[k]);
+            }
+        size_t mid = b;
+        if (e - b <= RT_LEAF_MAX) {
+            /* a small range of mixed kinds: one leaf per kind */
+            std::stable_sort(refs.begin() + b, refs.begin() + e,
+                             [](const PrimRef& x, const PrimRef& y) { return x.kind < y.kind; });
+            mid = b + 1;
+            while (refs[mid].kind == refs[b].kind) ++mid;
+        } else {
+            if (method == BUILD_SAH && level < sah_levels) mid = split_sah(refs, b, e, clo, chi);
+            if (mid == b || mid == e) mid = split_median(refs, b, e, clo, chi);
+        }
+        const size_t idx = out.size();
+        out.emplace_back();
         BinNode n;
-        n.child[0] = emit(b, mid, n.box[0]);
-        n.child[1] = emit(mid, e, n.box[1]);
-        nodes[idx] = n;
+        if (level < 4 && e - b > 65536) {
+            std::vector<BinNode> second;
+            int32_t c1 = 0;
+            std::thread t([&] { c1 = emit(mid, e, level + 1, second, n.box[1]); });
+            n.child[0] = emit(b, mid, level + 1, out, n.box[0]);
+            t.join();
+            const int32_t off = (int32_t)out.size();
+            for (BinNode m : second) {
+                for (int s = 0; s < 2; ++s)
+                    if (m.child[s] >= 0) m.child[s] += off;
+                out.push_back(m);
+            }
+            n.child[1] = c1 >= 0 ? c1 + off : c1;
+        } else {
+            n.child[0] = emit(b, mid, level + 1, out, n.box[0]);
+            n.child[1] = emit(mid, e, level + 1, out, n.box[1]);
+        }
+        out[idx] = n;
         box.grow(n.box[0].lo, n.box[0].hi);
         box.grow(n.box[1].lo, n.box[1].hi);
         return (int32_t)idx;
@@ -163,7 +253,7 @@ struct Wide {
 
 }  // namespace
 
-int build(std::vector<PrimRef>& refs, BuildResult& out)
+int build(std::vector<PrimRef>& refs, BuildResult& out, int method, int sah_levels)
 {
     out.nodes.clear();
     out.depth = 0;
@@ -172,27 +262,36 @@ int build(std::vector<PrimRef>& refs, BuildResult& out)
     for (auto& r : refs)
         for (int k = 0; k < 3; ++k) r.c[k] = 0.5f * (r.lo[k] + r.hi[k]);
     if (refs.empty()) return 0;
-    partition(refs, 0, refs.size(), 0);
-    Binary bt{refs, {}};
-    bt.nodes.reserve(refs.size() / 2 + 1);
+    Builder builder{refs, method, sah_levels};
+    std::vector<BinNode> bin;
+    bin.reserve(refs.size() / 2 + 1);
     Box3 root;
-    int32_t r = bt.emit(0, refs.size(), root);
+    int32_t r = builder.emit(0, refs.size(), 0, bin, root);
     if (r < 0) {
         // A root that is a leaf still gets an interior node.
         BinNode n;
         n.box[0] = root;
         n.child[0] = r;
         n.child[1] = (int32_t)~(uint32_t)0;  // empty leaf with an empty box
-        bt.nodes.push_back(n);
+        bin.push_back(n);
         r = 0;
     }
-    Wide wt{bt.nodes, out.nodes};
+    Wide wt{bin, out.nodes};
     int need = 0;
     wt.collapse(r, 0, need);
     out.depth = wt.max_depth + 1;
     out.stack_need = need;
     if (need + 1 > RT_STACK_DEPTH) return -2;
     return 0;
+}
+
+int build_best(std::vector<PrimRef>& refs, BuildResult& out)
+{
+    for (int levels : {64, 32, 24, 16, 8}) {
+        const int r = build(refs, out, BUILD_SAH, levels);
+        if (r != -2) return r;
+    }
+    return build(refs, out, BUILD_MEDIAN);
 }
 
 }  // namespace rtb

@@ -27,9 +27,19 @@ struct BuildResult {
     int stack_need = 0;        // most traversal-stack entries a ray can hold at once
 };
 
+// Split rule of the interior nodes: median of the box centres along the
+// longest axis, or the surface-area heuristic (binned) down to binary level
+// sah_levels and the median below it (the median keeps subtrees balanced,
+// which bounds the depth and so the traversal stack).
+enum { BUILD_MEDIAN = 0, BUILD_SAH = 1 };
+
 // Returns 0 on success, -1 if there are too many primitives for the leaf
 // encoding, -2 if a traversal could need more stack than RT_STACK_DEPTH.
-int build(std::vector<PrimRef>& refs, BuildResult& out);
+int build(std::vector<PrimRef>& refs, BuildResult& out, int method = BUILD_MEDIAN, int sah_levels = 64);
+
+// The SAH tree with the most SAH levels (64, 32, 24, 16, 8) whose traversal
+// fits the stack, else the median tree.  Same return values as build().
+int build_best(std::vector<PrimRef>& refs, BuildResult& out);
 
 }  // namespace rtb
 

@@ -332,6 +332,7 @@ static int check_prims(rt_ctx* c, const rt_scene_desc* d, int* env_out)
 {
     if (d->n_vertices < 0 || d->n_triangles < 0 || d->n_spheres < 0 || d->n_materials < 0 || d->n_lights < 0)
         return set_err(c, RT_ERR_INVALID, "negative count");
+    if (d->bvh_build != RT_BVH_MEDIAN && d->bvh_build != RT_BVH_SAH) return set_err(c, RT_ERR_INVALID, "unknown bvh_build");
     if ((d->n_triangles && (!d->tri_vertices || !d->tri_material || !d->tri_prim_id || !d->positions)) ||
         (d->n_spheres && (!d->spheres || !d->sphere_material || !d->sphere_prim_id)) ||
         (d->n_materials && !d->materials) || (d->n_lights && !d->lights))
@@ -421,7 +422,7 @@ static int build_bvh(rt_ctx* c, const rt_scene_desc* d, rtb::BuildResult& br, st
             lo[k] = std::min(lo[k], r.lo[k]);
             hi[k] = std::max(hi[k], r.hi[k]);
         }
-    int br_status = rtb::build(refs, br);
+    int br_status = d->bvh_build == RT_BVH_SAH ? rtb::build_best(refs, br) : rtb::build(refs, br);
     if (br_status != 0)
         return set_err(c, RT_ERR_BUILD, br_status == -2 ? "BVH deeper than the traversal stack" : "too many primitives");
     /* node fetches use 32-bit byte offsets from the node base */

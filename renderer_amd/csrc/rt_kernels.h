@@ -205,7 +205,7 @@ hipError_t launch_trace_count(const HostDevScene& s, const float4* o, const floa
  * entries per lane): noalloc_scratch_init(device), with that device current,
  * looks up its address once per device; noalloc_scratch(device) returns it
  * (nullptr before the lookup). */
-#define RT_NOALLOC_BLOCKS 2048
+#define RT_NOALLOC_BLOCKS 1024
 #define RT_NOALLOC_MAX_DEVICES 64
 hipError_t noalloc_scratch_init(int device);
 int32_t* noalloc_scratch(int device);

@@ -203,7 +203,7 @@ hipError_t launch_trace_count(const HostDevScene& s, const float4* o, const floa
  * Allocation-free closest hit (rt_trace_closest_device_noalloc): the stack
  * overflow of a launch of at most RT_NOALLOC_BLOCKS blocks lives in this
  * file-scope array instead of memory the context allocates.  Each GPU holds
- * its own copy (24 MiB) from the first use of the library on it until the
+ * its own copy (28 MiB) from the first use of the library on it until the
  * process ends.  Host code never passes `scratch` itself to a kernel: that
  * name is a host-side stand-in, not a device address.  rt_create looks up the
  * array's address on its device once (noalloc_scratch_init) and every launch

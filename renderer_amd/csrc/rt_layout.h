@@ -68,7 +68,7 @@ static inline RT_HD int rt_probe_texel(float dx, float dy, float dz, int W, int 
 #define RT_KIND_PARA 2
 
 #define RT_LEAF_MAX 4
-#define RT_STACK_DEPTH 32
+#define RT_STACK_DEPTH 48
 /* traversal stack entries per lane: in LDS, in registers, in global overflow */
 #ifndef RT_STACK_LDS
 #define RT_STACK_LDS 16

@@ -301,6 +301,7 @@ class SceneArrays:
         self.object_prim = np.zeros(0, np.int32) if object_prim is None else np.asarray(object_prim, np.int32)
         self.textures = list(textures)
         self.tri_texcoords = tri_texcoords
+        self.bvh_build = 0  # BVH split rule: 0 median (the reference's), 1 surface-area heuristic
         self._desc = None
 
     @property
@@ -314,6 +315,7 @@ class SceneArrays:
 
     def desc(self):
         if self._desc is not None:
+            self._desc.bvh_build = self.bvh_build
             return self._desc
         d = _abi.RtSceneDesc()
 
@@ -352,6 +354,7 @@ class SceneArrays:
             d.textures = ctypes.addressof(self._tex_c)
         if self.tri_texcoords is not None:
             d.tri_texcoords = self.tri_texcoords.ctypes.data
+        d.bvh_build = self.bvh_build
         self._desc = d
         return d
 

@@ -25,7 +25,7 @@ import numpy as np
 import pytest
 
 from oracle.oracle import Oracle
-from renderer_amd import MODE_COMPAT, MODE_GUIDED, MODE_UNBIASED, scenes
+from renderer_amd import BVH_MEDIAN, BVH_SAH, MODE_COMPAT, MODE_GUIDED, MODE_UNBIASED, scenes
 from renderer_amd.scene import PointLight
 from tests.helpers import camera_rays, compare_hits, random_rays, rel_mse
 from tests.test_oracle_phong_roulette import (_mixture, closed_diffuse_box, nb_length_weights, phong_plane_expected,
@@ -92,6 +92,16 @@ def _tree_parity(gpu_ctx, sc, lo, hi, n_random, cam_stride):
     _say(f"tree: {exact} exact, {ties} ties, {len(bad)} differ; hit fraction {(pr >= 0).mean():.3f}")
     assert len(bad) == 0
     assert (pr >= 0).mean() > 0.3
+    # the surface-area-heuristic tree of the same scene: the same hits
+    arr.bvh_build = BVH_SAH
+    gpu_ctx.upload(arr)
+    st = gpu_ctx.stats()
+    _say(f"SAH tree: {st['bvh_nodes']} nodes, depth {st['bvh_depth']}, stack need {st['bvh_stack_need']}")
+    ps, ts = gpu_ctx.trace_closest(o4, d4)
+    exact, ties, bad = compare_hits(orc, o4, d4, ps, ts, pr, tr)
+    _say(f"SAH tree: {exact} exact, {ties} ties, {len(bad)} differ")
+    assert len(bad) == 0
+    arr.bvh_build = BVH_MEDIAN
     return len(o4)
 
 

@@ -3,12 +3,13 @@ rt_trace_any_count_device) against the oracle's traversal of the same
 exported BVH in the same visiting order (oc_flat_trace): interior nodes
 fetched and primitive records tested must agree ray for ray, exactly.  The
 algorithmic bytes of the traversal kernels' roofline are these counts times
-128 B per node and 48 B per record, so this is what they rest on."""
+128 B per node and 48 B per record, so this is what they rest on.  Both
+split rules of the tree (median, surface-area heuristic) are checked."""
 import numpy as np
 import pytest
 
 from oracle.oracle import flat_trace
-from renderer_amd import scenes
+from renderer_amd import BVH_MEDIAN, BVH_SAH, scenes
 from renderer_amd.integrator import bvh_export
 from tests.helpers import camera_rays, random_rays
 from tests.test_oracle_flat_bvh import _sphere_cloud_scene
@@ -32,6 +33,13 @@ def _device_counts(ctx, o4, d4, any_hit):
 
 
 def _check(ctx, arr, o4, d4, label):
+    for rule in (BVH_MEDIAN, BVH_SAH):
+        arr.bvh_build = rule
+        _check_tree(ctx, arr, o4, d4, f"{label}, {'SAH' if rule == BVH_SAH else 'median'} tree")
+    arr.bvh_build = BVH_MEDIAN
+
+
+def _check_tree(ctx, arr, o4, d4, label):
     ctx.upload(arr)
     bvh_nodes, recs, _ = bvh_export(arr)
     for any_hit in (False, True):
