@@ -702,9 +702,47 @@ def cpu_baseline(arrays, scene, budget_s):
         frames += 1
     orc.close()
     return {"value": round(rays / t_render / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
-            "cpu": cpu_model, "cpu_mhz": cpu_mhz,
+            "cpu": cpu_model, "cpu_mhz": cpu_mhz, "box_c1": box_c1(threads),
             "sample": f"{frames} frames of {cam.width}x{cam.height} at 1 spp (seeds 0..{frames - 1}), {rays} rays "
                       f"in {t_render:.1f} s on {threads} threads; reference BVH build {build_s:.1f} s not timed"}
+
+
+def box_c1(threads):
+    """The box scene at its configuration (256 x 256, 16 spp jittered, max
+    depth 8, roulette after 3) rendered whole in both estimators, by the
+    oracle on the host's cores and by the GPU: wall time and Mrays/s of each
+    (one warm-up render on the GPU first)."""
+    import torch
+
+    from oracle.oracle import Oracle
+    from renderer_amd import MODE_COMPAT, MODE_UNBIASED, GpuContext, scenes
+
+    scene = scenes.box_scene(256, 256)
+    arrays = scene.flatten()
+    orc = Oracle(arrays, accel=1)
+    ctx = GpuContext(0)
+    ctx.upload(arrays)
+    film = torch.zeros((256, 256, 3), dtype=torch.float32, device="cuda:0")
+    out = {"scene": "box, 256x256, 16 spp, max depth 8, roulette after 3 (p 0.5)"}
+    for name, mode in (("compat", MODE_COMPAT), ("unbiased", MODE_UNBIASED)):
+        kw = dict(mode=mode, max_depth=8, rr_start=3, rr_prob=0.5)
+        t0 = time.time()
+        _, r = orc.render((scene.camera, 4, 4), seed=0, threads=threads, **kw)
+        cpu_s = time.time() - t0
+        ctx.render_device([(scene.camera, 4, 4)], [film.data_ptr()], seed=1, **kw)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ctx.render_device([(scene.camera, 4, 4)], [film.data_ptr()], seed=0, **kw)
+        torch.cuda.synchronize()
+        gpu_s = time.perf_counter() - t0
+        st = ctx.stats()
+        grays = st["camera_rays"] + st["extension_rays"] + st["shadow_rays"]
+        out[name] = {"cpu_s": round(cpu_s, 3), "cpu_mrays_per_s": round(r / cpu_s / 1e6, 3),
+                     "gpu_s": round(gpu_s, 5), "gpu_mrays_per_s": round(grays / gpu_s / 1e6, 1)}
+    orc.close()
+    ctx.close()
+    out["cores"] = threads
+    return out
 
 
 def host_cpu():
