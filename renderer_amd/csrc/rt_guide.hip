@@ -742,34 +742,41 @@ __global__ __launch_bounds__(GD_TRAIN_THREADS) __attribute__((amdgpu_waves_per_e
             for (int ot = 0; ot < 3; ++ot)
 #pragma unroll
                 for (int e = 0; e < 4; ++e) r[4 * ot + e] = y[st][ot][e];
+            /* the hardware exponential, logarithm and reciprocal (about 1
+             * ulp): the gradients are rounded to FP16 right after */
             vec3 mu[2];
-            float mlen[2], kap[2], vv[2];
+            float rlen[2], kap[2], vv[2], om1[2];
             bool kin[2];
 #pragma unroll
             for (int s = 0; s < 2; ++s) {
-                lobe_head(r + 5 * s, mu[s], mlen[s], kap[s], kin[s]);
-                vv[s] = vmf_pdf(mu[s], kap[s], om);
+                const float* rs = r + 5 * s;
+                const vec3 m = mk(rs[0], rs[1], rs[2]);
+                rlen[s] = __builtin_amdgcn_rcpf(fmaxf(sqrtf(dot(m, m)), 1e-6f));
+                mu[s] = m * rlen[s];
+                kin[s] = rs[3] > GD_KLO && rs[3] < GD_KHI;
+                kap[s] = __expf(fminf(fmaxf(rs[3], GD_KLO), GD_KHI));
+                om1[s] = 1.f - __expf(-2.f * kap[s]); /* >= 0.094 for kappa >= e^-3 */
+                vv[s] = kap[s] * __builtin_amdgcn_rcpf(6.28318530718f * om1[s]) * __expf(kap[s] * (dot(mu[s], om) - 1.f));
             }
             const float mx = quad_max(fmaxf(r[4], r[9]));
-            const float e0 = expf(r[4] - mx), e1 = expf(r[9] - mx);
-            const float inv = 1.f / quad_sum(e0 + e1);
+            const float e0 = __expf(r[4] - mx), e1 = __expf(r[9] - mx);
+            const float inv = __builtin_amdgcn_rcpf(quad_sum(e0 + e1));
             const float pi[2] = {e0 * inv, e1 * inv};
             const float qd = quad_sum(pi[0] * vv[0] + pi[1] * vv[1]);
-            const float sg = 1.f / (1.f + expf(-__shfl(r[10], n)));
+            const float sg = __builtin_amdgcn_rcpf(1.f + __expf(-__shfl(r[10], n)));
             const float alpha = GD_ALPHA_LO + GD_ALPHA_SPAN * sg;
             const float pm = alpha * qd + (1.f - alpha) * pb;
-            const float c = valid ? -sw / pm : 0.f;
+            const float c = valid ? -sw * __builtin_amdgcn_rcpf(pm) : 0.f;
             const float dq = c * alpha;
-            if (valid && q == 0) lacc += -sw * logf(pm);
+            if (valid && q == 0) lacc += -sw * __logf(pm);
             float dr[12];
 #pragma unroll
             for (int s = 0; s < 2; ++s) {
                 const float dv = dq * pi[s];
                 const float cw = dot(mu[s], om);
-                const float em = expf(-2.f * kap[s]);
-                const float dlogc = 1.f - 2.f * kap[s] * em / (-expm1f(-2.f * kap[s]));
+                const float dlogc = 1.f - 2.f * kap[s] * (1.f - om1[s]) * __builtin_amdgcn_rcpf(om1[s]);
                 const float dk = kin[s] ? dv * vv[s] * (dlogc + kap[s] * (cw - 1.f)) : 0.f;
-                const float gm = dv * vv[s] * kap[s] / mlen[s];
+                const float gm = dv * vv[s] * kap[s] * rlen[s];
                 const vec3 dm = (om - mu[s] * cw) * gm;
                 dr[5 * s] = dm.x;
                 dr[5 * s + 1] = dm.y;
