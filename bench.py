@@ -220,10 +220,21 @@ def equal_time(scene, desc, max_depth, budget_s, ref_spp, pass_spp, train_passes
     plain = progressive(MODE_UNBIASED, budget_s, 0)
     ctx.guide_init(7)
     guided = progressive(MODE_GUIDED, budget_s, train_passes)
+    # the guided renderer with plain PT as fallback and error judge: after
+    # training, six passes of each estimator decide which one renders the
+    # rest of the budget (its own seeds; the probe is on the clock)
+    from renderer_amd.progressive import GuidedProgressive
+    ctx.guide_init(7)
+    adaptive = []
+    gp = GuidedProgressive(ctx, scene.camera, pass_spp=pass_spp, train_passes=train_passes, probe_passes=6, lr=lr,
+                           **common)
+    _, ainfo = gp.run(budget_s, seed0=50000,
+                      on_pass=lambda t, spp, img: adaptive.append((t, spp, rel_mse(img.cpu().numpy(), ref) - ref_noise)))
     ctx.close()
     rp = plain[-1][2]
     # time at which guided first reaches the plain estimator's final relMSE
     t_reach = next((t for t, _, e in guided if e <= rp), None)
+    t_reach_a = next((t for t, _, e in adaptive if e <= rp), None)
 
     def tflops(flop, ms):
         return round(flop / (ms * 1e-3) / 1e12, 2) if ms > 0 else None
@@ -244,6 +255,13 @@ def equal_time(scene, desc, max_depth, budget_s, ref_spp, pass_spp, train_passes
         "relmse_ratio_guided_over_plain": round(guided[-1][2] / rp, 4) if guided else None,
         "time_to_plain_relmse_s": round(t_reach, 3) if t_reach else None,
         "speedup_to_equal_relmse": round(plain[-1][0] / t_reach, 3) if t_reach else None,
+        "guided_with_fallback": {
+            "relmse": round(adaptive[-1][2], 6), "spp": adaptive[-1][1], "seconds": round(adaptive[-1][0], 3),
+            "choice": ainfo["choice"],
+            "probe": {k: round(v, 6) for k, v in ainfo.get("probe", {}).items()},
+            "relmse_ratio_over_plain": round(adaptive[-1][2] / rp, 4),
+            "speedup_to_equal_relmse": round(plain[-1][0] / t_reach_a, 3) if t_reach_a else None,
+        },
         "network": {
             "infer_ms": round(net["infer_ms"], 2), "infer_samples": int(net["infer_samples"]),
             "infer_tflops": inf_tf,

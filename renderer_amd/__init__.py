@@ -8,6 +8,7 @@ from ._abi import (LIGHT_AREA, LIGHT_CONE, LIGHT_ENV, LIGHT_POINT, MAT_BLINN, MA
                    MODE_COMPAT, MODE_GUIDED, MODE_UNBIASED, BVH_MEDIAN, BVH_SAH, RtError)
 from .integrator import (EnergyRedistributionRenderer, GpuContext, GpuUniDiPathTracingIntegrator, MetropolisFltRenderer,
                          MetropolisRenderer, PerPixelRenderer)
+from .progressive import GuidedProgressive
 from .scene import (AreaLight, BlinnMaterial, Camera, ConeLight, DarkMatter, DiffuseMaterial, EnvironmentMap,
                     IntersectableList, Mesh, MirrorMaterial, PhongMaterial, PointLight, Quad, Scene, Sphere,
                     TextureMaterial, TransparentMaterial, Triangle)
@@ -20,5 +21,6 @@ __all__ = [
     "BlinnMaterial", "TransparentMaterial", "TextureMaterial", "ConeLight", "MetropolisRenderer", "EnergyRedistributionRenderer",
     "MetropolisFltRenderer", "MLT_FILTERED", "FLT_NO_FILTER", "FLT_NO_CROSSFILTERING", "FLT_NO_VARIANCE_FILTER",
     "FLT_SIMPLE_VARIANCE_FILTER",
-    "MODE_COMPAT", "MODE_UNBIASED", "MODE_GUIDED", "MLT_METROPOLIS", "MLT_ERPT",
+    "MODE_COMPAT", "MODE_UNBIASED", "MODE_GUIDED", "MLT_METROPOLIS", "MLT_ERPT", "BVH_MEDIAN", "BVH_SAH",
+    "GuidedProgressive",
 ]
