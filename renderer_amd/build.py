@@ -15,9 +15,14 @@ SOURCES = ["rt_kernels.hip", "rt_guide.hip", "rt_film.hip", "rt_mlt.hip", "rt_fi
            "film_io.cpp"]
 
 # -packed-fp32-ops: keep the compiler from forming packed FP32 instructions.
+# -amdgpu-mfma-vgpr-form (device code only): MFMA results in architectural
+# VGPRs, so that the VALU reads them without copies out of the accumulation
+# registers (the training kernel holds its gradient sums in registers and
+# otherwise moved every other MFMA result through them).
 HIPCC_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "-shared", "--offload-arch=gfx950",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Xarch_device", "-mllvm=-amdgpu-mfma-vgpr-form",
     "-Wno-unused-command-line-argument",
 ]
 

@@ -74,15 +74,38 @@ __device__ __forceinline__ void lobe_head_fast(const float* r, vec3& mu, float& 
     kappa = __expf(fminf(fmaxf(r[3], GD_KLO), GD_KHI));
 }
 
+/* Reductions over the four lanes l, l ^ 16, l ^ 32, l ^ 48 (the four lane
+ * groups that hold one sample) with gfx950's lane swaps, which are VALU
+ * instructions (no LDS round trip).  permlane32_swap(v, v) returns
+ * {v of lane l % 32, v of lane 32 + l % 32}; permlane16_swap(v, v) returns
+ * {v of row 2 (r / 2), v of row 2 (r / 2) + 1} for a lane in row r of 16
+ * lanes.  Either pair sums (or maxes) to the reduction over its two lanes. */
+__device__ __forceinline__ f2 swap32(float v)
+{
+    const auto p = __builtin_amdgcn_permlane32_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+    return f2{__uint_as_float(p[0]), __uint_as_float(p[1])};
+}
+__device__ __forceinline__ f2 swap16(float v)
+{
+    const auto p = __builtin_amdgcn_permlane16_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+    return f2{__uint_as_float(p[0]), __uint_as_float(p[1])};
+}
 __device__ __forceinline__ float quad_max(float v)
 {
-    v = fmaxf(v, __shfl_xor(v, 16));
-    return fmaxf(v, __shfl_xor(v, 32));
+    const f2 a = swap32(v);
+    const f2 b = swap16(fmaxf(a[0], a[1]));
+    return fmaxf(b[0], b[1]);
 }
 __device__ __forceinline__ float quad_sum(float v)
 {
-    v += __shfl_xor(v, 16);
-    return v + __shfl_xor(v, 32);
+    const f2 a = swap32(v);
+    const f2 b = swap16(a[0] + a[1]);
+    return b[0] + b[1];
+}
+/* v of lane l % 16 (row 0) in every lane */
+__device__ __forceinline__ float row0_bcast(float v)
+{
+    return swap16(swap32(v)[0])[0];
 }
 
 /* ===========================================================================
@@ -177,61 +200,71 @@ __device__ __forceinline__ h8 feature_group(int kb, int q, const float* v)
 {
     if (kb == 0) return freq_group(q < 3 ? v[0] : v[1], 4 * (q < 3 ? q : 0));
     if (kb == 1) return freq_group(q < 2 ? v[1] : v[2], 4 * ((q + 1) % 3));
-    if (q == 0) return freq_group(v[2], 8);
-    h8 r = {0, 0, 0, 0, 0, 0, 0, 0};
-    if (q == 1)
+    /* kb 2 without branches on the lane group (a divergent branch would run
+     * every case for the whole wave anyway): the frequency group of lane
+     * group 0, the six direction values of group 1, zeros elsewhere */
+    const h8 f = freq_group(v[2], 8);
+    typedef uint32_t u4v __attribute__((ext_vector_type(4)));
+    h8 d = {0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
-        for (int j = 0; j < 6; ++j) r[j] = (_Float16)v[3 + j];
-    return r;
+    for (int j = 0; j < 6; ++j) d[j] = (_Float16)v[3 + j];
+    const u4v fu = __builtin_bit_cast(u4v, f), du = __builtin_bit_cast(u4v, d);
+    u4v r;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) r[j] = q == 0 ? fu[j] : q == 1 ? du[j] : 0u;
+    return __builtin_bit_cast(h8, r);
 }
 
-/* bias of D tile ot in every sample tile */
-template <int OT>
-__device__ __forceinline__ void bias_init(const float* b, f4 (&out)[4][OT])
+/* bias of D tile ot in every sample tile (ST tiles of 16 samples) */
+template <int ST, int OT>
+__device__ __forceinline__ void bias_init(const float* b, f4 (&out)[ST][OT])
 {
     const int q = __lane_id() >> 4;
 #pragma unroll
     for (int ot = 0; ot < OT; ++ot) {
         const f4 bias = {b[16 * ot + 4 * q], b[16 * ot + 4 * q + 1], b[16 * ot + 4 * q + 2], b[16 * ot + 4 * q + 3]};
 #pragma unroll
-        for (int st = 0; st < 4; ++st) out[st][ot] = bias;
+        for (int st = 0; st < ST; ++st) out[st][ot] = bias;
     }
 }
 
 /* out[st][ot] += W tile (ot, kb) x in[st] for every output tile: one weight
- * read from LDS per tile, four MFMAs on it */
-template <int KB, int OT>
-__device__ __forceinline__ void dense32_k(const _Float16* W, int kb, const h8 (&in)[4], f4 (&out)[4][OT])
+ * read from LDS per tile, ST MFMAs on it */
+template <int ST, int KB, int OT>
+__device__ __forceinline__ void dense32_k(const _Float16* W, int kb, const h8 (&in)[ST], f4 (&out)[ST][OT])
 {
     const int lane = __lane_id();
 #pragma unroll
     for (int ot = 0; ot < OT; ++ot) {
         const h8 a = *reinterpret_cast<const h8*>(W + ((((ot * KB + kb) << 6) + lane) << 3));
 #pragma unroll
-        for (int st = 0; st < 4; ++st) out[st][ot] = mfma32(a, in[st], out[st][ot]);
+        for (int st = 0; st < ST; ++st) out[st][ot] = mfma32(a, in[st], out[st][ot]);
     }
 }
 
 /* out = W x in + b for a hidden or output layer whose input is two B operands */
-template <int OT>
-__device__ __forceinline__ void dense32(const _Float16* W, const float* b, const h8 (&in)[4][2], f4 (&out)[4][OT])
+template <int ST, int OT>
+__device__ __forceinline__ void dense32(const _Float16* W, const float* b, const h8 (&in)[ST][2], f4 (&out)[ST][OT])
 {
-    bias_init<OT>(b, out);
+    bias_init<ST, OT>(b, out);
 #pragma unroll
     for (int kb = 0; kb < 2; ++kb) {
-        const h8 x[4] = {in[0][kb], in[1][kb], in[2][kb], in[3][kb]};
-        dense32_k<2, OT>(W, kb, x, out);
+        h8 x[ST];
+#pragma unroll
+        for (int st = 0; st < ST; ++st) x[st] = in[st][kb];
+        dense32_k<ST, 2, OT>(W, kb, x, out);
     }
 }
 
 /* ReLU of four D tiles into the two B operands of the next layer: pairs
  * converted to FP16 first, then clamped at zero as pairs (rounding keeps the
  * sign, so this is the ReLU of the FP32 value, rounded) */
-__device__ __forceinline__ void relu_pack(const f4 (&z)[4][4], h8 (&h)[4][2])
+template <int ST>
+__device__ __forceinline__ void relu_pack(const f4 (&z)[ST][4], h8 (&h)[ST][2])
 {
     const h2 zero = {(_Float16)0.f, (_Float16)0.f};
 #pragma unroll
-    for (int st = 0; st < 4; ++st)
+    for (int st = 0; st < ST; ++st)
 #pragma unroll
         for (int kb = 0; kb < 2; ++kb)
 #pragma unroll
@@ -342,23 +375,23 @@ __global__ __launch_bounds__(GB) RT_GUIDE_NET_OCCUPANCY void k_guide_net(PathSta
         wave_lds_sync();
         /* layer 1, one K block of features at a time */
         f4 z[4][4];
-        bias_init<4>(L.b, z);
+        bias_init<4, 4>(L.b, z);
 #pragma unroll
         for (int kb = 0; kb < 3; ++kb) {
             h8 x[4];
 #pragma unroll
             for (int st = 0; st < 4; ++st) x[st] = feature_group(kb, q, IN[16 * st + n]);
-            dense32_k<3, 4>(L.w + GD_W16I_W1, kb, x, z);
+            dense32_k<4, 3, 4>(L.w + GD_W16I_W1, kb, x, z);
         }
         wave_lds_sync();
         h8 h[4][2];
-        relu_pack(z, h);
-        dense32<4>(L.w + GD_W16I_W2, L.b + GD_H, h, z);
-        relu_pack(z, h);
-        dense32<4>(L.w + GD_W16I_W3, L.b + 2 * GD_H, h, z);
-        relu_pack(z, h);
+        relu_pack<4>(z, h);
+        dense32<4, 4>(L.w + GD_W16I_W2, L.b + GD_H, h, z);
+        relu_pack<4>(z, h);
+        dense32<4, 4>(L.w + GD_W16I_W3, L.b + 2 * GD_H, h, z);
+        relu_pack<4>(z, h);
         f4 y[4][3];
-        dense32<3>(L.w + GD_W16I_W4, L.b + 3 * GD_H, h, y);
+        dense32<4, 3>(L.w + GD_W16I_W4, L.b + 3 * GD_H, h, y);
 #pragma unroll
         for (int st = 0; st < 4; ++st) {
             const int i = base + 16 * st + n;
@@ -488,26 +521,28 @@ hipError_t launch_guide_zero_acc(const GuideState& g, hipStream_t st)
  * gradients summed over the records into g.grad (Adam divides by N)
  * ========================================================================= */
 #define GD_TW (GD_TRAIN_THREADS / 64) /* waves per training block */
+#define GD_TST 2                       /* sample tiles per iteration: 32 records per wave */
+#define GD_TCH (16 * GD_TST)           /* records per wave and iteration */
 #define GD_TIN 15                      /* staged floats per sample (odd stride: distinct banks) */
 #define GD_NTILE 64                    /* weight-gradient tiles: W1 20, W2 16, W3 16, W4 12 */
 #define GD_NBT 15                      /* bias-gradient tiles: b1..b3 4 each, b4 3 */
+#define GD_NSUM (GD_NTILE + 1)         /* register sums per wave: the weight tiles, then all bias tiles */
+static_assert(GD_TST == 2, "the training kernel's operand lists name two sample tiles");
 
-#define GD_NT1 20  /* gradient tiles of W1 */
-/* FP16 forward and backward tiles (after the last chunk: the block's sums
- * of the W2..W4 gradient tiles, [tile][lane][e]), FP32 biases in D-tile
- * order, and per wave the staged inputs of its 64 samples (network inputs
- * (9), sampled direction (3), BSDF pdf, normalised weight), its bias-gradient
- * sums ([tile][e][q]) and its W1 gradient sums ([tile][lane][e]).  The W2..W4
- * sums of a wave stay in its accumulation registers. */
+/* FP16 forward and backward tiles, FP32 biases in D-tile order and per wave
+ * the staged inputs of its 32 samples (network inputs (9), sampled direction
+ * (3), BSDF pdf, normalised weight); after the last iteration the same space
+ * holds the block's gradient sums [tile][lane][e].  A wave's own sums stay
+ * in its registers until then. */
 struct alignas(16) TrainLds {
     union {
-        _Float16 w[GD_NW16];
-        float acc[(GD_NTILE - GD_NT1) * 256];
+        struct {
+            _Float16 w[GD_NW16];
+            float b[4 * GD_H];
+            float in[GD_TW][GD_TCH][GD_TIN];
+        } s;
+        float acc[GD_NSUM * 256];
     };
-    float b[4 * GD_H];
-    float in[GD_TW][64][GD_TIN];
-    float bacc[GD_TW][GD_NBT * 16];
-    float acc1[GD_TW][GD_NT1 * 256];
 };
 
 /* Identity selection of one 16-feature tile of a K block, as the B operand
@@ -533,57 +568,55 @@ __device__ __forceinline__ h8 pack8(f4 a, f4 b)
 }
 
 /* One 16-feature tile of a matrix held as B operands M[st] (the K block
- * that contains it; sel picks it) with samples along K: S[ks] holds, in
- * lane c + 16 q, feature c of the tile for the samples
- * 16 (2 ks + j / 4) + 4 q + j % 4, j = 0..7.  FP16 values pass the product
- * exactly. */
+ * that contains it; sel picks it) with the wave's 32 samples along K: lane
+ * c + 16 q gets feature c of the tile for samples 4 q .. 4 q + 3 of both
+ * sample tiles.  Every operand made this way orders the samples alike, so
+ * two of them are the A and B operands of a reduction over the samples.
+ * FP16 values pass the product exactly. */
 template <int T, bool NATURAL>
-__device__ __forceinline__ void to_sil(const h8 (&M)[4], h8 (&S)[2])
+__device__ __forceinline__ h8 to_sil(const h8 (&M)[GD_TST])
 {
-    const h8 sel = sel_op(T, NATURAL);  /* rematerialised at each use: fewer live registers */
+    const h8 sel = sel_op(T, NATURAL); /* rematerialised at each use: fewer live registers */
     const f4 zero = {0.f, 0.f, 0.f, 0.f};
-    f4 d[4];
-#pragma unroll
-    for (int st = 0; st < 4; ++st) d[st] = mfma32(M[st], sel, zero);
-    S[0] = pack8(d[0], d[1]);
-    S[1] = pack8(d[2], d[3]);
+    return pack8(mfma32(M[0], sel, zero), mfma32(M[1], sel, zero));
 }
 
-/* Weight-gradient tile: the wave's 64 samples of output gradients (rows)
- * times inputs (columns), added to the wave's register sums. */
-__device__ __forceinline__ void grad_tile(f4& acc, const h8 (&dz)[2], const h8 (&a)[2])
+/* Bias-gradient tile `tile`: its output gradients summed over the samples,
+ * as the product with a B operand whose column `tile` is ones and whose
+ * other columns are zero, so that all fifteen bias tiles add into one
+ * accumulator (lane c + 16 q: rows 4 q .. 4 q + 3 of tile c). */
+__device__ __forceinline__ void bias_tile(f4& bacc, int tile, h8 zS)
 {
-    acc = mfma32(dz[0], a[0], acc);
-    acc = mfma32(dz[1], a[1], acc);
+    const _Float16 one = (__lane_id() & 15) == tile ? (_Float16)1.f : (_Float16)0.f;
+    const h8 sel = {one, one, one, one, one, one, one, one};
+    bacc = mfma32(zS, sel, bacc);
 }
 
-/* Bias-gradient tile: the output gradients summed over the samples (a
- * product with ones, every column the same sum), added by the lanes of
- * column 0 to the wave's own sums in LDS (no other wave writes them). */
-__device__ __forceinline__ void bias_tile(float* bacc, int tile, const h8 (&dz)[2], h8 ones)
+/* Saturating FP16 pair of two gradient terms, zeroed where the ReLU input
+ * pair m (FP16, >= 0) is zero: min(m 2^24, 1) is 1 for every positive FP16
+ * value (subnormals included) and 0 for zero, so the mask is two packed
+ * products and a packed minimum */
+__device__ __forceinline__ h2 relu_grad_pair(float a, float b, h2 m)
 {
-    f4 d = {0.f, 0.f, 0.f, 0.f};
-    d = mfma32(dz[0], ones, d);
-    d = mfma32(dz[1], ones, d);
-    const int lane = __lane_id();
-    if ((lane & 15) == 0)
-#pragma unroll
-        for (int e = 0; e < 4; ++e) bacc[(tile * 4 + e) * 4 + (lane >> 4)] += d[e];
+    const f2 v = {fminf(fmaxf(a, -65000.f), 65000.f), fminf(fmaxf(b, -65000.f), 65000.f)};
+    const h2 big = {(_Float16)4096.f, (_Float16)4096.f}, one = {(_Float16)1.f, (_Float16)1.f};
+    const h2 mask = __builtin_elementwise_min(m * big * big, one);
+    return __builtin_convertvector(v, h2) * mask;
 }
 
 /* Output gradients of the layer below (ReLU activations h, B operands) from
  * this layer's output gradients `in`: W^T (backward tiles (it, kb)) x in,
  * masked by h > 0 and packed as B operands (slot j of K block kb is input
  * 16 (2 kb + j / 4) + 4 q + j % 4), one K block of the result at a time. */
-__device__ __forceinline__ void back_relu(const _Float16* WT, const h8 (&in)[4][2], const h8 (&h)[4][2],
-                                          h8 (&out)[4][2])
+__device__ __forceinline__ void back_relu(const _Float16* WT, const h8 (&in)[GD_TST][2], const h8 (&h)[GD_TST][2],
+                                          h8 (&out)[GD_TST][2])
 {
     const int lane = __lane_id();
 #pragma unroll
     for (int ko = 0; ko < 2; ++ko) {
-        f4 d[4][2];
+        f4 d[GD_TST][2];
 #pragma unroll
-        for (int st = 0; st < 4; ++st) d[st][0] = d[st][1] = f4{0.f, 0.f, 0.f, 0.f};
+        for (int st = 0; st < GD_TST; ++st) d[st][0] = d[st][1] = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int kb = 0; kb < 2; ++kb)
 #pragma unroll
@@ -591,49 +624,46 @@ __device__ __forceinline__ void back_relu(const _Float16* WT, const h8 (&in)[4][
                 const int it = 2 * ko + t;
                 const h8 a = *reinterpret_cast<const h8*>(WT + ((((it * 2 + kb) << 6) + lane) << 3));
 #pragma unroll
-                for (int st = 0; st < 4; ++st) d[st][t] = mfma32(a, in[st][kb], d[st][t]);
+                for (int st = 0; st < GD_TST; ++st) d[st][t] = mfma32(a, in[st][kb], d[st][t]);
             }
 #pragma unroll
-        for (int st = 0; st < 4; ++st)
+        for (int st = 0; st < GD_TST; ++st)
 #pragma unroll
-            for (int j = 0; j < 8; ++j)
-                out[st][ko][j] = h[st][ko][j] > (_Float16)0.f ? to_h_sat(d[st][j >> 2][j & 3]) : (_Float16)0.f;
+            for (int j = 0; j < 8; j += 2) {
+                const h2 m = {h[st][ko][j], h[st][ko][j + 1]};
+                const h2 v = relu_grad_pair(d[st][j >> 2][j & 3], d[st][j >> 2][(j & 3) + 1], m);
+                out[st][ko][j] = v[0];
+                out[st][ko][j + 1] = v[1];
+            }
     }
 }
 
 /* One hidden layer of the backward pass: the gradients of its weights
- * (tiles tile0 + 4 ot + it) and biases (tiles btile0 + ot) from its output
+ * (tiles TILE0 + 4 ot + it) and biases (tiles BTILE0 + ot) from its output
  * gradients dz and inputs a, then dz <- the output gradients of the layer
  * below (through W^T and that layer's ReLU, whose activations a are). */
 template <int TILE0, int BTILE0>
-__device__ __forceinline__ void hidden_back(f4 (&acc)[GD_NTILE], float* bacc, const _Float16* WT,
-                                            const h8 (&a)[4][2], h8 (&dz)[4][2], h8 ones)
+__device__ __forceinline__ void hidden_back(f4 (&acc)[GD_NSUM], const _Float16* WT, const h8 (&a)[GD_TST][2],
+                                            h8 (&dz)[GD_TST][2])
 {
-    h8 zS[4][2];
+    h8 zS[4];
 #pragma unroll
     for (int ot = 0; ot < 4; ++ot) {
-        const h8 m[4] = {dz[0][ot >> 1], dz[1][ot >> 1], dz[2][ot >> 1], dz[3][ot >> 1]};
-        if (ot & 1)
-                to_sil<1, false>(m, zS[ot]);
-            else
-                to_sil<0, false>(m, zS[ot]);
-        bias_tile(bacc, BTILE0 + ot, zS[ot], ones);
+        const h8 m[GD_TST] = {dz[0][ot >> 1], dz[1][ot >> 1]};
+        zS[ot] = (ot & 1) ? to_sil<1, false>(m) : to_sil<0, false>(m);
+        bias_tile(acc[GD_NTILE], BTILE0 + ot, zS[ot]);
     }
 #pragma unroll
     for (int it = 0; it < 4; ++it) {
-        const h8 ma[4] = {a[0][it >> 1], a[1][it >> 1], a[2][it >> 1], a[3][it >> 1]};
-        h8 aS[2];
-        if (it & 1)
-                to_sil<1, false>(ma, aS);
-            else
-                to_sil<0, false>(ma, aS);
+        const h8 ma[GD_TST] = {a[0][it >> 1], a[1][it >> 1]};
+        const h8 aS = (it & 1) ? to_sil<1, false>(ma) : to_sil<0, false>(ma);
 #pragma unroll
-        for (int ot = 0; ot < 4; ++ot) grad_tile(acc[TILE0 + 4 * ot + it], zS[ot], aS);
+        for (int ot = 0; ot < 4; ++ot) acc[TILE0 + 4 * ot + it] = mfma32(zS[ot], aS, acc[TILE0 + 4 * ot + it]);
     }
-    h8 below[4][2];
+    h8 below[GD_TST][2];
     back_relu(WT, dz, a, below);
 #pragma unroll
-    for (int st = 0; st < 4; ++st) dz[st][0] = below[st][0], dz[st][1] = below[st][1];
+    for (int st = 0; st < GD_TST; ++st) dz[st][0] = below[st][0], dz[st][1] = below[st][1];
 }
 
 /* Master parameter of gradient tile `tile`, element e, lane (-1: none). */
@@ -655,83 +685,92 @@ __device__ __forceinline__ int bias_param(int tile, int e, int q)
     return o < 0 ? -1 : GD_OFF_B4 + o;
 }
 
-__global__ __launch_bounds__(GD_TRAIN_THREADS) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_guide_train(GuideState g, const float* rec0, const int32_t* count,
-                                                                  int off, int max_n)
+__global__ __launch_bounds__(GD_TRAIN_THREADS) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_guide_train(
+    GuideState g, const float* rec0, const int32_t* count, int off, int max_n)
 {
     __shared__ TrainLds L;
     {
         const uint4* src = reinterpret_cast<const uint4*>(g.w16i);
-        uint4* dst = reinterpret_cast<uint4*>(L.w);
+        uint4* dst = reinterpret_cast<uint4*>(L.s.w);
         for (int i = threadIdx.x; i < GD_NW16 / 8; i += blockDim.x) dst[i] = src[i];
-        for (int i = threadIdx.x; i < 4 * GD_H; i += blockDim.x) L.b[i] = bias_value(g, i);
-        for (int i = threadIdx.x; i < GD_TW * GD_NT1 * 256; i += blockDim.x) (&L.acc1[0][0])[i] = 0.f;
-        for (int i = threadIdx.x; i < GD_TW * GD_NBT * 16; i += blockDim.x) (&L.bacc[0][0])[i] = 0.f;
+        for (int i = threadIdx.x; i < 4 * GD_H; i += blockDim.x) L.s.b[i] = bias_value(g, i);
     }
     __syncthreads();
-    const _Float16* WF = L.w;
-    const _Float16* WB = L.w + GD_NW16I;
+    const _Float16* WF = L.s.w;
+    const _Float16* WB = L.s.w + GD_NW16I;
     const int lane = __lane_id(), n = lane & 15, q = lane >> 4, w = threadIdx.x >> 6;
     const int N = chunk_len(g, count, off, max_n);
     const float* rec = rec0 + (size_t)off * GD_REC;
     const float wsum = g.acc[0];
     const float wscale = wsum > 0.f ? (float)N / wsum : 0.f;
-    float (*IN)[GD_TIN] = L.in[w];
-    h8 ones;
-#pragma unroll
-    for (int k = 0; k < 8; ++k) ones[k] = (_Float16)1.f;
+    float (*IN)[GD_TIN] = L.s.in[w];
     float lacc = 0.f;
-    float* bacc = L.bacc[w];
-    f4* acc1 = reinterpret_cast<f4*>(L.acc1[w]) + lane;  /* tile t at acc1[64 t] */
-    f4 gacc[GD_NTILE];  /* W2..W4 tiles (20..63) used */
+    f4 acc[GD_NSUM];
 #pragma unroll
-    for (int t = GD_NT1; t < GD_NTILE; ++t) gacc[t] = f4{0.f, 0.f, 0.f, 0.f};
-    const int stride = gridDim.x * GD_TW * 64;
-    for (int base = (blockIdx.x * GD_TW + w) * 64; base < N; base += stride) {
-        /* stage: lane l holds record base + l */
+    for (int t = 0; t < GD_NSUM; ++t) acc[t] = f4{0.f, 0.f, 0.f, 0.f};
+    /* record loads: lane l reads floats 8 (l / 32) .. + 7 of record l % 32,
+     * the next iteration's while this one is computed */
+    const int half = lane >> 5, rl = lane & 31;
+    const int stride = gridDim.x * GD_TW * GD_TCH;
+    int base = (blockIdx.x * GD_TW + w) * GD_TCH;
+    float4 pa = make_float4(0.f, 0.f, 0.f, 0.f), pb4 = pa;
+    if (base + rl < N) {
+        const float4* r = reinterpret_cast<const float4*>(rec + (size_t)(base + rl) * GD_REC) + 2 * half;
+        pa = r[0];
+        pb4 = r[1];
+    }
+    for (; base < N; base += stride) {
+        /* stage: record base + rl, floats 8 half .. (float 15 is not staged) */
         {
-            const int i = base + lane;
-            float v[GD_TIN];
+            /* the position mapped into the scene box (floats 0..2), the
+             * weight normalised by the step's weight sum and capped (float
+             * 13), the rest as stored; the kernel arguments indexed by
+             * compile-time constants only */
+            const float v[8] = {pa.x, pa.y, pa.z, pa.w, pb4.x, pb4.y, pb4.z, pb4.w};
 #pragma unroll
-            for (int c = 0; c < GD_TIN; ++c) v[c] = 0.f;
-            if (i < N) {
-                const float* r = rec + (size_t)i * GD_REC;
-#pragma unroll
-                for (int k = 0; k < 3; ++k) v[k] = (r[k] - g.lo[k]) * g.inv_ext[k];
-#pragma unroll
-                for (int k = 3; k < 13; ++k) v[k] = r[k];
-                v[13] = fminf(r[13] * wscale, GD_WCLAMP);
+            for (int k = 0; k < 8; ++k) {
+                float val = v[k];
+                if (k < 3) val = half == 0 ? (val - g.lo[k]) * g.inv_ext[k] : val;
+                if (k == 13 - 8) val = half == 1 ? fminf(val * wscale, GD_WCLAMP) : val;
+                if (k < GD_TIN - 8 || half == 0) IN[rl][8 * half + k] = val;
             }
-#pragma unroll
-            for (int c = 0; c < GD_TIN; ++c) IN[lane][c] = v[c];
+            const int j = base + stride + rl;
+            pa = make_float4(0.f, 0.f, 0.f, 0.f);
+            pb4 = pa;
+            if (j < N) {
+                const float4* r = reinterpret_cast<const float4*>(rec + (size_t)j * GD_REC) + 2 * half;
+                pa = r[0];
+                pb4 = r[1];
+            }
         }
         wave_lds_sync();
 
         /* forward (as inference), keeping the activations as B operands */
-        h8 h1[4][2], h2[4][2], h3[4][2];
-        f4 y[4][3];
+        h8 h1[GD_TST][2], h2[GD_TST][2], h3[GD_TST][2];
+        f4 y[GD_TST][3];
         {
-            f4 z[4][4];
-            bias_init<4>(L.b, z);
+            f4 z[GD_TST][4];
+            bias_init<GD_TST, 4>(L.s.b, z);
 #pragma unroll
             for (int kb = 0; kb < 3; ++kb) {
-                h8 x[4];
+                h8 x[GD_TST];
 #pragma unroll
-                for (int st = 0; st < 4; ++st) x[st] = feature_group(kb, q, IN[16 * st + n]);
-                dense32_k<3, 4>(WF + GD_W16I_W1, kb, x, z);
+                for (int st = 0; st < GD_TST; ++st) x[st] = feature_group(kb, q, IN[16 * st + n]);
+                dense32_k<GD_TST, 3, 4>(WF + GD_W16I_W1, kb, x, z);
             }
-            relu_pack(z, h1);
-            dense32<4>(WF + GD_W16I_W2, L.b + GD_H, h1, z);
-            relu_pack(z, h2);
-            dense32<4>(WF + GD_W16I_W3, L.b + 2 * GD_H, h2, z);
-            relu_pack(z, h3);
-            dense32<3>(WF + GD_W16I_W4, L.b + 3 * GD_H, h3, y);
+            relu_pack<GD_TST>(z, h1);
+            dense32<GD_TST, 4>(WF + GD_W16I_W2, L.s.b + GD_H, h1, z);
+            relu_pack<GD_TST>(z, h2);
+            dense32<GD_TST, 4>(WF + GD_W16I_W3, L.s.b + 2 * GD_H, h2, z);
+            relu_pack<GD_TST>(z, h3);
+            dense32<GD_TST, 3>(WF + GD_W16I_W4, L.s.b + 3 * GD_H, h3, y);
         }
 
         /* head: loss and its gradient in the lanes that hold the outputs
          * (lane group q: lobes 2q, 2q+1; group 0 also the guide logit) */
-        h8 dy[4][2];
+        h8 dy[GD_TST][2];
 #pragma unroll
-        for (int st = 0; st < 4; ++st) {
+        for (int st = 0; st < GD_TST; ++st) {
             const int i = base + 16 * st + n;
             const bool valid = i < N;
             const float* v = IN[16 * st + n];
@@ -763,7 +802,7 @@ __global__ __launch_bounds__(GD_TRAIN_THREADS) __attribute__((amdgpu_waves_per_e
             const float inv = __builtin_amdgcn_rcpf(quad_sum(e0 + e1));
             const float pi[2] = {e0 * inv, e1 * inv};
             const float qd = quad_sum(pi[0] * vv[0] + pi[1] * vv[1]);
-            const float sg = __builtin_amdgcn_rcpf(1.f + __expf(-__shfl(r[10], n)));
+            const float sg = __builtin_amdgcn_rcpf(1.f + __expf(-row0_bcast(r[10])));
             const float alpha = GD_ALPHA_LO + GD_ALPHA_SPAN * sg;
             const float pm = alpha * qd + (1.f - alpha) * pb;
             const float c = valid ? -sw * __builtin_amdgcn_rcpf(pm) : 0.f;
@@ -793,104 +832,75 @@ __global__ __launch_bounds__(GD_TRAIN_THREADS) __attribute__((amdgpu_waves_per_e
             }
         }
 
-        /* W4: gradients (rows: the head's slots as the forward W4 tile rows) */
+        /* W4 and b4: gradients (rows: the head's slots as the forward W4 tile rows) */
         {
-            h8 dyS[3][2];
+            h8 dyS[3];
             {
-                const h8 m0[4] = {dy[0][0], dy[1][0], dy[2][0], dy[3][0]};
-                const h8 m1[4] = {dy[0][1], dy[1][1], dy[2][1], dy[3][1]};
-                to_sil<0, false>(m0, dyS[0]);
-                to_sil<1, false>(m0, dyS[1]);
-                to_sil<0, false>(m1, dyS[2]);
+                const h8 m0[GD_TST] = {dy[0][0], dy[1][0]};
+                const h8 m1[GD_TST] = {dy[0][1], dy[1][1]};
+                dyS[0] = to_sil<0, false>(m0);
+                dyS[1] = to_sil<1, false>(m0);
+                dyS[2] = to_sil<0, false>(m1);
             }
+#pragma unroll
+            for (int ot = 0; ot < 3; ++ot) bias_tile(acc[GD_NTILE], 12 + ot, dyS[ot]);
 #pragma unroll
             for (int it = 0; it < 4; ++it) {
-                const h8 m[4] = {h3[0][it >> 1], h3[1][it >> 1], h3[2][it >> 1], h3[3][it >> 1]};
-                h8 aS[2];
-                if (it & 1)
-                to_sil<1, false>(m, aS);
-            else
-                to_sil<0, false>(m, aS);
+                const h8 m[GD_TST] = {h3[0][it >> 1], h3[1][it >> 1]};
+                const h8 aS = (it & 1) ? to_sil<1, false>(m) : to_sil<0, false>(m);
 #pragma unroll
-                for (int ot = 0; ot < 3; ++ot) grad_tile(gacc[52 + 4 * ot + it], dyS[ot], aS);
+                for (int ot = 0; ot < 3; ++ot) acc[52 + 4 * ot + it] = mfma32(dyS[ot], aS, acc[52 + 4 * ot + it]);
             }
-#pragma unroll
-            for (int ot = 0; ot < 3; ++ot) bias_tile(bacc, 12 + ot, dyS[ot], ones);
         }
         /* back through W4, W3, W2; each layer's gradients as soon as its
          * output gradients exist */
-        h8 dz[4][2];
+        h8 dz[GD_TST][2];
         back_relu(WB + GD_W16B_W4, dy, h3, dz);
-        hidden_back<36, 8>(gacc, bacc, WB + GD_W16B_W3, h2, dz, ones);
-        hidden_back<20, 4>(gacc, bacc, WB + GD_W16B_W2, h1, dz, ones);
-        /* W1: output gradients dz (layer 1), inputs recomputed */
+        hidden_back<36, 8>(acc, WB + GD_W16B_W3, h2, dz);
+        hidden_back<20, 4>(acc, WB + GD_W16B_W2, h1, dz);
+        /* W1 and b1: output gradients dz (layer 1), inputs recomputed */
         {
-            h8 zS[4][2];
+            h8 zS[4];
 #pragma unroll
             for (int ot = 0; ot < 4; ++ot) {
-                const h8 m[4] = {dz[0][ot >> 1], dz[1][ot >> 1], dz[2][ot >> 1], dz[3][ot >> 1]};
-                if (ot & 1)
-                to_sil<1, false>(m, zS[ot]);
-            else
-                to_sil<0, false>(m, zS[ot]);
-                bias_tile(bacc, ot, zS[ot], ones);
+                const h8 m[GD_TST] = {dz[0][ot >> 1], dz[1][ot >> 1]};
+                zS[ot] = (ot & 1) ? to_sil<1, false>(m) : to_sil<0, false>(m);
+                bias_tile(acc[GD_NTILE], ot, zS[ot]);
             }
 #pragma unroll
             for (int kb = 0; kb < 3; ++kb) {
-                h8 x[4];
+                h8 x[GD_TST];
 #pragma unroll
-                for (int st = 0; st < 4; ++st) x[st] = feature_group(kb, q, IN[16 * st + n]);
+                for (int st = 0; st < GD_TST; ++st) x[st] = feature_group(kb, q, IN[16 * st + n]);
 #pragma unroll
                 for (int t = 0; t < 2; ++t) {
                     const int it = 2 * kb + t;
-                    if (it >= 5) continue;  /* inputs 80..95 are padding */
-                    h8 xS[2];
-                    if (t)
-                to_sil<1, true>(x, xS);
-            else
-                to_sil<0, true>(x, xS);
+                    if (it >= 5) continue; /* inputs 80..95 are padding */
+                    const h8 xS = t ? to_sil<1, true>(x) : to_sil<0, true>(x);
 #pragma unroll
-                    for (int ot = 0; ot < 4; ++ot) {
-                        f4 v = acc1[64 * (5 * ot + it)];
-                        grad_tile(v, zS[ot], xS);
-                        acc1[64 * (5 * ot + it)] = v;
-                    }
+                    for (int ot = 0; ot < 4; ++ot) acc[5 * ot + it] = mfma32(zS[ot], xS, acc[5 * ot + it]);
                 }
             }
         }
         wave_lds_sync();
     }
     /* the block's sums: the waves add their register sums in turn (into the
-     * weights' space, free once every wave is past its last chunk) */
+     * weights' space, free once every wave is past its last iteration) */
     __syncthreads();
     for (int turn = 0; turn < GD_TW; ++turn) {
         if (w == turn)
 #pragma unroll
-            for (int t = GD_NT1; t < GD_NTILE; ++t) {
-                f4* dst = reinterpret_cast<f4*>(&L.acc[((t - GD_NT1) * 64 + lane) * 4]);
-                *dst = turn == 0 ? gacc[t] : *dst + gacc[t];
+            for (int t = 0; t < GD_NSUM; ++t) {
+                f4* dst = reinterpret_cast<f4*>(&L.acc[(t * 64 + lane) * 4]);
+                *dst = turn == 0 ? acc[t] : *dst + acc[t];
             }
         __syncthreads();
     }
     float* part = g.part + (size_t)blockIdx.x * GD_NPARAM;
-    for (int k = threadIdx.x; k < GD_NTILE * 256; k += blockDim.x) {
-        const int t = k >> 8, pi = grad_param(t, k & 3, (k >> 2) & 63);
-        float v;
-        if (t < GD_NT1) {
-            v = 0.f;
-#pragma unroll
-            for (int ww = 0; ww < GD_TW; ++ww) v += L.acc1[ww][k];
-        } else {
-            v = L.acc[k - GD_NT1 * 256];
-        }
-        if (pi >= 0) part[pi] = v;
-    }
-    for (int k = threadIdx.x; k < GD_NBT * 16; k += blockDim.x) {
-        const int pi = bias_param(k >> 4, (k >> 2) & 3, k & 3);
-        float v = 0.f;
-#pragma unroll
-        for (int ww = 0; ww < GD_TW; ++ww) v += L.bacc[ww][k];
-        if (pi >= 0) part[pi] = v;
+    for (int k = threadIdx.x; k < GD_NSUM * 256; k += blockDim.x) {
+        const int t = k >> 8, e = k & 3, l = (k >> 2) & 63;
+        const int pi = t < GD_NTILE ? grad_param(t, e, l) : ((l & 15) < GD_NBT ? bias_param(l & 15, e, l >> 4) : -1);
+        if (pi >= 0) part[pi] = L.acc[k];
     }
     for (int o = 32; o > 0; o >>= 1) lacc += __shfl_xor(lacc, o);
     if (lane == 0 && lacc != 0.f) atomicAdd(&g.acc[1], lacc);
