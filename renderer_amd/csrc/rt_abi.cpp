@@ -1235,7 +1235,16 @@ int rt_guide_head(rt_ctx* c, const float* rec, int32_t n, float* out)
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     std::vector<_Float16> h((size_t)n * GD_PARAMS);
     HIP_TRY(c, hipMemcpy(h.data(), c->g_tmp_y.p, h.size() * 2, hipMemcpyDeviceToHost));
-    for (size_t i = 0; i < h.size(); ++i) out[i] = (float)h[i];
+    /* from the kernel's lane-group layout to lobe j at [5j .. 5j+4], the
+     * guide probability at [40], zeros after it */
+    for (int32_t i = 0; i < n; ++i) {
+        const _Float16* src = h.data() + (size_t)i * GD_PARAMS;
+        float* dst = out + (size_t)i * GD_PARAMS;
+        for (int k = 0; k < GD_PARAMS; ++k) dst[k] = 0.f;
+        for (int j = 0; j < GD_LOBES; ++j)
+            for (int e = 0; e < 5; ++e) dst[5 * j + e] = (float)src[gd_lobe_off(j) + e];
+        dst[40] = (float)src[GD_ALPHA_OFF];
+    }
     return RT_OK;
 }
 

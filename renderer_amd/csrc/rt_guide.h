@@ -76,7 +76,9 @@
  * (LDS: 45 KB per inference block, 155 KB per training block).  More blocks
  * would only reload the weights into LDS and, for training, add one round
  * of gradient atomics per block. */
+#ifndef GD_INFER_BLOCKS_PER_CU
 #define GD_INFER_BLOCKS_PER_CU 3
+#endif
 #define GD_TRAIN_BLOCKS_PER_CU 1
 #define GD_TRAIN_THREADS 256   /* 4 waves: one per SIMD, its gradient sums in the accumulation registers */
 
@@ -111,6 +113,15 @@ struct GuideState {
 };
 
 /* Output of W4 that inference lane group g keeps in slot s (0..11), or -1. */
+/* Head outputs of a path vertex as the network kernel stores them (48
+ * halves, 96 B): the 12 halves of lane group q hold lobes 2q and 2q + 1 (5
+ * halves each: mean direction, concentration, mixture weight), group 0 also
+ * the probability of sampling the guide at half GD_ALPHA_OFF; the others
+ * are zero.  Each lane of the network kernel writes its group as one
+ * contiguous 24-byte piece, so every 96-byte row is written whole. */
+#define GD_ALPHA_OFF 10
+__host__ __device__ inline int gd_lobe_off(int j) { return 12 * (j >> 1) + 5 * (j & 1); }
+
 __host__ __device__ inline int gd_head_output(int g, int s)
 {
     return s < 10 ? 10 * g + s : (s == 10 && g == 0 ? 40 : -1);

@@ -112,6 +112,14 @@ __device__ __forceinline__ float row0_bcast(float v)
  * Inference: the network on 64 samples per wave
  * ========================================================================= */
 #define GD_INS 9  /* inputs staged per sample: position in [0,1]^3, normal, outgoing direction */
+/* sample tiles of 16 a wave carries per iteration, and threads per block */
+#ifndef GD_NET_ST
+#define GD_NET_ST 4
+#endif
+#ifndef GD_NET_THREADS
+#define GD_NET_THREADS 256
+#endif
+#define GD_NET_CH (16 * GD_NET_ST) /* samples per wave and iteration */
 /* waves per SIMD the network kernel is compiled for (0: the compiler's choice) */
 #ifndef RT_GUIDE_NET_WAVES
 #define RT_GUIDE_NET_WAVES 3
@@ -128,7 +136,7 @@ __device__ __forceinline__ float row0_bcast(float v)
 struct alignas(16) InferLds {
     _Float16 w[GD_NW16I];
     float b[4 * GD_H];
-    float in[GB / 64][64][GD_INS];
+    float in[GD_NET_THREADS / 64][GD_NET_CH][GD_INS];
 };
 
 /* Bias i of the D-tile order: b1, b2, b3, then b4 in the row order of the
@@ -318,14 +326,20 @@ __device__ __forceinline__ void unpack_inputs(float4 a, float2 b, float* v)
     v[6] = (float)n2w0[1], v[7] = (float)w12[0], v[8] = (float)w12[1];
 }
 
+/* one lane group's 24 bytes of head output (4-byte aligned: a 16-byte and
+ * an 8-byte store) */
+struct __attribute__((packed, aligned(4))) HeadGroup {
+    uint32_t v[6];
+};
+
 /* Head outputs of the path vertex (or record) at queue position i: 48
- * halves, lobe j at [5j .. 5j+4] = (mu.xyz, kappa, weight), [40] the
- * probability of sampling the guide.  RAW: the 48 raw network outputs as
- * floats instead (records only, the parity surface).  Path vertices take
- * their inputs from k_guide_gather's compact records, the next chunk's read
- * while this one is computed. */
+ * halves, lobe j at gd_lobe_off(j) .. +4 = (mu.xyz, kappa, weight),
+ * [GD_ALPHA_OFF] the probability of sampling the guide (rt_guide.h).  RAW:
+ * the 48 raw network outputs as floats instead (records only, the parity
+ * surface).  Path vertices take their inputs from k_guide_gather's compact
+ * records, the next chunk's read while this one is computed. */
 template <bool FROM_REC, bool RAW>
-__global__ __launch_bounds__(GB) RT_GUIDE_NET_OCCUPANCY void k_guide_net(PathState ps, GuideState g, int qsel,
+__global__ __launch_bounds__(GD_NET_THREADS) RT_GUIDE_NET_OCCUPANCY void k_guide_net(PathState ps, GuideState g, int qsel,
                                                                         const float* rec, int nrec, _Float16* hout,
                                                                         float* yout)
 {
@@ -335,18 +349,18 @@ __global__ __launch_bounds__(GB) RT_GUIDE_NET_OCCUPANCY void k_guide_net(PathSta
     const int cnt = FROM_REC ? nrec : ps.cnt->q_count[qsel];
     if (!FROM_REC && blockIdx.x == 0 && threadIdx.x == 0 && cnt > 0) atomicAdd(g.inferred, (unsigned long long)cnt);
     float (*IN)[GD_INS] = L.in[w];
-    const int stride = gridDim.x * (GB / 64) * 64;
-    int base = (blockIdx.x * (GB / 64) + w) * 64;
+    const int stride = gridDim.x * (GD_NET_THREADS / 64) * GD_NET_CH;
+    int base = (blockIdx.x * (GD_NET_THREADS / 64) + w) * GD_NET_CH;
     float4 na = make_float4(0.f, 0.f, 0.f, 0.f);
     float2 nb = make_float2(0.f, 0.f);
-    if (!FROM_REC && base + lane < cnt) {
+    if (!FROM_REC && lane < GD_NET_CH && base + lane < cnt) {
         na = g.in_a[base + lane];
         nb = g.in_b[base + lane];
     }
     for (; base < cnt; base += stride) {
         /* stage: lane l holds sample base + l */
         {
-            const int i = base + lane;
+            const int i = lane < GD_NET_CH ? base + lane : cnt;
             float v[GD_INS];
 #pragma unroll
             for (int c = 0; c < GD_INS; ++c) v[c] = 0.f;
@@ -364,36 +378,37 @@ __global__ __launch_bounds__(GB) RT_GUIDE_NET_OCCUPANCY void k_guide_net(PathSta
                 const int j = i + stride;
                 na = make_float4(0.f, 0.f, 0.f, 0.f);
                 nb = make_float2(0.f, 0.f);
-                if (j < cnt) {
+                if (lane < GD_NET_CH && j < cnt) {
                     na = g.in_a[j];
                     nb = g.in_b[j];
                 }
             }
+            if (lane < GD_NET_CH)
 #pragma unroll
-            for (int c = 0; c < GD_INS; ++c) IN[lane][c] = v[c];
+                for (int c = 0; c < GD_INS; ++c) IN[lane][c] = v[c];
         }
         wave_lds_sync();
         /* layer 1, one K block of features at a time */
-        f4 z[4][4];
-        bias_init<4, 4>(L.b, z);
+        f4 z[GD_NET_ST][4];
+        bias_init<GD_NET_ST, 4>(L.b, z);
 #pragma unroll
         for (int kb = 0; kb < 3; ++kb) {
-            h8 x[4];
+            h8 x[GD_NET_ST];
 #pragma unroll
-            for (int st = 0; st < 4; ++st) x[st] = feature_group(kb, q, IN[16 * st + n]);
-            dense32_k<4, 3, 4>(L.w + GD_W16I_W1, kb, x, z);
+            for (int st = 0; st < GD_NET_ST; ++st) x[st] = feature_group(kb, q, IN[16 * st + n]);
+            dense32_k<GD_NET_ST, 3, 4>(L.w + GD_W16I_W1, kb, x, z);
         }
         wave_lds_sync();
-        h8 h[4][2];
-        relu_pack<4>(z, h);
-        dense32<4, 4>(L.w + GD_W16I_W2, L.b + GD_H, h, z);
-        relu_pack<4>(z, h);
-        dense32<4, 4>(L.w + GD_W16I_W3, L.b + 2 * GD_H, h, z);
-        relu_pack<4>(z, h);
-        f4 y[4][3];
-        dense32<4, 3>(L.w + GD_W16I_W4, L.b + 3 * GD_H, h, y);
+        h8 h[GD_NET_ST][2];
+        relu_pack<GD_NET_ST>(z, h);
+        dense32<GD_NET_ST, 4>(L.w + GD_W16I_W2, L.b + GD_H, h, z);
+        relu_pack<GD_NET_ST>(z, h);
+        dense32<GD_NET_ST, 4>(L.w + GD_W16I_W3, L.b + 2 * GD_H, h, z);
+        relu_pack<GD_NET_ST>(z, h);
+        f4 y[GD_NET_ST][3];
+        dense32<GD_NET_ST, 3>(L.w + GD_W16I_W4, L.b + 3 * GD_H, h, y);
 #pragma unroll
-        for (int st = 0; st < 4; ++st) {
+        for (int st = 0; st < GD_NET_ST; ++st) {
             const int i = base + 16 * st + n;
             const bool valid = i < cnt;
             float r[12];
@@ -416,14 +431,17 @@ __global__ __launch_bounds__(GB) RT_GUIDE_NET_OCCUPANCY void k_guide_net(PathSta
             const float mx = quad_max(fmaxf(r[4], r[9]));
             const float e0 = __expf(r[4] - mx), e1 = __expf(r[9] - mx);
             const float inv = __builtin_amdgcn_rcpf(quad_sum(e0 + e1));
+            /* the probability of sampling the guide (lane group 0), zero
+             * in the other groups' spare half */
+            const float ga = q == 0 ? GD_ALPHA_LO + GD_ALPHA_SPAN * __builtin_amdgcn_rcpf(1.f + __expf(-r[10])) : 0.f;
             if (valid) {
-                h2* o = reinterpret_cast<h2*>(hout + (size_t)i * GD_PARAMS + 10 * q);
-                o[0] = h2{(_Float16)mu0.x, (_Float16)mu0.y};
-                o[1] = h2{(_Float16)mu0.z, (_Float16)k0};
-                o[2] = h2{(_Float16)(e0 * inv), (_Float16)mu1.x};
-                o[3] = h2{(_Float16)mu1.y, (_Float16)mu1.z};
-                o[4] = h2{(_Float16)k1, (_Float16)(e1 * inv)};
-                if (q == 0) hout[(size_t)i * GD_PARAMS + 40] = (_Float16)(GD_ALPHA_LO + GD_ALPHA_SPAN * __builtin_amdgcn_rcpf(1.f + __expf(-r[10])));
+                const h2 v[6] = {h2{(_Float16)mu0.x, (_Float16)mu0.y}, h2{(_Float16)mu0.z, (_Float16)k0},
+                                 h2{(_Float16)(e0 * inv), (_Float16)mu1.x}, h2{(_Float16)mu1.y, (_Float16)mu1.z},
+                                 h2{(_Float16)k1, (_Float16)(e1 * inv)}, h2{(_Float16)ga, (_Float16)0.f}};
+                HeadGroup hg;
+#pragma unroll
+                for (int k = 0; k < 6; ++k) hg.v[k] = __builtin_bit_cast(uint32_t, v[k]);
+                *reinterpret_cast<HeadGroup*>(hout + (size_t)i * GD_PARAMS + 12 * q) = hg;
             }
         }
     }
@@ -433,7 +451,7 @@ hipError_t launch_guide_infer(const HostDevScene& s, const PathState& ps, const 
                               hipStream_t st)
 {
     hipLaunchKernelGGL(k_guide_gather, dim3(grid * 4), dim3(GB), 0, st, as_dev_g(s), ps, g, qsel);
-    hipLaunchKernelGGL((k_guide_net<false, false>), dim3(grid), dim3(GB), 0, st, ps, g, qsel, (const float*)nullptr, 0,
+    hipLaunchKernelGGL((k_guide_net<false, false>), dim3(grid), dim3(GD_NET_THREADS), 0, st, ps, g, qsel, (const float*)nullptr, 0,
                        g.out, (float*)nullptr);
     return hipGetLastError();
 }
@@ -441,7 +459,7 @@ hipError_t launch_guide_infer(const HostDevScene& s, const PathState& ps, const 
 /* the network on records (parity surface): raw outputs y [n x 48] */
 hipError_t launch_guide_forward(const GuideState& g, const float* rec, int n, float* y, int grid, hipStream_t st)
 {
-    hipLaunchKernelGGL((k_guide_net<true, true>), dim3(grid), dim3(GB), 0, st, PathState{}, g, 0, rec, n,
+    hipLaunchKernelGGL((k_guide_net<true, true>), dim3(grid), dim3(GD_NET_THREADS), 0, st, PathState{}, g, 0, rec, n,
                        (_Float16*)nullptr, y);
     return hipGetLastError();
 }
@@ -449,7 +467,7 @@ hipError_t launch_guide_forward(const GuideState& g, const float* rec, int n, fl
 /* head outputs of records as the shading kernel receives them [n x 48 halves] */
 hipError_t launch_guide_head(const GuideState& g, const float* rec, int n, _Float16* out, int grid, hipStream_t st)
 {
-    hipLaunchKernelGGL((k_guide_net<true, false>), dim3(grid), dim3(GB), 0, st, PathState{}, g, 0, rec, n, out,
+    hipLaunchKernelGGL((k_guide_net<true, false>), dim3(grid), dim3(GD_NET_THREADS), 0, st, PathState{}, g, 0, rec, n, out,
                        (float*)nullptr);
     return hipGetLastError();
 }

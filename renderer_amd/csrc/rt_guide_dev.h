@@ -38,17 +38,19 @@ __device__ __forceinline__ vec3 vmf_sample(vec3 mu, float kappa, float u0, float
     return norm(t * (r * cp) + v * (r * sp) + mu * w);
 }
 
-/* Head output of one path vertex: 48 halves, lobe j at [5j .. 5j+4] =
- * (mu.xyz, kappa, weight), [40] the probability of sampling the guide.  The
+/* Head output of one path vertex: 48 halves, lobe j at gd_lobe_off(j) ..
+ * +4 = (mu.xyz, kappa, weight), [GD_ALPHA_OFF] the probability of sampling
+ * the guide.  The
  * lobes are read as the distribution the rounded halves define: mu
  * renormalised and the weights divided by their sum, the same values in
  * guide_sample and guide_pdf, so that the density is exactly that of the
  * samples. */
 __device__ __forceinline__ void guide_lobe(const _Float16* o, int j, vec3& mu, float& kappa, float& w)
 {
-    mu = norm(mk((float)o[5 * j], (float)o[5 * j + 1], (float)o[5 * j + 2]));
-    kappa = (float)o[5 * j + 3];
-    w = (float)o[5 * j + 4];
+    const _Float16* l = o + gd_lobe_off(j);
+    mu = norm(mk((float)l[0], (float)l[1], (float)l[2]));
+    kappa = (float)l[3];
+    w = (float)l[4];
 }
 
 __device__ __forceinline__ float guide_pdf(const _Float16* o, vec3 w)
@@ -69,13 +71,13 @@ __device__ __forceinline__ vec3 guide_sample(const _Float16* o, float ul, float 
 {
     float ws = 0.f;
 #pragma unroll
-    for (int j = 0; j < GD_LOBES; ++j) ws += (float)o[5 * j + 4];
+    for (int j = 0; j < GD_LOBES; ++j) ws += (float)o[gd_lobe_off(j) + 4];
     const float t = ul * ws;
     int j = 0;
     float acc = 0.f;
 #pragma unroll
     for (int k = 0; k < GD_LOBES - 1; ++k) {
-        acc += (float)o[5 * k + 4];
+        acc += (float)o[gd_lobe_off(k) + 4];
         if (t >= acc) j = k + 1;
     }
     vec3 mu;

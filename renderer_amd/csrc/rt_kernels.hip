@@ -1261,7 +1261,7 @@ __device__ void shade_guided(const DevScene& S, const PathState& ps, const Guide
     /* the network is not evaluated for the last bounce; its outputs are
      * stored by queue position */
     const _Float16* go = G.out + (size_t)qi * GD_PARAMS;
-    float ag = last ? 0.f : (float)go[40];
+    float ag = last ? 0.f : (float)go[GD_ALPHA_OFF];
     if (S.n_lights > 0) {
         U4 gl = rng4(P.seed, (uint32_t)info.z, (uint32_t)info.x, (uint32_t)info.y, 2u + 2u * depth);
         vec3 wi, le;
