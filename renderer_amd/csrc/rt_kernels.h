@@ -56,7 +56,9 @@ struct DevView {
 
 /* A tile: samples [s0, s0+ns) of pixels [pix0, pix0+npix) of one view,
  * occupying path slots [slot0, slot0 + npix*ns) of its pass.  Slot
- * slot0 + j maps to pixel pix0 + j % npix, sample s0 + j / npix. */
+ * slot0 + j is sample s0 + j % ns of the (j / ns)-th pixel of the tile's
+ * walk (tile_pixel): a pixel's samples are consecutive, so the camera rays
+ * of a wave leave from one or a few pixels. */
 struct DevTile {
     int32_t view, pix0, npix, s0, ns, slot0;
     int32_t acc0;  /* first pixel of this tile among the pass's pixels */

@@ -607,11 +607,11 @@ __device__ __forceinline__ bool add_emission(const DevScene& S, const PathState&
     return true;
 }
 
-/* Pixel of the q-th slot of a tile's sample block.  A tile of whole image
- * rows is walked in 8x8 pixel blocks (bands of 8 rows, blocks left to right;
- * a narrower last block and a shorter last band keep row order), so the 64
- * camera rays of a wave start from one compact block rather than a 64-pixel
- * row and share more BVH nodes.  Any other tile is walked in row order.
+/* The q-th pixel of a tile's walk.  A tile of whole image rows is walked in
+ * 8x8 pixel blocks (bands of 8 rows, blocks left to right; a narrower last
+ * block and a shorter last band keep row order), so the camera rays of a
+ * wave start from one compact block rather than a row and share more BVH
+ * nodes.  Any other tile is walked in row order.
  * This is a bijection on [0, npix); the random stream is keyed by the pixel,
  * so the film does not depend on the order. */
 __device__ __forceinline__ int tile_pixel(const DevTile& T, const DevView& V, int q)
@@ -686,8 +686,8 @@ __global__ __launch_bounds__(PT_BLOCK) void k_generate(PathState ps, const DevVi
         const DevTile T = tiles[lo];
         const DevView& V = views[T.view];
         int j = slot - T.slot0;
-        int pixel = tile_pixel(T, V, j % T.npix);
-        int s = T.s0 + j / T.npix;
+        int pixel = tile_pixel(T, V, j / T.ns);
+        int s = T.s0 + j % T.ns;
         U4 g0 = rng4(P.seed, (uint32_t)T.view, (uint32_t)pixel, (uint32_t)s, 0u);
         int a = s / V.spp_y, b = s % V.spp_y;
         /* src/jitteredsampler.cpp:23-35 and src/perpixelrenderer.cpp:79-84 in
@@ -1508,34 +1508,80 @@ hipError_t launch_advance(const PathState& ps, int qsel, int depth, hipStream_t 
  * increasing sample order (src/perpixelrenderer.cpp:92) - deterministic and
  * independent of how samples were split into passes.
  * ========================================================================= */
+/* film += the pass's samples of each pixel, in sample order (the
+ * reference's film[i][j] += s/spp, src/perpixelrenderer.cpp:92).  A
+ * pixel's samples occupy consecutive slots, so a thread reading its own
+ * would touch a different cache line than its neighbours at every load; the
+ * block instead reads its pixels' samples four at a time, four lanes to
+ * each 64-byte group of slots, into LDS, and each thread then adds its
+ * pixel's terms in order. */
+#define ACC_CHUNK 4
 __global__ __launch_bounds__(PT_BLOCK) void k_accumulate(PathState ps, const DevView* __restrict__ views,
                                                         const DevTile* __restrict__ tiles,
                                                         const DevPass* __restrict__ passes)
 {
+    __shared__ int s_base[PT_BLOCK], s_ns[PT_BLOCK];
+    __shared__ float s_inv[PT_BLOCK];
+    __shared__ float s_term[PT_BLOCK][ACC_CHUNK][3];
+    __shared__ int s_max;
     const DevPass pass = passes[ps.cnt->pass];
-    for (int j = blockIdx.x * PT_BLOCK + threadIdx.x; j < pass.pixels; j += gridDim.x * PT_BLOCK) {
-        int lo = pass.tile0, hi = pass.tile1;
-        while (hi - lo > 1) {
-            int mid = (lo + hi) >> 1;
-            if (tiles[mid].acc0 <= j) lo = mid;
-            else hi = mid;
+    const int t = threadIdx.x;
+    for (int j0 = blockIdx.x * PT_BLOCK; j0 < pass.pixels; j0 += gridDim.x * PT_BLOCK) {
+        const int j = j0 + t;
+        float* f = nullptr;
+        float r = 0.f, g = 0.f, b = 0.f;
+        int ns = 0;
+        __syncthreads(); /* the previous round's readers of the block's tables are done */
+        if (t == 0) s_max = 0;
+        if (j < pass.pixels) {
+            int lo = pass.tile0, hi = pass.tile1;
+            while (hi - lo > 1) {
+                int mid = (lo + hi) >> 1;
+                if (tiles[mid].acc0 <= j) lo = mid;
+                else hi = mid;
+            }
+            const DevTile T = tiles[lo];
+            const DevView& V = views[T.view];
+            const int local = j - T.acc0; /* pixel of the tile's walk */
+            f = V.film + 3 * (size_t)tile_pixel(T, V, local);
+            r = f[0], g = f[1], b = f[2];
+            ns = T.ns;
+            s_base[t] = T.slot0 + local * T.ns;
+            s_inv[t] = V.inv_spp;
         }
-        const DevTile T = tiles[lo];
-        const DevView& V = views[T.view];
-        int local = j - T.acc0;  /* slot offset within the tile's sample block */
-        int pixel = tile_pixel(T, V, local);
-        float* f = V.film + 3 * (size_t)pixel;
-        float r = f[0], g = f[1], b = f[2];
-        for (int s = 0; s < T.ns; ++s) {
-            int slot = T.slot0 + s * T.npix + local;
-            float4 L = ps.radiance[slot], c = ps.pend_c[slot];
-            r += (L.x + c.x) * V.inv_spp;
-            g += (L.y + c.y) * V.inv_spp;
-            b += (L.z + c.z) * V.inv_spp;
+        s_ns[t] = ns;
+        __syncthreads();
+        if (ns > 0) atomicMax(&s_max, ns);
+        __syncthreads();
+        const int maxns = s_max;
+        for (int c0 = 0; c0 < maxns; c0 += ACC_CHUNK) {
+#pragma unroll
+            for (int k = 0; k < ACC_CHUNK; ++k) {
+                const int e = k * PT_BLOCK + t, pe = e / ACC_CHUNK, se = c0 + e % ACC_CHUNK;
+                if (se < s_ns[pe]) {
+                    const int slot = s_base[pe] + se;
+                    const float4 L = ps.radiance[slot], cc = ps.pend_c[slot];
+                    const float inv = s_inv[pe];
+                    s_term[pe][e % ACC_CHUNK][0] = (L.x + cc.x) * inv;
+                    s_term[pe][e % ACC_CHUNK][1] = (L.y + cc.y) * inv;
+                    s_term[pe][e % ACC_CHUNK][2] = (L.z + cc.z) * inv;
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < ACC_CHUNK; ++k)
+                if (c0 + k < ns) {
+                    r += s_term[t][k][0];
+                    g += s_term[t][k][1];
+                    b += s_term[t][k][2];
+                }
+            __syncthreads();
         }
-        f[0] = r;
-        f[1] = g;
-        f[2] = b;
+        if (f) {
+            f[0] = r;
+            f[1] = g;
+            f[2] = b;
+        }
     }
 }
 hipError_t launch_accumulate(const PathState& ps, const DevView* views, const DevTile* tiles,
