@@ -57,8 +57,9 @@ __global__ void k_sf_dist(const float* __restrict__ guide, const float* __restri
 {
     const int64_t n = (int64_t)W * H;
     const float h2 = 0.4f * 0.4f;
-    for (int64_t i = (int64_t)blockIdx.x * FB + threadIdx.x; i < n; i += (int64_t)gridDim.x * FB) {
-        const int y = (int)(i / W), x = (int)(i % W);
+    const int x = blockIdx.x * FB + threadIdx.x;
+    for (int y = blockIdx.y; y < H && x < W; y += gridDim.y) {
+        const int64_t i = (int64_t)y * W + x;
         const bool in1 = y - dy >= 0 && y - dy < H && x - dx >= 0 && x - dx < W;
         const bool in2 = y + dy >= 0 && y + dy < H && x + dx >= 0 && x + dx < W;
         const int64_t j1 = in1 ? (int64_t)(y - dy) * W + (x - dx) : 0, j2 = in2 ? (int64_t)(y + dy) * W + (x + dx) : 0;
@@ -88,13 +89,13 @@ __global__ void k_box_rows(const float* __restrict__ in, float* out, int W, int 
 {
     const int64_t n = (int64_t)W * H;
     const int r = p / 2;
-    for (int64_t i = (int64_t)blockIdx.x * FB + threadIdx.x; i < 3 * n; i += (int64_t)gridDim.x * FB) {
-        const int64_t k = i / n, j = i % n;
-        const int y = (int)(j / W), x = (int)(j % W);
+    const int x = blockIdx.x * FB + threadIdx.x;
+    for (int y = blockIdx.y; y < H && x < W; y += gridDim.y)
+    for (int k = 0; k < 3; ++k) {
         const float* row = in + k * n + (int64_t)y * W;
         float s = 0.f;
         for (int q = -r; q <= r; ++q) s += row[reflect(x + q, W)];
-        out[i] = s;
+        out[k * n + (int64_t)y * W + x] = s;
     }
 }
 __global__ void k_box_cols(const float* __restrict__ in, float* out, int W, int H, int p)
@@ -102,13 +103,13 @@ __global__ void k_box_cols(const float* __restrict__ in, float* out, int W, int 
     const int64_t n = (int64_t)W * H;
     const int r = p / 2;
     const float inv = 1.f / (float)(p * p);
-    for (int64_t i = (int64_t)blockIdx.x * FB + threadIdx.x; i < 3 * n; i += (int64_t)gridDim.x * FB) {
-        const int64_t k = i / n, j = i % n;
-        const int y = (int)(j / W), x = (int)(j % W);
+    const int x = blockIdx.x * FB + threadIdx.x;
+    for (int y = blockIdx.y; y < H && x < W; y += gridDim.y)
+    for (int k = 0; k < 3; ++k) {
         const float* pl = in + k * n;
         float s = 0.f;
         for (int q = -r; q <= r; ++q) s += pl[(int64_t)reflect(y + q, H) * W + x];
-        out[i] = s * inv;
+        out[k * n + (int64_t)y * W + x] = s * inv;
     }
 }
 
@@ -125,7 +126,9 @@ __global__ void k_sf_accum(const float* __restrict__ w, const float* __restrict_
                            float* area, float* acc)
 {
     const int64_t n = (int64_t)W * H;
-    for (int64_t i = (int64_t)blockIdx.x * FB + threadIdx.x; i < n; i += (int64_t)gridDim.x * FB) {
+    const int x = blockIdx.x * FB + threadIdx.x;
+    for (int y = blockIdx.y; y < H && x < W; y += gridDim.y) {
+        const int64_t i = (int64_t)y * W + x;
         float ww[3];
         for (int k = 0; k < 3; ++k) {
             float x = fmaxf(w[k * n + i], 0.f);
@@ -133,7 +136,6 @@ __global__ void k_sf_accum(const float* __restrict__ w, const float* __restrict_
         }
         float w1 = ww[0], w2 = ww[1];
         if (ww[2] > fmaxf(w1, w2)) w1 = w2 = ww[2];
-        const int y = (int)(i / W), x = (int)(i % W);
         const bool in1 = y - dy >= 0 && y - dy < H && x - dx >= 0 && x - dx < W;
         const bool in2 = y + dy >= 0 && y + dy < H && x + dx >= 0 && x + dx < W;
         const int64_t j1 = (int64_t)(y - dy) * W + (x - dx), j2 = (int64_t)(y + dy) * W + (x + dx);
@@ -290,6 +292,7 @@ hipError_t launch_sym_filter(const float* image, const float* guide, const float
           *acc = tmp + 13 * n;  /* acc ends at 16 n = RT_SYM_FILTER_TMP * n */
     static_assert(RT_SYM_FILTER_TMP == 16, "scratch layout");
     const int g1 = blocks(n, grid), g3 = blocks(3 * n, grid);
+    const dim3 rows((W + FB - 1) / FB, std::min(H, 65535)); /* block rows walk image rows: no index division */
     hipLaunchKernelGGL(k_sf_init, dim3(g1), dim3(FB), 0, st, area, acc, n);
     for (int dx = -radius; dx <= 0; ++dx)
         for (int dy = -radius; dy <= radius; ++dy) {
@@ -297,21 +300,21 @@ hipError_t launch_sym_filter(const float* image, const float* guide, const float
                 hipLaunchKernelGGL(k_sf_centre, dim3(g1), dim3(FB), 0, st, image, area, acc, n);
                 break;
             }
-            hipLaunchKernelGGL(k_sf_dist, dim3(g1), dim3(FB), 0, st, guide, pixvar, W, H, dx, dy, d2, den);
+            hipLaunchKernelGGL(k_sf_dist, rows, dim3(FB), 0, st, guide, pixvar, W, H, dx, dy, d2, den);
             const float* db = d2;
             if (patch > 1) {
-                hipLaunchKernelGGL(k_box_rows, dim3(g3), dim3(FB), 0, st, d2, b0, W, H, patch);
-                hipLaunchKernelGGL(k_box_cols, dim3(g3), dim3(FB), 0, st, b0, b1, W, H, patch);
+                hipLaunchKernelGGL(k_box_rows, rows, dim3(FB), 0, st, d2, b0, W, H, patch);
+                hipLaunchKernelGGL(k_box_cols, rows, dim3(FB), 0, st, b0, b1, W, H, patch);
                 db = b1;
             }
             hipLaunchKernelGGL(k_sf_exp, dim3(g3), dim3(FB), 0, st, db, den, b0, 3 * n);
             const float* wb = b0;
             if (patch > 1) {
-                hipLaunchKernelGGL(k_box_rows, dim3(g3), dim3(FB), 0, st, b0, d2, W, H, patch);
-                hipLaunchKernelGGL(k_box_cols, dim3(g3), dim3(FB), 0, st, d2, b1, W, H, patch);
+                hipLaunchKernelGGL(k_box_rows, rows, dim3(FB), 0, st, b0, d2, W, H, patch);
+                hipLaunchKernelGGL(k_box_cols, rows, dim3(FB), 0, st, d2, b1, W, H, patch);
                 wb = b1;
             }
-            hipLaunchKernelGGL(k_sf_accum, dim3(g1), dim3(FB), 0, st, wb, image, W, H, dx, dy, area, acc);
+            hipLaunchKernelGGL(k_sf_accum, rows, dim3(FB), 0, st, wb, image, W, H, dx, dy, area, acc);
         }
     hipLaunchKernelGGL(k_sf_final, dim3(g1), dim3(FB), 0, st, area, acc, out, n);
     return hipGetLastError();

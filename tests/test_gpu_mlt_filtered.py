@@ -16,10 +16,13 @@ from tests.test_oracle_mlt import _cornell
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("patch,radius,H,W", [(7, 10, 37, 45), (1, 2, 37, 45), (3, 4, 37, 45), (7, 10, 220, 300)])
+@pytest.mark.parametrize("patch,radius,H,W", [(7, 10, 37, 45), (1, 2, 37, 45), (3, 4, 37, 45), (7, 10, 220, 300),
+                                                (11, 3, 37, 45), (7, 3, 6, 5)])
 def test_symmetric_filter_matches_oracle(gpu_ctx, patch, radius, H, W):
     # 300 x 220: the filter's scratch (16 floats per pixel) is the last
-    # region of its buffer, so a short scratch would write 256 KiB past it
+    # region of its buffer, so a short scratch would write 256 KiB past it;
+    # patch 11 takes the multi-pass form (blur radius above the one-launch
+    # kernel's), 5 x 6 under patch 7 reflects indices more than once
     rng = np.random.default_rng(patch)
     base = np.zeros((H, W, 3), np.float32)
     base[:, W // 3:] = (0.8, 0.5, 0.2)
