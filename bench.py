@@ -228,8 +228,17 @@ def equal_time(scene, desc, max_depth, budget_s, ref_spp, pass_spp, train_passes
     adaptive = []
     gp = GuidedProgressive(ctx, scene.camera, pass_spp=pass_spp, train_passes=train_passes, probe_passes=6, lr=lr,
                            **common)
-    _, ainfo = gp.run(budget_s, seed0=50000,
-                      on_pass=lambda t, spp, img: adaptive.append((t, spp, rel_mse(img.cpu().numpy(), ref) - ref_noise)))
+    # network work of the fallback run too, so that kernel-trace totals of
+    # the whole section can be divided by the vertices and records behind them
+    net_fb = dict(infer_samples=0, train_samples=0, train_steps=0)
+
+    def on_pass(t, spp, img):
+        adaptive.append((t, spp, rel_mse(img.cpu().numpy(), ref) - ref_noise))
+        s = ctx.stats()
+        for k in net_fb:
+            net_fb[k] += s["guide_" + k]
+
+    _, ainfo = gp.run(budget_s, seed0=50000, on_pass=on_pass)
     ctx.close()
     rp = plain[-1][2]
     # time at which guided first reaches the plain estimator's final relMSE
@@ -271,6 +280,7 @@ def equal_time(scene, desc, max_depth, budget_s, ref_spp, pass_spp, train_passes
             "mfma_peak_tflops": MFMA_FP16_PEAK_TFLOPS,
             "infer_mfma_frac": round(inf_tf / MFMA_FP16_PEAK_TFLOPS, 4) if inf_tf else None,
             "train_mfma_frac": round(trn_tf / MFMA_FP16_PEAK_TFLOPS, 4) if trn_tf else None,
+            "with_fallback_run": {k: int(net[k] + net_fb[k]) for k in net_fb},
         },
     }
 

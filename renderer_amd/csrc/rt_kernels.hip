@@ -1512,10 +1512,10 @@ hipError_t launch_advance(const PathState& ps, int qsel, int depth, hipStream_t 
  * reference's film[i][j] += s/spp, src/perpixelrenderer.cpp:92).  A
  * pixel's samples occupy consecutive slots, so a thread reading its own
  * would touch a different cache line than its neighbours at every load; the
- * block instead reads its pixels' samples four at a time, four lanes to
- * each 64-byte group of slots, into LDS, and each thread then adds its
- * pixel's terms in order. */
-#define ACC_CHUNK 4
+ * block instead reads its pixels' samples eight at a time, eight lanes to
+ * each 128-byte group of slots (one L2 line of each array), into LDS, and
+ * each thread then adds its pixel's terms in order. */
+#define ACC_CHUNK 8
 __global__ __launch_bounds__(PT_BLOCK) void k_accumulate(PathState ps, const DevView* __restrict__ views,
                                                         const DevTile* __restrict__ tiles,
                                                         const DevPass* __restrict__ passes)
