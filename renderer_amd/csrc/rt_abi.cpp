@@ -1068,7 +1068,7 @@ static int guide_alloc_base(rt_ctx* c)
     int r;
     if ((r = ensure(c, c->g_theta, GD_NPARAM * 4)) || (r = ensure(c, c->g_w16, GD_NW16 * 2)) ||
         (r = ensure(c, c->g_m, GD_NPARAM * 4)) || (r = ensure(c, c->g_v, GD_NPARAM * 4)) ||
-        (r = ensure(c, c->g_grad, GD_NPARAM * 4)) || (r = ensure(c, c->g_acc, 32)) || (r = ensure(c, c->g_count, 32)) ||
+        (r = ensure(c, c->g_grad, GD_NPARAM * 4)) || (r = ensure(c, c->g_acc, GD_ACC_FLOATS * 4)) || (r = ensure(c, c->g_count, 32)) ||
         (r = ensure(c, c->g_part, (size_t)c->n_cu * GD_TRAIN_BLOCKS_PER_CU * GD_NPARAM * 4)))
         return r;
     GuideState& g = c->guide;
@@ -1128,6 +1128,7 @@ static int guide_upload_params(rt_ctx* c, const float* theta)
     HIP_TRY(c, hipMemset(c->guide.adam_v, 0, GD_NPARAM * 4));
     HIP_TRY(c, hipMemset(c->guide.grad, 0, GD_NPARAM * 4));
     HIP_TRY(c, hipMemset(c->guide.acc, 0, 16));
+    HIP_TRY(c, hipMemset(c->guide.acc + GD_WSUM_OFF, 0, GD_WSUM_N * 4));
     HIP_TRY(c, hipMemset(c->guide.rec_count, 0, 32));
     c->guide_step = 0;
     c->guide_ready = true;

@@ -80,6 +80,12 @@
 #define GD_INFER_BLOCKS_PER_CU 3
 #endif
 #define GD_TRAIN_BLOCKS_PER_CU 1
+/* The weight sum of a training step is accumulated in GD_WSUM_N parts
+ * (block b adds to part b % GD_WSUM_N: same-address atomics from a
+ * thousand blocks serialise), summed by the training kernel. */
+#define GD_WSUM_OFF 8
+#define GD_WSUM_N 16
+#define GD_ACC_FLOATS (GD_WSUM_OFF + GD_WSUM_N)
 #define GD_TRAIN_THREADS 256   /* 4 waves: one per SIMD, its gradient sums in the accumulation registers */
 
 /* Training record (16 floats):
@@ -97,7 +103,7 @@ struct GuideState {
     float* grad;
     float* part;        /* per training block: its gradient sums, GD_NPARAM each */
     int32_t part_blocks;
-    float* acc;         /* [0] weight sum, [1] loss sum, [2] records trained */
+    float* acc;         /* [0] weight sum, [1] loss sum, [2] records trained, [GD_WSUM_OFF ..] weight-sum parts */
     float* keep;        /* probability that a render keeps a path vertex as a training record */
     int32_t* rec_count;
     unsigned long long* trained;  /* running total of records over all Adam steps */

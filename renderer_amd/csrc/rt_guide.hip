@@ -510,7 +510,7 @@ __global__ __launch_bounds__(GB) void k_guide_weights(GuideState g, PathState ps
     if (threadIdx.x == 0) {
         float s = 0.f;
         for (int k = 0; k < GB / 64; ++k) s += part[k];
-        if (s != 0.f) atomicAdd(&g.acc[0], s);
+        if (s != 0.f) atomicAdd(&g.acc[GD_WSUM_OFF + blockIdx.x % GD_WSUM_N], s);
     }
 }
 
@@ -526,6 +526,7 @@ hipError_t launch_guide_weights(const GuideState& g, const PathState& ps, int of
 __global__ void k_guide_zero_acc(GuideState g)
 {
     if (threadIdx.x < 3) g.acc[threadIdx.x] = 0.f;
+    if (threadIdx.x < GD_WSUM_N) g.acc[GD_WSUM_OFF + threadIdx.x] = 0.f;
 }
 
 hipError_t launch_guide_zero_acc(const GuideState& g, hipStream_t st)
@@ -719,7 +720,9 @@ __global__ __launch_bounds__(GD_TRAIN_THREADS) __attribute__((amdgpu_waves_per_e
     const int lane = __lane_id(), n = lane & 15, q = lane >> 4, w = threadIdx.x >> 6;
     const int N = chunk_len(g, count, off, max_n);
     const float* rec = rec0 + (size_t)off * GD_REC;
-    const float wsum = g.acc[0];
+    float wsum = 0.f;
+#pragma unroll
+    for (int j = 0; j < GD_WSUM_N; ++j) wsum += g.acc[GD_WSUM_OFF + j];
     const float wscale = wsum > 0.f ? (float)N / wsum : 0.f;
     float (*IN)[GD_TIN] = L.s.in[w];
     float lacc = 0.f;
@@ -915,10 +918,14 @@ __global__ __launch_bounds__(GD_TRAIN_THREADS) __attribute__((amdgpu_waves_per_e
         __syncthreads();
     }
     float* part = g.part + (size_t)blockIdx.x * GD_NPARAM;
-    for (int k = threadIdx.x; k < GD_NSUM * 256; k += blockDim.x) {
-        const int t = k >> 8, e = k & 3, l = (k >> 2) & 63;
-        const int pi = t < GD_NTILE ? grad_param(t, e, l) : ((l & 15) < GD_NBT ? bias_param(l & 15, e, l >> 4) : -1);
-        if (pi >= 0) part[pi] = L.acc[k];
+    static_assert(GD_TRAIN_THREADS == 256, "one thread per (lane, element) of a tile");
+    {
+        /* tile-major, so the tile index is uniform and its arithmetic scalar */
+        const int e = threadIdx.x & 3, l = threadIdx.x >> 2;
+        for (int t = 0; t < GD_NSUM; ++t) {
+            const int pi = t < GD_NTILE ? grad_param(t, e, l) : ((l & 15) < GD_NBT ? bias_param(l & 15, e, l >> 4) : -1);
+            if (pi >= 0) part[pi] = L.acc[t * 256 + threadIdx.x];
+        }
     }
     for (int o = 32; o > 0; o >>= 1) lacc += __shfl_xor(lacc, o);
     if (lane == 0 && lacc != 0.f) atomicAdd(&g.acc[1], lacc);
@@ -1005,6 +1012,7 @@ __global__ void k_guide_reset(GuideState g, int slots)
     if (i == 0) {
         *g.rec_count = 0;
         g.acc[0] = g.acc[1] = g.acc[2] = 0.f;
+        for (int j = 0; j < GD_WSUM_N; ++j) g.acc[GD_WSUM_OFF + j] = 0.f;
     }
     for (; i < slots; i += gridDim.x * GB) g.open_rec[i] = -1;
 }
