@@ -251,6 +251,41 @@ struct Wide {
     }
 };
 
+/* Renumbers the 4-wide nodes so that the first levels come first, breadth
+ * first (RT_TOP_NODES of them at most), and the others keep the depth-first
+ * order of the collapse.  Child words follow
+ * the new numbering; a traversal visits the same nodes in the same order. */
+void top_levels_first(std::vector<float>& nodes)
+{
+    const size_t n = nodes.size() / RT_NODE_FLOATS;
+    auto word = [&](const std::vector<float>& v, size_t node, int k) {
+        int32_t w;
+        std::memcpy(&w, &v[node * RT_NODE_FLOATS + 24 + k], 4);
+        return w;
+    };
+    std::vector<int32_t> order{0};
+    for (size_t i = 0; i < order.size() && order.size() < RT_TOP_NODES; ++i)
+        for (int k = 0; k < 4 && order.size() < RT_TOP_NODES; ++k) {
+            const int32_t w = word(nodes, (size_t)order[i], k);
+            if (w >= 0) order.push_back(w);
+        }
+    std::vector<int32_t> idx(n, -1);
+    for (size_t j = 0; j < order.size(); ++j) idx[(size_t)order[j]] = (int32_t)j;
+    int32_t next = (int32_t)order.size();
+    for (size_t i = 0; i < n; ++i)
+        if (idx[i] < 0) idx[i] = next++;
+    std::vector<float> out(nodes.size());
+    for (size_t i = 0; i < n; ++i) {
+        float* dst = &out[(size_t)idx[i] * RT_NODE_FLOATS];
+        std::memcpy(dst, &nodes[i * RT_NODE_FLOATS], RT_NODE_FLOATS * sizeof(float));
+        for (int k = 0; k < 4; ++k) {
+            const int32_t w = word(nodes, i, k);
+            if (w >= 0) std::memcpy(&dst[24 + k], &idx[(size_t)w], 4);
+        }
+    }
+    nodes.swap(out);
+}
+
 }  // namespace
 
 int build(std::vector<PrimRef>& refs, BuildResult& out, int method, int sah_levels)
@@ -279,6 +314,7 @@ int build(std::vector<PrimRef>& refs, BuildResult& out, int method, int sah_leve
     Wide wt{bin, out.nodes};
     int need = 0;
     wt.collapse(r, 0, need);
+    top_levels_first(out.nodes);
     out.depth = wt.max_depth + 1;
     out.stack_need = need;
     if (need + 1 > RT_STACK_DEPTH) return -2;

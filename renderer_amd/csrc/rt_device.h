@@ -538,13 +538,17 @@ struct LaneStack {
  * when the traversal of this ray is complete.  The stack needs room for the
  * builder's stack_need. */
 template <bool ANY, bool COUNT, class Stack>
-__device__ __forceinline__ bool trav_step(const DevScene& S, Trav& T, Stack& stack, int& cn, int& cp)
+__device__ __forceinline__ bool trav_step(const DevScene& S, Trav& T, Stack& stack, int& cn, int& cp,
+                                          const char* top = nullptr)
 {
     /* The near and far planes of each axis are fetched directly (the pair
      * lo/hi sits 16 B apart, so near = base + T.n?, far = that ^ 16): no
-     * per-child selects.  Offsets are 32-bit from the uniform node base, which
-     * bounds the tree at 2^25 nodes (the upload checks it). */
-    const char* nb = reinterpret_cast<const char*>(S.nodes);
+     * per-child selects.  Offsets are 32-bit from the node base, which
+     * bounds the tree at 2^25 nodes (the upload checks it).  `top`: the
+     * block's LDS copy of nodes 0 .. RT_TOP_NODES - 1 (a generic pointer, so
+     * that one load instruction serves the lanes at the top of the tree from
+     * LDS and the others from memory), or none. */
+    const char* nb = top && T.cur < RT_TOP_NODES ? top : reinterpret_cast<const char*>(S.nodes);
     const uint32_t c = (uint32_t)T.cur << 7;
     const uint32_t ax = c + T.nx, ay = c + 32u + T.ny, az = c + 64u + T.nz;
     const float4 nx4 = *reinterpret_cast<const float4*>(nb + ax);

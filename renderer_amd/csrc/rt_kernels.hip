@@ -111,6 +111,18 @@ __device__ __forceinline__ void block_append(const bool* pred, int32_t* const* c
     __syncthreads();  /* cnt and base are reused by the next call */
 }
 
+/* The block's LDS copy of the tree's first nodes (RT_TOP_NODES), which
+ * trav_step reads for the rays at the top of the tree; call before any
+ * traversal, in uniform control flow. */
+__device__ __forceinline__ const char* stage_top_nodes(const DevScene& S, float4* top)
+{
+    const int nt = min(S.n_nodes, RT_TOP_NODES) * (RT_NODE_FLOATS / 4);
+    const float4* src = reinterpret_cast<const float4*>(S.nodes);
+    for (int i = threadIdx.x; i < nt; i += blockDim.x) top[i] = src[i];
+    __syncthreads();
+    return reinterpret_cast<const char*>(top);
+}
+
 /* Dynamic work distribution of the persistent traversal kernels: a wave
  * claims queue positions from a global counter in chunks of 64 (one atomic
  * per chunk) and hands them out, in lane order, to the lanes that have
@@ -855,11 +867,14 @@ hipError_t launch_ray_values(const PathState& ps, int n, float* rgb, int grid, h
  * (WaveFeed), instead of idling until the slowest ray of its wave finishes;
  * waves claim rays dynamically, so the launch ends evenly.
  * The traversal stack (LaneStack) keeps RT_STACK_LDS entries per lane in
- * LDS: 16 KB per block, so LDS leaves room for 10 blocks per CU. */
+ * LDS: 16 KB per block, and 8.6 KB for the top nodes: room for the 6
+ * blocks per CU that the registers allow. */
 template <bool COUNT, bool OBJ>
 __global__ __launch_bounds__(PT_BLOCK) RT_TRAV_OCCUPANCY void k_extend(DevScene S, PathState ps, int qsel)
 {
     __shared__ int32_t stack[RT_STACK_LDS * PT_BLOCK];
+    __shared__ float4 top_nodes[RT_TOP_NODES * (RT_NODE_FLOATS / 4)];
+    const char* top = stage_top_nodes(S, top_nodes);
     LaneStack my(stack + threadIdx.x, PT_BLOCK, S.stack_ovf);
     const int n = ps.cnt->q_count[qsel];
     const int32_t* q = ps.queue[qsel];
@@ -884,7 +899,7 @@ __global__ __launch_bounds__(PT_BLOCK) RT_TRAV_OCCUPANCY void k_extend(DevScene 
             }
         }
         if (!__ballot(live)) break;  /* queue drained and every lane idle */
-        if (live && (T.cur < 0 || trav_step<false, COUNT>(S, T, my, cn, cp))) {
+        if (live && (T.cur < 0 || trav_step<false, COUNT>(S, T, my, cn, cp, top))) {
             ps.hit[p] = make_float4(T.g >= 0 ? T.best : RT_INF, __int_as_float(T.g), T.u, T.v);
             live = false;
         }
@@ -1422,6 +1437,8 @@ template <bool OBJ, bool COUNT>
 __global__ __launch_bounds__(PT_BLOCK) RT_TRAV_OCCUPANCY void k_shadow(DevScene S, PathState ps, int compat)
 {
     __shared__ int32_t stack[RT_STACK_LDS * PT_BLOCK];
+    __shared__ float4 top_nodes[RT_TOP_NODES * (RT_NODE_FLOATS / 4)];
+    const char* top = stage_top_nodes(S, top_nodes);
     LaneStack my(stack + threadIdx.x, PT_BLOCK, S.stack_ovf);
     const int n = ps.cnt->shadow_count;
     int cn = 0, cp = 0, item = 0;
@@ -1448,7 +1465,7 @@ __global__ __launch_bounds__(PT_BLOCK) RT_TRAV_OCCUPANCY void k_shadow(DevScene 
             }
         }
         if (!__ballot(live)) break;  /* queue drained and every lane idle */
-        if (live && (T.cur < 0 || trav_step<true, COUNT>(S, T, my, cn, cp))) {
+        if (live && (T.cur < 0 || trav_step<true, COUNT>(S, T, my, cn, cp, top))) {
             if (T.g < 0) {
                 /* unoccluded: add the contribution to its path */
                 float4 c = ps.sh_c[item];

@@ -78,6 +78,14 @@ static inline RT_HD int rt_probe_texel(float dx, float dy, float dz, int W, int 
 #endif
 #define RT_STACK_OVF (RT_STACK_DEPTH - RT_STACK_LDS - RT_STACK_REG)
 #define RT_NODE_FLOATS 32  /* 128 B per 4-wide node */
+/* Nodes 0 .. RT_TOP_NODES - 1 are the tree's first levels in breadth-first
+ * order (the root, its children, theirs, and the first 48 nodes of the
+ * fourth level); the traversal kernels keep them in LDS (8.6 KB per block),
+ * since every ray starts there.  With the 16 KB stack a block takes 24.6 KB
+ * of LDS, so the 6 blocks per CU that the registers allow still fit. */
+#ifndef RT_TOP_NODES
+#define RT_TOP_NODES 69
+#endif
 
 /* Material record (64 B).  type: rt_material_type. */
 struct RtMaterial {
