@@ -679,7 +679,8 @@ __device__ __forceinline__ bool trav_step(const DevScene& S, Trav& T, Stack& sta
 /* Whole traversal of one ray (batch kernels and counting). */
 template <bool ANY, bool COUNT, class Stack>
 __device__ __forceinline__ TraceOut traverse(const DevScene& S, vec3 o, vec3 d, float tmin, float tmax,
-                                             Stack& stack, int* n_nodes = nullptr, int* n_prims = nullptr)
+                                             Stack& stack, int* n_nodes = nullptr, int* n_prims = nullptr,
+                                             const char* top = nullptr)
 {
     Trav T;
     trav_init(S, T, o, d, tmin, tmax);
@@ -695,7 +696,7 @@ __device__ __forceinline__ TraceOut traverse(const DevScene& S, vec3 o, vec3 d, 
         }
     }
     if (T.cur >= 0)
-        while (!trav_step<ANY, COUNT>(S, T, stack, cn, cp)) {
+        while (!trav_step<ANY, COUNT>(S, T, stack, cn, cp, top)) {
         }
     if (COUNT) {
         *n_nodes = cn;

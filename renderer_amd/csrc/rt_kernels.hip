@@ -111,18 +111,6 @@ __device__ __forceinline__ void block_append(const bool* pred, int32_t* const* c
     __syncthreads();  /* cnt and base are reused by the next call */
 }
 
-/* The block's LDS copy of the tree's first nodes (RT_TOP_NODES), which
- * trav_step reads for the rays at the top of the tree; call before any
- * traversal, in uniform control flow. */
-__device__ __forceinline__ const char* stage_top_nodes(const DevScene& S, float4* top)
-{
-    const int nt = min(S.n_nodes, RT_TOP_NODES) * (RT_NODE_FLOATS / 4);
-    const float4* src = reinterpret_cast<const float4*>(S.nodes);
-    for (int i = threadIdx.x; i < nt; i += blockDim.x) top[i] = src[i];
-    __syncthreads();
-    return reinterpret_cast<const char*>(top);
-}
-
 /* Dynamic work distribution of the persistent traversal kernels: a wave
  * claims queue positions from a global counter in chunks of 64 (one atomic
  * per chunk) and hands them out, in lane order, to the lanes that have
@@ -158,17 +146,31 @@ __device__ __forceinline__ int feed_take(WaveFeed& f, bool want, int32_t* counte
 /* ===========================================================================
  * Batch traversal kernels
  * ========================================================================= */
+/* The block's LDS copy of the tree's first nodes (RT_TOP_NODES), which
+ * trav_step reads for the rays at the top of the tree; call before any
+ * traversal, in uniform control flow. */
+__device__ __forceinline__ const char* stage_top_nodes(const DevScene& S, float4* top)
+{
+    const int nt = min(S.n_nodes, RT_TOP_NODES) * (RT_NODE_FLOATS / 4);
+    const float4* src = reinterpret_cast<const float4*>(S.nodes);
+    for (int i = threadIdx.x; i < nt; i += blockDim.x) top[i] = src[i];
+    __syncthreads();
+    return reinterpret_cast<const char*>(top);
+}
+
 template <bool ANY, bool COUNT>
 __global__ __launch_bounds__(PT_BLOCK) void k_trace(DevScene S, const float4* __restrict__ o,
                                                    const float4* __restrict__ d, int64_t n, int32_t* prim, float* tout,
                                                    uint8_t* occ, int32_t* nn, int32_t* np)
 {
     __shared__ int32_t stack[RT_STACK_LDS * PT_BLOCK];
+    __shared__ float4 top_nodes[RT_TOP_NODES * (RT_NODE_FLOATS / 4)];
+    const char* top = stage_top_nodes(S, top_nodes);
     LaneStack my(stack + threadIdx.x, PT_BLOCK, S.stack_ovf);
     for (int64_t i = (int64_t)blockIdx.x * PT_BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * PT_BLOCK) {
         float4 a = o[i], b = d[i];
         int cn = 0, cp = 0;
-        TraceOut h = traverse<ANY, COUNT>(S, xyz(a), xyz(b), a.w, b.w, my, &cn, &cp);
+        TraceOut h = traverse<ANY, COUNT>(S, xyz(a), xyz(b), a.w, b.w, my, &cn, &cp, top);
         /* counting launches write only the counts (the hit outputs are null) */
         if (COUNT) {
             nn[i] = cn;
