@@ -26,6 +26,8 @@ HBM_PEAK_GBS = 8000.0
 RAY_BYTES = 32 + 16      # ray (origin, tmin, direction, tmax) + hit (t, primitive, u, v)
 SHADOW_RAY_BYTES = 32 + 16  # ray + its contribution record (rgb, path slot)
 NODE_BYTES = 128         # one 4-wide node: four child boxes + child words
+# node visits to the tree's first nodes read the traversal kernels' LDS copy
+# of them: no memory traffic, so they count no bytes
 PRIM_BYTES = 48          # one primitive record
 
 
@@ -843,12 +845,14 @@ def main():
     cs = ctx.stats()
     ctx.set_launch_config(count_traversal=0)
     ext_rays = cs["camera_rays"] + cs["extension_rays"]
-    alg_bytes = ext_rays * RAY_BYTES + cs["node_visits"] * NODE_BYTES + cs["prim_tests"] * PRIM_BYTES
+    mem_nodes = cs["node_visits"] - cs["node_visits_lds"]
+    alg_bytes = ext_rays * RAY_BYTES + mem_nodes * NODE_BYTES + cs["prim_tests"] * PRIM_BYTES
     per_launch_bytes = alg_bytes / max(1, cs["extend_launches"])
     avg_launch_ms = ext_ms / max(1, ext_launches)
     achieved = per_launch_bytes / (avg_launch_ms * 1e-3) / 1e9 if avg_launch_ms > 0 else None
     # the same for the any-hit launches (one per bounce, like the closest-hit ones)
-    sh_bytes = (cs["shadow_rays"] * SHADOW_RAY_BYTES + cs["shadow_node_visits"] * NODE_BYTES +
+    sh_mem_nodes = cs["shadow_node_visits"] - cs["shadow_node_visits_lds"]
+    sh_bytes = (cs["shadow_rays"] * SHADOW_RAY_BYTES + sh_mem_nodes * NODE_BYTES +
                 cs["shadow_prim_tests"] * PRIM_BYTES)
     sh_per_launch = sh_bytes / max(1, cs["extend_launches"])
     sh_avg_ms = sh_ms / max(1, ext_launches)
@@ -956,6 +960,7 @@ def main():
             "avg_launch_ms": round(avg_launch_ms, 4),
             "alg_bytes_per_launch": int(per_launch_bytes),
             "nodes_per_ray": round(cs["node_visits"] / max(1, ext_rays), 2),
+            "lds_nodes_per_ray": round(cs["node_visits_lds"] / max(1, ext_rays), 2),
             "prims_per_ray": round(cs["prim_tests"] / max(1, ext_rays), 2),
             "l2_hit_rate": l2,
         },
@@ -970,6 +975,7 @@ def main():
             "avg_launch_ms": round(sh_avg_ms, 4),
             "alg_bytes_per_launch": int(sh_per_launch),
             "nodes_per_ray": round(cs["shadow_node_visits"] / max(1, cs["shadow_rays"]), 2),
+            "lds_nodes_per_ray": round(cs["shadow_node_visits_lds"] / max(1, cs["shadow_rays"]), 2),
             "prims_per_ray": round(cs["shadow_prim_tests"] / max(1, cs["shadow_rays"]), 2),
             "l2_hit_rate": sh_l2,
         },

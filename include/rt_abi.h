@@ -77,6 +77,11 @@ typedef struct rt_stats {
     double tonemap_ms;            /* duration of the last rt_film_tonemap kernel (GPU events) */
     int64_t shadow_node_visits;   /* interior nodes fetched by any-hit traversal (count_traversal) */
     int64_t shadow_prim_tests;    /* primitive records tested by any-hit traversal (count_traversal) */
+    /* of node_visits and shadow_node_visits, the visits to the tree's first
+     * nodes, which the traversal kernels read from their LDS copy (no
+     * memory traffic; count_traversal) */
+    int64_t node_visits_lds;
+    int64_t shadow_node_visits_lds;
 } rt_stats;
 
 /* Library-wide. */

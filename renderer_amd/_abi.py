@@ -104,6 +104,8 @@ class RtStats(ctypes.Structure):
         ("tonemap_ms", ctypes.c_double),
         ("shadow_node_visits", ctypes.c_int64),
         ("shadow_prim_tests", ctypes.c_int64),
+        ("node_visits_lds", ctypes.c_int64),
+        ("shadow_node_visits_lds", ctypes.c_int64),
     ]
 
 

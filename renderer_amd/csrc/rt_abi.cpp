@@ -1570,6 +1570,8 @@ static int render_impl(rt_ctx* c, const rt_view* views, int32_t nv, const rt_par
     c->stats.prim_tests = (int64_t)hc.prim_tests;
     c->stats.shadow_node_visits = (int64_t)hc.sh_node_visits;
     c->stats.shadow_prim_tests = (int64_t)hc.sh_prim_tests;
+    c->stats.node_visits_lds = (int64_t)hc.node_visits_lds;
+    c->stats.shadow_node_visits_lds = (int64_t)hc.sh_node_visits_lds;
     c->stats.paths_per_pass = (int32_t)P;
     c->stats.passes = (int32_t)passes.size();
     if (!device_films) {

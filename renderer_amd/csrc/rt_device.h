@@ -539,7 +539,7 @@ struct LaneStack {
  * builder's stack_need. */
 template <bool ANY, bool COUNT, class Stack>
 __device__ __forceinline__ bool trav_step(const DevScene& S, Trav& T, Stack& stack, int& cn, int& cp,
-                                          const char* top = nullptr)
+                                          const char* top = nullptr, int* ct = nullptr)
 {
     /* The near and far planes of each axis are fetched directly (the pair
      * lo/hi sits 16 B apart, so near = base + T.n?, far = that ^ 16): no
@@ -558,7 +558,10 @@ __device__ __forceinline__ bool trav_step(const DevScene& S, Trav& T, Stack& sta
     const float4 nz4 = *reinterpret_cast<const float4*>(nb + az);
     const float4 fz4 = *reinterpret_cast<const float4*>(nb + (az ^ 16u));
     const float4 cw = *reinterpret_cast<const float4*>(nb + c + 96u);
-    if (COUNT) ++cn;
+    if (COUNT) {
+        ++cn;
+        if (ct && nb == top) ++*ct; /* a visit read from the LDS copy */
+    }
     const float slack = 9.5367431640625e-07f; /* 2^-20 */
     float bound = T.best;
     bound = bound + fabsf(bound) * slack;

@@ -80,6 +80,7 @@ struct DevCounters {
     unsigned long long camera_rays, extension_rays, shadow_rays;
     unsigned long long node_visits, prim_tests;
     unsigned long long sh_node_visits, sh_prim_tests;  /* any-hit traversal (count_traversal) */
+    unsigned long long node_visits_lds, sh_node_visits_lds; /* of those, visits read from the LDS copy */
 };
 
 /* Path state in HBM (structure of arrays, capacity = path slots). */

@@ -880,7 +880,7 @@ __global__ __launch_bounds__(PT_BLOCK) RT_TRAV_OCCUPANCY void k_extend(DevScene 
     LaneStack my(stack + threadIdx.x, PT_BLOCK, S.stack_ovf);
     const int n = ps.cnt->q_count[qsel];
     const int32_t* q = ps.queue[qsel];
-    int cn = 0, cp = 0, p = 0;
+    int cn = 0, cp = 0, ct = 0, p = 0;
     bool live = false, more = true;
     WaveFeed feed;
     Trav T;
@@ -901,7 +901,7 @@ __global__ __launch_bounds__(PT_BLOCK) RT_TRAV_OCCUPANCY void k_extend(DevScene 
             }
         }
         if (!__ballot(live)) break;  /* queue drained and every lane idle */
-        if (live && (T.cur < 0 || trav_step<false, COUNT>(S, T, my, cn, cp, top))) {
+        if (live && (T.cur < 0 || trav_step<false, COUNT>(S, T, my, cn, cp, top, &ct))) {
             ps.hit[p] = make_float4(T.g >= 0 ? T.best : RT_INF, __int_as_float(T.g), T.u, T.v);
             live = false;
         }
@@ -909,6 +909,7 @@ __global__ __launch_bounds__(PT_BLOCK) RT_TRAV_OCCUPANCY void k_extend(DevScene 
     if (COUNT && (cn || cp)) {
         atomicAdd(&ps.cnt->node_visits, (unsigned long long)cn);
         atomicAdd(&ps.cnt->prim_tests, (unsigned long long)cp);
+        atomicAdd(&ps.cnt->node_visits_lds, (unsigned long long)ct);
     }
 }
 
@@ -1443,7 +1444,7 @@ __global__ __launch_bounds__(PT_BLOCK) RT_TRAV_OCCUPANCY void k_shadow(DevScene 
     const char* top = stage_top_nodes(S, top_nodes);
     LaneStack my(stack + threadIdx.x, PT_BLOCK, S.stack_ovf);
     const int n = ps.cnt->shadow_count;
-    int cn = 0, cp = 0, item = 0;
+    int cn = 0, cp = 0, ct = 0, item = 0;
     bool live = false, more = true;
     WaveFeed feed;
     Trav T;
@@ -1467,7 +1468,7 @@ __global__ __launch_bounds__(PT_BLOCK) RT_TRAV_OCCUPANCY void k_shadow(DevScene 
             }
         }
         if (!__ballot(live)) break;  /* queue drained and every lane idle */
-        if (live && (T.cur < 0 || trav_step<true, COUNT>(S, T, my, cn, cp, top))) {
+        if (live && (T.cur < 0 || trav_step<true, COUNT>(S, T, my, cn, cp, top, &ct))) {
             if (T.g < 0) {
                 /* unoccluded: add the contribution to its path */
                 float4 c = ps.sh_c[item];
@@ -1485,6 +1486,7 @@ __global__ __launch_bounds__(PT_BLOCK) RT_TRAV_OCCUPANCY void k_shadow(DevScene 
     if (COUNT && (cn || cp)) {
         atomicAdd(&ps.cnt->sh_node_visits, (unsigned long long)cn);
         atomicAdd(&ps.cnt->sh_prim_tests, (unsigned long long)cp);
+        atomicAdd(&ps.cnt->sh_node_visits_lds, (unsigned long long)ct);
     }
 }
 
@@ -1627,6 +1629,7 @@ __global__ void k_reset(PathState ps)
     c->camera_rays = c->extension_rays = c->shadow_rays = 0ull;
     c->node_visits = c->prim_tests = 0ull;
     c->sh_node_visits = c->sh_prim_tests = 0ull;
+    c->node_visits_lds = c->sh_node_visits_lds = 0ull;
 }
 hipError_t launch_reset_counters(const PathState& ps, hipStream_t st)
 {

@@ -71,3 +71,30 @@ def test_counts_box_scene(gpu_ctx):
     arr = scenes.box_scene(32, 32).flatten()
     o4, d4 = random_rays(50000, (0.02, 0.02, 0.02), (0.98, 0.98, 1.5), seed=24)
     _check(gpu_ctx, arr, o4, d4, "box scene")
+
+
+@pytest.mark.parametrize("which", ["box", "mesh"])
+def test_render_counts_visits_read_from_lds(gpu_ctx, which):
+    # the traversal kernels read the tree's first 69 nodes from LDS and count
+    # those visits apart (they carry no memory traffic in the roofline):
+    # every visit of the box scene's 4-node tree is one, a 131K-triangle
+    # mesh's deeper nodes are not
+    from renderer_amd import MODE_UNBIASED
+
+    sc = scenes.box_scene(32, 32) if which == "box" else scenes.mesh_scene(64, 64, n=128, seed=1)
+    gpu_ctx.upload(sc.flatten())
+    gpu_ctx.set_launch_config(count_traversal=1)
+    try:
+        gpu_ctx.render([(sc.camera, 2, 2)], mode=MODE_UNBIASED, max_depth=4, seed=5)
+        s = gpu_ctx.stats()
+    finally:
+        gpu_ctx.set_launch_config(count_traversal=0)
+    print(f"{which}: {s['bvh_nodes']} nodes; closest-hit visits {s['node_visits']} ({s['node_visits_lds']} from LDS), "
+          f"any-hit {s['shadow_node_visits']} ({s['shadow_node_visits_lds']} from LDS)", flush=True)
+    assert s["node_visits"] > 0 and s["shadow_node_visits"] > 0
+    if which == "box":
+        assert s["node_visits_lds"] == s["node_visits"]
+        assert s["shadow_node_visits_lds"] == s["shadow_node_visits"]
+    else:
+        assert 0 < s["node_visits_lds"] < s["node_visits"]
+        assert 0 < s["shadow_node_visits_lds"] < s["shadow_node_visits"]
