@@ -71,12 +71,19 @@ struct DevPass {
 };
 
 /* Device counters of the wavefront pipeline. */
+/* Work counters of the persistent traversal launches: the queue is cut into
+ * RT_XCDS equal segments, one per XCD (workgroups go to the XCDs round-robin,
+ * so block b runs on XCD b % RT_XCDS), each with its own claim counter on a
+ * 128-byte line of its own. */
+#define RT_XCDS 8
+#define RT_CLAIM_STRIDE 32
+
 struct DevCounters {
     int32_t q_count[2];      /* path queue lengths (ping-pong) */
     int32_t shadow_count;
     int32_t pass;            /* current pass index */
-    int32_t ext_next;        /* work counters of the closest-hit and any-hit launches: */
-    int32_t sh_next;         /* the next unclaimed queue position (zero before each launch) */
+    int32_t ext_next[RT_XCDS * RT_CLAIM_STRIDE]; /* closest-hit and any-hit launches: the next unclaimed */
+    int32_t sh_next[RT_XCDS * RT_CLAIM_STRIDE];  /* position in each segment (zero before each launch) */
     unsigned long long camera_rays, extension_rays, shadow_rays;
     unsigned long long node_visits, prim_tests;
     unsigned long long sh_node_visits, sh_prim_tests;  /* any-hit traversal (count_traversal) */

@@ -111,29 +111,47 @@ __device__ __forceinline__ void block_append(const bool* pred, int32_t* const* c
     __syncthreads();  /* cnt and base are reused by the next call */
 }
 
-/* Dynamic work distribution of the persistent traversal kernels: a wave
- * claims queue positions from a global counter in chunks of 64 (one atomic
- * per chunk) and hands them out, in lane order, to the lanes that have
- * finished their previous ray.  Call in wave-uniform control flow.  `base`
- * and `used` are the wave's chunk (uniform across the wave); returns the
- * position for lanes with `want` (>= the queue length once it is drained). */
+/* Dynamic work distribution of the persistent traversal kernels: the queue
+ * is cut into RT_XCDS segments of consecutive positions, and a wave claims
+ * positions from the segment of its block's XCD, in chunks of 64 (one atomic
+ * per chunk), handing them out in lane order to the lanes that have finished
+ * their previous ray.  Consecutive queue positions hold rays of neighbouring
+ * pixels, so each XCD's L2 caches the part of the scene its own screen
+ * region needs.  A wave that finds its segment drained moves on to the next
+ * one; once it has found all of them drained it hands out positions >= n.
+ * Call in wave-uniform control flow.  The chunk and segment are uniform
+ * across the wave; returns the position for lanes with `want`. */
 struct WaveFeed {
-    int base = 0, used = 64;
+    int base = 0, used = 0, lim = 0;  /* the chunk: positions base+used .. base+lim-1 are unclaimed */
+    int seg, left = RT_XCDS;          /* the segment claimed from; segments not yet found drained */
+    __device__ WaveFeed() : seg(blockIdx.x % RT_XCDS) {}
 };
-__device__ __forceinline__ int feed_take(WaveFeed& f, bool want, int32_t* counter)
+__device__ __forceinline__ int feed_seg_start(int n, int s) { return (int)(((long long)n * s) / RT_XCDS); }
+__device__ __forceinline__ int feed_take(WaveFeed& f, bool want, int32_t* counters, int n)
 {
     const int lane = __lane_id();
     const unsigned long long below = (1ull << lane) - 1ull;
     int idx = 0x7fffffff;
     unsigned long long m = __ballot(want);
     while (m) {
-        if (f.used == 64) {
-            int b = 0;
-            if (lane == 0) b = atomicAdd(counter, 64);
-            f.base = __builtin_amdgcn_readfirstlane(b);
-            f.used = 0;
+        if (f.used == f.lim) {
+            if (f.left == 0) {
+                f.base = n, f.used = 0, f.lim = 64;  /* all drained */
+            } else {
+                const int lo = feed_seg_start(n, f.seg), len = feed_seg_start(n, f.seg + 1) - lo;
+                int b = 0;
+                if (lane == 0) b = atomicAdd(counters + f.seg * RT_CLAIM_STRIDE, 64);
+                b = __builtin_amdgcn_readfirstlane(b);
+                if (b < len) {
+                    f.base = lo + b, f.used = 0, f.lim = min(64, len - b);
+                } else {
+                    f.seg = f.seg + 1 == RT_XCDS ? 0 : f.seg + 1;
+                    f.left -= 1;
+                    continue;
+                }
+            }
         }
-        int avail = 64 - f.used;
+        int avail = f.lim - f.used;
         int rank = __popcll(m & below);
         bool mine = ((m >> lane) & 1ull) && rank < avail;
         if (mine) idx = f.base + f.used + rank;
@@ -141,6 +159,10 @@ __device__ __forceinline__ int feed_take(WaveFeed& f, bool want, int32_t* counte
         m &= ~__ballot(mine);
     }
     return idx;
+}
+__device__ __forceinline__ void clear_claims(DevCounters* c)
+{
+    for (int x = 0; x < RT_XCDS; ++x) c->ext_next[x * RT_CLAIM_STRIDE] = c->sh_next[x * RT_CLAIM_STRIDE] = 0;
 }
 
 /* ===========================================================================
@@ -686,8 +708,7 @@ __global__ __launch_bounds__(PT_BLOCK) void k_generate(PathState ps, const DevVi
         ps.cnt->q_count[0] = pass.slots;
         ps.cnt->q_count[1] = 0;
         ps.cnt->shadow_count = 0;
-        ps.cnt->ext_next = 0;
-        ps.cnt->sh_next = 0;
+        clear_claims(ps.cnt);
         ps.cnt->camera_rays += (unsigned long long)pass.slots;
     }
     for (int slot = blockIdx.x * PT_BLOCK + threadIdx.x; slot < pass.slots; slot += gridDim.x * PT_BLOCK) {
@@ -747,8 +768,7 @@ __global__ __launch_bounds__(PT_BLOCK) void k_generate_pss(PathState ps, const D
         ps.cnt->q_count[0] = n;
         ps.cnt->q_count[1] = 0;
         ps.cnt->shadow_count = 0;
-        ps.cnt->ext_next = 0;
-        ps.cnt->sh_next = 0;
+        clear_claims(ps.cnt);
         ps.cnt->camera_rays += (unsigned long long)n;
     }
     for (int slot = blockIdx.x * PT_BLOCK + threadIdx.x; slot < n; slot += gridDim.x * PT_BLOCK) {
@@ -801,8 +821,7 @@ __global__ __launch_bounds__(PT_BLOCK) void k_generate_rays(PathState ps, const 
         ps.cnt->q_count[0] = n;
         ps.cnt->q_count[1] = 0;
         ps.cnt->shadow_count = 0;
-        ps.cnt->ext_next = 0;
-        ps.cnt->sh_next = 0;
+        clear_claims(ps.cnt);
         ps.cnt->camera_rays += (unsigned long long)n;
     }
     for (int slot = blockIdx.x * PT_BLOCK + threadIdx.x; slot < n; slot += gridDim.x * PT_BLOCK) {
@@ -887,7 +906,7 @@ __global__ __launch_bounds__(PT_BLOCK) RT_TRAV_OCCUPANCY void k_extend(DevScene 
     while (true) {
         bool want = !live && more;
         if (__ballot(want)) {
-            int i = feed_take(feed, want, &ps.cnt->ext_next);
+            int i = feed_take(feed, want, ps.cnt->ext_next, n);
             if (want) {
                 if (i < n) {
                     p = q[i];
@@ -1451,7 +1470,7 @@ __global__ __launch_bounds__(PT_BLOCK) RT_TRAV_OCCUPANCY void k_shadow(DevScene 
     while (true) {
         bool want = !live && more;
         if (__ballot(want)) {
-            int i = feed_take(feed, want, &ps.cnt->sh_next);
+            int i = feed_take(feed, want, ps.cnt->sh_next, n);
             if (want) {
                 if (i < n) {
                     float4 a = ps.sh_o[i], b = ps.sh_d[i];
@@ -1515,8 +1534,7 @@ __global__ void k_advance(PathState ps, int qsel, int depth)
     c->shadow_rays += (unsigned long long)c->shadow_count;
     c->q_count[qsel] = 0;
     c->shadow_count = 0;
-    c->ext_next = 0;
-    c->sh_next = 0;
+    clear_claims(c);
 }
 hipError_t launch_advance(const PathState& ps, int qsel, int depth, hipStream_t st)
 {
@@ -1625,7 +1643,7 @@ __global__ void k_reset(PathState ps)
     c->q_count[0] = c->q_count[1] = 0;
     c->shadow_count = 0;
     c->pass = 0;
-    c->ext_next = c->sh_next = 0;
+    clear_claims(c);
     c->camera_rays = c->extension_rays = c->shadow_rays = 0ull;
     c->node_visits = c->prim_tests = 0ull;
     c->sh_node_visits = c->sh_prim_tests = 0ull;
