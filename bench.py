@@ -79,6 +79,14 @@ def parse():
     return ap.parse_args()
 
 
+def count_depth(ctx, max_depth):
+    """Depth of a counting render that traces the bounces the per-bounce
+    closest-hit and any-hit launches of a max_depth render trace (those
+    before finish_depth; the finishing launch traces the rest)."""
+    fin = ctx.launch_config()["finish_depth"]
+    return min(max_depth, fin) if fin > 0 else max_depth
+
+
 def rel_mse(img, ref, eps=1e-2):
     """mean over pixels and channels of (I - R)^2 / (R^2 + eps)."""
     return float(((img - ref) ** 2 / (ref ** 2 + eps)).mean())
@@ -333,7 +341,7 @@ def sah_tree(ctx, arrays, views, params, film, args):
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     ctx.set_launch_config(count_traversal=1)
-    ctx.render_device(views, [film.data_ptr()], seed=0, **params)
+    ctx.render_device(views, [film.data_ptr()], seed=0, **dict(params, max_depth=count_depth(ctx, params["max_depth"])))
     cs = ctx.stats()
     ctx.set_launch_config(count_traversal=0)
     ext_rays = cs["camera_rays"] + cs["extension_rays"]
@@ -825,6 +833,8 @@ def main():
     ext_ms = 0.0
     sh_ms = 0.0
     ext_launches = 0
+    fin_ms = 0.0
+    fin_rays = 0
     t0 = time.perf_counter()
     for i in range(args.steps):
         ctx.render_device(views, [film.data_ptr()], seed=i, **params)
@@ -833,18 +843,28 @@ def main():
         ext_ms += s["extend_ms"]
         sh_ms += s["shadow_ms"]
         ext_launches += s["extend_launches"]
+        fin_ms += s["finish_ms"]
+        fin_rays += s["finish_rays"] + s["finish_shadow_rays"]
         log(f"step {i}: {s['render_ms']:.1f} ms, {(s['camera_rays'] + s['extension_rays'] + s['shadow_rays']) / 1e6:.1f} Mrays")
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     last = ctx.stats()
 
     # algorithmic bytes of the closest-hit launches: counted in a separate,
-    # untimed render of the same workload (same seeds as step 0)
+    # untimed render of the same workload (same seeds as step 0).  The timed
+    # per-bounce launches trace the bounces before finish_depth, the
+    # finishing launch the rest; paths are the same whatever the render's
+    # depth, so a counting render cut at finish_depth counts exactly the
+    # bounces those launches trace.
+    cut = count_depth(ctx, params["max_depth"])
     ctx.set_launch_config(count_traversal=1)
-    ctx.render_device(views, [film.data_ptr()], seed=0, **params)
+    ctx.render_device(views, [film.data_ptr()], seed=0, **dict(params, max_depth=cut))
     cs = ctx.stats()
     ctx.set_launch_config(count_traversal=0)
     ext_rays = cs["camera_rays"] + cs["extension_rays"]
+    timed_ext = last["camera_rays"] + last["extension_rays"] - last["finish_rays"]
+    if ext_rays != timed_ext:
+        log(f"warning: the counting render traced {ext_rays} closest-hit rays, the timed launches {timed_ext}")
     mem_nodes = cs["node_visits"] - cs["node_visits_lds"]
     alg_bytes = ext_rays * RAY_BYTES + mem_nodes * NODE_BYTES + cs["prim_tests"] * PRIM_BYTES
     per_launch_bytes = alg_bytes / max(1, cs["extend_launches"])
@@ -978,6 +998,12 @@ def main():
             "lds_nodes_per_ray": round(cs["shadow_node_visits_lds"] / max(1, cs["shadow_rays"]), 2),
             "prims_per_ray": round(cs["shadow_prim_tests"] / max(1, cs["shadow_rays"]), 2),
             "l2_hit_rate": sh_l2,
+        },
+        "finish_launch": {
+            "kernel": "k_finish (the bounces from finish_depth on, a lane per path)",
+            "finish_depth": ctx.launch_config()["finish_depth"],
+            "ms_per_step": round(fin_ms / args.steps, 3),
+            "rays_share": round(fin_rays / max(1, rays), 4),
         },
         "cpu_baseline": cpu,
         "batched_views": batched,

@@ -106,13 +106,17 @@ class RtStats(ctypes.Structure):
         ("shadow_prim_tests", ctypes.c_int64),
         ("node_visits_lds", ctypes.c_int64),
         ("shadow_node_visits_lds", ctypes.c_int64),
+        ("finish_ms", ctypes.c_double),
+        ("finish_rays", ctypes.c_int64),
+        ("finish_shadow_rays", ctypes.c_int64),
     ]
 
 
 class RtLaunchConfig(ctypes.Structure):
     _fields_ = [("path_slots", ctypes.c_int32), ("block", ctypes.c_int32), ("blocks_per_cu", ctypes.c_int32),
                 ("use_graph", ctypes.c_int32),
-                ("count_traversal", ctypes.c_int32), ("guide_epochs", ctypes.c_int32)]
+                ("count_traversal", ctypes.c_int32), ("guide_epochs", ctypes.c_int32),
+                ("finish_depth", ctypes.c_int32)]
 
 
 class RtMltParams(ctypes.Structure):

@@ -64,7 +64,7 @@ struct rt_ctx {
      * 32M path slots per pass (see render_impl), 8 persistent blocks per CU
      * (all resident: the traversal kernels claim work dynamically, so extra
      * blocks only add launch overhead) */
-    rt_launch_config cfg{0, 256, 8, 0, 0, 1};
+    rt_launch_config cfg{0, 256, 8, 0, 0, 1, RT_FINISH_DEPTH};
 
     // guiding network
     GuideState guide{};
@@ -309,6 +309,7 @@ int rt_set_launch_config(rt_ctx* c, const rt_launch_config* cfg)
     c->cfg.use_graph = cfg->use_graph;
     c->cfg.count_traversal = cfg->count_traversal;
     if (cfg->guide_epochs > 0) c->cfg.guide_epochs = std::min(cfg->guide_epochs, 64);
+    if (cfg->finish_depth) c->cfg.finish_depth = cfg->finish_depth;
     return RT_OK;
 }
 
@@ -1447,7 +1448,8 @@ static int render_impl(rt_ctx* c, const rt_view* views, int32_t nv, const rt_par
     const int steps0 = c->guide_step;
     HIP_TRY(c, hipEventRecord(c->ev_a, c->stream));
     size_t ev = 0;
-    std::vector<size_t> bounce_ev, train_ev;  /* index of the first event of each group */
+    std::vector<size_t> bounce_ev, train_ev, finish_ev;  /* index of the first event of each group */
+    const int finish = !guided && !c->cfg.count_traversal && c->cfg.finish_depth > 0 ? c->cfg.finish_depth : depth;
     bool cancelled = false;
     /* one pass: generate, max_depth x (extend, [inference], shade, shadow,
      * advance), accumulate, [training steps], pass end */
@@ -1461,6 +1463,16 @@ static int render_impl(rt_ctx* c, const rt_view* views, int32_t nv, const rt_par
         QCHK(launch_generate(c->ps, d_views, d_tiles, d_passes, pp, grid, c->stream));
         for (int dd = 0; dd < depth; ++dd) {
             int q = dd & 1;
+            if (dd == finish) {
+                /* the remaining bounces in one launch; it counts its own
+                 * rays, so the advance only clears the queue */
+                if (timing) finish_ev.push_back(ev);
+                QCHK(mark());
+                QCHK(launch_finish(c->scene, c->ps, pp, dd, q, grid, c->stream));
+                QCHK(mark());
+                QCHK(launch_advance(c->ps, q, 0, c->stream));
+                break;
+            }
             if (timing) bounce_ev.push_back(ev);
             QCHK(mark());
             QCHK(launch_extend(c->scene, c->ps, q, c->cfg.count_traversal, grid, c->stream));
@@ -1547,6 +1559,9 @@ static int render_impl(rt_ctx* c, const rt_view* views, int32_t nv, const rt_par
         sh += span(i + 3, i + 4);
     }
     for (size_t i : train_ev) trn += span(i, i + 1);
+    double fin = 0;
+    for (size_t i : finish_ev) fin += span(i, i + 1);
+    c->stats.finish_ms = fin;
     c->stats.extend_ms = ext;
     c->stats.shadow_ms = sh;
     c->stats.extend_launches = (int64_t)bounce_ev.size();
@@ -1572,6 +1587,8 @@ static int render_impl(rt_ctx* c, const rt_view* views, int32_t nv, const rt_par
     c->stats.shadow_prim_tests = (int64_t)hc.sh_prim_tests;
     c->stats.node_visits_lds = (int64_t)hc.node_visits_lds;
     c->stats.shadow_node_visits_lds = (int64_t)hc.sh_node_visits_lds;
+    c->stats.finish_rays = (int64_t)hc.finish_rays;
+    c->stats.finish_shadow_rays = (int64_t)hc.finish_shadow_rays;
     c->stats.paths_per_pass = (int32_t)P;
     c->stats.passes = (int32_t)passes.size();
     if (!device_films) {
@@ -1589,6 +1606,32 @@ static int render_impl(rt_ctx* c, const rt_view* views, int32_t nv, const rt_par
  * ------------------------------------------------------------------------- */
 namespace {
 
+/* The bounces of the n paths in queue 0 (compat or unbiased estimator):
+ * extend, shade, shadow and advance launches per bounce, then one finishing
+ * launch - from cfg.finish_depth on, or from bounce 1 for a batch of at
+ * most RT_FINISH_PATHS_PER_CU paths per CU, whose bounces after the camera
+ * rays are all short launches (the Metropolis renderer's 1M-path batches
+ * on the 1M-triangle mesh: 612 to 823 Mrays/s; ERPT's 4M-path batches gain
+ * 1% from bounce 4 and lose from bounce 1). */
+hipError_t trace_bounces(rt_ctx* c, const PtParams& pp, int64_t n, int grid)
+{
+    const bool small = n <= (int64_t)c->n_cu * RT_FINISH_PATHS_PER_CU;
+    const int finish = c->cfg.finish_depth <= 0 ? pp.max_depth : small ? 1 : c->cfg.finish_depth;
+    hipError_t e;
+    for (int dd = 0; dd < pp.max_depth; ++dd) {
+        const int q = dd & 1;
+        if (dd == finish) {
+            if ((e = launch_finish(c->scene, c->ps, pp, dd, q, grid, c->stream))) return e;
+            return launch_advance(c->ps, q, 0, c->stream);
+        }
+        if ((e = launch_extend(c->scene, c->ps, q, 0, grid, c->stream))) return e;
+        if ((e = launch_shade(c->scene, c->ps, pp, dd, q, grid, c->stream))) return e;
+        if ((e = launch_shadow(c->scene, c->ps, pp, 0, grid, c->stream))) return e;
+        if ((e = launch_advance(c->ps, q, dd, c->stream))) return e;
+    }
+    return hipSuccess;
+}
+
 /* Trace the n paths whose primary samples are in c->mlt_prop (stride cap):
  * the compat path kernels, depth bounces, with the given length rule. */
 hipError_t mlt_trace(rt_ctx* c, int n, int cap, int max_depth, int rr_start, float rr_prob, uint32_t seed,
@@ -1599,14 +1642,7 @@ hipError_t mlt_trace(rt_ctx* c, int n, int cap, int max_depth, int rr_start, flo
     hipError_t e;
     if ((e = launch_generate_pss(c->ps, (const DevView*)c->mlt_view.p, n, pp, pixel_units, grid, c->stream)))
         return e;
-    for (int dd = 0; dd < max_depth; ++dd) {
-        int q = dd & 1;
-        if ((e = launch_extend(c->scene, c->ps, q, 0, grid, c->stream))) return e;
-        if ((e = launch_shade(c->scene, c->ps, pp, dd, q, grid, c->stream))) return e;
-        if ((e = launch_shadow(c->scene, c->ps, pp, 0, grid, c->stream))) return e;
-        if ((e = launch_advance(c->ps, q, dd, c->stream))) return e;
-    }
-    return hipSuccess;
+    return trace_bounces(c, pp, n, grid);
 }
 
 /* Path slots, proposal and state buffers for cap chains of G groups. */
@@ -2049,6 +2085,9 @@ int render_mlt_impl(rt_ctx* c, const rt_camera* cam, const rt_mlt_params* p, flo
     c->stats.extend_ms = 0;
     c->stats.shadow_ms = 0;
     c->stats.extend_launches = 0;
+    c->stats.finish_ms = 0;
+    c->stats.finish_rays = (int64_t)hc.finish_rays;
+    c->stats.finish_shadow_rays = (int64_t)hc.finish_shadow_rays;
     if (out) *out = res;
     return RT_OK;
 }
@@ -2119,13 +2158,7 @@ static int integrate_rays_impl(rt_ctx* c, const float* o, const float* d, int64_
         const int m = (int)std::min<int64_t>(cap, n - base);
         HIP_TRY(c, launch_generate_rays(c->ps, (const float4*)o + base, (const float4*)d + base, m, pp,
                                         key0 + (uint32_t)base, grid, c->stream));
-        for (int dd = 0; dd < p->max_depth; ++dd) {
-            const int q = dd & 1;
-            HIP_TRY(c, launch_extend(c->scene, c->ps, q, 0, grid, c->stream));
-            HIP_TRY(c, launch_shade(c->scene, c->ps, pp, dd, q, grid, c->stream));
-            HIP_TRY(c, launch_shadow(c->scene, c->ps, pp, 0, grid, c->stream));
-            HIP_TRY(c, launch_advance(c->ps, q, dd, c->stream));
-        }
+        HIP_TRY(c, trace_bounces(c, pp, m, grid));
         HIP_TRY(c, launch_ray_values(c->ps, m, rgb + 3 * base, grid, c->stream));
         HIP_TRY(c, flight.unit_done());
     }
@@ -2139,6 +2172,8 @@ static int integrate_rays_impl(rt_ctx* c, const float* o, const float* d, int64_
     c->stats.camera_rays = (int64_t)hc.camera_rays;
     c->stats.extension_rays = (int64_t)hc.extension_rays;
     c->stats.shadow_rays = (int64_t)hc.shadow_rays;
+    c->stats.finish_rays = (int64_t)hc.finish_rays;
+    c->stats.finish_shadow_rays = (int64_t)hc.finish_shadow_rays;
     return RT_OK;
 }
 

@@ -78,6 +78,21 @@ struct DevPass {
 #define RT_XCDS 8
 #define RT_CLAIM_STRIDE 32
 
+/* Default bounce from which rt_render traces a pass's remaining paths to
+ * their ends in one launch (k_finish; rt_launch_config.finish_depth).  A
+ * 1024^2, 64 spp render of a 1M-triangle mesh (max depth 8, roulette from
+ * bounce 3) takes 83.9 ms from bounce 4 (the launch 4.0 ms, where the
+ * separate launches of those bounces took 5.8 ms), 85.5 ms from bounce 3
+ * and 86.2 ms with every bounce in separate launches. */
+#ifndef RT_FINISH_DEPTH
+#define RT_FINISH_DEPTH 4
+#endif
+/* Batches of the Markov-chain renderers and of rt_integrate_rays with at
+ * most this many paths per CU finish from bounce 1. */
+#ifndef RT_FINISH_PATHS_PER_CU
+#define RT_FINISH_PATHS_PER_CU 4096
+#endif
+
 struct DevCounters {
     int32_t q_count[2];      /* path queue lengths (ping-pong) */
     int32_t shadow_count;
@@ -88,6 +103,7 @@ struct DevCounters {
     unsigned long long node_visits, prim_tests;
     unsigned long long sh_node_visits, sh_prim_tests;  /* any-hit traversal (count_traversal) */
     unsigned long long node_visits_lds, sh_node_visits_lds; /* of those, visits read from the LDS copy */
+    unsigned long long finish_rays, finish_shadow_rays; /* of the extension and shadow rays, those k_finish traced */
 };
 
 /* Path state in HBM (structure of arrays, capacity = path slots). */
@@ -238,6 +254,11 @@ hipError_t launch_extend(const HostDevScene& s, const PathState& ps, int qsel, i
 hipError_t launch_shade(const HostDevScene& s, const PathState& ps, PtParams p, int depth, int qsel, int grid,
                         hipStream_t st);
 hipError_t launch_shadow(const HostDevScene& s, const PathState& ps, PtParams p, int count, int grid, hipStream_t st);
+/* Compat and unbiased estimators: every path of queue qsel (at bounce
+ * `depth`) traced, shaded and shadow-tested to its end in one launch, a lane
+ * per path; counts its rays after bounce 0 as extension and shadow rays. */
+hipError_t launch_finish(const HostDevScene& s, const PathState& ps, PtParams p, int depth, int qsel, int grid,
+                         hipStream_t st);
 /* lanes of a traversal launch of `grid` blocks (the stack overflow array
  * holds RT_STACK_OVF entries for each) */
 int64_t trav_lanes(int grid);

@@ -59,6 +59,19 @@ static_assert(sizeof(HostDevScene) == sizeof(DevScene), "scene view layouts diff
 #else
 #define RT_SHADE_GUIDED_OCCUPANCY
 #endif
+/* Register budget of k_finish (waves per SIMD; 0 = the compiler's choice).
+ * Left alone it takes 184 VGPRs (2 waves), too few lanes for the paths
+ * alive at the fifth bounce of a 32M-path pass; 4 waves (128 VGPRs, 69
+ * spilled) finish them in 4.0 instead of 4.2 ms per render, 5 and 6 waves
+ * spill more and gain nothing. */
+#ifndef RT_FINISH_WAVES
+#define RT_FINISH_WAVES 4
+#endif
+#if RT_FINISH_WAVES > 0
+#define RT_FINISH_OCCUPANCY __attribute__((amdgpu_waves_per_eu(RT_FINISH_WAVES, 8)))
+#else
+#define RT_FINISH_OCCUPANCY
+#endif
 #if RT_SHADE_WAVES > 0
 #define RT_SHADE_OCCUPANCY __attribute__((amdgpu_waves_per_eu(RT_SHADE_WAVES, 8)))
 #else
@@ -1524,6 +1537,135 @@ hipError_t launch_shadow(const HostDevScene& s, const PathState& ps, PtParams p,
     return hipGetLastError();
 }
 
+/* ===========================================================================
+ * finish: the last bounces of the compat and unbiased estimators in one launch
+ * ========================================================================= */
+/* Late bounces hold few paths (1.5% of the camera rays from the fifth
+ * bounce on in a 1024^2, 64 spp render of a 1M-triangle mesh), and a launch
+ * of the separate kernels costs about the latency chain of its slowest ray
+ * whatever its size, 300-450 us each there.  Here a lane takes a path from the queue and runs it to its end:
+ * closest hit, shade (the same per-path functions as k_shade, so the path's
+ * state changes in the same order), its shadow ray, the next bounce; a lane
+ * whose path has ended takes the next one (WaveFeed).  The film is the one
+ * the separate kernels give.  Rays are counted as extension and shadow rays
+ * and, separately, as the ones traced here. */
+__device__ __forceinline__ unsigned wave_sum(unsigned v)
+{
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+template <int MODE, bool OBJ>
+__global__ __launch_bounds__(PT_BLOCK) RT_FINISH_OCCUPANCY void k_finish(DevScene S, PathState ps, PtParams P,
+                                                                         int depth0, int qsel)
+{
+    __shared__ int32_t stack[RT_STACK_LDS * PT_BLOCK];
+    __shared__ float4 top_nodes[RT_TOP_NODES * (RT_NODE_FLOATS / 4)];
+    const char* top = stage_top_nodes(S, top_nodes);
+    LaneStack my(stack + threadIdx.x, PT_BLOCK, S.stack_ovf);
+    const int n = ps.cnt->q_count[qsel];
+    const int32_t* q = ps.queue[qsel];
+    int cn = 0, cp = 0, p = 0, depth = 0;
+    bool live = false, more = true, in_shadow = false, cont = false;
+    float4 sc = make_float4(0.f, 0.f, 0.f, 0.f);
+    unsigned n_ext = 0, n_sh = 0;
+    WaveFeed feed;
+    Trav T;
+    auto start_extend = [&]() {
+        const float4 a = ps.ray_o[p], b = ps.ray_d[p];
+        trav_init(S, T, xyz(a), xyz(b), a.w, b.w);
+        if (OBJ && a.w <= b.w) obj_closest(S, T, b.w);
+        if (depth > 0) ++n_ext;  /* bounce 0 traces the camera rays, counted by k_generate */
+    };
+    while (true) {
+        bool want = !live && more;
+        if (__ballot(want)) {
+            int i = feed_take(feed, want, ps.cnt->ext_next, n);
+            if (want) {
+                if (i < n) {
+                    p = q[i];
+                    depth = depth0;
+                    in_shadow = false;
+                    start_extend();
+                    live = true;
+                } else {
+                    more = false;
+                }
+            }
+        }
+        if (!__ballot(live)) break;  /* queue drained and every lane idle */
+        const bool done = live && (T.cur < 0 || (in_shadow ? trav_step<true, false>(S, T, my, cn, cp, top)
+                                                           : trav_step<false, false>(S, T, my, cn, cp, top)));
+        if (done && in_shadow) {
+            if (T.g < 0) {  /* unoccluded: as k_shadow */
+                if (MODE == RT_MODE_COMPAT) {
+                    ps.pend_c[p] = make_float4(sc.x, sc.y, sc.z, 0.f);
+                } else {
+                    const float4 r = ps.radiance[p];
+                    ps.radiance[p] = make_float4(r.x + sc.x, r.y + sc.y, r.z + sc.z, 0.f);
+                }
+            }
+            in_shadow = false;
+            if (cont) {
+                ++depth;
+                start_extend();
+            } else {
+                live = false;
+            }
+        } else if (done) {
+            ps.hit[p] = make_float4(T.g >= 0 ? T.best : RT_INF, __int_as_float(T.g), T.u, T.v);
+            bool shadow = false;
+            float4 so, sd;
+            if (MODE == RT_MODE_COMPAT)
+                shade_compat(S, ps, P, depth, p, cont, shadow, so, sd, sc);
+            else
+                shade_unbiased(S, ps, P, depth, p, cont, shadow, so, sd, sc);
+            if (shadow) {
+                in_shadow = true;
+                ++n_sh;
+                trav_init(S, T, xyz(so), xyz(sd), so.w, sd.w);
+                if (OBJ && so.w <= sd.w && obj_any(S, T.o, T.d, so.w, sd.w)) {
+                    T.g = 0;  /* occluded by an object: no traversal */
+                    T.cur = -1;
+                }
+            } else if (cont) {
+                ++depth;
+                start_extend();
+            } else {
+                live = false;
+            }
+        }
+    }
+    n_ext = wave_sum(n_ext);
+    n_sh = wave_sum(n_sh);
+    if (__lane_id() == 0 && (n_ext | n_sh)) {
+        atomicAdd(&ps.cnt->extension_rays, (unsigned long long)n_ext);
+        atomicAdd(&ps.cnt->finish_rays, (unsigned long long)n_ext);
+        atomicAdd(&ps.cnt->shadow_rays, (unsigned long long)n_sh);
+        atomicAdd(&ps.cnt->finish_shadow_rays, (unsigned long long)n_sh);
+    }
+}
+
+hipError_t launch_finish(const HostDevScene& s, const PathState& ps, PtParams p, int depth, int qsel, int grid,
+                         hipStream_t st)
+{
+    const bool obj = s.n_obj > 0;
+    const bool compat = p.mode == RT_MODE_COMPAT;
+    if (compat && obj)
+        hipLaunchKernelGGL((k_finish<RT_MODE_COMPAT, true>), dim3(grid), dim3(PT_BLOCK), 0, st, as_dev(s), ps, p,
+                           depth, qsel);
+    else if (compat)
+        hipLaunchKernelGGL((k_finish<RT_MODE_COMPAT, false>), dim3(grid), dim3(PT_BLOCK), 0, st, as_dev(s), ps, p,
+                           depth, qsel);
+    else if (obj)
+        hipLaunchKernelGGL((k_finish<RT_MODE_UNBIASED, true>), dim3(grid), dim3(PT_BLOCK), 0, st, as_dev(s), ps, p,
+                           depth, qsel);
+    else
+        hipLaunchKernelGGL((k_finish<RT_MODE_UNBIASED, false>), dim3(grid), dim3(PT_BLOCK), 0, st, as_dev(s), ps, p,
+                           depth, qsel);
+    return hipGetLastError();
+}
+
 /* End of a bounce: count the rays the bounce traced (its queue length and
  * its shadow-ray count; the queue of bounce 0 holds the camera rays, counted
  * by k_generate), then clear the consumed queue and the work counters. */
@@ -1648,6 +1790,7 @@ __global__ void k_reset(PathState ps)
     c->node_visits = c->prim_tests = 0ull;
     c->sh_node_visits = c->sh_prim_tests = 0ull;
     c->node_visits_lds = c->sh_node_visits_lds = 0ull;
+    c->finish_rays = c->finish_shadow_rays = 0ull;
 }
 hipError_t launch_reset_counters(const PathState& ps, hipStream_t st)
 {
