@@ -79,12 +79,14 @@ def parse():
     return ap.parse_args()
 
 
-def count_depth(ctx, max_depth):
+def count_depth(stats, max_depth):
     """Depth of a counting render that traces the bounces the per-bounce
-    closest-hit and any-hit launches of a max_depth render trace (those
-    before finish_depth; the finishing launch traces the rest)."""
-    fin = ctx.launch_config()["finish_depth"]
-    return min(max_depth, fin) if fin > 0 else max_depth
+    closest-hit and any-hit launches of a timed render (its stats) traced:
+    those before the bounce its passes moved to the finishing launch from
+    (the counting render runs every bounce in per-bounce launches; paths are
+    the same whatever the render's depth)."""
+    fb = stats["finish_bounce"]
+    return min(max_depth, fb) if fb > 0 else max_depth
 
 
 def rel_mse(img, ref, eps=1e-2):
@@ -334,6 +336,8 @@ def sah_tree(ctx, arrays, views, params, film, args):
     for i in range(args.steps):
         ctx.render_device(views, [film.data_ptr()], seed=i, **params)
         s = ctx.stats()
+        if i == 0:
+            first = s
         rays += s["camera_rays"] + s["extension_rays"] + s["shadow_rays"]
         ext_ms += s["extend_ms"]
         sh_ms += s["shadow_ms"]
@@ -341,7 +345,7 @@ def sah_tree(ctx, arrays, views, params, film, args):
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     ctx.set_launch_config(count_traversal=1)
-    ctx.render_device(views, [film.data_ptr()], seed=0, **dict(params, max_depth=count_depth(ctx, params["max_depth"])))
+    ctx.render_device(views, [film.data_ptr()], seed=0, **dict(params, max_depth=count_depth(first, params["max_depth"])))
     cs = ctx.stats()
     ctx.set_launch_config(count_traversal=0)
     ext_rays = cs["camera_rays"] + cs["extension_rays"]
@@ -845,24 +849,25 @@ def main():
         ext_launches += s["extend_launches"]
         fin_ms += s["finish_ms"]
         fin_rays += s["finish_rays"] + s["finish_shadow_rays"]
+        if i == 0:
+            first = s
         log(f"step {i}: {s['render_ms']:.1f} ms, {(s['camera_rays'] + s['extension_rays'] + s['shadow_rays']) / 1e6:.1f} Mrays")
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     last = ctx.stats()
 
     # algorithmic bytes of the closest-hit launches: counted in a separate,
-    # untimed render of the same workload (same seeds as step 0).  The timed
-    # per-bounce launches trace the bounces before finish_depth, the
-    # finishing launch the rest; paths are the same whatever the render's
-    # depth, so a counting render cut at finish_depth counts exactly the
-    # bounces those launches trace.
-    cut = count_depth(ctx, params["max_depth"])
+    # untimed render of the same workload (same seeds as step 0), cut at the
+    # bounce from which the timed render's passes went to the finishing
+    # launch, so it counts exactly the bounces the per-bounce launches traced
+    finish_depth = ctx.launch_config()["finish_depth"]
+    cut = count_depth(first, params["max_depth"])
     ctx.set_launch_config(count_traversal=1)
     ctx.render_device(views, [film.data_ptr()], seed=0, **dict(params, max_depth=cut))
     cs = ctx.stats()
     ctx.set_launch_config(count_traversal=0)
     ext_rays = cs["camera_rays"] + cs["extension_rays"]
-    timed_ext = last["camera_rays"] + last["extension_rays"] - last["finish_rays"]
+    timed_ext = first["camera_rays"] + first["extension_rays"] - first["finish_rays"]  # step 0: seed 0 too
     if ext_rays != timed_ext:
         log(f"warning: the counting render traced {ext_rays} closest-hit rays, the timed launches {timed_ext}")
     mem_nodes = cs["node_visits"] - cs["node_visits_lds"]
@@ -1001,7 +1006,8 @@ def main():
         },
         "finish_launch": {
             "kernel": "k_finish (the bounces from finish_depth on, a lane per path)",
-            "finish_depth": ctx.launch_config()["finish_depth"],
+            "finish_depth": finish_depth,
+            "from_bounce": first["finish_bounce"],
             "ms_per_step": round(fin_ms / args.steps, 3),
             "rays_share": round(fin_rays / max(1, rays), 4),
         },

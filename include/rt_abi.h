@@ -59,7 +59,7 @@ typedef struct rt_stats {
     double render_ms;         /* wall time of the last rt_render, GPU events */
     double extend_ms;         /* summed duration of closest-hit kernel launches */
     double shadow_ms;         /* summed duration of any-hit kernel launches */
-    int64_t extend_launches;  /* number of closest-hit kernel launches timed */
+    int64_t extend_launches;  /* number of per-bounce closest-hit launches that traced rays (timed renders) */
     int64_t bvh_nodes;        /* interior nodes of the uploaded BVH */
     int64_t bvh_prims;        /* primitive records (triangles, spheres, parallelograms) */
     int32_t bvh_depth;        /* depth of the uploaded BVH */
@@ -82,13 +82,15 @@ typedef struct rt_stats {
      * memory traffic; count_traversal) */
     int64_t node_visits_lds;
     int64_t shadow_node_visits_lds;
-    /* the bounces from rt_launch_config.finish_depth on, run in one launch
-     * per pass: its summed duration and, of extension_rays and shadow_rays,
-     * the rays it traced (the rest were traced by the per-bounce launches
-     * that extend_ms and shadow_ms time) */
-    double finish_ms;
+    /* the last bounces of a pass, run in one launch (rt_launch_config
+     * finish_depth, finish_paths): its summed duration and, of
+     * extension_rays and shadow_rays, the rays it traced (the rest were
+     * traced by the per-bounce launches that extend_ms and shadow_ms time) */
+    double finish_ms;         /* (extend_ms and shadow_ms include the per-bounce launches from
+                                 there on, which return at once) */
     int64_t finish_rays;
     int64_t finish_shadow_rays;
+    int64_t finish_bounce;    /* the bounce the last pass moved to it from, -1 if it did not */
 } rt_stats;
 
 /* Library-wide. */
@@ -202,15 +204,15 @@ typedef struct rt_launch_config {
                               times in rt_stats are then 0 */
     int32_t count_traversal; /* 1 = closest-hit and any-hit kernels also count nodes and primitives */
     int32_t guide_epochs;  /* GUIDED + training: passes over each render pass's records (0 = 1) */
-    int32_t finish_depth;  /* COMPAT and UNBIASED: the bounce from which a pass's remaining paths
-                              are traced, shaded and shadow-tested to their ends in one launch
-                              instead of one launch per kernel and bounce (late bounces hold few
-                              paths, and each launch costs about its slowest ray's latency);
-                              the image is the same either way.  The Markov-chain renderers
-                              and rt_integrate_rays finish batches of at most 4096 paths per
-                              CU from bounce 1.  0 leaves it unchanged, negative = every
-                              bounce in separate launches.  Not used with count_traversal
-                              or in GUIDED mode. */
+    int32_t finish_depth;  /* COMPAT and UNBIASED: a pass's remaining paths are traced, shaded
+                              and shadow-tested to their ends in one launch, instead of one
+                              launch per kernel and bounce, from the first bounce >=
+                              finish_depth whose queue holds at most finish_paths paths (late
+                              bounces hold few paths, and each launch costs about its slowest
+                              ray's latency); the image is the same either way.  Default 1;
+                              0 leaves it unchanged, negative = every bounce in separate
+                              launches.  Not used with count_traversal or in GUIDED mode. */
+    int32_t finish_paths;  /* that queue bound; default 4096 per CU, 0 leaves it unchanged */
 } rt_launch_config;
 int rt_set_launch_config(rt_ctx* ctx, const rt_launch_config* cfg);
 int rt_get_launch_config(rt_ctx* ctx, rt_launch_config* cfg);

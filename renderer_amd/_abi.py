@@ -109,6 +109,7 @@ class RtStats(ctypes.Structure):
         ("finish_ms", ctypes.c_double),
         ("finish_rays", ctypes.c_int64),
         ("finish_shadow_rays", ctypes.c_int64),
+        ("finish_bounce", ctypes.c_int64),
     ]
 
 
@@ -116,7 +117,7 @@ class RtLaunchConfig(ctypes.Structure):
     _fields_ = [("path_slots", ctypes.c_int32), ("block", ctypes.c_int32), ("blocks_per_cu", ctypes.c_int32),
                 ("use_graph", ctypes.c_int32),
                 ("count_traversal", ctypes.c_int32), ("guide_epochs", ctypes.c_int32),
-                ("finish_depth", ctypes.c_int32)]
+                ("finish_depth", ctypes.c_int32), ("finish_paths", ctypes.c_int32)]
 
 
 class RtMltParams(ctypes.Structure):

@@ -3,6 +3,7 @@
  * context and device memory, scene upload (host BVH build + flattening),
  * ray-batch entry points, and the wavefront render driver.
  */
+#include <climits>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -64,7 +65,7 @@ struct rt_ctx {
      * 32M path slots per pass (see render_impl), 8 persistent blocks per CU
      * (all resident: the traversal kernels claim work dynamically, so extra
      * blocks only add launch overhead) */
-    rt_launch_config cfg{0, 256, 8, 0, 0, 1, RT_FINISH_DEPTH};
+    rt_launch_config cfg{0, 256, 8, 0, 0, 1, RT_FINISH_DEPTH, 256 * RT_FINISH_PATHS_PER_CU};
 
     // guiding network
     GuideState guide{};
@@ -161,6 +162,14 @@ float lum_rgb(const float c[3]) { return (float)(.299 * c[0] + .587 * c[1] + .11
 
 int grid_for(rt_ctx* c) { return c->n_cu * std::max(1, c->cfg.blocks_per_cu); }
 
+/* When the compat and unbiased estimators move a pass's remaining paths to
+ * the finishing launch (cfg.finish_depth <= 0: never). */
+FinishRule finish_rule(const rt_ctx* c)
+{
+    if (c->cfg.finish_depth <= 0) return FinishRule{INT_MAX, 0};
+    return FinishRule{c->cfg.finish_depth, c->cfg.finish_paths};
+}
+
 /* Bounds the units of work (render passes, Markov-chain mutations) queued on
  * the context's stream ahead of the GPU: after each unit an event is
  * recorded, and before recording the unit `depth` places later the host
@@ -247,6 +256,7 @@ int rt_create(int device, rt_ctx** out)
     }
     int cu = 0;
     if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cu > 0) c->n_cu = cu;
+    c->cfg.finish_paths = c->n_cu * RT_FINISH_PATHS_PER_CU;
     /* the allocation-free trace's static scratch: its address on this device */
     if (noalloc_scratch_init(device) != hipSuccess) {
         hipStreamDestroy(c->stream);
@@ -310,6 +320,7 @@ int rt_set_launch_config(rt_ctx* c, const rt_launch_config* cfg)
     c->cfg.count_traversal = cfg->count_traversal;
     if (cfg->guide_epochs > 0) c->cfg.guide_epochs = std::min(cfg->guide_epochs, 64);
     if (cfg->finish_depth) c->cfg.finish_depth = cfg->finish_depth;
+    if (cfg->finish_paths > 0) c->cfg.finish_paths = cfg->finish_paths;
     return RT_OK;
 }
 
@@ -1433,9 +1444,11 @@ static int render_impl(rt_ctx* c, const rt_view* views, int32_t nv, const rt_par
      * is a per-step argument.  A graph carries no per-kernel timing events. */
     const bool graph = c->cfg.use_graph != 0 && !train;
     const bool timing = c->timing != 0 && !graph;
-    /* per bounce: before extend, after extend, after inference, after shade,
-     * after shadow; per pass with training: before and after the steps */
-    size_t need_ev = timing ? (size_t)passes.size() * (depth * 5 + (train ? 2 : 0)) : 0;
+    /* per bounce: before and after the finishing launch (from
+     * fr.min_depth on), before extend, after extend, after inference, after
+     * shade, after shadow; per pass with training: before and after the
+     * steps */
+    size_t need_ev = timing ? (size_t)passes.size() * (depth * 7 + (train ? 2 : 0)) : 0;
     while (c->ev_pool.size() < need_ev) {
         hipEvent_t e;
         HIP_TRY(c, hipEventCreate(&e));
@@ -1449,7 +1462,7 @@ static int render_impl(rt_ctx* c, const rt_view* views, int32_t nv, const rt_par
     HIP_TRY(c, hipEventRecord(c->ev_a, c->stream));
     size_t ev = 0;
     std::vector<size_t> bounce_ev, train_ev, finish_ev;  /* index of the first event of each group */
-    const int finish = !guided && !c->cfg.count_traversal && c->cfg.finish_depth > 0 ? c->cfg.finish_depth : depth;
+    const FinishRule fr = !guided && !c->cfg.count_traversal ? finish_rule(c) : FinishRule{INT_MAX, 0};
     bool cancelled = false;
     /* one pass: generate, max_depth x (extend, [inference], shade, shadow,
      * advance), accumulate, [training steps], pass end */
@@ -1459,23 +1472,25 @@ static int render_impl(rt_ctx* c, const rt_view* views, int32_t nv, const rt_par
     do {                                     \
         if ((e = (x)) != hipSuccess) return e; \
     } while (0)
-        auto mark = [&]() { return timing ? hipEventRecord(c->ev_pool[ev++], c->stream) : hipSuccess; };
+        auto mark = [&]() {
+            if (!timing) return hipSuccess;
+            if (ev >= c->ev_pool.size()) return hipErrorInvalidValue; /* the pool is sized for the marks above */
+            return hipEventRecord(c->ev_pool[ev++], c->stream);
+        };
         QCHK(launch_generate(c->ps, d_views, d_tiles, d_passes, pp, grid, c->stream));
         for (int dd = 0; dd < depth; ++dd) {
             int q = dd & 1;
-            if (dd == finish) {
-                /* the remaining bounces in one launch; it counts its own
-                 * rays, so the advance only clears the queue */
+            if (dd >= fr.min_depth) {
+                /* the remaining bounces in one launch once the queue is
+                 * short (decided on the device; a no-op otherwise) */
                 if (timing) finish_ev.push_back(ev);
                 QCHK(mark());
-                QCHK(launch_finish(c->scene, c->ps, pp, dd, q, grid, c->stream));
+                QCHK(launch_finish(c->scene, c->ps, pp, dd, q, fr, grid, c->stream));
                 QCHK(mark());
-                QCHK(launch_advance(c->ps, q, 0, c->stream));
-                break;
             }
             if (timing) bounce_ev.push_back(ev);
             QCHK(mark());
-            QCHK(launch_extend(c->scene, c->ps, q, c->cfg.count_traversal, grid, c->stream));
+            QCHK(launch_extend(c->scene, c->ps, q, dd, fr, c->cfg.count_traversal, grid, c->stream));
             QCHK(mark());
             if (guided) {
                 /* the last bounce neither samples the guide nor weights against it */
@@ -1486,12 +1501,12 @@ static int render_impl(rt_ctx* c, const rt_view* views, int32_t nv, const rt_par
                 QCHK(launch_shade_guided(c->scene, c->ps, c->guide, pp, dd, q, grid, c->stream));
             } else {
                 QCHK(mark());
-                QCHK(launch_shade(c->scene, c->ps, pp, dd, q, grid, c->stream));
+                QCHK(launch_shade(c->scene, c->ps, pp, dd, q, fr, grid, c->stream));
             }
             QCHK(mark());
-            QCHK(launch_shadow(c->scene, c->ps, pp, c->cfg.count_traversal, grid, c->stream));
+            QCHK(launch_shadow(c->scene, c->ps, pp, q, dd, fr, c->cfg.count_traversal, grid, c->stream));
             QCHK(mark());
-            QCHK(launch_advance(c->ps, q, dd, c->stream));
+            QCHK(launch_advance(c->ps, q, dd, fr, c->stream));
         }
         QCHK(launch_accumulate(c->ps, d_views, d_tiles, d_passes, grid, c->stream));
         if (train) {
@@ -1564,7 +1579,7 @@ static int render_impl(rt_ctx* c, const rt_view* views, int32_t nv, const rt_par
     c->stats.finish_ms = fin;
     c->stats.extend_ms = ext;
     c->stats.shadow_ms = sh;
-    c->stats.extend_launches = (int64_t)bounce_ev.size();
+    c->stats.extend_launches = 0;  /* set from the device counters below */
     c->stats.guide_infer_ms = guided ? inf : 0.0;
     c->stats.guide_train_ms = trn;
     c->stats.guide_train_steps = c->guide_step - steps0;
@@ -1589,6 +1604,8 @@ static int render_impl(rt_ctx* c, const rt_view* views, int32_t nv, const rt_par
     c->stats.shadow_node_visits_lds = (int64_t)hc.sh_node_visits_lds;
     c->stats.finish_rays = (int64_t)hc.finish_rays;
     c->stats.finish_shadow_rays = (int64_t)hc.finish_shadow_rays;
+    c->stats.finish_bounce = hc.finished;
+    c->stats.extend_launches = timing ? (int64_t)hc.busy_bounces : 0;
     c->stats.paths_per_pass = (int32_t)P;
     c->stats.passes = (int32_t)passes.size();
     if (!device_films) {
@@ -1606,28 +1623,20 @@ static int render_impl(rt_ctx* c, const rt_view* views, int32_t nv, const rt_par
  * ------------------------------------------------------------------------- */
 namespace {
 
-/* The bounces of the n paths in queue 0 (compat or unbiased estimator):
- * extend, shade, shadow and advance launches per bounce, then one finishing
- * launch - from cfg.finish_depth on, or from bounce 1 for a batch of at
- * most RT_FINISH_PATHS_PER_CU paths per CU, whose bounces after the camera
- * rays are all short launches (the Metropolis renderer's 1M-path batches
- * on the 1M-triangle mesh: 612 to 823 Mrays/s; ERPT's 4M-path batches gain
- * 1% from bounce 4 and lose from bounce 1). */
-hipError_t trace_bounces(rt_ctx* c, const PtParams& pp, int64_t n, int grid)
+/* The bounces of the paths in queue 0 (compat or unbiased estimator):
+ * per bounce, the finishing launch (a no-op until the queue is short) and
+ * the extend, shade, shadow and advance launches (no-ops from then on). */
+hipError_t trace_bounces(rt_ctx* c, const PtParams& pp, int grid)
 {
-    const bool small = n <= (int64_t)c->n_cu * RT_FINISH_PATHS_PER_CU;
-    const int finish = c->cfg.finish_depth <= 0 ? pp.max_depth : small ? 1 : c->cfg.finish_depth;
+    const FinishRule fr = finish_rule(c);
     hipError_t e;
     for (int dd = 0; dd < pp.max_depth; ++dd) {
         const int q = dd & 1;
-        if (dd == finish) {
-            if ((e = launch_finish(c->scene, c->ps, pp, dd, q, grid, c->stream))) return e;
-            return launch_advance(c->ps, q, 0, c->stream);
-        }
-        if ((e = launch_extend(c->scene, c->ps, q, 0, grid, c->stream))) return e;
-        if ((e = launch_shade(c->scene, c->ps, pp, dd, q, grid, c->stream))) return e;
-        if ((e = launch_shadow(c->scene, c->ps, pp, 0, grid, c->stream))) return e;
-        if ((e = launch_advance(c->ps, q, dd, c->stream))) return e;
+        if (dd >= fr.min_depth && (e = launch_finish(c->scene, c->ps, pp, dd, q, fr, grid, c->stream))) return e;
+        if ((e = launch_extend(c->scene, c->ps, q, dd, fr, 0, grid, c->stream))) return e;
+        if ((e = launch_shade(c->scene, c->ps, pp, dd, q, fr, grid, c->stream))) return e;
+        if ((e = launch_shadow(c->scene, c->ps, pp, q, dd, fr, 0, grid, c->stream))) return e;
+        if ((e = launch_advance(c->ps, q, dd, fr, c->stream))) return e;
     }
     return hipSuccess;
 }
@@ -1642,7 +1651,7 @@ hipError_t mlt_trace(rt_ctx* c, int n, int cap, int max_depth, int rr_start, flo
     hipError_t e;
     if ((e = launch_generate_pss(c->ps, (const DevView*)c->mlt_view.p, n, pp, pixel_units, grid, c->stream)))
         return e;
-    return trace_bounces(c, pp, n, grid);
+    return trace_bounces(c, pp, grid);
 }
 
 /* Path slots, proposal and state buffers for cap chains of G groups. */
@@ -2088,6 +2097,7 @@ int render_mlt_impl(rt_ctx* c, const rt_camera* cam, const rt_mlt_params* p, flo
     c->stats.finish_ms = 0;
     c->stats.finish_rays = (int64_t)hc.finish_rays;
     c->stats.finish_shadow_rays = (int64_t)hc.finish_shadow_rays;
+    c->stats.finish_bounce = hc.finished;
     if (out) *out = res;
     return RT_OK;
 }
@@ -2158,7 +2168,7 @@ static int integrate_rays_impl(rt_ctx* c, const float* o, const float* d, int64_
         const int m = (int)std::min<int64_t>(cap, n - base);
         HIP_TRY(c, launch_generate_rays(c->ps, (const float4*)o + base, (const float4*)d + base, m, pp,
                                         key0 + (uint32_t)base, grid, c->stream));
-        HIP_TRY(c, trace_bounces(c, pp, m, grid));
+        HIP_TRY(c, trace_bounces(c, pp, grid));
         HIP_TRY(c, launch_ray_values(c->ps, m, rgb + 3 * base, grid, c->stream));
         HIP_TRY(c, flight.unit_done());
     }
@@ -2174,6 +2184,7 @@ static int integrate_rays_impl(rt_ctx* c, const float* o, const float* d, int64_
     c->stats.shadow_rays = (int64_t)hc.shadow_rays;
     c->stats.finish_rays = (int64_t)hc.finish_rays;
     c->stats.finish_shadow_rays = (int64_t)hc.finish_shadow_rays;
+    c->stats.finish_bounce = hc.finished;
     return RT_OK;
 }
 

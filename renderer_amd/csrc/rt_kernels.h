@@ -78,17 +78,27 @@ struct DevPass {
 #define RT_XCDS 8
 #define RT_CLAIM_STRIDE 32
 
-/* Default bounce from which rt_render traces a pass's remaining paths to
- * their ends in one launch (k_finish; rt_launch_config.finish_depth).  A
+/* When a pass's remaining paths go to the finishing launch (k_finish): at
+ * the first bounce >= min_depth whose queue holds at most `limit` paths
+ * (min_depth = INT_MAX: never).  The host enqueues k_finish and the
+ * per-bounce kernels for every bounce; each evaluates this rule on the same
+ * device counters, so they agree on the bounce without a round trip to the
+ * host, and the kernels of the bounces from there on return at once. */
+struct FinishRule {
+    int32_t min_depth, limit;
+};
+/* Defaults (rt_launch_config.finish_depth, finish_paths): from bounce 1, at
+ * most 4096 paths per CU.  The finishing launch runs a path per lane at 4
+ * waves per SIMD (4096 lanes per CU), and a launch of the per-bounce kernels
+ * costs about its slowest ray's latency chain however few rays it holds.  A
  * 1024^2, 64 spp render of a 1M-triangle mesh (max depth 8, roulette from
- * bounce 3) takes 83.9 ms from bounce 4 (the launch 4.0 ms, where the
- * separate launches of those bounces took 5.8 ms), 85.5 ms from bounce 3
- * and 86.2 ms with every bounce in separate launches. */
+ * bounce 3) then finishes from bounce 4 (0.34M paths of a 32M-path pass):
+ * 83.9 ms per render, where every bounce in separate launches took 86.2 ms
+ * and finishing from bounce 3 (1.45M paths) 85.5 ms; Metropolis batches of
+ * 1M paths finish from bounce 1 (612 to 825 Mrays/s). */
 #ifndef RT_FINISH_DEPTH
-#define RT_FINISH_DEPTH 4
+#define RT_FINISH_DEPTH 1
 #endif
-/* Batches of the Markov-chain renderers and of rt_integrate_rays with at
- * most this many paths per CU finish from bounce 1. */
 #ifndef RT_FINISH_PATHS_PER_CU
 #define RT_FINISH_PATHS_PER_CU 4096
 #endif
@@ -97,6 +107,8 @@ struct DevCounters {
     int32_t q_count[2];      /* path queue lengths (ping-pong) */
     int32_t shadow_count;
     int32_t pass;            /* current pass index */
+    int32_t finished;        /* the bounce from which this pass's paths went to k_finish, -1 before */
+    int32_t busy_bounces;    /* bounces traced by per-bounce launches (not skipped), summed over passes */
     int32_t ext_next[RT_XCDS * RT_CLAIM_STRIDE]; /* closest-hit and any-hit launches: the next unclaimed */
     int32_t sh_next[RT_XCDS * RT_CLAIM_STRIDE];  /* position in each segment (zero before each launch) */
     unsigned long long camera_rays, extension_rays, shadow_rays;
@@ -250,19 +262,24 @@ hipError_t launch_generate_rays(const PathState& ps, const float4* o, const floa
                                 uint32_t key0, int grid, hipStream_t st);
 /* rgb[3 j .. 3 j + 2] = radiance of path slot j, j < n */
 hipError_t launch_ray_values(const PathState& ps, int n, float* rgb, int grid, hipStream_t st);
-hipError_t launch_extend(const HostDevScene& s, const PathState& ps, int qsel, int count, int grid, hipStream_t st);
-hipError_t launch_shade(const HostDevScene& s, const PathState& ps, PtParams p, int depth, int qsel, int grid,
-                        hipStream_t st);
-hipError_t launch_shadow(const HostDevScene& s, const PathState& ps, PtParams p, int count, int grid, hipStream_t st);
-/* Compat and unbiased estimators: every path of queue qsel (at bounce
- * `depth`) traced, shaded and shadow-tested to its end in one launch, a lane
- * per path; counts its rays after bounce 0 as extension and shadow rays. */
-hipError_t launch_finish(const HostDevScene& s, const PathState& ps, PtParams p, int depth, int qsel, int grid,
-                         hipStream_t st);
+/* The per-bounce kernels of bounce `depth` (queue qsel); under FinishRule
+ * `fr` they return at once from the finishing bounce on. */
+hipError_t launch_extend(const HostDevScene& s, const PathState& ps, int qsel, int depth, FinishRule fr, int count,
+                         int grid, hipStream_t st);
+hipError_t launch_shade(const HostDevScene& s, const PathState& ps, PtParams p, int depth, int qsel, FinishRule fr,
+                        int grid, hipStream_t st);
+hipError_t launch_shadow(const HostDevScene& s, const PathState& ps, PtParams p, int qsel, int depth, FinishRule fr,
+                         int count, int grid, hipStream_t st);
+/* Compat and unbiased estimators: if bounce `depth` is the finishing bounce
+ * under `fr`, every path of queue qsel traced, shaded and shadow-tested to
+ * its end in one launch, a lane per path (else it returns at once); counts
+ * its rays after bounce 0 as extension and shadow rays. */
+hipError_t launch_finish(const HostDevScene& s, const PathState& ps, PtParams p, int depth, int qsel, FinishRule fr,
+                         int grid, hipStream_t st);
 /* lanes of a traversal launch of `grid` blocks (the stack overflow array
  * holds RT_STACK_OVF entries for each) */
 int64_t trav_lanes(int grid);
-hipError_t launch_advance(const PathState& ps, int qsel, int depth, hipStream_t st);
+hipError_t launch_advance(const PathState& ps, int qsel, int depth, FinishRule fr, hipStream_t st);
 hipError_t launch_accumulate(const PathState& ps, const DevView* views, const DevTile* tiles,
                              const DevPass* passes, int grid, hipStream_t st);
 hipError_t launch_pass_end(const PathState& ps, hipStream_t st);

@@ -178,6 +178,19 @@ __device__ __forceinline__ void clear_claims(DevCounters* c)
     for (int x = 0; x < RT_XCDS; ++x) c->ext_next[x * RT_CLAIM_STRIDE] = c->sh_next[x * RT_CLAIM_STRIDE] = 0;
 }
 
+/* Bounce `depth` (queue qsel) is the one from which k_finish traces the
+ * pass's remaining paths; the counters it reads change only between the
+ * launches of a bounce, so every kernel of the bounce sees the same. */
+__device__ __forceinline__ bool finish_here(const DevCounters* c, int qsel, int depth, FinishRule fr)
+{
+    return c->finished < 0 && depth >= fr.min_depth && c->q_count[qsel] <= fr.limit;
+}
+/* ... or a later one: the per-bounce kernels have nothing to do */
+__device__ __forceinline__ bool bounce_finished(const DevCounters* c, int qsel, int depth, FinishRule fr)
+{
+    return c->finished >= 0 || finish_here(c, qsel, depth, fr);
+}
+
 /* ===========================================================================
  * Batch traversal kernels
  * ========================================================================= */
@@ -722,6 +735,7 @@ __global__ __launch_bounds__(PT_BLOCK) void k_generate(PathState ps, const DevVi
         ps.cnt->q_count[1] = 0;
         ps.cnt->shadow_count = 0;
         clear_claims(ps.cnt);
+        ps.cnt->finished = -1;
         ps.cnt->camera_rays += (unsigned long long)pass.slots;
     }
     for (int slot = blockIdx.x * PT_BLOCK + threadIdx.x; slot < pass.slots; slot += gridDim.x * PT_BLOCK) {
@@ -782,6 +796,7 @@ __global__ __launch_bounds__(PT_BLOCK) void k_generate_pss(PathState ps, const D
         ps.cnt->q_count[1] = 0;
         ps.cnt->shadow_count = 0;
         clear_claims(ps.cnt);
+        ps.cnt->finished = -1;
         ps.cnt->camera_rays += (unsigned long long)n;
     }
     for (int slot = blockIdx.x * PT_BLOCK + threadIdx.x; slot < n; slot += gridDim.x * PT_BLOCK) {
@@ -835,6 +850,7 @@ __global__ __launch_bounds__(PT_BLOCK) void k_generate_rays(PathState ps, const 
         ps.cnt->q_count[1] = 0;
         ps.cnt->shadow_count = 0;
         clear_claims(ps.cnt);
+        ps.cnt->finished = -1;
         ps.cnt->camera_rays += (unsigned long long)n;
     }
     for (int slot = blockIdx.x * PT_BLOCK + threadIdx.x; slot < n; slot += gridDim.x * PT_BLOCK) {
@@ -904,8 +920,10 @@ hipError_t launch_ray_values(const PathState& ps, int n, float* rgb, int grid, h
  * LDS: 16 KB per block, and 8.6 KB for the top nodes: room for the 6
  * blocks per CU that the registers allow. */
 template <bool COUNT, bool OBJ>
-__global__ __launch_bounds__(PT_BLOCK) RT_TRAV_OCCUPANCY void k_extend(DevScene S, PathState ps, int qsel)
+__global__ __launch_bounds__(PT_BLOCK) RT_TRAV_OCCUPANCY void k_extend(DevScene S, PathState ps, int qsel, int depth,
+                                                                       FinishRule fr)
 {
+    if (bounce_finished(ps.cnt, qsel, depth, fr)) return;
     __shared__ int32_t stack[RT_STACK_LDS * PT_BLOCK];
     __shared__ float4 top_nodes[RT_TOP_NODES * (RT_NODE_FLOATS / 4)];
     const char* top = stage_top_nodes(S, top_nodes);
@@ -945,18 +963,20 @@ __global__ __launch_bounds__(PT_BLOCK) RT_TRAV_OCCUPANCY void k_extend(DevScene 
     }
 }
 
-hipError_t launch_extend(const HostDevScene& s, const PathState& ps, int qsel, int count, int grid, hipStream_t st)
+hipError_t launch_extend(const HostDevScene& s, const PathState& ps, int qsel, int depth, FinishRule fr, int count,
+                         int grid, hipStream_t st)
 {
     /* scenes without analytic objects run the variant without their test */
     const bool obj = s.n_obj > 0;
     if (count && obj)
-        hipLaunchKernelGGL((k_extend<true, true>), dim3(grid), dim3(PT_BLOCK), 0, st, as_dev(s), ps, qsel);
+        hipLaunchKernelGGL((k_extend<true, true>), dim3(grid), dim3(PT_BLOCK), 0, st, as_dev(s), ps, qsel, depth, fr);
     else if (count)
-        hipLaunchKernelGGL((k_extend<true, false>), dim3(grid), dim3(PT_BLOCK), 0, st, as_dev(s), ps, qsel);
+        hipLaunchKernelGGL((k_extend<true, false>), dim3(grid), dim3(PT_BLOCK), 0, st, as_dev(s), ps, qsel, depth, fr);
     else if (obj)
-        hipLaunchKernelGGL((k_extend<false, true>), dim3(grid), dim3(PT_BLOCK), 0, st, as_dev(s), ps, qsel);
+        hipLaunchKernelGGL((k_extend<false, true>), dim3(grid), dim3(PT_BLOCK), 0, st, as_dev(s), ps, qsel, depth, fr);
     else
-        hipLaunchKernelGGL((k_extend<false, false>), dim3(grid), dim3(PT_BLOCK), 0, st, as_dev(s), ps, qsel);
+        hipLaunchKernelGGL((k_extend<false, false>), dim3(grid), dim3(PT_BLOCK), 0, st, as_dev(s), ps, qsel, depth,
+                           fr);
     return hipGetLastError();
 }
 
@@ -1424,8 +1444,10 @@ hipError_t launch_shade_guided(const HostDevScene& s, const PathState& ps, const
 }
 
 template <int MODE>
-__global__ __launch_bounds__(PT_BLOCK) RT_SHADE_OCCUPANCY void k_shade(DevScene S, PathState ps, PtParams P, int depth, int qsel)
+__global__ __launch_bounds__(PT_BLOCK) RT_SHADE_OCCUPANCY void k_shade(DevScene S, PathState ps, PtParams P, int depth, int qsel,
+                                                                       FinishRule fr)
 {
+    if (bounce_finished(ps.cnt, qsel, depth, fr)) return;
     const int n = ps.cnt->q_count[qsel];
     const int32_t* q = ps.queue[qsel];
     int32_t* qn = ps.queue[qsel ^ 1];
@@ -1454,14 +1476,15 @@ __global__ __launch_bounds__(PT_BLOCK) RT_SHADE_OCCUPANCY void k_shade(DevScene 
     }
 }
 
-hipError_t launch_shade(const HostDevScene& s, const PathState& ps, PtParams p, int depth, int qsel, int grid,
-                        hipStream_t st)
+hipError_t launch_shade(const HostDevScene& s, const PathState& ps, PtParams p, int depth, int qsel, FinishRule fr,
+                        int grid, hipStream_t st)
 {
     if (p.mode == RT_MODE_COMPAT)
-        hipLaunchKernelGGL(k_shade<RT_MODE_COMPAT>, dim3(grid), dim3(PT_BLOCK), 0, st, as_dev(s), ps, p, depth, qsel);
+        hipLaunchKernelGGL(k_shade<RT_MODE_COMPAT>, dim3(grid), dim3(PT_BLOCK), 0, st, as_dev(s), ps, p, depth, qsel,
+                           fr);
     else
         hipLaunchKernelGGL(k_shade<RT_MODE_UNBIASED>, dim3(grid), dim3(PT_BLOCK), 0, st, as_dev(s), ps, p, depth,
-                           qsel);
+                           qsel, fr);
     return hipGetLastError();
 }
 
@@ -1469,8 +1492,10 @@ hipError_t launch_shade(const HostDevScene& s, const PathState& ps, PtParams p, 
  * shadow: any-hit test of next-event rays
  * ========================================================================= */
 template <bool OBJ, bool COUNT>
-__global__ __launch_bounds__(PT_BLOCK) RT_TRAV_OCCUPANCY void k_shadow(DevScene S, PathState ps, int compat)
+__global__ __launch_bounds__(PT_BLOCK) RT_TRAV_OCCUPANCY void k_shadow(DevScene S, PathState ps, int compat, int qsel,
+                                                                       int depth, FinishRule fr)
 {
+    if (bounce_finished(ps.cnt, qsel, depth, fr)) return;
     __shared__ int32_t stack[RT_STACK_LDS * PT_BLOCK];
     __shared__ float4 top_nodes[RT_TOP_NODES * (RT_NODE_FLOATS / 4)];
     const char* top = stage_top_nodes(S, top_nodes);
@@ -1522,18 +1547,23 @@ __global__ __launch_bounds__(PT_BLOCK) RT_TRAV_OCCUPANCY void k_shadow(DevScene 
     }
 }
 
-hipError_t launch_shadow(const HostDevScene& s, const PathState& ps, PtParams p, int count, int grid, hipStream_t st)
+hipError_t launch_shadow(const HostDevScene& s, const PathState& ps, PtParams p, int qsel, int depth, FinishRule fr,
+                         int count, int grid, hipStream_t st)
 {
     int compat = p.mode == RT_MODE_COMPAT ? 1 : 0;
     const bool obj = s.n_obj > 0;
     if (count && obj)
-        hipLaunchKernelGGL((k_shadow<true, true>), dim3(grid), dim3(PT_BLOCK), 0, st, as_dev(s), ps, compat);
+        hipLaunchKernelGGL((k_shadow<true, true>), dim3(grid), dim3(PT_BLOCK), 0, st, as_dev(s), ps, compat, qsel,
+                           depth, fr);
     else if (count)
-        hipLaunchKernelGGL((k_shadow<false, true>), dim3(grid), dim3(PT_BLOCK), 0, st, as_dev(s), ps, compat);
+        hipLaunchKernelGGL((k_shadow<false, true>), dim3(grid), dim3(PT_BLOCK), 0, st, as_dev(s), ps, compat, qsel,
+                           depth, fr);
     else if (obj)
-        hipLaunchKernelGGL((k_shadow<true, false>), dim3(grid), dim3(PT_BLOCK), 0, st, as_dev(s), ps, compat);
+        hipLaunchKernelGGL((k_shadow<true, false>), dim3(grid), dim3(PT_BLOCK), 0, st, as_dev(s), ps, compat, qsel,
+                           depth, fr);
     else
-        hipLaunchKernelGGL((k_shadow<false, false>), dim3(grid), dim3(PT_BLOCK), 0, st, as_dev(s), ps, compat);
+        hipLaunchKernelGGL((k_shadow<false, false>), dim3(grid), dim3(PT_BLOCK), 0, st, as_dev(s), ps, compat, qsel,
+                           depth, fr);
     return hipGetLastError();
 }
 
@@ -1557,8 +1587,9 @@ __device__ __forceinline__ unsigned wave_sum(unsigned v)
 
 template <int MODE, bool OBJ>
 __global__ __launch_bounds__(PT_BLOCK) RT_FINISH_OCCUPANCY void k_finish(DevScene S, PathState ps, PtParams P,
-                                                                         int depth0, int qsel)
+                                                                         int depth0, int qsel, FinishRule fr)
 {
+    if (!finish_here(ps.cnt, qsel, depth0, fr)) return;
     __shared__ int32_t stack[RT_STACK_LDS * PT_BLOCK];
     __shared__ float4 top_nodes[RT_TOP_NODES * (RT_NODE_FLOATS / 4)];
     const char* top = stage_top_nodes(S, top_nodes);
@@ -1646,41 +1677,47 @@ __global__ __launch_bounds__(PT_BLOCK) RT_FINISH_OCCUPANCY void k_finish(DevScen
     }
 }
 
-hipError_t launch_finish(const HostDevScene& s, const PathState& ps, PtParams p, int depth, int qsel, int grid,
-                         hipStream_t st)
+hipError_t launch_finish(const HostDevScene& s, const PathState& ps, PtParams p, int depth, int qsel, FinishRule fr,
+                         int grid, hipStream_t st)
 {
     const bool obj = s.n_obj > 0;
     const bool compat = p.mode == RT_MODE_COMPAT;
     if (compat && obj)
         hipLaunchKernelGGL((k_finish<RT_MODE_COMPAT, true>), dim3(grid), dim3(PT_BLOCK), 0, st, as_dev(s), ps, p,
-                           depth, qsel);
+                           depth, qsel, fr);
     else if (compat)
         hipLaunchKernelGGL((k_finish<RT_MODE_COMPAT, false>), dim3(grid), dim3(PT_BLOCK), 0, st, as_dev(s), ps, p,
-                           depth, qsel);
+                           depth, qsel, fr);
     else if (obj)
         hipLaunchKernelGGL((k_finish<RT_MODE_UNBIASED, true>), dim3(grid), dim3(PT_BLOCK), 0, st, as_dev(s), ps, p,
-                           depth, qsel);
+                           depth, qsel, fr);
     else
         hipLaunchKernelGGL((k_finish<RT_MODE_UNBIASED, false>), dim3(grid), dim3(PT_BLOCK), 0, st, as_dev(s), ps, p,
-                           depth, qsel);
+                           depth, qsel, fr);
     return hipGetLastError();
 }
 
 /* End of a bounce: count the rays the bounce traced (its queue length and
  * its shadow-ray count; the queue of bounce 0 holds the camera rays, counted
- * by k_generate), then clear the consumed queue and the work counters. */
-__global__ void k_advance(PathState ps, int qsel, int depth)
+ * by k_generate), then clear the consumed queue and the work counters.  At
+ * the finishing bounce, record it (k_finish counted its own rays). */
+__global__ void k_advance(PathState ps, int qsel, int depth, FinishRule fr)
 {
     DevCounters* c = ps.cnt;
-    if (depth > 0) c->extension_rays += (unsigned long long)c->q_count[qsel];
-    c->shadow_rays += (unsigned long long)c->shadow_count;
+    if (finish_here(c, qsel, depth, fr)) {
+        c->finished = depth;
+    } else if (c->finished < 0) {
+        if (depth > 0) c->extension_rays += (unsigned long long)c->q_count[qsel];
+        c->shadow_rays += (unsigned long long)c->shadow_count;
+        c->busy_bounces += 1;
+    }
     c->q_count[qsel] = 0;
     c->shadow_count = 0;
     clear_claims(c);
 }
-hipError_t launch_advance(const PathState& ps, int qsel, int depth, hipStream_t st)
+hipError_t launch_advance(const PathState& ps, int qsel, int depth, FinishRule fr, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, st, ps, qsel, depth);
+    hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, st, ps, qsel, depth, fr);
     return hipGetLastError();
 }
 
@@ -1785,6 +1822,8 @@ __global__ void k_reset(PathState ps)
     c->q_count[0] = c->q_count[1] = 0;
     c->shadow_count = 0;
     c->pass = 0;
+    c->finished = -1;
+    c->busy_bounces = 0;
     clear_claims(c);
     c->camera_rays = c->extension_rays = c->shadow_rays = 0ull;
     c->node_visits = c->prim_tests = 0ull;

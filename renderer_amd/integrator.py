@@ -223,9 +223,9 @@ class GpuContext:
         return out
 
     def set_launch_config(self, path_slots=0, block=0, blocks_per_cu=0, use_graph=0, count_traversal=0,
-                          guide_epochs=0, finish_depth=0):
+                          guide_epochs=0, finish_depth=0, finish_paths=0):
         c = _abi.RtLaunchConfig(int(path_slots), int(block), int(blocks_per_cu), int(use_graph), int(count_traversal),
-                                int(guide_epochs), int(finish_depth))
+                                int(guide_epochs), int(finish_depth), int(finish_paths))
         _check(self.lib, self.h, self.lib.rt_set_launch_config(self.h, ctypes.byref(c)))
 
     def launch_config(self):
