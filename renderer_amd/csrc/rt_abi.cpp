@@ -51,7 +51,6 @@ struct rt_ctx {
     // ray batches (host entry points)
     DevBuf rb_o, rb_d, rb_out0, rb_out1;
     DevBuf stack_ovf;  /* traversal stack overflow of the largest launch so far */
-    DevBuf tm_list, tm_count;  /* tonemapper: values deferred to the double-precision path */
 
     // render
     PathState ps{};
@@ -276,7 +275,7 @@ int rt_destroy(rt_ctx* c)
     hipStreamSynchronize(c->stream);
     DevBuf* all[] = {&c->nodes, &c->prims, &c->prim_id, &c->mat, &c->vnormals, &c->geo, &c->mats, &c->lights,
                      &c->env_rgb, &c->env_row, &c->env_col, &c->env_p, &c->probe_rgb, &c->rb_o, &c->rb_d, &c->rb_out0,
-                     &c->rb_out1, &c->stack_ovf, &c->tm_list, &c->tm_count, &c->counters, &c->views, &c->tiles, &c->passes, &c->films, &c->g_theta,
+                     &c->rb_out1, &c->stack_ovf, &c->counters, &c->views, &c->tiles, &c->passes, &c->films, &c->g_theta,
                      &c->g_w16, &c->g_m, &c->g_v, &c->g_grad, &c->g_part, &c->g_acc, &c->g_count, &c->g_rec, &c->g_aux,
                      &c->g_open, &c->g_out, &c->g_in, &c->g_tmp_rec, &c->g_tmp_y, &c->obj_nodes, &c->obj_first, &c->obj_xf, &c->leaf_obj,
                      &c->tex_rgb, &c->tex_info, &c->tri_tc,
@@ -970,15 +969,9 @@ int rt_trace_any(rt_ctx* c, const float* o, const float* d, int64_t n, uint8_t* 
  * ------------------------------------------------------------------------- */
 static int tonemap_impl(rt_ctx* c, const float* film, int64_t n, float gamma, uint8_t* out)
 {
-    const int64_t cap = std::min<int64_t>(n, RT_TONEMAP_LIST);
     const int grid = c->n_cu * 8;
-    int r;
-    if ((r = ensure(c, c->tm_list, (size_t)std::max<int64_t>(cap, 1) * 8)) ||
-        (r = ensure(c, c->tm_count, (size_t)tonemap_grid(n, grid) * 4)))
-        return r;
     HIP_TRY(c, hipEventRecord(c->ev_a, c->stream));
-    hipError_t e = launch_tonemap(film, out, n, gamma, (int64_t*)c->tm_list.p, (unsigned int*)c->tm_count.p, cap,
-                                  grid, c->stream);
+    hipError_t e = launch_tonemap(film, out, n, gamma, grid, c->stream);
     if (e != hipSuccess) return set_err(c, RT_ERR_HIP, std::string("tonemap launch: ") + hipGetErrorString(e));
     HIP_TRY(c, hipEventRecord(c->ev_b, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));

@@ -285,12 +285,8 @@ hipError_t launch_accumulate(const PathState& ps, const DevView* views, const De
 hipError_t launch_pass_end(const PathState& ps, hipStream_t st);
 hipError_t launch_reset_counters(const PathState& ps, hipStream_t st);
 
-/* rt_film.hip: n floats of a film -> n bytes (src/tonemapper.cpp:50-56);
- * list (cap entries) and count (tonemap_grid(n, grid) entries) are scratch
- * for the values that need the double-precision path */
-#define RT_TONEMAP_LIST (1 << 22)
+/* rt_film.hip: n floats of a film -> n bytes (src/tonemapper.cpp:50-56) */
 int tonemap_grid(int64_t n, int grid);
-hipError_t launch_tonemap(const float* film, uint8_t* out, int64_t n, float gamma, int64_t* list,
-                          unsigned int* count, int64_t cap, int grid, hipStream_t st);
+hipError_t launch_tonemap(const float* film, uint8_t* out, int64_t n, float gamma, int grid, hipStream_t st);
 
 #endif

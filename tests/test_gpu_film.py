@@ -56,3 +56,15 @@ def test_tonemap_device_of_a_rendered_film(gpu_ctx):
     assert f.mean() > 0
     assert np.array_equal(out.cpu().numpy(), tonemap(f, 2.2))
     assert gpu_ctx.stats()["tonemap_ms"] > 0
+
+
+def test_tonemap_every_value_undecided(gpu_ctx):
+    """A film of values next to grey-level boundaries only: the float
+    estimate decides none of them, so each block's list of values for the
+    double-precision path fills up and the rest are evaluated in place."""
+    k = np.arange(1, 256, dtype=np.float64)
+    b = ((k / 255.0) ** 2.2).astype(np.float32)
+    v = np.concatenate([b, np.nextafter(b, np.float32(np.inf)), np.nextafter(b, np.float32(-np.inf))])
+    film = np.tile(v, (1 << 21) // len(v) + 1)[: 3 * (1 << 19)].reshape(-1, 512, 3)
+    got = gpu_ctx.tonemap(film, 2.2)
+    assert np.array_equal(got, tonemap(film, 2.2))
