@@ -68,6 +68,11 @@ def load():
                                          P, I64, P, ctypes.c_int]
     lib.oc_mlt_perturb.argtypes = [P, P, P, I64, P]
     lib.oc_sym_filter.argtypes = [P, P, P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, P]
+    lib.oc_box_blur.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, P]
+    lib.oc_sep3_reflect101.argtypes = [P, ctypes.c_int, ctypes.c_int, P]
+    lib.oc_var2.argtypes = [P, P, I64, P, P]
+    lib.oc_flt_importance.argtypes = [P, P, ctypes.c_int, ctypes.c_int, P]
+    lib.oc_flt_accumulate.argtypes = [ctypes.c_int, ctypes.c_int, P, P, P, P, P, P, I64, P, P, P]
     lib.oc_bsdf_sample.restype = ctypes.c_int
     lib.oc_bsdf_sample.argtypes = [P, ctypes.c_int, P, P, ctypes.c_float, ctypes.c_float, ctypes.c_float, P]
     _lib = lib
@@ -304,3 +309,58 @@ def sym_filter(image, guide, pixvar, patch=7, radius=10):
     lib.oc_sym_filter(a.ctypes.data, g.ctypes.data, v.ctypes.data, a.shape[1], a.shape[0], int(patch), int(radius),
                       out.ctypes.data)
     return out
+
+
+def box_blur(img, p):
+    """cv::blur(img, Size(p, p)) with BORDER_REFLECT of an (H, W) float32 image, as the filter restates it."""
+    lib = load()
+    a = np.ascontiguousarray(img, np.float32)
+    out = np.zeros_like(a)
+    lib.oc_box_blur(a.ctypes.data, a.shape[1], a.shape[0], int(p), out.ctypes.data)
+    return out
+
+
+def sep3_reflect101(img):
+    """sepFilter2D with [.1 .8 .1] both ways and BORDER_REFLECT_101 of an (H, W, 3) float32 image."""
+    lib = load()
+    a = np.ascontiguousarray(img, np.float32)
+    out = np.zeros_like(a)
+    lib.oc_sep3_reflect101(a.ctypes.data, a.shape[1], a.shape[0], out.ctypes.data)
+    return out
+
+
+def var2(a, b):
+    """mathhelper var() of two float32 arrays: (mean, variance)."""
+    lib = load()
+    x, y = (np.ascontiguousarray(v, np.float32).ravel() for v in (a, b))
+    m, v = np.zeros_like(x), np.zeros_like(x)
+    lib.oc_var2(x.ctypes.data, y.ctypes.data, len(x), m.ctypes.data, v.ctypes.data)
+    return m.reshape(np.shape(a)), v.reshape(np.shape(a))
+
+
+def flt_importance(var_filtered, mean_filtered):
+    """The filtered renderer's importance map of a later pass from the (H, W, 3)
+    variance and mean of the filtered estimate."""
+    lib = load()
+    v, m = (np.ascontiguousarray(x, np.float32) for x in (var_filtered, mean_filtered))
+    out = np.zeros(v.shape[:2], np.float32)
+    lib.oc_flt_importance(v.ctypes.data, m.ctypes.data, v.shape[1], v.shape[0], out.ctypes.data)
+    return out
+
+
+def flt_accumulate(state, pos, wt, rgb, sumimp=None, newsum=None):
+    """One film's running sums (dict of float32 arrays film, mean, m2 (H, W, 3)
+    and sw (H, W)), rescaled to a new pass's importance if sumimp/newsum are
+    given, then the deposits (pixel index, weight, rgb) added in order; the
+    state is updated in place."""
+    lib = load()
+    h, w = state["sw"].shape
+    pos = np.ascontiguousarray(pos, np.int32)
+    wt = np.ascontiguousarray(wt, np.float32)
+    rgb = np.ascontiguousarray(rgb, np.float32)
+    si = np.ascontiguousarray(sumimp, np.float32) if sumimp is not None else None
+    ns = np.ascontiguousarray(newsum, np.float32) if newsum is not None else None
+    lib.oc_flt_accumulate(w, h, state["film"].ctypes.data, state["mean"].ctypes.data, state["m2"].ctypes.data,
+                          state["sw"].ctypes.data, si.ctypes.data if si is not None else None,
+                          ns.ctypes.data if ns is not None else None, len(pos), pos.ctypes.data, wt.ctypes.data,
+                          rgb.ctypes.data)

@@ -19,11 +19,14 @@
  * the stream is fixed by philox_uniforms() below and is the contract that
  * lets the GPU image be compared pixel by pixel with this one.
  *
- * Parity unpinned (no reference vectors and no closed form checks them):
- * the filtered Metropolis renderer's importance map of later passes, its
- * weighted running variance, and the border arithmetic of the OpenCV calls
- * it restates (blur, sepFilter2D); the symmetric filter itself and one pass
- * without filtering are pinned by the tests of test_oracle_mlt.py.
+ * The filtered Metropolis renderer's pieces that no reference vectors cover
+ * are pinned one by one (tests/test_oracle_flt_pieces.py): the OpenCV blur
+ * and sepFilter2D border arithmetic against scipy's reflect / mirror
+ * filters, the importance map of later passes at hand-computed values, the
+ * weighted running mean and variance against the direct weighted formulas;
+ * the symmetric filter and one pass without filtering by test_oracle_mlt.py.
+ * Parity unpinned: how the reference composes the passes (the order of
+ * rescale, deposits and filtering) beyond GPU-oracle agreement.
  */
 #include "pt_oracle.h"
 
@@ -1918,6 +1921,23 @@ struct FltState {
     Img film[2], mean[2], m2[2], sw[2];
 };
 
+// addSample (src/metropolisfltrenderer.cpp:54-75) of film n at pixel pos:
+// the weighted running mean and sum of weighted squared deviations (West's
+// update), the weighted sum of the values and the sum of the weights
+void flt_add_sample(FltState& st, int n, int pos, float w, const float vc[3])
+{
+    const int W = st.film[n].W;
+    const float ow = st.sw[n].v[pos], nw = w + ow;
+    for (int c = 0; c < 3; ++c) {
+        float delta = vc[c] - st.mean[n].at(pos / W, pos % W, c);
+        float Rr = delta * (w / nw);
+        st.mean[n].at(pos / W, pos % W, c) += Rr;
+        st.m2[n].at(pos / W, pos % W, c) += ow * (delta * Rr);
+        st.film[n].at(pos / W, pos % W, c) += w * vc[c];
+    }
+    st.sw[n].v[pos] = nw;
+}
+
 void flt_pass(MltRun& R, int pass, const Img& imp, const Img& newsum, float b, FltState& st, int64_t& rays,
               int64_t& accepted, int64_t& mutations)
 {
@@ -1989,25 +2009,77 @@ void flt_pass(MltRun& R, int pass, const Img& imp, const Img& newsum, float b, F
         // addSample, in chain order
         for (auto& dep : deps)
             for (auto& d : dep) {
-                const int pos = d.first;
-                const float w = d.second.first;
                 const Col v = d.second.second;
-                const float ow = st.sw[n].v[pos], nw = w + ow;
                 const float vc[3] = {v.r, v.g, v.b};
-                for (int c = 0; c < 3; ++c) {
-                    float delta = vc[c] - st.mean[n].at(pos / W, pos % W, c);
-                    float Rr = delta * (w / nw);
-                    st.mean[n].at(pos / W, pos % W, c) += Rr;
-                    st.m2[n].at(pos / W, pos % W, c) += ow * (delta * Rr);
-                    st.film[n].at(pos / W, pos % W, c) += w * vc[c];
-                }
-                st.sw[n].v[pos] = nw;
+                flt_add_sample(st, n, d.first, d.second.first, vc);
             }
         mutations += C * iters;
     }
 }
 
 // MetropolisFltRenderer::render (src/metropolisfltrenderer.cpp:207-320)
+// relative variance of the filtered estimate -> importance (:231-239),
+// normalised to mean 1
+Img flt_importance(const Img& varOfFiltered, const Img& filteredMean)
+{
+    const int W = varOfFiltered.W, H = varOfFiltered.H;
+    const int64_t area = (int64_t)W * H;
+    Img imp(W, H, 1, 1.f);
+    double sum = 0;
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            float vm = chan_mean3(varOfFiltered.at(y, x, 0), varOfFiltered.at(y, x, 1), varOfFiltered.at(y, x, 2));
+            float mm = chan_mean3(filteredMean.at(y, x, 0), filteredMean.at(y, x, 1), filteredMean.at(y, x, 2));
+            float rel = vm / (mm * mm + 1e-2f) + 1e-8f;
+            float e = std::exp(-rel);
+            imp.at(y, x) = -20 * (e - 1) + 1;
+            sum += imp.at(y, x);
+        }
+    const float k = (float)(area / sum);
+    for (float& x : imp.v) x *= k;
+    return imp;
+}
+
+// the running sums of later passes: each sample so far weighted by r, the
+// share of the importance it was deposited under (:241-250)
+void flt_rescale(FltState& st, const Img& sumimp, const Img& newsum)
+{
+    const int64_t area = (int64_t)sumimp.W * sumimp.H;
+    for (int n = 0; n < 2; ++n)
+        for (int64_t k = 0; k < area; ++k) {
+            const float r = sumimp.v[k] / newsum.v[k];
+            for (int c = 0; c < 3; ++c) {
+                st.film[n].v[3 * k + c] = r * st.film[n].v[3 * k + c];
+                st.m2[n].v[3 * k + c] = r * (r * st.m2[n].v[3 * k + c]);
+            }
+        }
+}
+
+// sepFilter2D(.., [.1 .8 .1], [.1 .8 .1]) with BORDER_REFLECT_101 of a
+// three-channel image (the FLT_SIMPLE_VARIANCE_FILTER path, :270-272)
+Img sep3_reflect101(const Img& in)
+{
+    const int W = in.W, H = in.H;
+    auto r101 = [](int i, int n) { return i < 0 ? -i : i >= n ? 2 * n - i - 2 : i; };
+    Img t(W, H, 3), out(W, H, 3);
+    const float k3[3] = {.1f, .8f, .1f};
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x)
+            for (int c = 0; c < 3; ++c) {
+                float s = 0;
+                for (int j = -1; j <= 1; ++j) s += k3[j + 1] * in.at(y, std::min(std::max(r101(x + j, W), 0), W - 1), c);
+                t.at(y, x, c) = s;
+            }
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x)
+            for (int c = 0; c < 3; ++c) {
+                float s = 0;
+                for (int j = -1; j <= 1; ++j) s += k3[j + 1] * t.at(std::min(std::max(r101(y + j, H), 0), H - 1), x, c);
+                out.at(y, x, c) = s;
+            }
+    return out;
+}
+
 void run_filtered(MltRun& R)
 {
     const int W = R.cam.W, H = R.cam.H, P = R.p.flt_passes, F = R.p.flt_flags;
@@ -2022,22 +2094,7 @@ void run_filtered(MltRun& R)
     Img sumimp(W, H, 1), varOfFiltered, filteredMean;
     int64_t rays = 0, accepted = 0, mutations = 0;
     for (int i = 0; i < P; ++i) {
-        Img imp(W, H, 1, 1.f);
-        if (i > 0) {
-            // relative variance -> importance (:231-239), normalised to mean 1
-            double sum = 0;
-            for (int y = 0; y < H; ++y)
-                for (int x = 0; x < W; ++x) {
-                    float vm = chan_mean3(varOfFiltered.at(y, x, 0), varOfFiltered.at(y, x, 1), varOfFiltered.at(y, x, 2));
-                    float mm = chan_mean3(filteredMean.at(y, x, 0), filteredMean.at(y, x, 1), filteredMean.at(y, x, 2));
-                    float rel = vm / (mm * mm + 1e-2f) + 1e-8f;
-                    float e = std::exp(-rel);
-                    imp.at(y, x) = -20 * (e - 1) + 1;
-                    sum += imp.at(y, x);
-                }
-            const float k = (float)(area / sum);
-            for (float& x : imp.v) x *= k;
-        }
+        const Img imp = i > 0 ? flt_importance(varOfFiltered, filteredMean) : Img(W, H, 1, 1.f);
         // b: the bootstrap's mean norm (first pass) or the first film's (:118-124)
         float b;
         if (i == 0) {
@@ -2066,14 +2123,7 @@ void run_filtered(MltRun& R)
         if (i == 0) R.res.b = b;
         Img newsum(W, H, 1);
         for (size_t k = 0; k < newsum.v.size(); ++k) newsum.v[k] = sumimp.v[k] + imp.v[k];
-        for (int n = 0; n < 2; ++n)
-            for (int64_t k = 0; k < area; ++k) {
-                const float r = sumimp.v[k] / newsum.v[k];
-                for (int c = 0; c < 3; ++c) {
-                    st.film[n].v[3 * k + c] = r * st.film[n].v[3 * k + c];
-                    st.m2[n].v[3 * k + c] = r * (r * st.m2[n].v[3 * k + c]);
-                }
-            }
+        flt_rescale(st, sumimp, newsum);
         flt_pass(R, i, imp, newsum, b, st, rays, accepted, mutations);
         // filtering (:252-300)
         Img bv[2];
@@ -2088,25 +2138,7 @@ void run_filtered(MltRun& R)
         if ((F & RT_FLT_NO_FILTER) || (F & RT_FLT_NO_VARIANCE_FILTER)) {
             filteredVar = noisy;
         } else if (F & RT_FLT_SIMPLE_VARIANCE_FILTER) {
-            // sepFilter2D with [.1 .8 .1] both ways (BORDER_REFLECT_101)
-            auto r101 = [](int i, int n) { return i < 0 ? -i : i >= n ? 2 * n - i - 2 : i; };
-            Img t(W, H, 3);
-            filteredVar = Img(W, H, 3);
-            const float k3[3] = {.1f, .8f, .1f};
-            for (int y = 0; y < H; ++y)
-                for (int x = 0; x < W; ++x)
-                    for (int c = 0; c < 3; ++c) {
-                        float s = 0;
-                        for (int j = -1; j <= 1; ++j) s += k3[j + 1] * noisy.at(y, std::min(std::max(r101(x + j, W), 0), W - 1), c);
-                        t.at(y, x, c) = s;
-                    }
-            for (int y = 0; y < H; ++y)
-                for (int x = 0; x < W; ++x)
-                    for (int c = 0; c < 3; ++c) {
-                        float s = 0;
-                        for (int j = -1; j <= 1; ++j) s += k3[j + 1] * t.at(std::min(std::max(r101(y + j, H), 0), H - 1), x, c);
-                        filteredVar.at(y, x, c) = s;
-                    }
+            filteredVar = sep3_reflect101(noisy);
         } else {
             filteredVar = sym_filter(noisy, meanvar, varvar, 1, 2);
         }
@@ -2547,6 +2579,70 @@ static int tonemap_clamp(float x)
 void oc_tonemap(const float* film, int64_t n, float gamma, uint8_t* out)
 {
     for (int64_t i = 0; i < n; ++i) out[i] = (uint8_t)(tonemap_clamp(255 * std::pow(film[i], 1. / gamma)) & 0xff);
+}
+
+/* The filtered Metropolis renderer's pieces, for the tests that pin them. */
+void oc_box_blur(const float* in, int w, int h, int p, float* out)
+{
+    Img a(w, h, 1);
+    std::memcpy(a.v.data(), in, a.v.size() * 4);
+    const Img b = box_blur(a, p);
+    std::memcpy(out, b.v.data(), b.v.size() * 4);
+}
+void oc_sep3_reflect101(const float* in, int w, int h, float* out)
+{
+    Img a(w, h, 3);
+    std::memcpy(a.v.data(), in, a.v.size() * 4);
+    const Img b = sep3_reflect101(a);
+    std::memcpy(out, b.v.data(), b.v.size() * 4);
+}
+void oc_var2(const float* a, const float* b, int64_t n, float* mean, float* var)
+{
+    Img x(1, 1, 1), y(1, 1, 1), m, v;
+    x.W = y.W = (int)n;
+    x.v.assign(a, a + n);
+    y.v.assign(b, b + n);
+    var2(x, y, m, v);
+    std::memcpy(mean, m.v.data(), n * 4);
+    std::memcpy(var, v.v.data(), n * 4);
+}
+void oc_flt_importance(const float* var_filtered, const float* mean_filtered, int w, int h, float* imp)
+{
+    Img v(w, h, 3), m(w, h, 3);
+    std::memcpy(v.v.data(), var_filtered, v.v.size() * 4);
+    std::memcpy(m.v.data(), mean_filtered, m.v.size() * 4);
+    const Img out = flt_importance(v, m);
+    std::memcpy(imp, out.v.data(), out.v.size() * 4);
+}
+/* One film's running sums (film, mean, m2: w*h*3; sw: w*h): rescaled to a
+ * new pass's importance (sumimp, newsum: w*h, or null for none), then the
+ * deposits (pixel, weight, rgb) added in order. */
+void oc_flt_accumulate(int w, int h, float* film, float* mean, float* m2, float* sw, const float* sumimp,
+                       const float* newsum, int64_t ndep, const int32_t* pos, const float* wt, const float* rgb)
+{
+    FltState st;
+    const size_t n3 = (size_t)w * h * 3, n1 = (size_t)w * h;
+    for (int k = 0; k < 2; ++k) {
+        st.film[k] = Img(w, h, 3);
+        st.mean[k] = Img(w, h, 3);
+        st.m2[k] = Img(w, h, 3);
+        st.sw[k] = Img(w, h, 1);
+    }
+    std::memcpy(st.film[0].v.data(), film, n3 * 4);
+    std::memcpy(st.mean[0].v.data(), mean, n3 * 4);
+    std::memcpy(st.m2[0].v.data(), m2, n3 * 4);
+    std::memcpy(st.sw[0].v.data(), sw, n1 * 4);
+    if (sumimp && newsum) {
+        Img a(w, h, 1), b(w, h, 1);
+        std::memcpy(a.v.data(), sumimp, n1 * 4);
+        std::memcpy(b.v.data(), newsum, n1 * 4);
+        flt_rescale(st, a, b);
+    }
+    for (int64_t i = 0; i < ndep; ++i) flt_add_sample(st, 0, pos[i], wt[i], rgb + 3 * i);
+    std::memcpy(film, st.film[0].v.data(), n3 * 4);
+    std::memcpy(mean, st.mean[0].v.data(), n3 * 4);
+    std::memcpy(m2, st.m2[0].v.data(), n3 * 4);
+    std::memcpy(sw, st.sw[0].v.data(), n1 * 4);
 }
 
 void oc_sym_filter(const float* image, const float* guide, const float* pixvar, int w, int h, int patch, int radius,

@@ -131,6 +131,18 @@ void oc_sym_filter(const float* image, const float* guide, const float* pixvar, 
                    float* out);
 /* The primary-sample perturbation (src/metropolissample.cpp:62-78) of n coordinates. */
 void oc_mlt_perturb(const float* s, const float* u, const float* v, int64_t n, float* out);
+/* Pieces of the filtered Metropolis renderer (src/metropolisfltrenderer.cpp), on
+ * h * w (* 3) float images: cv::blur with BORDER_REFLECT (odd p), the [.1 .8 .1]
+ * sepFilter2D with BORDER_REFLECT_101, mathhelper var() of two images, the
+ * importance map of a later pass (:231-239), and one film's running sums
+ * rescaled to a new pass (:241-250, sumimp/newsum may be null) plus deposits
+ * added by addSample (:54-75). */
+void oc_box_blur(const float* in, int w, int h, int p, float* out);
+void oc_sep3_reflect101(const float* in, int w, int h, float* out);
+void oc_var2(const float* a, const float* b, int64_t n, float* mean, float* var);
+void oc_flt_importance(const float* var_filtered, const float* mean_filtered, int w, int h, float* imp);
+void oc_flt_accumulate(int w, int h, float* film, float* mean, float* m2, float* sw, const float* sumimp,
+                       const float* newsum, int64_t ndep, const int32_t* pos, const float* wt, const float* rgb);
 
 #ifdef __cplusplus
 }
