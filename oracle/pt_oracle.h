@@ -21,10 +21,12 @@
  * Philox-4x32-10 stream keyed by (seed, view, pixel, sample, dimension) that
  * the GPU path uses, and MAX_DEPTH / RR start are runtime parameters.
  *
- * The environment light is a latitude-longitude map (the reference reads an
- * angular light probe through OpenCV); its lookup and importance sampling
- * are restated from the textbook method and are parity-unpinned against the
- * reference.
+ * The environment light is a latitude-longitude map, a format of this
+ * library's own (the reference reads an angular light probe through OpenCV,
+ * which is restated too); its lookup and importance sampling follow the
+ * textbook method and are pinned by a white furnace and by the consistency
+ * of the sampling density with the sampled directions, there being no
+ * reference counterpart to compare with.
  */
 #ifndef PT_ORACLE_H
 #define PT_ORACLE_H
