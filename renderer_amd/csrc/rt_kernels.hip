@@ -1573,12 +1573,17 @@ hipError_t launch_shadow(const HostDevScene& s, const PathState& ps, PtParams p,
 /* Late bounces hold few paths (1.5% of the camera rays from the fifth
  * bounce on in a 1024^2, 64 spp render of a 1M-triangle mesh), and a launch
  * of the separate kernels costs about the latency chain of its slowest ray
- * whatever its size, 300-450 us each there.  Here a lane takes a path from the queue and runs it to its end:
- * closest hit, shade (the same per-path functions as k_shade, so the path's
- * state changes in the same order), its shadow ray, the next bounce; a lane
- * whose path has ended takes the next one (WaveFeed).  The film is the one
- * the separate kernels give.  Rays are counted as extension and shadow rays
- * and, separately, as the ones traced here. */
+ * whatever its size, 300-450 us each there.  Here a lane takes a path from
+ * the queue and runs it to its end: closest hit, shade (the same per-path
+ * functions as k_shade, so the path's state changes in the same order), its
+ * shadow ray, the next bounce; a lane whose path has ended takes the next
+ * one (WaveFeed).  A lane whose closest hit is found waits until its wave's
+ * waiting lanes are at least as many as those still traversing (or none
+ * traverses), so the wave runs the shading code once for many lanes rather
+ * than once per lane: 4.0 -> 3.1 ms per 1024^2 x 64 spp render of the
+ * 1M-triangle mesh, Metropolis 815 -> 930 Mrays/s.  The film is the one the
+ * separate kernels give.  Rays are counted as extension and shadow rays and,
+ * separately, as the ones traced here. */
 __device__ __forceinline__ unsigned wave_sum(unsigned v)
 {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
@@ -1597,7 +1602,7 @@ __global__ __launch_bounds__(PT_BLOCK) RT_FINISH_OCCUPANCY void k_finish(DevScen
     const int n = ps.cnt->q_count[qsel];
     const int32_t* q = ps.queue[qsel];
     int cn = 0, cp = 0, p = 0, depth = 0;
-    bool live = false, more = true, in_shadow = false, cont = false;
+    bool live = false, more = true, in_shadow = false, cont = false, to_shade = false;
     float4 sc = make_float4(0.f, 0.f, 0.f, 0.f);
     unsigned n_ext = 0, n_sh = 0;
     WaveFeed feed;
@@ -1625,8 +1630,18 @@ __global__ __launch_bounds__(PT_BLOCK) RT_FINISH_OCCUPANCY void k_finish(DevScen
             }
         }
         if (!__ballot(live)) break;  /* queue drained and every lane idle */
-        const bool done = live && (T.cur < 0 || (in_shadow ? trav_step<true, false>(S, T, my, cn, cp, top)
-                                                           : trav_step<false, false>(S, T, my, cn, cp, top)));
+        const bool done = live && !to_shade &&
+                          (T.cur < 0 || (in_shadow ? trav_step<true, false>(S, T, my, cn, cp, top)
+                                                   : trav_step<false, false>(S, T, my, cn, cp, top)));
+        if (done && !in_shadow) {
+            ps.hit[p] = make_float4(T.g >= 0 ? T.best : RT_INF, __int_as_float(T.g), T.u, T.v);
+            to_shade = true;
+        }
+        /* a wave runs the shading code once for all the lanes that wait for
+         * it: when they are at least as many as the lanes still traversing,
+         * or no lane traverses */
+        const unsigned long long waiting = __ballot(to_shade), tracing = __ballot(live && !to_shade);
+        const bool shade_now = to_shade && (tracing == 0 || __popcll(waiting) >= __popcll(tracing));
         if (done && in_shadow) {
             if (T.g < 0) {  /* unoccluded: as k_shadow */
                 if (MODE == RT_MODE_COMPAT) {
@@ -1643,8 +1658,8 @@ __global__ __launch_bounds__(PT_BLOCK) RT_FINISH_OCCUPANCY void k_finish(DevScen
             } else {
                 live = false;
             }
-        } else if (done) {
-            ps.hit[p] = make_float4(T.g >= 0 ? T.best : RT_INF, __int_as_float(T.g), T.u, T.v);
+        } else if (shade_now) {
+            to_shade = false;
             bool shadow = false;
             float4 so, sd;
             if (MODE == RT_MODE_COMPAT)
