@@ -1501,7 +1501,7 @@ static int render_impl(rt_ctx* c, const rt_view* views, int32_t nv, const rt_par
             QCHK(mark());
             QCHK(launch_advance(c->ps, q, dd, fr, c->stream));
         }
-        QCHK(launch_accumulate(c->ps, d_views, d_tiles, d_passes, grid, c->stream));
+        QCHK(launch_accumulate(c->ps, d_views, d_tiles, d_passes, pp.mode == RT_MODE_COMPAT, grid, c->stream));
         if (train) {
             if (timing) train_ev.push_back(ev);
             QCHK(mark());

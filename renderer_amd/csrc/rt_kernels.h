@@ -281,7 +281,7 @@ hipError_t launch_finish(const HostDevScene& s, const PathState& ps, PtParams p,
 int64_t trav_lanes(int grid);
 hipError_t launch_advance(const PathState& ps, int qsel, int depth, FinishRule fr, hipStream_t st);
 hipError_t launch_accumulate(const PathState& ps, const DevView* views, const DevTile* tiles,
-                             const DevPass* passes, int grid, hipStream_t st);
+                             const DevPass* passes, int compat, int grid, hipStream_t st);
 hipError_t launch_pass_end(const PathState& ps, hipStream_t st);
 hipError_t launch_reset_counters(const PathState& ps, hipStream_t st);
 

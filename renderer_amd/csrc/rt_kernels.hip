@@ -770,7 +770,7 @@ __global__ __launch_bounds__(PT_BLOCK) void k_generate(PathState ps, const DevVi
         ps.ray_d[slot] = make_float4(d.x, d.y, d.z, RT_INF);
         ps.alpha[slot] = make_float4(1.f, 1.f, 1.f, 0.f);
         ps.radiance[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
-        ps.pend_c[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (P.mode == RT_MODE_COMPAT) ps.pend_c[slot] = make_float4(0.f, 0.f, 0.f, 0.f); /* only compat keeps one */
         ps.info[slot] = make_int4(pixel, s, T.view, pk(0, L, 1));
         ps.queue[0][slot] = slot;
     }
@@ -1751,7 +1751,7 @@ hipError_t launch_advance(const PathState& ps, int qsel, int depth, FinishRule f
 #define ACC_CHUNK 8
 __global__ __launch_bounds__(PT_BLOCK) void k_accumulate(PathState ps, const DevView* __restrict__ views,
                                                         const DevTile* __restrict__ tiles,
-                                                        const DevPass* __restrict__ passes)
+                                                        const DevPass* __restrict__ passes, int compat)
 {
     __shared__ int s_base[PT_BLOCK], s_ns[PT_BLOCK];
     __shared__ float s_inv[PT_BLOCK];
@@ -1793,7 +1793,8 @@ __global__ __launch_bounds__(PT_BLOCK) void k_accumulate(PathState ps, const Dev
                 const int e = k * PT_BLOCK + t, pe = e / ACC_CHUNK, se = c0 + e % ACC_CHUNK;
                 if (se < s_ns[pe]) {
                     const int slot = s_base[pe] + se;
-                    const float4 L = ps.radiance[slot], cc = ps.pend_c[slot];
+                    const float4 L = ps.radiance[slot];
+                    const float4 cc = compat ? ps.pend_c[slot] : make_float4(0.f, 0.f, 0.f, 0.f);
                     const float inv = s_inv[pe];
                     s_term[pe][e % ACC_CHUNK][0] = (L.x + cc.x) * inv;
                     s_term[pe][e % ACC_CHUNK][1] = (L.y + cc.y) * inv;
@@ -1818,9 +1819,9 @@ __global__ __launch_bounds__(PT_BLOCK) void k_accumulate(PathState ps, const Dev
     }
 }
 hipError_t launch_accumulate(const PathState& ps, const DevView* views, const DevTile* tiles,
-                             const DevPass* passes, int grid, hipStream_t st)
+                             const DevPass* passes, int compat, int grid, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_accumulate, dim3(grid), dim3(PT_BLOCK), 0, st, ps, views, tiles, passes);
+    hipLaunchKernelGGL(k_accumulate, dim3(grid), dim3(PT_BLOCK), 0, st, ps, views, tiles, passes, compat);
     return hipGetLastError();
 }
 
