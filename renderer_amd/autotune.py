@@ -18,7 +18,7 @@ import json
 import math
 import os
 
-DEFAULT_SLOTS = (1 << 22, 1 << 23, 1 << 24, 1 << 25)
+DEFAULT_SLOTS = (1 << 23, 1 << 24, 1 << 25, 1 << 26, 1 << 27)
 DEFAULT_BPC = (4, 8, 16)
 
 

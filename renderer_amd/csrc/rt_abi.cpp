@@ -1333,9 +1333,11 @@ static int render_impl(rt_ctx* c, const rt_view* views, int32_t nv, const rt_par
         max_pix = std::max(max_pix, npix);
     }
     hipSetDevice(c->device);
-    /* default 32M path slots (~7.4 GB of path state): fewer passes and fuller
-     * late-bounce launches; the 1024^2 x 64 spp film takes two passes */
-    int64_t P = c->cfg.path_slots > 0 ? c->cfg.path_slots : (int64_t)1 << 25;
+    /* default 64M path slots (~15.6 GB of path state, 28 GB with the guide's):
+     * every pass pays its bounces' launch tails and its finishing launch's
+     * latency chain once, so fewer passes are faster; the 1024^2 x 64 spp
+     * film takes one pass (83.2 -> 80.2 ms against 32M slots and two) */
+    int64_t P = c->cfg.path_slots > 0 ? c->cfg.path_slots : (int64_t)1 << 26;
     P = std::min<int64_t>(P, std::max<int64_t>(total_samples, 1));
     P = std::max<int64_t>(P, 256);
 
